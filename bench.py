@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Throughput of the MFMA matrix Fourier transform: FraunhoferPropagator forward,
+1024^2 pupil -> 512^2 focal plane (q = 4, 64 lambda/D), complex64, batch 1.
+
+One step = one propagation (data split + two MFMA GEMM stages) of a field already resident in
+HBM.  The timed steps are replayed from HIP graphs of `--chunk` propagations each.  Also
+reported: the plain-FP32 MFMA path on the same workload, the peak-relative intensity error of
+both against this library's FP64 MFMA path (itself parity-tested against the FP64 oracle), the
+dominant kernel's roofline from HIP events, and the CPU baseline (numpy/scipy restatement
+of the reference's two cgemm calls on the host cores).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+METRIC = 'MFT propagations/s 1024²→512² (split-prec MFMA vs FP32) at ≤1e-7 peak-rel err'
+PEAK_TFLOPS = {'f16x2': 2500.0, 'fp32': 157.3, 'fp64': 78.6}  # MI355X dense MFMA peaks
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2000)
+    ap.add_argument('--warmup', type=int, default=50)
+    ap.add_argument('--precision', default='f16x2', choices=['f16x2', 'fp32'])
+    ap.add_argument('--chunk', type=int, default=50, help='propagations per captured HIP graph')
+    ap.add_argument('--no-graph', action='store_true')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-seconds', type=float, default=2.0, help='t_max of the CPU timing loop')
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: the MFT path has no multi-GPU mode in this build '
+              '(--gpus %d requested).' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import hcipy_amd as ha
+    from hcipy_amd import _lib, workloads
+
+    dev = torch.device('cuda', 0)
+    torch.cuda.set_device(dev)
+    pupil, focal, E = workloads.plain_mft_1024()
+    E64 = E.astype(np.complex64)
+    x = torch.as_tensor(E64, device=dev)
+    wavelength = 1.0
+
+    def setup(precision):
+        prop = ha.FraunhoferPropagator(pupil, focal, precision=precision)
+        d = prop.instance(wavelength)
+        mft = d.fourier_transform
+        alpha = complex(pupil.weights) * d.norm_factor
+        cdt = torch.complex64 if precision != 'fp64' else torch.complex128
+        plan = mft.plan_for(cdt, dev)
+        ws = mft._workspace_for(plan, 1, dev)
+        return prop, plan, ws, alpha
+
+    def time_path(precision, steps, warmup):
+        prop, plan, ws, alpha = setup(precision)
+        out = torch.empty(focal.size, dtype=torch.complex64, device=dev)
+
+        def step():
+            plan.execute(_lib.FORWARD, x, out, 1, alpha, ws)
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        chunk = max(1, min(args.chunk, steps))
+        graph = None
+        if not args.no_graph:
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                step()
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                for _ in range(chunk):
+                    step()
+            graph.replay()
+            torch.cuda.synchronize()
+        n_rep = max(1, steps // chunk)
+        steps_done = n_rep * chunk if graph is not None else steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if graph is not None:
+            for _ in range(n_rep):
+                graph.replay()
+        else:
+            for _ in range(steps):
+                step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        # per-kernel durations from HIP events on the launching stream
+        kms = (ctypes.c_float * 3)()
+        st = _lib.load().hcipy_mft_profile(plan.handle, _lib.FORWARD, _lib.dptr(x), _lib.dptr(out), 1,
+                                           float(alpha.real), float(alpha.imag), _lib.dptr(ws), ws.numel(), 200,
+                                           kms, _lib.stream_ptr(dev))
+        _lib.check(st, 'hcipy_mft_profile')
+        return (t1 - t0) / steps_done, steps_done, [kms[0], kms[1], kms[2]], out.clone()
+
+    log('timing f16x2 / fp32 paths ...')
+    sec_main, steps_done, kms_main, out_main = time_path(args.precision, args.steps, args.warmup)
+    other = 'fp32' if args.precision == 'f16x2' else 'f16x2'
+    sec_other, _, kms_other, out_other = time_path(other, max(200, args.steps // 4), 20)
+
+    # Error against the FP64 MFMA path on the same (complex64-rounded) input.
+    prop64 = ha.FraunhoferPropagator(pupil, focal)
+    ref = prop64(ha.Wavefront(ha.Field(torch.as_tensor(E64.astype(np.complex128), device=dev), pupil),
+                              wavelength)).electric_field.as_subclass(torch.Tensor)
+    I_ref = (ref.abs() ** 2)
+    peak = I_ref.max().item()
+
+    def err(o):
+        I = o.to(torch.complex128).abs() ** 2
+        return (I - I_ref).abs().max().item() / peak
+
+    err_main, err_other = err(out_main), err(out_other)
+
+    # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC).
+    ny, nx = pupil.shape
+    nv, nu = focal.shape
+    flops_stage1 = 8.0 * ny * nx * nu
+    flops_stage2 = 8.0 * nv * ny * nu
+    ms1 = kms_main[1]
+    achieved = flops_stage1 / (ms1 * 1e-3) / 1e12
+    peak_tf = PEAK_TFLOPS[args.precision]
+    roofline = {'bound': 'mfma', 'kernel': 'mft_stage_kernel (stage 1, E.M2)', 'achieved': round(achieved, 2),
+                'peak': peak_tf, 'unit': 'TFLOP/s', 'frac': round(achieved / peak_tf, 4), 'traffic': None,
+                'kernel_ms': {'split': round(kms_main[0], 5), 'stage1': round(kms_main[1], 5),
+                              'stage2': round(kms_main[2], 5)},
+                'algorithmic_flop_per_launch': flops_stage1}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log('cpu baseline ...')
+        from oracle import hcipy_oracle as o
+        mcpu = o.MftCpu(o.Grid(pupil.delta, tuple(int(v) for v in pupil.dims), pupil.zero),
+                        o.Grid(focal.scaled(2 * np.pi).delta, tuple(int(v) for v in focal.dims),
+                               focal.scaled(2 * np.pi).zero), 'complex64')
+        f = E64.copy()
+        t = o.time_it(lambda: mcpu.forward(f), t_max=args.cpu_seconds)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {'value': round(1.0 / t, 3), 'unit': 'propagations/s', 'cores': int(os.environ.get('OPENBLAS_NUM_THREADS', cores)),
+               'kind': 'port',
+               'sample': 'same 1024^2->512^2 complex64 MFT: scipy cgemm x2 with the reference transpose pattern, '
+                         'precomputed matrices, median of 5 runs of >= %.1f s (reference _time_it)' % args.cpu_seconds}
+
+    value = 1.0 / sec_main
+    res = {
+        'metric': METRIC,
+        'value': round(value, 2),
+        'unit': 'propagations/s',
+        'n_gpus': 1,
+        'steps': steps_done,
+        'warmup': args.warmup,
+        'ms_per_step': round(sec_main * 1e3, 5),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'fp16x2' if args.precision == 'f16x2' else 'fp32',
+        'data': 'synthetic',
+        'config': {'workload': 'fraunhofer_mft_1024_to_512_complex64', 'pupil': [1024, 1024], 'focal': [512, 512],
+                   'q': 4, 'num_airy': 64, 'batch': 1, 'precision': args.precision,
+                   'graph_chunk': None if args.no_graph else args.chunk},
+        'peak_rel_intensity_error': err_main,
+        'comparison': {'precision': other, 'value': round(1.0 / sec_other, 2), 'ms_per_step': round(sec_other * 1e3, 5),
+                       'peak_rel_intensity_error': err_other,
+                       'kernel_ms': [round(v, 5) for v in kms_other],
+                       'speedup_of_%s' % args.precision: round(sec_other / sec_main, 3)},
+        'algorithmic_gflop_per_step': round((flops_stage1 + flops_stage2) / 1e9, 4),
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == '__main__':
+    main()

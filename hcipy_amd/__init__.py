@@ -1,0 +1,19 @@
+"""MI355X (gfx950) implementation of HCIPy's Fourier-propagation path.
+
+Host-side Python mirrors HCIPy's Field / Wavefront / FourierTransform / OpticalElement API;
+the arithmetic runs in hand-written HIP kernels behind the C ABI of ``libhcipy_amd.so``
+(include/hcipy_amd.h).  Fields are torch tensors on the ROCm device.
+"""
+from .config import Configuration
+from .field import (CartesianGrid, Field, RegularCoords, SeparatedCoords, UnstructuredCoords, default_device,
+                    evaluate_supersampled, make_focal_grid, make_pupil_grid, make_uniform_grid, set_default_device,
+                    subsample_field)
+from .fourier import (ComputationalComplexity, FourierTransform, MatrixFourierTransform, get_fft_parameters,
+                      is_fft_grid, make_fft_grid)
+from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, OpticalSystem, PhaseApodizer, Wavefront
+from .propagation import FraunhoferPropagator
+from .coronagraphy import LyotCoronagraph
+from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
+from . import pointwise
+
+__version__ = '0.1.0'

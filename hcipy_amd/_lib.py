@@ -1,0 +1,108 @@
+"""ctypes binding of the C-ABI library ``libhcipy_amd.so`` (declarations in include/hcipy_amd.h).
+
+There is no fallback: if the library is missing or no ROCm device is present the operations
+raise.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libhcipy_amd.so')
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+PREC_F16X2 = 2
+PRECISIONS = {'fp64': PREC_FP64, 'fp32': PREC_FP32, 'f16x2': PREC_F16X2}
+FORWARD = 0
+BACKWARD = 1
+
+_c_int64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every symbol include/hcipy_amd.h declares.
+SIGNATURES = {
+    'hcipy_abi_version': (ctypes.c_int, []),
+    'hcipy_get_last_error': (ctypes.c_char_p, []),
+    'hcipy_mft_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _dp, _c_int64, _dp,
+                                             _c_int64, _dp, _c_int64, _dp, _c_int64, _vp]),
+    'hcipy_mft_plan_destroy': (ctypes.c_int, [_vp]),
+    'hcipy_mft_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_mft_plan_bytes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_mft_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
+                                         _vp, ctypes.c_size_t, _vp]),
+    'hcipy_mft_profile': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
+                                         _vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float), _vp]),
+    'hcipy_pw_sub_mul': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _c_int64, ctypes.c_int, _vp, ctypes.c_int,
+                                        _c_int64, _c_int64, _vp]),
+    'hcipy_pw_phase': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _vp, ctypes.c_int, _c_int64,
+                                      _c_int64, _vp]),
+    'hcipy_pw_intensity': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _c_int64, _vp]),
+    'hcipy_pw_power_workspace': (ctypes.c_int, [_c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_pw_power': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _c_int64,
+                                      _c_int64, _vp, _vp, ctypes.c_size_t, _vp]),
+}
+
+DTYPE_C64 = 0
+DTYPE_C128 = 1
+DTYPE_F32 = 2
+DTYPE_F64 = 3
+_DTYPE_CODES = {torch.complex64: DTYPE_C64, torch.complex128: DTYPE_C128, torch.float32: DTYPE_F32,
+                torch.float64: DTYPE_F64}
+
+
+def dtype_code(t):
+    try:
+        return _DTYPE_CODES[t.dtype]
+    except KeyError:
+        raise HcipyError('unsupported dtype %s' % t.dtype)
+
+_lib = None
+
+
+class HcipyError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads the library (once) and declares the signatures."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HcipyError('%s is not built; run `python -m hcipy_amd.build`' % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status, what=''):
+    if status != 0:
+        msg = load().hcipy_get_last_error().decode(errors='replace')
+        raise HcipyError('%s failed (status %d): %s' % (what, status, msg))
+
+
+def require_device(t):
+    """Raises unless `t` lives on a ROCm device (the kernels have no CPU path)."""
+    if not isinstance(t, torch.Tensor) or t.device.type != 'cuda':
+        raise HcipyError('hcipy_amd kernels need tensors on a ROCm GPU device; got %s'
+                         % (t.device if isinstance(t, torch.Tensor) else type(t)))
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def darray(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)

@@ -1,0 +1,410 @@
+// C-ABI of the library: error state and the matrix-Fourier-transform plan.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/hcipy_amd.h"
+#include "mft_gemm.hpp"
+
+namespace hcipy {
+
+static thread_local char g_last_error[512] = "";
+
+void set_last_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return kOk;
+  set_last_error("%s: %s", what, hipGetErrorString(e));
+  return kHipError;
+}
+
+int launch_stage(int prec, const StageArgs& a, int nz, bool b_aligned, hipStream_t s);
+int launch_split_rows(const float* in, int64_t ld_in, int K, int64_t rows, _Float16* hi, _Float16* lo,
+                      int64_t ld_out, float* row_scale, hipStream_t s);
+int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, void* out,
+                      void* out_lo, int64_t lda, int rows_pad, hipStream_t s);
+
+namespace {
+
+constexpr int64_t kAlign = 256;
+constexpr int kMaxGridZ = 65535;
+
+size_t elem_size(int prec) { return prec == kF64 ? 8 : (prec == kF32 ? 4 : 2); }
+
+int ilog2_ceil(int64_t v) {
+  int e = 0;
+  while ((int64_t(1) << e) < v) ++e;
+  return e;
+}
+
+// One expanded Fourier matrix (2 * nc rows of 2 * nk reals, padded).
+struct FMatrix {
+  void* hi = nullptr;
+  void* lo = nullptr;
+  int nc = 0, nk = 0;
+  int64_t lda = 0;
+  int rows_pad = 0;
+  size_t bytes() const { return hi == nullptr ? 0 : (size_t)rows_pad * lda; }
+};
+
+struct Plan {
+  int ndim = 0, prec = 0, device = 0;
+  int64_t nx = 1, ny = 1, nu = 1, nv = 1;
+  FMatrix f1, f2, b1, b2;  // forward stage 1/2, backward stage 1/2
+  double* coords = nullptr;
+  size_t matrix_bytes = 0;
+};
+
+int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, int nk, double sign,
+                hipStream_t s) {
+  m.nc = nc;
+  m.nk = nk;
+  m.lda = round_up(2 * (int64_t)nk, 64);
+  m.rows_pad = (int)round_up(2 * (int64_t)nc, 128);
+  const size_t bytes = (size_t)m.rows_pad * m.lda * elem_size(pl.prec);
+  HCIPY_HIP(hipMalloc(&m.hi, bytes));
+  pl.matrix_bytes += bytes;
+  if (pl.prec == kF16x2) {
+    HCIPY_HIP(hipMalloc(&m.lo, bytes));
+    pl.matrix_bytes += bytes;
+  }
+  return launch_gen_matrix(a, nc, b, nk, sign, pl.prec, m.hi, m.lo, m.lda, m.rows_pad, s);
+}
+
+void free_matrix(FMatrix& m) {
+  if (m.hi) (void)hipFree(m.hi);
+  if (m.lo) (void)hipFree(m.lo);
+  m.hi = m.lo = nullptr;
+}
+
+// Geometry of one direction of a 2-D (or 1-D) transform.
+struct Dir {
+  const FMatrix* m1;  // stage 1 constant (rows c1, contraction k1)
+  const FMatrix* m2;  // stage 2 constant (rows c2, contraction n1), 2-D only
+  int64_t n1, k1, c1, c2;
+};
+
+Dir direction(const Plan& pl, int dir) {
+  Dir d{};
+  if (dir == HCIPY_FORWARD) {
+    d.m1 = &pl.f1;
+    d.m2 = &pl.f2;
+    d.n1 = pl.ny;
+    d.k1 = pl.nx;
+    d.c1 = pl.nu;
+    d.c2 = pl.nv;
+  } else {
+    d.m1 = &pl.b1;
+    d.m2 = &pl.b2;
+    d.n1 = pl.nv;
+    d.k1 = pl.nu;
+    d.c1 = pl.nx;
+    d.c2 = pl.ny;
+  }
+  return d;
+}
+
+// Workspace carve-up for one direction.
+struct Ws {
+  size_t split_hi = 0, split_lo = 0, row_scale = 0, t = 0, z_hi = 0, z_lo = 0, total = 0;
+  int64_t ld1 = 0, n1_pad = 0;
+};
+
+Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
+  Ws w;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (size_t)round_up((int64_t)bytes, kAlign);
+    return at;
+  };
+  const int64_t rows1 = batch * (pl.ndim == 2 ? d.n1 : 1);
+  if (pl.prec == kF16x2) {
+    w.ld1 = round_up(2 * d.k1, 64);
+    w.split_hi = take((size_t)rows1 * w.ld1 * 2);
+    w.split_lo = take((size_t)rows1 * w.ld1 * 2);
+    w.row_scale = take((size_t)rows1 * 4);
+    w.t = take((size_t)batch * 4);
+  }
+  if (pl.ndim == 2) {
+    w.n1_pad = round_up(d.n1, 32);
+    const size_t zelems = (size_t)batch * d.c1 * 2 * w.n1_pad;  // reals
+    w.z_hi = take(zelems * elem_size(pl.prec));
+    if (pl.prec == kF16x2) w.z_lo = take(zelems * 2);
+  }
+  w.total = off;
+  return w;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// ev (optional, 4 events): recorded before the split, before stage 1, before stage 2, after stage 2.
+int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai,
+                  char* ws, hipStream_t s, hipEvent_t* ev = nullptr) {
+  auto mark = [&](int i) -> int {
+    if (ev != nullptr) return check_hip(hipEventRecord(ev[i], s), "hipEventRecord");
+    return kOk;
+  };
+  const Dir d = direction(pl, dir);
+  const Ws w = workspace_layout(pl, d, batch);
+  const int out_mode = pl.prec == kF64 ? kOutC128 : kOutC64;
+  const size_t es = elem_size(pl.prec);
+  const int64_t rows1 = batch * (pl.ndim == 2 ? d.n1 : 1);
+  HCIPY_REQUIRE(rows1 <= INT32_MAX / 2, "batch too large (%lld data rows)", (long long)rows1);
+
+  // Stage-1 data operand.
+  StageArgs a1{};
+  bool b_aligned = true;
+  float* row_scale = nullptr;
+  float* t = nullptr;
+  if (int st0 = mark(0)) return st0;
+  if (pl.prec == kF16x2) {
+    _Float16* hi = reinterpret_cast<_Float16*>(ws + w.split_hi);
+    _Float16* lo = reinterpret_cast<_Float16*>(ws + w.split_lo);
+    row_scale = reinterpret_cast<float*>(ws + w.row_scale);
+    t = reinterpret_cast<float*>(ws + w.t);
+    int st = launch_split_rows(static_cast<const float*>(in), 2 * d.k1, (int)(2 * d.k1), rows1, hi, lo, w.ld1,
+                               row_scale, s);
+    if (st != kOk) return st;
+    a1.b = hi;
+    a1.b_lo = lo;
+    a1.ldb = w.ld1;
+    a1.row_scale = row_scale;
+  } else {
+    a1.b = in;
+    a1.ldb = 2 * d.k1;
+    b_aligned = aligned16(in) && ((2 * d.k1 * (int64_t)es) % 16 == 0);
+  }
+  a1.a = d.m1->hi;
+  a1.a_lo = d.m1->lo;
+  a1.lda = d.m1->lda;
+  a1.M = (int)(2 * d.c1);
+  a1.N = (int)rows1;
+  a1.K = (int)(2 * d.k1);
+  a1.t_log2k = ilog2_ceil(a1.K);
+  if (int st0 = mark(1)) return st0;
+
+  if (pl.ndim == 1) {
+    a1.out = out;
+    a1.ldo = d.c1;
+    a1.nc_layout = 1;
+    a1.out_mode = out_mode;
+    a1.alpha_re = ar;
+    a1.alpha_im = ai;
+    int st1 = launch_stage(pl.prec, a1, 1, b_aligned, s);
+    if (!st1) st1 = mark(2);
+    if (!st1) st1 = mark(3);
+    return st1;
+  }
+
+  // Stage 1 writes the intermediate, one (c1 x n1_pad) block per field.
+  a1.out = ws + w.z_hi;
+  a1.out_lo = pl.prec == kF16x2 ? ws + w.z_lo : nullptr;
+  a1.ldo = w.n1_pad;
+  a1.fold = (int)d.n1;
+  a1.fold_stride = d.c1 * w.n1_pad;
+  a1.out_mode = pl.prec == kF16x2 ? kOutSplit : out_mode;
+  a1.t_out = t;
+  a1.alpha_re = 1.0;
+  a1.alpha_im = 0.0;
+  int st = launch_stage(pl.prec, a1, 1, b_aligned, s);
+  if (st != kOk) return st;
+  if ((st = mark(2))) return st;
+
+  // Stage 2: one grid.z slice per field.
+  StageArgs a2{};
+  a2.a = d.m2->hi;
+  a2.a_lo = d.m2->lo;
+  a2.lda = d.m2->lda;
+  a2.b = ws + w.z_hi;
+  a2.b_lo = pl.prec == kF16x2 ? ws + w.z_lo : nullptr;
+  a2.ldb = 2 * w.n1_pad;
+  a2.b_zstride = d.c1 * 2 * w.n1_pad;
+  a2.M = (int)(2 * d.c2);
+  a2.N = (int)d.c1;
+  a2.K = (int)(2 * d.n1);
+  a2.out_mode = out_mode;
+  a2.ldo = d.c1;
+  a2.out_zstride = d.c2 * d.c1;
+  a2.alpha_re = ar;
+  a2.alpha_im = ai;
+  const size_t out_es = (pl.prec == kF64 ? 16 : 8);
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridZ) {
+    const int nz = (int)std::min<int64_t>(kMaxGridZ, batch - b0);
+    StageArgs a = a2;
+    a.b = static_cast<const char*>(a2.b) + b0 * a2.b_zstride * es;
+    if (a2.b_lo) a.b_lo = static_cast<const char*>(a2.b_lo) + b0 * a2.b_zstride * es;
+    a.out = static_cast<char*>(out) + b0 * a2.out_zstride * out_es;
+    a.z_scale = t ? t + b0 : nullptr;
+    st = launch_stage(pl.prec, a, nz, true, s);
+    if (st != kOk) return st;
+  }
+  return mark(3);
+}
+
+}  // namespace
+}  // namespace hcipy
+
+using namespace hcipy;
+
+struct hcipy_mft_plan_s {
+  Plan p;
+};
+
+extern "C" {
+
+int hcipy_abi_version(void) { return 1; }
+
+const char* hcipy_get_last_error(void) { return g_last_error; }
+
+int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const double* x_in, int64_t nx,
+                          const double* y_in, int64_t ny, const double* u_out, int64_t nu, const double* v_out,
+                          int64_t nv, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr, "plan pointer is NULL");
+  *plan = nullptr;
+  HCIPY_REQUIRE(ndim == 1 || ndim == 2, "The input_grid must be one- or two-dimensional (got ndim=%d).", ndim);
+  HCIPY_REQUIRE(precision == kF64 || precision == kF32 || precision == kF16x2, "unknown precision %d", precision);
+  HCIPY_REQUIRE(nx > 0 && nu > 0 && x_in != nullptr && u_out != nullptr, "empty grid");
+  if (ndim == 1) {
+    HCIPY_REQUIRE(ny == 1 && nv == 1, "1-D transform requires ny == nv == 1");
+  } else {
+    HCIPY_REQUIRE(ny > 0 && nv > 0 && y_in != nullptr && v_out != nullptr, "empty grid");
+  }
+  const int64_t lim = (int64_t)1 << 24;
+  HCIPY_REQUIRE(nx < lim && ny < lim && nu < lim && nv < lim, "grid dimension too large");
+
+  hcipy_mft_plan_s* h = new (std::nothrow) hcipy_mft_plan_s();
+  HCIPY_REQUIRE(h != nullptr, "out of host memory");
+  Plan& pl = h->p;
+  pl.ndim = ndim;
+  pl.prec = precision;
+  pl.nx = nx;
+  pl.ny = ny;
+  pl.nu = nu;
+  pl.nv = nv;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto fail = [&](int st) {
+    hcipy_mft_plan_destroy(h);
+    return st;
+  };
+  if (hipGetDevice(&pl.device) != hipSuccess) return fail(check_hip(hipGetLastError(), "hipGetDevice"));
+
+  const int64_t ncoord = nx + ny + nu + nv;
+  if (int st = check_hip(hipMalloc(&pl.coords, ncoord * sizeof(double)), "hipMalloc(coords)")) return fail(st);
+  double* dx = pl.coords;
+  double* dy = dx + nx;
+  double* du = dy + ny;
+  double* dv = du + nu;
+  int st = check_hip(hipMemcpyAsync(dx, x_in, nx * sizeof(double), hipMemcpyHostToDevice, s), "copy x");
+  if (!st) st = check_hip(hipMemcpyAsync(du, u_out, nu * sizeof(double), hipMemcpyHostToDevice, s), "copy u");
+  if (!st && ndim == 2) {
+    st = check_hip(hipMemcpyAsync(dy, y_in, ny * sizeof(double), hipMemcpyHostToDevice, s), "copy y");
+    if (!st) st = check_hip(hipMemcpyAsync(dv, v_out, nv * sizeof(double), hipMemcpyHostToDevice, s), "copy v");
+  }
+  if (st) return fail(st);
+
+  // forward stage 1: rows u, contraction x: exp(-i x u)   (M2^T)
+  // forward stage 2: rows v, contraction y: exp(-i v y)   (M1)
+  // backward stage 1: rows x, contraction u: exp(+i x u)  (M2^H)
+  // backward stage 2: rows y, contraction v: exp(+i v y)  (M1^H)
+  st = make_matrix(pl, pl.f1, du, (int)nu, dx, (int)nx, -1.0, s);
+  if (!st) st = make_matrix(pl, pl.b1, dx, (int)nx, du, (int)nu, +1.0, s);
+  if (!st && ndim == 2) {
+    st = make_matrix(pl, pl.f2, dv, (int)nv, dy, (int)ny, -1.0, s);
+    if (!st) st = make_matrix(pl, pl.b2, dy, (int)ny, dv, (int)nv, +1.0, s);
+  }
+  if (!st) st = check_hip(hipStreamSynchronize(s), "plan creation");
+  if (st) return fail(st);
+  *plan = h;
+  return kOk;
+}
+
+int hcipy_mft_plan_destroy(hcipy_mft_plan plan) {
+  if (plan == nullptr) return kOk;
+  Plan& pl = plan->p;
+  free_matrix(pl.f1);
+  free_matrix(pl.f2);
+  free_matrix(pl.b1);
+  free_matrix(pl.b2);
+  if (pl.coords) (void)hipFree(pl.coords);
+  delete plan;
+  return kOk;
+}
+
+int hcipy_mft_workspace_size(hcipy_mft_plan plan, int64_t batch, size_t* bytes) {
+  HCIPY_REQUIRE(plan != nullptr && bytes != nullptr, "NULL argument");
+  HCIPY_REQUIRE(batch >= 0, "negative batch");
+  const Plan& pl = plan->p;
+  const Ws f = workspace_layout(pl, direction(pl, HCIPY_FORWARD), batch);
+  const Ws b = workspace_layout(pl, direction(pl, HCIPY_BACKWARD), batch);
+  *bytes = std::max(f.total, b.total);
+  return kOk;
+}
+
+int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes) {
+  HCIPY_REQUIRE(plan != nullptr && bytes != nullptr, "NULL argument");
+  *bytes = plan->p.matrix_bytes;
+  return kOk;
+}
+
+int hcipy_mft_execute(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch,
+                      double alpha_re, double alpha_im, void* workspace, size_t workspace_bytes, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
+  HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
+  HCIPY_REQUIRE(batch >= 0, "negative batch");
+  if (batch == 0) return kOk;
+  HCIPY_REQUIRE(in != nullptr && out != nullptr, "NULL data pointer");
+  const Plan& pl = plan->p;
+  const Ws w = workspace_layout(pl, direction(pl, direction_), batch);
+  HCIPY_REQUIRE(workspace_bytes >= w.total, "workspace too small: %zu < %zu bytes", workspace_bytes, w.total);
+  HCIPY_REQUIRE(w.total == 0 || workspace != nullptr, "workspace is NULL");
+  HCIPY_REQUIRE(aligned16(workspace), "workspace must be 16-byte aligned");
+  int cur = -1;
+  HCIPY_HIP(hipGetDevice(&cur));
+  HCIPY_REQUIRE(cur == pl.device, "plan was created on device %d but device %d is current", pl.device, cur);
+  return run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace),
+                       static_cast<hipStream_t>(stream));
+}
+
+int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch, double alpha_re,
+                      double alpha_im, void* workspace, size_t workspace_bytes, int repeats, float* kernel_ms,
+                      void* stream) {
+  HCIPY_REQUIRE(plan != nullptr && kernel_ms != nullptr && repeats > 0, "bad argument");
+  HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
+  HCIPY_REQUIRE(batch > 0 && in != nullptr && out != nullptr, "bad data arguments");
+  const Plan& pl = plan->p;
+  const Ws w = workspace_layout(pl, direction(pl, direction_), batch);
+  HCIPY_REQUIRE(workspace_bytes >= w.total, "workspace too small: %zu < %zu bytes", workspace_bytes, w.total);
+  HCIPY_REQUIRE(aligned16(workspace), "workspace must be 16-byte aligned");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nev = 4 * repeats;
+  hipEvent_t* ev = new (std::nothrow) hipEvent_t[nev];
+  HCIPY_REQUIRE(ev != nullptr, "out of host memory");
+  int created = 0;
+  int st = kOk;
+  for (; created < nev && !st; ++created) st = check_hip(hipEventCreate(&ev[created]), "hipEventCreate");
+  for (int r = 0; r < repeats && !st; ++r)
+    st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s, ev + 4 * r);
+  if (!st) st = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  for (int k = 0; k < 3 && !st; ++k) {
+    double acc = 0.0;
+    for (int r = 0; r < repeats && !st; ++r) {
+      float ms = 0.0f;
+      st = check_hip(hipEventElapsedTime(&ms, ev[4 * r + k], ev[4 * r + k + 1]), "hipEventElapsedTime");
+      acc += ms;
+    }
+    kernel_ms[k] = (float)(acc / repeats);
+  }
+  for (int i = 0; i < created; ++i) (void)hipEventDestroy(ev[i]);
+  delete[] ev;
+  return st;
+}
+
+}  // extern "C"

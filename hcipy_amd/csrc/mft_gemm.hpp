@@ -1,0 +1,105 @@
+// MFMA kernels for one stage of the separable matrix Fourier transform.
+//
+// A 2-D MFT is  O = alpha * M1 . E . M2  (reference: hcipy/fourier/matrix_fourier_transform.py:171-172,
+// backward :219-220).  Each of the two complex contractions is run here as ONE real NT-GEMM
+//
+//     C[m][n] = sum_k  A[m][k] * B[n][k]            (A: M x K, B: N x K, both K-contiguous)
+//
+// where
+//   * B is the data operand exactly as it sits in HBM: rows of interleaved complex numbers
+//     (re, im, re, im, ...), i.e. K = 2 * (complex contraction length);
+//   * A is the constant Fourier matrix, expanded once per plan into 2x2 real blocks so that row
+//     2c gives Re and row 2c+1 gives Im of the complex result:
+//         A[2c  ][2k] =  Re m(c,k)   A[2c  ][2k+1] = -Im m(c,k)
+//         A[2c+1][2k] =  Im m(c,k)   A[2c+1][2k+1] =  Re m(c,k)
+//   * the MFMA accumulator layout puts rows (2c, 2c+1) of one column in the same lane, so the
+//     epilogue holds complete complex values and writes them interleaved, coalesced along n.
+//
+// Three arithmetic variants share the tiling:
+//   kF64   : v_mfma_f64_16x16x4_f64   (complex128 fields; parity with the FP64 reference)
+//   kF32   : v_mfma_f32_32x32x2_f32   (plain FP32; single-level FP32 accumulation)
+//   kF16x2 : v_mfma_f32_32x32x16_f16  on hi/lo FP16 splits of both operands:
+//              x = hi + lo * 2^-11,  A.B ~= Ah.Bh + 2^-11 (Ah.Bl + Al.Bh)
+//            The Ah.Bh accumulator is flushed into FP64 every BK (=64) real k; the cross
+//            accumulator is scaled by 2^-11 at each flush.  Data rows carry a power-of-two scale
+//            so that every split value lies in [-1, 1].
+#pragma once
+
+#include "common.hpp"
+
+namespace hcipy {
+
+enum Prec : int { kF64 = 0, kF32 = 1, kF16x2 = 2 };
+
+// Output formats written by the epilogue.
+enum OutMode : int {
+  kOutC128 = 0,   // interleaved complex128
+  kOutC64 = 1,    // interleaved complex64
+  kOutSplit = 2,  // interleaved hi/lo FP16 pair arrays (feeds the next kF16x2 stage)
+};
+
+template <int P> struct PrecTraits;
+template <> struct PrecTraits<kF64> {
+  using T = double;
+  static constexpr int MT = 16;      // MFMA tile edge
+  static constexpr int GROUPS = 4;   // lane groups along k inside one MFMA
+  static constexpr int KPL = 4;      // consecutive k values one lane reads per k-vector
+  static constexpr int KVEC = GROUPS * KPL;
+  static constexpr int NACC = 4;     // accumulator registers per lane per MFMA tile
+  static constexpr bool SPLIT = false;
+};
+template <> struct PrecTraits<kF32> {
+  using T = float;
+  static constexpr int MT = 32;
+  static constexpr int GROUPS = 2;
+  static constexpr int KPL = 4;
+  static constexpr int KVEC = GROUPS * KPL;
+  static constexpr int NACC = 16;
+  static constexpr bool SPLIT = false;
+};
+template <> struct PrecTraits<kF16x2> {
+  using T = _Float16;
+  static constexpr int MT = 32;
+  static constexpr int GROUPS = 2;
+  static constexpr int KPL = 8;
+  static constexpr int KVEC = GROUPS * KPL;
+  static constexpr int NACC = 16;
+  static constexpr bool SPLIT = true;
+};
+
+// Scale of the lo part of a split value: x = hi + lo * kLoScale.
+constexpr double kLoScale = 1.0 / 2048.0;
+constexpr float kLoInv = 2048.0f;
+
+struct StageArgs {
+  // Constant operand (expanded Fourier matrix), padded so that tile loads never leave it.
+  const void* a;
+  const void* a_lo;
+  int64_t lda;
+  // Data operand.
+  const void* b;
+  const void* b_lo;
+  int64_t ldb;
+  int64_t b_zstride;            // elements between batch items along gridDim.z
+  // Problem size: M = 2 * (number of constant rows), N = data rows, K = real contraction length.
+  int M, N, K;
+  // Output.
+  void* out;
+  void* out_lo;
+  int64_t ldo;                  // complex elements between consecutive constant rows
+  int64_t out_zstride;          // complex elements between batch items along gridDim.z
+  int out_mode;
+  int nc_layout;                // 1: out[n * ldo + c] instead of out[c * ldo + n]
+  // Folding of several fields into the N dimension: n = item * fold + y is written to
+  // out[item * fold_stride + c * ldo + y].  fold = 0 disables folding.
+  int fold;
+  int64_t fold_stride;
+  // Scales of the split (kF16x2) path.
+  const float* row_scale;       // data row n was multiplied by row_scale[n]   (or nullptr)
+  const float* z_scale;         // data of batch item z was multiplied by z_scale[z] (or nullptr)
+  float* t_out;                 // split output: per folded item scale, written here (or nullptr)
+  int t_log2k;                  // ceil(log2(K)): bound |C| <= K / min(row_scale)
+  double alpha_re, alpha_im;
+};
+
+}  // namespace hcipy

@@ -1,0 +1,229 @@
+// Pointwise and reduction kernels of the propagation chain: apodizers, phase screens, DM phase,
+// Babinet subtraction, intensity and FP64 (weighted, masked) power sums.
+//
+// All kernels are HBM-bound streaming kernels: one pass, coalesced along the flattened field,
+// a batch of fields on gridDim.y with the modulating array broadcast (stride 0) or per field.
+#include "common.hpp"
+#include "../../include/hcipy_amd.h"
+
+namespace hcipy {
+
+namespace {
+
+struct C64 {
+  float re, im;
+};
+struct C128 {
+  double re, im;
+};
+
+template <typename C>
+__device__ inline void load_c(const void* p, int64_t i, double& re, double& im) {
+  const C v = static_cast<const C*>(p)[i];
+  re = v.re;
+  im = v.im;
+}
+
+__device__ inline void load_any(const void* p, int dtype, int64_t i, double& re, double& im) {
+  switch (dtype) {
+    case HCIPY_DTYPE_C64: load_c<C64>(p, i, re, im); break;
+    case HCIPY_DTYPE_C128: load_c<C128>(p, i, re, im); break;
+    case HCIPY_DTYPE_F32: re = static_cast<const float*>(p)[i]; im = 0.0; break;
+    default: re = static_cast<const double*>(p)[i]; im = 0.0; break;
+  }
+}
+
+template <typename C>
+__device__ inline void store_c(void* p, int64_t i, double re, double im);
+template <>
+__device__ inline void store_c<C64>(void* p, int64_t i, double re, double im) {
+  static_cast<C64*>(p)[i] = C64{(float)re, (float)im};
+}
+template <>
+__device__ inline void store_c<C128>(void* p, int64_t i, double re, double im) {
+  static_cast<C128*>(p)[i] = C128{re, im};
+}
+
+// out = (x - y_opt) * a_opt, with conj(a) if requested.  y and a may be null.
+// Arithmetic in FP32 for complex64 fields (the reference's numpy complex64 ufuncs) and FP64
+// for complex128 fields.
+template <typename C, typename R>
+__global__ void sub_mul_kernel(const C* __restrict__ x, const C* __restrict__ y, const void* __restrict__ a,
+                               int a_dtype, int64_t a_stride, int conj, C* __restrict__ out, int64_t n) {
+  const int64_t b = blockIdx.y;
+  const C* xb = x + b * n;
+  const C* yb = y ? y + b * n : nullptr;
+  C* ob = out + b * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    R re = xb[i].re, im = xb[i].im;
+    if (yb) {
+      re -= (R)yb[i].re;
+      im -= (R)yb[i].im;
+    }
+    if (a) {
+      double ar, ai;
+      load_any(a, a_dtype, b * a_stride + i, ar, ai);
+      if (conj) ai = -ai;
+      const R r2 = re * (R)ar - im * (R)ai;
+      const R i2 = re * (R)ai + im * (R)ar;
+      re = r2;
+      im = i2;
+    }
+    ob[i] = C{re, im};
+  }
+}
+
+// out = x * exp(i * scale * phi), phase evaluated in FP64.
+template <typename C, typename R>
+__global__ void phase_kernel(const C* __restrict__ x, const void* __restrict__ phi, int phi_dtype, int64_t phi_stride,
+                             double scale, C* __restrict__ out, int64_t n) {
+  const int64_t b = blockIdx.y;
+  const C* xb = x + b * n;
+  C* ob = out + b * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double p, dummy;
+    load_any(phi, phi_dtype, b * phi_stride + i, p, dummy);
+    double s, c;
+    sincos(scale * p, &s, &c);
+    const R re = xb[i].re, im = xb[i].im;
+    ob[i] = C{(R)(re * (R)c - im * (R)s), (R)(re * (R)s + im * (R)c)};
+  }
+}
+
+template <typename C, typename R>
+__global__ void intensity_kernel(const C* __restrict__ x, R* __restrict__ out, int64_t total) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const R re = x[i].re, im = x[i].im;
+    out[i] = re * re + im * im;
+  }
+}
+
+// partial[b * nblk + blockIdx.x] = sum_i |x_b[i]|^2 * w_i  (FP64), w from mask (optional) times weight.
+template <typename C>
+__global__ __launch_bounds__(256) void power_partial_kernel(const C* __restrict__ x, const void* __restrict__ mask,
+                                                            int mask_dtype, int64_t mask_stride, double weight,
+                                                            double* __restrict__ partial, int64_t n) {
+  __shared__ double red[4];
+  const int64_t b = blockIdx.y;
+  const C* xb = x + b * n;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double re = xb[i].re, im = xb[i].im;
+    double w = 1.0, dummy;
+    if (mask) load_any(mask, mask_dtype, b * mask_stride + i, w, dummy);
+    acc += (re * re + im * im) * w;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[b * gridDim.x + blockIdx.x] = (red[0] + red[1] + red[2] + red[3]) * weight;
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double* __restrict__ partial, int nblk,
+                                                           double* __restrict__ out) {
+  __shared__ double red[4];
+  const int64_t b = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) acc += partial[b * nblk + i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[b] = red[0] + red[1] + red[2] + red[3];
+}
+
+constexpr int kThreads = 256;
+constexpr int kPowerBlocks = 256;
+
+dim3 grid_for(int64_t n, int64_t batch) {
+  const int64_t bx = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kThreads), 4096));
+  return dim3((unsigned)bx, (unsigned)batch);
+}
+
+bool is_complex(int d) { return d == HCIPY_DTYPE_C64 || d == HCIPY_DTYPE_C128; }
+bool is_valid(int d) { return d >= HCIPY_DTYPE_C64 && d <= HCIPY_DTYPE_F64; }
+
+}  // namespace
+}  // namespace hcipy
+
+using namespace hcipy;
+
+extern "C" {
+
+int hcipy_pw_sub_mul(const void* x, const void* y, const void* a, int a_dtype, int64_t a_stride, int conjugate_a,
+                     void* out, int field_dtype, int64_t n, int64_t batch, void* stream) {
+  HCIPY_REQUIRE(field_dtype == HCIPY_DTYPE_C64 || field_dtype == HCIPY_DTYPE_C128, "field must be complex");
+  HCIPY_REQUIRE(a == nullptr || is_valid(a_dtype), "bad dtype of the multiplier");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  if (n == 0 || batch == 0) return kOk;
+  HCIPY_REQUIRE(x != nullptr && out != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 g = grid_for(n, batch);
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((sub_mul_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, (const C64*)y, a, a_dtype,
+                       a_stride, conjugate_a, (C64*)out, n);
+  else
+    hipLaunchKernelGGL((sub_mul_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, (const C128*)y, a,
+                       a_dtype, a_stride, conjugate_a, (C128*)out, n);
+  return check_hip(hipGetLastError(), "sub_mul_kernel");
+}
+
+int hcipy_pw_phase(const void* x, const void* phi, int phi_dtype, int64_t phi_stride, double scale, void* out,
+                   int field_dtype, int64_t n, int64_t batch, void* stream) {
+  HCIPY_REQUIRE(field_dtype == HCIPY_DTYPE_C64 || field_dtype == HCIPY_DTYPE_C128, "field must be complex");
+  HCIPY_REQUIRE(phi_dtype == HCIPY_DTYPE_F32 || phi_dtype == HCIPY_DTYPE_F64, "phase must be real");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  if (n == 0 || batch == 0) return kOk;
+  HCIPY_REQUIRE(x != nullptr && out != nullptr && phi != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 g = grid_for(n, batch);
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((phase_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, phi, phi_dtype, phi_stride,
+                       scale, (C64*)out, n);
+  else
+    hipLaunchKernelGGL((phase_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, phi, phi_dtype,
+                       phi_stride, scale, (C128*)out, n);
+  return check_hip(hipGetLastError(), "phase_kernel");
+}
+
+int hcipy_pw_intensity(const void* x, int field_dtype, void* out, int64_t total, void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
+  HCIPY_REQUIRE(total >= 0, "bad size");
+  if (total == 0) return kOk;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 g = grid_for(total, 1);
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((intensity_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, (float*)out, total);
+  else
+    hipLaunchKernelGGL((intensity_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, (double*)out, total);
+  return check_hip(hipGetLastError(), "intensity_kernel");
+}
+
+int hcipy_pw_power_workspace(int64_t batch, size_t* bytes) {
+  HCIPY_REQUIRE(bytes != nullptr && batch >= 0, "bad argument");
+  *bytes = (size_t)batch * kPowerBlocks * sizeof(double);
+  return kOk;
+}
+
+int hcipy_pw_power(const void* x, int field_dtype, const void* mask, int mask_dtype, int64_t mask_stride,
+                   double weight, int64_t n, int64_t batch, double* out, void* workspace, size_t workspace_bytes,
+                   void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
+  HCIPY_REQUIRE(mask == nullptr || is_valid(mask_dtype), "bad mask dtype");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(workspace_bytes >= (size_t)batch * kPowerBlocks * sizeof(double), "workspace too small");
+  if (batch == 0) return kOk;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double* partial = static_cast<double*>(workspace);
+  const dim3 g(kPowerBlocks, (unsigned)batch);
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((power_partial_kernel<C64>), g, dim3(kThreads), 0, s, (const C64*)x, mask, mask_dtype,
+                       mask_stride, weight, partial, n);
+  else
+    hipLaunchKernelGGL((power_partial_kernel<C128>), g, dim3(kThreads), 0, s, (const C128*)x, mask, mask_dtype,
+                       mask_stride, weight, partial, n);
+  HCIPY_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)batch), dim3(kThreads), 0, s, partial, kPowerBlocks, out);
+  return check_hip(hipGetLastError(), "sum_partials_kernel");
+}
+
+}  // extern "C"

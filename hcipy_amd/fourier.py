@@ -1,0 +1,352 @@
+"""Fourier transforms: the FourierTransform protocol, FFT-grid geometry and the MFMA matrix
+Fourier transform.
+
+Protocol (hcipy/fourier/fourier_transform.py:17-183): ``forward(field) -> Field`` on
+``output_grid``; ``backward(field) -> Field`` on ``input_grid``; classmethods
+``check_if_supported`` (raises ValueError), ``is_supported`` and ``compute_complexity``.
+"""
+import ctypes
+import math
+from collections.abc import Iterable
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import Configuration
+from .field import CartesianGrid, Field, RegularCoords, field_dtype
+
+__all__ = [
+    'ComputationalComplexity', 'FourierTransform', 'MatrixFourierTransform', 'make_fft_grid',
+    'get_fft_parameters', 'is_fft_grid', 'complex_dtype_for',
+]
+
+
+@dataclass(frozen=True)
+class ComputationalComplexity:
+    """hcipy/fourier/fourier_transform.py:7-15."""
+    num_multiplications: int = 0
+    num_additions: int = 0
+    expected_execution_time: float = 0
+
+    @property
+    def num_operations(self):
+        return self.num_multiplications + self.num_additions
+
+
+def complex_dtype_for(dtype):
+    """float32/complex64 -> complex64, everything else -> complex128
+    (hcipy/fourier/fourier_transform.py:323-343, hcipy/optics/wavefront.py:77-88)."""
+    return torch.complex64 if dtype in (torch.float32, torch.complex64) else torch.complex128
+
+
+def _as_field_tensor(field, grid):
+    """Returns (tensor on device, complex dtype) for a Field / tensor / array input."""
+    if not isinstance(field, torch.Tensor):
+        field = Field(field, grid)
+    t = field.as_subclass(torch.Tensor) if isinstance(field, Field) else field
+    cdt = complex_dtype_for(t.dtype)
+    _lib.require_device(t)
+    return t.to(cdt).contiguous(), cdt
+
+
+class FourierTransform:
+    """Base class (hcipy/fourier/fourier_transform.py:17-183)."""
+
+    def forward(self, field):
+        raise NotImplementedError()
+
+    def backward(self, field):
+        raise NotImplementedError()
+
+    @classmethod
+    def is_supported(cls, input_grid, output_grid):
+        try:
+            cls.check_if_supported(input_grid, output_grid)
+        except ValueError:
+            return False
+        return True
+
+    @classmethod
+    def check_if_supported(cls, input_grid, output_grid):
+        raise NotImplementedError()
+
+    @classmethod
+    def compute_complexity(cls, input_grid, output_grid):
+        raise NotImplementedError()
+
+    @classmethod
+    def _predict_execution_time(cls, num_operations, prediction_coefficients):
+        """t_ms = e^a (Gop)^b + e^c, returned in seconds (fourier_transform.py:159-183)."""
+        a = prediction_coefficients['a']
+        b = prediction_coefficients['b']
+        c = prediction_coefficients['c']
+        return (math.exp(a) * (num_operations / 1e9) ** b + math.exp(c)) * 1e-3
+
+
+# ----------------------------------------------------------------------------------------------
+# FFT grid geometry (hcipy/fourier/fast_fourier_transform.py:23-130)
+
+
+def make_fft_grid(input_grid, q=1, fov=1, shift=0):
+    """Grid of an FFT of `input_grid` with zero padding q, crop fov and shift
+    (fast_fourier_transform.py:23-62)."""
+    nd = input_grid.ndim
+    if not isinstance(q, Iterable):
+        q = (q,) * nd
+    if not isinstance(fov, Iterable):
+        fov = (fov,) * nd
+    if not isinstance(shift, Iterable):
+        shift = (shift,) * nd
+    if not input_grid.is_regular:
+        raise ValueError('The input_grid must be regular.')
+    if not input_grid.is_('cartesian'):
+        raise ValueError('The input_grid must be cartesian.')
+    dims_in = [int(d) for d in input_grid.dims]
+    q = tuple(round(q_i * d_in) / d_in for q_i, d_in in zip(q, dims_in))
+    delta = tuple(float((2 * math.pi / (d_in * s_in)) / q_i) for d_in, s_in, q_i in zip(input_grid.delta, dims_in, q))
+    dims = tuple(int(d_in * f_i * q_i) for d_in, f_i, q_i in zip(dims_in, fov, q))
+    zero = tuple(float(d_i * (-dim / 2 + (dim % 2) * 0.5) + s_i) for d_i, dim, s_i in zip(delta, dims, shift))
+    return CartesianGrid(RegularCoords(delta, dims, zero))
+
+
+def get_fft_parameters(fft_grid, input_grid):
+    """Recovers (q, fov, shift) of an FFT grid or raises ValueError
+    (fast_fourier_transform.py:64-130)."""
+    if not input_grid.is_regular:
+        raise ValueError('The input grid must be regular to reconstruct an fft grid.')
+    if not fft_grid.is_regular:
+        raise ValueError('The fft grid is not regular and therefore cannot be an fft grid.')
+    if input_grid.ndim != fft_grid.ndim:
+        raise ValueError('The fft grid does not have the same number of dimensions as input_grid.')
+    dims_in = [int(d) for d in input_grid.dims]
+    fdims = [int(d) for d in fft_grid.dims]
+    q = tuple((2 * math.pi / (d_in * s_in)) / f_delta for d_in, s_in, f_delta in zip(input_grid.delta, dims_in, fft_grid.delta))
+    if any(q_i < 1 for q_i in q):
+        raise ValueError(f'fft_grid is not an FFT grid of input_grid: q of {q} would be < 1.')
+    zp = tuple(q_i * d_in for q_i, d_in in zip(q, dims_in))
+    if any(abs(zn - round(zn)) > 1e-10 for zn in zp):
+        raise ValueError(f'fft_grid is not an FFT grid of input_grid: q of {q} does not correspond to an integer zeropadding.')
+    fov = tuple(f_dim / zp_dim for f_dim, zp_dim in zip(fdims, zp))
+    if any(n > int(zn + 0.5) for n, zn in zip(fdims, zp)):
+        raise ValueError(f'fft_grid is not an FFT grid of input_grid: fov of {fov} would be > 1 .')
+    dummy = make_fft_grid(input_grid, q, fov)
+    wrong = tuple(int(n) != m for n, m in zip(dummy.dims, fdims))
+    fov = tuple((f_dim + 0.5) / (d_in * q_i) if w else f for w, f, f_dim, d_in, q_i in zip(wrong, fov, fdims, dims_in, q))
+    shift = tuple(f_zero - f_delta * (-f_dim / 2 + (f_dim % 2) * 0.5) for f_zero, f_delta, f_dim in zip(fft_grid.zero, fft_grid.delta, fdims))
+    return q, fov, shift
+
+
+def is_fft_grid(grid, input_grid):
+    try:
+        get_fft_parameters(grid, input_grid)
+    except ValueError:
+        return False
+    return True
+
+
+# ----------------------------------------------------------------------------------------------
+# Matrix Fourier transform
+
+
+class _MftPlan:
+    """Owns one C-ABI plan (device Fourier matrices in one operand format)."""
+
+    def __init__(self, input_grid, output_grid, precision, device):
+        lib = _lib.load()
+        self.device = device
+        self.precision = precision
+        with torch.cuda.device(device):
+            sep_in = input_grid.separated_coords
+            sep_out = output_grid.separated_coords
+            x, px = _lib.darray(sep_in[0])
+            u, pu = _lib.darray(sep_out[0])
+            if input_grid.ndim == 2:
+                y, py = _lib.darray(sep_in[1])
+                v, pv = _lib.darray(sep_out[1])
+                ny, nv = len(y), len(v)
+            else:
+                py = pv = None
+                ny = nv = 1
+            h = ctypes.c_void_p()
+            st = lib.hcipy_mft_plan_create(ctypes.byref(h), input_grid.ndim, _lib.PRECISIONS[precision], px, len(x),
+                                           py, ny, pu, len(u), pv, nv, _lib.stream_ptr(device))
+            _lib.check(st, 'hcipy_mft_plan_create')
+            self.handle = h
+            n = ctypes.c_size_t()
+            _lib.check(lib.hcipy_mft_plan_bytes(h, ctypes.byref(n)), 'hcipy_mft_plan_bytes')
+            self.nbytes = n.value
+
+    def workspace_bytes(self, batch):
+        n = ctypes.c_size_t()
+        _lib.check(_lib.load().hcipy_mft_workspace_size(self.handle, batch, ctypes.byref(n)), 'hcipy_mft_workspace_size')
+        return n.value
+
+    def execute(self, direction, x, out, batch, alpha, workspace):
+        st = _lib.load().hcipy_mft_execute(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch,
+                                           float(alpha.real), float(alpha.imag),
+                                           _lib.dptr(workspace) if workspace is not None else None,
+                                           0 if workspace is None else workspace.numel(),
+                                           _lib.stream_ptr(x.device))
+        _lib.check(st, 'hcipy_mft_execute')
+
+    def close(self):
+        if getattr(self, 'handle', None) is not None and _lib._lib is not None:
+            _lib.load().hcipy_mft_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MatrixFourierTransform(FourierTransform):
+    """Matrix Fourier transform between separated Cartesian grids, on MFMA.
+
+    Same protocol and flags as hcipy/fourier/matrix_fourier_transform.py:8-309.  ``forward``
+    computes ``w_in * M1 . E . M2`` with ``M1 = exp(-i v y)``, ``M2 = exp(-i x u)``;
+    ``backward`` computes ``w_out/(2 pi)^ndim * M1^H . F . M2^H``.  Leading tensor dimensions
+    of the field are one batched launch (the reference loops over them,
+    fourier_transform.py:302-321).
+
+    ``precision`` picks the complex64 arithmetic ('f16x2' or 'fp32'; default from
+    ``Configuration().fourier.mft.precision_complex64``).  complex128 always runs in FP64.
+    """
+
+    def __init__(self, input_grid, output_grid, precompute_matrices=None, allocate_intermediate=None, precision=None):
+        self.check_if_supported(input_grid, output_grid)
+        self.input_grid = input_grid
+        self.output_grid = output_grid
+        self.shape_input = input_grid.shape
+        self.shape_output = output_grid.shape
+        self.ndim = input_grid.ndim
+        cfg = Configuration().fourier.mft
+        self.precompute_matrices = cfg.precompute_matrices if precompute_matrices is None else precompute_matrices
+        self.allocate_intermediate = cfg.allocate_intermediate if allocate_intermediate is None else allocate_intermediate
+        if precision is not None and precision not in ('f16x2', 'fp32'):
+            raise ValueError("precision must be 'f16x2' or 'fp32' (complex64 fields)")
+        self.precision = precision
+        self._plans = {}
+        self._workspace = {}
+        self.matrices_dtype = None
+        self.intermediate_dtype = None
+
+    # The reference exposes M1/M2 (or M) only while matrices are kept (test_fourier.py:256-271).
+    @property
+    def M1(self):
+        return next(iter(self._plans.values())) if (self._plans and self.ndim == 2) else None
+
+    @property
+    def M2(self):
+        return self.M1
+
+    @property
+    def M(self):
+        return next(iter(self._plans.values())) if (self._plans and self.ndim == 1) else None
+
+    @property
+    def intermediate_array(self):
+        return next(iter(self._workspace.values())) if self._workspace else None
+
+    def precision_for(self, cdt):
+        if cdt == torch.complex128:
+            return 'fp64'
+        return self.precision or Configuration().fourier.mft.precision_complex64
+
+    def plan_for(self, cdt, device):
+        prec = self.precision_for(cdt)
+        key = (prec, device.index)
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = _MftPlan(self.input_grid, self.output_grid, prec, device)
+            self._plans[key] = plan
+        self.matrices_dtype = cdt
+        return plan
+
+    def _workspace_for(self, plan, batch, device):
+        need = plan.workspace_bytes(batch)
+        if need == 0:
+            return None
+        ws = self._workspace.get(device.index)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(need, dtype=torch.uint8, device=device)
+            self._workspace[device.index] = ws
+        return ws
+
+    def _release(self):
+        if not self.precompute_matrices:
+            for p in self._plans.values():
+                p.close()
+            self._plans = {}
+            self.matrices_dtype = None
+        if not self.allocate_intermediate:
+            self._workspace = {}
+
+    def _apply(self, field, direction, extra_alpha=1.0, out=None):
+        in_grid, out_grid = (self.input_grid, self.output_grid) if direction == _lib.FORWARD else (self.output_grid, self.input_grid)
+        t, cdt = _as_field_tensor(field, in_grid)
+        if t.shape[-1] != in_grid.size:
+            raise ValueError('Field size %d does not match the grid size %d' % (t.shape[-1], in_grid.size))
+        if direction == _lib.FORWARD:
+            w = in_grid.weights
+        else:
+            # weights of the transform's output grid, which is this call's input grid
+            w = in_grid.weights / (2 * np.pi) ** self.ndim
+        if np.isscalar(w) or np.size(w) == 1 or np.all(np.asarray(w) == np.asarray(w).ravel()[0]):
+            alpha = complex(float(np.asarray(w).ravel()[0])) * complex(extra_alpha)
+        else:
+            t = (t * torch.as_tensor(np.asarray(w), device=t.device)).to(cdt).contiguous()
+            alpha = complex(extra_alpha)
+        lead = tuple(t.shape[:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        plan = self.plan_for(cdt, t.device)
+        if out is None:
+            out = torch.empty(lead + (out_grid.size,), dtype=cdt, device=t.device)
+        ws = self._workspace_for(plan, batch, t.device)
+        plan.execute(direction, t, out, batch, alpha, ws)
+        self._release()
+        return Field(out, out_grid)
+
+    def forward(self, field):
+        """Forward transform of a Field on ``input_grid`` (matrix_fourier_transform.py:132-176)."""
+        return self._apply(field, _lib.FORWARD)
+
+    def backward(self, field):
+        """Adjoint-weighted inverse transform (matrix_fourier_transform.py:178-224)."""
+        return self._apply(field, _lib.BACKWARD)
+
+    @classmethod
+    def check_if_supported(cls, input_grid, output_grid):
+        """matrix_fourier_transform.py:227-254."""
+        if not input_grid.is_separated or not input_grid.is_('cartesian'):
+            raise ValueError('The input_grid must be separable in cartesian coordinates.')
+        if not output_grid.is_separated or not output_grid.is_('cartesian'):
+            raise ValueError('The output_grid must be separable in cartesian coordinates.')
+        if input_grid.ndim not in [1, 2]:
+            raise ValueError('The input_grid must be one- or two-dimensional.')
+        if input_grid.ndim != output_grid.ndim:
+            raise ValueError('The input_grid must have the same dimensions as the output_grid.')
+
+    @classmethod
+    def compute_complexity(cls, input_grid, output_grid):
+        """Real-operation count of the two GEMMs (matrix_fourier_transform.py:256-309)."""
+        cls.check_if_supported(input_grid, output_grid)
+        if input_grid.ndim == 1:
+            cm = input_grid.size * output_grid.size
+            ca = (input_grid.size - 1) * output_grid.size
+        else:
+            nox, noy = output_grid.shape
+            nix, niy = input_grid.shape
+            cm = niy * nix * noy
+            ca = noy * nix * (niy - 1)
+            cm += nix * nox * noy
+            ca += nox * noy * (nix - 1)
+            cm += input_grid.size
+        num_mult = 4 * cm
+        num_add = 2 * cm + 2 * ca
+        coeffs = Configuration().fourier.mft.execution_time_prediction_coefficients
+        t = FourierTransform._predict_execution_time(num_mult + num_add, coeffs)
+        return ComputationalComplexity(num_multiplications=num_mult, num_additions=num_add, expected_execution_time=t)

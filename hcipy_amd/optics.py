@@ -1,0 +1,231 @@
+"""Wavefronts and optical elements (hcipy/optics/wavefront.py, hcipy/optics/optical_element.py,
+hcipy/optics/apodization.py).
+
+Protocol: ``OpticalElement.forward(wavefront) -> Wavefront``, ``backward`` the adjoint-style
+inverse, ``__call__ = forward`` (optical_element.py:15-66).
+"""
+import copy
+import math
+from collections import OrderedDict
+
+import numpy as np
+import torch
+
+from . import pointwise
+from .field import Field, field_dtype
+
+__all__ = ['Wavefront', 'OpticalElement', 'AgnosticOpticalElement', 'OpticalSystem', 'Apodizer', 'PhaseApodizer']
+
+
+def _tensor(f):
+    return f.as_subclass(torch.Tensor) if isinstance(f, Field) else f
+
+
+class Wavefront:
+    """Scalar (or Jones-vector) electric field with a wavelength (wavefront.py:15-360).
+
+    The electric field is cast to complex64 when it is float32/complex64 and to complex128
+    otherwise (wavefront.py:77-88).
+    """
+
+    def __init__(self, electric_field, wavelength=1, input_stokes_vector=None):
+        if input_stokes_vector is not None:
+            raise NotImplementedError('partially polarised wavefronts are outside this library')
+        self.electric_field = electric_field
+        self.wavelength = wavelength
+        self._input_stokes_vector = None
+
+    def copy(self):
+        return Wavefront(Field(_tensor(self._electric_field).clone(), self._electric_field.grid), self.wavelength)
+
+    @property
+    def electric_field(self):
+        return self._electric_field
+
+    @electric_field.setter
+    def electric_field(self, electric_field):
+        if not hasattr(electric_field, 'grid'):
+            raise ValueError('The electric field must be a Field.')
+        t = _tensor(electric_field) if isinstance(electric_field, torch.Tensor) else Field(electric_field, electric_field.grid).as_subclass(torch.Tensor)
+        cdt = torch.complex64 if t.dtype in (torch.float32, torch.complex64) else torch.complex128
+        self._electric_field = Field(t.to(cdt), electric_field.grid)
+
+    @property
+    def input_stokes_vector(self):
+        return self._input_stokes_vector
+
+    @property
+    def wavenumber(self):
+        return 2 * np.pi / self.wavelength
+
+    @wavenumber.setter
+    def wavenumber(self, k):
+        self.wavelength = 2 * np.pi / k
+
+    @property
+    def grid(self):
+        return self._electric_field.grid
+
+    @property
+    def is_scalar(self):
+        return self._electric_field.is_scalar_field
+
+    @property
+    def I(self):  # noqa: E743 (HCIPy name)
+        """|E|^2 summed over vector components (wavefront.py:113-143)."""
+        I = pointwise.intensity(self._electric_field)
+        if I.dim() > 1:
+            I = Field(_tensor(I).sum(dim=tuple(range(I.dim() - 1))), self.grid)
+        return I
+
+    @property
+    def intensity(self):
+        return self.I
+
+    @property
+    def amplitude(self):
+        return Field(torch.abs(_tensor(self._electric_field)), self.grid)
+
+    @property
+    def phase(self):
+        return Field(torch.angle(_tensor(self._electric_field)), self.grid)
+
+    @property
+    def power(self):
+        """Intensity times grid weights (wavefront.py:310-330)."""
+        return self.I * torch.as_tensor(np.asarray(self.grid.weights), device=self._electric_field.device)
+
+    @property
+    def total_power(self):
+        """FP64 sum of intensity x weights (wavefront.py:332-360)."""
+        return pointwise.total_power(self._electric_field, self.grid.weights)
+
+    @total_power.setter
+    def total_power(self, p):
+        cur = self.total_power
+        if cur == 0:
+            raise ValueError('Cannot set the total power of a wavefront with zero power.')
+        E = _tensor(self._electric_field) * math.sqrt(p / cur)
+        self._electric_field = Field(E, self.grid)
+
+
+class OpticalElement:
+    """Base class (optical_element.py:6-151)."""
+
+    def __call__(self, wavefront):
+        return self.forward(wavefront)
+
+    def forward(self, wavefront):
+        raise NotImplementedError()
+
+    def backward(self, wavefront):
+        raise NotImplementedError()
+
+
+def wavelength_key(wavelength):
+    """Cache key of a wavelength: round(log(wl) / log(1 + 1e-9)) (optical_element.py:333)."""
+    return int(round(math.log(wavelength) / math.log(1 + 1e-9)))
+
+
+class AgnosticOpticalElement(OpticalElement):
+    """Element whose per-(grid, wavelength) instance data is built on first use and kept in a
+    FIFO cache of at most ``max_in_cache`` entries (optical_element.py:261-380, 600-644)."""
+
+    def __init__(self, grid_dependent=True, wavelength_dependent=True, max_in_cache=11):
+        self.grid_dependent = grid_dependent
+        self.wavelength_dependent = wavelength_dependent
+        self.max_in_cache = max_in_cache
+        self._cache = OrderedDict()
+
+    class _InstanceData:
+        pass
+
+    def clear_cache(self):
+        self._cache = OrderedDict()
+
+    def get_instance_data(self, input_grid, output_grid, wavelength):
+        if input_grid is None:
+            input_grid = self.get_input_grid(output_grid, wavelength)
+        if output_grid is None:
+            output_grid = self.get_output_grid(input_grid, wavelength)
+        key = (input_grid if self.grid_dependent else None,
+               output_grid if self.grid_dependent else None,
+               wavelength_key(wavelength) if self.wavelength_dependent else None)
+        data = self._cache.get(key)
+        if data is None:
+            data = AgnosticOpticalElement._InstanceData()
+            data.input_grid = input_grid
+            data.output_grid = output_grid
+            data.wavelength = wavelength
+            self.make_instance(data, input_grid, output_grid, wavelength)
+            self._cache[key] = data
+            while len(self._cache) > self.max_in_cache:
+                self._cache.popitem(last=False)
+        return data
+
+    def evaluate_parameter(self, parameter, input_grid, output_grid, wavelength):
+        return parameter(wavelength) if callable(parameter) else parameter
+
+    def get_input_grid(self, output_grid, wavelength):
+        return output_grid
+
+    def get_output_grid(self, input_grid, wavelength):
+        return input_grid
+
+    def make_instance(self, instance_data, input_grid, output_grid, wavelength):
+        raise NotImplementedError()
+
+
+class OpticalSystem(OpticalElement):
+    """Sequence of elements; backward runs the elements' backward in reverse
+    (optical_element.py:848-949)."""
+
+    def __init__(self, optical_elements):
+        self.optical_elements = list(optical_elements)
+
+    def forward(self, wavefront):
+        wf = wavefront
+        for el in self.optical_elements:
+            wf = el.forward(wf)
+        return wf
+
+    def backward(self, wavefront):
+        wf = wavefront
+        for el in reversed(self.optical_elements):
+            wf = el.backward(wf)
+        return wf
+
+
+class Apodizer(OpticalElement):
+    """E * a forward, E * conj(a) backward (apodization.py:6-35)."""
+
+    def __init__(self, apodization):
+        self.apodization = apodization
+
+    def _get_apodization(self, wavelength):
+        return self.apodization(wavelength) if callable(self.apodization) and not isinstance(self.apodization, torch.Tensor) else self.apodization
+
+    def forward(self, wavefront):
+        a = self._get_apodization(wavefront.wavelength)
+        return Wavefront(pointwise.multiply(wavefront.electric_field, a, conjugate=False), wavefront.wavelength)
+
+    def backward(self, wavefront):
+        a = self._get_apodization(wavefront.wavelength)
+        return Wavefront(pointwise.multiply(wavefront.electric_field, a, conjugate=True), wavefront.wavelength)
+
+
+class PhaseApodizer(OpticalElement):
+    """E * exp(i phi) forward, E * exp(-i phi) backward (apodization.py:37-62)."""
+
+    def __init__(self, phase):
+        self._phase = phase
+
+    @property
+    def phase(self):
+        return self._phase
+
+    def forward(self, wavefront):
+        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self._phase, 1.0), wavefront.wavelength)
+
+    def backward(self, wavefront):
+        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self._phase, -1.0), wavefront.wavelength)

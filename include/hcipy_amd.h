@@ -1,0 +1,134 @@
+/* C ABI of the MI355X (gfx950) Fourier-propagation library.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory (for example the
+ * data_ptr() of a torch tensor on a ROCm device); `stream` is a hipStream_t passed as void*.
+ * Every function returns 0 on success or a nonzero status; hcipy_get_last_error() then returns a
+ * thread-local message.  No function throws across the ABI.
+ *
+ * Complex arrays are interleaved (re, im) in the layout of numpy / torch complex64 and
+ * complex128 arrays; a 2-D field of grid.shape (Ny, Nx) is row-major with x fastest
+ * (hcipy/field/coordinates.py:671-675), a batch of fields is contiguous.
+ */
+#ifndef HCIPY_AMD_H_
+#define HCIPY_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define HCIPY_OK 0
+#define HCIPY_ERROR_INVALID_VALUE 1
+#define HCIPY_ERROR_HIP 2
+#define HCIPY_ERROR_OUT_OF_RANGE 3
+#define HCIPY_ERROR_UNSUPPORTED 4
+
+/* Arithmetic of the MFT kernels.  The dtype contract does not depend on it:
+ * FP64 reads/writes complex128, FP32 and F16X2 read/write complex64. */
+#define HCIPY_PREC_FP64 0  /* FP64 MFMA (complex128 fields)                                 */
+#define HCIPY_PREC_FP32 1  /* FP32 MFMA, single-level FP32 accumulation (complex64 fields)  */
+#define HCIPY_PREC_F16X2 2 /* hi/lo FP16 split MFMA, FP64 flush of partial sums (complex64) */
+
+#define HCIPY_FORWARD 0
+#define HCIPY_BACKWARD 1
+
+/* Version of this ABI; bumped on any incompatible change. */
+int hcipy_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* hcipy_get_last_error(void);
+
+typedef struct hcipy_mft_plan_s* hcipy_mft_plan;
+
+/* Creates a matrix Fourier transform between separated Cartesian grids.
+ * Replaces MatrixFourierTransform.__init__ + _compute_matrices
+ * (hcipy/fourier/matrix_fourier_transform.py:42-65, 67-111).
+ *
+ * ndim      : 1 or 2.
+ * x_in[nx]  : input x coordinates (FP64, host memory); y_in[ny] the y coordinates (2-D only).
+ * u_out[nu] : output x coordinates; v_out[nv] the output y coordinates (2-D only).
+ *   For a 1-D transform ny and nv must be 1 and y_in / v_out may be NULL.
+ * precision : HCIPY_PREC_*.
+ * The Fourier matrices exp(-i v y) and exp(-i x u) (and their adjoints) are generated on the
+ * current device from FP64 phase products and stored in the precision's operand format. */
+int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const double* x_in, int64_t nx,
+                          const double* y_in, int64_t ny, const double* u_out, int64_t nu, const double* v_out,
+                          int64_t nv, void* stream);
+
+/* Releases the device matrices of a plan.  Safe on NULL. */
+int hcipy_mft_plan_destroy(hcipy_mft_plan plan);
+
+/* Bytes of device workspace hcipy_mft_execute needs for `batch` fields (either direction). */
+int hcipy_mft_workspace_size(hcipy_mft_plan plan, int64_t batch, size_t* bytes);
+
+/* Bytes of device memory held by the plan's matrices. */
+int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
+
+/* Runs `batch` transforms on `stream`.
+ * Replaces MatrixFourierTransform.forward / .backward (matrix_fourier_transform.py:132-176,
+ * 178-224) for scalar grid weights:
+ *   forward : out[b] = alpha * M1 . in[b] . M2          in: (ny, nx) -> out: (nv, nu)
+ *   backward: out[b] = alpha * M1^H . in[b] . M2^H      in: (nv, nu) -> out: (ny, nx)
+ * The caller folds the grid weight (w_in, or w_out / (2 pi)^ndim for backward) and any
+ * propagator normalisation into alpha.  `in` and `out` must not overlap.  `workspace` must hold
+ * hcipy_mft_workspace_size(plan, batch) bytes and is not read before being written.  Calls that
+ * share a workspace must be ordered on one stream.  No allocation happens in this call. */
+int hcipy_mft_execute(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch,
+                      double alpha_re, double alpha_im, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Runs hcipy_mft_execute `repeats` times on `stream` with HIP events recorded around each of its
+ * kernels, waits, and stores the mean duration in milliseconds of: kernel_ms[0] the data split
+ * (split-FP16 only, else ~0), kernel_ms[1] stage 1, kernel_ms[2] stage 2 (all batch slices).
+ * Measurement aid with no counterpart in the reference (whose MFT is timed by _time_it,
+ * hcipy/fourier/fourier_transform.py:185-215); not for production loops. */
+int hcipy_mft_profile(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch, double alpha_re,
+                      double alpha_im, void* workspace, size_t workspace_bytes, int repeats, float* kernel_ms,
+                      void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Pointwise kernels of the propagation chain.  `batch` fields of `n` samples each lie
+ * contiguously; a modulating array is broadcast over the batch when its stride is 0, or read
+ * per field with stride n.  Complex64 fields are combined in FP32, complex128 in FP64. */
+
+#define HCIPY_DTYPE_C64 0
+#define HCIPY_DTYPE_C128 1
+#define HCIPY_DTYPE_F32 2
+#define HCIPY_DTYPE_F64 3
+
+/* out = (x - y) * a    (y and a optional; conj(a) if conjugate_a).
+ * Replaces Apodizer.forward/backward (hcipy/optics/apodization.py:25-35) with y = NULL, the
+ * Babinet step F -= M F of the Lyot coronagraph (hcipy/coronagraphy/lyot.py:80) with a = 1-M,
+ * and E - P^dagger F followed by the Lyot stop (lyot.py:87-90) in one pass.  out may alias x. */
+int hcipy_pw_sub_mul(const void* x, const void* y, const void* a, int a_dtype, int64_t a_stride, int conjugate_a,
+                     void* out, int field_dtype, int64_t n, int64_t batch, void* stream);
+
+/* out = x * exp(i * scale * phi) with the phase evaluated in FP64.
+ * Replaces PhaseApodizer (hcipy/optics/apodization.py:37-62, scale = +-1) and the
+ * DeformableMirror phase exp(+-2ik s) (hcipy/optics/deformable_mirror.py:200-238,
+ * scale = +-2k, phi = surface).  out may alias x. */
+int hcipy_pw_phase(const void* x, const void* phi, int phi_dtype, int64_t phi_stride, double scale, void* out,
+                   int field_dtype, int64_t n, int64_t batch, void* stream);
+
+/* out[i] = |x[i]|^2 (float32 for complex64, float64 for complex128).
+ * Replaces Wavefront.I for scalar fields (hcipy/optics/wavefront.py:113-119). */
+int hcipy_pw_intensity(const void* x, int field_dtype, void* out, int64_t total, void* stream);
+
+/* Bytes of workspace hcipy_pw_power needs for `batch` fields. */
+int hcipy_pw_power_workspace(int64_t batch, size_t* bytes);
+
+/* out[b] = weight * sum_i mask[i] |x_b[i]|^2, accumulated in FP64 (mask optional).
+ * Replaces Wavefront.total_power (hcipy/optics/wavefront.py:332-360, weight = grid weight)
+ * and the dark-zone sums behind get_mean_intensity_in_roi (hcipy/metrics/contrast.py:37-52).
+ * Deterministic two-pass reduction; `out` is device memory. */
+int hcipy_pw_power(const void* x, int field_dtype, const void* mask, int mask_dtype, int64_t mask_stride,
+                   double weight, int64_t n, int64_t batch, double* out, void* workspace, size_t workspace_bytes,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HCIPY_AMD_H_ */

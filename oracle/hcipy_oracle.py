@@ -1,0 +1,709 @@
+"""FP64 numpy/scipy restatement of HCIPy's Fourier-propagation path (the oracle).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py``; never by the product package.
+
+Each function cites the reference file:line it restates.  How each part is pinned, so that a
+dropped term, a sign error, a swapped index or a transposed operand fails a check
+(``tests/test_oracle.py``):
+
+* grids (``make_uniform_grid``/``make_pupil_grid``/``make_focal_grid``/``make_fft_grid``/
+  ``get_fft_parameters``): the reference's own grid-reconstruction round trip
+  (tests/test_fourier.py:185-229) and hand-computed coordinates;
+* ``mft_forward``/``mft_backward``: brute-force dense DFT ``naive_ft`` (the reference's
+  NaiveFourierTransform, hcipy/fourier/naive_fourier_transform.py:82-124) on small grids,
+  Parseval/round trip at fov=1 (tests/test_fourier.py:51-59), even/odd symmetry
+  (tests/test_fourier.py:89-124), the weighted-adjoint identity, and numpy's FFT on FFT grids;
+* ``fraunhofer_forward``: closed-form Airy 2 J1(x)/x and sinc*sinc fields at the reference's
+  own tolerances (tests/test_propagation.py:6-62);
+* ``fft_forward``/``fft_backward``/``fourier_filter``: numpy.fft as library routine, the
+  MFT on the same grids, and FourierFilter == FFT.backward(FFT.forward(f) * tf)
+  (tests/test_fourier.py:273-309);
+* ``angular_spectrum_tf``/``fresnel_tf``: the Fresnel-integral closed form of a rectangular
+  aperture and +-z symmetry (tests/test_propagation.py:64-138);
+* apertures: the reference's FITS golden rasters (tests/baseline_for_apertures/circular,
+  obstructed_circular);
+* ``lyot_forward``: the explicit (1-M) Babinet algebra and the reference thresholds
+  (tests/test_coronagraphy.py:272-304);
+* ``gaussian_influence_functions``/``dm_surface``: crosstalk definition (value at the
+  neighbouring actuator) and cutoff (hcipy/optics/deformable_mirror.py:78-79).
+"""
+import math
+
+import numpy as np
+from scipy.linalg import blas
+
+# ----------------------------------------------------------------------------------------------
+# Grids (hcipy/field/coordinates.py:533-675, hcipy/field/cartesian_grid.py:83-104,167-189)
+
+
+class Grid:
+    """Cartesian grid with separated coordinates; regular when delta/zero are set.
+
+    Layout follows HCIPy: ``dims`` = (nx, ny), ``shape`` = (ny, nx), x fastest
+    (hcipy/field/coordinates.py:671-675).  ``weights`` is a scalar for regular grids
+    (hcipy/field/cartesian_grid.py:171-172) and is multiplied by |s|^ndim on scaling
+    (hcipy/field/cartesian_grid.py:83-104).
+    """
+
+    def __init__(self, delta=None, dims=None, zero=None, separated=None, weights=None):
+        if separated is not None:
+            self.sep = [np.asarray(s, dtype=float) for s in separated]
+            self.delta = None
+            self.zero = None
+            self.dims = tuple(len(s) for s in self.sep)
+        else:
+            self.delta = np.asarray(delta, dtype=float)
+            self.dims = tuple(int(d) for d in dims)
+            self.zero = np.asarray(zero, dtype=float)
+            # RegularCoords.separated_coords: arange(n) * delta + zero (coordinates.py:646-652)
+            self.sep = [np.arange(n, dtype=float) * d + z for d, n, z in zip(self.delta, self.dims, self.zero)]
+        self._weights = weights
+
+    @property
+    def is_regular(self):
+        return self.delta is not None
+
+    @property
+    def ndim(self):
+        return len(self.dims)
+
+    @property
+    def shape(self):
+        return tuple(reversed(self.dims))
+
+    @property
+    def size(self):
+        return int(np.prod(self.dims))
+
+    @property
+    def weights(self):
+        if self._weights is not None:
+            return self._weights
+        if self.is_regular:
+            return float(np.prod(self.delta))
+        # cartesian_grid.py:174-189: midpoint spacing per axis, outer product
+        w = None
+        for i, x in enumerate(self.sep):
+            wi = np.concatenate(([x[1] - x[0]], (x[2:] - x[:-2]) / 2, [x[-1] - x[-2]]))
+            shape = (1,) * (self.ndim - i - 1) + (-1,) + (1,) * i
+            wi = wi.reshape(shape)
+            w = wi if w is None else w * wi
+        return w.ravel()
+
+    def scaled(self, s):
+        """Grid.scaled: coords *= s, weights *= |s|^ndim (cartesian_grid.py:83-104)."""
+        w = self.weights
+        if np.isscalar(s):
+            w = w * abs(float(s)) ** self.ndim
+        else:
+            w = w * np.prod(np.abs(s))
+        s = np.ones(self.ndim) * s
+        if self.is_regular:
+            return Grid(self.delta * s, self.dims, self.zero * s, weights=w)
+        return Grid(separated=[x * si for x, si in zip(self.sep, s)], weights=w)
+
+    def shifted(self, shift):
+        shift = np.ones(self.ndim) * shift
+        if self.is_regular:
+            return Grid(self.delta, self.dims, self.zero + shift, weights=self._weights)
+        return Grid(separated=[x + si for x, si in zip(self.sep, shift)], weights=self._weights)
+
+    @property
+    def coords(self):
+        if self.ndim == 1:
+            return [self.sep[0]]
+        g = np.meshgrid(*self.sep)
+        return [c.ravel() for c in g]
+
+    @property
+    def x(self):
+        return self.coords[0]
+
+    @property
+    def y(self):
+        return self.coords[1]
+
+
+def make_uniform_grid(dims, extent, center=0, has_center=False):
+    """hcipy/field/util.py:68-125."""
+    dims = np.atleast_1d(dims)
+    extent = np.atleast_1d(np.asarray(extent, dtype=float))
+    n = max(len(dims), len(extent), np.size(center))
+    dims = np.ones(n, dtype=int) * dims
+    extent = np.ones(n) * extent
+    center = np.ones(n) * center
+    delta = extent / dims
+    zero = -extent / 2 + center + delta / 2
+    if has_center:
+        zero -= delta / 2 * (1 - dims % 2)
+    return Grid(delta, tuple(int(d) for d in dims), zero)
+
+
+def make_pupil_grid(dims, diameter=1):
+    """hcipy/field/util.py:127-163 (always 2-D)."""
+    return make_uniform_grid(np.ones(2, dtype=int) * dims, np.ones(2) * diameter)
+
+
+def make_focal_grid(q, num_airy, spatial_resolution=1.0):
+    """hcipy/field/util.py:222-318 (normalised or given spatial resolution)."""
+    q = np.ones(2) * q
+    num_airy = np.ones(2) * num_airy
+    sr = np.ones(2) * spatial_resolution
+    delta = sr / q
+    dims = tuple(int(2 * n * qi) for n, qi in zip(num_airy, q))
+    dims_arr = np.asarray(dims, dtype=float)
+    zero = delta * (-dims_arr / 2 + (dims_arr % 2) * 0.5)
+    return Grid(delta, dims, zero)
+
+
+def make_fft_grid(input_grid, q=1, fov=1, shift=0):
+    """hcipy/fourier/fast_fourier_transform.py:23-62."""
+    nd = input_grid.ndim
+    q = tuple(np.ones(nd) * q)
+    fov = tuple(np.ones(nd) * fov)
+    shift = tuple(np.ones(nd) * shift)
+    q = tuple(round(q_i * d_in) / d_in for q_i, d_in in zip(q, input_grid.dims))
+    delta = tuple(float((2 * math.pi / (d_in * s_in)) / q_i) for d_in, s_in, q_i in zip(input_grid.delta, input_grid.dims, q))
+    dims = tuple(int(d_in * f_i * q_i) for d_in, f_i, q_i in zip(input_grid.dims, fov, q))
+    zero = tuple(float(d_i * (-dim / 2 + (dim % 2) * 0.5) + s_i) for d_i, dim, s_i in zip(delta, dims, shift))
+    return Grid(delta, dims, zero)
+
+
+# ----------------------------------------------------------------------------------------------
+# Matrix Fourier transform (hcipy/fourier/matrix_fourier_transform.py)
+
+
+def mft_matrices(input_grid, output_grid, complex_dtype='complex128'):
+    """_compute_matrices (matrix_fourier_transform.py:67-111): FP64 phases, exp, cast."""
+    if input_grid.ndim == 1:
+        xu = np.outer(output_grid.sep[0], input_grid.sep[0])
+        return (np.exp(-1j * xu).astype(complex_dtype),)
+    x, y = input_grid.sep
+    u, v = output_grid.sep
+    M1 = np.exp(-1j * np.outer(v, y)).astype(complex_dtype)
+    M2 = np.exp(-1j * np.outer(x, u)).astype(complex_dtype)
+    return M1, M2
+
+
+def _weights(grid, div=1.0):
+    w = grid.weights / div
+    if not np.isscalar(w) and np.all(w == w[0]):
+        w = w[0]
+    return w
+
+
+def mft_forward(field, input_grid, output_grid):
+    """MatrixFourierTransform.forward (matrix_fourier_transform.py:132-176).
+
+    ``field`` is (..., Ny*Nx); complex64 input is computed with cgemm on complex64 matrices,
+    anything else with zgemm (fourier_transform.py:323-343).
+    """
+    field = np.asarray(field)
+    cdt = 'complex64' if field.dtype in (np.float32, np.complex64) else 'complex128'
+    lead = field.shape[:-1]
+    f_all = field.reshape((-1, field.shape[-1])).astype(cdt)
+    out = []
+    mats = mft_matrices(input_grid, output_grid, cdt)
+    w_in = _weights(input_grid)
+    for f in f_all:
+        if input_grid.ndim == 1:
+            out.append(mats[0].dot(f * w_in))
+            continue
+        M1, M2 = mats
+        gemm = blas.cgemm if cdt == 'complex64' else blas.zgemm
+        if np.isscalar(w_in):
+            fs = f.reshape(input_grid.shape)
+            alpha = w_in
+        else:
+            fs = (f * w_in).reshape(input_grid.shape)
+            alpha = 1
+        inter = gemm(1, M2.T, fs.T).T  # (Ny, Nu) = f . M2          (matrix_fourier_transform.py:171)
+        res = gemm(alpha, inter.T, M1.T).T  # (Nv, Nu) = alpha M1 . inter  (:172)
+        out.append(res.reshape(-1))
+    return np.asarray(out).reshape(lead + (output_grid.size,))
+
+
+def mft_backward(field, input_grid, output_grid):
+    """MatrixFourierTransform.backward (matrix_fourier_transform.py:178-224)."""
+    field = np.asarray(field)
+    cdt = 'complex64' if field.dtype in (np.float32, np.complex64) else 'complex128'
+    lead = field.shape[:-1]
+    f_all = field.reshape((-1, field.shape[-1])).astype(cdt)
+    mats = mft_matrices(input_grid, output_grid, cdt)
+    w_out = _weights(output_grid, (2 * np.pi) ** input_grid.ndim)
+    out = []
+    for f in f_all:
+        if input_grid.ndim == 1:
+            out.append(mats[0].conj().T.dot(f * w_out))
+            continue
+        M1, M2 = mats
+        gemm = blas.cgemm if cdt == 'complex64' else blas.zgemm
+        if np.isscalar(w_out):
+            fs = f.reshape(output_grid.shape)
+            alpha = w_out
+        else:
+            fs = (f * w_out).reshape(output_grid.shape)
+            alpha = 1
+        inter = gemm(1, fs.T, M1.T, trans_b=2).T  # (Ny, Nu) = M1^H . f   (:219)
+        res = gemm(alpha, M2.T, inter.T, trans_a=2).T  # (Ny, Nx) = inter . M2^H  (:220)
+        out.append(res.reshape(-1))
+    return np.asarray(out).reshape(lead + (input_grid.size,))
+
+
+def naive_ft(field, input_grid, output_grid, adjoint=False):
+    """Dense DFT on arbitrary grids (NaiveFourierTransform, naive_fourier_transform.py:82-124)."""
+    cin = np.array(input_grid.coords)
+    cout = np.array(output_grid.coords)
+    if not adjoint:
+        A = np.exp(-1j * np.dot(cout.T, cin)) * input_grid.weights
+    else:
+        A = np.exp(1j * np.dot(cin.T, cout)) * output_grid.weights / (2 * np.pi) ** input_grid.ndim
+    return A.dot(np.asarray(field, dtype=complex))
+
+
+def mft_complexity_ops(input_grid, output_grid):
+    """compute_complexity (matrix_fourier_transform.py:256-309): real mults + adds."""
+    if input_grid.ndim == 1:
+        cm = input_grid.size * output_grid.size
+        ca = (input_grid.size - 1) * output_grid.size
+    else:
+        nox, noy = output_grid.shape
+        nix, niy = input_grid.shape
+        cm = niy * nix * noy
+        ca = noy * nix * (niy - 1)
+        cm += nix * nox * noy
+        ca += nox * noy * (nix - 1)
+        cm += input_grid.size
+    return 4 * cm, 2 * cm + 2 * ca
+
+
+def predict_time(num_operations, a, b, c):
+    """FourierTransform._predict_execution_time (fourier_transform.py:159-183), seconds."""
+    return (math.exp(a) * (num_operations / 1e9) ** b + math.exp(c)) * 1e-3
+
+
+# ----------------------------------------------------------------------------------------------
+# Fraunhofer propagation (hcipy/propagation/fraunhofer.py:32-88)
+
+
+def fraunhofer_forward(field, pupil_grid, focal_grid, wavelength=1.0, focal_length=1.0):
+    uv = focal_grid.scaled(2 * np.pi / (focal_length * wavelength))
+    norm = 1 / (1j * focal_length * wavelength)
+    return mft_forward(field, pupil_grid, uv) * norm
+
+
+def fraunhofer_backward(field, pupil_grid, focal_grid, wavelength=1.0, focal_length=1.0):
+    uv = focal_grid.scaled(2 * np.pi / (focal_length * wavelength))
+    norm = 1 / (1j * focal_length * wavelength)
+    return mft_backward(field, pupil_grid, uv) / norm
+
+
+# ----------------------------------------------------------------------------------------------
+# FFT and FourierFilter (hcipy/fourier/fast_fourier_transform.py:188-406,
+# hcipy/fourier/fourier_operations.py:8-151)
+
+
+def _grid_shift(shift, grid):
+    c = grid.coords
+    return np.exp(1j * sum(s * ci for s, ci in zip(shift, c)))
+
+
+class FFT:
+    """FastFourierTransform with emulate_fftshifts as configured (default True)."""
+
+    def __init__(self, input_grid, q=1, fov=1, shift=0, emulate_fftshifts=True):
+        self.input_grid = input_grid
+        self.emulate = emulate_fftshifts
+        self.shape_in = input_grid.shape
+        self.weights = input_grid.weights
+        self.output_grid = make_fft_grid(input_grid, q, fov, shift)
+        self.internal_grid = make_fft_grid(input_grid, q, 1)
+        self.shape_out = self.output_grid.shape
+        self.internal_shape = self.internal_grid.shape
+        if self.internal_shape == self.shape_in:
+            self.cutout_input = None
+        else:
+            st = tuple(int(i / 2) - int(n / 2) for i, n in zip(self.internal_shape, self.shape_in))
+            self.cutout_input = tuple(slice(s, s + n) for s, n in zip(st, self.shape_in))
+        if self.internal_shape == self.shape_out:
+            self.cutout_output = None
+        else:
+            st = tuple(int(i / 2) - int(n / 2) for i, n in zip(self.internal_shape, self.shape_out))
+            self.cutout_output = tuple(slice(s, s + n) for s, n in zip(st, self.shape_out))
+        center = input_grid.zero + input_grid.delta * (np.asarray(input_grid.dims) // 2)
+        self.shift_input = _grid_shift(-center, self.output_grid)
+        self.shift_input /= np.fft.ifftshift(self.shift_input.reshape(self.shape_out)).ravel()[0]
+        if self.emulate:
+            f_shift = input_grid.delta * (np.asarray(self.internal_shape[::-1]) // 2)
+            fftshift = _grid_shift(f_shift, self.internal_grid)
+            if self.cutout_output:
+                self.shift_input *= fftshift.reshape(self.internal_shape)[self.cutout_output].ravel()
+            else:
+                self.shift_input *= fftshift
+        self.shift_input *= self.weights
+        shift = np.ones(input_grid.ndim) * shift
+        if np.allclose(shift, 0):
+            self.shift_output = 1
+        else:
+            self.shift_output = _grid_shift(-shift, input_grid)
+        if self.emulate:
+            f_shift = input_grid.delta * (np.asarray(self.internal_shape[::-1]) // 2)
+            fftshift = _grid_shift(f_shift, self.internal_grid)
+            fftshift *= np.exp(-1j * np.dot(f_shift, self.internal_grid.zero))
+            if self.cutout_input:
+                self.shift_output = self.shift_output * fftshift.reshape(self.internal_shape)[self.cutout_input].ravel()
+            else:
+                self.shift_output = self.shift_output * fftshift
+        if np.isscalar(self.shift_output) and np.allclose(self.shift_output, 1):
+            self.shift_output = None
+
+    def forward(self, field):
+        """fast_fourier_transform.py:325-360."""
+        arr = np.zeros(self.internal_shape, dtype=complex)
+        f = np.asarray(field, dtype=complex).reshape(self.shape_in)
+        if self.shift_output is not None:
+            f = f * self.shift_output.reshape(self.shape_in)
+        if self.cutout_input is None:
+            arr[:] = f
+        else:
+            arr[self.cutout_input] = f
+        if not self.emulate:
+            arr = np.fft.ifftshift(arr)
+        r = np.fft.fftn(arr)
+        if not self.emulate:
+            r = np.fft.fftshift(r)
+        res = r.ravel() if self.cutout_output is None else r[self.cutout_output].ravel()
+        return res * self.shift_input
+
+    def backward(self, field):
+        """fast_fourier_transform.py:362-404."""
+        arr = np.zeros(self.internal_shape, dtype=complex)
+        f = np.asarray(field, dtype=complex).reshape(self.shape_out) / self.shift_input.reshape(self.shape_out)
+        if self.cutout_output is None:
+            arr[:] = f
+        else:
+            arr[self.cutout_output] = f
+        if not self.emulate:
+            arr = np.fft.ifftshift(arr)
+        r = np.fft.ifftn(arr)
+        if not self.emulate:
+            r = np.fft.fftshift(r)
+        res = r.ravel() if self.cutout_input is None else r[self.cutout_input].ravel()
+        if self.shift_output is not None:
+            res = res / self.shift_output
+        return res
+
+
+def fourier_filter(field, input_grid, tf_internal, q=1, adjoint=False):
+    """FourierFilter._operation for a scalar transfer function (fourier_operations.py:89-151).
+
+    ``tf_internal`` is the transfer function sampled on the internal (FFT output, q, fov=1) grid
+    in that grid's natural (centred) order; it is ifftshifted here as in :41-52.
+    ``field`` may carry leading batch dimensions.
+    """
+    fft = FFT(input_grid, q)
+    internal_grid = fft.output_grid
+    tf = np.fft.ifftshift(np.asarray(tf_internal).reshape(internal_grid.shape))
+    field = np.asarray(field)
+    lead = field.shape[:-1]
+    f = field.reshape(lead + input_grid.shape)
+    if fft.cutout_input is not None:
+        arr = np.zeros(lead + internal_grid.shape, dtype=np.result_type(field.dtype, np.complex64))
+        c = (slice(None),) * len(lead) + fft.cutout_input
+        arr[c] = f
+    else:
+        arr = f.astype(np.result_type(field.dtype, np.complex64))
+        c = None
+    axes = tuple(range(-input_grid.ndim, 0))
+    tf = tf.astype(arr.dtype)
+    F = np.fft.fftn(arr, axes=axes)
+    F = F * (tf.conj() if adjoint else tf)
+    r = np.fft.ifftn(F, axes=axes)
+    if c is not None:
+        r = r[c]
+    return r.reshape(lead + (-1,)).astype(arr.dtype)
+
+
+def evaluate_supersampled(fn, grid, oversampling):
+    """evaluate_supersampled for separated grids (hcipy/field/util.py:606-698): mean over
+    the dithered sub-grids, dithers from make_uniform_grid(oversampling, 1)."""
+    dith = make_uniform_grid(np.ones(grid.ndim, dtype=int) * int(round(oversampling)), 1)
+    pts = np.array(dith.coords).T
+    deltas = []
+    for x in grid.sep:
+        d = (x[2:] - x[:-2]) / 2.
+        deltas.append(np.concatenate(([x[1] - x[0]], d, [x[-1] - x[-2]])))
+    acc = 0
+    for p in pts:
+        g = Grid(separated=[c + float(p[i]) * dl for i, (c, dl) in enumerate(zip(grid.sep, deltas))])
+        acc = acc + fn(g)
+    return acc / len(pts)
+
+
+def angular_spectrum_tf(internal_grid, wavelength, distance, num_oversampling=2, refractive_index=1):
+    """Native transfer function branch (hcipy/propagation/angular_spectrum.py:71-78)."""
+    k = 2 * np.pi / wavelength * refractive_index
+
+    def tf_native(g):
+        k_squared = np.hypot(g.x, g.y) ** 2
+        k_z = np.sqrt(k ** 2 - k_squared + 0j)
+        return np.exp(1j * k_z * distance)
+
+    return evaluate_supersampled(tf_native, internal_grid, num_oversampling)
+
+
+def angular_spectrum_uses_native_tf(input_grid, wavelength, distance):
+    """Branch rule of angular_spectrum.py:53-55 (True: native transfer function)."""
+    L_max = np.max(np.asarray(input_grid.dims) * input_grid.delta)
+    return not np.any(input_grid.delta < wavelength * abs(distance) / L_max)
+
+
+def fresnel_tf(internal_grid, wavelength, distance, num_oversampling=2, refractive_index=1):
+    """Native transfer function branch (hcipy/propagation/fresnel.py:69-80)."""
+    k = 2 * np.pi / wavelength * refractive_index
+    phase_factor = np.exp(1j * k * distance)
+    alpha = -0.5j * distance / k
+
+    def tf_native(g):
+        x = g.x
+        y = g.y
+        return phase_factor * np.exp(alpha * (x * x + y * y))
+
+    return evaluate_supersampled(tf_native, internal_grid, num_oversampling)
+
+
+def angular_spectrum_forward(field, input_grid, wavelength, distance, num_oversampling=2, adjoint=False):
+    internal = make_fft_grid(input_grid, 2, 1)
+    tf = angular_spectrum_tf(internal, wavelength, distance, num_oversampling)
+    return fourier_filter(field, input_grid, tf, q=2, adjoint=adjoint)
+
+
+def fresnel_forward(field, input_grid, wavelength, distance, num_oversampling=2, zero_padding=2, adjoint=False):
+    internal = make_fft_grid(input_grid, zero_padding, 1)
+    tf = fresnel_tf(internal, wavelength, distance, num_oversampling)
+    return fourier_filter(field, input_grid, tf, q=zero_padding, adjoint=adjoint)
+
+
+# ----------------------------------------------------------------------------------------------
+# Apertures (hcipy/aperture/generic.py:11-46, 301-348, 394-429)
+
+
+def circular_aperture(grid, diameter, center=(0, 0)):
+    x = grid.sep[0][np.newaxis, :]
+    y = grid.sep[1][:, np.newaxis]
+    return (((x - center[0]) ** 2 + (y - center[1]) ** 2) <= (diameter / 2) ** 2).ravel().astype(float)
+
+
+def rectangular_aperture(grid, size):
+    size = np.ones(2) * size
+    x = grid.sep[0][np.newaxis, :]
+    y = grid.sep[1][:, np.newaxis]
+    return ((np.abs(x) <= size[0] / 2) * (np.abs(y) <= size[1] / 2)).ravel().astype(float)
+
+
+def spider(grid, p1, p2, width):
+    delta = np.array(p2) - np.array(p1)
+    shift = delta / 2 + np.array(p1)
+    ang = np.arctan2(delta[1], delta[0])
+    length = np.linalg.norm(delta)
+    x = grid.sep[0][np.newaxis, :] - shift[0]
+    y = grid.sep[1][:, np.newaxis] - shift[1]
+    xn = x * np.cos(ang) + y * np.sin(ang)
+    yn = y * np.cos(ang) - x * np.sin(ang)
+    s = (xn <= length / 2) * (xn >= -length / 2) * (yn <= width / 2) * (yn >= -width / 2)
+    return (1 - s.ravel()).astype(float)
+
+
+def obstructed_circular_aperture(grid, pupil_diameter, obscuration_ratio, num_spiders=0, spider_width=0.01):
+    outer = circular_aperture(grid, pupil_diameter)
+    inner = circular_aperture(grid, pupil_diameter * obscuration_ratio)
+    sp = 1
+    for angle in np.linspace(0, 2 * np.pi, num_spiders, endpoint=False):
+        sp = sp * spider(grid, (0, 0), (pupil_diameter * np.cos(angle), pupil_diameter * np.sin(angle)), spider_width)
+    return (outer - inner) * sp
+
+
+# ----------------------------------------------------------------------------------------------
+# Optics: apodizers, Lyot coronagraph, DM, intensity (hcipy/optics/apodization.py:6-62,
+# hcipy/coronagraphy/lyot.py:66-122, hcipy/optics/deformable_mirror.py:43-93,200-263,
+# hcipy/optics/wavefront.py:113-143,310-360, hcipy/metrics/contrast.py:37-52)
+
+
+def lyot_forward(E, pupil_grid, fpm_grid, fpm, lyot_stop=None, wavelength=1.0, focal_length=1.0):
+    """lyot.py:66-92: F = P E; F -= M F; L = E - P^dagger F; L *= stop."""
+    F = fraunhofer_forward(E, pupil_grid, fpm_grid, wavelength, focal_length)
+    F = F - F * fpm
+    L = fraunhofer_backward(F, pupil_grid, fpm_grid, wavelength, focal_length)
+    L = E - L
+    if lyot_stop is not None:
+        L = L * lyot_stop
+    return L
+
+
+def lyot_backward(E, pupil_grid, fpm_grid, fpm, lyot_stop=None, wavelength=1.0, focal_length=1.0):
+    """lyot.py:94-122 (forward propagator, conj(M), backward propagator, subtract)."""
+    wf = E * np.conj(lyot_stop) if lyot_stop is not None else E
+    F = fraunhofer_forward(wf, pupil_grid, fpm_grid, wavelength, focal_length)
+    F = F - F * np.conj(fpm)
+    P = fraunhofer_backward(F, pupil_grid, fpm_grid, wavelength, focal_length)
+    return wf - P
+
+
+def actuator_positions(n_act, pitch):
+    """make_actuator_positions without tilts (deformable_mirror.py:11-41)."""
+    return make_uniform_grid(n_act, [pitch * n_act, pitch * n_act])
+
+
+def gaussian_influence_functions(pupil_grid, n_act, pitch, crosstalk=0.15, cutoff=3):
+    """make_gaussian_influence_functions (deformable_mirror.py:43-93) with
+    make_gaussian_pokes (mode_basis/gaussian_pokes.py:5-54), no tilt, oversampling 1.
+    Returns a dense (n_act^2, Npix) array."""
+    pos = actuator_positions(n_act, pitch)
+    sigma = pitch / np.sqrt(-2 * np.log(crosstalk))
+    cut = pitch / sigma * cutoff
+    px, py = pos.coords
+    x = pupil_grid.x
+    y = pupil_grid.y
+    out = np.zeros((len(px), pupil_grid.size))
+    for i, (mx, my) in enumerate(zip(px, py)):
+        r2 = (x - mx) ** 2 + (y - my) ** 2
+        res = np.exp(-0.5 * r2 / sigma ** 2)
+        res -= np.exp(-0.5 * cut ** 2)
+        res[r2 > (cut * sigma) ** 2] = 0
+        out[i] = res
+    return out
+
+
+def dm_surface(influence, actuators):
+    """ModeBasis.linear_combination (mode_basis/mode_basis.py:239-258)."""
+    return np.asarray(actuators) @ influence
+
+
+def dm_forward(E, surface, wavelength):
+    """deformable_mirror.py:200-218: E * exp(2i k s)."""
+    k = 2 * np.pi / wavelength
+    return E * np.exp(2j * k * surface)
+
+
+def intensity(E):
+    return np.abs(E) ** 2
+
+
+def total_power(E, grid):
+    return np.sum(np.abs(E) ** 2 * grid.weights)
+
+
+def mean_intensity_in_roi(I, mask):
+    """get_mean_intensity_in_roi (metrics/contrast.py:37-52)."""
+    return np.mean(I[np.asarray(mask) > 0])
+
+
+# ----------------------------------------------------------------------------------------------
+# Synthetic inputs of the benchmark workloads (seeded).
+
+
+def gaussian_smoothed_phase(shape, sigma_px, rms, mask, seed):
+    """i.i.d. N(0,1) per pixel (numpy default_rng(seed)), Gaussian smoothing (sigma in px,
+    periodic, via FFT), scaled to `rms` radians over mask>0."""
+    rng = np.random.default_rng(seed)
+    n = rng.standard_normal(shape)
+    ky = np.fft.fftfreq(shape[0])[:, None]
+    kx = np.fft.fftfreq(shape[1])[None, :]
+    g = np.exp(-2 * np.pi ** 2 * sigma_px ** 2 * (kx ** 2 + ky ** 2))
+    ph = np.real(np.fft.ifft2(np.fft.fft2(n) * g))
+    m = np.asarray(mask).reshape(shape) > 0
+    ph *= rms / np.sqrt(np.mean(ph[m] ** 2))
+    return ph.ravel()
+
+
+def power_law_phase(shape, exponent, rms, mask, seed):
+    """Power-law PSD phase screen (PSD ~ f^exponent), FFT synthesis, rms over mask."""
+    rng = np.random.default_rng(seed)
+    ky = np.fft.fftfreq(shape[0])[:, None]
+    kx = np.fft.fftfreq(shape[1])[None, :]
+    f = np.hypot(kx, ky)
+    f[0, 0] = np.inf
+    amp = f ** (exponent / 2)
+    c = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * amp
+    ph = np.real(np.fft.ifft2(c))
+    m = np.asarray(mask).reshape(shape) > 0
+    ph -= ph[m].mean()
+    ph *= rms / np.sqrt(np.mean(ph[m] ** 2))
+    return ph.ravel()
+
+
+def read_fits_image(path):
+    """Reads the primary image of a (gzipped) FITS file: 2880-byte header blocks, big-endian
+    data of BITPIX -64 / -32 / 8 / 16 / 32."""
+    import gzip
+    with gzip.open(path, 'rb') as fh:
+        raw = fh.read()
+    hdr = {}
+    pos = 0
+    while True:
+        block = raw[pos:pos + 2880]
+        pos += 2880
+        end = False
+        for i in range(0, 2880, 80):
+            card = block[i:i + 80].decode('ascii')
+            key = card[:8].strip()
+            if key == 'END':
+                end = True
+                break
+            if card[8:10] == '= ':
+                val = card[10:].split('/')[0].strip()
+                hdr[key] = val
+        if end:
+            break
+    bitpix = int(hdr['BITPIX'])
+    naxis = int(hdr['NAXIS'])
+    dims = [int(hdr['NAXIS%d' % (i + 1)]) for i in range(naxis)]
+    dt = {-64: '>f8', -32: '>f4', 8: 'u1', 16: '>i2', 32: '>i4'}[bitpix]
+    n = int(np.prod(dims))
+    data = np.frombuffer(raw[pos:pos + n * np.dtype(dt).itemsize], dtype=dt).astype(float)
+    return data.reshape(dims[::-1])
+
+
+# ----------------------------------------------------------------------------------------------
+# CPU baseline: the reference's MFT with precomputed matrices (precompute_matrices=True,
+# allocate_intermediate=True; hcipy/fourier/matrix_fourier_transform.py:67-111, 132-176).
+
+
+class MftCpu:
+    """Two BLAS gemm calls with the reference's transpose pattern (matrix_fourier_transform.py:171-172)."""
+
+    def __init__(self, input_grid, output_grid, complex_dtype='complex64'):
+        self.M1, self.M2 = mft_matrices(input_grid, output_grid, complex_dtype)
+        self.shape_in = input_grid.shape
+        self.alpha = input_grid.weights
+        self.gemm = blas.cgemm if complex_dtype == 'complex64' else blas.zgemm
+        self.inter = np.empty((input_grid.shape[0], self.M2.shape[1]), dtype=complex_dtype)
+
+    def forward(self, f):
+        fs = f.reshape(self.shape_in)
+        self.gemm(1, self.M2.T, fs.T, c=self.inter.T, overwrite_c=True)
+        return self.gemm(self.alpha, self.inter.T, self.M1.T).T.reshape(-1)
+
+
+def time_it(function, t_max=0.1, repeat_max=5):
+    """_time_it (hcipy/fourier/fourier_transform.py:185-215): double the iteration count until
+    the run exceeds t_max, then the median of `repeat_max` runs (seconds per call)."""
+    import time
+
+    def run(n):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            function()
+        return (time.perf_counter() - t0) / n
+
+    n = 1
+    while True:
+        t = run(n)
+        if t * n > t_max:
+            break
+        n *= 2
+    if n == 1 and t > repeat_max * t_max:
+        return t
+    times = [t] + [run(n) for _ in range(repeat_max - 1)]
+    return float(np.median(times))

@@ -1,0 +1,63 @@
+"""The C-ABI library loads, exports every symbol include/hcipy_amd.h declares, and rejects
+invalid arguments with a status code and message (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from hcipy_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'hcipy_amd.h')
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(hcipy_\w+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(lib, s), s
+        assert s in _lib.SIGNATURES, 'ctypes signature missing for %s' % s
+
+
+def test_abi_version():
+    assert _lib.load().hcipy_abi_version() == 1
+
+
+def test_plan_create_rejects_bad_ndim():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    x, px = _lib.darray([0.0, 1.0])
+    st = lib.hcipy_mft_plan_create(ctypes.byref(h), 3, 0, px, 2, px, 2, px, 2, px, 2, None)
+    assert st == 1
+    assert b'one- or two-dimensional' in lib.hcipy_get_last_error()
+    st = lib.hcipy_mft_plan_create(ctypes.byref(h), 2, 7, px, 2, px, 2, px, 2, px, 2, None)
+    assert st == 1 and b'precision' in lib.hcipy_get_last_error()
+    st = lib.hcipy_mft_plan_create(ctypes.byref(h), 1, 0, px, 2, px, 2, px, 2, px, 2, None)
+    assert st == 1
+
+
+def test_null_arguments_are_errors():
+    lib = _lib.load()
+    n = ctypes.c_size_t()
+    assert lib.hcipy_mft_workspace_size(None, 1, ctypes.byref(n)) == 1
+    assert lib.hcipy_mft_execute(None, 0, None, None, 1, 1.0, 0.0, None, 0, None) == 1
+    assert lib.hcipy_mft_plan_destroy(None) == 0
+    assert lib.hcipy_pw_sub_mul(None, None, None, 0, 0, 0, None, 5, 1, 1, None) == 1
+    assert lib.hcipy_pw_power_workspace(4, ctypes.byref(n)) == 0 and n.value > 0
+
+
+def test_kernels_refuse_cpu_tensors():
+    import torch
+    from hcipy_amd import make_pupil_grid, make_focal_grid, MatrixFourierTransform
+    g = make_pupil_grid(8)
+    mft = MatrixFourierTransform(g, make_focal_grid(2, 2))
+    with pytest.raises(_lib.HcipyError):
+        mft.forward(torch.zeros(g.size, dtype=torch.complex128))

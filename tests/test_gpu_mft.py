@@ -1,0 +1,191 @@
+"""Parity of the MFMA matrix Fourier transform with the FP64 oracle (GPU).
+
+Tolerances (peak-relative, i.e. max error / max |reference|):
+  complex128 (FP64 MFMA)                 field 1e-12
+  complex64 split FP16x2 (FP64 flushes)  intensity 1e-7 at 1024^2 -> 512^2 (the bar of the
+                                         throughput metric), field 2e-7
+  complex64 plain FP32 MFMA              intensity 3e-6, field 3e-6
+"""
+import numpy as np
+import pytest
+import torch
+
+import hcipy_amd as ha
+from hcipy_amd import workloads
+from oracle import hcipy_oracle as o
+
+from helpers import dev, host, og, peak_rel_field, peak_rel_intensity, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(n, seed, lead=()):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(lead + (n,)) + 1j * rng.standard_normal(lead + (n,))
+
+
+@pytest.mark.parametrize('dims_in,dims_out', [([7, 5], [6, 9]), ([16, 16], [8, 8]), ([33, 65], [70, 31])])
+def test_fp64_forward_backward_small(dims_in, dims_out):
+    gi = ha.make_uniform_grid(dims_in, [1.0, 0.8]).shifted([0.03, -0.02])
+    go = ha.make_uniform_grid(dims_out, [20.0, 30.0]).shifted([0.7, -0.4])
+    f = _rand(gi.size, 0)
+    F = _rand(go.size, 1)
+    mft = ha.MatrixFourierTransform(gi, go)
+    say('fp64 small', dims_in, dims_out)
+    a = host(mft.forward(ha.Field(dev(f), gi)))
+    b = o.mft_forward(f, og(gi), og(go))
+    assert peak_rel_field(a, b) < 1e-12
+    a = host(mft.backward(ha.Field(dev(F), go)))
+    b = o.mft_backward(F, og(gi), og(go))
+    assert peak_rel_field(a, b) < 1e-12
+
+
+def test_fp64_batched_tensor_field():
+    gi = ha.make_pupil_grid(24)
+    go = ha.make_focal_grid(3, 5)
+    f = _rand(gi.size, 2, (2, 3))
+    mft = ha.MatrixFourierTransform(gi, go)
+    a = host(mft.forward(ha.Field(dev(f), gi)))
+    assert a.shape == (2, 3, go.size)
+    b = o.mft_forward(f, og(gi), og(go))
+    assert peak_rel_field(a, b) < 1e-12
+    a = host(mft.backward(ha.Field(dev(b), go)))
+    assert peak_rel_field(a, o.mft_backward(b, og(gi), og(go))) < 1e-12
+
+
+def test_fp64_one_dimensional():
+    gi = ha.make_uniform_grid(65, 1.0)
+    go = ha.make_uniform_grid(50, 40.0).shifted(0.3)
+    f = _rand(gi.size, 3, (3,))
+    mft = ha.MatrixFourierTransform(gi, go)
+    a = host(mft.forward(ha.Field(dev(f), gi)))
+    assert peak_rel_field(a, o.mft_forward(f, og(gi), og(go))) < 1e-12
+    a = host(mft.backward(ha.Field(dev(f[:, :50]), go)))
+    assert peak_rel_field(a, o.mft_backward(f[:, :50], og(gi), og(go))) < 1e-12
+
+
+def test_fraunhofer_256_to_128_complex128():
+    """256^2 pupil -> 128^2 focal (q=4, 16 lambda/D), complex128, parity 1e-12."""
+    pupil, focal, E = workloads.plain_mft_256()
+    prop = ha.FraunhoferPropagator(pupil, focal)
+    say('fraunhofer 256->128 c128')
+    a = host(prop(ha.Wavefront(ha.Field(dev(E), pupil), 1.0)).electric_field)
+    b = o.fraunhofer_forward(E, og(pupil), og(focal))
+    assert peak_rel_field(a, b) < 1e-12
+    assert peak_rel_intensity(a, b) < 1e-12
+
+
+@pytest.mark.parametrize('precision,tol_i,tol_f', [('f16x2', 1e-7, 2e-7), ('fp32', 3e-6, 3e-6)])
+def test_fraunhofer_1024_to_512_complex64(precision, tol_i, tol_f):
+    """1024^2 -> 512^2 complex64 against the FP64 result on the same (complex64) input."""
+    pupil, focal, E = workloads.plain_mft_1024()
+    E64 = E.astype(np.complex64)
+    prop = ha.FraunhoferPropagator(pupil, focal, precision=precision)
+    say('fraunhofer 1024->512 c64', precision)
+    a = host(prop(ha.Wavefront(ha.Field(dev(E64), pupil), 1.0)).electric_field)
+    assert a.dtype == np.complex64
+    b = o.fraunhofer_forward(E64.astype(np.complex128), og(pupil), og(focal))
+    ei = peak_rel_intensity(a, b)
+    ef = peak_rel_field(a, b)
+    say('  intensity peak-rel %.3e  field peak-rel %.3e' % (ei, ef))
+    assert ei < tol_i
+    assert ef < tol_f
+
+
+@pytest.mark.parametrize('precision,tol', [('f16x2', 2e-7), ('fp32', 3e-6)])
+def test_backward_256_to_1024_complex64(precision, tol):
+    """Lyot return leg: 256^2 focal (q=32, 4 lambda/D) -> 1024^2 pupil, adjoint MFT."""
+    pupil = ha.make_pupil_grid(1024)
+    fpm_grid = ha.make_focal_grid(32, 4)
+    F = _rand(fpm_grid.size, 7).astype(np.complex64)
+    prop = ha.FraunhoferPropagator(pupil, fpm_grid, precision=precision)
+    a = host(prop.backward(ha.Wavefront(ha.Field(dev(F), fpm_grid), 1.0)).electric_field)
+    b = o.fraunhofer_backward(F.astype(np.complex128), og(pupil), og(fpm_grid))
+    e = peak_rel_field(a, b)
+    say('backward 256->1024', precision, 'field peak-rel %.3e' % e)
+    assert e < tol
+
+
+def test_f16x2_handles_wide_dynamic_range():
+    """Per-row power-of-two scaling keeps the split path accurate for tiny and huge fields."""
+    gi = ha.make_pupil_grid(64)
+    go = ha.make_focal_grid(4, 8)
+    f = _rand(gi.size, 9)
+    mft = ha.MatrixFourierTransform(gi, go, precision='f16x2')
+    b = o.mft_forward(f, og(gi), og(go))
+    for scale in [1e-12, 1.0, 1e9]:
+        a = host(mft.forward(ha.Field(dev(f * scale).to(torch.complex64), gi))) / scale
+        assert peak_rel_field(a, b) < 2e-6, scale
+
+
+def test_precompute_and_intermediate_flags():
+    """M1 / intermediate_array presence follows the flags (tests/test_fourier.py:256-271)."""
+    gi = ha.make_pupil_grid(32)
+    go = ha.make_fft_grid(gi, 1, 0.25)
+    f = _rand(gi.size, 4)
+    ref = None
+    for pm in [True, False]:
+        for ai in [True, False]:
+            mft = ha.MatrixFourierTransform(gi, go, precompute_matrices=pm, allocate_intermediate=ai)
+            mft.forward(ha.Field(dev(np.zeros(gi.size)), gi))
+            r = host(mft.forward(ha.Field(dev(f), gi)))
+            assert (mft.M1 is not None) == pm
+            assert (mft.intermediate_array is not None) == ai
+            ref = r if ref is None else ref
+            assert np.array_equal(r, ref)
+
+
+@pytest.mark.parametrize('dtype', ['complex128', 'complex64'])
+def test_energy_and_roundtrip_fov1(dtype):
+    """forward x backward at q=1, fov=1 is the identity; energy conserved
+    (tests/test_fourier.py:22-87 bars: c128 1e-12, c64 1e-6 / 1e-5)."""
+    g = ha.make_uniform_grid([9, 18], 1, has_center=True).shifted([0.1, 0.1]).scaled(2)
+    go = ha.make_fft_grid(g, 1, 1, [0.1, 0.1])
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal(g.size).astype(dtype)
+    mft = ha.MatrixFourierTransform(g, go)
+    F = mft.forward(ha.Field(dev(f), g))
+    back = host(mft.backward(F))
+    assert back.dtype == np.dtype(dtype)
+    e_in = np.sum(np.abs(f) ** 2) * g.weights
+    e_out = np.sum(np.abs(back) ** 2) * g.weights
+    tol = 1e-12 if dtype == 'complex128' else 1e-5
+    assert abs(e_out / e_in - 1) < tol
+    assert np.std(np.abs(back - f)) / np.abs(f).mean() < (1e-12 if dtype == 'complex128' else 1e-6)
+
+
+@pytest.mark.parametrize('dtype', ['complex128', 'complex64'])
+def test_even_odd_symmetry_and_linearity(dtype):
+    tol = 1e-12 if dtype == 'complex128' else 1e-6
+    g = ha.make_uniform_grid([9, 9], 1, has_center=False).scaled(2)
+    go = ha.make_fft_grid(g, 3, 0.8, [0.1, 0.1])
+    go = ha.make_fft_grid(g, 3, 0.8)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal(g.size)
+    fe = ((f + f[::-1]) / 2).astype(dtype)
+    fo = ((f - f[::-1]) / 2).astype(dtype)
+    mft = ha.MatrixFourierTransform(g, go)
+    Fe = host(mft.forward(ha.Field(dev(fe), g)))
+    Fo = host(mft.forward(ha.Field(dev(fo), g)))
+    assert np.abs(Fe.imag).max() < tol * max(1, np.abs(Fe).max())
+    assert np.abs(Fo.real).max() < tol * max(1, np.abs(Fo).max())
+    Fb = host(mft.forward(ha.Field(dev(f.astype(dtype)), g)))
+    assert np.allclose(Fb, Fe + Fo, atol=tol * np.abs(Fb).max())
+
+
+def test_fraunhofer_airy_closed_form():
+    """tests/test_propagation.py:6-35 through this library (512^2 pupil, bar 3e-5)."""
+    import scipy.special
+    pupil = ha.make_pupil_grid(512)
+    focal = ha.make_focal_grid(16, 8)
+    ap = ha.evaluate_supersampled(ha.make_circular_aperture(0.7), pupil, 8)
+    prop = ha.FraunhoferPropagator(pupil, focal, focal_length=0.8)
+    img = host(prop(ha.Wavefront(ap, 0.5)).electric_field)
+    img = img / img[np.argmax(np.abs(img))]
+    r = np.hypot(focal.x, focal.y)
+    x = r * np.pi / 0.5 * 0.7 / 0.8
+    ind = np.argmin(r)
+    x[ind] = 1
+    ref = 2 * scipy.special.jv(1, x) / x
+    ref[ind] = 1
+    assert np.abs(img - ref).max() < 3e-5

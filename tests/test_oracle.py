@@ -1,0 +1,301 @@
+"""Pins the CPU oracle (oracle/hcipy_oracle.py) against the reference's golden rasters, closed
+forms, brute-force DFTs and invariants.  A dropped term, sign error, swapped axis or transposed
+operand in the oracle fails at least one of these."""
+import glob
+import os
+
+import numpy as np
+import pytest
+import scipy.special
+
+from oracle import hcipy_oracle as o
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+# --- grids -----------------------------------------------------------------------------------
+
+def test_pupil_and_focal_grid_coordinates():
+    g = o.make_pupil_grid(4, 2.0)
+    assert g.shape == (4, 4)
+    np.testing.assert_array_equal(g.sep[0], [-0.75, -0.25, 0.25, 0.75])
+    assert g.weights == 0.25
+    f = o.make_focal_grid(2, 3)
+    assert f.dims == (12, 12)
+    np.testing.assert_allclose(f.sep[0][:3], [-3.0, -2.5, -2.0])
+    assert 0.0 in f.sep[0]
+    # odd sizes contain the origin too (make_focal_grid always has (0, 0))
+    f = o.make_focal_grid(1.5, 3)
+    assert f.dims == (9, 9) and np.any(np.isclose(f.sep[0], 0))
+    s = f.scaled(2.0)
+    assert s.weights == pytest.approx(f.weights * 4)
+
+
+@pytest.mark.parametrize('dims', [[8, 8], [8, 16], [9, 9], [9, 18]])
+@pytest.mark.parametrize('q', [1, 1.234, 3, 4])
+@pytest.mark.parametrize('fov', [1, 0.5, 0.8])
+def test_fft_grid_geometry(dims, q, fov):
+    """FFT grid spacing 2 pi/(N dx q), centred zero (tests/test_fourier.py:185-211)."""
+    g = o.make_uniform_grid(dims, 1, has_center=True).shifted([0.1, 0.1]).scaled(2)
+    f = o.make_fft_grid(g, q, fov, [0.1, 0.1])
+    qq = [round(q * d) / d for d in g.dims]
+    np.testing.assert_allclose(f.delta, 2 * np.pi / (np.asarray(g.dims) * g.delta) / qq)
+    assert f.dims == tuple(int(d * fov * qi) for d, qi in zip(g.dims, qq))
+
+
+# --- MFT -------------------------------------------------------------------------------------
+
+def _small_grids(seed=0):
+    rng = np.random.default_rng(seed)
+    gi = o.make_uniform_grid([7, 5], [1.0, 0.8]).shifted([0.03, -0.02])
+    go = o.make_uniform_grid([6, 9], [20.0, 30.0]).shifted([0.7, -0.4])
+    f = rng.standard_normal(gi.size) + 1j * rng.standard_normal(gi.size)
+    return gi, go, f
+
+
+def test_mft_forward_equals_bruteforce_dft():
+    gi, go, f = _small_grids()
+    a = o.mft_forward(f, gi, go)
+    b = o.naive_ft(f, gi, go)
+    assert np.abs(a - b).max() / np.abs(b).max() < 1e-13
+
+
+def test_mft_backward_equals_bruteforce_adjoint():
+    gi, go, _ = _small_grids()
+    rng = np.random.default_rng(3)
+    F = rng.standard_normal(go.size) + 1j * rng.standard_normal(go.size)
+    a = o.mft_backward(F, gi, go)
+    b = o.naive_ft(F, gi, go, adjoint=True)
+    assert np.abs(a - b).max() / np.abs(b).max() < 1e-13
+
+
+def test_mft_weighted_adjoint_identity():
+    """<A E, F>_{w_out/(2pi)^2} == <E, A^dagger F>_{w_in}: backward is the weighted adjoint."""
+    gi, go, f = _small_grids()
+    rng = np.random.default_rng(5)
+    F = rng.standard_normal(go.size) + 1j * rng.standard_normal(go.size)
+    lhs = np.vdot(F, o.mft_forward(f, gi, go)) * go.weights / (2 * np.pi) ** 2
+    rhs = np.vdot(o.mft_backward(F, gi, go), f) * gi.weights
+    assert abs(lhs - rhs) / abs(lhs) < 1e-13
+
+
+@pytest.mark.parametrize('dims', [[8, 8], [9, 18]])
+def test_mft_energy_roundtrip_fov1(dims):
+    """Parseval and exact round trip at q=1, fov=1 (tests/test_fourier.py:51-59)."""
+    g = o.make_uniform_grid(dims, 1, has_center=True)
+    go = o.make_fft_grid(g, 1, 1)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal(g.size).astype(complex)
+    F = o.mft_forward(f, g, go)
+    e_in = np.sum(np.abs(f) ** 2 * g.weights)
+    e_out = np.sum(np.abs(F) ** 2 * go.weights) / (2 * np.pi) ** 2
+    assert abs(e_out / e_in - 1) < 1e-12
+    back = o.mft_backward(F, g, go)
+    assert np.abs(back - f).max() < 1e-12
+
+
+@pytest.mark.parametrize('dims', [[8, 8], [9, 9]])
+def test_mft_even_odd_symmetry(dims):
+    """Even field -> real transform, odd -> imaginary (tests/test_fourier.py:89-124)."""
+    g = o.make_uniform_grid(dims, 1)
+    go = o.make_fft_grid(g, 3, 0.8)
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal(g.size)
+    fe = (f + f[::-1]) / 2
+    fo = (f - f[::-1]) / 2
+    assert np.abs(o.mft_forward(fe, g, go).imag).max() < 1e-12
+    assert np.abs(o.mft_forward(fo, g, go).real).max() < 1e-12
+
+
+def test_mft_matches_numpy_fft_on_fft_grid():
+    """On the native FFT grid (q=1, fov=1, centred) the MFT equals fftshift(fft(ifftshift))."""
+    n = 16
+    g = o.make_uniform_grid([n, n], 1)
+    go = o.make_fft_grid(g, 1, 1)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal(g.size) + 1j * rng.standard_normal(g.size)
+    mft = o.mft_forward(f, g, go).reshape(n, n)
+    # input centred at -1/2 pixel: phase ramps make the FFT comparable on magnitudes
+    fft = np.fft.fftshift(np.fft.fft2(f.reshape(n, n))) * g.weights
+    assert np.abs(np.abs(mft) - np.abs(fft)).max() < 1e-12
+
+
+def test_mft_complex64_path_uses_complex64():
+    gi, go, f = _small_grids()
+    a64 = o.mft_forward(f.astype(np.complex64), gi, go)
+    assert a64.dtype == np.complex64
+    a = o.mft_forward(f, gi, go)
+    assert np.abs(a64 - a).max() / np.abs(a).max() < 1e-5
+
+
+def test_complexity_counts():
+    gi = o.make_pupil_grid(1024)
+    go = o.make_focal_grid(4, 64)
+    nm, na = o.mft_complexity_ops(gi, go)
+    # 805,306,368 complex MACs + 1024^2 weight multiplies
+    assert nm == 4 * (805306368 + 1024 ** 2)
+    t = o.predict_time(nm + na, 1.667, 0.716, -4.624)
+    assert 0.015 < t < 0.025  # ~20 ms on the reference's calibration machine
+
+
+# --- Fraunhofer closed forms (tests/test_propagation.py:6-62) --------------------------------
+
+@pytest.mark.parametrize('diameter,wavelength,focal_length', [(1, 1, 1), (0.7, 0.5, 0.8)])
+def test_fraunhofer_airy(diameter, wavelength, focal_length):
+    pupil = o.make_pupil_grid(512)
+    focal = o.make_focal_grid(16, 8)
+    ap = o.evaluate_supersampled(lambda g: o.circular_aperture(g, diameter), pupil, 8)
+    img = o.fraunhofer_forward(ap, pupil, focal, wavelength, focal_length)
+    img = img / img[np.argmax(np.abs(img))]
+    r = np.hypot(focal.x, focal.y)
+    x = r * np.pi / wavelength * diameter / focal_length
+    ind = np.argmin(r)
+    x[ind] = 1
+    ref = 2 * scipy.special.jv(1, x) / x
+    ref[ind] = 1
+    assert np.abs(img - ref).max() < 3e-5
+
+
+def test_fraunhofer_sinc():
+    pupil = o.make_pupil_grid(512)
+    focal = o.make_focal_grid(16, 8)
+    size = [0.75, 1.0]
+    ap = o.evaluate_supersampled(lambda g: o.rectangular_aperture(g, size), pupil, 8)
+    img = o.fraunhofer_forward(ap, pupil, focal, 0.8, 1.3)
+    img = img / img[np.argmax(np.abs(img))]
+    kx, ky = np.array(size) / 0.8 / 1.3
+    ref = np.sinc(kx * focal.x) * np.sinc(ky * focal.y)
+    assert np.abs(img - ref).max() < 5e-5
+
+
+# --- FFT / FourierFilter / near field --------------------------------------------------------
+
+@pytest.mark.parametrize('emulate', [True, False])
+@pytest.mark.parametrize('q,fov,shift', [(1, 1, 0), (2, 0.5, 0.1), (3, 0.8, 0)])
+def test_fft_equals_mft(emulate, q, fov, shift):
+    g = o.make_uniform_grid([8, 10], 1, has_center=True).shifted([0.1, 0.1]).scaled(2)
+    fft = o.FFT(g, q, fov, shift, emulate_fftshifts=emulate)
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal(g.size) + 1j * rng.standard_normal(g.size)
+    a = fft.forward(f)
+    b = o.mft_forward(f, g, fft.output_grid)
+    assert np.abs(a - b).max() / np.abs(b).max() < 1e-12
+    F = rng.standard_normal(fft.output_grid.size) + 1j * rng.standard_normal(fft.output_grid.size)
+    if q == 1 and fov == 1:
+        assert np.abs(fft.backward(fft.forward(f)) - f).max() < 1e-12
+
+
+@pytest.mark.parametrize('n,q', [(16, 1), (17, 2), (16, 3)])
+def test_fourier_filter_equals_fft_chain(n, q):
+    """FourierFilter == FFT.backward(FFT.forward(f) * tf) (tests/test_fourier.py:273-309)."""
+    g = o.make_pupil_grid(n, 2)
+    fft = o.FFT(g, q)
+    rng = np.random.default_rng(6)
+    tf = rng.standard_normal(fft.output_grid.size) + 1j * rng.standard_normal(fft.output_grid.size)
+    f = rng.standard_normal(g.size) + 1j * rng.standard_normal(g.size)
+    a = o.fourier_filter(f, g, tf, q)
+    b = fft.backward(fft.forward(f) * tf)
+    assert np.abs(a - b).max() < 1e-10
+    a = o.fourier_filter(f, g, tf, q, adjoint=True)
+    b = fft.backward(fft.forward(f) * tf.conj())
+    assert np.abs(a - b).max() < 1e-10
+
+
+def _rect_fresnel_reference(grid, a, b, distance, wavelength):
+    """Fresnel-integral closed form of a rectangle (tests/test_propagation.py:96-125)."""
+    k = 2 * np.pi / wavelength
+
+    def gen(g):
+        s = np.sqrt(2 / (distance * wavelength))
+        fr = scipy.special.fresnel
+        S1, C1 = fr(s * (g.x - a))
+        S2, C2 = fr(s * (g.x + a))
+        S3, C3 = fr(s * (g.y - b))
+        S4, C4 = fr(s * (g.y + b))
+        ref = ((C2 + 1j * S2) - (C1 + 1j * S1)) * ((C4 + 1j * S4) - (C3 + 1j * S3))
+        ref = ref * -1j * np.exp(1j * k * distance) / 2
+        ref = ref * np.exp(1j * k * (g.x ** 2 + g.y ** 2) / (2 * distance))
+        return np.abs(ref) ** 2
+    return o.evaluate_supersampled(gen, grid, 2)
+
+
+def _subsample(f, n, s):
+    return f.reshape(n // s, s, n // s, s).mean(axis=(1, 3)).ravel()
+
+
+@pytest.mark.parametrize('which', ['fresnel', 'angular_spectrum'])
+@pytest.mark.parametrize('rel,thr', [(0.2, 0.02), (5, 0.01)])
+def test_nearfield_rectangle(which, rel, thr):
+    n, wl, a, b = 256, 500e-9, 0.001, 0.001
+    g = o.make_pupil_grid(n, [16 * a, 16 * b])
+    zc = np.min(g.delta * np.array([16 * a, 16 * b]) / wl)
+    z = rel * zc
+    ap = o.evaluate_supersampled(lambda gg: o.rectangular_aperture(gg, [2 * a, 2 * b]), g, 2)
+    if which == 'fresnel':
+        assert o.angular_spectrum_uses_native_tf(g, wl, z) or rel > 1
+        fwd = o.fresnel_forward(ap, g, wl, z)
+        neg = o.fresnel_forward(ap, g, wl, -z, adjoint=True)
+    else:
+        fwd = o.angular_spectrum_forward(ap, g, wl, z)
+        neg = o.angular_spectrum_forward(ap, g, wl, -z, adjoint=True)
+    # forward(+z) == backward(-z) (tests/test_propagation.py:88-94)
+    assert np.allclose(np.abs(fwd) ** 2, np.abs(neg) ** 2)
+    ref = _rect_fresnel_reference(g, a, b, z, wl)
+    err = np.abs(_subsample(np.abs(fwd) ** 2, n, 8) - _subsample(ref, n, 8)).max()
+    assert err < thr
+
+
+# --- apertures: the reference's golden rasters ----------------------------------------------
+
+@pytest.mark.parametrize('path', sorted(glob.glob(os.path.join(GOLDEN, 'apertures', '*.fits.gz'))))
+def test_aperture_golden(path):
+    ref = o.read_fits_image(path).ravel()
+    name = os.path.basename(path)
+    g = o.make_uniform_grid(256, [1, 1])
+    if name.startswith('pupil_'):
+        d = 0.5 if 'small' in name else 1.0
+        f = o.evaluate_supersampled(lambda gg: o.circular_aperture(gg, d), g, 8)
+    else:
+        ratio = 0.1 if 'small_obscuration' in name else 0.3
+        ns = 3 if '3spiders' in name else 5
+        w = 0.01 if 'thin' in name else 0.03
+        f = o.evaluate_supersampled(lambda gg: o.obstructed_circular_aperture(gg, 1, ratio, ns, w), g, 8)
+    assert np.allclose(f, ref)
+
+
+# --- Lyot coronagraph (tests/test_coronagraphy.py:272-304) -----------------------------------
+
+def test_lyot_suppression_and_babinet_identity():
+    pupil = o.make_pupil_grid(128, 1.1)
+    ap = o.evaluate_supersampled(lambda g: o.circular_aperture(g, 1.0), pupil, 8)
+    stop = o.evaluate_supersampled(lambda g: o.circular_aperture(g, 0.95), pupil, 8)
+    fpm_grid = o.make_focal_grid(32, 3)
+    fpm = 1 - o.evaluate_supersampled(lambda g: o.circular_aperture(g, 5), fpm_grid, 8)
+    E = ap / np.sqrt(np.sum(ap ** 2 * pupil.weights))
+    focal = o.make_focal_grid(3, 25)
+    norm = np.max(np.abs(o.fraunhofer_forward(E, pupil, focal)) ** 2 * focal.weights)
+    L = o.lyot_forward(E, pupil, fpm_grid, fpm, stop)
+    img = np.abs(o.fraunhofer_forward(L, pupil, focal)) ** 2 * focal.weights
+    assert img.max() / norm < 5e-3
+    # Babinet form == explicit occulted form  stop * P^dagger(M P E) + stop * (E - P^dagger P E)
+    F = o.fraunhofer_forward(E, pupil, fpm_grid)
+    explicit = stop * (E - o.fraunhofer_backward(F, pupil, fpm_grid) + o.fraunhofer_backward(F * fpm, pupil, fpm_grid))
+    assert np.abs(explicit - L).max() < 1e-12
+
+
+# --- DM --------------------------------------------------------------------------------------
+
+def test_gaussian_influence_crosstalk_and_cutoff():
+    pupil = o.make_pupil_grid(64, 1.0)
+    n_act = 8
+    pitch = 1.0 / n_act
+    IF = o.gaussian_influence_functions(pupil, n_act, pitch, crosstalk=0.15, cutoff=3)
+    sigma = pitch / np.sqrt(-2 * np.log(0.15))
+    # value at the neighbouring actuator relative to the peak is the crosstalk (before offset)
+    assert np.isclose(np.exp(-0.5 * pitch ** 2 / sigma ** 2), 0.15)
+    pos = o.actuator_positions(n_act, pitch)
+    px, py = pos.coords
+    r2 = (pupil.x - px[27]) ** 2 + (pupil.y - py[27]) ** 2
+    assert np.all(IF[27][r2 > (3 * pitch) ** 2] == 0)
+    s = o.dm_surface(IF, np.eye(n_act * n_act)[27] * 2.0)
+    np.testing.assert_allclose(s, 2 * IF[27])
