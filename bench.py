@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
-"""Throughput of the MFMA matrix Fourier transform: FraunhoferPropagator forward,
-1024^2 pupil -> 512^2 focal plane (q = 4, 64 lambda/D), complex64, batch 1.
+"""Throughput of the MFMA matrix Fourier transform: FraunhoferPropagator forward to the focal
+intensity, 1024^2 pupil -> 512^2 focal plane (q = 4, 64 lambda/D), complex64 input, batch 1.
 
-One step = one propagation (data split + two MFMA GEMM stages) of a field already resident in
-HBM.  The timed steps are replayed from HIP graphs of `--chunk` propagations each.  Also
-reported: the plain-FP32 MFMA path on the same workload, the peak-relative intensity error of
-both against this library's FP64 MFMA path (itself parity-tested against the FP64 oracle), the
-dominant kernel's roofline from HIP events, and the CPU baseline (numpy/scipy restatement
-of the reference's two cgemm calls on the host cores).
+One step = one propagation (data split + two MFMA GEMM stages, |E|^2 formed in the epilogue)
+of a field already resident in HBM.  The timed steps are replayed from HIP graphs of `--chunk`
+propagations each.  Also reported: the same propagation with complex64 / complex128 field
+output and the plain-FP32 MFMA path, each with its peak-relative intensity error against this
+library's FP64 MFMA path (itself parity-tested against the FP64 oracle); the dominant kernel's
+roofline from HIP events; and the CPU baseline (numpy/scipy restatement of the reference's two
+cgemm calls, timed on the host cores).
 """
 import argparse
 import ctypes
@@ -66,12 +67,14 @@ def main():
         ws = mft._workspace_for(plan, 1, dev)
         return prop, plan, ws, alpha
 
-    def time_path(precision, steps, warmup):
+    def time_path(precision, steps, warmup, output):
         prop, plan, ws, alpha = setup(precision)
-        out = torch.empty(focal.size, dtype=torch.complex64, device=dev)
+        odt = {_lib.OUTPUT_FIELD: torch.complex64, _lib.OUTPUT_INTENSITY: torch.float32,
+               _lib.OUTPUT_FIELD_C128: torch.complex128}[output]
+        out = torch.empty(focal.size, dtype=odt, device=dev)
 
         def step():
-            plan.execute(_lib.FORWARD, x, out, 1, alpha, ws)
+            plan.execute(_lib.FORWARD, x, out, 1, alpha, ws, output)
 
         for _ in range(warmup):
             step()
@@ -106,28 +109,36 @@ def main():
         # per-kernel durations from HIP events on the launching stream
         kms = (ctypes.c_float * 3)()
         st = _lib.load().hcipy_mft_profile(plan.handle, _lib.FORWARD, _lib.dptr(x), _lib.dptr(out), 1,
-                                           float(alpha.real), float(alpha.imag), _lib.dptr(ws), ws.numel(), 200,
-                                           kms, _lib.stream_ptr(dev))
+                                           float(alpha.real), float(alpha.imag), output, _lib.dptr(ws), ws.numel(),
+                                           200, kms, _lib.stream_ptr(dev))
         _lib.check(st, 'hcipy_mft_profile')
         return (t1 - t0) / steps_done, steps_done, [kms[0], kms[1], kms[2]], out.clone()
 
-    log('timing f16x2 / fp32 paths ...')
-    sec_main, steps_done, kms_main, out_main = time_path(args.precision, args.steps, args.warmup)
-    other = 'fp32' if args.precision == 'f16x2' else 'f16x2'
-    sec_other, _, kms_other, out_other = time_path(other, max(200, args.steps // 4), 20)
-
-    # Error against the FP64 MFMA path on the same (complex64-rounded) input.
+    # Reference intensity: this library's FP64 MFMA path on the same complex64-rounded input.
     prop64 = ha.FraunhoferPropagator(pupil, focal)
-    ref = prop64(ha.Wavefront(ha.Field(torch.as_tensor(E64.astype(np.complex128), device=dev), pupil),
-                              wavelength)).electric_field.as_subclass(torch.Tensor)
-    I_ref = (ref.abs() ** 2)
+    wf64 = ha.Wavefront(ha.Field(torch.as_tensor(E64.astype(np.complex128), device=dev), pupil), wavelength)
+    I_ref = prop64.forward_intensity(wf64).as_subclass(torch.Tensor)
     peak = I_ref.max().item()
 
     def err(o):
-        I = o.to(torch.complex128).abs() ** 2
+        I = o.to(torch.float64) if not o.is_complex() else o.to(torch.complex128).abs() ** 2
         return (I - I_ref).abs().max().item() / peak
 
-    err_main, err_other = err(out_main), err(out_other)
+    log('timing ...')
+    main_out = _lib.OUTPUT_INTENSITY
+    sec_main, steps_done, kms_main, out_main = time_path(args.precision, args.steps, args.warmup, main_out)
+    err_main = err(out_main)
+    variants = {}
+    for name, prec, output in [('%s_field_c64' % args.precision, args.precision, _lib.OUTPUT_FIELD),
+                               ('%s_field_c128' % args.precision, args.precision, _lib.OUTPUT_FIELD_C128),
+                               ('fp32_field_c64', 'fp32', _lib.OUTPUT_FIELD),
+                               ('fp32_intensity', 'fp32', _lib.OUTPUT_INTENSITY)]:
+        if prec == args.precision and output == main_out:
+            continue
+        sec, _, kms, o = time_path(prec, max(200, args.steps // 4), 20, output)
+        variants[name] = {'value': round(1.0 / sec, 2), 'ms_per_step': round(sec * 1e3, 5),
+                          'peak_rel_intensity_error': err(o), 'kernel_ms': [round(v, 5) for v in kms]}
+    fp32_ref = variants.get('fp32_intensity') or variants.get('fp32_field_c64')
 
     # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC).
     ny, nx = pupil.shape
@@ -172,14 +183,13 @@ def main():
         'vs_baseline': None,
         'dtype': 'fp16x2' if args.precision == 'f16x2' else 'fp32',
         'data': 'synthetic',
-        'config': {'workload': 'fraunhofer_mft_1024_to_512_complex64', 'pupil': [1024, 1024], 'focal': [512, 512],
-                   'q': 4, 'num_airy': 64, 'batch': 1, 'precision': args.precision,
+        'config': {'workload': 'fraunhofer_mft_1024_to_512_complex64_focal_intensity', 'pupil': [1024, 1024],
+                   'focal': [512, 512], 'q': 4, 'num_airy': 64, 'batch': 1, 'precision': args.precision,
+                   'output': 'focal intensity (float32) from the FP64 epilogue value',
                    'graph_chunk': None if args.no_graph else args.chunk},
         'peak_rel_intensity_error': err_main,
-        'comparison': {'precision': other, 'value': round(1.0 / sec_other, 2), 'ms_per_step': round(sec_other * 1e3, 5),
-                       'peak_rel_intensity_error': err_other,
-                       'kernel_ms': [round(v, 5) for v in kms_other],
-                       'speedup_of_%s' % args.precision: round(sec_other / sec_main, 3)},
+        'speedup_vs_fp32': round(value / fp32_ref['value'], 3) if fp32_ref else None,
+        'variants': variants,
         'algorithmic_gflop_per_step': round((flops_stage1 + flops_stage2) / 1e9, 4),
         'roofline': roofline,
         'cpu_baseline': cpu,

@@ -8,10 +8,11 @@ from .config import Configuration
 from .field import (CartesianGrid, Field, RegularCoords, SeparatedCoords, UnstructuredCoords, default_device,
                     evaluate_supersampled, make_focal_grid, make_pupil_grid, make_uniform_grid, set_default_device,
                     subsample_field)
-from .fourier import (ComputationalComplexity, FourierTransform, MatrixFourierTransform, get_fft_parameters,
-                      is_fft_grid, make_fft_grid)
+from .fourier import (ComputationalComplexity, FastFourierTransform, FourierTransform, MatrixFourierTransform,
+                      get_fft_parameters, is_fft_grid, make_fft_grid, make_fourier_transform)
 from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, OpticalSystem, PhaseApodizer, Wavefront
-from .propagation import FraunhoferPropagator
+from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, FresnelPropagator
+from .fourier_filter import FourierFilter, NearFieldTransferFunction
 from .coronagraphy import LyotCoronagraph
 from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
 from . import pointwise

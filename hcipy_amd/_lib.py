@@ -17,6 +17,10 @@ PREC_F16X2 = 2
 PRECISIONS = {'fp64': PREC_FP64, 'fp32': PREC_FP32, 'f16x2': PREC_F16X2}
 FORWARD = 0
 BACKWARD = 1
+OUTPUT_FIELD = 0
+OUTPUT_FIELD_C128 = 1
+OUTPUT_INTENSITY = 2
+OUTPUT_INTENSITY_F64 = 3
 
 _c_int64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -33,8 +37,11 @@ SIGNATURES = {
     'hcipy_mft_plan_bytes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_mft_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
                                          _vp, ctypes.c_size_t, _vp]),
+    'hcipy_mft_execute_ex': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, _vp, ctypes.c_size_t, _vp]),
     'hcipy_mft_profile': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
-                                         _vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float), _vp]),
+                                         ctypes.c_int, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                         _vp]),
     'hcipy_pw_sub_mul': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _c_int64, ctypes.c_int, _vp, ctypes.c_int,
                                         _c_int64, _c_int64, _vp]),
     'hcipy_pw_phase': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _vp, ctypes.c_int, _c_int64,
@@ -43,6 +50,15 @@ SIGNATURES = {
     'hcipy_pw_power_workspace': (ctypes.c_int, [_c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_pw_power': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _c_int64,
                                       _c_int64, _vp, _vp, ctypes.c_size_t, _vp]),
+    'hcipy_filter_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, _vp]),
+    'hcipy_filter_plan_destroy': (ctypes.c_int, [_vp]),
+    'hcipy_filter_set_transfer_function': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    'hcipy_filter_nearfield_transfer_function': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                                ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                                ctypes.c_double, ctypes.c_int, _vp]),
+    'hcipy_filter_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_filter_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, _vp, ctypes.c_size_t, _vp]),
 }
 
 DTYPE_C64 = 0

@@ -25,7 +25,8 @@ int check_hip(hipError_t e, const char* what) {
   return kHipError;
 }
 
-int launch_stage(int prec, const StageArgs& a, int nz, bool b_aligned, hipStream_t s);
+int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s);
+constexpr int kModePadded = 0, kModeAligned = 1, kModeUnaligned = 2;  // BMode of mft_gemm.hip
 int launch_split_rows(const float* in, int64_t ld_in, int K, int64_t rows, _Float16* hi, _Float16* lo,
                       int64_t ld_out, float* row_scale, hipStream_t s);
 int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, void* out,
@@ -34,6 +35,14 @@ int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double s
 namespace {
 
 constexpr int64_t kAlign = 256;
+// Padded operand rows (constant matrices, split data, intermediate) hold a multiple of this many
+// elements, the largest k-tile of the stage kernels, so full-vector loads of the last k-tile
+// stay inside the row...
+constexpr int64_t kRowPad = 128;
+// ...plus this many more, so that row starts are not all aligned to the same offset modulo the
+// L2 channel interleave (rows of 2^n bytes would send every row of a k-tile to one channel).
+constexpr int64_t kRowSkew = 64;
+int64_t padded_row(int64_t k) { return round_up(k, kRowPad) + kRowSkew; }
 constexpr int kMaxGridZ = 65535;
 
 size_t elem_size(int prec) { return prec == kF64 ? 8 : (prec == kF32 ? 4 : 2); }
@@ -44,7 +53,7 @@ int ilog2_ceil(int64_t v) {
   return e;
 }
 
-// One expanded Fourier matrix (2 * nc rows of 2 * nk reals, padded).
+// One compact Fourier matrix (nc rows of 2 * nk interleaved reals, padded).
 struct FMatrix {
   void* hi = nullptr;
   void* lo = nullptr;
@@ -66,8 +75,9 @@ int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, 
                 hipStream_t s) {
   m.nc = nc;
   m.nk = nk;
-  m.lda = round_up(2 * (int64_t)nk, 64);
-  m.rows_pad = (int)round_up(2 * (int64_t)nc, 128);
+  // compact complex rows (the stage kernel expands them); padded so tile loads stay inside
+  m.lda = padded_row(2 * (int64_t)nk);
+  m.rows_pad = (int)round_up((int64_t)nc, 64);
   const size_t bytes = (size_t)m.rows_pad * m.lda * elem_size(pl.prec);
   HCIPY_HIP(hipMalloc(&m.hi, bytes));
   pl.matrix_bytes += bytes;
@@ -127,14 +137,14 @@ Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
   };
   const int64_t rows1 = batch * (pl.ndim == 2 ? d.n1 : 1);
   if (pl.prec == kF16x2) {
-    w.ld1 = round_up(2 * d.k1, 64);
+    w.ld1 = padded_row(2 * d.k1);
     w.split_hi = take((size_t)rows1 * w.ld1 * 2);
     w.split_lo = take((size_t)rows1 * w.ld1 * 2);
     w.row_scale = take((size_t)rows1 * 4);
     w.t = take((size_t)batch * 4);
   }
   if (pl.ndim == 2) {
-    w.n1_pad = round_up(d.n1, 32);
+    w.n1_pad = padded_row(2 * d.n1) / 2;  // complex elements per intermediate row
     const size_t zelems = (size_t)batch * d.c1 * 2 * w.n1_pad;  // reals
     w.z_hi = take(zelems * elem_size(pl.prec));
     if (pl.prec == kF16x2) w.z_lo = take(zelems * 2);
@@ -146,22 +156,42 @@ Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 // ev (optional, 4 events): recorded before the split, before stage 1, before stage 2, after stage 2.
+int out_mode_for(const Plan& pl, int output) {
+  switch (output) {
+    case HCIPY_OUTPUT_FIELD: return pl.prec == kF64 ? kOutC128 : kOutC64;
+    case HCIPY_OUTPUT_FIELD_C128: return kOutC128;
+    case HCIPY_OUTPUT_INTENSITY: return pl.prec == kF64 ? kOutI64 : kOutI32;
+    case HCIPY_OUTPUT_INTENSITY_F64: return kOutI64;
+    default: return -1;
+  }
+}
+
+size_t out_elem_bytes(int mode) {
+  switch (mode) {
+    case kOutC128: return 16;
+    case kOutC64: return 8;
+    case kOutI32: return 4;
+    default: return 8;
+  }
+}
+
 int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai,
-                  char* ws, hipStream_t s, hipEvent_t* ev = nullptr) {
+                  char* ws, hipStream_t s, hipEvent_t* ev = nullptr, int output = HCIPY_OUTPUT_FIELD) {
   auto mark = [&](int i) -> int {
     if (ev != nullptr) return check_hip(hipEventRecord(ev[i], s), "hipEventRecord");
     return kOk;
   };
   const Dir d = direction(pl, dir);
   const Ws w = workspace_layout(pl, d, batch);
-  const int out_mode = pl.prec == kF64 ? kOutC128 : kOutC64;
+  const int out_mode = out_mode_for(pl, output);
+  HCIPY_REQUIRE(out_mode >= 0, "unknown output kind %d", output);
   const size_t es = elem_size(pl.prec);
   const int64_t rows1 = batch * (pl.ndim == 2 ? d.n1 : 1);
   HCIPY_REQUIRE(rows1 <= INT32_MAX / 2, "batch too large (%lld data rows)", (long long)rows1);
 
   // Stage-1 data operand.
   StageArgs a1{};
-  bool b_aligned = true;
+  int b_mode = kModePadded;
   float* row_scale = nullptr;
   float* t = nullptr;
   if (int st0 = mark(0)) return st0;
@@ -180,7 +210,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   } else {
     a1.b = in;
     a1.ldb = 2 * d.k1;
-    b_aligned = aligned16(in) && ((2 * d.k1 * (int64_t)es) % 16 == 0);
+    b_mode = (aligned16(in) && ((2 * d.k1 * (int64_t)es) % 16 == 0)) ? kModeAligned : kModeUnaligned;
   }
   a1.a = d.m1->hi;
   a1.a_lo = d.m1->lo;
@@ -198,7 +228,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
     a1.out_mode = out_mode;
     a1.alpha_re = ar;
     a1.alpha_im = ai;
-    int st1 = launch_stage(pl.prec, a1, 1, b_aligned, s);
+    int st1 = launch_stage(pl.prec, a1, 1, b_mode, s);
     if (!st1) st1 = mark(2);
     if (!st1) st1 = mark(3);
     return st1;
@@ -210,11 +240,11 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   a1.ldo = w.n1_pad;
   a1.fold = (int)d.n1;
   a1.fold_stride = d.c1 * w.n1_pad;
-  a1.out_mode = pl.prec == kF16x2 ? kOutSplit : out_mode;
+  a1.out_mode = pl.prec == kF16x2 ? kOutSplit : (pl.prec == kF64 ? kOutC128 : kOutC64);
   a1.t_out = t;
   a1.alpha_re = 1.0;
   a1.alpha_im = 0.0;
-  int st = launch_stage(pl.prec, a1, 1, b_aligned, s);
+  int st = launch_stage(pl.prec, a1, 1, b_mode, s);
   if (st != kOk) return st;
   if ((st = mark(2))) return st;
 
@@ -235,7 +265,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   a2.out_zstride = d.c2 * d.c1;
   a2.alpha_re = ar;
   a2.alpha_im = ai;
-  const size_t out_es = (pl.prec == kF64 ? 16 : 8);
+  const size_t out_es = out_elem_bytes(out_mode);
   for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridZ) {
     const int nz = (int)std::min<int64_t>(kMaxGridZ, batch - b0);
     StageArgs a = a2;
@@ -243,7 +273,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
     if (a2.b_lo) a.b_lo = static_cast<const char*>(a2.b_lo) + b0 * a2.b_zstride * es;
     a.out = static_cast<char*>(out) + b0 * a2.out_zstride * out_es;
     a.z_scale = t ? t + b0 : nullptr;
-    st = launch_stage(pl.prec, a, nz, true, s);
+    st = launch_stage(pl.prec, a, nz, kModePadded, s);
     if (st != kOk) return st;
   }
   return mark(3);
@@ -356,6 +386,13 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes) {
 
 int hcipy_mft_execute(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch,
                       double alpha_re, double alpha_im, void* workspace, size_t workspace_bytes, void* stream) {
+  return hcipy_mft_execute_ex(plan, direction_, in, out, batch, alpha_re, alpha_im, HCIPY_OUTPUT_FIELD, workspace,
+                              workspace_bytes, stream);
+}
+
+int hcipy_mft_execute_ex(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch,
+                         double alpha_re, double alpha_im, int output, void* workspace, size_t workspace_bytes,
+                         void* stream) {
   HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
   HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
   HCIPY_REQUIRE(batch >= 0, "negative batch");
@@ -370,12 +407,12 @@ int hcipy_mft_execute(hcipy_mft_plan plan, int direction_, const void* in, void*
   HCIPY_HIP(hipGetDevice(&cur));
   HCIPY_REQUIRE(cur == pl.device, "plan was created on device %d but device %d is current", pl.device, cur);
   return run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace),
-                       static_cast<hipStream_t>(stream));
+                       static_cast<hipStream_t>(stream), nullptr, output);
 }
 
 int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch, double alpha_re,
-                      double alpha_im, void* workspace, size_t workspace_bytes, int repeats, float* kernel_ms,
-                      void* stream) {
+                      double alpha_im, int output, void* workspace, size_t workspace_bytes, int repeats,
+                      float* kernel_ms, void* stream) {
   HCIPY_REQUIRE(plan != nullptr && kernel_ms != nullptr && repeats > 0, "bad argument");
   HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
   HCIPY_REQUIRE(batch > 0 && in != nullptr && out != nullptr, "bad data arguments");
@@ -391,7 +428,8 @@ int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void*
   int st = kOk;
   for (; created < nev && !st; ++created) st = check_hip(hipEventCreate(&ev[created]), "hipEventCreate");
   for (int r = 0; r < repeats && !st; ++r)
-    st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s, ev + 4 * r);
+    st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s, ev + 4 * r,
+                       output);
   if (!st) st = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
   for (int k = 0; k < 3 && !st; ++k) {
     double acc = 0.0;

@@ -1,0 +1,573 @@
+// FourierFilter on power-of-two internal grids: zero-pad -> 2-D FFT -> transfer function ->
+// 2-D inverse FFT -> crop, in three passes over HBM (reference: hcipy/fourier/fourier_operations.py:89-151).
+//
+//   pass A  rows   : load one input row into the centre of a zeroed line of length NX, forward FFT,
+//                    write the line (ny rows x NX).
+//   pass B  columns: gather C adjacent columns of pass A's output into LDS lines of length NY
+//                    (zero outside the centred ny rows), forward FFT, multiply by the transfer
+//                    function (conjugated for the adjoint), inverse FFT, write back only the
+//                    centred ny rows (the crop along y happens here).
+//   pass C  rows   : inverse FFT of each of the ny rows, write the centred nx samples scaled by
+//                    1 / (NX NY) (numpy.fft.ifftn normalisation) -- the crop along x.
+//
+// Line FFTs are radix-4 Stockham passes (one radix-2 pass when log2 N is odd) in LDS, in place,
+// with all loads of a pass in registers before its stores; twiddles come from an FP64-generated
+// table exp(-2 pi i t / N).
+#include "common.hpp"
+#include "../../include/hcipy_amd.h"
+
+#include <algorithm>
+#include <new>
+
+namespace hcipy {
+
+namespace {
+
+// Complex values as native 2-vectors (register-resident; the HIP float2/double2 structs are
+// copied through memory by the compiler).
+typedef float cf32_t __attribute__((ext_vector_type(2)));
+typedef double cf64_t __attribute__((ext_vector_type(2)));
+template <typename T> struct Cplx;
+template <> struct Cplx<float> { using V = cf32_t; };
+template <> struct Cplx<double> { using V = cf64_t; };
+
+template <typename V>
+__device__ inline V cmul(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename V>
+__device__ inline V cadd(V a, V b) {
+  V r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename V>
+__device__ inline V csub(V a, V b) {
+  V r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+// a * (s i), s = +-1
+template <typename V, typename T>
+__device__ inline V cmul_si(V a, T s) {
+  V r;
+  r.x = -s * a.y;
+  r.y = s * a.x;
+  return r;
+}
+
+// In-place FFT of one LDS line of length N (power of two) by TPL cooperating threads.
+// sign = -1: forward (numpy fft), +1: inverse without 1/N.  tw[t] = exp(-2 pi i t / N).
+template <typename T, int N, int TPL>
+__device__ void line_fft(typename Cplx<T>::V* line, int tl, const typename Cplx<T>::V* __restrict__ tw, int sign,
+                         int sync_barrier_all) {
+  using V = typename Cplx<T>::V;
+  constexpr int LOG2N = __builtin_ctz(N);
+  constexpr int NR4 = LOG2N / 2;            // radix-4 passes
+  constexpr bool R2 = (LOG2N % 2) != 0;     // one final radix-2 pass
+  constexpr int B4 = (N / 4 + TPL - 1) / TPL;  // radix-4 butterflies per thread
+  constexpr int B2 = (N / 2 + TPL - 1) / TPL;
+  const T s = (T)sign;
+  int Ns = 1;
+#pragma unroll 1
+  for (int p = 0; p < NR4; ++p) {
+    V v[B4][4];
+#pragma unroll
+    for (int b = 0; b < B4; ++b) {
+      const int j = tl + b * TPL;
+      if (j < N / 4) {
+        const int k = j & (Ns - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[b][r] = line[j + r * (N / 4)];
+        if (Ns > 1) {
+          const int step = N / (Ns * 4);
+#pragma unroll
+          for (int r = 1; r < 4; ++r) {
+            V w = tw[(r * k * step) & (N - 1)];
+            w.y *= -s;  // tw holds exp(-i..); forward (s=-1) keeps it, inverse conjugates
+            v[b][r] = cmul(v[b][r], w);
+          }
+        }
+        const V a = cadd(v[b][0], v[b][2]);
+        const V bb = csub(v[b][0], v[b][2]);
+        const V c = cadd(v[b][1], v[b][3]);
+        const V d = cmul_si(csub(v[b][1], v[b][3]), s);
+        v[b][0] = cadd(a, c);
+        v[b][2] = csub(a, c);
+        v[b][1] = cadd(bb, d);
+        v[b][3] = csub(bb, d);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B4; ++b) {
+      const int j = tl + b * TPL;
+      if (j < N / 4) {
+        const int k = j & (Ns - 1);
+        const int o = (j / Ns) * Ns * 4 + k;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) line[o + r * Ns] = v[b][r];
+      }
+    }
+    __syncthreads();
+    Ns *= 4;
+  }
+  if constexpr (R2) {
+    V v[B2][2];
+#pragma unroll
+    for (int b = 0; b < B2; ++b) {
+      const int j = tl + b * TPL;
+      if (j < N / 2) {
+        const int k = j & (Ns - 1);
+        v[b][0] = line[j];
+        v[b][1] = line[j + N / 2];
+        if (Ns > 1) {
+          V w = tw[(k * (N / (Ns * 2))) & (N - 1)];
+          w.y *= -s;
+          v[b][1] = cmul(v[b][1], w);
+        }
+        const V a = v[b][0];
+        v[b][0] = cadd(a, v[b][1]);
+        v[b][1] = csub(a, v[b][1]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B2; ++b) {
+      const int j = tl + b * TPL;
+      if (j < N / 2) {
+        const int k = j & (Ns - 1);
+        const int o = (j / Ns) * Ns * 2 + k;
+        line[o] = v[b][0];
+        line[o + Ns] = v[b][1];
+      }
+    }
+    __syncthreads();
+  }
+  (void)sync_barrier_all;
+}
+
+template <int N>
+struct RowCfg {
+  static constexpr int TPL = (N / 4 < 256) ? (N / 4 < 1 ? 1 : N / 4) : 256;
+  static constexpr int LINES = 256 / TPL;
+};
+
+// pass A: forward row FFTs of padded input rows.
+template <typename T, int NX>
+__global__ __launch_bounds__(256) void rows_forward_kernel(const typename Cplx<T>::V* __restrict__ in,
+                                                           typename Cplx<T>::V* __restrict__ w1,
+                                                           const typename Cplx<T>::V* __restrict__ twx, int nx, int ny,
+                                                           int c0x) {
+  using V = typename Cplx<T>::V;
+  constexpr int TPL = RowCfg<NX>::TPL;
+  constexpr int LINES = RowCfg<NX>::LINES;
+  __shared__ V lds[LINES][NX];
+  const int line = threadIdx.x / TPL;
+  const int tl = threadIdx.x % TPL;
+  const int y = blockIdx.x * LINES + line;
+  const int64_t b = blockIdx.y;
+  const bool valid = y < ny;
+  V* L = lds[line];
+  const V zero = {0, 0};
+  for (int i = tl; i < NX; i += TPL) {
+    V v = zero;
+    const int xi = i - c0x;
+    if (valid && xi >= 0 && xi < nx) v = in[(b * ny + y) * (int64_t)nx + xi];
+    L[i] = v;
+  }
+  __syncthreads();
+  line_fft<T, NX, TPL>(L, tl, twx, -1, 0);
+  if (valid) {
+    V* o = w1 + (b * ny + y) * (int64_t)NX;
+    for (int i = tl; i < NX; i += TPL) o[i] = L[i];
+  }
+}
+
+// pass C: inverse row FFTs, crop to nx, scale.
+template <typename T, int NX>
+__global__ __launch_bounds__(256) void rows_inverse_kernel(const typename Cplx<T>::V* __restrict__ w2,
+                                                           typename Cplx<T>::V* __restrict__ out,
+                                                           const typename Cplx<T>::V* __restrict__ twx, int nx, int ny,
+                                                           int c0x, T scale) {
+  using V = typename Cplx<T>::V;
+  constexpr int TPL = RowCfg<NX>::TPL;
+  constexpr int LINES = RowCfg<NX>::LINES;
+  __shared__ V lds[LINES][NX];
+  const int line = threadIdx.x / TPL;
+  const int tl = threadIdx.x % TPL;
+  const int y = blockIdx.x * LINES + line;
+  const int64_t b = blockIdx.y;
+  const bool valid = y < ny;
+  V* L = lds[line];
+  const V zero = {0, 0};
+  for (int i = tl; i < NX; i += TPL) L[i] = valid ? w2[(b * ny + y) * (int64_t)NX + i] : zero;
+  __syncthreads();
+  line_fft<T, NX, TPL>(L, tl, twx, +1, 0);
+  if (valid) {
+    V* o = out + (b * ny + y) * (int64_t)nx;
+    for (int i = tl; i < nx; i += TPL) {
+      V v = L[c0x + i];
+      v.x *= scale;
+      v.y *= scale;
+      o[i] = v;
+    }
+  }
+}
+
+template <typename T, int NY>
+struct ColCfg {
+  // as many lines as fit in 128 KiB of LDS, at most 16; 512 threads share them
+  static constexpr int NT = 512;
+  static constexpr int CF = 131072 / (NY * 2 * (int)sizeof(T));
+  static constexpr int C = CF > 16 ? 16 : (CF < 1 ? 1 : CF);
+  static constexpr int TPL = NT / C;
+  static constexpr int LDN = NY + 1;  // padded line length
+};
+
+// pass B: column FFT, transfer function, inverse column FFT, crop to the centred ny rows.
+template <typename T, int NY>
+__global__ __launch_bounds__(512) void columns_kernel(const typename Cplx<T>::V* __restrict__ w1,
+                                                      typename Cplx<T>::V* __restrict__ w2,
+                                                      const typename Cplx<T>::V* __restrict__ tf,
+                                                      const typename Cplx<T>::V* __restrict__ twy, int NX, int ny,
+                                                      int c0y, int conj_tf) {
+  using V = typename Cplx<T>::V;
+  using Cf = ColCfg<T, NY>;
+  constexpr int C = Cf::C;
+  constexpr int TPL = Cf::TPL;
+  constexpr int LDN = Cf::LDN;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  V* lds = reinterpret_cast<V*>(smem);
+  const int kx0 = blockIdx.x * C;
+  const int64_t b = blockIdx.y;
+  const V zero = {0, 0};
+  const int tid = threadIdx.x;
+  // zero the padding rows, gather the ny valid rows (C consecutive columns per row: coalesced)
+  for (int idx = tid; idx < C * NY; idx += Cf::NT) {
+    const int c = idx % C;
+    const int r = idx / C;
+    const int yi = r - c0y;
+    V v = zero;
+    if (yi >= 0 && yi < ny && kx0 + c < NX) v = w1[(b * ny + yi) * (int64_t)NX + kx0 + c];
+    lds[c * LDN + r] = v;
+  }
+  __syncthreads();
+  const int line = tid / TPL;
+  const int tl = tid % TPL;
+  V* L = lds + line * LDN;
+  line_fft<T, NY, TPL>(L, tl, twy, -1, 0);
+  // transfer function (already in FFT order): tf[ky][kx]
+  for (int idx = tid; idx < C * NY; idx += Cf::NT) {
+    const int c = idx % C;
+    const int r = idx / C;
+    if (kx0 + c < NX) {
+      V t = tf[(int64_t)r * NX + kx0 + c];
+      if (conj_tf) t.y = -t.y;
+      lds[c * LDN + r] = cmul(lds[c * LDN + r], t);
+    }
+  }
+  __syncthreads();
+  line_fft<T, NY, TPL>(L, tl, twy, +1, 0);
+  for (int idx = tid; idx < C * ny; idx += Cf::NT) {
+    const int c = idx % C;
+    const int yi = idx / C;
+    if (kx0 + c < NX) w2[(b * ny + yi) * (int64_t)NX + kx0 + c] = lds[c * LDN + c0y + yi];
+  }
+}
+
+__global__ void twiddle_kernel(int n, int dtype, void* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double s, c;
+  sincospi(-2.0 * (double)t / (double)n, &s, &c);
+  if (dtype == HCIPY_DTYPE_C64)
+    static_cast<float2*>(out)[t] = make_float2((float)c, (float)s);
+  else
+    static_cast<double2*>(out)[t] = make_double2(c, s);
+}
+
+// tf_out[ky][kx] = tf_centred[(ky + NY/2) % NY][(kx + NX/2) % NX]  (numpy ifftshift), cast.
+__global__ void ifftshift_cast_kernel(const void* in, int in_dtype, void* out, int out_dtype, int NX, int NY) {
+  const int64_t total = (int64_t)NX * NY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int ky = (int)(i / NX);
+    const int kx = (int)(i % NX);
+    const int64_t src = (int64_t)((ky + NY / 2) % NY) * NX + (kx + NX / 2) % NX;
+    double re, im;
+    if (in_dtype == HCIPY_DTYPE_C64) {
+      const float2 v = static_cast<const float2*>(in)[src];
+      re = v.x;
+      im = v.y;
+    } else {
+      const double2 v = static_cast<const double2*>(in)[src];
+      re = v.x;
+      im = v.y;
+    }
+    if (out_dtype == HCIPY_DTYPE_C64)
+      static_cast<float2*>(out)[i] = make_float2((float)re, (float)im);
+    else
+      static_cast<double2*>(out)[i] = make_double2(re, im);
+  }
+}
+
+// Transfer functions of the near-field propagators on the filter's internal (FFT) grid, averaged
+// over oversampling^2 dithered sub-grids (hcipy/field/util.py:606-698).  Coordinates follow the
+// reference's FP64 arithmetic: separated coords arange(n) * delta + zero, dither offsets from the
+// per-sample spacing (x[i+1] - x[i-1]) / 2 (end samples: the one-sided difference).
+//   kind 0: angular spectrum  exp(i sqrt(k^2 - |k_perp|^2 + 0j) z)      (angular_spectrum.py:71-78)
+//   kind 1: Fresnel           exp(i k z) exp(-i z / (2k) |k_perp|^2)    (fresnel.py:69-80)
+__device__ inline double sep_coord(int i, double delta, double zero) { return (double)i * delta + zero; }
+__device__ inline double sep_spacing(int i, int n, double delta, double zero) {
+  if (i == 0) return sep_coord(1, delta, zero) - sep_coord(0, delta, zero);
+  if (i == n - 1) return sep_coord(n - 1, delta, zero) - sep_coord(n - 2, delta, zero);
+  return (sep_coord(i + 1, delta, zero) - sep_coord(i - 1, delta, zero)) / 2.;
+}
+
+__global__ void nearfield_tf_kernel(int kind, int NX, int NY, double dx, double zx, double dy, double zy, double k,
+                                    double z, int ovs, double pf_re, double pf_im, void* out, int out_dtype) {
+  const int64_t total = (int64_t)NX * NY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int iy = (int)(i / NX);
+    const int ix = (int)(i % NX);
+    const double x0 = sep_coord(ix, dx, zx), y0 = sep_coord(iy, dy, zy);
+    const double sx = sep_spacing(ix, NX, dx, zx), sy = sep_spacing(iy, NY, dy, zy);
+    double acc_re = 0.0, acc_im = 0.0;
+    const double dd = 1.0 / ovs;
+    for (int a = 0; a < ovs * ovs; ++a) {
+      // dithers = make_uniform_grid(ovs, 1).points, x fastest
+      const double ox = (double)(a % ovs) * dd + (-0.5 + dd / 2);
+      const double oy = (double)(a / ovs) * dd + (-0.5 + dd / 2);
+      const double x = x0 + ox * sx;
+      const double y = y0 + oy * sy;
+      double re, im;
+      if (kind == 0) {
+        const double r = hypot(x, y);
+        const double a2 = k * k - r * r;
+        if (a2 >= 0) {
+          const double kz = sqrt(a2);
+          double s, c;
+          sincos(kz * z, &s, &c);
+          re = c;
+          im = s;
+        } else {
+          const double kappa = sqrt(-a2);
+          re = exp(-(kappa * z));
+          im = 0.0;
+        }
+      } else {
+        const double ph = (-0.5 * z / k) * (x * x + y * y);
+        double s, c;
+        sincos(ph, &s, &c);
+        re = pf_re * c - pf_im * s;
+        im = pf_re * s + pf_im * c;
+      }
+      acc_re += re;
+      acc_im += im;
+    }
+    acc_re /= (double)(ovs * ovs);
+    acc_im /= (double)(ovs * ovs);
+    // stored in FFT order (ifftshift of the centred internal grid)
+    const int oxi = (ix + NX - NX / 2) % NX;
+    const int oyi = (iy + NY - NY / 2) % NY;
+    const int64_t o = (int64_t)oyi * NX + oxi;
+    if (out_dtype == HCIPY_DTYPE_C64)
+      static_cast<float2*>(out)[o] = make_float2((float)acc_re, (float)acc_im);
+    else
+      static_cast<double2*>(out)[o] = make_double2(acc_re, acc_im);
+  }
+}
+
+struct FilterPlan {
+  int nx, ny, NX, NY, dtype, device;
+  int c0x, c0y;
+  void* tf = nullptr;
+  void* twx = nullptr;
+  void* twy = nullptr;
+};
+
+size_t cbytes(int dtype) { return dtype == HCIPY_DTYPE_C64 ? 8 : 16; }
+bool pow2(int n) { return n >= 4 && n <= 4096 && (n & (n - 1)) == 0; }
+
+template <typename T, int N>
+void launch_rows_fwd(const FilterPlan& p, const void* in, void* w1, int64_t batch, hipStream_t s) {
+  using V = typename Cplx<T>::V;
+  constexpr int LINES = RowCfg<N>::LINES;
+  dim3 g((unsigned)ceil_div(p.ny, LINES), (unsigned)batch);
+  hipLaunchKernelGGL((rows_forward_kernel<T, N>), g, dim3(256), 0, s, (const V*)in, (V*)w1, (const V*)p.twx, p.nx,
+                     p.ny, p.c0x);
+}
+template <typename T, int N>
+void launch_rows_inv(const FilterPlan& p, const void* w2, void* out, int64_t batch, hipStream_t s) {
+  using V = typename Cplx<T>::V;
+  constexpr int LINES = RowCfg<N>::LINES;
+  dim3 g((unsigned)ceil_div(p.ny, LINES), (unsigned)batch);
+  const T scale = (T)(1.0 / ((double)p.NX * (double)p.NY));
+  hipLaunchKernelGGL((rows_inverse_kernel<T, N>), g, dim3(256), 0, s, (const V*)w2, (V*)out, (const V*)p.twx, p.nx,
+                     p.ny, p.c0x, scale);
+}
+template <typename T, int N>
+int launch_cols(const FilterPlan& p, const void* w1, void* w2, int conj, int64_t batch, hipStream_t s) {
+  using V = typename Cplx<T>::V;
+  using Cf = ColCfg<T, N>;
+  const size_t lds = (size_t)Cf::C * Cf::LDN * sizeof(V);
+  auto kern = columns_kernel<T, N>;
+  HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  dim3 g((unsigned)ceil_div(p.NX, Cf::C), (unsigned)batch);
+  hipLaunchKernelGGL(kern, g, dim3(Cf::NT), lds, s, (const V*)w1, (V*)w2, (const V*)p.tf, (const V*)p.twy, p.NX,
+                     p.ny, p.c0y, conj);
+  return kOk;
+}
+
+#define HCIPY_POW2_SWITCH(N, CALL) \
+  switch (N) {                     \
+    case 4: CALL(4); break;        \
+    case 8: CALL(8); break;        \
+    case 16: CALL(16); break;      \
+    case 32: CALL(32); break;      \
+    case 64: CALL(64); break;      \
+    case 128: CALL(128); break;    \
+    case 256: CALL(256); break;    \
+    case 512: CALL(512); break;    \
+    case 1024: CALL(1024); break;  \
+    case 2048: CALL(2048); break;  \
+    case 4096: CALL(4096); break;  \
+    default: break;                \
+  }
+
+template <typename T>
+int run_filter(const FilterPlan& p, int adjoint, const void* in, void* out, int64_t batch, char* ws, hipStream_t s) {
+  const size_t lbytes = (size_t)batch * p.ny * p.NX * cbytes(p.dtype);
+  void* w1 = ws;
+  void* w2 = ws + round_up((int64_t)lbytes, 256);
+#define ROWS_F(n) launch_rows_fwd<T, n>(p, in, w1, batch, s)
+  HCIPY_POW2_SWITCH(p.NX, ROWS_F)
+#undef ROWS_F
+  HCIPY_HIP(hipGetLastError());
+  int st = kOk;
+#define COLS(n) st = launch_cols<T, n>(p, w1, w2, adjoint, batch, s)
+  HCIPY_POW2_SWITCH(p.NY, COLS)
+#undef COLS
+  if (st) return st;
+  HCIPY_HIP(hipGetLastError());
+#define ROWS_I(n) launch_rows_inv<T, n>(p, w2, out, batch, s)
+  HCIPY_POW2_SWITCH(p.NX, ROWS_I)
+#undef ROWS_I
+  return check_hip(hipGetLastError(), "fourier filter launch");
+}
+
+}  // namespace
+}  // namespace hcipy
+
+using namespace hcipy;
+
+struct hcipy_filter_plan_s {
+  FilterPlan p;
+};
+
+extern "C" {
+
+int hcipy_filter_plan_create(hcipy_filter_plan* plan, int nx, int ny, int NX, int NY, int dtype, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr, "plan pointer is NULL");
+  *plan = nullptr;
+  HCIPY_REQUIRE(dtype == HCIPY_DTYPE_C64 || dtype == HCIPY_DTYPE_C128, "filter dtype must be complex");
+  HCIPY_REQUIRE(pow2(NX) && pow2(NY), "internal grid %d x %d: FFT kernels need powers of two in [4, 4096]", NX, NY);
+  HCIPY_REQUIRE(nx >= 1 && ny >= 1 && nx <= NX && ny <= NY, "input %d x %d does not fit the internal grid", nx, ny);
+  hcipy_filter_plan_s* h = new (std::nothrow) hcipy_filter_plan_s();
+  HCIPY_REQUIRE(h != nullptr, "out of host memory");
+  FilterPlan& p = h->p;
+  p.nx = nx;
+  p.ny = ny;
+  p.NX = NX;
+  p.NY = NY;
+  p.dtype = dtype;
+  // FastFourierTransform.cutout_input: int(internal/2) - int(input/2) (fast_fourier_transform.py:262-266)
+  p.c0x = NX / 2 - nx / 2;
+  p.c0y = NY / 2 - ny / 2;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int st = check_hip(hipGetDevice(&p.device), "hipGetDevice");
+  if (!st) st = check_hip(hipMalloc(&p.tf, (size_t)NX * NY * cbytes(dtype)), "hipMalloc(tf)");
+  if (!st) st = check_hip(hipMalloc(&p.twx, (size_t)NX * cbytes(dtype)), "hipMalloc(twiddles)");
+  if (!st) st = check_hip(hipMalloc(&p.twy, (size_t)NY * cbytes(dtype)), "hipMalloc(twiddles)");
+  if (!st) {
+    hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)ceil_div(NX, 256)), dim3(256), 0, s, NX, dtype, p.twx);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)ceil_div(NY, 256)), dim3(256), 0, s, NY, dtype, p.twy);
+    st = check_hip(hipGetLastError(), "twiddle_kernel");
+  }
+  if (!st) st = check_hip(hipMemsetAsync(p.tf, 0, (size_t)NX * NY * cbytes(dtype), s), "hipMemsetAsync");
+  if (st) {
+    hcipy_filter_plan_destroy(h);
+    return st;
+  }
+  *plan = h;
+  return kOk;
+}
+
+int hcipy_filter_plan_destroy(hcipy_filter_plan plan) {
+  if (plan == nullptr) return kOk;
+  if (plan->p.tf) (void)hipFree(plan->p.tf);
+  if (plan->p.twx) (void)hipFree(plan->p.twx);
+  if (plan->p.twy) (void)hipFree(plan->p.twy);
+  delete plan;
+  return kOk;
+}
+
+int hcipy_filter_set_transfer_function(hcipy_filter_plan plan, const void* tf_centred, int tf_dtype, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr && tf_centred != nullptr, "NULL argument");
+  HCIPY_REQUIRE(tf_dtype == HCIPY_DTYPE_C64 || tf_dtype == HCIPY_DTYPE_C128, "transfer function must be complex");
+  const FilterPlan& p = plan->p;
+  const int64_t total = (int64_t)p.NX * p.NY;
+  hipLaunchKernelGGL(ifftshift_cast_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256),
+                     0, static_cast<hipStream_t>(stream), tf_centred, tf_dtype, p.tf, p.dtype, p.NX, p.NY);
+  return check_hip(hipGetLastError(), "ifftshift_cast_kernel");
+}
+
+int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, double delta_x, double zero_x,
+                                             double delta_y, double zero_y, double wavenumber, double distance,
+                                             int oversampling, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr, "NULL plan");
+  HCIPY_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (angular spectrum) or 1 (Fresnel)");
+  HCIPY_REQUIRE(oversampling >= 1 && oversampling <= 16, "oversampling out of range");
+  const FilterPlan& p = plan->p;
+  // Fresnel phase factor exp(i k z), computed on the host in FP64 as the reference does.
+  const double pf_re = cos(wavenumber * distance), pf_im = sin(wavenumber * distance);
+  const int64_t total = (int64_t)p.NX * p.NY;
+  hipLaunchKernelGGL(nearfield_tf_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), kind, p.NX, p.NY, delta_x, zero_x, delta_y, zero_y, wavenumber,
+                     distance, oversampling, pf_re, pf_im, p.tf, p.dtype);
+  return check_hip(hipGetLastError(), "nearfield_tf_kernel");
+}
+
+int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* bytes) {
+  HCIPY_REQUIRE(plan != nullptr && bytes != nullptr && batch >= 0, "bad argument");
+  const FilterPlan& p = plan->p;
+  const size_t lbytes = (size_t)batch * p.ny * p.NX * cbytes(p.dtype);
+  *bytes = 2 * (size_t)round_up((int64_t)lbytes, 256);
+  return kOk;
+}
+
+int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, void* out, int64_t batch,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr, "NULL plan");
+  HCIPY_REQUIRE(batch >= 0 && batch <= 65535, "batch out of range");
+  if (batch == 0) return kOk;
+  HCIPY_REQUIRE(in != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
+  size_t need = 0;
+  hcipy_filter_workspace_size(plan, batch, &need);
+  HCIPY_REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+  int cur = -1;
+  HCIPY_HIP(hipGetDevice(&cur));
+  HCIPY_REQUIRE(cur == plan->p.device, "plan was created on device %d but device %d is current", plan->p.device, cur);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (plan->p.dtype == HCIPY_DTYPE_C64)
+    return run_filter<float>(plan->p, adjoint ? 1 : 0, in, out, batch, static_cast<char*>(workspace), s);
+  return run_filter<double>(plan->p, adjoint ? 1 : 0, in, out, batch, static_cast<char*>(workspace), s);
+}
+
+}  // extern "C"

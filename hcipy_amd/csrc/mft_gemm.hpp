@@ -36,6 +36,8 @@ enum OutMode : int {
   kOutC128 = 0,   // interleaved complex128
   kOutC64 = 1,    // interleaved complex64
   kOutSplit = 2,  // interleaved hi/lo FP16 pair arrays (feeds the next kF16x2 stage)
+  kOutI32 = 3,    // |z|^2 as float32, evaluated from the FP64 result
+  kOutI64 = 4,    // |z|^2 as float64
 };
 
 template <int P> struct PrecTraits;

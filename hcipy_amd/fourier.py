@@ -183,13 +183,13 @@ class _MftPlan:
         _lib.check(_lib.load().hcipy_mft_workspace_size(self.handle, batch, ctypes.byref(n)), 'hcipy_mft_workspace_size')
         return n.value
 
-    def execute(self, direction, x, out, batch, alpha, workspace):
-        st = _lib.load().hcipy_mft_execute(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch,
-                                           float(alpha.real), float(alpha.imag),
-                                           _lib.dptr(workspace) if workspace is not None else None,
-                                           0 if workspace is None else workspace.numel(),
-                                           _lib.stream_ptr(x.device))
-        _lib.check(st, 'hcipy_mft_execute')
+    def execute(self, direction, x, out, batch, alpha, workspace, output=_lib.OUTPUT_FIELD):
+        st = _lib.load().hcipy_mft_execute_ex(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch,
+                                              float(alpha.real), float(alpha.imag), output,
+                                              _lib.dptr(workspace) if workspace is not None else None,
+                                              0 if workspace is None else workspace.numel(),
+                                              _lib.stream_ptr(x.device))
+        _lib.check(st, 'hcipy_mft_execute_ex')
 
     def close(self):
         if getattr(self, 'handle', None) is not None and _lib._lib is not None:
@@ -285,7 +285,7 @@ class MatrixFourierTransform(FourierTransform):
         if not self.allocate_intermediate:
             self._workspace = {}
 
-    def _apply(self, field, direction, extra_alpha=1.0, out=None):
+    def _apply(self, field, direction, extra_alpha=1.0, out=None, output=_lib.OUTPUT_FIELD):
         in_grid, out_grid = (self.input_grid, self.output_grid) if direction == _lib.FORWARD else (self.output_grid, self.input_grid)
         t, cdt = _as_field_tensor(field, in_grid)
         if t.shape[-1] != in_grid.size:
@@ -304,9 +304,12 @@ class MatrixFourierTransform(FourierTransform):
         batch = int(np.prod(lead)) if lead else 1
         plan = self.plan_for(cdt, t.device)
         if out is None:
-            out = torch.empty(lead + (out_grid.size,), dtype=cdt, device=t.device)
+            odt = {_lib.OUTPUT_FIELD: cdt, _lib.OUTPUT_FIELD_C128: torch.complex128,
+                   _lib.OUTPUT_INTENSITY: torch.float64 if cdt == torch.complex128 else torch.float32,
+                   _lib.OUTPUT_INTENSITY_F64: torch.float64}[output]
+            out = torch.empty(lead + (out_grid.size,), dtype=odt, device=t.device)
         ws = self._workspace_for(plan, batch, t.device)
-        plan.execute(direction, t, out, batch, alpha, ws)
+        plan.execute(direction, t, out, batch, alpha, ws, output)
         self._release()
         return Field(out, out_grid)
 
@@ -317,6 +320,15 @@ class MatrixFourierTransform(FourierTransform):
     def backward(self, field):
         """Adjoint-weighted inverse transform (matrix_fourier_transform.py:178-224)."""
         return self._apply(field, _lib.BACKWARD)
+
+    def forward_intensity(self, field, float64=False):
+        """|forward(field)|^2 evaluated in the GEMM epilogue from the FP64 result (float32 for
+        complex64 fields unless ``float64``), before any rounding of the field."""
+        return self._apply(field, _lib.FORWARD, output=_lib.OUTPUT_INTENSITY_F64 if float64 else _lib.OUTPUT_INTENSITY)
+
+    def forward_c128(self, field):
+        """forward(field) returned as complex128 whatever the arithmetic."""
+        return self._apply(field, _lib.FORWARD, output=_lib.OUTPUT_FIELD_C128)
 
     @classmethod
     def check_if_supported(cls, input_grid, output_grid):
@@ -350,3 +362,87 @@ class MatrixFourierTransform(FourierTransform):
         coeffs = Configuration().fourier.mft.execution_time_prediction_coefficients
         t = FourierTransform._predict_execution_time(num_mult + num_add, coeffs)
         return ComputationalComplexity(num_multiplications=num_mult, num_additions=num_add, expected_execution_time=t)
+
+
+# ----------------------------------------------------------------------------------------------
+# FastFourierTransform and the planner
+
+
+class FastFourierTransform(FourierTransform):
+    """Zero-padded / cropped / shifted Fourier transform onto an FFT grid
+    (hcipy/fourier/fast_fourier_transform.py:188-482).
+
+    The reference's FFT output (with its input/output phase-ramp corrections) equals the matrix
+    Fourier transform onto ``make_fft_grid(input_grid, q, fov, shift)``; this class runs that
+    transform on MFMA.  ``emulate_fftshifts`` is accepted for interface compatibility (it only
+    changes the reference's rounding).
+    """
+
+    def __init__(self, input_grid, q=1, fov=1, shift=0, emulate_fftshifts=None, precision=None):
+        if not input_grid.is_regular:
+            raise ValueError('The input_grid must be regular.')
+        if not input_grid.is_('cartesian'):
+            raise ValueError('The input_grid must be Cartesian.')
+        q_check = (q,) if np.isscalar(q) else q
+        fov_check = (fov,) if np.isscalar(fov) else fov
+        if any(q_i < 1 for q_i in q_check):
+            raise ValueError('The amount of zeropadding (q) must be larger than 1.')
+        if any(fov_i < 0 for fov_i in fov_check):
+            raise ValueError('The amount of cropping (fov) must be positive.')
+        self.input_grid = input_grid
+        self.emulate_fftshifts = Configuration().fourier.fft.emulate_fftshifts if emulate_fftshifts is None else emulate_fftshifts
+        self.output_grid = make_fft_grid(input_grid, q, fov, shift)
+        self.internal_grid = make_fft_grid(input_grid, q, 1)
+        if np.any(np.asarray(self.output_grid.dims) > np.asarray(self.internal_grid.dims)):
+            raise ValueError('The amount of cropping (fov) must be smaller than 1.')
+        self.shape_in = input_grid.shape
+        self.shape_out = self.output_grid.shape
+        self.internal_shape = self.internal_grid.shape
+        self._mft = MatrixFourierTransform(input_grid, self.output_grid, precision=precision)
+
+    def forward(self, field):
+        return self._mft.forward(field)
+
+    def backward(self, field):
+        return self._mft.backward(field)
+
+    @classmethod
+    def check_if_supported(cls, input_grid, output_grid):
+        get_fft_parameters(output_grid, input_grid)
+
+    @classmethod
+    def compute_complexity(cls, input_grid, output_grid):
+        """Operation count of the reference FFT (fast_fourier_transform.py:430-482)."""
+        q, _, shift = get_fft_parameters(output_grid, input_grid)
+        shape = tuple(n * qq for n, qq in zip(input_grid.shape, q))
+        n_int = math.prod(shape)
+        cm = 0.5 * n_int * math.log2(n_int)
+        ca = n_int * math.log2(n_int)
+        if not all(abs(s) <= 1e-8 for s in shift):
+            cm += input_grid.size
+        cm += output_grid.size
+        num_mult = 4 * cm
+        num_add = 2 * cm + 2 * ca
+        coeffs = Configuration().fourier.fft.execution_time_prediction_coefficients
+        t = FourierTransform._predict_execution_time(num_mult + num_add, coeffs)
+        return ComputationalComplexity(num_multiplications=num_mult, num_additions=num_add, expected_execution_time=t)
+
+
+def make_fourier_transform(input_grid, output_grid=None, q=1, fov=1, shift=0, planner='estimate', precision=None):
+    """Fourier transform between two grids (hcipy/fourier/fourier_transform.py:217-300).
+
+    With no output grid the FFT grid of (q, fov, shift) is used.  Every transform of this
+    library between separated Cartesian grids runs on the MFMA matrix Fourier transform, so the
+    planner returns a MatrixFourierTransform whenever the grids are separated and raises
+    ValueError otherwise (unstructured grids have no device implementation here).
+    """
+    if planner not in ('estimate', 'measure'):
+        raise ValueError("planner must be 'estimate' or 'measure'")
+    if output_grid is None:
+        output_grid = make_fft_grid(input_grid, q, fov, shift)
+    if MatrixFourierTransform.is_supported(input_grid, output_grid):
+        return MatrixFourierTransform(input_grid, output_grid, precision=precision)
+    raise ValueError('No Fourier transform of this library supports these grids (separated Cartesian grids are required).')
+
+
+__all__ += ['FastFourierTransform', 'make_fourier_transform']

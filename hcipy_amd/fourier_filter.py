@@ -1,0 +1,152 @@
+"""FourierFilter: zero-pad -> FFT -> transfer function -> inverse FFT -> crop
+(hcipy/fourier/fourier_operations.py:8-151).
+
+Power-of-two internal grids run on the library's three-pass Stockham FFT kernels
+(``hcipy_filter_*``).  Other sizes run the same operator as two matrix Fourier transforms on
+MFMA (input grid -> internal frequency grid -> cropped input grid); on the FFT grid the two
+give identical results up to rounding, since the weights multiply to 1/(NX NY) and the
+kernel exp(i u (x' - x)) is N-periodic.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .field import Field
+from .fourier import MatrixFourierTransform, complex_dtype_for, make_fft_grid
+
+__all__ = ['FourierFilter', 'NearFieldTransferFunction']
+
+
+class NearFieldTransferFunction:
+    """Transfer function generated on the device in FP64 (kind 'angular_spectrum' or 'fresnel'),
+    averaged over oversampling^2 dithered sub-grids of the internal grid."""
+
+    KINDS = {'angular_spectrum': 0, 'fresnel': 1}
+
+    def __init__(self, kind, wavenumber, distance, oversampling=2):
+        self.kind = kind
+        self.wavenumber = float(wavenumber)
+        self.distance = float(distance)
+        self.oversampling = int(round(oversampling))
+
+
+def _pow2(n):
+    return 4 <= n <= 4096 and (n & (n - 1)) == 0
+
+
+class _FilterPlan:
+    def __init__(self, filt, cdt, device):
+        lib = _lib.load()
+        self.device = device
+        self.dtype = cdt
+        ny, nx = filt.input_grid.shape
+        NY, NX = filt.internal_grid.shape
+        dcode = _lib.DTYPE_C64 if cdt == torch.complex64 else _lib.DTYPE_C128
+        h = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            stream = _lib.stream_ptr(device)
+            _lib.check(lib.hcipy_filter_plan_create(ctypes.byref(h), nx, ny, NX, NY, dcode, stream), 'hcipy_filter_plan_create')
+            self.handle = h
+            tf = filt.transfer_function
+            if isinstance(tf, NearFieldTransferFunction):
+                g = filt.internal_grid
+                st = lib.hcipy_filter_nearfield_transfer_function(
+                    h, NearFieldTransferFunction.KINDS[tf.kind], float(g.delta[0]), float(g.zero[0]), float(g.delta[1]),
+                    float(g.zero[1]), tf.wavenumber, tf.distance, tf.oversampling, stream)
+                _lib.check(st, 'hcipy_filter_nearfield_transfer_function')
+            else:
+                t = filt.evaluate_transfer_function(device).to(cdt).contiguous()
+                _lib.check(lib.hcipy_filter_set_transfer_function(h, _lib.dptr(t), dcode, stream),
+                           'hcipy_filter_set_transfer_function')
+                torch.cuda.current_stream(device).synchronize()
+        self._ws = None
+
+    def workspace(self, batch):
+        n = ctypes.c_size_t()
+        _lib.check(_lib.load().hcipy_filter_workspace_size(self.handle, batch, ctypes.byref(n)), 'hcipy_filter_workspace_size')
+        if self._ws is None or self._ws.numel() < n.value:
+            self._ws = torch.empty(max(n.value, 256), dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def execute(self, adjoint, x, out, batch):
+        ws = self.workspace(batch)
+        st = _lib.load().hcipy_filter_execute(self.handle, int(adjoint), _lib.dptr(x), _lib.dptr(out), batch,
+                                              _lib.dptr(ws), ws.numel(), _lib.stream_ptr(x.device))
+        _lib.check(st, 'hcipy_filter_execute')
+
+    def __del__(self):
+        try:
+            if getattr(self, 'handle', None) is not None and _lib._lib is not None:
+                _lib.load().hcipy_filter_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class FourierFilter:
+    """Filter in the Fourier domain with zero padding q (fourier_operations.py:8-151).
+
+    ``transfer_function`` is a Field on the internal grid (centred order), a callable that
+    maps the internal grid to such a Field, or a NearFieldTransferFunction.
+    """
+
+    def __init__(self, input_grid, transfer_function, q=1):
+        if not input_grid.is_regular or input_grid.ndim != 2:
+            raise ValueError('FourierFilter needs a regular two-dimensional input grid.')
+        self.input_grid = input_grid
+        self.internal_grid = make_fft_grid(input_grid, q, 1)
+        self.q = q
+        self.shape_in = input_grid.shape
+        self.transfer_function = transfer_function
+        self._plans = {}
+        self._mft = None
+
+    def evaluate_transfer_function(self, device):
+        tf = self.transfer_function
+        if callable(tf) and not isinstance(tf, torch.Tensor):
+            tf = tf(self.internal_grid)
+        if not isinstance(tf, torch.Tensor):
+            tf = torch.as_tensor(np.asarray(tf), device=device)
+        tf = tf.as_subclass(torch.Tensor) if isinstance(tf, Field) else tf
+        if tf.dim() != 1 or tf.shape[0] != self.internal_grid.size:
+            raise ValueError('only scalar transfer functions on the internal grid are supported')
+        return tf.to(device)
+
+    def _operation(self, field, adjoint):
+        grid = self.input_grid
+        if not isinstance(field, torch.Tensor):
+            field = Field(field, grid)
+        t = field.as_subclass(torch.Tensor) if isinstance(field, Field) else field
+        _lib.require_device(t)
+        cdt = complex_dtype_for(t.dtype)
+        t = t.to(cdt).contiguous()
+        lead = tuple(t.shape[:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        NY, NX = self.internal_grid.shape
+        if _pow2(NX) and _pow2(NY) and batch <= 65535:
+            key = (cdt, t.device.index)
+            plan = self._plans.get(key)
+            if plan is None:
+                plan = _FilterPlan(self, cdt, t.device)
+                self._plans[key] = plan
+            out = torch.empty_like(t)
+            plan.execute(adjoint, t, out, batch)
+            return Field(out, grid)
+        return self._mft_operation(t, adjoint, cdt)
+
+    def _mft_operation(self, t, adjoint, cdt):
+        if self._mft is None:
+            self._mft = MatrixFourierTransform(self.input_grid, self.internal_grid)
+        F = self._mft.forward(Field(t, self.input_grid))
+        tf = self.evaluate_transfer_function(t.device).to(cdt)
+        if adjoint:
+            tf = tf.conj()
+        F = Field(F.as_subclass(torch.Tensor) * tf, self.internal_grid)
+        return self._mft.backward(F)
+
+    def forward(self, field):
+        return self._operation(field, adjoint=False)
+
+    def backward(self, field):
+        return self._operation(field, adjoint=True)

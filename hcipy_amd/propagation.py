@@ -61,3 +61,96 @@ class FraunhoferPropagator(AgnosticOpticalElement):
         d = self.instance(wavefront.wavelength)
         U = d.fourier_transform._apply(wavefront.electric_field, _lib.BACKWARD, extra_alpha=1 / d.norm_factor)
         return Wavefront(Field(U, self._input_grid), wavefront.wavelength)
+
+    def forward_intensity(self, wavefront, float64=False):
+        """``forward(wavefront).intensity`` in one pass: |E|^2 is evaluated in the MFT epilogue
+        from the FP64 value, so complex64 inputs are not limited by the complex64 rounding of the
+        focal field."""
+        d = self.instance(wavefront.wavelength)
+        out = _lib.OUTPUT_INTENSITY_F64 if float64 else _lib.OUTPUT_INTENSITY
+        I = d.fourier_transform._apply(wavefront.electric_field, _lib.FORWARD, extra_alpha=d.norm_factor, output=out)
+        return Field(I, self._output_grid)
+
+
+def _nearfield_branch(input_grid, wavelength, distance):
+    """True when the native transfer function applies (angular_spectrum.py:53-55, fresnel.py:48-50)."""
+    L_max = max(n * d for n, d in zip(input_grid.dims, input_grid.delta))
+    return not any(d < wavelength * abs(distance) / L_max for d in input_grid.delta)
+
+
+class _NearFieldPropagator(AgnosticOpticalElement):
+    KIND = None
+
+    def __init__(self, input_grid, distance, num_oversampling=2, refractive_index=1, zero_padding=2):
+        self._distance = distance
+        self._num_oversampling = num_oversampling
+        self._refractive_index = refractive_index
+        self._zero_padding = zero_padding
+        AgnosticOpticalElement.__init__(self, grid_dependent=True, wavelength_dependent=True)
+
+    def make_instance(self, instance_data, input_grid, output_grid, wavelength):
+        from .fourier_filter import FourierFilter, NearFieldTransferFunction
+        if not input_grid.is_regular or not input_grid.is_('cartesian'):
+            raise ValueError('The input grid must be a regular, Cartesian grid.')
+        n = self.evaluate_parameter(self.refractive_index, input_grid, output_grid, wavelength)
+        k = 2 * np.pi / wavelength * n
+        if not _nearfield_branch(input_grid, wavelength, self.distance):
+            raise NotImplementedError('the impulse-response transfer function branch (grid spacing below '
+                                      'lambda |z| / L) is not implemented on the device')
+        tf = NearFieldTransferFunction(self.KIND, k, self.distance, self.num_oversampling)
+        instance_data.fourier_filter = FourierFilter(input_grid, tf, q=self._q())
+
+    def _q(self):
+        return self._zero_padding
+
+    @property
+    def distance(self):
+        return self._distance
+
+    @distance.setter
+    def distance(self, distance):
+        self._distance = distance
+        self.clear_cache()
+
+    @property
+    def num_oversampling(self):
+        return self._num_oversampling
+
+    @property
+    def refractive_index(self):
+        return self._refractive_index
+
+    def get_input_grid(self, output_grid, wavelength):
+        return output_grid
+
+    def get_output_grid(self, input_grid, wavelength):
+        return input_grid
+
+    def forward(self, wavefront):
+        d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
+        return Wavefront(d.fourier_filter.forward(wavefront.electric_field), wavefront.wavelength)
+
+    def backward(self, wavefront):
+        d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
+        return Wavefront(d.fourier_filter.backward(wavefront.electric_field), wavefront.wavelength)
+
+
+class AngularSpectrumPropagator(_NearFieldPropagator):
+    """Angular-spectrum propagation, transfer function exp(i sqrt(k^2 - |k|^2 + 0j) z)
+    supersampled x num_oversampling, FourierFilter with q = 2 (angular_spectrum.py:7-152)."""
+    KIND = 'angular_spectrum'
+
+    def __init__(self, input_grid, distance, num_oversampling=2, refractive_index=1):
+        _NearFieldPropagator.__init__(self, input_grid, distance, num_oversampling, refractive_index, 2)
+
+
+class FresnelPropagator(_NearFieldPropagator):
+    """Paraxial Fresnel propagation, transfer function exp(ikz) exp(-i z |k|^2 / 2k)
+    (fresnel.py:8-164), FourierFilter with q = zero_padding."""
+    KIND = 'fresnel'
+
+    def __init__(self, input_grid, distance, num_oversampling=2, refractive_index=1, zero_padding=2):
+        _NearFieldPropagator.__init__(self, input_grid, distance, num_oversampling, refractive_index, zero_padding)
+
+
+__all__ += ['AngularSpectrumPropagator', 'FresnelPropagator']

@@ -79,14 +79,29 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
 int hcipy_mft_execute(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch,
                       double alpha_re, double alpha_im, void* workspace, size_t workspace_bytes, void* stream);
 
-/* Runs hcipy_mft_execute `repeats` times on `stream` with HIP events recorded around each of its
+/* Output kinds of hcipy_mft_execute_ex. */
+#define HCIPY_OUTPUT_FIELD 0         /* complex field in the plan's dtype (complex128 / complex64) */
+#define HCIPY_OUTPUT_FIELD_C128 1    /* complex128 field from any precision                       */
+#define HCIPY_OUTPUT_INTENSITY 2     /* |out|^2 from the FP64 epilogue value: float64 for FP64
+                                        plans, float32 otherwise                                  */
+#define HCIPY_OUTPUT_INTENSITY_F64 3 /* |out|^2 as float64                                        */
+
+/* hcipy_mft_execute with a choice of output: the field, or the intensity |out|^2 evaluated in the
+ * epilogue before any rounding of the field to the output dtype.  The intensity output replaces
+ * FraunhoferPropagator.forward followed by Wavefront.intensity (hcipy/propagation/fraunhofer.py:56-71,
+ * hcipy/optics/wavefront.py:113-119) in one pass. */
+int hcipy_mft_execute_ex(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch,
+                         double alpha_re, double alpha_im, int output, void* workspace, size_t workspace_bytes,
+                         void* stream);
+
+/* Runs hcipy_mft_execute_ex `repeats` times on `stream` with HIP events recorded around each of its
  * kernels, waits, and stores the mean duration in milliseconds of: kernel_ms[0] the data split
  * (split-FP16 only, else ~0), kernel_ms[1] stage 1, kernel_ms[2] stage 2 (all batch slices).
  * Measurement aid with no counterpart in the reference (whose MFT is timed by _time_it,
  * hcipy/fourier/fourier_transform.py:185-215); not for production loops. */
 int hcipy_mft_profile(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch, double alpha_re,
-                      double alpha_im, void* workspace, size_t workspace_bytes, int repeats, float* kernel_ms,
-                      void* stream);
+                      double alpha_im, int output, void* workspace, size_t workspace_bytes, int repeats,
+                      float* kernel_ms, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Pointwise kernels of the propagation chain.  `batch` fields of `n` samples each lie
@@ -126,6 +141,43 @@ int hcipy_pw_power_workspace(int64_t batch, size_t* bytes);
 int hcipy_pw_power(const void* x, int field_dtype, const void* mask, int mask_dtype, int64_t mask_stride,
                    double weight, int64_t n, int64_t batch, double* out, void* workspace, size_t workspace_bytes,
                    void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Fourier filter: zero-pad -> 2-D FFT -> transfer function -> inverse 2-D FFT -> crop.
+ * Replaces FourierFilter (hcipy/fourier/fourier_operations.py:8-151) for scalar transfer
+ * functions and power-of-two internal grids, the engine of AngularSpectrumPropagator
+ * (hcipy/propagation/angular_spectrum.py:45-152) and FresnelPropagator
+ * (hcipy/propagation/fresnel.py:43-164).  Field: ny x nx (x fastest) centred in the NY x NX
+ * internal grid at int(N/2) - int(n/2) (fast_fourier_transform.py:262-266). */
+
+typedef struct hcipy_filter_plan_s* hcipy_filter_plan;
+
+/* nx, ny: field size; NX, NY: internal (padded) size, powers of two in [4, 4096].
+ * dtype: HCIPY_DTYPE_C64 or HCIPY_DTYPE_C128 (fields and transfer function). */
+int hcipy_filter_plan_create(hcipy_filter_plan* plan, int nx, int ny, int NX, int NY, int dtype, void* stream);
+
+int hcipy_filter_plan_destroy(hcipy_filter_plan plan);
+
+/* Sets the transfer function from an NY x NX device array sampled on the centred internal grid
+ * (the reference ifftshifts it, fourier_operations.py:41-52; so does this call). */
+int hcipy_filter_set_transfer_function(hcipy_filter_plan plan, const void* tf_centred, int tf_dtype, void* stream);
+
+/* Generates a near-field transfer function on the device in FP64, averaged over
+ * oversampling^2 dithered sub-grids (hcipy/field/util.py:606-698).  The internal grid has
+ * separated coordinates i * delta + zero.  kind 0: angular spectrum
+ * exp(i sqrt(k^2 - |k|^2 + 0j) z) (angular_spectrum.py:71-78); kind 1: Fresnel
+ * exp(i k z) exp(-i z |k|^2 / (2k)) (fresnel.py:69-80). */
+int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, double delta_x, double zero_x,
+                                             double delta_y, double zero_y, double wavenumber, double distance,
+                                             int oversampling, void* stream);
+
+/* Bytes of workspace hcipy_filter_execute needs for `batch` fields. */
+int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* bytes);
+
+/* out[b] = crop(ifft2(fft2(pad(in[b])) * tf)) (conj(tf) when adjoint != 0); batch <= 65535.
+ * in and out are batch x ny x nx complex arrays of the plan's dtype and must not overlap. */
+int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, void* out, int64_t batch,
+                         void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -75,26 +75,40 @@ def test_fraunhofer_256_to_128_complex128():
     assert peak_rel_intensity(a, b) < 1e-12
 
 
-@pytest.mark.parametrize('precision,tol_i,tol_f', [('f16x2', 1e-7, 2e-7), ('fp32', 3e-6, 3e-6)])
-def test_fraunhofer_1024_to_512_complex64(precision, tol_i, tol_f):
-    """1024^2 -> 512^2 complex64 against the FP64 result on the same (complex64) input."""
+@pytest.mark.parametrize('precision', ['f16x2', 'fp32'])
+def test_fraunhofer_1024_to_512_complex64(precision):
+    """1024^2 -> 512^2 complex64 against the FP64 result on the same (complex64) input.
+
+    Bars (peak-relative): f16x2 field 1e-7, fused intensity 1e-7, intensity of the complex64
+    field 2.5e-7 (rounding re/im to FP32 alone moves |E|^2 by up to 2^-23 = 1.2e-7 at the peak);
+    fp32 field and intensities 3e-6.
+    """
     pupil, focal, E = workloads.plain_mft_1024()
     E64 = E.astype(np.complex64)
     prop = ha.FraunhoferPropagator(pupil, focal, precision=precision)
     say('fraunhofer 1024->512 c64', precision)
-    a = host(prop(ha.Wavefront(ha.Field(dev(E64), pupil), 1.0)).electric_field)
+    wf = ha.Wavefront(ha.Field(dev(E64), pupil), 1.0)
+    a = host(prop(wf).electric_field)
     assert a.dtype == np.complex64
+    I = host(prop.forward_intensity(wf))
+    assert I.dtype == np.float32
     b = o.fraunhofer_forward(E64.astype(np.complex128), og(pupil), og(focal))
-    ei = peak_rel_intensity(a, b)
+    Ib = np.abs(b) ** 2
     ef = peak_rel_field(a, b)
-    say('  intensity peak-rel %.3e  field peak-rel %.3e' % (ei, ef))
-    assert ei < tol_i
-    assert ef < tol_f
+    ei_field = peak_rel_intensity(a, b)
+    ei = np.abs(I.astype(np.float64) - Ib).max() / Ib.max()
+    say('  field %.3e  intensity(fused) %.3e  intensity(from c64 field) %.3e' % (ef, ei, ei_field))
+    if precision == 'f16x2':
+        assert ef < 1e-7 and ei < 1e-7 and ei_field < 2.5e-7
+    else:
+        assert ef < 3e-6 and ei < 3e-6 and ei_field < 3e-6
 
 
-@pytest.mark.parametrize('precision,tol', [('f16x2', 2e-7), ('fp32', 3e-6)])
+@pytest.mark.parametrize('precision,tol', [('f16x2', 5e-7), ('fp32', 3e-6)])
 def test_backward_256_to_1024_complex64(precision, tol):
-    """Lyot return leg: 256^2 focal (q=32, 4 lambda/D) -> 1024^2 pupil, adjoint MFT."""
+    """Lyot return leg: 256^2 focal (q=32, 4 lambda/D) -> 1024^2 pupil, adjoint MFT, on random
+    complex data.  Random data has no coherent peak, so the error is set by the 22-bit hi/lo
+    representation (2^-22 = 2.4e-7 per element, x sqrt(2) for re/im); bar 5e-7."""
     pupil = ha.make_pupil_grid(1024)
     fpm_grid = ha.make_focal_grid(32, 4)
     F = _rand(fpm_grid.size, 7).astype(np.complex64)
