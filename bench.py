@@ -30,6 +30,7 @@ def parse():
     ap.add_argument('--chunk', type=int, default=50, help='propagations per captured HIP graph')
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-autotune', action='store_true', help='keep the default tile shapes')
     ap.add_argument('--cpu-seconds', type=float, default=2.0, help='t_max of the CPU timing loop')
     return ap.parse_args()
 
@@ -57,6 +58,8 @@ def main():
     x = torch.as_tensor(E64, device=dev)
     wavelength = 1.0
 
+    tiles = {}
+
     def setup(precision):
         prop = ha.FraunhoferPropagator(pupil, focal, precision=precision)
         d = prop.instance(wavelength)
@@ -64,6 +67,8 @@ def main():
         alpha = complex(pupil.weights) * d.norm_factor
         cdt = torch.complex64 if precision != 'fp64' else torch.complex128
         plan = mft.plan_for(cdt, dev)
+        if not args.no_autotune:
+            tiles[precision] = mft.autotune(ha.Field(x, pupil), repeats=10)['forward']
         ws = mft._workspace_for(plan, 1, dev)
         return prop, plan, ws, alpha
 
@@ -148,7 +153,9 @@ def main():
     ms1 = kms_main[1]
     achieved = flops_stage1 / (ms1 * 1e-3) / 1e12
     peak_tf = PEAK_TFLOPS[args.precision]
-    roofline = {'bound': 'mfma', 'kernel': 'mft_stage_kernel (stage 1, E.M2)', 'achieved': round(achieved, 2),
+    t0_cfg = tiles.get(args.precision, (-1, -1))[0]
+    kname = 'mft_f16x2_kernel' if t0_cfg >= _lib.TILES_C64X64_W4 else 'mft_stage_kernel'
+    roofline = {'bound': 'mfma', 'kernel': '%s (stage 1, E.M2)' % kname, 'achieved': round(achieved, 2),
                 'peak': peak_tf, 'unit': 'TFLOP/s', 'frac': round(achieved / peak_tf, 4), 'traffic': None,
                 'kernel_ms': {'split': round(kms_main[0], 5), 'stage1': round(kms_main[1], 5),
                               'stage2': round(kms_main[2], 5)},
@@ -186,7 +193,8 @@ def main():
         'config': {'workload': 'fraunhofer_mft_1024_to_512_complex64_focal_intensity', 'pupil': [1024, 1024],
                    'focal': [512, 512], 'q': 4, 'num_airy': 64, 'batch': 1, 'precision': args.precision,
                    'output': 'focal intensity (float32) from the FP64 epilogue value',
-                   'graph_chunk': None if args.no_graph else args.chunk},
+                   'graph_chunk': None if args.no_graph else args.chunk,
+                   'tile_shapes': {k: list(v) for k, v in tiles.items()} or 'default'},
         'peak_rel_intensity_error': err_main,
         'speedup_vs_fp32': round(value / fp32_ref['value'], 3) if fp32_ref else None,
         'variants': variants,

@@ -21,6 +21,20 @@ OUTPUT_FIELD = 0
 OUTPUT_FIELD_C128 = 1
 OUTPUT_INTENSITY = 2
 OUTPUT_INTENSITY_F64 = 3
+TILES_AUTO = -1
+TILES_64X64_W2X2 = 0
+TILES_64X32_W2X1 = 1
+TILES_64X64_K4 = 2
+TILES_64X32_K4 = 3
+TILES_128X64 = 4
+TILES_64X128 = 5
+TILES_64X64_W8 = 6
+TILES_64X32_W8 = 7
+TILES_C64X64_W4 = 8
+TILES_C64X64_W8 = 9
+TILES_C64X32_W4 = 10
+TILES_C128X32_W8 = 11
+TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(12))}
 
 _c_int64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -35,6 +49,10 @@ SIGNATURES = {
     'hcipy_mft_plan_destroy': (ctypes.c_int, [_vp]),
     'hcipy_mft_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_mft_plan_bytes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_mft_plan_set_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    'hcipy_mft_plan_get_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    'hcipy_mft_autotune': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_int, _vp, ctypes.c_size_t,
+                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp]),
     'hcipy_mft_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
                                          _vp, ctypes.c_size_t, _vp]),
     'hcipy_mft_execute_ex': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,

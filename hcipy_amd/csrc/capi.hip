@@ -37,12 +37,9 @@ namespace {
 constexpr int64_t kAlign = 256;
 // Padded operand rows (constant matrices, split data, intermediate) hold a multiple of this many
 // elements, the largest k-tile of the stage kernels, so full-vector loads of the last k-tile
-// stay inside the row...
+// stay inside the row.
 constexpr int64_t kRowPad = 128;
-// ...plus this many more, so that row starts are not all aligned to the same offset modulo the
-// L2 channel interleave (rows of 2^n bytes would send every row of a k-tile to one channel).
-constexpr int64_t kRowSkew = 64;
-int64_t padded_row(int64_t k) { return round_up(k, kRowPad) + kRowSkew; }
+int64_t padded_row(int64_t k) { return round_up(k, kRowPad); }
 constexpr int kMaxGridZ = 65535;
 
 size_t elem_size(int prec) { return prec == kF64 ? 8 : (prec == kF32 ? 4 : 2); }
@@ -69,6 +66,7 @@ struct Plan {
   FMatrix f1, f2, b1, b2;  // forward stage 1/2, backward stage 1/2
   double* coords = nullptr;
   size_t matrix_bytes = 0;
+  int tiles[2][2] = {{kTileAuto, kTileAuto}, {kTileAuto, kTileAuto}};  // [direction][stage]
 };
 
 int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, int nk, double sign,
@@ -144,7 +142,7 @@ Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
     w.t = take((size_t)batch * 4);
   }
   if (pl.ndim == 2) {
-    w.n1_pad = padded_row(2 * d.n1) / 2;  // complex elements per intermediate row
+    w.n1_pad = padded_row(2 * d.n1) / 2;  // complex elements per intermediate row (k-tile multiple)
     const size_t zelems = (size_t)batch * d.c1 * 2 * w.n1_pad;  // reals
     w.z_hi = take(zelems * elem_size(pl.prec));
     if (pl.prec == kF16x2) w.z_lo = take(zelems * 2);
@@ -191,6 +189,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
 
   // Stage-1 data operand.
   StageArgs a1{};
+  a1.tile_cfg = pl.tiles[dir][0];
   int b_mode = kModePadded;
   float* row_scale = nullptr;
   float* t = nullptr;
@@ -250,6 +249,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
 
   // Stage 2: one grid.z slice per field.
   StageArgs a2{};
+  a2.tile_cfg = pl.tiles[dir][1];
   a2.a = d.m2->hi;
   a2.a_lo = d.m2->lo;
   a2.lda = d.m2->lda;
@@ -384,6 +384,24 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes) {
   return kOk;
 }
 
+int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction_, int stage, int tile_cfg) {
+  HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
+  HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "unknown direction %d", direction_);
+  HCIPY_REQUIRE(stage == 0 || stage == 1, "stage must be 0 or 1, got %d", stage);
+  HCIPY_REQUIRE(tile_cfg == HCIPY_TILES_AUTO || tile_cfg_supported(plan->p.prec, tile_cfg),
+                "tile configuration %d is not available for precision %d", tile_cfg, plan->p.prec);
+  plan->p.tiles[direction_][stage] = tile_cfg;
+  return kOk;
+}
+
+int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction_, int stage, int* tile_cfg) {
+  HCIPY_REQUIRE(plan != nullptr && tile_cfg != nullptr, "NULL argument");
+  HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "unknown direction %d", direction_);
+  HCIPY_REQUIRE(stage == 0 || stage == 1, "stage must be 0 or 1, got %d", stage);
+  *tile_cfg = plan->p.tiles[direction_][stage];
+  return kOk;
+}
+
 int hcipy_mft_execute(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch,
                       double alpha_re, double alpha_im, void* workspace, size_t workspace_bytes, void* stream) {
   return hcipy_mft_execute_ex(plan, direction_, in, out, batch, alpha_re, alpha_im, HCIPY_OUTPUT_FIELD, workspace,
@@ -443,6 +461,37 @@ int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void*
   for (int i = 0; i < created; ++i) (void)hipEventDestroy(ev[i]);
   delete[] ev;
   return st;
+}
+
+int hcipy_mft_autotune(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch, int output,
+                       void* workspace, size_t workspace_bytes, int repeats, int* chosen, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr && repeats > 0, "bad argument");
+  HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
+  Plan& pl = plan->p;
+  const int nstage = pl.ndim == 2 ? 2 : 1;
+  for (int stage = 0; stage < nstage; ++stage) {
+    int best = kTileAuto;
+    float best_ms = 0.0f;
+    for (int cfg = 0; cfg < kNumTileCfg; ++cfg) {
+      if (!tile_cfg_supported(pl.prec, cfg)) continue;
+      pl.tiles[direction_][stage] = cfg;
+      float ms[3] = {0.0f, 0.0f, 0.0f};
+      const int st = hcipy_mft_profile(plan, direction_, in, out, batch, 1.0, 0.0, output, workspace, workspace_bytes,
+                                       repeats, ms, stream);
+      if (st != kOk) {
+        pl.tiles[direction_][stage] = kTileAuto;
+        return st;
+      }
+      if (best == kTileAuto || ms[1 + stage] < best_ms) {
+        best = cfg;
+        best_ms = ms[1 + stage];
+      }
+    }
+    pl.tiles[direction_][stage] = best;
+    if (chosen != nullptr) chosen[stage] = best;
+  }
+  if (chosen != nullptr && nstage == 1) chosen[1] = kTileAuto;
+  return kOk;
 }
 
 }  // extern "C"

@@ -48,6 +48,14 @@ int check_hip(hipError_t e, const char* what);
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 inline int64_t ceil_div(int64_t v, int64_t m) { return (v + m - 1) / m; }
 
+// float -> half with the native conversion (round to nearest even).  The empty asm makes the
+// float opaque, so that a preceding double -> float rounding is not merged into one double ->
+// half conversion, which the compiler would emulate in ~25 integer instructions.
+__device__ inline _Float16 f32_to_f16(float f) {
+  __asm__ volatile("" : "+v"(f));
+  return (_Float16)f;
+}
+
 // Wave-level reductions over the 64 lanes of a CDNA wavefront.
 __device__ inline float wave_max(float v) {
 #pragma unroll

@@ -150,13 +150,13 @@ __device__ inline u32x4_t expand_odd(u32x4_t v) {
 // alternates between two sets per k-tile, and the set of tile t-1 is flushed into FP64 while
 // the MFMAs of tile t run.
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
-__global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
+__global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_kernel(StageArgs p) {
   using TR = PrecTraits<P>;
   using T = typename TR::T;
   constexpr int MT = TR::MT;
   constexpr int KVEC = TR::KVEC;
   constexpr int KPL = TR::KPL;
-  constexpr int NTHR = 256;
+  constexpr int NTHR = 64 * WM * WN * WK;
   constexpr int VEC = 16 / (int)sizeof(T);
   constexpr int LDK = BK + VEC;  // 16-byte row padding against LDS bank conflicts
   constexpr int NARR = TR::SPLIT ? 2 : 1;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
   static_assert(PF == 2 || PF == 4, "k-tiles in flight: 2 or 4 (even, so accumulator sets alternate with t)");
   constexpr int ABUF = NARR * BM * LDK;
   constexpr int BUF = ABUF + NARR * BN * LDK;  // elements per LDS buffer
-  static_assert(WM * WN * WK == 4, "4 waves per workgroup");
+  static_assert(NTHR == 256 || NTHR == 512, "4 or 8 waves per workgroup");
   static_assert(TM >= 1 && TN >= 1, "wave tile smaller than MFMA tile");
   static_assert(KV_PER_TILE % WK == 0, "k-vectors must split evenly over WK");
   static_assert(ACH % NTHR == 0 && BCH % NTHR == 0, "tiles must split evenly over the threads");
@@ -206,25 +206,8 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
   if constexpr (NARR == 2) Bg[1] += (int64_t)z * p.b_zstride;
   Bg[0] += (int64_t)z * p.b_zstride;
 
-  // Split output: per folded item scale t = min(row_scale over the item) * 2^-ceil(log2 K).
   const bool split_out = (p.out_mode == kOutSplit);
   int item0 = 0;
-  if (split_out) {
-    const int nend = min(n0 + BN, N);
-    item0 = n0 / p.fold;
-    const int nitems = (nend - 1) / p.fold - item0 + 1;
-    for (int it = wave; it < nitems; it += 4) {
-      const int r0 = (item0 + it) * p.fold;
-      float mn = FLT_MAX;
-      for (int r = lane; r < p.fold; r += 64) mn = fminf(mn, p.row_scale[r0 + r]);
-      mn = wave_min(mn);
-      if (lane == 0) {
-        const float t = ldexpf(mn, -p.t_log2k);
-        s_t[it] = t;
-        if (bx == 0 && p.t_out != nullptr) p.t_out[item0 + it] = t;
-      }
-    }
-  }
 
   // Per-thread chunk geometry (fixed over the k loop).
   int a_off[AL];
@@ -244,9 +227,6 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
     b_off[i] = (int64_t)min(b_row[i], N - 1) * p.ldb;
   }
   const int nk_full = K / BK;  // k-tiles with no element beyond K
-  // Workgroups start their k loop at different k-tiles (when there is no partial k-tile) so that
-  // concurrent workgroups read different row offsets and spread over the L2 channels.
-  const int rot = (K % BK == 0) ? (int)((bx * 7 + by * 3) % nk) : 0;
   // Largest k offset a full-vector load may start at: padded rows allow every chunk of the last
   // k-tile; exact rows (K a multiple of VEC) are clamped to the last chunk.
   const int kmax_vec = (BMODE == kRowsPadded) ? (nk * BK - VEC) : (K - VEC);
@@ -256,9 +236,7 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
 
   auto load_set = [&](auto sc, int tile) __attribute__((always_inline)) {
     constexpr int S = decltype(sc)::value;
-    int pt = min(tile, nk - 1) + rot;
-    pt -= (pt >= nk) ? nk : 0;
-    const int k0 = pt * BK;
+    const int k0 = min(tile, nk - 1) * BK;
 #pragma unroll
     for (int arr = 0; arr < NARR; ++arr) {
 #pragma unroll
@@ -311,11 +289,20 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
 
   using AccT = typename std::conditional<P == kF64, doublex4_t, floatx16_t>::type;
   constexpr int NACC = TR::NACC;
-  constexpr int NSET = TR::SPLIT ? 2 : 1;
+  // The hi.hi FP32 accumulators are flushed into FP64 after every kFlushK real k values a wave
+  // has accumulated (FE k-tiles).  With FE = 1 and a single MFMA tile per wave, two sets
+  // alternate over k-tiles so that the flush of one overlaps the MFMAs of the other.
+  constexpr int KPW = KV_PER_WAVE * KVEC;  // real k per wave per k-tile
+  constexpr int FE = (KPW >= kFlushK) ? 1 : kFlushK / KPW;
+  static_assert(FE * KPW == kFlushK || FE == 1, "flush interval must be a whole number of k-tiles");
+  constexpr int NSET = (TR::SPLIT && FE == 1 && TM * TN < 4) ? 2 : 1;
+  // Cross products go to two accumulators for a single tile (independent MFMAs back to back),
+  // to one when the wave has several tiles.
+  constexpr int NXA = (TR::SPLIT && TM * TN < 2) ? 2 : 1;
   AccT acc[NSET][TM][TN];
   // cross products Ah.Bl and Al.Bh in separate accumulators: the three MFMAs of a k-vector
   // are independent and issue back to back
-  AccT accx[TR::SPLIT ? 2 : 1][TR::SPLIT ? TM : 1][TR::SPLIT ? TN : 1];
+  AccT accx[NXA][TR::SPLIT ? TM : 1][TR::SPLIT ? TN : 1];
   double acc64[TR::SPLIT ? TM : 1][TR::SPLIT ? TN : 1][TR::SPLIT ? NACC : 1];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -326,8 +313,8 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
 #pragma unroll
         for (int q = 0; q < NSET; ++q) acc[q][i][j][r] = 0;
         if constexpr (TR::SPLIT) {
-          accx[0][i][j][r] = 0;
-          accx[1][i][j][r] = 0;
+#pragma unroll
+          for (int q = 0; q < NXA; ++q) accx[q][i][j][r] = 0;
           acc64[i][j][r] = 0.0;
         }
       }
@@ -367,7 +354,8 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
           for (int j = 0; j < TN; ++j) {
             acc[Q][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cb][0][i], fb[cb][0][j], acc[Q][i][j], 0, 0, 0);
             accx[0][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cb][0][i], fb[cb][1][j], accx[0][i][j], 0, 0, 0);
-            accx[1][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cb][1][i], fb[cb][0][j], accx[1][i][j], 0, 0, 0);
+            accx[NXA - 1][i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cb][1][i], fb[cb][0][j], accx[NXA - 1][i][j], 0, 0, 0);
           }
       }
     } else if constexpr (P == kF32) {
@@ -437,6 +425,10 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
 
   // One k-tile t = kb + D; D is a compile-time position inside the PF-unrolled loop.  The main
   // loop (guarded = false) only stores full k-tiles; the tail checks bounds and masks.
+  // Order: tile t + 1 goes from registers to the other LDS buffer (no wave reads it during this
+  // step), the freed registers take the loads of tile t + 1 + PF, and the MFMAs of tile t then
+  // run while those loads pass through the address units.  One barrier ends the step: it
+  // publishes tile t + 1 and frees buffer t for the store of the next step.
   auto step = [&](int kb, auto dc, auto gc) __attribute__((always_inline)) {
     constexpr int D = decltype(dc)::value;
     constexpr bool GUARDED = decltype(gc)::value;
@@ -445,11 +437,13 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
     constexpr int QP = (D + NSET - 1) % NSET;  // set of tile t - 1
     const int t = kb + D;
     if (GUARDED && t >= nk) return;
-    compute_tile(std::integral_constant<int, Q>{}, smem + (t & 1) * BUF);
-    if constexpr (NSET == 2) flush(std::integral_constant<int, QP>{});
     if (!GUARDED || t + 1 < nk)
       store_set(std::integral_constant<int, NXT>{}, std::integral_constant<bool, GUARDED>{}, t + 1, (t + 1) & 1);
     load_set(std::integral_constant<int, NXT>{}, t + 1 + PF);
+    compute_tile(std::integral_constant<int, Q>{}, smem + (t & 1) * BUF);
+    if constexpr (NSET == 2) flush(std::integral_constant<int, QP>{});
+    else if constexpr (FE == 1) flush(std::integral_constant<int, 0>{});
+    else if ((t + 1) % FE == 0) flush(std::integral_constant<int, 0>{});
     __syncthreads();
   };
   using kFalse = std::integral_constant<bool, false>;
@@ -464,6 +458,36 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
     load_set(std::integral_constant<int, 3>{}, 3);
   }
   load_set(std::integral_constant<int, 0>{}, PF);
+
+  // Split output: per folded item scale t = min(row_scale over the item) * 2^-ceil(log2 K).
+  // All threads load the item's row scales (independent loads, issued after the operand
+  // prologue so that their latency overlaps it).
+  if (split_out) {
+    __shared__ float s_red[NTHR / 64];
+    const int nend = min(n0 + BN, N);
+    item0 = n0 / p.fold;
+    const int nitems = (nend - 1) / p.fold - item0 + 1;
+    for (int it = 0; it < nitems; ++it) {
+      const float* rs = p.row_scale + (int64_t)(item0 + it) * p.fold;
+      float mn = FLT_MAX;
+      int r = tid;
+      for (; r + 3 * NTHR < p.fold; r += 4 * NTHR)
+        mn = fminf(mn, fminf(fminf(rs[r], rs[r + NTHR]), fminf(rs[r + 2 * NTHR], rs[r + 3 * NTHR])));
+      for (; r < p.fold; r += NTHR) mn = fminf(mn, rs[r]);
+      mn = wave_min(mn);
+      if (lane == 0) s_red[wave] = mn;
+      __syncthreads();
+      if (tid == 0) {
+        float m = s_red[0];
+#pragma unroll
+        for (int w = 1; w < NTHR / 64; ++w) m = fminf(m, s_red[w]);
+        const float t = ldexpf(m, -p.t_log2k);
+        s_t[it] = t;
+        if (bx == 0 && p.t_out != nullptr) p.t_out[item0 + it] = t;
+      }
+      __syncthreads();
+    }
+  }
   __syncthreads();
 
   // Main loop: every tile it stores (up to kb + PF) is a full k-tile.
@@ -485,21 +509,30 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
     }
   }
   if constexpr (TR::SPLIT) {
-    flush(std::integral_constant<int, 0>{});
-    flush(std::integral_constant<int, 1>{});
+    if constexpr (NSET == 2) {
+      flush(std::integral_constant<int, 0>{});
+      flush(std::integral_constant<int, 1>{});
+    } else if constexpr (FE > 1) {
+      flush(std::integral_constant<int, 0>{});  // k-tiles after the last full flush interval
+    }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j)
 #pragma unroll
-        for (int r = 0; r < NACC; ++r)
-          acc64[i][j][r] += ((double)accx[0][i][j][r] + (double)accx[1][i][j][r]) * kLoScale;
+        for (int r = 0; r < NACC; ++r) {
+          double x = (double)accx[0][i][j][r];
+          if constexpr (NXA == 2) x += (double)accx[1][i][j][r];
+          acc64[i][j][r] += x * kLoScale;
+        }
   }
 
   // Reduce the WK partial sums through LDS (the operand buffers are free now).
   if constexpr (WK > 1) {
     using RT = typename std::conditional<TR::SPLIT || P == kF64, double, float>::type;
     constexpr int NR = TM * TN * NACC;
+    static_assert((size_t)(WK - 1) * WM * WN * 64 * NR * sizeof(RT) <= 2 * (size_t)BUF * sizeof(T),
+                  "k-split partial sums must fit in the operand buffers");
     RT* red = reinterpret_cast<RT*>(smem_raw);
     const int slot = (wave % (WM * WN)) * 64 + lane;
     if (wk > 0) {
@@ -586,12 +619,429 @@ __global__ __launch_bounds__(256) void mft_stage_kernel(StageArgs p) {
         } else if (p.out_mode == kOutI64) {
           reinterpret_cast<double*>(p.out)[idx] = orr * orr + oii * oii;
         } else {
+          // hi through FP32 (native conversions; a double rounding only moves hi by one ulp, and
+          // lo below takes the exact remainder either way)
           const double xr = orr * (double)t;
           const double xi = oii * (double)t;
-          const _Float16 hr = (_Float16)xr;
-          const _Float16 hi = (_Float16)xi;
+          const _Float16 hr = f32_to_f16((float)xr);
+          const _Float16 hi = f32_to_f16((float)xi);
           half2_t h = {hr, hi};
-          half2_t l = {(_Float16)((xr - (double)hr) * (double)kLoInv), (_Float16)((xi - (double)hi) * (double)kLoInv)};
+          half2_t l = {f32_to_f16((float)((xr - (double)hr) * (double)kLoInv)),
+                       f32_to_f16((float)((xi - (double)hi) * (double)kLoInv))};
+          reinterpret_cast<half2_t*>(p.out)[idx] = h;
+          reinterpret_cast<half2_t*>(p.out_lo)[idx] = l;
+        }
+      }
+    }
+  }
+}
+
+// Split-FP16 stage kernel with the constant operand kept compact.
+//
+// Same contraction as mft_stage_kernel<kF16x2, ...>, arranged so that neither LDS nor HBM ever
+// holds the expanded 2x2 real blocks of the constant: the workgroup tile is BMC compact constant
+// rows (complex) x BN data rows, and every wave reads one compact A fragment per k-vector and
+// forms in registers the operand of an "even" MFMA tile, rows (re, -im) -> Re of the result, and
+// of an "odd" tile, rows (im, re) -> Im of the result (one xor / one rotate per dword).  Both
+// tiles share the data fragment.  Per k-vector a wave reads 4 fragments (A hi/lo, B hi/lo) for
+// 6 MFMAs per 32 x 32 tile pair, and LDS holds half the constant bytes of the expanded form.
+//
+// Waves form WM x WN x WK: WM x WN split the tile, WK split the k-vectors of every k-tile.
+// Each wave flushes its FP32 hi.hi accumulators into FP64 after kFlushK real k values.
+template <int BMC, int BN, int BK, int WM, int WN, int WK, int PF>
+__global__ __launch_bounds__(64 * WM * WN * WK) void mft_f16x2_kernel(StageArgs p) {
+  using T = _Float16;
+  constexpr int NTHR = 64 * WM * WN * WK;
+  constexpr int VEC = 8;                  // fp16 elements per 16-byte chunk
+  constexpr int LDK = BK + VEC;           // LDS row stride (16-byte pad against bank conflicts)
+  constexpr int WTC = BMC / WM;           // compact constant rows per wave
+  constexpr int WTN = BN / WN;            // data rows per wave
+  constexpr int TMC = WTC / 32;
+  constexpr int TN = WTN / 32;
+  constexpr int CPR = BK / VEC;           // chunks per tile row
+  constexpr int ACH = BMC * CPR;
+  constexpr int BCH = BN * CPR;
+  constexpr int AL = ACH / NTHR;
+  constexpr int BL = BCH / NTHR;
+  constexpr int KV_PER_TILE = BK / 16;
+  constexpr int KV_PER_WAVE = KV_PER_TILE / WK;
+  constexpr int KPW = KV_PER_WAVE * 16;   // real k per wave per k-tile
+  constexpr int FE = (KPW >= kFlushK) ? 1 : kFlushK / KPW;
+  constexpr int ABUF = 2 * BMC * LDK;     // hi, lo constant rows
+  constexpr int BUF = ABUF + 2 * BN * LDK;
+  static_assert(NTHR == 256 || NTHR == 512, "4 or 8 waves per workgroup");
+  static_assert(TMC >= 1 && TN >= 1 && WTC % 32 == 0 && WTN % 32 == 0, "wave tile of whole 32 x 32 MFMA tiles");
+  static_assert(ACH % NTHR == 0 && BCH % NTHR == 0 && AL >= 1 && BL >= 1, "tiles must split evenly over the threads");
+  static_assert(KV_PER_TILE % WK == 0 && KV_PER_WAVE >= 1, "k-vectors must split evenly over WK");
+  static_assert(FE * KPW == kFlushK || FE == 1, "flush interval must be a whole number of k-tiles");
+  static_assert(PF == 2, "two k-tiles in flight");
+
+  // The two operand buffers are separate LDS objects, so the compiler knows that the stores of
+  // tile t + 1 and the fragment reads of tile t never alias and may interleave them.
+  __shared__ __attribute__((aligned(16))) T lds0[BUF];
+  __shared__ __attribute__((aligned(16))) T lds1[BUF];
+  __shared__ float s_t[BN];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wk = wave / (WM * WN);
+  const int wm = (wave % (WM * WN)) / WN;
+  const int wn = wave % WN;
+  const int Mc = p.M / 2;  // compact constant rows
+  const int gx = (Mc + BMC - 1) / BMC;
+  const int gy = (p.N + BN - 1) / BN;
+  int bx, by;
+  tile_of(blockIdx.x, gx, gy, bx, by);
+  const int c0 = bx * BMC;
+  const int n0 = by * BN;
+  const int z = blockIdx.z;
+  const int N = p.N, K = p.K;
+  const int nk = (K + BK - 1) / BK;
+  const int nk_full = K / BK;
+
+  const T* Ag[2] = {static_cast<const T*>(p.a), static_cast<const T*>(p.a_lo)};
+  const T* Bg[2] = {static_cast<const T*>(p.b) + (int64_t)z * p.b_zstride,
+                    static_cast<const T*>(p.b_lo) + (int64_t)z * p.b_zstride};
+  const bool split_out = (p.out_mode == kOutSplit);
+  int item0 = 0;
+
+  // Per-thread chunk geometry.  Constant rows are padded to a multiple of 64 rows and data
+  // rows >= N are clamped (they only feed output columns that are never written).
+  int a_lds[AL], b_lds[BL];
+  int64_t a_off[AL], b_off[BL];
+#pragma unroll
+  for (int i = 0; i < AL; ++i) {
+    const int ch = tid + i * NTHR;
+    a_lds[i] = (ch / CPR) * LDK + (ch % CPR) * VEC;
+    a_off[i] = (int64_t)(c0 + ch / CPR) * p.lda + (ch % CPR) * VEC;
+  }
+  int b_kc[BL];
+#pragma unroll
+  for (int i = 0; i < BL; ++i) {
+    const int ch = tid + i * NTHR;
+    b_lds[i] = (ch / CPR) * LDK + (ch % CPR) * VEC;
+    b_kc[i] = (ch % CPR) * VEC;
+    b_off[i] = (int64_t)min(n0 + ch / CPR, N - 1) * p.ldb + b_kc[i];
+  }
+
+  u32x4_t ra[PF][2][AL];
+  u32x4_t rb[PF][2][BL];
+  // Loads of k-tile `tile` (clamped to the last one) into register set S.  Rows are padded to a
+  // whole number of k-tiles, so full 16-byte loads of the last k-tile stay inside the row.
+  auto load_set = [&](auto sc, int tile) __attribute__((always_inline)) {
+    constexpr int S = decltype(sc)::value;
+    const int k0 = min(tile, nk - 1) * BK;
+#pragma unroll
+    for (int arr = 0; arr < 2; ++arr) {
+#pragma unroll
+      for (int i = 0; i < AL; ++i) ra[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Ag[arr] + a_off[i] + k0);
+#pragma unroll
+      for (int i = 0; i < BL; ++i) rb[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Bg[arr] + b_off[i] + k0);
+    }
+  };
+  // Register set S (k-tile `tile`) to LDS buffer `buf`; MASK zeroes data elements with k >= K.
+  auto store_set = [&](auto sc, auto mc, int tile, T* base) __attribute__((always_inline)) {
+    constexpr int S = decltype(sc)::value;
+    constexpr bool MASK = decltype(mc)::value;
+#pragma unroll
+    for (int arr = 0; arr < 2; ++arr) {
+      T* sA = base + arr * BMC * LDK;
+      T* sB = base + ABUF + arr * BN * LDK;
+#pragma unroll
+      for (int i = 0; i < AL; ++i) *reinterpret_cast<u32x4_t*>(&sA[a_lds[i]]) = ra[S][arr][i];
+#pragma unroll
+      for (int i = 0; i < BL; ++i) {
+        u32x4_t v = rb[S][arr][i];
+        if constexpr (MASK) v = mask_tail<T>(v, K - (tile * BK + b_kc[i]));
+        *reinterpret_cast<u32x4_t*>(&sB[b_lds[i]]) = v;
+      }
+    }
+  };
+
+  // Accumulators: [constant tile][0: Re (even rows), 1: Im (odd rows)][data tile].
+  floatx16_t acc[TMC][2][TN];   // hi.hi
+  floatx16_t accx[TMC][2][TN];  // hi.lo + lo.hi
+  double acc64[TMC][2][TN][16];
+#pragma unroll
+  for (int i = 0; i < TMC; ++i)
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          acc[i][e][j][r] = 0.0f;
+          accx[i][e][j][r] = 0.0f;
+          acc64[i][e][j][r] = 0.0;
+        }
+
+  // Fragments of k-vector q + 1 are read while the MFMAs of q run (two fragment buffers) with 4
+  // waves; with 8 waves the other wave of the SIMD covers the LDS latency and one buffer saves
+  // the registers.
+  constexpr int FB = (NTHR == 256) ? 2 : 1;
+  auto compute_tile = [&](const T* base) __attribute__((always_inline)) {
+    half8_t fa[FB][2][TMC], fb[FB][2][TN];  // [buffer][hi/lo][tile]
+    auto read_frags = [&](int buf, int q) __attribute__((always_inline)) {
+      const int kofs = (q * WK + wk) * 16 + (lane >> 5) * 8;
+#pragma unroll
+      for (int arr = 0; arr < 2; ++arr) {
+        const T* sA = base + arr * BMC * LDK;
+        const T* sB = base + ABUF + arr * BN * LDK;
+#pragma unroll
+        for (int i = 0; i < TMC; ++i)
+          fa[buf][arr][i] = *reinterpret_cast<const half8_t*>(&sA[(wm * WTC + i * 32 + (lane & 31)) * LDK + kofs]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          fb[buf][arr][j] = *reinterpret_cast<const half8_t*>(&sB[(wn * WTN + j * 32 + (lane & 31)) * LDK + kofs]);
+      }
+    };
+    if constexpr (FB == 2) read_frags(0, 0);
+#pragma unroll
+    for (int q = 0; q < KV_PER_WAVE; ++q) {
+      const int cb = q % FB;
+      if constexpr (FB == 2) {
+        if (q + 1 < KV_PER_WAVE) read_frags(cb ^ 1, q + 1);
+      } else {
+        read_frags(0, q);
+      }
+      half8_t ae[2][TMC], ao[2][TMC];
+#pragma unroll
+      for (int arr = 0; arr < 2; ++arr)
+#pragma unroll
+        for (int i = 0; i < TMC; ++i) {
+          const u32x4_t v = __builtin_bit_cast(u32x4_t, fa[cb][arr][i]);
+          ae[arr][i] = __builtin_bit_cast(half8_t, v ^ u32x4_t{0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u});
+          ao[arr][i] = __builtin_bit_cast(half8_t, (v >> 16) | (v << 16));
+        }
+#pragma unroll
+      for (int i = 0; i < TMC; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[i][0][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ae[0][i], fb[cb][0][j], acc[i][0][j], 0, 0, 0);
+          acc[i][1][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ao[0][i], fb[cb][0][j], acc[i][1][j], 0, 0, 0);
+        }
+#pragma unroll
+      for (int i = 0; i < TMC; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          accx[i][0][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ae[0][i], fb[cb][1][j], accx[i][0][j], 0, 0, 0);
+          accx[i][1][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ao[0][i], fb[cb][1][j], accx[i][1][j], 0, 0, 0);
+        }
+#pragma unroll
+      for (int i = 0; i < TMC; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          accx[i][0][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ae[1][i], fb[cb][0][j], accx[i][0][j], 0, 0, 0);
+          accx[i][1][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ao[1][i], fb[cb][0][j], accx[i][1][j], 0, 0, 0);
+        }
+    }
+  };
+
+  auto flush = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < TMC; ++i)
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            acc64[i][e][j][r] += (double)acc[i][e][j][r];
+            acc[i][e][j][r] = 0.0f;
+          }
+  };
+
+  // Instruction interleaving of one main-loop step: the LDS stores of tile t + 1 and the global
+  // loads of tile t + 1 + PF are spread between the MFMAs of tile t, so that the LDS, the address
+  // units and the matrix cores work at the same time instead of one after the other (all waves
+  // of a workgroup run the same phase between two barriers).
+  constexpr int NMF = 6 * TMC * TN;       // MFMAs per k-vector
+  constexpr int NDR = 2 * (TMC + TN);     // fragment reads per k-vector
+  constexpr int NST = 2 * (AL + BL);      // LDS stores (= global loads) per step
+  constexpr int ST_PER_KV = (NST + KV_PER_WAVE - 1) / KV_PER_WAVE;
+  auto interleave = [&]() __attribute__((always_inline)) {
+    // one LDS store and one global load after each of the first NST MFMAs, fragment reads of
+    // the next k-vector in between
+#pragma unroll
+    for (int q = 0; q < KV_PER_WAVE; ++q) {
+#pragma unroll
+      for (int m = 0; m < NMF; ++m) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        if (q * NMF + m < NST) {
+          __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS store
+          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load
+        }
+        if (m < NDR && q + 1 < KV_PER_WAVE) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // fragment read
+      }
+    }
+  };
+
+  // Step t: tile t + 1 to the other LDS buffer, loads of tile t + 1 + PF into the freed
+  // registers, MFMAs of tile t, flush at the end of a flush interval, one barrier.
+  auto step = [&](int kb, auto dc, auto gc) __attribute__((always_inline)) {
+    constexpr int D = decltype(dc)::value;
+    constexpr bool GUARDED = decltype(gc)::value;
+    constexpr int NXT = (D + 1) % PF;
+    const int t = kb + D;  // kb is a multiple of PF = 2, so tile t lives in buffer D & 1
+    T* cur = (D & 1) ? lds1 : lds0;
+    T* nxt = (D & 1) ? lds0 : lds1;
+    if (GUARDED && t >= nk) return;
+    if (!GUARDED || t + 1 < nk)
+      store_set(std::integral_constant<int, NXT>{}, std::integral_constant<bool, GUARDED>{}, t + 1, nxt);
+    load_set(std::integral_constant<int, NXT>{}, t + 1 + PF);
+    compute_tile(cur);
+    // kb is a multiple of PF: for FE dividing PF the flush test is a compile-time one
+    if constexpr (FE == 1) flush();
+    else if constexpr (PF % FE == 0) { if constexpr ((D + 1) % FE == 0) flush(); }
+    else if ((t + 1) % FE == 0) flush();
+    if constexpr (!GUARDED) interleave();
+    __syncthreads();
+  };
+  using kFalse = std::integral_constant<bool, false>;
+  using kTrue = std::integral_constant<bool, true>;
+
+  // Prologue: tile 0 into LDS buffer 0, tiles 1 and 2 in flight.
+  load_set(std::integral_constant<int, 0>{}, 0);
+  store_set(std::integral_constant<int, 0>{}, kTrue{}, 0, lds0);
+  load_set(std::integral_constant<int, 1>{}, 1);
+  load_set(std::integral_constant<int, 0>{}, 2);
+
+  // Split output: per folded item scale t = min(row_scale over the item) * 2^-ceil(log2 K).
+  if (split_out) {
+    __shared__ float s_red[NTHR / 64];
+    const int nend = min(n0 + BN, N);
+    item0 = n0 / p.fold;
+    const int nitems = (nend - 1) / p.fold - item0 + 1;
+    for (int it = 0; it < nitems; ++it) {
+      const float* rs = p.row_scale + (int64_t)(item0 + it) * p.fold;
+      float mn = FLT_MAX;
+      int r = tid;
+      for (; r + 3 * NTHR < p.fold; r += 4 * NTHR)
+        mn = fminf(mn, fminf(fminf(rs[r], rs[r + NTHR]), fminf(rs[r + 2 * NTHR], rs[r + 3 * NTHR])));
+      for (; r < p.fold; r += NTHR) mn = fminf(mn, rs[r]);
+      mn = wave_min(mn);
+      if (lane == 0) s_red[wave] = mn;
+      __syncthreads();
+      if (tid == 0) {
+        float m = s_red[0];
+#pragma unroll
+        for (int w = 1; w < NTHR / 64; ++w) m = fminf(m, s_red[w]);
+        const float tt = ldexpf(m, -p.t_log2k);
+        s_t[it] = tt;
+        if (bx == 0 && p.t_out != nullptr) p.t_out[item0 + it] = tt;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+
+  int kb = 0;
+  for (; kb + PF < nk_full; kb += PF) {
+    step(kb, std::integral_constant<int, 0>{}, kFalse{});
+    step(kb, std::integral_constant<int, 1>{}, kFalse{});
+  }
+  for (; kb < nk; kb += PF) {
+    step(kb, std::integral_constant<int, 0>{}, kTrue{});
+    step(kb, std::integral_constant<int, 1>{}, kTrue{});
+  }
+  if constexpr (FE > 1) flush();  // k-tiles after the last full flush interval
+#pragma unroll
+  for (int i = 0; i < TMC; ++i)
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc64[i][e][j][r] += (double)accx[i][e][j][r] * kLoScale;
+
+  // Sum the WK partial results through LDS, one k-group per round (the operand buffers are free
+  // after the last barrier of the k loop).
+  if constexpr (WK > 1) {
+    constexpr int NR = TMC * 2 * TN * 16;
+    static_assert((size_t)WM * WN * 64 * NR * sizeof(double) <= (size_t)BUF * sizeof(T),
+                  "one k-group of partial sums must fit in an operand buffer");
+    double* red = reinterpret_cast<double*>(lds0) + ((wave % (WM * WN)) * 64 + lane) * NR;
+#pragma unroll 1
+    for (int g = 1; g < WK; ++g) {
+      if (wk == g) {
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < TMC; ++i)
+#pragma unroll
+          for (int e = 0; e < 2; ++e)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+              for (int r = 0; r < 16; ++r, ++q) red[q] = acc64[i][e][j][r];
+      }
+      __syncthreads();
+      if (wk == 0) {
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < TMC; ++i)
+#pragma unroll
+          for (int e = 0; e < 2; ++e)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+              for (int r = 0; r < 16; ++r, ++q) acc64[i][e][j][r] += red[q];
+      }
+      __syncthreads();
+    }
+    if (wk > 0) return;
+  }
+
+  // Epilogue: register r of the Re / Im tiles holds compact row (r/4)*8 + (lane/32)*4 + r%4 of
+  // data column lane%32.
+  const double inv_z = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[z] : 1.0;
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int n = n0 + wn * WTN + j * 32 + (lane & 31);
+    if (n >= N) continue;
+    double sc = inv_z;
+    if (p.row_scale != nullptr) sc /= (double)p.row_scale[n];
+    int64_t base, cstride;
+    float tt = 1.0f;
+    if (p.fold > 0) {
+      const int item = n / p.fold;
+      base = (int64_t)item * p.fold_stride + (n - item * p.fold);
+      cstride = p.ldo;
+      if (split_out) tt = s_t[item - item0];
+    } else if (p.nc_layout) {
+      base = (int64_t)z * p.out_zstride + (int64_t)n * p.ldo;
+      cstride = 1;
+    } else {
+      base = (int64_t)z * p.out_zstride + n;
+      cstride = p.ldo;
+    }
+    const double ar = p.alpha_re * sc, ai = p.alpha_im * sc;
+#pragma unroll
+    for (int i = 0; i < TMC; ++i) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int c = c0 + wm * WTC + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        if (c >= Mc) continue;
+        const double zr = acc64[i][0][j][r], zi = acc64[i][1][j][r];
+        const double orr = zr * ar - zi * ai;
+        const double oii = zr * ai + zi * ar;
+        const int64_t idx = base + (int64_t)c * cstride;
+        if (p.out_mode == kOutC128) {
+          reinterpret_cast<double2*>(p.out)[idx] = make_double2(orr, oii);
+        } else if (p.out_mode == kOutC64) {
+          reinterpret_cast<float2*>(p.out)[idx] = make_float2((float)orr, (float)oii);
+        } else if (p.out_mode == kOutI32) {
+          reinterpret_cast<float*>(p.out)[idx] = (float)(orr * orr + oii * oii);
+        } else if (p.out_mode == kOutI64) {
+          reinterpret_cast<double*>(p.out)[idx] = orr * orr + oii * oii;
+        } else {
+          const double xr = orr * (double)tt;
+          const double xi = oii * (double)tt;
+          const _Float16 hr = f32_to_f16((float)xr);
+          const _Float16 hi = f32_to_f16((float)xi);
+          half2_t h = {hr, hi};
+          half2_t l = {f32_to_f16((float)((xr - (double)hr) * (double)kLoInv)),
+                       f32_to_f16((float)((xi - (double)hi) * (double)kLoInv))};
           reinterpret_cast<half2_t*>(p.out)[idx] = h;
           reinterpret_cast<half2_t*>(p.out_lo)[idx] = l;
         }
@@ -777,18 +1227,63 @@ static int launch_stage_t(const StageArgs& a, int nz, hipStream_t s) {
   }
   const int gx = (int)ceil_div(a.M, BM);
   const int gy = (int)ceil_div(a.N, BN);
-  hipLaunchKernelGGL(kern, dim3((unsigned)(gx * gy), 1, (unsigned)nz), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(gx * gy), 1, (unsigned)nz), dim3(64 * WM * WN * WK), lds, s, a);
   return kOk;
 }
 
-// Tile shape: 64 x 64 tiles (2 x 2 waves) when they give at least 256 workgroups, else 64 x 32
-// tiles with the k-vectors split over 2 waves.  BK / PF per precision keep 32-64 KiB of operands
-// in flight per workgroup.
+template <int BMC, int BN, int BK, int WM, int WN, int WK>
+static int launch_f16x2_t(const StageArgs& a, int nz, hipStream_t s) {
+  static_assert(2 * (size_t)(2 * (BMC + BN) * (BK + 8)) * sizeof(_Float16) <= 150 * 1024, "LDS budget of one CU");
+  auto kern = mft_f16x2_kernel<BMC, BN, BK, WM, WN, WK, 2>;  // operand buffers are static LDS
+  const int gx = (int)ceil_div(a.M / 2, BMC);
+  const int gy = (int)ceil_div(a.N, BN);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(gx * gy), 1, (unsigned)nz), dim3(64 * WM * WN * WK), 0, s, a);
+  return kOk;
+}
+
+bool tile_cfg_supported(int prec, int cfg) {
+  if (cfg < 0 || cfg >= kNumTileCfg) return false;
+  if (cfg >= kC64x64_w4) return prec == kF16x2;  // compact-constant shapes: split-FP16 only
+  // FP64: 64 x 64 wave tiles need more LDS for the k-split reduction than the operand buffers
+  // hold, and its 32-deep k-tiles have too few k-vectors for a 4-way split
+  return prec != kF64 || cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8;
+}
+
+// Default tile shape: 64 x 64 tiles when they give at least 256 workgroups (one per CU), else
+// 64 x 32 tiles.
+// FP32 / split-FP16: the 8-wave shapes (two waves per SIMD hide each other's LDS and load
+// latencies; measured fastest at 1024^2 -> 512^2).  FP64 keeps the 4-wave shapes.
+template <int P>
+static int default_tile_cfg(const StageArgs& a, int nz) {
+  const int64_t tiles64 = ceil_div(a.M, 64) * ceil_div(a.N, 64) * nz;
+  if constexpr (P == kF64) return tiles64 >= 256 ? k64x64_w2x2 : k64x32_w2x1;
+  return tiles64 >= 256 ? k64x64_w8 : k64x32_w8;
+}
+
 template <int P, int BK, int BMODE, int PF>
 static int launch_prec(const StageArgs& a, int nz, hipStream_t s) {
-  const int64_t tiles64 = ceil_div(a.M, 64) * ceil_div(a.N, 64) * nz;
-  if (tiles64 >= 256) return launch_stage_t<P, 64, 64, BK, 2, 2, 1, BMODE, PF>(a, nz, s);
-  return launch_stage_t<P, 64, 32, BK, 2, 1, 2, BMODE, PF>(a, nz, s);
+  const int cfg = a.tile_cfg == kTileAuto ? default_tile_cfg<P>(a, nz) : a.tile_cfg;
+  switch (cfg) {
+    case k64x64_w2x2: return launch_stage_t<P, 64, 64, BK, 2, 2, 1, BMODE, PF>(a, nz, s);
+    case k64x32_w2x1: return launch_stage_t<P, 64, 32, BK, 2, 1, 2, BMODE, PF>(a, nz, s);
+    case k64x64_w8: return launch_stage_t<P, 64, 64, BK, 2, 2, 2, BMODE, PF>(a, nz, s);
+    default: break;
+  }
+  if constexpr (P == kF16x2) {
+    if (cfg == kC64x64_w4) return launch_f16x2_t<32, 64, BK, 1, 2, 2>(a, nz, s);
+    if (cfg == kC64x64_w8) return launch_f16x2_t<32, 64, BK, 1, 2, 4>(a, nz, s);
+    if (cfg == kC64x32_w4) return launch_f16x2_t<32, 32, BK, 1, 1, 4>(a, nz, s);
+    if (cfg == kC128x32_w8) return launch_f16x2_t<64, 32, BK, 2, 1, 4>(a, nz, s);
+  }
+  if constexpr (P != kF64) {
+    if (cfg == k64x32_w8) return launch_stage_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(a, nz, s);
+    if (cfg == k64x64_k4) return launch_stage_t<P, 64, 64, BK, 1, 1, 4, BMODE, PF>(a, nz, s);
+    if (cfg == k64x32_k4) return launch_stage_t<P, 64, 32, BK, 1, 1, 4, BMODE, PF>(a, nz, s);
+    if (cfg == k128x64) return launch_stage_t<P, 128, 64, BK / 2, 2, 2, 1, BMODE, PF>(a, nz, s);
+    if (cfg == k64x128) return launch_stage_t<P, 64, 128, BK / 2, 2, 2, 1, BMODE, PF>(a, nz, s);
+  }
+  set_last_error("tile configuration %d is not available for this precision", cfg);
+  return kInvalidValue;
 }
 
 int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s) {

@@ -72,6 +72,8 @@ template <> struct PrecTraits<kF16x2> {
 // Scale of the lo part of a split value: x = hi + lo * kLoScale.
 constexpr double kLoScale = 1.0 / 2048.0;
 constexpr float kLoInv = 2048.0f;
+// Real k values accumulated in FP32 by the hi.hi MFMAs between flushes into FP64.
+constexpr int kFlushK = 128;
 
 struct StageArgs {
   // Constant operand (expanded Fourier matrix), padded so that tile loads never leave it.
@@ -102,6 +104,41 @@ struct StageArgs {
   float* t_out;                 // split output: per folded item scale, written here (or nullptr)
   int t_log2k;                  // ceil(log2(K)): bound |C| <= K / min(row_scale)
   double alpha_re, alpha_im;
+  int tile_cfg;                 // kTileAuto or one of TileCfg
 };
+
+// Workgroup tile shapes of the stage kernel (256 threads = 4 waves in every shape):
+//   k64x64_w2x2 : 64 x 64 tile, 2 x 2 waves of 32 x 32
+//   k64x32_w2x1 : 64 x 32 tile, 2 x 1 waves of 32 x 32, k-vectors split over 2 waves
+//   k64x64_k4   : 64 x 64 tile, every wave the whole tile on a quarter of the k-vectors
+//   k64x32_k4   : 64 x 32 tile, every wave the whole tile on a quarter of the k-vectors
+//   k128x64     : 128 x 64 tile, 2 x 2 waves of 64 x 32, half-depth k-tiles (fewer bytes per output)
+//   k64x128     : 64 x 128 tile, 2 x 2 waves of 32 x 64, half-depth k-tiles
+//   k64x64_w8   : 64 x 64 tile, 8 waves (2 per SIMD): 2 x 2 waves of 32 x 32, k-vectors split over 2
+//   k64x32_w8   : 64 x 32 tile, 8 waves: 2 x 1 waves of 32 x 32, k-vectors split over 4
+// Split-FP16 only, constant kept compact (32 complex rows = 64 expanded rows per tile):
+//   kC64x64_w4  : 32 x 64 compact tile, 4 waves: 1 x 2 waves, k-vectors split over 2
+//   kC64x64_w8  : 32 x 64 compact tile, 8 waves: 1 x 2 waves, k-vectors split over 4
+//   kC64x32_w4  : 32 x 32 compact tile, 4 waves, k-vectors split over 4
+//   kC128x32_w8 : 64 x 32 compact tile, 8 waves: 2 x 1 waves, k-vectors split over 4
+enum TileCfg : int {
+  kTileAuto = -1,
+  k64x64_w2x2 = 0,
+  k64x32_w2x1 = 1,
+  k64x64_k4 = 2,
+  k64x32_k4 = 3,
+  k128x64 = 4,
+  k64x128 = 5,
+  k64x64_w8 = 6,
+  k64x32_w8 = 7,
+  kC64x64_w4 = 8,
+  kC64x64_w8 = 9,
+  kC64x32_w4 = 10,
+  kC128x32_w8 = 11,
+  kNumTileCfg = 12,
+};
+
+// True if `cfg` is instantiated for precision `prec`.
+bool tile_cfg_supported(int prec, int cfg);
 
 }  // namespace hcipy

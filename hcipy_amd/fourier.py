@@ -183,6 +183,30 @@ class _MftPlan:
         _lib.check(_lib.load().hcipy_mft_workspace_size(self.handle, batch, ctypes.byref(n)), 'hcipy_mft_workspace_size')
         return n.value
 
+    def set_tiles(self, direction, stage, tile_cfg):
+        """Pins the workgroup tile shape of one stage (``_lib.TILES_*``; ``TILES_AUTO`` resets)."""
+        _lib.check(_lib.load().hcipy_mft_plan_set_tiles(self.handle, direction, stage, tile_cfg),
+                   'hcipy_mft_plan_set_tiles')
+
+    def get_tiles(self, direction, stage):
+        v = ctypes.c_int()
+        _lib.check(_lib.load().hcipy_mft_plan_get_tiles(self.handle, direction, stage, ctypes.byref(v)),
+                   'hcipy_mft_plan_get_tiles')
+        return v.value
+
+    def autotune(self, direction, x, out, batch, workspace, output=_lib.OUTPUT_FIELD, repeats=20):
+        """Times every tile shape of each stage on these buffers and pins the fastest.
+
+        Returns the chosen shape per stage.  ``out`` and ``workspace`` are overwritten.
+        """
+        chosen = (ctypes.c_int * 2)()
+        st = _lib.load().hcipy_mft_autotune(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch, output,
+                                            _lib.dptr(workspace) if workspace is not None else None,
+                                            0 if workspace is None else workspace.numel(), repeats, chosen,
+                                            _lib.stream_ptr(x.device))
+        _lib.check(st, 'hcipy_mft_autotune')
+        return chosen[0], chosen[1]
+
     def execute(self, direction, x, out, batch, alpha, workspace, output=_lib.OUTPUT_FIELD):
         st = _lib.load().hcipy_mft_execute_ex(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch,
                                               float(alpha.real), float(alpha.imag), output,
@@ -312,6 +336,21 @@ class MatrixFourierTransform(FourierTransform):
         plan.execute(direction, t, out, batch, alpha, ws, output)
         self._release()
         return Field(out, out_grid)
+
+    def autotune(self, field, repeats=20):
+        """Times every kernel tile shape of both directions on fields like ``field`` (same dtype
+        and leading shape) and pins the fastest on the plan.  Returns the chosen shapes
+        ``{'forward': (stage0, stage1), 'backward': (...)}``.  The pinned shapes live as long as
+        the plan (for the whole object with ``precompute_matrices=True``)."""
+        t, cdt = _as_field_tensor(field, self.input_grid)
+        lead = tuple(t.shape[:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        plan = self.plan_for(cdt, t.device)
+        ws = self._workspace_for(plan, batch, t.device)
+        out = torch.empty(lead + (self.output_grid.size,), dtype=cdt, device=t.device)
+        fwd = plan.autotune(_lib.FORWARD, t, out, batch, ws, repeats=repeats)
+        bwd = plan.autotune(_lib.BACKWARD, out, torch.empty_like(t), batch, ws, repeats=repeats)
+        return {'forward': fwd, 'backward': bwd}
 
     def forward(self, field):
         """Forward transform of a Field on ``input_grid`` (matrix_fourier_transform.py:132-176)."""

@@ -67,6 +67,38 @@ int hcipy_mft_workspace_size(hcipy_mft_plan plan, int64_t batch, size_t* bytes);
 /* Bytes of device memory held by the plan's matrices. */
 int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
 
+/* Workgroup tile shape of one contraction stage (stage 0 contracts the field's x axis, stage 1
+ * its y axis).  No counterpart in the reference (its BLAS picks blocking internally); used by
+ * hcipy_mft_autotune and by callers that pin a measured choice.  All shapes give identical
+ * results up to summation order.  Plans start with HCIPY_TILES_AUTO (a size heuristic). */
+#define HCIPY_TILES_AUTO -1
+#define HCIPY_TILES_64X64_W2X2 0 /* 64x64 tile, 2x2 waves of 32x32                          */
+#define HCIPY_TILES_64X32_W2X1 1 /* 64x32 tile, 2x1 waves, k-vectors split over 2 waves      */
+#define HCIPY_TILES_64X64_K4 2   /* 64x64 tile per wave, k-vectors split over 4 waves (not FP64) */
+#define HCIPY_TILES_64X32_K4 3   /* 64x32 tile per wave, k-vectors split over 4 waves (not FP64) */
+#define HCIPY_TILES_128X64 4     /* 128x64 tile, 2x2 waves of 64x32, half-depth k-tiles (not FP64) */
+#define HCIPY_TILES_64X128 5     /* 64x128 tile, 2x2 waves of 32x64, half-depth k-tiles (not FP64) */
+#define HCIPY_TILES_64X64_W8 6   /* 64x64 tile, 8 waves: 2x2 waves of 32x32, k-vectors split over 2 */
+#define HCIPY_TILES_64X32_W8 7   /* 64x32 tile, 8 waves: 2x1 waves of 32x32, k-vectors split over 4 (not FP64) */
+/* F16X2 only: the constant matrix stays compact (complex) in LDS and each wave forms the Re and Im
+ * operands in registers; tiles of 32 complex constant rows (= 64 real rows). */
+#define HCIPY_TILES_C64X64_W4 8  /* 32x64 compact tile, 4 waves, k-vectors split over 2 */
+#define HCIPY_TILES_C64X64_W8 9  /* 32x64 compact tile, 8 waves, k-vectors split over 4 */
+#define HCIPY_TILES_C64X32_W4 10 /* 32x32 compact tile, 4 waves, k-vectors split over 4 */
+#define HCIPY_TILES_C128X32_W8 11 /* 64x32 compact tile, 8 waves, k-vectors split over 4 */
+int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction, int stage, int tile_cfg);
+int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction, int stage, int* tile_cfg);
+
+/* Autotuner: for each stage of `direction`, times every tile shape available for the plan's
+ * precision on the caller's buffers (`repeats` runs each, HIP events around the stage kernel),
+ * pins the fastest on the plan and writes it to chosen[stage] (chosen may be NULL; chosen[1] is
+ * HCIPY_TILES_AUTO for 1-D plans).  `out` and `workspace` are overwritten.  No counterpart in
+ * the reference; it replaces the reference's measured planner choice
+ * (make_fourier_transform(planner='measure'), hcipy/fourier/fourier_transform.py:217-300) at the
+ * level of kernel tiles. */
+int hcipy_mft_autotune(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch, int output,
+                       void* workspace, size_t workspace_bytes, int repeats, int* chosen, void* stream);
+
 /* Runs `batch` transforms on `stream`.
  * Replaces MatrixFourierTransform.forward / .backward (matrix_fourier_transform.py:132-176,
  * 178-224) for scalar grid weights:

@@ -203,3 +203,52 @@ def test_fraunhofer_airy_closed_form():
     ref = 2 * scipy.special.jv(1, x) / x
     ref[ind] = 1
     assert np.abs(img - ref).max() < 3e-5
+
+
+@pytest.mark.parametrize('precision,cdt,tol', [('fp64', np.complex128, 1e-12), ('fp32', np.complex64, 3e-6),
+                                               ('f16x2', np.complex64, 5e-7)])
+def test_every_tile_shape_matches_oracle(precision, cdt, tol):
+    """Each workgroup tile shape (pinned on the plan for both stages and directions) gives the
+    oracle's result: ragged sizes exercise the k tails, the row clamps and the k-split sums."""
+    from hcipy_amd import _lib
+    gi = ha.make_uniform_grid([100, 90], [1.0, 0.9]).shifted([0.01, -0.02])
+    go = ha.make_uniform_grid([70, 50], [30.0, 25.0]).shifted([0.4, -0.3])
+    f = _rand(gi.size, 11, (2,)).astype(cdt)
+    F = _rand(go.size, 12, (2,)).astype(cdt)
+    ref_f = o.mft_forward(f.astype(np.complex128), og(gi), og(go))
+    ref_b = o.mft_backward(F.astype(np.complex128), og(gi), og(go))
+    mft = ha.MatrixFourierTransform(gi, go, precision=None if precision == 'fp64' else precision)
+    tdt = torch.complex128 if cdt == np.complex128 else torch.complex64
+    plan = mft.plan_for(tdt, torch.device('cuda', 0))
+    for cfg in _lib.TILE_CFGS[precision]:
+        for d in (_lib.FORWARD, _lib.BACKWARD):
+            for stage in (0, 1):
+                plan.set_tiles(d, stage, cfg)
+                assert plan.get_tiles(d, stage) == cfg
+        say(precision, 'tile shape', cfg)
+        a = host(mft.forward(ha.Field(dev(f), gi)))
+        b = host(mft.backward(ha.Field(dev(F), go)))
+        assert peak_rel_field(a, ref_f) < tol, cfg
+        assert peak_rel_field(b, ref_b) < tol, cfg
+
+
+def test_autotune_pins_supported_shapes_and_keeps_results():
+    """The autotuner picks one available shape per stage and direction; results after tuning
+    still match the oracle."""
+    from hcipy_amd import _lib
+    gi = ha.make_pupil_grid(96)
+    go = ha.make_focal_grid(4, 6)
+    f = _rand(gi.size, 21, (2,)).astype(np.complex64)
+    mft = ha.MatrixFourierTransform(gi, go, precision='f16x2')
+    chosen = mft.autotune(ha.Field(dev(f), gi), repeats=2)
+    say('autotune chose', chosen)
+    plan = mft.plan_for(torch.complex64, torch.device('cuda', 0))
+    for d, key in ((_lib.FORWARD, 'forward'), (_lib.BACKWARD, 'backward')):
+        for stage in (0, 1):
+            assert chosen[key][stage] in _lib.TILE_CFGS['f16x2']
+            assert plan.get_tiles(d, stage) == chosen[key][stage]
+    a = host(mft.forward(ha.Field(dev(f), gi)))
+    assert peak_rel_field(a, o.mft_forward(f.astype(np.complex128), og(gi), og(go))) < 5e-7
+    F = host(mft.forward(ha.Field(dev(f), gi)))
+    b = host(mft.backward(ha.Field(dev(F.astype(np.complex64)), go)))
+    assert peak_rel_field(b, o.mft_backward(F.astype(np.complex128), og(gi), og(go))) < 5e-7
