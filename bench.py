@@ -31,6 +31,9 @@ def parse():
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-autotune', action='store_true', help='keep the default tile shapes')
+    ap.add_argument('--no-variants', action='store_true', help='time only the main path (for counter runs)')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only run a plain 256 MiB device-to-device copy 3 times (scale of the HBM byte counters)')
     ap.add_argument('--cpu-seconds', type=float, default=2.0, help='t_max of the CPU timing loop')
     return ap.parse_args()
 
@@ -48,11 +51,19 @@ def main():
     import numpy as np
     import torch
 
-    import hcipy_amd as ha
-    from hcipy_amd import _lib, workloads
-
     dev = torch.device('cuda', 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read': src.numel() * 4, 'copy_bytes_written': dst.numel() * 4, 'copies': 3}))
+        return
+
+    import hcipy_amd as ha
+    from hcipy_amd import _lib, workloads
     pupil, focal, E = workloads.plain_mft_1024()
     E64 = E.astype(np.complex64)
     x = torch.as_tensor(E64, device=dev)
@@ -138,7 +149,7 @@ def main():
                                ('%s_field_c128' % args.precision, args.precision, _lib.OUTPUT_FIELD_C128),
                                ('fp32_field_c64', 'fp32', _lib.OUTPUT_FIELD),
                                ('fp32_intensity', 'fp32', _lib.OUTPUT_INTENSITY)]:
-        if prec == args.precision and output == main_out:
+        if args.no_variants or (prec == args.precision and output == main_out):
             continue
         sec, _, kms, o = time_path(prec, max(200, args.steps // 4), 20, output)
         variants[name] = {'value': round(1.0 / sec, 2), 'ms_per_step': round(sec * 1e3, 5),

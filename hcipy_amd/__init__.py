@@ -15,6 +15,9 @@ from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, Fresne
 from .fourier_filter import FourierFilter, NearFieldTransferFunction
 from .coronagraphy import LyotCoronagraph
 from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
+from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, make_actuator_positions,
+                                make_gaussian_influence_functions)
+from .metrics import get_mean_intensity_in_roi, get_mean_raw_contrast, get_strehl_from_focal, get_strehl_from_pupil
 from . import pointwise
 
 __version__ = '0.1.0'

@@ -22,6 +22,8 @@ OUTPUT_FIELD_C128 = 1
 OUTPUT_INTENSITY = 2
 OUTPUT_INTENSITY_F64 = 3
 TILES_AUTO = -1
+
+
 TILES_64X64_W2X2 = 0
 TILES_64X32_W2X1 = 1
 TILES_64X64_K4 = 2
@@ -35,6 +37,16 @@ TILES_C64X64_W8 = 9
 TILES_C64X32_W4 = 10
 TILES_C128X32_W8 = 11
 TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(12))}
+
+
+class DmGaussianDesc(ctypes.Structure):
+    """hcipy_dm_gaussian_desc of include/hcipy_amd.h."""
+    _fields_ = [('nx', ctypes.c_int64), ('ny', ctypes.c_int64), ('x0', ctypes.c_double), ('y0', ctypes.c_double),
+                ('dx', ctypes.c_double), ('dy', ctypes.c_double), ('n_act', ctypes.c_int64),
+                ('act_x0', ctypes.c_double), ('act_y0', ctypes.c_double), ('pitch', ctypes.c_double),
+                ('sigma', ctypes.c_double), ('radius', ctypes.c_double), ('x_tilt', ctypes.c_double),
+                ('y_tilt', ctypes.c_double), ('z_tilt', ctypes.c_double), ('oversampling', ctypes.c_int64)]
+
 
 _c_int64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -68,6 +80,11 @@ SIGNATURES = {
     'hcipy_pw_power_workspace': (ctypes.c_int, [_c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_pw_power': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _c_int64,
                                       _c_int64, _vp, _vp, ctypes.c_size_t, _vp]),
+    'hcipy_pw_masked_mean_workspace': (ctypes.c_int, [_c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_pw_masked_mean': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_int64, _c_int64, _c_int64, _vp,
+                                            _vp, ctypes.c_size_t, _vp]),
+    'hcipy_dm_gaussian_surface': (ctypes.c_int, [_vp, _vp, _vp, _c_int64, _vp]),
+    'hcipy_dm_gaussian_surface_adjoint': (ctypes.c_int, [_vp, _vp, _vp, _c_int64, _vp]),
     'hcipy_filter_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_int, _vp]),
     'hcipy_filter_plan_destroy': (ctypes.c_int, [_vp]),

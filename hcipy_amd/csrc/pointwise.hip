@@ -131,6 +131,59 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double* __restr
   if (threadIdx.x == 0) out[b] = red[0] + red[1] + red[2] + red[3];
 }
 
+// partial[2 (b * nblk + blockIdx.x) + {0, 1}] = (sum_i x_b[i] [m_i != 0], count of m_i != 0), FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void masked_partial_kernel(const R* __restrict__ x, const void* __restrict__ mask,
+                                                             int mask_dtype, int64_t mask_stride,
+                                                             double* __restrict__ partial, int64_t n) {
+  __shared__ double red[2][4];
+  const int64_t b = blockIdx.y;
+  const R* xb = x + b * n;
+  double acc = 0.0, cnt = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double m = 1.0, dummy;
+    if (mask) load_any(mask, mask_dtype, b * mask_stride + i, m, dummy);
+    if (m != 0.0) {
+      acc += (double)xb[i];
+      cnt += 1.0;
+    }
+  }
+  acc = wave_sum(acc);
+  cnt = wave_sum(cnt);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = acc;
+    red[1][threadIdx.x >> 6] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t k = 2 * (b * gridDim.x + blockIdx.x);
+    partial[k] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    partial[k + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+__global__ __launch_bounds__(256) void mean_partials_kernel(const double* __restrict__ partial, int nblk,
+                                                            double* __restrict__ out) {
+  __shared__ double red[2][4];
+  const int64_t b = blockIdx.x;
+  double acc = 0.0, cnt = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) {
+    acc += partial[2 * (b * nblk + i)];
+    cnt += partial[2 * (b * nblk + i) + 1];
+  }
+  acc = wave_sum(acc);
+  cnt = wave_sum(cnt);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = acc;
+    red[1][threadIdx.x >> 6] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double c = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    out[b] = (red[0][0] + red[0][1] + red[0][2] + red[0][3]) / c;  // NaN for an empty region, like numpy
+  }
+}
+
 constexpr int kThreads = 256;
 constexpr int kPowerBlocks = 256;
 
@@ -224,6 +277,34 @@ int hcipy_pw_power(const void* x, int field_dtype, const void* mask, int mask_dt
   HCIPY_HIP(hipGetLastError());
   hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)batch), dim3(kThreads), 0, s, partial, kPowerBlocks, out);
   return check_hip(hipGetLastError(), "sum_partials_kernel");
+}
+
+int hcipy_pw_masked_mean_workspace(int64_t batch, size_t* bytes) {
+  HCIPY_REQUIRE(bytes != nullptr && batch >= 0, "bad argument");
+  *bytes = (size_t)batch * kPowerBlocks * 2 * sizeof(double);
+  return kOk;
+}
+
+int hcipy_pw_masked_mean(const void* x, int dtype, const void* mask, int mask_dtype, int64_t mask_stride, int64_t n,
+                         int64_t batch, double* out, void* workspace, size_t workspace_bytes, void* stream) {
+  HCIPY_REQUIRE(dtype == HCIPY_DTYPE_F32 || dtype == HCIPY_DTYPE_F64, "values must be real");
+  HCIPY_REQUIRE(mask == nullptr || is_valid(mask_dtype), "bad mask dtype");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(workspace_bytes >= (size_t)batch * kPowerBlocks * 2 * sizeof(double), "workspace too small");
+  if (batch == 0) return kOk;
+  HCIPY_REQUIRE(x != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double* partial = static_cast<double*>(workspace);
+  const dim3 g(kPowerBlocks, (unsigned)batch);
+  if (dtype == HCIPY_DTYPE_F32)
+    hipLaunchKernelGGL(masked_partial_kernel<float>, g, dim3(kThreads), 0, s, (const float*)x, mask, mask_dtype,
+                       mask_stride, partial, n);
+  else
+    hipLaunchKernelGGL(masked_partial_kernel<double>, g, dim3(kThreads), 0, s, (const double*)x, mask, mask_dtype,
+                       mask_stride, partial, n);
+  HCIPY_HIP(hipGetLastError());
+  hipLaunchKernelGGL(mean_partials_kernel, dim3((unsigned)batch), dim3(kThreads), 0, s, partial, kPowerBlocks, out);
+  return check_hip(hipGetLastError(), "mean_partials_kernel");
 }
 
 }  // extern "C"

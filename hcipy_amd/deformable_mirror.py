@@ -1,0 +1,200 @@
+"""Deformable mirror with Gaussian influence functions (hcipy/optics/deformable_mirror.py).
+
+The influence functions are never stored as a matrix: ``GaussianInfluenceFunctions`` describes
+them (actuator grid, Gaussian width, cutoff, tilts, oversampling) and evaluates linear
+combinations and their adjoint with the ``hcipy_dm_gaussian_*`` kernels, in FP64.  It offers the
+parts of the reference's ModeBasis protocol the mirror uses: ``len``, ``grid``, iteration and
+indexing (one influence function as a Field), ``linear_combination`` and ``is_sparse``.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, pointwise
+from .field import Field, default_device, make_uniform_grid
+from .optics import OpticalElement, Wavefront
+
+__all__ = ['make_actuator_positions', 'make_gaussian_influence_functions', 'GaussianInfluenceFunctions',
+           'DeformableMirror']
+
+
+def make_actuator_positions(num_actuators_across_pupil, actuator_spacing, x_tilt=0, y_tilt=0, z_tilt=0):
+    """Actuator positions, BMC convention (deformable_mirror.py:11-41)."""
+    extent = actuator_spacing * num_actuators_across_pupil
+    grid = make_uniform_grid(num_actuators_across_pupil, [extent, extent]).scaled(np.cos([y_tilt, x_tilt]))
+    if z_tilt == 0:
+        return grid
+    return grid.rotated(z_tilt)
+
+
+class GaussianInfluenceFunctions:
+    """Truncated Gaussian influence functions of an n x n actuator grid on a regular 2-D pupil
+    grid (make_gaussian_influence_functions, deformable_mirror.py:43-93, with
+    make_gaussian_pokes, mode_basis/gaussian_pokes.py:5-54)."""
+
+    is_sparse = True
+
+    def __init__(self, pupil_grid, num_actuators_across_pupil, actuator_spacing, crosstalk=0.15, cutoff=3,
+                 x_tilt=0, y_tilt=0, z_tilt=0, oversampling=1):
+        if pupil_grid.ndim != 2 or not pupil_grid.is_regular:
+            raise ValueError('Gaussian influence functions need a regular two-dimensional pupil grid.')
+        n = int(num_actuators_across_pupil)
+        pos = make_actuator_positions(n, actuator_spacing)
+        self.grid = pupil_grid
+        self.num_actuators_across_pupil = n
+        self.actuator_spacing = float(actuator_spacing)
+        self.sigma = actuator_spacing / math.sqrt(-2 * math.log(crosstalk))
+        # cutoff in units of sigma as make_gaussian_pokes receives it, times sigma
+        self.radius = (actuator_spacing / self.sigma * cutoff) * self.sigma
+        d = _lib.DmGaussianDesc()
+        d.nx, d.ny = (int(v) for v in pupil_grid.dims)
+        d.x0, d.y0 = (float(v) for v in pupil_grid.zero)
+        d.dx, d.dy = (float(v) for v in pupil_grid.delta)
+        d.n_act = n
+        d.act_x0, d.act_y0 = (float(v) for v in pos.zero)
+        d.pitch = float(pos.delta[0])
+        d.sigma = self.sigma
+        d.radius = self.radius
+        d.x_tilt, d.y_tilt, d.z_tilt = float(x_tilt), float(y_tilt), float(z_tilt)
+        d.oversampling = int(round(oversampling))
+        self._desc = d
+
+    def __len__(self):
+        return self.num_actuators_across_pupil ** 2
+
+    @property
+    def num_modes(self):
+        return len(self)
+
+    def _coeffs(self, coefficients):
+        c = coefficients
+        if isinstance(c, torch.Tensor):
+            c = c.to(device=default_device(), dtype=torch.float64)
+        else:
+            c = torch.as_tensor(np.asarray(c, dtype=np.float64), device=default_device())
+        if c.shape[-1] != len(self):
+            raise ValueError('Expected %d coefficients, got %d.' % (len(self), c.shape[-1]))
+        return c.contiguous()
+
+    def linear_combination(self, coefficients):
+        """sum_a c[a] IF_a as an FP64 Field (mode_basis.py:239-258); leading dimensions of the
+        coefficients are a batch."""
+        c = self._coeffs(coefficients)
+        lead = tuple(c.shape[:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        out = torch.empty(lead + (self.grid.size,), dtype=torch.float64, device=c.device)
+        st = _lib.load().hcipy_dm_gaussian_surface(ctypes.byref(self._desc), _lib.dptr(c), _lib.dptr(out), batch,
+                                                   _lib.stream_ptr(c.device))
+        _lib.check(st, 'hcipy_dm_gaussian_surface')
+        return Field(out, self.grid)
+
+    def linear_combination_adjoint(self, field):
+        """g[a] = sum_pixel field[pixel] IF_a(pixel): the transposed influence matrix applied to
+        a pupil-plane array (FP64); leading dimensions are a batch."""
+        t = field.as_subclass(torch.Tensor) if isinstance(field, Field) else torch.as_tensor(field)
+        t = t.to(device=default_device(), dtype=torch.float64).contiguous()
+        if t.shape[-1] != self.grid.size:
+            raise ValueError('Field size %d does not match the pupil grid size %d' % (t.shape[-1], self.grid.size))
+        lead = tuple(t.shape[:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        out = torch.empty(lead + (len(self),), dtype=torch.float64, device=t.device)
+        st = _lib.load().hcipy_dm_gaussian_surface_adjoint(ctypes.byref(self._desc), _lib.dptr(t), _lib.dptr(out),
+                                                           batch, _lib.stream_ptr(t.device))
+        _lib.check(st, 'hcipy_dm_gaussian_surface_adjoint')
+        return out
+
+    def __getitem__(self, i):
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        c = torch.zeros(len(self), dtype=torch.float64, device=default_device())
+        c[i] = 1.0
+        return self.linear_combination(c)
+
+    def __iter__(self):
+        for i in range(len(self)):
+            yield self[i]
+
+    @property
+    def transformation_matrix(self):
+        """Dense (pixels x actuators) FP64 matrix, for small configurations."""
+        if self.grid.size * len(self) * 8 > (1 << 31):
+            raise ValueError('The dense influence matrix would exceed 2 GiB.')
+        eye = torch.eye(len(self), dtype=torch.float64, device=default_device())
+        return self.linear_combination(eye).as_subclass(torch.Tensor).T
+
+
+def make_gaussian_influence_functions(pupil_grid, num_actuators_across_pupil, actuator_spacing, crosstalk=0.15,
+                                      cutoff=3, x_tilt=0, y_tilt=0, z_tilt=0, oversampling=1):
+    """Gaussian influence functions (deformable_mirror.py:43-93): sigma = spacing /
+    sqrt(-2 ln crosstalk), truncated at ``cutoff`` actuator spacings."""
+    return GaussianInfluenceFunctions(pupil_grid, num_actuators_across_pupil, actuator_spacing, crosstalk, cutoff,
+                                      x_tilt, y_tilt, z_tilt, oversampling)
+
+
+class DeformableMirror(OpticalElement):
+    """A deformable mirror using influence functions (deformable_mirror.py:168-299).
+
+    ``actuators`` is a host float64 array (as in the reference) and may be changed in place;
+    the surface is recomputed on the GPU when it changed.  forward multiplies the field by
+    exp(2 i k surface), backward by exp(-2 i k surface).
+    """
+
+    def __init__(self, influence_functions):
+        self.influence_functions = influence_functions
+        self.actuators = np.zeros(len(influence_functions))
+        self.input_grid = influence_functions.grid
+        self._surface = None
+
+    @property
+    def influence_functions(self):
+        return self._influence_functions
+
+    @influence_functions.setter
+    def influence_functions(self, influence_functions):
+        self._influence_functions = influence_functions
+        self._actuators_for_cached_surface = None
+
+    @property
+    def num_actuators(self):
+        return self._actuators.size
+
+    @property
+    def actuators(self):
+        return self._actuators
+
+    @actuators.setter
+    def actuators(self, actuators):
+        self._actuators = np.asarray(actuators, dtype=np.float64)
+
+    @property
+    def surface(self):
+        """The mirror surface in meters (FP64 Field on the device)."""
+        if self._actuators_for_cached_surface is not None and np.array_equal(self._actuators,
+                                                                            self._actuators_for_cached_surface):
+            return self._surface
+        self._surface = self.influence_functions.linear_combination(self._actuators)
+        self._actuators_for_cached_surface = self._actuators.copy()
+        return self._surface
+
+    @property
+    def opd(self):
+        return 2 * self.surface
+
+    def phase_for(self, wavelength):
+        return self.surface * (4 * np.pi / wavelength)
+
+    def random(self, rms):
+        self._actuators = np.random.randn(self._actuators.size) * rms
+
+    def flatten(self):
+        self._actuators = np.zeros(len(self.influence_functions))
+
+    def forward(self, wavefront):
+        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self.surface, 2 * wavefront.wavenumber),
+                         wavefront.wavelength)
+
+    def backward(self, wavefront):
+        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self.surface, -2 * wavefront.wavenumber),
+                         wavefront.wavelength)

@@ -284,6 +284,15 @@ class CartesianGrid:
     def shifted(self, shift):
         return CartesianGrid(self.coords_obj.shifted(shift), self._weights)
 
+    def rotated(self, angle):
+        """2-D rotation of the points by ``angle`` (R = [[cos, -sin], [sin, cos]] applied to the
+        points, hcipy/field/cartesian_grid.py:12-32,144-165); the result is unstructured."""
+        if self.ndim != 2:
+            raise ValueError('Only two-dimensional grids can be rotated here.')
+        x, y = self.coords
+        c, s = math.cos(angle), math.sin(angle)
+        return CartesianGrid(UnstructuredCoords([c * x - s * y, s * x + c * y]), self._weights)
+
     def copy(self):
         return CartesianGrid(self.coords_obj, self._weights)
 

@@ -174,6 +174,47 @@ int hcipy_pw_power(const void* x, int field_dtype, const void* mask, int mask_dt
                    double weight, int64_t n, int64_t batch, double* out, void* workspace, size_t workspace_bytes,
                    void* stream);
 
+/* out[b] = mean of x[b][i] over the i with mask[b * mask_stride + i] != 0 (all i if mask is
+ * NULL), for real float32 / float64 values (HCIPY_DTYPE_F32 / F64), summed in FP64 in a fixed
+ * order; NaN for an empty region.  Replaces get_mean_intensity_in_roi
+ * (hcipy/metrics/contrast.py:37-52, np.mean(img[mask])).  `out` is device memory; the workspace
+ * holds hcipy_pw_masked_mean_workspace(batch) bytes. */
+int hcipy_pw_masked_mean_workspace(int64_t batch, size_t* bytes);
+int hcipy_pw_masked_mean(const void* x, int dtype, const void* mask, int mask_dtype, int64_t mask_stride, int64_t n,
+                         int64_t batch, double* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Deformable mirror with Gaussian influence functions on a regular pupil grid, evaluated
+ * without forming the influence matrix.  Replaces make_gaussian_influence_functions +
+ * DeformableMirror.surface (hcipy/optics/deformable_mirror.py:43-93, 255-267;
+ * hcipy/mode_basis/gaussian_pokes.py:5-54; ModeBasis.linear_combination,
+ * hcipy/mode_basis/mode_basis.py:239-258).  Influence function of actuator a at pixel p:
+ *   q = R(-z_tilt) . diag(1/cos(y_tilt), 1/cos(x_tilt)) . (p + dither),  r2 = |q - m_a|^2,
+ *   g = exp(-r2 / (2 sigma^2)) - exp(-radius^2 / (2 sigma^2)) for r2 <= radius^2, else 0,
+ *   IF_a(p) = mean of g over oversampling^2 dithers / (cos(x_tilt) cos(y_tilt)),
+ * actuators m_a on an n_act x n_act grid (x fastest) of the given origin and pitch.
+ * The reference's defaults: sigma = pitch / sqrt(-2 ln crosstalk), radius = 3 pitch. */
+typedef struct {
+  int64_t nx, ny;          /* pupil grid samples, x fastest */
+  double x0, y0, dx, dy;   /* pupil grid coordinates x_i = x0 + i dx, y_j = y0 + j dy */
+  int64_t n_act;           /* actuators across (n_act^2 actuators) */
+  double act_x0, act_y0;   /* position of actuator (0, 0) */
+  double pitch;            /* actuator spacing */
+  double sigma, radius;    /* Gaussian width and support radius */
+  double x_tilt, y_tilt, z_tilt;
+  int64_t oversampling;    /* 1..16 */
+} hcipy_dm_gaussian_desc;
+
+/* surface[b][pixel] = sum_a actuators[b][a] IF_a(pixel), FP64, for `batch` actuator vectors. */
+int hcipy_dm_gaussian_surface(const hcipy_dm_gaussian_desc* desc, const double* actuators, double* surface,
+                              int64_t batch, void* stream);
+
+/* grad_actuators[b][a] = sum_pixel grad_surface[b][pixel] IF_a(pixel): the adjoint of the
+ * surface map, i.e. the transposed influence matrix (the reference applies
+ * influence_functions.transformation_matrix.T for DM Jacobians).  Deterministic order. */
+int hcipy_dm_gaussian_surface_adjoint(const hcipy_dm_gaussian_desc* desc, const double* grad_surface,
+                                      double* grad_actuators, int64_t batch, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Fourier filter: zero-pad -> 2-D FFT -> transfer function -> inverse 2-D FFT -> crop.
  * Replaces FourierFilter (hcipy/fourier/fourier_operations.py:8-151) for scalar transfer

@@ -555,24 +555,38 @@ def actuator_positions(n_act, pitch):
     return make_uniform_grid(n_act, [pitch * n_act, pitch * n_act])
 
 
-def gaussian_influence_functions(pupil_grid, n_act, pitch, crosstalk=0.15, cutoff=3):
-    """make_gaussian_influence_functions (deformable_mirror.py:43-93) with
-    make_gaussian_pokes (mode_basis/gaussian_pokes.py:5-54), no tilt, oversampling 1.
-    Returns a dense (n_act^2, Npix) array."""
+def gaussian_influence_functions(pupil_grid, n_act, pitch, crosstalk=0.15, cutoff=3, x_tilt=0, y_tilt=0, z_tilt=0,
+                                 oversampling=1):
+    """make_gaussian_influence_functions (deformable_mirror.py:43-93) with make_gaussian_pokes
+    (mode_basis/gaussian_pokes.py:5-54) and evaluate_supersampled (field/util.py:606-698):
+    each poke is evaluated on the pupil grid scaled by 1/cos([y_tilt, x_tilt]) and rotated by
+    -z_tilt (R = [[cos, -sin], [sin, cos]] on the points, cartesian_grid.py:12-32), divided by
+    cos(x_tilt) cos(y_tilt), and averaged over the oversampling dithers ((k + 0.5)/ovs - 0.5)
+    times the grid step.  Returns a dense (n_act^2, Npix) array."""
     pos = actuator_positions(n_act, pitch)
     sigma = pitch / np.sqrt(-2 * np.log(crosstalk))
     cut = pitch / sigma * cutoff
     px, py = pos.coords
-    x = pupil_grid.x
-    y = pupil_grid.y
+    x0 = pupil_grid.x
+    y0 = pupil_grid.y
+    dx, dy = pupil_grid.delta
+    ovs = int(round(oversampling))
+    dith = (np.arange(ovs) + 0.5) / ovs - 0.5
     out = np.zeros((len(px), pupil_grid.size))
-    for i, (mx, my) in enumerate(zip(px, py)):
-        r2 = (x - mx) ** 2 + (y - my) ** 2
-        res = np.exp(-0.5 * r2 / sigma ** 2)
-        res -= np.exp(-0.5 * cut ** 2)
-        res[r2 > (cut * sigma) ** 2] = 0
-        out[i] = res
-    return out
+    ca, sa = np.cos(-z_tilt), np.sin(-z_tilt)
+    for fy in dith:
+        for fx in dith:
+            xs = (x0 + fx * dx) / np.cos(y_tilt)
+            ys = (y0 + fy * dy) / np.cos(x_tilt)
+            qx = ca * xs - sa * ys
+            qy = sa * xs + ca * ys
+            for i, (mx, my) in enumerate(zip(px, py)):
+                r2 = (qx - mx) ** 2 + (qy - my) ** 2
+                res = np.exp(-0.5 * r2 / sigma ** 2)
+                res -= np.exp(-0.5 * cut ** 2)
+                res[r2 > (cut * sigma) ** 2] = 0
+                out[i] += res / (np.cos(x_tilt) * np.cos(y_tilt))
+    return out / ovs ** 2
 
 
 def dm_surface(influence, actuators):

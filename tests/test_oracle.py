@@ -299,3 +299,51 @@ def test_gaussian_influence_crosstalk_and_cutoff():
     assert np.all(IF[27][r2 > (3 * pitch) ** 2] == 0)
     s = o.dm_surface(IF, np.eye(n_act * n_act)[27] * 2.0)
     np.testing.assert_allclose(s, 2 * IF[27])
+
+
+@pytest.mark.parametrize('n_act', [12, 16])
+def test_tilted_influence_centroids(n_act):
+    """Port of the reference's DM test (tests/test_optics.py:119-162): the centroid of every
+    tilted Gaussian influence function lies within 0.02 of the tilted actuator position."""
+    x_tilt, y_tilt, z_tilt = np.radians(15), np.radians(10), np.radians(20)
+    grid = o.make_pupil_grid(96, 1.5)
+    pitch = 1.0 / n_act
+    IF = o.gaussian_influence_functions(grid, n_act, pitch, x_tilt=x_tilt, y_tilt=y_tilt, z_tilt=z_tilt)
+    pos = o.actuator_positions(n_act, pitch)
+    px, py = pos.coords
+    px, py = px * np.cos(y_tilt), py * np.cos(x_tilt)
+    c, s = np.cos(z_tilt), np.sin(z_tilt)
+    px, py = c * px - s * py, s * px + c * py
+    for k in range(n_act * n_act):
+        w = IF[k]
+        assert np.allclose([(w * grid.x).sum() / w.sum(), (w * grid.y).sum() / w.sum()], [px[k], py[k]], atol=0.02)
+
+
+def test_oversampled_influence_matches_pixel_average():
+    """With oversampling the influence function is the pixel average of the Gaussian: near the
+    centre (no truncation) it matches the closed-form erf average of exp(-r^2/2 sigma^2) over
+    the pixel, minus the cutoff offset."""
+    from scipy.special import erf
+    grid = o.make_pupil_grid(48, 1.0)
+    n_act, pitch = 4, 0.25
+    ovs = 16
+    IF = o.gaussian_influence_functions(grid, n_act, pitch, oversampling=ovs)
+    sigma = pitch / np.sqrt(-2 * np.log(0.15))
+    offset = np.exp(-0.5 * (3 * pitch / sigma) ** 2)
+    pos = o.actuator_positions(n_act, pitch)
+    k = 5
+    mx, my = pos.coords[0][k], pos.coords[1][k]
+    dx = grid.delta[0]
+
+    def avg1(x, m):
+        a = (x - dx / 2 - m) / (np.sqrt(2) * sigma)
+        b = (x + dx / 2 - m) / (np.sqrt(2) * sigma)
+        return sigma * np.sqrt(np.pi / 2) * (erf(b) - erf(a)) / dx
+
+    exact = avg1(grid.x, mx) * avg1(grid.y, my) - offset
+    near = (grid.x - mx) ** 2 + (grid.y - my) ** 2 < (1.5 * sigma) ** 2
+    assert near.sum() > 10
+    np.testing.assert_allclose(IF[k][near], exact[near], rtol=0, atol=2e-5)
+    # a dither grid without the half-step centring would be biased by ~dx/2 and fail this
+    IF1 = o.gaussian_influence_functions(grid, n_act, pitch, oversampling=1)
+    assert np.abs(IF1[k][near] - exact[near]).max() > 10 * np.abs(IF[k][near] - exact[near]).max()
