@@ -165,9 +165,21 @@ def main():
     achieved = flops_stage1 / (ms1 * 1e-3) / 1e12
     peak_tf = PEAK_TFLOPS[args.precision]
     t0_cfg = tiles.get(args.precision, (-1, -1))[0]
+    # HBM bytes per stage-1 launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # (copy-calibrated), valid for the tile shape they were measured with
+    traffic, traffic_src = None, None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'mft_stage1_hbm_traffic.json')
+    if args.precision == 'f16x2' and os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        cfg_used = t0_cfg if t0_cfg >= 0 else _lib.TILES_64X64_W8
+        if tj.get('tile_cfg_stage1') == cfg_used:
+            traffic = tj['fetch_bytes_per_launch'] + tj['write_bytes_per_launch']
+            traffic_src = 'profiles/mft_stage1_hbm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)' 
     kname = 'mft_f16x2_kernel' if t0_cfg >= _lib.TILES_C64X64_W4 else 'mft_stage_kernel'
     roofline = {'bound': 'mfma', 'kernel': '%s (stage 1, E.M2)' % kname, 'achieved': round(achieved, 2),
-                'peak': peak_tf, 'unit': 'TFLOP/s', 'frac': round(achieved / peak_tf, 4), 'traffic': None,
+                'peak': peak_tf, 'unit': 'TFLOP/s', 'frac': round(achieved / peak_tf, 4), 'traffic': traffic,
+                'traffic_unit': 'bytes per launch', 'traffic_source': traffic_src,
                 'kernel_ms': {'split': round(kms_main[0], 5), 'stage1': round(kms_main[1], 5),
                               'stage2': round(kms_main[2], 5)},
                 'algorithmic_flop_per_launch': flops_stage1}

@@ -173,8 +173,10 @@ size_t out_elem_bytes(int mode) {
   }
 }
 
+// Runs one direction: data split (split-FP16 only), stage 1, stage 2.  `only` >= 0 launches just
+// that kernel (0 split, 1 stage 1, 2 stage 2) on a workspace a full run has filled (timing).
 int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai,
-                  char* ws, hipStream_t s, hipEvent_t* ev = nullptr, int output = HCIPY_OUTPUT_FIELD) {
+                  char* ws, hipStream_t s, hipEvent_t* ev = nullptr, int output = HCIPY_OUTPUT_FIELD, int only = -1) {
   auto mark = [&](int i) -> int {
     if (ev != nullptr) return check_hip(hipEventRecord(ev[i], s), "hipEventRecord");
     return kOk;
@@ -199,9 +201,11 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
     _Float16* lo = reinterpret_cast<_Float16*>(ws + w.split_lo);
     row_scale = reinterpret_cast<float*>(ws + w.row_scale);
     t = reinterpret_cast<float*>(ws + w.t);
-    int st = launch_split_rows(static_cast<const float*>(in), 2 * d.k1, (int)(2 * d.k1), rows1, hi, lo, w.ld1,
-                               row_scale, s);
-    if (st != kOk) return st;
+    if (only < 0 || only == 0) {
+      int st = launch_split_rows(static_cast<const float*>(in), 2 * d.k1, (int)(2 * d.k1), rows1, hi, lo, w.ld1,
+                                 row_scale, s);
+      if (st != kOk) return st;
+    }
     a1.b = hi;
     a1.b_lo = lo;
     a1.ldb = w.ld1;
@@ -227,7 +231,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
     a1.out_mode = out_mode;
     a1.alpha_re = ar;
     a1.alpha_im = ai;
-    int st1 = launch_stage(pl.prec, a1, 1, b_mode, s);
+    int st1 = (only < 0 || only == 1) ? launch_stage(pl.prec, a1, 1, b_mode, s) : kOk;
     if (!st1) st1 = mark(2);
     if (!st1) st1 = mark(3);
     return st1;
@@ -243,7 +247,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   a1.t_out = t;
   a1.alpha_re = 1.0;
   a1.alpha_im = 0.0;
-  int st = launch_stage(pl.prec, a1, 1, b_mode, s);
+  int st = (only < 0 || only == 1) ? launch_stage(pl.prec, a1, 1, b_mode, s) : kOk;
   if (st != kOk) return st;
   if ((st = mark(2))) return st;
 
@@ -266,7 +270,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   a2.alpha_re = ar;
   a2.alpha_im = ai;
   const size_t out_es = out_elem_bytes(out_mode);
-  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridZ) {
+  for (int64_t b0 = 0; b0 < batch && (only < 0 || only == 2); b0 += kMaxGridZ) {
     const int nz = (int)std::min<int64_t>(kMaxGridZ, batch - b0);
     StageArgs a = a2;
     a.b = static_cast<const char*>(a2.b) + b0 * a2.b_zstride * es;
@@ -439,27 +443,26 @@ int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void*
   HCIPY_REQUIRE(workspace_bytes >= w.total, "workspace too small: %zu < %zu bytes", workspace_bytes, w.total);
   HCIPY_REQUIRE(aligned16(workspace), "workspace must be 16-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int nev = 4 * repeats;
-  hipEvent_t* ev = new (std::nothrow) hipEvent_t[nev];
-  HCIPY_REQUIRE(ev != nullptr, "out of host memory");
+  hipEvent_t ev[2];
   int created = 0;
   int st = kOk;
-  for (; created < nev && !st; ++created) st = check_hip(hipEventCreate(&ev[created]), "hipEventCreate");
-  for (int r = 0; r < repeats && !st; ++r)
-    st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s, ev + 4 * r,
+  for (; created < 2 && !st; ++created) st = check_hip(hipEventCreate(&ev[created]), "hipEventCreate");
+  // one full run fills the workspace, then each kernel is timed over `repeats` back-to-back launches
+  if (!st)
+    st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s, nullptr,
                        output);
-  if (!st) st = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
   for (int k = 0; k < 3 && !st; ++k) {
-    double acc = 0.0;
-    for (int r = 0; r < repeats && !st; ++r) {
-      float ms = 0.0f;
-      st = check_hip(hipEventElapsedTime(&ms, ev[4 * r + k], ev[4 * r + k + 1]), "hipEventElapsedTime");
-      acc += ms;
-    }
-    kernel_ms[k] = (float)(acc / repeats);
+    st = check_hip(hipEventRecord(ev[0], s), "hipEventRecord");
+    for (int r = 0; r < repeats && !st; ++r)
+      st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s,
+                         nullptr, output, k);
+    if (!st) st = check_hip(hipEventRecord(ev[1], s), "hipEventRecord");
+    if (!st) st = check_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
+    float ms = 0.0f;
+    if (!st) st = check_hip(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
+    kernel_ms[k] = ms / (float)repeats;
   }
   for (int i = 0; i < created; ++i) (void)hipEventDestroy(ev[i]);
-  delete[] ev;
   return st;
 }
 

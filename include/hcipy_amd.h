@@ -126,9 +126,10 @@ int hcipy_mft_execute_ex(hcipy_mft_plan plan, int direction, const void* in, voi
                          double alpha_re, double alpha_im, int output, void* workspace, size_t workspace_bytes,
                          void* stream);
 
-/* Runs hcipy_mft_execute_ex `repeats` times on `stream` with HIP events recorded around each of its
- * kernels, waits, and stores the mean duration in milliseconds of: kernel_ms[0] the data split
- * (split-FP16 only, else ~0), kernel_ms[1] stage 1, kernel_ms[2] stage 2 (all batch slices).
+/* Runs hcipy_mft_execute_ex once on `stream`, then launches each of its kernels alone `repeats`
+ * times back to back between two HIP events, waits, and stores the mean duration per launch in
+ * milliseconds of: kernel_ms[0] the data split (split-FP16 only, else ~0), kernel_ms[1] stage 1,
+ * kernel_ms[2] stage 2 (all batch slices).  `out` and `workspace` are overwritten.
  * Measurement aid with no counterpart in the reference (whose MFT is timed by _time_it,
  * hcipy/fourier/fourier_transform.py:185-215); not for production loops. */
 int hcipy_mft_profile(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch, double alpha_re,
