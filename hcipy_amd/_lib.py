@@ -82,7 +82,13 @@ SIGNATURES = {
                                       _c_int64, _vp, _vp, ctypes.c_size_t, _vp]),
     'hcipy_pw_masked_mean_workspace': (ctypes.c_int, [_c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_pw_masked_mean': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_int64, _c_int64, _c_int64, _vp,
-                                            _vp, ctypes.c_size_t, _vp]),
+                                            _vp, _vp, ctypes.c_size_t, _vp]),
+    'hcipy_pw_masked_mean_adjoint': (ctypes.c_int, [_vp, ctypes.c_int, _c_int64, _vp, _vp, _vp, ctypes.c_int, _c_int64,
+                                                    _c_int64, _vp]),
+    'hcipy_pw_phase_adjoint': (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _c_int64, _c_int64, _c_int64,
+                                              _vp]),
+    'hcipy_pw_intensity_adjoint': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_int64, _vp, _c_int64, _c_int64,
+                                                  _vp]),
     'hcipy_dm_gaussian_surface': (ctypes.c_int, [_vp, _vp, _vp, _c_int64, _vp]),
     'hcipy_dm_gaussian_surface_adjoint': (ctypes.c_int, [_vp, _vp, _vp, _c_int64, _vp]),
     'hcipy_filter_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,

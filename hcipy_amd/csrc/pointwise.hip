@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void masked_partial_kernel(const R* __restrict
 }
 
 __global__ __launch_bounds__(256) void mean_partials_kernel(const double* __restrict__ partial, int nblk,
-                                                            double* __restrict__ out) {
+                                                            double* __restrict__ out, double* __restrict__ count) {
   __shared__ double red[2][4];
   const int64_t b = blockIdx.x;
   double acc = 0.0, cnt = 0.0;
@@ -181,6 +181,53 @@ __global__ __launch_bounds__(256) void mean_partials_kernel(const double* __rest
   if (threadIdx.x == 0) {
     const double c = red[1][0] + red[1][1] + red[1][2] + red[1][3];
     out[b] = (red[0][0] + red[0][1] + red[0][2] + red[0][3]) / c;  // NaN for an empty region, like numpy
+    if (count != nullptr) count[b] = c;
+  }
+}
+
+// Adjoint of out = x exp(i s phi) with respect to phi: d[i] = s Re(conj(g) i y) = -s Im(conj(g) y),
+// summed over the batch when phi is broadcast (phi_stride 0), FP64.
+template <typename C>
+__global__ void phase_adjoint_kernel(const C* __restrict__ y, const C* __restrict__ g, double scale,
+                                     double* __restrict__ out, int64_t phi_stride, int64_t n, int64_t batch) {
+  const int64_t nb = (phi_stride == 0) ? 1 : batch;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n * nb; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k % n;
+    const int64_t b0 = (phi_stride == 0) ? 0 : k / n;
+    const int64_t b1 = (phi_stride == 0) ? batch : b0 + 1;
+    double acc = 0.0;
+    for (int64_t b = b0; b < b1; ++b) {
+      const C gv = g[b * n + i], yv = y[b * n + i];
+      acc += (double)gv.re * (double)yv.im - (double)gv.im * (double)yv.re;  // Im(conj(g) y)
+    }
+    out[k] = -scale * acc;
+  }
+}
+
+// Adjoint of I = |x|^2: out = 2 g x (g real, broadcast when g_stride is 0).
+template <typename C, typename R>
+__global__ void intensity_adjoint_kernel(const C* __restrict__ x, const void* __restrict__ g, int g_dtype,
+                                         int64_t g_stride, C* __restrict__ out, int64_t n) {
+  const int64_t b = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double gv, dummy;
+    load_any(g, g_dtype, b * g_stride + i, gv, dummy);
+    const C v = x[b * n + i];
+    out[b * n + i] = C{(R)(2.0 * gv * v.re), (R)(2.0 * gv * v.im)};
+  }
+}
+
+// Adjoint of the region-of-interest mean: out[b][i] = g[b] / count[b] where mask != 0, else 0.
+template <typename R>
+__global__ void masked_mean_adjoint_kernel(const void* __restrict__ mask, int mask_dtype, int64_t mask_stride,
+                                           const double* __restrict__ g, const double* __restrict__ count,
+                                           R* __restrict__ out, int64_t n) {
+  const int64_t b = blockIdx.y;
+  const double v = g[b] / count[b];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double m = 1.0, dummy;
+    if (mask) load_any(mask, mask_dtype, b * mask_stride + i, m, dummy);
+    out[b * n + i] = (R)((m != 0.0) ? v : 0.0);
   }
 }
 
@@ -286,7 +333,8 @@ int hcipy_pw_masked_mean_workspace(int64_t batch, size_t* bytes) {
 }
 
 int hcipy_pw_masked_mean(const void* x, int dtype, const void* mask, int mask_dtype, int64_t mask_stride, int64_t n,
-                         int64_t batch, double* out, void* workspace, size_t workspace_bytes, void* stream) {
+                         int64_t batch, double* out, double* count, void* workspace, size_t workspace_bytes,
+                         void* stream) {
   HCIPY_REQUIRE(dtype == HCIPY_DTYPE_F32 || dtype == HCIPY_DTYPE_F64, "values must be real");
   HCIPY_REQUIRE(mask == nullptr || is_valid(mask_dtype), "bad mask dtype");
   HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
@@ -303,8 +351,65 @@ int hcipy_pw_masked_mean(const void* x, int dtype, const void* mask, int mask_dt
     hipLaunchKernelGGL(masked_partial_kernel<double>, g, dim3(kThreads), 0, s, (const double*)x, mask, mask_dtype,
                        mask_stride, partial, n);
   HCIPY_HIP(hipGetLastError());
-  hipLaunchKernelGGL(mean_partials_kernel, dim3((unsigned)batch), dim3(kThreads), 0, s, partial, kPowerBlocks, out);
+  hipLaunchKernelGGL(mean_partials_kernel, dim3((unsigned)batch), dim3(kThreads), 0, s, partial, kPowerBlocks, out,
+                     count);
   return check_hip(hipGetLastError(), "mean_partials_kernel");
+}
+
+int hcipy_pw_masked_mean_adjoint(const void* mask, int mask_dtype, int64_t mask_stride, const double* g,
+                                 const double* count, void* out, int out_dtype, int64_t n, int64_t batch,
+                                 void* stream) {
+  HCIPY_REQUIRE(out_dtype == HCIPY_DTYPE_F32 || out_dtype == HCIPY_DTYPE_F64, "gradient must be real");
+  HCIPY_REQUIRE(mask == nullptr || is_valid(mask_dtype), "bad mask dtype");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  if (n == 0 || batch == 0) return kOk;
+  HCIPY_REQUIRE(g != nullptr && count != nullptr && out != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid = grid_for(n, batch);
+  if (out_dtype == HCIPY_DTYPE_F32)
+    hipLaunchKernelGGL(masked_mean_adjoint_kernel<float>, grid, dim3(kThreads), 0, s, mask, mask_dtype, mask_stride, g,
+                       count, (float*)out, n);
+  else
+    hipLaunchKernelGGL(masked_mean_adjoint_kernel<double>, grid, dim3(kThreads), 0, s, mask, mask_dtype, mask_stride,
+                       g, count, (double*)out, n);
+  return check_hip(hipGetLastError(), "masked_mean_adjoint_kernel");
+}
+
+int hcipy_pw_phase_adjoint(const void* y, const void* g, int field_dtype, double scale, double* out,
+                           int64_t phi_stride, int64_t n, int64_t batch, void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
+  HCIPY_REQUIRE(phi_stride == 0 || phi_stride == n, "phase stride must be 0 or n");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
+  if (n == 0 || batch == 0) return kOk;
+  HCIPY_REQUIRE(y != nullptr && g != nullptr && out != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t total = n * (phi_stride == 0 ? 1 : batch);
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, kThreads), 8192)));
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL(phase_adjoint_kernel<C64>, grid, dim3(kThreads), 0, s, (const C64*)y, (const C64*)g, scale, out,
+                       phi_stride, n, batch);
+  else
+    hipLaunchKernelGGL(phase_adjoint_kernel<C128>, grid, dim3(kThreads), 0, s, (const C128*)y, (const C128*)g, scale,
+                       out, phi_stride, n, batch);
+  return check_hip(hipGetLastError(), "phase_adjoint_kernel");
+}
+
+int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, int g_dtype, int64_t g_stride, void* out,
+                               int64_t n, int64_t batch, void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
+  HCIPY_REQUIRE(g_dtype == HCIPY_DTYPE_F32 || g_dtype == HCIPY_DTYPE_F64, "gradient of the intensity must be real");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  if (n == 0 || batch == 0) return kOk;
+  HCIPY_REQUIRE(x != nullptr && g != nullptr && out != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid = grid_for(n, batch);
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((intensity_adjoint_kernel<C64, float>), grid, dim3(kThreads), 0, s, (const C64*)x, g, g_dtype,
+                       g_stride, (C64*)out, n);
+  else
+    hipLaunchKernelGGL((intensity_adjoint_kernel<C128, double>), grid, dim3(kThreads), 0, s, (const C128*)x, g,
+                       g_dtype, g_stride, (C128*)out, n);
+  return check_hip(hipGetLastError(), "intensity_adjoint_kernel");
 }
 
 }  // extern "C"

@@ -29,6 +29,19 @@ def make_actuator_positions(num_actuators_across_pupil, actuator_spacing, x_tilt
     return grid.rotated(z_tilt)
 
 
+class _SurfaceFn(torch.autograd.Function):
+    """surface = IF . c with the transposed influence functions as its backward."""
+
+    @staticmethod
+    def forward(ctx, c, basis):
+        ctx.basis = basis
+        return basis._surface_plain(c)
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.basis.linear_combination_adjoint(g.contiguous()), None
+
+
 class GaussianInfluenceFunctions:
     """Truncated Gaussian influence functions of an n x n actuator grid on a regular 2-D pupil
     grid (make_gaussian_influence_functions, deformable_mirror.py:43-93, with
@@ -71,7 +84,7 @@ class GaussianInfluenceFunctions:
     def _coeffs(self, coefficients):
         c = coefficients
         if isinstance(c, torch.Tensor):
-            c = c.to(device=default_device(), dtype=torch.float64)
+            c = c.to(device=default_device() if c.device.type != 'cuda' else c.device, dtype=torch.float64)
         else:
             c = torch.as_tensor(np.asarray(c, dtype=np.float64), device=default_device())
         if c.shape[-1] != len(self):
@@ -80,15 +93,21 @@ class GaussianInfluenceFunctions:
 
     def linear_combination(self, coefficients):
         """sum_a c[a] IF_a as an FP64 Field (mode_basis.py:239-258); leading dimensions of the
-        coefficients are a batch."""
+        coefficients are a batch.  Differentiable with respect to tensor coefficients (the
+        backward is ``linear_combination_adjoint``)."""
         c = self._coeffs(coefficients)
+        if c.requires_grad and torch.is_grad_enabled():
+            return Field(_SurfaceFn.apply(c, self), self.grid)
+        return Field(self._surface_plain(c), self.grid)
+
+    def _surface_plain(self, c):
         lead = tuple(c.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         out = torch.empty(lead + (self.grid.size,), dtype=torch.float64, device=c.device)
         st = _lib.load().hcipy_dm_gaussian_surface(ctypes.byref(self._desc), _lib.dptr(c), _lib.dptr(out), batch,
                                                    _lib.stream_ptr(c.device))
         _lib.check(st, 'hcipy_dm_gaussian_surface')
-        return Field(out, self.grid)
+        return out
 
     def linear_combination_adjoint(self, field):
         """g[a] = sum_pixel field[pixel] IF_a(pixel): the transposed influence matrix applied to
@@ -166,11 +185,19 @@ class DeformableMirror(OpticalElement):
 
     @actuators.setter
     def actuators(self, actuators):
-        self._actuators = np.asarray(actuators, dtype=np.float64)
+        # a torch tensor (for example one that requires grad) is kept as is, anything else
+        # becomes a host float64 array as in the reference
+        if isinstance(actuators, torch.Tensor):
+            self._actuators = actuators
+        else:
+            self._actuators = np.asarray(actuators, dtype=np.float64)
 
     @property
     def surface(self):
-        """The mirror surface in meters (FP64 Field on the device)."""
+        """The mirror surface in meters (FP64 Field on the device).  Cached for host actuator
+        arrays; recomputed (and differentiable) for tensor actuators."""
+        if isinstance(self._actuators, torch.Tensor):
+            return self.influence_functions.linear_combination(self._actuators)
         if self._actuators_for_cached_surface is not None and np.array_equal(self._actuators,
                                                                             self._actuators_for_cached_surface):
             return self._surface

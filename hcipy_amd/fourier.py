@@ -227,6 +227,22 @@ class _MftPlan:
             pass
 
 
+class _MftFn(torch.autograd.Function):
+    """One MFT direction as an autograd node.  The vector-Jacobian product of
+    out = alpha M1 . in . M2 is conj(alpha) M1^H . g . M2^H: the opposite direction's kernels with
+    the conjugate scale (and vice versa), in the same precision."""
+
+    @staticmethod
+    def forward(ctx, t, mft, direction, alpha):
+        ctx.mft, ctx.direction, ctx.alpha = mft, direction, alpha
+        return mft._execute_plain(t, direction, alpha)
+
+    @staticmethod
+    def backward(ctx, g):
+        other = _lib.BACKWARD if ctx.direction == _lib.FORWARD else _lib.FORWARD
+        return ctx.mft._execute_plain(g.contiguous(), other, ctx.alpha.conjugate()), None, None, None
+
+
 class MatrixFourierTransform(FourierTransform):
     """Matrix Fourier transform between separated Cartesian grids, on MFMA.
 
@@ -324,6 +340,23 @@ class MatrixFourierTransform(FourierTransform):
         else:
             t = (t * torch.as_tensor(np.asarray(w), device=t.device)).to(cdt).contiguous()
             alpha = complex(extra_alpha)
+        if t.requires_grad and torch.is_grad_enabled():
+            if out is not None:
+                raise ValueError('out= is not supported while recording gradients')
+            o = _MftFn.apply(t, self, direction, alpha)
+            if output == _lib.OUTPUT_FIELD:
+                return Field(o, out_grid)
+            if output == _lib.OUTPUT_FIELD_C128:
+                return Field(o.to(torch.complex128), out_grid)
+            from .pointwise import intensity
+            i = intensity(o)
+            return Field(i.to(torch.float64) if output == _lib.OUTPUT_INTENSITY_F64 else i, out_grid)
+        return Field(self._execute_plain(t, direction, alpha, output, out), out_grid)
+
+    def _execute_plain(self, t, direction, alpha, output=_lib.OUTPUT_FIELD, out=None):
+        """Runs the kernels on a contiguous complex tensor; returns the output tensor."""
+        out_grid = self.output_grid if direction == _lib.FORWARD else self.input_grid
+        cdt = t.dtype
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         plan = self.plan_for(cdt, t.device)
@@ -335,7 +368,7 @@ class MatrixFourierTransform(FourierTransform):
         ws = self._workspace_for(plan, batch, t.device)
         plan.execute(direction, t, out, batch, alpha, ws, output)
         self._release()
-        return Field(out, out_grid)
+        return out
 
     def autotune(self, field, repeats=20):
         """Times every kernel tile shape of both directions on fields like ``field`` (same dtype

@@ -84,6 +84,20 @@ class _FilterPlan:
             pass
 
 
+class _FilterFn(torch.autograd.Function):
+    """FourierFilter as an autograd node: crop . IFFT . diag(tf) . FFT . pad has the adjoint
+    crop . IFFT . diag(conj tf) . FFT . pad, i.e. the filter with ``adjoint`` flipped."""
+
+    @staticmethod
+    def forward(ctx, t, flt, adjoint):
+        ctx.flt, ctx.adjoint = flt, adjoint
+        return flt._run_plain(t, adjoint)
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.flt._run_plain(g.contiguous(), not ctx.adjoint), None, None
+
+
 class FourierFilter:
     """Filter in the Fourier domain with zero padding q (fourier_operations.py:8-151).
 
@@ -121,6 +135,13 @@ class FourierFilter:
         _lib.require_device(t)
         cdt = complex_dtype_for(t.dtype)
         t = t.to(cdt).contiguous()
+        if t.requires_grad and torch.is_grad_enabled():
+            return Field(_FilterFn.apply(t, self, adjoint), grid)
+        return Field(self._run_plain(t, adjoint), grid)
+
+    def _run_plain(self, t, adjoint):
+        """Filter (or its adjoint) of a contiguous complex tensor; returns a plain tensor."""
+        cdt = t.dtype
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         NY, NX = self.internal_grid.shape
@@ -132,8 +153,8 @@ class FourierFilter:
                 self._plans[key] = plan
             out = torch.empty_like(t)
             plan.execute(adjoint, t, out, batch)
-            return Field(out, grid)
-        return self._mft_operation(t, adjoint, cdt)
+            return out
+        return self._mft_operation(t, adjoint, cdt).as_subclass(torch.Tensor)
 
     def _mft_operation(self, t, adjoint, cdt):
         if self._mft is None:

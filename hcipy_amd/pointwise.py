@@ -1,5 +1,10 @@
 """Pointwise operations on Fields through the library's HBM-bound kernels
-(C-ABI ``hcipy_pw_*``, include/hcipy_amd.h)."""
+(C-ABI ``hcipy_pw_*``, include/hcipy_amd.h).
+
+Every operation is differentiable: when an input requires grad (and grad mode is on) it runs
+inside a torch.autograd.Function whose backward is the adjoint kernel (conjugate multiply,
+``hcipy_pw_phase_adjoint``, ``hcipy_pw_intensity_adjoint``), with PyTorch's convention for complex
+gradients.  The modulating array of ``sub_mul``/``multiply`` is treated as a constant."""
 import ctypes
 
 import numpy as np
@@ -8,7 +13,7 @@ import torch
 from . import _lib
 from .field import Field
 
-__all__ = ['multiply', 'sub_mul', 'multiply_phase', 'intensity', 'power_sum', 'total_power']
+__all__ = ['multiply', 'sub_mul', 'multiply_phase', 'intensity', 'power_sum', 'total_power', 'needs_grad']
 
 
 def _plain(t):
@@ -37,6 +42,41 @@ def _operand(a, ref):
     return a, stride
 
 
+def needs_grad(*ts):
+    """True when autograd must record an operation on these inputs."""
+    return torch.is_grad_enabled() and any(isinstance(t, torch.Tensor) and t.requires_grad for t in ts)
+
+
+def _sub_mul_raw(xt, yt, at, stride, conjugate, o=None):
+    o = torch.empty_like(xt) if o is None else o
+    n = xt.shape[-1]
+    batch = xt.numel() // n if n else 0
+    st = _lib.load().hcipy_pw_sub_mul(_lib.dptr(xt), _lib.dptr(yt) if yt is not None else None,
+                                      _lib.dptr(at) if at is not None else None,
+                                      _lib.dtype_code(at) if at is not None else 0, stride, int(bool(conjugate)),
+                                      _lib.dptr(o), _lib.dtype_code(xt), n, batch, _lib.stream_ptr(xt.device))
+    _lib.check(st, 'hcipy_pw_sub_mul')
+    return o
+
+
+class _SubMulFn(torch.autograd.Function):
+    """(x - y) a; adjoint g conj(a) for x and -g conj(a) for y."""
+
+    @staticmethod
+    def forward(ctx, xt, yt, at, stride, conjugate):
+        ctx.save_for_backward(at)
+        ctx.stride, ctx.conjugate, ctx.has_y = stride, conjugate, yt is not None
+        return _sub_mul_raw(xt, yt, at, stride, conjugate)
+
+    @staticmethod
+    def backward(ctx, g):
+        (at,) = ctx.saved_tensors
+        g = g.contiguous()
+        gx = _sub_mul_raw(g, None, at, ctx.stride, not ctx.conjugate) if at is not None else g
+        gy = gx.neg() if ctx.has_y and ctx.needs_input_grad[1] else None
+        return (gx if ctx.needs_input_grad[0] else None), gy, None, None, None
+
+
 def sub_mul(x, y=None, a=None, conjugate=False, out=None):
     """(x - y) * a (conj(a) if `conjugate`) in one pass; returns a Field on x's grid."""
     grid = getattr(x, 'grid', None)
@@ -46,20 +86,55 @@ def sub_mul(x, y=None, a=None, conjugate=False, out=None):
     if y is not None:
         yt = _plain(y).to(xt.dtype).contiguous()
     at, stride = _operand(a, xt)
-    o = torch.empty_like(xt) if out is None else _plain(out)
-    n = xt.shape[-1]
-    batch = xt.numel() // n if n else 0
-    st = _lib.load().hcipy_pw_sub_mul(_lib.dptr(xt), _lib.dptr(yt) if yt is not None else None,
-                                      _lib.dptr(at) if at is not None else None,
-                                      _lib.dtype_code(at) if at is not None else 0, stride, int(bool(conjugate)),
-                                      _lib.dptr(o), _lib.dtype_code(xt), n, batch, _lib.stream_ptr(xt.device))
-    _lib.check(st, 'hcipy_pw_sub_mul')
+    if needs_grad(xt, yt):
+        if out is not None:
+            raise ValueError('out= is not supported while recording gradients')
+        o = _SubMulFn.apply(xt, yt, at, stride, bool(conjugate))
+    else:
+        o = _sub_mul_raw(xt, yt, at, stride, conjugate, None if out is None else _plain(out))
     return Field(o, grid) if grid is not None else o
 
 
 def multiply(x, a, conjugate=False):
     """x * a (or x * conj(a))."""
     return sub_mul(x, None, a, conjugate)
+
+
+def _phase_raw(xt, pt, stride, scale):
+    o = torch.empty_like(xt)
+    n = xt.shape[-1]
+    batch = xt.numel() // n if n else 0
+    st = _lib.load().hcipy_pw_phase(_lib.dptr(xt), _lib.dptr(pt), _lib.dtype_code(pt), stride, float(scale),
+                                    _lib.dptr(o), _lib.dtype_code(xt), n, batch, _lib.stream_ptr(xt.device))
+    _lib.check(st, 'hcipy_pw_phase')
+    return o
+
+
+class _PhaseFn(torch.autograd.Function):
+    """y = x exp(i s phi); adjoints g exp(-i s phi) for x and -s Im(conj(g) y) for phi."""
+
+    @staticmethod
+    def forward(ctx, xt, pt, stride, scale):
+        y = _phase_raw(xt, pt, stride, scale)
+        ctx.save_for_backward(pt, y)
+        ctx.stride, ctx.scale = stride, scale
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        pt, y = ctx.saved_tensors
+        g = g.contiguous()
+        gx = _phase_raw(g, pt, ctx.stride, -ctx.scale) if ctx.needs_input_grad[0] else None
+        gp = None
+        if ctx.needs_input_grad[1]:
+            n = y.shape[-1]
+            batch = y.numel() // n if n else 0
+            gp = torch.empty(pt.shape, dtype=torch.float64, device=y.device)
+            st = _lib.load().hcipy_pw_phase_adjoint(_lib.dptr(y), _lib.dptr(g), _lib.dtype_code(y), float(ctx.scale),
+                                                    _lib.dptr(gp), ctx.stride, n, batch, _lib.stream_ptr(y.device))
+            _lib.check(st, 'hcipy_pw_phase_adjoint')
+            gp = gp.to(pt.dtype)
+        return gx, gp, None, None
 
 
 def multiply_phase(x, phi, scale=1.0):
@@ -70,12 +145,10 @@ def multiply_phase(x, phi, scale=1.0):
     pt, stride = _operand(phi, xt)
     if pt.is_complex():
         raise ValueError('phase must be real')
-    o = torch.empty_like(xt)
-    n = xt.shape[-1]
-    batch = xt.numel() // n if n else 0
-    st = _lib.load().hcipy_pw_phase(_lib.dptr(xt), _lib.dptr(pt), _lib.dtype_code(pt), stride, float(scale),
-                                    _lib.dptr(o), _lib.dtype_code(xt), n, batch, _lib.stream_ptr(xt.device))
-    _lib.check(st, 'hcipy_pw_phase')
+    if needs_grad(xt, pt):
+        o = _PhaseFn.apply(xt, pt, stride, float(scale))
+    else:
+        o = _phase_raw(xt, pt, stride, scale)
     return Field(o, grid) if grid is not None else o
 
 
@@ -84,14 +157,40 @@ def intensity(x):
     grid = getattr(x, 'grid', None)
     xt = _plain(x).contiguous()
     _lib.require_device(xt)
-    rdt = torch.float32 if xt.dtype == torch.complex64 else torch.float64
     if not xt.is_complex():
         xt = xt.to(torch.complex64 if xt.dtype == torch.float32 else torch.complex128)
+    o = _IntensityFn.apply(xt) if needs_grad(xt) else _intensity_raw(xt)
+    return Field(o, grid) if grid is not None else o
+
+
+def _intensity_raw(xt):
+    rdt = torch.float32 if xt.dtype == torch.complex64 else torch.float64
     o = torch.empty(xt.shape, dtype=rdt, device=xt.device)
     st = _lib.load().hcipy_pw_intensity(_lib.dptr(xt), _lib.dtype_code(xt), _lib.dptr(o), xt.numel(),
                                         _lib.stream_ptr(xt.device))
     _lib.check(st, 'hcipy_pw_intensity')
-    return Field(o, grid) if grid is not None else o
+    return o
+
+
+class _IntensityFn(torch.autograd.Function):
+    """I = |x|^2; adjoint 2 g x."""
+
+    @staticmethod
+    def forward(ctx, xt):
+        ctx.save_for_backward(xt)
+        return _intensity_raw(xt)
+
+    @staticmethod
+    def backward(ctx, g):
+        (xt,) = ctx.saved_tensors
+        g = g.contiguous()
+        n = xt.shape[-1]
+        batch = xt.numel() // n if n else 0
+        gx = torch.empty_like(xt)
+        st = _lib.load().hcipy_pw_intensity_adjoint(_lib.dptr(xt), _lib.dtype_code(xt), _lib.dptr(g), _lib.dtype_code(g),
+                                                    n, _lib.dptr(gx), n, batch, _lib.stream_ptr(xt.device))
+        _lib.check(st, 'hcipy_pw_intensity_adjoint')
+        return gx
 
 
 def power_sum(x, weight=1.0, mask=None):

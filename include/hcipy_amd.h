@@ -182,7 +182,29 @@ int hcipy_pw_power(const void* x, int field_dtype, const void* mask, int mask_dt
  * holds hcipy_pw_masked_mean_workspace(batch) bytes. */
 int hcipy_pw_masked_mean_workspace(int64_t batch, size_t* bytes);
 int hcipy_pw_masked_mean(const void* x, int dtype, const void* mask, int mask_dtype, int64_t mask_stride, int64_t n,
-                         int64_t batch, double* out, void* workspace, size_t workspace_bytes, void* stream);
+                         int64_t batch, double* out, double* count, void* workspace, size_t workspace_bytes,
+                         void* stream);
+/* (count, device, may be NULL: the number of selected samples per batch item.) */
+
+/* ---------------------------------------------------------------------------------------------
+ * Adjoints (vector-Jacobian products) of the pointwise operations, with PyTorch's convention
+ * for complex gradients (g = dL/d conj(output)).  They serve the torch.autograd.Function wrappers
+ * of the package; the reference has no differentiation (its backward() of an optical element is
+ * the adjoint propagation these reproduce for the linear operations). */
+
+/* out[b][i] = g[b] / count[b] where mask != 0, else 0: adjoint of hcipy_pw_masked_mean. */
+int hcipy_pw_masked_mean_adjoint(const void* mask, int mask_dtype, int64_t mask_stride, const double* g,
+                                 const double* count, void* out, int out_dtype, int64_t n, int64_t batch,
+                                 void* stream);
+
+/* Gradient with respect to phi of y = x exp(i scale phi) (hcipy_pw_phase) given y and g:
+ * out = -scale Im(conj(g) y) in FP64, summed over the batch when phi_stride is 0. */
+int hcipy_pw_phase_adjoint(const void* y, const void* g, int field_dtype, double scale, double* out,
+                           int64_t phi_stride, int64_t n, int64_t batch, void* stream);
+
+/* Gradient with respect to x of I = |x|^2 (hcipy_pw_intensity): out = 2 g x, g real. */
+int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, int g_dtype, int64_t g_stride, void* out,
+                               int64_t n, int64_t batch, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Deformable mirror with Gaussian influence functions on a regular pupil grid, evaluated
