@@ -1,0 +1,68 @@
+"""The propagation chain inside a captured HIP graph: every replay (with eager work and new
+inputs in between) must equal the eager result of the same input bit for bit.  Plans and
+workspaces are created by an eager warm-up call; no execute path allocates or records memsets."""
+import numpy as np
+import pytest
+import torch
+
+import hcipy_amd as ha
+
+from helpers import host, say
+
+pytestmark = pytest.mark.gpu
+
+
+def test_graph_replays_equal_eager():
+    pupil = ha.make_pupil_grid(128)
+    focal = ha.make_focal_grid(4, 16)
+    fpm_grid = ha.make_focal_grid(8, 4)
+    aperture = torch.as_tensor(host(ha.make_circular_aperture(1.0)(pupil)), device='cuda')
+    fpm = 1 - ha.make_circular_aperture(3.0)(fpm_grid)
+    lyot = ha.make_circular_aperture(0.95)(pupil)
+    prop32 = ha.FraunhoferPropagator(pupil, focal, precision='f16x2')
+    coro64 = ha.LyotCoronagraph(pupil, fpm, lyot, focal_plane_mask_grid=fpm_grid)
+    prop64 = ha.FraunhoferPropagator(pupil, focal)
+    asp_grid = ha.make_pupil_grid(128, 1e-2)
+    asp = ha.AngularSpectrumPropagator(asp_grid, 0.2)
+    IF = ha.make_gaussian_influence_functions(pupil, 8, 1 / 8)
+    dm = ha.DeformableMirror(IF)
+    dm.actuators = np.random.default_rng(1).standard_normal(64) * 1e-3
+
+    x = torch.zeros(pupil.size, dtype=torch.complex128, device='cuda')
+
+    def chain():
+        wf = ha.Wavefront(ha.Field(x * aperture, pupil), 1.0)
+        wf = dm.forward(wf)
+        i32 = prop32.forward(ha.Wavefront(ha.Field(wf.electric_field.as_subclass(torch.Tensor).to(torch.complex64), pupil), 1.0)).intensity
+        i64 = prop64.forward(coro64.forward(wf)).intensity
+        near = asp.forward(ha.Wavefront(ha.Field(x.to(torch.complex64), asp_grid), 5e-7)).electric_field
+        return [t.as_subclass(torch.Tensor) for t in (i32, i64, near)]
+
+    def new_input(seed):
+        g = torch.Generator().manual_seed(seed)
+        ph = torch.randn(pupil.size, generator=g, dtype=torch.float64) * 0.1
+        return torch.exp(1j * ph).cuda()
+
+    x.copy_(new_input(0))
+    chain()  # warm-up: plans, matrices, workspaces
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        chain()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        outs = chain()
+    torch.cuda.synchronize()
+    for r in range(5):
+        x.copy_(new_input(r + 1))
+        graph.replay()
+        torch.cuda.synchronize()
+        got = [o.clone() for o in outs]
+        ref = chain()  # eager, same input
+        torch.cuda.synchronize()
+        say('replay', r)
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b)
