@@ -8,7 +8,7 @@ from .config import Configuration
 from .field import (CartesianGrid, Field, RegularCoords, SeparatedCoords, UnstructuredCoords, default_device,
                     evaluate_supersampled, make_focal_grid, make_pupil_grid, make_uniform_grid, set_default_device,
                     subsample_field)
-from .fourier import (ComputationalComplexity, FastFourierTransform, FourierTransform, MatrixFourierTransform,
+from .fourier import (ComputationalComplexity, FastFourierTransform, FourierTransform, MatrixFourierTransform, MftGroup,
                       get_fft_parameters, is_fft_grid, make_fft_grid, make_fourier_transform)
 from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, OpticalSystem, PhaseApodizer, Wavefront
 from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, FresnelPropagator

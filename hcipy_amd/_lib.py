@@ -39,6 +39,14 @@ TILES_C128X32_W8 = 11
 TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(12))}
 
 
+class MftGroupItem(ctypes.Structure):
+    """hcipy_mft_group_item of include/hcipy_amd.h."""
+    _fields_ = [('plan', ctypes.c_void_p), ('direction', ctypes.c_int), ('in_', ctypes.c_void_p),
+                ('out', ctypes.c_void_p), ('batch', ctypes.c_int64), ('alpha_re', ctypes.c_double),
+                ('alpha_im', ctypes.c_double), ('output', ctypes.c_int), ('workspace', ctypes.c_void_p),
+                ('workspace_bytes', ctypes.c_size_t)]
+
+
 class DmGaussianDesc(ctypes.Structure):
     """hcipy_dm_gaussian_desc of include/hcipy_amd.h."""
     _fields_ = [('nx', ctypes.c_int64), ('ny', ctypes.c_int64), ('x0', ctypes.c_double), ('y0', ctypes.c_double),
@@ -63,6 +71,9 @@ SIGNATURES = {
     'hcipy_mft_plan_bytes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_mft_plan_set_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     'hcipy_mft_plan_get_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    'hcipy_mft_group_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
+    'hcipy_mft_group_destroy': (ctypes.c_int, [_vp]),
+    'hcipy_mft_execute_grouped': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     'hcipy_mft_autotune': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_int, _vp, ctypes.c_size_t,
                                           ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp]),
     'hcipy_mft_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,

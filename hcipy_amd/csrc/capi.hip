@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/hcipy_amd.h"
 #include "mft_gemm.hpp"
@@ -26,6 +27,9 @@ int check_hip(hipError_t e, const char* what) {
 }
 
 int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s);
+int launch_stage_grouped(int prec, int stage, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
+                         int64_t tiles, hipStream_t s);
+void grouped_tile_shape(int stage, int* bm, int* bn);
 constexpr int kModePadded = 0, kModeAligned = 1, kModeUnaligned = 2;  // BMode of mft_gemm.hip
 int launch_split_rows(const float* in, int64_t ld_in, int K, int64_t rows, _Float16* hi, _Float16* lo,
                       int64_t ld_out, float* row_scale, hipStream_t s);
@@ -173,10 +177,19 @@ size_t out_elem_bytes(int mode) {
   }
 }
 
+// Stage arguments of one direction, gathered for a grouped launch instead of being launched.
+struct Collected {
+  StageArgs a1{}, a2{};
+  int b_mode1 = 0;
+  int64_t nz2 = 0;  // batch slices of stage 2 (0: 1-D plan, no stage 2)
+};
+
 // Runs one direction: data split (split-FP16 only), stage 1, stage 2.  `only` >= 0 launches just
 // that kernel (0 split, 1 stage 1, 2 stage 2) on a workspace a full run has filled (timing).
+// With `col`, the split is launched and the stage arguments are returned instead of launched.
 int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai,
-                  char* ws, hipStream_t s, hipEvent_t* ev = nullptr, int output = HCIPY_OUTPUT_FIELD, int only = -1) {
+                  char* ws, hipStream_t s, hipEvent_t* ev = nullptr, int output = HCIPY_OUTPUT_FIELD, int only = -1,
+                  Collected* col = nullptr) {
   auto mark = [&](int i) -> int {
     if (ev != nullptr) return check_hip(hipEventRecord(ev[i], s), "hipEventRecord");
     return kOk;
@@ -231,6 +244,12 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
     a1.out_mode = out_mode;
     a1.alpha_re = ar;
     a1.alpha_im = ai;
+    if (col != nullptr) {
+      col->a1 = a1;
+      col->b_mode1 = b_mode;
+      col->nz2 = 0;
+      return kOk;
+    }
     int st1 = (only < 0 || only == 1) ? launch_stage(pl.prec, a1, 1, b_mode, s) : kOk;
     if (!st1) st1 = mark(2);
     if (!st1) st1 = mark(3);
@@ -247,7 +266,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   a1.t_out = t;
   a1.alpha_re = 1.0;
   a1.alpha_im = 0.0;
-  int st = (only < 0 || only == 1) ? launch_stage(pl.prec, a1, 1, b_mode, s) : kOk;
+  int st = (only < 0 || only == 1) && col == nullptr ? launch_stage(pl.prec, a1, 1, b_mode, s) : kOk;
   if (st != kOk) return st;
   if ((st = mark(2))) return st;
 
@@ -269,6 +288,15 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   a2.out_zstride = d.c2 * d.c1;
   a2.alpha_re = ar;
   a2.alpha_im = ai;
+  if (col != nullptr) {
+    col->a1 = a1;
+    col->b_mode1 = b_mode;
+    a2.out = out;
+    a2.z_scale = t;
+    col->a2 = a2;
+    col->nz2 = batch;
+    return kOk;
+  }
   const size_t out_es = out_elem_bytes(out_mode);
   for (int64_t b0 = 0; b0 < batch && (only < 0 || only == 2); b0 += kMaxGridZ) {
     const int nz = (int)std::min<int64_t>(kMaxGridZ, batch - b0);
@@ -291,6 +319,24 @@ using namespace hcipy;
 struct hcipy_mft_plan_s {
   Plan p;
 };
+
+// Table of stage arguments for grouped launches: device copy and pinned host mirror.
+// Layout: StageArgs[2 * cap] | int64 tile starts[2 * (cap + 4)].
+struct hcipy_mft_group_s {
+  int cap = 0;
+  int device = 0;
+  size_t bytes = 0;
+  char* dev = nullptr;
+  char* host = nullptr;
+  hipEvent_t copied = nullptr;
+  bool pending = false;
+};
+
+namespace {
+size_t group_table_bytes(int cap) {
+  return round_up((int64_t)(2 * (size_t)cap * sizeof(StageArgs)), 64) + 2 * ((size_t)cap + 4) * sizeof(int64_t);
+}
+}  // namespace
 
 extern "C" {
 
@@ -464,6 +510,110 @@ int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void*
   }
   for (int i = 0; i < created; ++i) (void)hipEventDestroy(ev[i]);
   return st;
+}
+
+int hcipy_mft_group_create(hcipy_mft_group* group, int capacity) {
+  HCIPY_REQUIRE(group != nullptr && capacity > 0 && capacity <= (1 << 20), "bad argument");
+  *group = nullptr;
+  hcipy_mft_group g = new (std::nothrow) hcipy_mft_group_s;
+  HCIPY_REQUIRE(g != nullptr, "out of host memory");
+  g->cap = capacity;
+  g->bytes = group_table_bytes(capacity);
+  int st = check_hip(hipGetDevice(&g->device), "hipGetDevice");
+  if (!st) st = check_hip(hipMalloc(&g->dev, g->bytes), "hipMalloc(group table)");
+  if (!st) st = check_hip(hipHostMalloc(&g->host, g->bytes), "hipHostMalloc(group table)");
+  if (!st) st = check_hip(hipEventCreateWithFlags(&g->copied, hipEventDisableTiming), "hipEventCreate");
+  if (st) {
+    hcipy_mft_group_destroy(g);
+    return st;
+  }
+  *group = g;
+  return kOk;
+}
+
+int hcipy_mft_group_destroy(hcipy_mft_group group) {
+  if (group == nullptr) return kOk;
+  if (group->copied) {
+    (void)hipEventSynchronize(group->copied);
+    (void)hipEventDestroy(group->copied);
+  }
+  if (group->dev) (void)hipFree(group->dev);
+  if (group->host) (void)hipHostFree(group->host);
+  delete group;
+  return kOk;
+}
+
+int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item* items, int n_items, void* stream) {
+  HCIPY_REQUIRE(group != nullptr && items != nullptr, "NULL argument");
+  HCIPY_REQUIRE(n_items > 0 && n_items <= group->cap, "%d items for a group of capacity %d", n_items, group->cap);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int cur = 0;
+  HCIPY_HIP(hipGetDevice(&cur));
+  HCIPY_REQUIRE(cur == group->device, "the group was created on device %d, current device is %d", group->device, cur);
+  const int prec = items[0].plan != nullptr ? items[0].plan->p.prec : -1;
+  std::vector<Collected> col((size_t)n_items);
+  for (int i = 0; i < n_items; ++i) {
+    const hcipy_mft_group_item& it = items[i];
+    HCIPY_REQUIRE(it.plan != nullptr && it.in != nullptr && it.out != nullptr && it.batch > 0, "bad item %d", i);
+    const Plan& pl = it.plan->p;
+    HCIPY_REQUIRE(pl.prec == prec, "item %d: all items of a grouped launch share one precision", i);
+    HCIPY_REQUIRE(pl.device == cur, "item %d: plan lives on device %d", i, pl.device);
+    HCIPY_REQUIRE(it.direction == HCIPY_FORWARD || it.direction == HCIPY_BACKWARD, "item %d: bad direction", i);
+    const Ws w = workspace_layout(pl, direction(pl, it.direction), it.batch);
+    HCIPY_REQUIRE(it.workspace_bytes >= w.total && aligned16(it.workspace), "item %d: workspace too small", i);
+    // the data split (split-FP16) is launched here, per item; stage arguments are gathered
+    const int st = run_direction(pl, it.direction, it.in, it.out, it.batch, it.alpha_re, it.alpha_im,
+                                 static_cast<char*>(it.workspace), s, nullptr, it.output, -1, &col[(size_t)i]);
+    if (st != kOk) return st;
+  }
+  // The mirror may still be read by the previous call's copy: wait for it before rewriting, and
+  // only if it has not completed (a completed event needs no synchronisation, so a call inside
+  // stream capture that follows a finished eager call does not synchronise).
+  if (group->pending && hipEventQuery(group->copied) == hipErrorNotReady) HCIPY_HIP(hipEventSynchronize(group->copied));
+  // Table: stage-1 items ordered by data mode (one launch per mode), then stage-2 items.
+  StageArgs* hargs = reinterpret_cast<StageArgs*>(group->host);
+  int64_t* hstart = reinterpret_cast<int64_t*>(group->host + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
+  struct Launch {
+    int stage, b_mode, first, n;
+    int64_t start_off, tiles;
+  };
+  std::vector<Launch> launches;
+  int na = 0;
+  int64_t ns = 0;
+  for (int stage = 0; stage < 2; ++stage) {
+    for (int mode = 0; mode < 3; ++mode) {
+      if (stage == 1 && mode != kModePadded) continue;
+      int bm, bn;
+      grouped_tile_shape(stage, &bm, &bn);
+      Launch L{stage, mode, na, 0, ns, 0};
+      int64_t acc = 0;
+      for (int i = 0; i < n_items; ++i) {
+        const Collected& c = col[(size_t)i];
+        if (stage == 0 && c.b_mode1 != mode) continue;
+        if (stage == 1 && c.nz2 == 0) continue;
+        const StageArgs& a = stage == 0 ? c.a1 : c.a2;
+        const int64_t nz = stage == 0 ? 1 : c.nz2;
+        hargs[na++] = a;
+        hstart[ns++] = acc;
+        acc += ceil_div(a.M, bm) * ceil_div(a.N, bn) * nz;
+        ++L.n;
+      }
+      if (L.n == 0) continue;
+      hstart[ns++] = acc;
+      L.tiles = acc;
+      launches.push_back(L);
+    }
+  }
+  HCIPY_HIP(hipMemcpyAsync(group->dev, group->host, group->bytes, hipMemcpyHostToDevice, s));
+  HCIPY_HIP(hipEventRecord(group->copied, s));
+  group->pending = true;
+  const StageArgs* dargs = reinterpret_cast<const StageArgs*>(group->dev);
+  const int64_t* dstart = reinterpret_cast<const int64_t*>(group->dev + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
+  for (const Launch& L : launches) {
+    const int st = launch_stage_grouped(prec, L.stage, L.b_mode, dargs + L.first, dstart + L.start_off, L.n, L.tiles, s);
+    if (st != kOk) return st;
+  }
+  return kOk;
 }
 
 int hcipy_mft_autotune(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch, int output,

@@ -149,8 +149,9 @@ __device__ inline u32x4_t expand_odd(u32x4_t v) {
 // Split path: the cross accumulator runs over the whole k range; the main (hi.hi) accumulator
 // alternates between two sets per k-tile, and the set of tile t-1 is flushed into FP64 while
 // the MFMAs of tile t run.
+// Body of the stage kernel for workgroup tile h (in the XCD-aware order) of batch slice z.
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
-__global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_kernel(StageArgs p) {
+__device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z) {
   using TR = PrecTraits<P>;
   using T = typename TR::T;
   constexpr int MT = TR::MT;
@@ -193,11 +194,10 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_kernel(StageArgs 
   const int gx = (p.M + BM - 1) / BM;
   const int gy = (p.N + BN - 1) / BN;
   int bx, by;
-  tile_of(blockIdx.x, gx, gy, bx, by);
+  tile_of(h, gx, gy, bx, by);
   const int m0 = bx * BM;
   const int mc0 = m0 / 2;
   const int n0 = by * BN;
-  const int z = blockIdx.z;
   const int M = p.M, N = p.N, K = p.K;
   const int nk = (K + BK - 1) / BK;
 
@@ -634,6 +634,31 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_kernel(StageArgs 
       }
     }
   }
+}
+
+template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
+__global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_kernel(StageArgs p) {
+  stage_body<P, BM, BN, BK, WM, WN, WK, BMODE, PF>(p, blockIdx.x, blockIdx.z);
+}
+
+// Grouped launch: one workgroup per tile of every item of a table of stage arguments (items may
+// differ in matrices, sizes and batch).  Item i owns workgroups tile_start[i] .. tile_start[i+1]-1,
+// in (batch slice, tile) order.
+template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
+__global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_grouped_kernel(const StageArgs* __restrict__ table,
+                                                                              const int64_t* __restrict__ tile_start,
+                                                                              int n_items) {
+  const int64_t b = blockIdx.x;
+  int lo = 0, hi = n_items - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const StageArgs p = table[lo];
+  const int64_t tiles_xy = (int64_t)((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
+  const int64_t local = b - tile_start[lo];
+  stage_body<P, BM, BN, BK, WM, WN, WK, BMODE, PF>(p, (int)(local % tiles_xy), (int)(local / tiles_xy));
 }
 
 // Split-FP16 stage kernel with the constant operand kept compact.
@@ -1283,6 +1308,62 @@ static int launch_prec(const StageArgs& a, int nz, hipStream_t s) {
     if (cfg == k64x128) return launch_stage_t<P, 64, 128, BK / 2, 2, 2, 1, BMODE, PF>(a, nz, s);
   }
   set_last_error("tile configuration %d is not available for this precision", cfg);
+  return kInvalidValue;
+}
+
+template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
+static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, int n, int64_t tiles, hipStream_t s) {
+  using T = typename PrecTraits<P>::T;
+  constexpr int VEC = 16 / (int)sizeof(T);
+  constexpr int NARR = PrecTraits<P>::SPLIT ? 2 : 1;
+  constexpr size_t lds = 2 * (size_t)NARR * (BM + BN) * (BK + VEC) * sizeof(T);
+  auto kern = mft_stage_grouped_kernel<P, BM, BN, BK, WM, WN, WK, BMODE, PF>;
+  static bool configured = false;
+  if (!configured) {
+    HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    configured = true;
+  }
+  HCIPY_REQUIRE(tiles > 0 && tiles <= INT32_MAX, "grouped launch of %lld workgroups", (long long)tiles);
+  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * WM * WN * WK), lds, s, table, tile_start, n);
+  return check_hip(hipGetLastError(), "mft_stage_grouped_kernel launch");
+}
+
+template <int P, int BK, int BMODE, int PF>
+static int launch_grouped_prec(int stage, const StageArgs* table, const int64_t* tile_start, int n, int64_t tiles,
+                               hipStream_t s) {
+  if constexpr (P == kF64) {
+    if (stage == 0) return launch_grouped_t<P, 64, 64, BK, 2, 2, 1, BMODE, PF>(table, tile_start, n, tiles, s);
+    return launch_grouped_t<P, 64, 32, BK, 2, 1, 2, BMODE, PF>(table, tile_start, n, tiles, s);
+  } else {
+    if (stage == 0) return launch_grouped_t<P, 64, 64, BK, 2, 2, 2, BMODE, PF>(table, tile_start, n, tiles, s);
+    return launch_grouped_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(table, tile_start, n, tiles, s);
+  }
+}
+
+void grouped_tile_shape(int stage, int* bm, int* bn) {
+  *bm = 64;
+  *bn = stage == 0 ? 64 : 32;
+}
+
+int launch_stage_grouped(int prec, int stage, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
+                         int64_t tiles, hipStream_t s) {
+  if (n <= 0 || tiles <= 0) return kOk;
+  if (prec == kF64) {
+    if (b_mode == kRowsPadded) return launch_grouped_prec<kF64, 32, kRowsPadded, 4>(stage, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsAligned) return launch_grouped_prec<kF64, 32, kRowsAligned, 4>(stage, table, tile_start, n, tiles, s);
+    return launch_grouped_prec<kF64, 32, kRowsUnaligned, 4>(stage, table, tile_start, n, tiles, s);
+  }
+  if (prec == kF32) {
+    if (b_mode == kRowsPadded) return launch_grouped_prec<kF32, 64, kRowsPadded, 2>(stage, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsAligned) return launch_grouped_prec<kF32, 64, kRowsAligned, 2>(stage, table, tile_start, n, tiles, s);
+    return launch_grouped_prec<kF32, 64, kRowsUnaligned, 2>(stage, table, tile_start, n, tiles, s);
+  }
+  if (prec == kF16x2) {
+    HCIPY_REQUIRE(b_mode == kRowsPadded, "split-FP16 stage requires a padded data operand");
+    return launch_grouped_prec<kF16x2, 128, kRowsPadded, 2>(stage, table, tile_start, n, tiles, s);
+  }
+  set_last_error("unknown precision %d", prec);
   return kInvalidValue;
 }
 

@@ -18,7 +18,7 @@ from .config import Configuration
 from .field import CartesianGrid, Field, RegularCoords, field_dtype
 
 __all__ = [
-    'ComputationalComplexity', 'FourierTransform', 'MatrixFourierTransform', 'make_fft_grid',
+    'ComputationalComplexity', 'FourierTransform', 'MatrixFourierTransform', 'MftGroup', 'make_fft_grid',
     'get_fft_parameters', 'is_fft_grid', 'complex_dtype_for',
 ]
 
@@ -218,6 +218,78 @@ class _MftPlan:
     def close(self):
         if getattr(self, 'handle', None) is not None and _lib._lib is not None:
             _lib.load().hcipy_mft_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MftGroup:
+    """Grouped execution of many matrix Fourier transforms as one launch per contraction stage
+    (``hcipy_mft_execute_grouped``).  The calls may use different transforms (for example one
+    per wavelength, or pupils of different sizes), directions and batch sizes, but one precision.
+    Each item gets its own workspace, held by the group.  The transforms must keep their
+    matrices (``precompute_matrices=True``, the default)."""
+
+    def __init__(self, capacity=64, device=None):
+        self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
+        self.capacity = int(capacity)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.load().hcipy_mft_group_create(ctypes.byref(h), self.capacity), 'hcipy_mft_group_create')
+        self.handle = h
+        self._ws = []
+
+    def execute(self, calls):
+        """``calls``: sequence of ``(mft, direction, field, alpha, output)`` with direction
+        ``_lib.FORWARD``/``_lib.BACKWARD``, a complex field tensor or Field (leading dimensions
+        = batch), a complex scale and an ``_lib.OUTPUT_*`` kind.  Returns the output tensors."""
+        n = len(calls)
+        if n == 0:
+            return []
+        if n > self.capacity:
+            raise ValueError('%d calls for a group of capacity %d' % (n, self.capacity))
+        items = (_lib.MftGroupItem * n)()
+        outs, keep = [], []
+        precs = set()
+        while len(self._ws) < n:
+            self._ws.append(None)
+        for i, (mft, direction, field, alpha, output) in enumerate(calls):
+            in_grid = mft.input_grid if direction == _lib.FORWARD else mft.output_grid
+            out_grid = mft.output_grid if direction == _lib.FORWARD else mft.input_grid
+            t, cdt = _as_field_tensor(field, in_grid)
+            if t.device != self.device:
+                raise ValueError('call %d: field on %s, group on %s' % (i, t.device, self.device))
+            lead = tuple(t.shape[:-1])
+            batch = int(np.prod(lead)) if lead else 1
+            plan = mft.plan_for(cdt, t.device)
+            precs.add(plan.precision)
+            odt = {_lib.OUTPUT_FIELD: cdt, _lib.OUTPUT_FIELD_C128: torch.complex128,
+                   _lib.OUTPUT_INTENSITY: torch.float64 if cdt == torch.complex128 else torch.float32,
+                   _lib.OUTPUT_INTENSITY_F64: torch.float64}[output]
+            out = torch.empty(lead + (out_grid.size,), dtype=odt, device=t.device)
+            need = max(plan.workspace_bytes(batch), 256)
+            if self._ws[i] is None or self._ws[i].numel() < need:
+                self._ws[i] = torch.empty(need, dtype=torch.uint8, device=t.device)
+            ws = self._ws[i]
+            a = complex(alpha)
+            items[i] = _lib.MftGroupItem(plan.handle.value, direction, _lib.dptr(t), _lib.dptr(out), batch, a.real,
+                                         a.imag, output, _lib.dptr(ws), ws.numel())
+            outs.append(out)
+            keep.append(t)
+        if len(precs) != 1:
+            raise ValueError('all calls of a grouped launch share one precision, got %s' % sorted(precs))
+        st = _lib.load().hcipy_mft_execute_grouped(self.handle, ctypes.cast(items, ctypes.c_void_p), n,
+                                                   _lib.stream_ptr(self.device))
+        _lib.check(st, 'hcipy_mft_execute_grouped')
+        return outs
+
+    def close(self):
+        if getattr(self, 'handle', None) is not None and _lib._lib is not None:
+            _lib.load().hcipy_mft_group_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

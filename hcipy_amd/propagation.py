@@ -7,7 +7,7 @@ import numpy as np
 
 from . import _lib
 from .field import Field
-from .fourier import MatrixFourierTransform
+from .fourier import MatrixFourierTransform, MftGroup
 from .optics import AgnosticOpticalElement, Wavefront
 
 __all__ = ['FraunhoferPropagator']
@@ -61,6 +61,29 @@ class FraunhoferPropagator(AgnosticOpticalElement):
         d = self.instance(wavefront.wavelength)
         U = d.fourier_transform._apply(wavefront.electric_field, _lib.BACKWARD, extra_alpha=1 / d.norm_factor)
         return Wavefront(Field(U, self._input_grid), wavefront.wavelength)
+
+    def forward_multiple(self, wavefronts, output='field', group=None):
+        """Forward propagation of several wavefronts (for example the wavelengths of a broadband
+        source, each with its own Fourier matrices) as one grouped launch per MFT stage.
+        ``output`` is 'field' (complex, like ``forward``) or 'intensity' (fused |E|^2, like
+        ``forward_intensity``).  Returns a list of Wavefronts or intensity Fields."""
+        wavefronts = list(wavefronts)
+        if np.size(self._input_grid.weights) != 1:  # non-scalar weights: one call per wavefront
+            if output == 'field':
+                return [self.forward(wf) for wf in wavefronts]
+            return [self.forward_intensity(wf) for wf in wavefronts]
+        if group is None:
+            group = self._group = getattr(self, '_group', None) or MftGroup(max(64, len(wavefronts)))
+        kind = {'field': _lib.OUTPUT_FIELD, 'intensity': _lib.OUTPUT_INTENSITY}[output]
+        calls = []
+        for wf in wavefronts:
+            d = self.instance(wf.wavelength)
+            w = complex(np.asarray(self._input_grid.weights).ravel()[0])
+            calls.append((d.fourier_transform, _lib.FORWARD, wf.electric_field, w * d.norm_factor, kind))
+        outs = group.execute(calls)
+        if output == 'field':
+            return [Wavefront(Field(o, self._output_grid), wf.wavelength) for o, wf in zip(outs, wavefronts)]
+        return [Field(o, self._output_grid) for o in outs]
 
     def forward_intensity(self, wavefront, float64=False):
         """``forward(wavefront).intensity`` in one pass: |E|^2 is evaluated in the MFT epilogue

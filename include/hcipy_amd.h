@@ -111,6 +111,33 @@ int hcipy_mft_autotune(hcipy_mft_plan plan, int direction, const void* in, void*
 int hcipy_mft_execute(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch,
                       double alpha_re, double alpha_im, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Grouped execution: many transforms, with different plans (matrices, sizes, e.g. one per
+ * wavelength of a broadband source or per pupil size) and batches, as ONE launch per contraction
+ * stage (plus the per-item data split of split-FP16 plans).  Replaces the reference's loop over
+ * wavelengths / instances of MatrixFourierTransform.forward and .backward
+ * (hcipy/fourier/matrix_fourier_transform.py:132-224 called per AgnosticOpticalElement instance,
+ * hcipy/optics/optical_element.py:297-380); the stage kernels are the same as hcipy_mft_execute's
+ * (fixed tile shapes: 64x64 for stage 1, 64x32 for stage 2).  All items share one precision and
+ * live on the group's device; every item has its own workspace (hcipy_mft_workspace_size).
+ * A group owns a device table of stage arguments and a pinned host mirror (no allocation in
+ * hcipy_mft_execute_grouped).  Calls on one group are ordered on one stream; a captured grouped
+ * call replays the table of its capture, so a group used in a graph serves that graph only. */
+typedef struct hcipy_mft_group_s* hcipy_mft_group;
+typedef struct {
+  hcipy_mft_plan plan;
+  int direction;             /* HCIPY_FORWARD / HCIPY_BACKWARD */
+  const void* in;
+  void* out;
+  int64_t batch;
+  double alpha_re, alpha_im;
+  int output;                /* HCIPY_OUTPUT_* */
+  void* workspace;
+  size_t workspace_bytes;
+} hcipy_mft_group_item;
+int hcipy_mft_group_create(hcipy_mft_group* group, int capacity);
+int hcipy_mft_group_destroy(hcipy_mft_group group);
+int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item* items, int n_items, void* stream);
+
 /* Output kinds of hcipy_mft_execute_ex. */
 #define HCIPY_OUTPUT_FIELD 0         /* complex field in the plan's dtype (complex128 / complex64) */
 #define HCIPY_OUTPUT_FIELD_C128 1    /* complex128 field from any precision                       */
