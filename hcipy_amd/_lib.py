@@ -36,7 +36,11 @@ TILES_C64X64_W4 = 8
 TILES_C64X64_W8 = 9
 TILES_C64X32_W4 = 10
 TILES_C128X32_W8 = 11
-TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(12))}
+TILES_D64X64_W4 = 12
+TILES_D64X64_W8 = 13
+TILES_D64X32_W4 = 14
+TILES_D64X32_W8 = 15
+TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(16))}
 
 
 class MftGroupItem(ctypes.Structure):

@@ -1075,6 +1075,286 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_f16x2_kernel(StageArgs 
   }
 }
 
+// LDS DMA issue and wait, written as asm so that the compiler's wait-count pass, which cannot
+// tell the LDS stages apart and would wait for every outstanding DMA before any LDS read, leaves
+// the pipeline alone.  The DMA writes no VGPR (its data goes to LDS at M0 + 16 * lane); its
+// address VGPRs are read at issue.  Kernels using these issue no other vector-memory loads while
+// a DMA is outstanding and drain with dma_wait<0>() before any.
+// `lds_addr` must be uniform across the wave (it goes to M0).
+__device__ __forceinline__ void dma_load16(const void* gaddr, unsigned lds_addr) {
+  const unsigned m = __builtin_amdgcn_readfirstlane(lds_addr);
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(gaddr), "s"(m) : "memory", "m0");
+}
+template <int N>
+__device__ __forceinline__ void dma_wait() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("s_waitcnt vmcnt(%0)" : : "n"(N) : "memory");
+}
+
+// Split-FP16 stage kernel fed by LDS DMA.
+//
+// Same arithmetic and output as mft_f16x2_kernel (compact constant, Re / Im MFMA tiles formed in
+// registers), but the operand tiles go from HBM/L2 straight into LDS with global_load_lds_dwordx4:
+// no VGPR staging and no LDS store instructions.  Three LDS stages (tile t computed while tiles
+// t+1 and t+2 are in flight); each row of a stage holds the 16 16-byte chunks of one k-tile with
+// the chunk index XOR-ed by (row mod 16), so that the 32 rows a fragment read touches fall in
+// distinct banks.  A DMA wave instruction fills one contiguous KiB of LDS (lane i -> bytes 16 i),
+// so the swizzle is applied by choosing each lane's source chunk.  The compiler does not track
+// LDS DMA, so the waits are explicit: before computing tile t every wave waits until only its
+// loads of tile t+1 are outstanding, then one barrier makes tile t visible and frees the stage
+// of tile t-1 for the loads of tile t+2.  Requires K to be a multiple of the 128-element k-tile.
+template <int WN, int WK>
+__global__ __launch_bounds__(64 * WN * WK) void mft_f16x2_dma_kernel(StageArgs p) {
+  using T = _Float16;
+  constexpr int NTHR = 64 * WN * WK;
+  constexpr int NW = NTHR / 64;
+  constexpr int BMC = 32;                    // compact constant rows per tile
+  constexpr int BN = 32 * WN;                // data rows per tile
+  constexpr int BK = 128;
+  constexpr int CPR = 16;                    // 16-byte chunks per row per k-tile
+  constexpr int A_CH = BMC * CPR, B_CH = BN * CPR;
+  constexpr int INSTR = 2 * (A_CH + B_CH) / 64;  // DMA wave instructions per k-tile
+  static_assert(INSTR % NW == 0, "DMA instructions must split evenly over the waves");
+  constexpr int IPW = INSTR / NW;
+  constexpr int OFF_ALO = A_CH * 16, OFF_BHI = 2 * A_CH * 16, OFF_BLO = OFF_BHI + B_CH * 16;
+  constexpr int STAGE = 2 * (A_CH + B_CH) * 16;
+  constexpr int NS = 3;
+  constexpr int KV_PER_WAVE = (BK / 16) / WK;
+  constexpr int KPW = KV_PER_WAVE * 16;
+  constexpr int FE = (KPW >= kFlushK) ? 1 : kFlushK / KPW;
+  static_assert(KV_PER_WAVE >= 1 && (BK / 16) % WK == 0, "k-vectors must split evenly over WK");
+
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[NS * STAGE];
+  __shared__ float s_t[BN];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wk = wave / WN;
+  const int wn = wave % WN;
+  const int Mc = p.M / 2;
+  const int gx = (Mc + BMC - 1) / BMC;
+  const int gy = (p.N + BN - 1) / BN;
+  int bx, by;
+  tile_of(blockIdx.x, gx, gy, bx, by);
+  const int c0 = bx * BMC;
+  const int n0 = by * BN;
+  const int z = blockIdx.z;
+  const int N = p.N;
+  const int nk = p.K / BK;
+  const T* Ag[2] = {static_cast<const T*>(p.a), static_cast<const T*>(p.a_lo)};
+  const T* Bg[2] = {static_cast<const T*>(p.b) + (int64_t)z * p.b_zstride,
+                    static_cast<const T*>(p.b_lo) + (int64_t)z * p.b_zstride};
+  const bool split_out = (p.out_mode == kOutSplit);
+  int item0 = 0;
+
+  // Split output: per folded item scale t = min(row_scale over the item) * 2^-ceil(log2 K).
+  if (split_out) {
+    __shared__ float s_red[NTHR / 64];
+    const int nend = min(n0 + BN, N);
+    item0 = n0 / p.fold;
+    const int nitems = (nend - 1) / p.fold - item0 + 1;
+    for (int it = 0; it < nitems; ++it) {
+      const float* rs = p.row_scale + (int64_t)(item0 + it) * p.fold;
+      float mn = FLT_MAX;
+      int r = tid;
+      for (; r + 3 * NTHR < p.fold; r += 4 * NTHR)
+        mn = fminf(mn, fminf(fminf(rs[r], rs[r + NTHR]), fminf(rs[r + 2 * NTHR], rs[r + 3 * NTHR])));
+      for (; r < p.fold; r += NTHR) mn = fminf(mn, rs[r]);
+      mn = wave_min(mn);
+      if (lane == 0) s_red[wave] = mn;
+      __syncthreads();
+      if (tid == 0) {
+        float m = s_red[0];
+#pragma unroll
+        for (int w = 1; w < NTHR / 64; ++w) m = fminf(m, s_red[w]);
+        const float tt = ldexpf(m, -p.t_log2k);
+        s_t[it] = tt;
+        if (bx == 0 && p.t_out != nullptr) p.t_out[item0 + it] = tt;
+      }
+      __syncthreads();
+    }
+  }
+
+  // DMA sources of this wave's instructions: instruction j = wave + u * NW covers LDS chunks
+  // 64 j .. 64 j + 63 of a stage (one of the four arrays), lane i the chunk 64 j + i.
+  const T* src[IPW];
+  int dst[IPW];
+#pragma unroll
+  for (int u = 0; u < IPW; ++u) {
+    const int j = wave + u * NW;
+    int q = j * 64 + lane;
+    int arr;
+    if (q < A_CH) {
+      arr = 0;
+    } else if (q < 2 * A_CH) {
+      arr = 1;
+      q -= A_CH;
+    } else if (q < 2 * A_CH + B_CH) {
+      arr = 2;
+      q -= 2 * A_CH;
+    } else {
+      arr = 3;
+      q -= 2 * A_CH + B_CH;
+    }
+    const int row = q / CPR;
+    const int c = (q % CPR) ^ (row & 15);
+    if (arr < 2) src[u] = Ag[arr] + (int64_t)(c0 + row) * p.lda + c * 8;
+    else src[u] = Bg[arr - 2] + (int64_t)min(n0 + row, N - 1) * p.ldb + c * 8;
+    dst[u] = j * 1024;
+  }
+  const unsigned lds_base = (unsigned)reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) unsigned char*)lds);
+  auto issue = [&](int tile, int s) __attribute__((always_inline)) {
+    const int k0 = min(tile, nk - 1) * BK;
+#pragma unroll
+    for (int u = 0; u < IPW; ++u) dma_load16(src[u] + k0, lds_base + (unsigned)(s * STAGE + dst[u]));
+  };
+
+  floatx16_t acc[2], accx[2];  // [Re, Im]
+  double acc64[2][16];
+#pragma unroll
+  for (int e = 0; e < 2; ++e)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      acc[e][r] = 0.0f;
+      accx[e][r] = 0.0f;
+      acc64[e][r] = 0.0;
+    }
+
+  const int arow = lane & 31;
+  const int brow = wn * 32 + (lane & 31);
+  auto frag = [&](const unsigned char* st, int off, int row, int c) __attribute__((always_inline)) {
+    return *reinterpret_cast<const half8_t*>(st + off + (row * CPR + (c ^ (row & 15))) * 16);
+  };
+  auto compute = [&](const unsigned char* st) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < KV_PER_WAVE; ++q) {
+      const int c = 2 * (q * WK + wk) + (lane >> 5);
+      const half8_t ah = frag(st, 0, arow, c), al = frag(st, OFF_ALO, arow, c);
+      const half8_t bh = frag(st, OFF_BHI, brow, c), bl = frag(st, OFF_BLO, brow, c);
+      const u32x4_t vh = __builtin_bit_cast(u32x4_t, ah), vl = __builtin_bit_cast(u32x4_t, al);
+      const u32x4_t sgn = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
+      const half8_t aeh = __builtin_bit_cast(half8_t, vh ^ sgn), ael = __builtin_bit_cast(half8_t, vl ^ sgn);
+      const half8_t aoh = __builtin_bit_cast(half8_t, (vh >> 16) | (vh << 16));
+      const half8_t aol = __builtin_bit_cast(half8_t, (vl >> 16) | (vl << 16));
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aeh, bh, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aoh, bh, acc[1], 0, 0, 0);
+      accx[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aeh, bl, accx[0], 0, 0, 0);
+      accx[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aoh, bl, accx[1], 0, 0, 0);
+      accx[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ael, bh, accx[0], 0, 0, 0);
+      accx[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aol, bh, accx[1], 0, 0, 0);
+    }
+  };
+  auto flush = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        acc64[e][r] += (double)acc[e][r];
+        acc[e][r] = 0.0f;
+      }
+  };
+  auto step = [&](int kb, auto dc) __attribute__((always_inline)) {
+    constexpr int D = decltype(dc)::value;  // kb is a multiple of NS: tile t is in stage D
+    const int t = kb + D;
+    if (t >= nk) return;
+    dma_wait<IPW>();  // own loads of tile t done (those of tile t + 1 may be in flight)
+    __syncthreads();    // everyone's loads of tile t done; stage of tile t - 1 no longer read
+    issue(t + 2, (D + 2) % NS);
+    compute(lds + D * STAGE);
+    if (FE == 1 || (t + 1) % FE == 0) flush();
+  };
+
+  issue(0, 0);
+  issue(1, 1);
+  for (int kb = 0; kb < nk; kb += NS) {
+    step(kb, std::integral_constant<int, 0>{});
+    step(kb, std::integral_constant<int, 1>{});
+    step(kb, std::integral_constant<int, 2>{});
+  }
+  dma_wait<0>();
+  __syncthreads();
+  if constexpr (FE > 1) flush();
+#pragma unroll
+  for (int e = 0; e < 2; ++e)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc64[e][r] += (double)accx[e][r] * kLoScale;
+
+  // Sum the WK partial results through LDS, one k-group per round.
+  if constexpr (WK > 1) {
+    static_assert((size_t)WN * 64 * 32 * sizeof(double) <= (size_t)NS * STAGE, "partial sums fit in LDS");
+    double* red = reinterpret_cast<double*>(lds) + (wn * 64 + lane) * 32;
+#pragma unroll 1
+    for (int g = 1; g < WK; ++g) {
+      if (wk == g) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) red[e * 16 + r] = acc64[e][r];
+      }
+      __syncthreads();
+      if (wk == 0) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc64[e][r] += red[e * 16 + r];
+      }
+      __syncthreads();
+    }
+    if (wk > 0) return;
+  }
+
+  // Epilogue (same output formats as the other stage kernels).
+  const int n = n0 + wn * 32 + (lane & 31);
+  if (n >= N) return;
+  const double inv_z = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[z] : 1.0;
+  double sc = inv_z;
+  if (p.row_scale != nullptr) sc /= (double)p.row_scale[n];
+  int64_t base, cstride;
+  float tt = 1.0f;
+  if (p.fold > 0) {
+    const int item = n / p.fold;
+    base = (int64_t)item * p.fold_stride + (n - item * p.fold);
+    cstride = p.ldo;
+    if (split_out) tt = s_t[item - item0];
+  } else if (p.nc_layout) {
+    base = (int64_t)z * p.out_zstride + (int64_t)n * p.ldo;
+    cstride = 1;
+  } else {
+    base = (int64_t)z * p.out_zstride + n;
+    cstride = p.ldo;
+  }
+  const double ar = p.alpha_re * sc, ai = p.alpha_im * sc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int c = c0 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+    if (c >= Mc) continue;
+    const double zr = acc64[0][r], zi = acc64[1][r];
+    const double orr = zr * ar - zi * ai;
+    const double oii = zr * ai + zi * ar;
+    const int64_t idx = base + (int64_t)c * cstride;
+    if (p.out_mode == kOutC128) {
+      reinterpret_cast<double2*>(p.out)[idx] = make_double2(orr, oii);
+    } else if (p.out_mode == kOutC64) {
+      reinterpret_cast<float2*>(p.out)[idx] = make_float2((float)orr, (float)oii);
+    } else if (p.out_mode == kOutI32) {
+      reinterpret_cast<float*>(p.out)[idx] = (float)(orr * orr + oii * oii);
+    } else if (p.out_mode == kOutI64) {
+      reinterpret_cast<double*>(p.out)[idx] = orr * orr + oii * oii;
+    } else {
+      const double xr = orr * (double)tt;
+      const double xi = oii * (double)tt;
+      const _Float16 hr = f32_to_f16((float)xr);
+      const _Float16 hi = f32_to_f16((float)xi);
+      half2_t h = {hr, hi};
+      half2_t l = {f32_to_f16((float)((xr - (double)hr) * (double)kLoInv)),
+                   f32_to_f16((float)((xi - (double)hi) * (double)kLoInv))};
+      reinterpret_cast<half2_t*>(p.out)[idx] = h;
+      reinterpret_cast<half2_t*>(p.out_lo)[idx] = l;
+    }
+  }
+}
+
 // Splits rows of interleaved complex64 data into FP16 hi/lo arrays, with a power-of-two row scale
 // s = 2^-e chosen so that max|x| * s < 1.  One workgroup per row, 8 consecutive floats per thread
 // (two 16-byte loads, two 16-byte stores); row padding up to ld_out is zero-filled.
@@ -1266,6 +1546,15 @@ static int launch_f16x2_t(const StageArgs& a, int nz, hipStream_t s) {
   return kOk;
 }
 
+template <int WN, int WK>
+static int launch_f16x2_dma_t(const StageArgs& a, int nz, hipStream_t s) {
+  const int gx = (int)ceil_div(a.M / 2, 32);
+  const int gy = (int)ceil_div(a.N, 32 * WN);
+  hipLaunchKernelGGL((mft_f16x2_dma_kernel<WN, WK>), dim3((unsigned)(gx * gy), 1, (unsigned)nz), dim3(64 * WN * WK), 0,
+                     s, a);
+  return kOk;
+}
+
 bool tile_cfg_supported(int prec, int cfg) {
   if (cfg < 0 || cfg >= kNumTileCfg) return false;
   if (cfg >= kC64x64_w4) return prec == kF16x2;  // compact-constant shapes: split-FP16 only
@@ -1299,6 +1588,15 @@ static int launch_prec(const StageArgs& a, int nz, hipStream_t s) {
     if (cfg == kC64x64_w8) return launch_f16x2_t<32, 64, BK, 1, 2, 4>(a, nz, s);
     if (cfg == kC64x32_w4) return launch_f16x2_t<32, 32, BK, 1, 1, 4>(a, nz, s);
     if (cfg == kC128x32_w8) return launch_f16x2_t<64, 32, BK, 2, 1, 4>(a, nz, s);
+    if (cfg >= kD64x64_w4 && cfg <= kD64x32_w8) {
+      if (a.K % 128 != 0)  // the DMA kernels need whole k-tiles: same arithmetic, compact kernel
+        return cfg <= kD64x64_w8 ? launch_f16x2_t<32, 64, BK, 1, 2, 2>(a, nz, s)
+                                 : launch_f16x2_t<32, 32, BK, 1, 1, 4>(a, nz, s);
+      if (cfg == kD64x64_w4) return launch_f16x2_dma_t<2, 2>(a, nz, s);
+      if (cfg == kD64x64_w8) return launch_f16x2_dma_t<2, 4>(a, nz, s);
+      if (cfg == kD64x32_w4) return launch_f16x2_dma_t<1, 4>(a, nz, s);
+      return launch_f16x2_dma_t<1, 8>(a, nz, s);
+    }
   }
   if constexpr (P != kF64) {
     if (cfg == k64x32_w8) return launch_stage_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(a, nz, s);

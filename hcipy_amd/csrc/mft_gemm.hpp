@@ -121,6 +121,9 @@ struct StageArgs {
 //   kC64x64_w8  : 32 x 64 compact tile, 8 waves: 1 x 2 waves, k-vectors split over 4
 //   kC64x32_w4  : 32 x 32 compact tile, 4 waves, k-vectors split over 4
 //   kC128x32_w8 : 64 x 32 compact tile, 8 waves: 2 x 1 waves, k-vectors split over 4
+// Split-FP16 only, compact constant, operands loaded straight into LDS (three LDS stages):
+//   kD64x64_w4 / kD64x64_w8 : 32 x 64 compact tile, 4 / 8 waves (k-vectors split over 2 / 4)
+//   kD64x32_w4 / kD64x32_w8 : 32 x 32 compact tile, 4 / 8 waves (k-vectors split over 4 / 8)
 enum TileCfg : int {
   kTileAuto = -1,
   k64x64_w2x2 = 0,
@@ -135,7 +138,11 @@ enum TileCfg : int {
   kC64x64_w8 = 9,
   kC64x32_w4 = 10,
   kC128x32_w8 = 11,
-  kNumTileCfg = 12,
+  kD64x64_w4 = 12,
+  kD64x64_w8 = 13,
+  kD64x32_w4 = 14,
+  kD64x32_w8 = 15,
+  kNumTileCfg = 16,
 };
 
 // True if `cfg` is instantiated for precision `prec`.

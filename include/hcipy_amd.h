@@ -86,6 +86,12 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
 #define HCIPY_TILES_C64X64_W8 9  /* 32x64 compact tile, 8 waves, k-vectors split over 4 */
 #define HCIPY_TILES_C64X32_W4 10 /* 32x32 compact tile, 4 waves, k-vectors split over 4 */
 #define HCIPY_TILES_C128X32_W8 11 /* 64x32 compact tile, 8 waves, k-vectors split over 4 */
+/* F16X2 only: compact constant with the operand tiles loaded straight into LDS (three stages);
+ * for contraction lengths that are not whole 128-element k-tiles the compact kernel runs. */
+#define HCIPY_TILES_D64X64_W4 12 /* 32x64 compact tile, 4 waves */
+#define HCIPY_TILES_D64X64_W8 13 /* 32x64 compact tile, 8 waves */
+#define HCIPY_TILES_D64X32_W4 14 /* 32x32 compact tile, 4 waves */
+#define HCIPY_TILES_D64X32_W8 15 /* 32x32 compact tile, 8 waves */
 int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction, int stage, int tile_cfg);
 int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction, int stage, int* tile_cfg);
 

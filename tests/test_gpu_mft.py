@@ -252,3 +252,27 @@ def test_autotune_pins_supported_shapes_and_keeps_results():
     F = host(mft.forward(ha.Field(dev(f), gi)))
     b = host(mft.backward(ha.Field(dev(F.astype(np.complex64)), go)))
     assert peak_rel_field(b, o.mft_backward(F.astype(np.complex128), og(gi), og(go))) < 5e-7
+
+
+@pytest.mark.parametrize('dims_in,dims_out', [([128, 64], [64, 64]), ([256, 128], [64, 128]), ([96, 64], [70, 50])])
+def test_lds_dma_shapes_match_oracle(dims_in, dims_out):
+    """The LDS-DMA split-FP16 kernels (contraction lengths in whole 128-element k-tiles; other
+    lengths take the compact kernel) against the oracle, both directions, batch 2."""
+    from hcipy_amd import _lib
+    gi = ha.make_uniform_grid(dims_in, [1.0, 0.9]).shifted([0.01, -0.02])
+    go = ha.make_uniform_grid(dims_out, [30.0, 25.0]).shifted([0.4, -0.3])
+    f = _rand(gi.size, 13, (2,)).astype(np.complex64)
+    F = _rand(go.size, 14, (2,)).astype(np.complex64)
+    ref_f = o.mft_forward(f.astype(np.complex128), og(gi), og(go))
+    ref_b = o.mft_backward(F.astype(np.complex128), og(gi), og(go))
+    mft = ha.MatrixFourierTransform(gi, go, precision='f16x2')
+    plan = mft.plan_for(torch.complex64, torch.device('cuda', 0))
+    for cfg in (_lib.TILES_D64X64_W4, _lib.TILES_D64X64_W8, _lib.TILES_D64X32_W4, _lib.TILES_D64X32_W8):
+        for d in (_lib.FORWARD, _lib.BACKWARD):
+            for stage in (0, 1):
+                plan.set_tiles(d, stage, cfg)
+        say('lds dma', dims_in, dims_out, cfg)
+        a = host(mft.forward(ha.Field(dev(f), gi)))
+        b = host(mft.backward(ha.Field(dev(F), go)))
+        assert peak_rel_field(a, ref_f) < 5e-7, cfg
+        assert peak_rel_field(b, ref_b) < 5e-7, cfg
