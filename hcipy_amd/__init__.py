@@ -13,7 +13,7 @@ from .fourier import (ComputationalComplexity, FastFourierTransform, FourierTran
 from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, OpticalSystem, PhaseApodizer, Wavefront
 from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, FresnelPropagator
 from .fourier_filter import FourierFilter, NearFieldTransferFunction
-from .coronagraphy import LyotCoronagraph
+from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VortexCoronagraph
 from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
 from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, make_actuator_positions,
                                 make_gaussian_influence_functions)

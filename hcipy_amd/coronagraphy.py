@@ -1,10 +1,20 @@
-"""Lyot coronagraph in Babinet form (hcipy/coronagraphy/lyot.py:7-122)."""
-from . import pointwise
-from .field import Field
+"""Coronagraphs: the Lyot coronagraph in Babinet form (hcipy/coronagraphy/lyot.py:7-122) and the
+multi-scale focal-mask coronagraph with its vortex specialisation
+(hcipy/coronagraphy/multi_scale.py:10-182, hcipy/coronagraphy/vortex.py:12-15)."""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, pointwise
+from .aperture import make_circular_aperture
+from .field import Field, make_focal_grid
+from .fourier import FastFourierTransform, MatrixFourierTransform, MftGroup
+from .fourier_filter import FourierFilter
 from .optics import Apodizer, OpticalElement, Wavefront
 from .propagation import FraunhoferPropagator
 
-__all__ = ['LyotCoronagraph']
+__all__ = ['LyotCoronagraph', 'MultiScaleCoronagraph', 'VortexCoronagraph', 'periodic_tukey_window']
 
 
 class LyotCoronagraph(OpticalElement):
@@ -62,3 +72,142 @@ class LyotCoronagraph(OpticalElement):
         wf_foc = Wavefront(self._fpm_complement(wf_foc, True), wavefront.wavelength)
         pup = self.prop.backward(wf_foc)
         return Wavefront(pointwise.sub_mul(wf.electric_field, pup.electric_field), wavefront.wavelength)
+
+
+def periodic_tukey_window(n):
+    """Periodic Tukey window with taper fraction 1 (a periodic Hann window), i.e.
+    ``scipy.signal.windows.tukey(n, 1, False)``: ``0.5 - 0.5 cos(2 pi k / n)``, k = 0 .. n-1."""
+    k = np.arange(n, dtype=float)
+    return 0.5 - 0.5 * np.cos(2 * np.pi * k / n)
+
+
+class MultiScaleCoronagraph(OpticalElement):
+    """Focal-plane phase-mask coronagraph evaluated on nested focal grids
+    (hcipy/coronagraphy/multi_scale.py:10-182).
+
+    Level 0 samples the mask at q = 2 over the whole FFT field and runs as a FourierFilter with
+    the (windowed) level-0 mask as transfer function.  Level i samples the central
+    ``window_size`` pixels of level i - 1 at ``scaling_factor`` times the resolution; its mask is
+    windowed (all but the last level) and has the band-limited content of every coarser level
+    (FFT, then adjoint MFT onto its grid) subtracted, so that the levels add up to one mask
+    sampled finely near the centre.  The propagation runs at wavelength 1 (focal grids in
+    lambda/D):
+
+        L = FF_0(E) + sum_i P_i^dagger (M_i . P_i E),   then the Lyot stop,
+
+    with P_i the Fraunhofer propagator onto level i (conj(M_i) and the adjoint filter for
+    ``backward``).  Without gradients the level MFTs run as two grouped launches (all forward
+    transforms, then all adjoint transforms); the adjoint transforms are computed with a
+    negated scale so that each level is added by a fused subtract pass.
+    """
+
+    def __init__(self, input_grid, complex_mask, lyot_stop=None, q=1024, scaling_factor=4, window_size=32,
+                 precision=None):
+        self.input_grid = input_grid
+        pupil_diameter = np.asarray(input_grid.dims) * np.asarray(input_grid.delta)
+        if lyot_stop is not None and not isinstance(lyot_stop, OpticalElement):
+            lyot_stop = Apodizer(lyot_stop)
+        self.lyot_stop = lyot_stop
+
+        levels = int(math.ceil(math.log(q / 2) / math.log(scaling_factor))) + 1
+        qs = [2 * scaling_factor ** i for i in range(levels)]
+        num_airys = [tuple(n / 2 for n in input_grid.dims)]
+        for i in range(1, levels):
+            num_airys.append(tuple(n_airy * window_size / (2 * qs[i - 1] * n_airy) for n_airy in num_airys[i - 1]))
+
+        self.focal_grids = []
+        self.focal_masks = []
+        self.props = []
+        w1 = periodic_tukey_window(window_size)
+        w = np.outer(w1, w1)
+        for i in range(levels):
+            focal_grid = make_focal_grid(qs[i], num_airys[i], pupil_diameter=pupil_diameter, reference_wavelength=1,
+                                         focal_length=1)
+            mask = complex_mask(focal_grid)
+            mask = (mask.as_subclass(torch.Tensor) if isinstance(mask, torch.Tensor)
+                    else torch.as_tensor(np.asarray(mask))).to(torch.complex128)
+            dev = mask.device if mask.is_cuda else torch.device('cuda', torch.cuda.current_device())
+            mask = mask.to(dev)
+            if i != levels - 1:
+                # np.pad with one (before, after) pair built from the two axes, as the reference does
+                wp = np.pad(w, tuple((n_foc - n_w) // 2 for n_foc, n_w in zip(focal_grid.shape, w.shape)), 'constant')
+                mask = mask * torch.as_tensor(1 - wp.ravel(), device=dev)
+            for j in range(i):
+                fft = FastFourierTransform(self.focal_grids[j])
+                mft = MatrixFourierTransform(focal_grid, fft.output_grid)
+                corr = mft.backward(fft.forward(Field(self.focal_masks[j], self.focal_grids[j])))
+                mask = mask - corr.as_subclass(torch.Tensor)
+            if i == 0:
+                prop = FourierFilter(input_grid, Field(mask, focal_grid), qs[0])
+            else:
+                prop = FraunhoferPropagator(input_grid, focal_grid, precision=precision)
+            self.focal_grids.append(focal_grid)
+            self.focal_masks.append(Field(mask, focal_grid))
+            self.props.append(prop)
+        self._group = None
+
+    def _levels(self, E, conjugate):
+        """sum over levels >= 1 of P_i^dagger (M_i P_i E), accumulated onto the level-0 result."""
+        L = self.props[0].backward(E) if conjugate else self.props[0].forward(E)
+        rest = list(zip(self.focal_masks[1:], self.props[1:]))
+        if not rest:
+            return L
+        if pointwise.needs_grad(E):
+            for mask, prop in rest:
+                d = prop.instance(1)
+                F = d.fourier_transform._apply(E, _lib.FORWARD, extra_alpha=d.norm_factor)
+                F = pointwise.multiply(F, mask, conjugate=conjugate)
+                neg = d.fourier_transform._apply(F, _lib.BACKWARD, extra_alpha=-1 / d.norm_factor)
+                L = pointwise.sub_mul(L, neg)
+            return L
+        if self._group is None or self._group.device != E.device:
+            self._group = MftGroup(max(16, len(rest)), E.device)
+        w_in = complex(np.asarray(self.input_grid.weights).ravel()[0])
+        fwd = []
+        for mask, prop in rest:
+            d = prop.instance(1)
+            fwd.append((d.fourier_transform, _lib.FORWARD, E, w_in * d.norm_factor, _lib.OUTPUT_FIELD))
+        Fs = self._group.execute(fwd)
+        bwd = []
+        for F, (mask, prop) in zip(Fs, rest):
+            d = prop.instance(1)
+            F = pointwise.multiply(F, mask, conjugate=conjugate)
+            w_uv = complex(np.asarray(d.uv_grid.weights).ravel()[0]) / (2 * np.pi) ** 2
+            bwd.append((d.fourier_transform, _lib.BACKWARD, F, -w_uv / d.norm_factor, _lib.OUTPUT_FIELD))
+        for neg in self._group.execute(bwd):
+            L = pointwise.sub_mul(L, neg)
+        return L
+
+    def forward(self, wavefront):
+        """Pupil -> Lyot plane (multi_scale.py:108-145)."""
+        E = wavefront.electric_field
+        L = Field(self._levels(E, False), self.input_grid)
+        lyot = Wavefront(L, wavefront.wavelength)
+        if self.lyot_stop is not None:
+            lyot = self.lyot_stop.forward(lyot)
+        return lyot
+
+    def backward(self, wavefront):
+        """Lyot plane -> pupil: the stop's adjoint, then the levels with conj(M_i)
+        (multi_scale.py:147-182)."""
+        if self.lyot_stop is not None:
+            wavefront = self.lyot_stop.backward(wavefront)
+        E = wavefront.electric_field
+        return Wavefront(Field(self._levels(E, True), self.input_grid), wavefront.wavelength)
+
+
+class VortexCoronagraph(MultiScaleCoronagraph):
+    """Scalar vortex coronagraph of topological charge ``charge``: the multi-scale coronagraph
+    with the mask exp(i charge theta), its sample at the origin removed
+    (hcipy/coronagraphy/vortex.py:12-15)."""
+
+    def __init__(self, input_grid, charge, lyot_stop=None, q=1024, scaling_factor=4, window_size=32, precision=None):
+        self.charge = charge
+        hole = make_circular_aperture(1e-9)
+
+        def complex_mask(grid):
+            theta = grid.as_('polar').theta.as_subclass(torch.Tensor)
+            return torch.exp(1j * charge * theta) * (1 - hole(grid).as_subclass(torch.Tensor))
+
+        MultiScaleCoronagraph.__init__(self, input_grid, complex_mask, lyot_stop, q, scaling_factor, window_size,
+                                       precision)

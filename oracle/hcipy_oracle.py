@@ -550,6 +550,65 @@ def lyot_backward(E, pupil_grid, fpm_grid, fpm, lyot_stop=None, wavelength=1.0, 
     return wf - P
 
 
+def vortex_mask(charge):
+    """VortexCoronagraph's focal mask (hcipy/coronagraphy/vortex.py:12-15): exp(i l theta)
+    with the sample at the origin removed (times 1 - circ(1e-9))."""
+    def fn(grid):
+        x, y = grid.coords
+        return np.exp(1j * charge * np.arctan2(y, x)) * (1 - circular_aperture(grid, 1e-9))
+    return fn
+
+
+def multiscale_window(window_size, shape):
+    """Centred 2-D periodic Tukey(alpha = 1) window, padded as np.pad(w, ((n - w) // 2 per axis))
+    does (hcipy/coronagraphy/multi_scale.py:73-78)."""
+    from scipy.signal import windows
+    w1 = windows.tukey(window_size, 1, False)
+    w = np.outer(w1, w1)
+    return np.pad(w, tuple((n_foc - n_w) // 2 for n_foc, n_w in zip(shape, w.shape)), 'constant').ravel()
+
+
+def multiscale_masks(pupil_grid, complex_mask, q=1024, scaling_factor=4, window_size=32):
+    """MultiScaleCoronagraph.__init__ (hcipy/coronagraphy/multi_scale.py:44-106): the focal grids
+    of the levels and their corrected masks.  Level 0 is sampled at q = 2 over the full FFT
+    field; level i at q = 2 s^i over a window of ``window_size`` pixels of level i - 1.  Each
+    level's mask is windowed (all but the last) and has the band-limited content of every
+    coarser level, resampled onto its grid (FFT then adjoint MFT), subtracted."""
+    pupil_diameter = np.asarray(pupil_grid.dims) * pupil_grid.delta
+    levels = int(math.ceil(math.log(q / 2) / math.log(scaling_factor))) + 1
+    qs = [2 * scaling_factor ** i for i in range(levels)]
+    num_airys = [tuple(n / 2 for n in pupil_grid.dims)]
+    for i in range(1, levels):
+        num_airys.append(tuple(n_airy * window_size / (2 * qs[i - 1] * n_airy) for n_airy in num_airys[i - 1]))
+    grids, masks = [], []
+    for i in range(levels):
+        # make_focal_grid(q, num_airy, pupil_diameter, reference_wavelength=1, focal_length=1)
+        g = make_focal_grid(qs[i], num_airys[i], spatial_resolution=1.0 / pupil_diameter)
+        m = np.asarray(complex_mask(g), dtype=complex)
+        if i != levels - 1:
+            m = m * (1 - multiscale_window(window_size, g.shape))
+        for j in range(i):
+            fft = FFT(grids[j])
+            m = m - mft_backward(fft.forward(masks[j]), g, fft.output_grid)
+        grids.append(g)
+        masks.append(m)
+    return grids, masks
+
+
+def multiscale_forward(E, pupil_grid, grids, masks, adjoint=False):
+    """MultiScaleCoronagraph.forward / backward without the Lyot stop
+    (hcipy/coronagraphy/multi_scale.py:108-182), at wavelength 1: level 0 is a FourierFilter
+    with q = 2 and the level-0 mask as transfer function, level i adds
+    P_i^dagger (M_i P_i E) with the Fraunhofer propagator P_i onto the level's grid (conj(M_i)
+    for the backward direction)."""
+    L = fourier_filter(E, pupil_grid, masks[0], q=2, adjoint=adjoint)
+    for g, m in zip(grids[1:], masks[1:]):
+        F = fraunhofer_forward(E, pupil_grid, g)
+        F = F * (np.conj(m) if adjoint else m)
+        L = L + fraunhofer_backward(F, pupil_grid, g)
+    return L
+
+
 def actuator_positions(n_act, pitch):
     """make_actuator_positions without tilts (deformable_mirror.py:11-41)."""
     return make_uniform_grid(n_act, [pitch * n_act, pitch * n_act])

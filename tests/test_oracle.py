@@ -347,3 +347,44 @@ def test_oversampled_influence_matches_pixel_average():
     # a dither grid without the half-step centring would be biased by ~dx/2 and fail this
     IF1 = o.gaussian_influence_functions(grid, n_act, pitch, oversampling=1)
     assert np.abs(IF1[k][near] - exact[near]).max() > 10 * np.abs(IF[k][near] - exact[near]).max()
+
+
+def _vortex_setup():
+    pg = o.make_pupil_grid(256)
+    fg = o.make_focal_grid(4, 32)
+    ap = o.evaluate_supersampled(lambda g: o.circular_aperture(g, 1), pg, 8)
+    lyot = o.evaluate_supersampled(lambda g: o.circular_aperture(g, 0.99), pg, 8) > 1 - 1e-5
+    E = ap.astype(complex)
+    E = E / np.sqrt(np.sum(np.abs(E) ** 2) * pg.weights)
+    return pg, fg, E, lyot
+
+
+@pytest.mark.parametrize('charge', [2, 4, 6, 8])
+def test_vortex_rejection_reference_thresholds(charge):
+    """tests/test_coronagraphy.py:4-28: an on-axis source through a charge 2..8 vortex and a 0.99
+    Lyot stop leaves total power < 1e-6 and peak < 1e-8 of the unocculted peak."""
+    pg, fg, E, lyot = _vortex_setup()
+    ref = o.fraunhofer_forward(E, pg, fg)
+    grids, masks = o.multiscale_masks(pg, o.vortex_mask(charge))
+    assert [g.dims for g in grids] == [(512, 512)] + [(128, 128)] * 5
+    L = o.multiscale_forward(E, pg, grids, masks)
+    img = o.fraunhofer_forward(L * lyot, pg, fg)
+    assert np.sum(np.abs(img) ** 2) * fg.weights < 1e-6
+    assert np.abs(img).max() ** 2 / np.abs(ref).max() ** 2 < 1e-8
+
+
+def test_multiscale_levels_add_up_to_the_mask():
+    """With a mask that is 1 everywhere except the origin sample, the levels must reassemble the
+    identity (the corrections remove exactly what coarser levels already carry); a dropped or
+    sign-flipped correction leaves an O(1) error.  The vortex's origin hole costs ~2e-5."""
+    pg, fg, E, lyot = _vortex_setup()
+    grids, masks = o.multiscale_masks(pg, o.vortex_mask(0))
+    L = o.multiscale_forward(E, pg, grids, masks)
+    assert np.abs(L - E).max() / np.abs(E).max() < 5e-5
+    # the backward direction is the adjoint of the forward one (plain sums: same grid)
+    rng = np.random.default_rng(5)
+    U = rng.standard_normal(pg.size) + 1j * rng.standard_normal(pg.size)
+    grids, masks = o.multiscale_masks(pg, o.vortex_mask(2))
+    a = np.vdot(U, o.multiscale_forward(E, pg, grids, masks))
+    b = np.vdot(o.multiscale_forward(U, pg, grids, masks, adjoint=True), E)
+    assert abs(a - b) < 1e-11 * abs(a)
