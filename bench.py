@@ -11,6 +11,7 @@ roofline from HIP events; and the CPU baseline (numpy/scipy restatement of the r
 cgemm calls, timed on the host cores).
 """
 import argparse
+import math
 import ctypes
 import json
 import os
@@ -173,6 +174,68 @@ def main():
                 'grouped_propagations_per_s': round(n / (t1 - t0), 1),
                 'per_wavelength_calls_propagations_per_s': round(n / (t2 - t1), 1)}
 
+    def time_graph(fn, n_chunks=10, chunk=10):
+        """Seconds per call of ``fn`` replayed from a captured graph of ``chunk`` calls (inputs
+        resident in HBM), and the outputs of the last captured call."""
+        fn()
+        torch.cuda.synchronize()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            fn()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(chunk):
+                outs = fn()
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n_chunks):
+            g.replay()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / (n_chunks * chunk), outs
+
+    def time_other_paths():
+        """The other workloads of the hot path, each timed from captured graphs."""
+        out = {}
+        # Angular-spectrum propagation of the 1024^2 field: lambda = 500 nm, D = 1 m, z = 1 km,
+        # 2x zero padding (2048^2 FFT grid).  Error: complex64 field against the complex128 path.
+        asp_grid = ha.make_pupil_grid(1024, 1.0)
+        asp = ha.AngularSpectrumPropagator(asp_grid, 1000.0)
+        e64 = ha.Field(x, asp_grid)
+        e128 = ha.Field(x.to(torch.complex128), asp_grid)
+        sec, o = time_graph(lambda: asp.forward(ha.Wavefront(e64, 500e-9)).electric_field)
+        ref = asp.forward(ha.Wavefront(e128, 500e-9)).electric_field.as_subclass(torch.Tensor)
+        o = o.as_subclass(torch.Tensor).to(torch.complex128)
+        NX = 2048
+        compulsory = (8 + 8 + 32) * 2 ** 20           # field in + out (c64), transfer function (c64 2048^2)
+        passes = (8 + 16 + 16 + 32 + 16 + 16 + 8) * 2 ** 20   # rows, columns (+ transfer function), rows
+        out['angular_spectrum_1024_q2_c64'] = {
+            'value': round(1.0 / sec, 1), 'unit': 'propagations/s', 'ms': round(sec * 1e3, 4),
+            'peak_rel_field_error_vs_c128': (o - ref).abs().max().item() / ref.abs().max().item(),
+            'hbm_gbs_compulsory': round(compulsory / sec / 1e9, 1), 'hbm_gbs_pass_bytes': round(passes / sec / 1e9, 1),
+            'peak_gbs': 8000.0, 'internal_grid': [NX, NX],
+            'gflop': round(2 * 5 * NX * NX * math.log2(NX * NX) / 1e9, 3)}
+        # Lyot coronagraph chain: 1024^2 pupil -> 256^2 focal-plane mask -> Lyot plane (stop 0.95 D)
+        # -> 512^2 focal intensity, f16x2.
+        lp, lfg, lfocal, lE, lfpm, lstop, _ = workloads.lyot_1024()
+        coro = ha.LyotCoronagraph(lp, ha.Field(torch.as_tensor(lfpm, device=dev), lfg),
+                                  ha.Field(torch.as_tensor(lstop, device=dev), lp), precision='f16x2')
+        lprop = ha.FraunhoferPropagator(lp, lfocal, precision='f16x2')
+        lx = ha.Field(torch.as_tensor(lE.astype(np.complex64), device=dev), lp)
+        sec, _ = time_graph(lambda: lprop.forward_intensity(coro.forward(ha.Wavefront(lx, 1.0))))
+        out['lyot_chain_1024_f16x2'] = {'value': round(1.0 / sec, 1), 'unit': 'chains/s', 'ms': round(sec * 1e3, 4),
+                                        'gflop_per_chain': 11.81}
+        # Charge-2 vortex coronagraph on the 1024^2 pupil: FFT filter (2048^2) + 5 levels of MFTs
+        # 1024^2 <-> 128^2 as two grouped launches, f16x2.
+        vc = ha.VortexCoronagraph(asp_grid, 2, precision='f16x2')
+        sec, _ = time_graph(lambda: vc.forward(ha.Wavefront(e64, 1.0)).electric_field)
+        out['vortex_1024_charge2_f16x2'] = {'value': round(1.0 / sec, 1), 'unit': 'forward/s', 'ms': round(sec * 1e3, 4),
+                                            'levels': len(vc.focal_grids)}
+        return out
+
     log('timing ...')
     main_out = _lib.OUTPUT_INTENSITY
     sec_main, steps_done, kms_main, out_main = time_path(args.precision, args.steps, args.warmup, main_out)
@@ -189,6 +252,7 @@ def main():
                           'peak_rel_intensity_error': err(o), 'kernel_ms': [round(v, 5) for v in kms]}
     fp32_ref = variants.get('fp32_intensity') or variants.get('fp32_field_c64')
     broadband = None if args.no_variants else time_broadband()
+    paths = None if args.no_variants else time_other_paths()
 
     # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC).
     ny, nx = pupil.shape
@@ -256,6 +320,7 @@ def main():
         'speedup_vs_fp32': round(value / fp32_ref['value'], 3) if fp32_ref else None,
         'variants': variants,
         'broadband_grouped': broadband,
+        'paths': paths,
         'algorithmic_gflop_per_step': round((flops_stage1 + flops_stage2) / 1e9, 4),
         'roofline': roofline,
         'cpu_baseline': cpu,

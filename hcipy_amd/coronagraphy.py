@@ -120,6 +120,7 @@ class MultiScaleCoronagraph(OpticalElement):
         self.props = []
         w1 = periodic_tukey_window(window_size)
         w = np.outer(w1, w1)
+        spectra = [None] * levels
         for i in range(levels):
             focal_grid = make_focal_grid(qs[i], num_airys[i], pupil_diameter=pupil_diameter, reference_wavelength=1,
                                          focal_length=1)
@@ -133,10 +134,13 @@ class MultiScaleCoronagraph(OpticalElement):
                 wp = np.pad(w, tuple((n_foc - n_w) // 2 for n_foc, n_w in zip(focal_grid.shape, w.shape)), 'constant')
                 mask = mask * torch.as_tensor(1 - wp.ravel(), device=dev)
             for j in range(i):
-                fft = FastFourierTransform(self.focal_grids[j])
-                mft = MatrixFourierTransform(focal_grid, fft.output_grid)
-                corr = mft.backward(fft.forward(Field(self.focal_masks[j], self.focal_grids[j])))
-                mask = mask - corr.as_subclass(torch.Tensor)
+                # the FFT of level j's mask does not depend on i: computed once, on first use
+                if spectra[j] is None:
+                    fft = FastFourierTransform(self.focal_grids[j])
+                    spectra[j] = (fft.output_grid, fft.forward(Field(self.focal_masks[j], self.focal_grids[j])))
+                fft_grid, spectrum = spectra[j]
+                mft = MatrixFourierTransform(focal_grid, fft_grid)
+                mask = mask - mft.backward(spectrum).as_subclass(torch.Tensor)
             if i == 0:
                 prop = FourierFilter(input_grid, Field(mask, focal_grid), qs[0])
             else:
@@ -145,6 +149,8 @@ class MultiScaleCoronagraph(OpticalElement):
             self.focal_masks.append(Field(mask, focal_grid))
             self.props.append(prop)
         self._group = None
+        # grouped: the level transforms run as two grouped launches; False runs them one by one
+        self.grouped = True
 
     def _levels(self, E, conjugate):
         """sum over levels >= 1 of P_i^dagger (M_i P_i E), accumulated onto the level-0 result."""
@@ -152,7 +158,7 @@ class MultiScaleCoronagraph(OpticalElement):
         rest = list(zip(self.focal_masks[1:], self.props[1:]))
         if not rest:
             return L
-        if pointwise.needs_grad(E):
+        if pointwise.needs_grad(E) or not self.grouped:
             for mask, prop in rest:
                 d = prop.instance(1)
                 F = d.fourier_transform._apply(E, _lib.FORWARD, extra_alpha=d.norm_factor)
@@ -160,21 +166,24 @@ class MultiScaleCoronagraph(OpticalElement):
                 neg = d.fourier_transform._apply(F, _lib.BACKWARD, extra_alpha=-1 / d.norm_factor)
                 L = pointwise.sub_mul(L, neg)
             return L
-        if self._group is None or self._group.device != E.device:
-            self._group = MftGroup(max(16, len(rest)), E.device)
+        # Two groups, one per grouped launch: a group's argument table is copied from its host
+        # mirror when the launch runs, so under stream capture one group must not serve two
+        # launches of the same captured sequence.
+        if self._group is None or self._group[0].device != E.device:
+            self._group = (MftGroup(max(16, len(rest)), E.device), MftGroup(max(16, len(rest)), E.device))
         w_in = complex(np.asarray(self.input_grid.weights).ravel()[0])
         fwd = []
         for mask, prop in rest:
             d = prop.instance(1)
             fwd.append((d.fourier_transform, _lib.FORWARD, E, w_in * d.norm_factor, _lib.OUTPUT_FIELD))
-        Fs = self._group.execute(fwd)
+        Fs = self._group[0].execute(fwd)
         bwd = []
         for F, (mask, prop) in zip(Fs, rest):
             d = prop.instance(1)
             F = pointwise.multiply(F, mask, conjugate=conjugate)
             w_uv = complex(np.asarray(d.uv_grid.weights).ravel()[0]) / (2 * np.pi) ** 2
             bwd.append((d.fourier_transform, _lib.BACKWARD, F, -w_uv / d.norm_factor, _lib.OUTPUT_FIELD))
-        for neg in self._group.execute(bwd):
+        for neg in self._group[1].execute(bwd):
             L = pointwise.sub_mul(L, neg)
         return L
 

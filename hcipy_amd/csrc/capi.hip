@@ -187,9 +187,11 @@ struct Collected {
 // Runs one direction: data split (split-FP16 only), stage 1, stage 2.  `only` >= 0 launches just
 // that kernel (0 split, 1 stage 1, 2 stage 2) on a workspace a full run has filled (timing).
 // With `col`, the split is launched and the stage arguments are returned instead of launched.
+// With `shared_split` (split-FP16, `col` set), the split is not launched: stage 1 reads the split
+// data and row scales that `shared_split->a1` points at (an earlier item of the same input).
 int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai,
                   char* ws, hipStream_t s, hipEvent_t* ev = nullptr, int output = HCIPY_OUTPUT_FIELD, int only = -1,
-                  Collected* col = nullptr) {
+                  Collected* col = nullptr, const Collected* shared_split = nullptr) {
   auto mark = [&](int i) -> int {
     if (ev != nullptr) return check_hip(hipEventRecord(ev[i], s), "hipEventRecord");
     return kOk;
@@ -214,7 +216,11 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
     _Float16* lo = reinterpret_cast<_Float16*>(ws + w.split_lo);
     row_scale = reinterpret_cast<float*>(ws + w.row_scale);
     t = reinterpret_cast<float*>(ws + w.t);
-    if (only < 0 || only == 0) {
+    if (shared_split != nullptr) {
+      hi = static_cast<_Float16*>(const_cast<void*>(shared_split->a1.b));
+      lo = static_cast<_Float16*>(const_cast<void*>(shared_split->a1.b_lo));
+      row_scale = const_cast<float*>(shared_split->a1.row_scale);
+    } else if (only < 0 || only == 0) {
       int st = launch_split_rows(static_cast<const float*>(in), 2 * d.k1, (int)(2 * d.k1), rows1, hi, lo, w.ld1,
                                  row_scale, s);
       if (st != kOk) return st;
@@ -322,17 +328,26 @@ struct hcipy_mft_plan_s {
 
 // Table of stage arguments for grouped launches: device copy and pinned host mirror.
 // Layout: StageArgs[2 * cap] | int64 tile starts[2 * (cap + 4)].
+// Launches recorded under stream capture copy their table from a snapshot of their own (the copy
+// node reads the host buffer at every replay, so a shared mirror would hand every replay the
+// last table written): `snaps` holds `snap_count` tables, each used by one captured call.
 struct hcipy_mft_group_s {
   int cap = 0;
   int device = 0;
   size_t bytes = 0;
   char* dev = nullptr;
   char* host = nullptr;
+  char* snaps = nullptr;
+  int snap_count = 0;
+  int snap_used = 0;
   hipEvent_t copied = nullptr;
   bool pending = false;
 };
 
 namespace {
+// Tables kept for launches recorded under stream capture (at most 64 MiB per group).
+constexpr size_t kGroupCaptureSlots = 64;
+
 size_t group_table_bytes(int cap) {
   return round_up((int64_t)(2 * (size_t)cap * sizeof(StageArgs)), 64) + 2 * ((size_t)cap + 4) * sizeof(int64_t);
 }
@@ -522,6 +537,10 @@ int hcipy_mft_group_create(hcipy_mft_group* group, int capacity) {
   int st = check_hip(hipGetDevice(&g->device), "hipGetDevice");
   if (!st) st = check_hip(hipMalloc(&g->dev, g->bytes), "hipMalloc(group table)");
   if (!st) st = check_hip(hipHostMalloc(&g->host, g->bytes), "hipHostMalloc(group table)");
+  if (!st) {
+    g->snap_count = (int)std::max<size_t>(1, std::min<size_t>(kGroupCaptureSlots, (size_t(64) << 20) / g->bytes));
+    st = check_hip(hipHostMalloc(&g->snaps, g->bytes * (size_t)g->snap_count), "hipHostMalloc(group snapshots)");
+  }
   if (!st) st = check_hip(hipEventCreateWithFlags(&g->copied, hipEventDisableTiming), "hipEventCreate");
   if (st) {
     hcipy_mft_group_destroy(g);
@@ -539,6 +558,7 @@ int hcipy_mft_group_destroy(hcipy_mft_group group) {
   }
   if (group->dev) (void)hipFree(group->dev);
   if (group->host) (void)hipHostFree(group->host);
+  if (group->snaps) (void)hipHostFree(group->snaps);
   delete group;
   return kOk;
 }
@@ -559,20 +579,43 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
     HCIPY_REQUIRE(pl.prec == prec, "item %d: all items of a grouped launch share one precision", i);
     HCIPY_REQUIRE(pl.device == cur, "item %d: plan lives on device %d", i, pl.device);
     HCIPY_REQUIRE(it.direction == HCIPY_FORWARD || it.direction == HCIPY_BACKWARD, "item %d: bad direction", i);
-    const Ws w = workspace_layout(pl, direction(pl, it.direction), it.batch);
+    const Dir di = direction(pl, it.direction);
+    const Ws w = workspace_layout(pl, di, it.batch);
     HCIPY_REQUIRE(it.workspace_bytes >= w.total && aligned16(it.workspace), "item %d: workspace too small", i);
-    // the data split (split-FP16) is launched here, per item; stage arguments are gathered
+    // The data split (split-FP16) is launched here, per distinct input: an item that reads the
+    // same input as an earlier item, with the same rows, contraction length and split row pitch
+    // (for example one pupil field sent to several focal grids), reuses that item's split.
+    const Collected* shared = nullptr;
+    if (prec == kF16x2) {
+      for (int j = 0; j < i && shared == nullptr; ++j) {
+        const hcipy_mft_group_item& o = items[j];
+        const Plan& pj = o.plan->p;
+        const Dir dj = direction(pj, o.direction);
+        const int64_t rows_i = it.batch * (pl.ndim == 2 ? di.n1 : 1);
+        const int64_t rows_j = o.batch * (pj.ndim == 2 ? dj.n1 : 1);
+        if (o.in == it.in && rows_i == rows_j && dj.k1 == di.k1 && workspace_layout(pj, dj, o.batch).ld1 == w.ld1)
+          shared = &col[(size_t)j];
+      }
+    }
     const int st = run_direction(pl, it.direction, it.in, it.out, it.batch, it.alpha_re, it.alpha_im,
-                                 static_cast<char*>(it.workspace), s, nullptr, it.output, -1, &col[(size_t)i]);
+                                 static_cast<char*>(it.workspace), s, nullptr, it.output, -1, &col[(size_t)i], shared);
     if (st != kOk) return st;
   }
-  // The mirror may still be read by the previous call's copy: wait for it before rewriting, and
-  // only if it has not completed (a completed event needs no synchronisation, so a call inside
-  // stream capture that follows a finished eager call does not synchronise).
-  if (group->pending && hipEventQuery(group->copied) == hipErrorNotReady) HCIPY_HIP(hipEventSynchronize(group->copied));
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  HCIPY_HIP(hipStreamIsCapturing(s, &capture));
+  char* src = group->host;
+  if (capture != hipStreamCaptureStatusNone) {
+    HCIPY_REQUIRE(group->snap_used < group->snap_count,
+                  "this group has served its %d launches under stream capture; use another group", group->snap_count);
+    src = group->snaps + (size_t)group->snap_used * group->bytes;
+    ++group->snap_used;
+  } else if (group->pending && hipEventQuery(group->copied) == hipErrorNotReady) {
+    // the mirror may still be read by the previous call's copy: wait before rewriting it
+    HCIPY_HIP(hipEventSynchronize(group->copied));
+  }
   // Table: stage-1 items ordered by data mode (one launch per mode), then stage-2 items.
-  StageArgs* hargs = reinterpret_cast<StageArgs*>(group->host);
-  int64_t* hstart = reinterpret_cast<int64_t*>(group->host + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
+  StageArgs* hargs = reinterpret_cast<StageArgs*>(src);
+  int64_t* hstart = reinterpret_cast<int64_t*>(src + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
   struct Launch {
     int stage, b_mode, first, n;
     int64_t start_off, tiles;
@@ -604,9 +647,11 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
       launches.push_back(L);
     }
   }
-  HCIPY_HIP(hipMemcpyAsync(group->dev, group->host, group->bytes, hipMemcpyHostToDevice, s));
-  HCIPY_HIP(hipEventRecord(group->copied, s));
-  group->pending = true;
+  HCIPY_HIP(hipMemcpyAsync(group->dev, src, group->bytes, hipMemcpyHostToDevice, s));
+  if (capture == hipStreamCaptureStatusNone) {
+    HCIPY_HIP(hipEventRecord(group->copied, s));
+    group->pending = true;
+  }
   const StageArgs* dargs = reinterpret_cast<const StageArgs*>(group->dev);
   const int64_t* dstart = reinterpret_cast<const int64_t*>(group->dev + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
   for (const Launch& L : launches) {

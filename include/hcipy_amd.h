@@ -126,8 +126,11 @@ int hcipy_mft_execute(hcipy_mft_plan plan, int direction, const void* in, void* 
  * (fixed tile shapes: 64x64 for stage 1, 64x32 for stage 2).  All items share one precision and
  * live on the group's device; every item has its own workspace (hcipy_mft_workspace_size).
  * A group owns a device table of stage arguments and a pinned host mirror (no allocation in
- * hcipy_mft_execute_grouped).  Calls on one group are ordered on one stream; a captured grouped
- * call replays the table of its capture, so a group used in a graph serves that graph only. */
+ * hcipy_mft_execute_grouped).  Calls on one group are ordered on one stream.  Items that read the
+ * same input with the same rows and contraction length share one data split (split-FP16).
+ * A call recorded under stream capture copies its table from a pinned snapshot of its own, so every
+ * replay sees the table of its capture; a group holds up to 64 such snapshots over its lifetime
+ * (fewer for very large capacities) and then refuses further captured calls. */
 typedef struct hcipy_mft_group_s* hcipy_mft_group;
 typedef struct {
   hcipy_mft_plan plan;

@@ -66,3 +66,50 @@ def test_graph_replays_equal_eager():
         say('replay', r)
         for a, b in zip(got, ref):
             assert torch.equal(a, b)
+
+
+def test_grouped_launches_replay_in_graphs():
+    """Grouped MFT launches recorded under capture: the vortex coronagraph runs two grouped
+    launches per forward, and the captured sequence holds two forwards, so each group records two
+    launches with different tables.  Every replay must equal the eager result bit for bit."""
+    pupil = ha.make_pupil_grid(128)
+    vc = ha.VortexCoronagraph(pupil, 2, q=64)
+    assert len(vc.focal_grids) == 4
+    x1 = torch.zeros(pupil.size, dtype=torch.complex64, device='cuda')
+    x2 = torch.zeros_like(x1)
+
+    def chain():
+        a = vc.forward(ha.Wavefront(ha.Field(x1, pupil))).electric_field
+        b = vc.forward(ha.Wavefront(ha.Field(x2, pupil))).electric_field
+        return [t.as_subclass(torch.Tensor) for t in (a, b)]
+
+    def new_input(seed):
+        g = torch.Generator().manual_seed(seed)
+        return torch.complex(torch.randn(pupil.size, generator=g), torch.randn(pupil.size, generator=g)).cuda()
+
+    x1.copy_(new_input(0))
+    x2.copy_(new_input(1))
+    chain()
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        chain()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        outs = chain()
+    torch.cuda.synchronize()
+    for r in range(4):
+        x1.copy_(new_input(2 * r + 2))
+        x2.copy_(new_input(2 * r + 3))
+        graph.replay()
+        torch.cuda.synchronize()
+        got = [o.clone() for o in outs]
+        ref = chain()
+        torch.cuda.synchronize()
+        say('grouped replay', r)
+        assert not torch.equal(got[0], got[1])
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b)
