@@ -175,12 +175,15 @@ __global__ __launch_bounds__(256) void rows_forward_kernel(const typename Cplx<T
   const bool valid = y < ny;
   V* L = lds[line];
   const V zero = {0, 0};
-  for (int i = tl; i < NX; i += TPL) {
-    V v = zero;
-    const int xi = i - c0x;
-    if (valid && xi >= 0 && xi < nx) v = in[(b * ny + y) * (int64_t)nx + xi];
-    L[i] = v;
+  constexpr int PR = NX / TPL;  // samples per thread; all loads in flight before the LDS stores
+  V g[PR];
+#pragma unroll
+  for (int j = 0; j < PR; ++j) {
+    const int xi = tl + j * TPL - c0x;
+    g[j] = (valid && xi >= 0 && xi < nx) ? in[(b * ny + y) * (int64_t)nx + xi] : zero;
   }
+#pragma unroll
+  for (int j = 0; j < PR; ++j) L[tl + j * TPL] = g[j];
   __syncthreads();
   line_fft<T, NX, TPL>(L, tl, twx, -1, 0);
   if (valid) {
@@ -206,7 +209,12 @@ __global__ __launch_bounds__(256) void rows_inverse_kernel(const typename Cplx<T
   const bool valid = y < ny;
   V* L = lds[line];
   const V zero = {0, 0};
-  for (int i = tl; i < NX; i += TPL) L[i] = valid ? w2[(b * ny + y) * (int64_t)NX + i] : zero;
+  constexpr int PR = NX / TPL;
+  V g[PR];
+#pragma unroll
+  for (int j = 0; j < PR; ++j) g[j] = valid ? w2[(b * ny + y) * (int64_t)NX + tl + j * TPL] : zero;
+#pragma unroll
+  for (int j = 0; j < PR; ++j) L[tl + j * TPL] = g[j];
   __syncthreads();
   line_fft<T, NX, TPL>(L, tl, twx, +1, 0);
   if (valid) {
@@ -228,9 +236,17 @@ struct ColCfg {
   static constexpr int C = CF > 16 ? 16 : (CF < 1 ? 1 : CF);
   static constexpr int TPL = NT / C;
   static constexpr int LDN = NY + 1;  // padded line length
+  // elements of the C x NY block each thread gathers / multiplies (fully unrolled, so that all
+  // global loads of a loop are in flight together)
+  static constexpr int PER = (C * NY + NT - 1) / NT;
+  // the twiddle table sits in LDS next to the lines when both fit in the 160 KiB of a CU
+  static constexpr bool TW_LDS = (size_t)C * LDN * 2 * sizeof(T) + (size_t)NY * 2 * sizeof(T) <= 160 * 1024;
+  static constexpr size_t LDS_BYTES = ((size_t)C * LDN + (TW_LDS ? NY : 0)) * 2 * sizeof(T);
 };
 
 // pass B: column FFT, transfer function, inverse column FFT, crop to the centred ny rows.
+// The global loads of the gather, and those of the transfer function, are each issued as one
+// unrolled batch so that their latencies overlap instead of adding up.
 template <typename T, int NY>
 __global__ __launch_bounds__(512) void columns_kernel(const typename Cplx<T>::V* __restrict__ w1,
                                                       typename Cplx<T>::V* __restrict__ w2,
@@ -242,38 +258,62 @@ __global__ __launch_bounds__(512) void columns_kernel(const typename Cplx<T>::V*
   constexpr int C = Cf::C;
   constexpr int TPL = Cf::TPL;
   constexpr int LDN = Cf::LDN;
+  constexpr int PER = Cf::PER;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
   const int kx0 = blockIdx.x * C;
   const int64_t b = blockIdx.y;
   const V zero = {0, 0};
   const int tid = threadIdx.x;
-  // zero the padding rows, gather the ny valid rows (C consecutive columns per row: coalesced)
-  for (int idx = tid; idx < C * NY; idx += Cf::NT) {
+  // gather the ny valid rows (C consecutive columns per row: coalesced), zero the padding rows
+  V g[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int idx = tid + i * Cf::NT;
     const int c = idx % C;
     const int r = idx / C;
     const int yi = r - c0y;
     V v = zero;
-    if (yi >= 0 && yi < ny && kx0 + c < NX) v = w1[(b * ny + yi) * (int64_t)NX + kx0 + c];
-    lds[c * LDN + r] = v;
+    if (idx < C * NY && yi >= 0 && yi < ny && kx0 + c < NX) v = w1[(b * ny + yi) * (int64_t)NX + kx0 + c];
+    g[i] = v;
+  }
+  const V* tw = twy;
+  if constexpr (Cf::TW_LDS) {
+    V* tws = lds + C * LDN;
+    for (int i = tid; i < NY; i += Cf::NT) tws[i] = twy[i];
+    tw = tws;
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int idx = tid + i * Cf::NT;
+    if (idx < C * NY) lds[(idx % C) * LDN + idx / C] = g[i];
   }
   __syncthreads();
   const int line = tid / TPL;
   const int tl = tid % TPL;
   V* L = lds + line * LDN;
-  line_fft<T, NY, TPL>(L, tl, twy, -1, 0);
-  // transfer function (already in FFT order): tf[ky][kx]
-  for (int idx = tid; idx < C * NY; idx += Cf::NT) {
+  line_fft<T, NY, TPL>(L, tl, tw, -1, 0);
+  // transfer function (already in FFT order): tf[ky][kx]; all loads issued before the multiplies
+  V t[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int idx = tid + i * Cf::NT;
     const int c = idx % C;
     const int r = idx / C;
-    if (kx0 + c < NX) {
-      V t = tf[(int64_t)r * NX + kx0 + c];
-      if (conj_tf) t.y = -t.y;
-      lds[c * LDN + r] = cmul(lds[c * LDN + r], t);
+    t[i] = (idx < C * NY && kx0 + c < NX) ? tf[(int64_t)r * NX + kx0 + c] : zero;
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int idx = tid + i * Cf::NT;
+    if (idx < C * NY) {
+      V tv = t[i];
+      if (conj_tf) tv.y = -tv.y;
+      V* e = &lds[(idx % C) * LDN + idx / C];
+      *e = cmul(*e, tv);
     }
   }
   __syncthreads();
-  line_fft<T, NY, TPL>(L, tl, twy, +1, 0);
+  line_fft<T, NY, TPL>(L, tl, tw, +1, 0);
   for (int idx = tid; idx < C * ny; idx += Cf::NT) {
     const int c = idx % C;
     const int yi = idx / C;
@@ -415,7 +455,7 @@ template <typename T, int N>
 int launch_cols(const FilterPlan& p, const void* w1, void* w2, int conj, int64_t batch, hipStream_t s) {
   using V = typename Cplx<T>::V;
   using Cf = ColCfg<T, N>;
-  const size_t lds = (size_t)Cf::C * Cf::LDN * sizeof(V);
+  const size_t lds = Cf::LDS_BYTES;
   auto kern = columns_kernel<T, N>;
   HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
