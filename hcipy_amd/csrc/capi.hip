@@ -27,9 +27,10 @@ int check_hip(hipError_t e, const char* what) {
 }
 
 int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s);
-int launch_stage_grouped(int prec, int stage, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
+int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
                          int64_t tiles, hipStream_t s);
-void grouped_tile_shape(int stage, int* bm, int* bn);
+int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64);
+void grouped_tile_shape(int cfg, int* bm, int* bn);
 constexpr int kModePadded = 0, kModeAligned = 1, kModeUnaligned = 2;  // BMode of mft_gemm.hip
 int launch_split_rows(const float* in, int64_t ld_in, int K, int64_t rows, _Float16* hi, _Float16* lo,
                       int64_t ld_out, float* row_scale, hipStream_t s);
@@ -617,7 +618,7 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   StageArgs* hargs = reinterpret_cast<StageArgs*>(src);
   int64_t* hstart = reinterpret_cast<int64_t*>(src + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
   struct Launch {
-    int stage, b_mode, first, n;
+    int stage, b_mode, cfg, first, n;
     int64_t start_off, tiles;
   };
   std::vector<Launch> launches;
@@ -626,14 +627,25 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   for (int stage = 0; stage < 2; ++stage) {
     for (int mode = 0; mode < 3; ++mode) {
       if (stage == 1 && mode != kModePadded) continue;
+      auto member = [&](const Collected& c) { return stage == 0 ? c.b_mode1 == mode : c.nz2 != 0; };
+      // tile shape of this launch from its longest contraction and its size in 64x64 tiles
+      int max_k = 0;
+      int64_t tiles64 = 0;
+      for (int i = 0; i < n_items; ++i) {
+        const Collected& c = col[(size_t)i];
+        if (!member(c)) continue;
+        const StageArgs& a = stage == 0 ? c.a1 : c.a2;
+        max_k = std::max(max_k, a.K);
+        tiles64 += ceil_div(a.M, 64) * ceil_div(a.N, 64) * (stage == 0 ? 1 : c.nz2);
+      }
+      const int cfg = grouped_tile_cfg(prec, stage, max_k, tiles64);
       int bm, bn;
-      grouped_tile_shape(stage, &bm, &bn);
-      Launch L{stage, mode, na, 0, ns, 0};
+      grouped_tile_shape(cfg, &bm, &bn);
+      Launch L{stage, mode, cfg, na, 0, ns, 0};
       int64_t acc = 0;
       for (int i = 0; i < n_items; ++i) {
         const Collected& c = col[(size_t)i];
-        if (stage == 0 && c.b_mode1 != mode) continue;
-        if (stage == 1 && c.nz2 == 0) continue;
+        if (!member(c)) continue;
         const StageArgs& a = stage == 0 ? c.a1 : c.a2;
         const int64_t nz = stage == 0 ? 1 : c.nz2;
         hargs[na++] = a;
@@ -655,7 +667,7 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   const StageArgs* dargs = reinterpret_cast<const StageArgs*>(group->dev);
   const int64_t* dstart = reinterpret_cast<const int64_t*>(group->dev + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
   for (const Launch& L : launches) {
-    const int st = launch_stage_grouped(prec, L.stage, L.b_mode, dargs + L.first, dstart + L.start_off, L.n, L.tiles, s);
+    const int st = launch_stage_grouped(prec, L.cfg, L.b_mode, dargs + L.first, dstart + L.start_off, L.n, L.tiles, s);
     if (st != kOk) return st;
   }
   return kOk;

@@ -1628,38 +1628,55 @@ static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, i
 }
 
 template <int P, int BK, int BMODE, int PF>
-static int launch_grouped_prec(int stage, const StageArgs* table, const int64_t* tile_start, int n, int64_t tiles,
+static int launch_grouped_prec(int cfg, const StageArgs* table, const int64_t* tile_start, int n, int64_t tiles,
                                hipStream_t s) {
-  if constexpr (P == kF64) {
-    if (stage == 0) return launch_grouped_t<P, 64, 64, BK, 2, 2, 1, BMODE, PF>(table, tile_start, n, tiles, s);
-    return launch_grouped_t<P, 64, 32, BK, 2, 1, 2, BMODE, PF>(table, tile_start, n, tiles, s);
-  } else {
-    if (stage == 0) return launch_grouped_t<P, 64, 64, BK, 2, 2, 2, BMODE, PF>(table, tile_start, n, tiles, s);
-    return launch_grouped_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(table, tile_start, n, tiles, s);
+  switch (cfg) {
+    case k64x64_w2x2: return launch_grouped_t<P, 64, 64, BK, 2, 2, 1, BMODE, PF>(table, tile_start, n, tiles, s);
+    case k64x32_w2x1: return launch_grouped_t<P, 64, 32, BK, 2, 1, 2, BMODE, PF>(table, tile_start, n, tiles, s);
+    case k64x64_w8: return launch_grouped_t<P, 64, 64, BK, 2, 2, 2, BMODE, PF>(table, tile_start, n, tiles, s);
+    default: break;
   }
+  if constexpr (P != kF64) {
+    if (cfg == k64x32_w8) return launch_grouped_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(table, tile_start, n, tiles, s);
+    if (cfg == k128x64) return launch_grouped_t<P, 128, 64, BK / 2, 2, 2, 1, BMODE, PF>(table, tile_start, n, tiles, s);
+  }
+  set_last_error("tile configuration %d is not available to grouped launches", cfg);
+  return kInvalidValue;
 }
 
-void grouped_tile_shape(int stage, int* bm, int* bn) {
-  *bm = 64;
-  *bn = stage == 0 ? 64 : 32;
+// Tile shape of a grouped launch.  Short contractions (every item K <= 512: for example the
+// adjoint of a transform onto a small focal grid) are bound by the per-workgroup prologue and
+// epilogue, so they take shapes with fewer k-tiles in flight and more output per workgroup
+// (64x32 with 4 waves for stage 1, 128x64 half-depth for stage 2); long ones take the 8-wave
+// shapes, 64x64 when that still gives two workgroups per CU.
+int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64) {
+  if (prec == kF64) return stage == 0 ? k64x64_w2x2 : k64x32_w2x1;
+  if (max_k <= 512) return stage == 0 ? k64x32_w2x1 : k128x64;
+  if (stage == 0 && tiles64 >= 512) return k64x64_w8;
+  return k64x32_w8;
 }
 
-int launch_stage_grouped(int prec, int stage, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
+void grouped_tile_shape(int cfg, int* bm, int* bn) {
+  *bm = cfg == k128x64 ? 128 : 64;
+  *bn = (cfg == k64x32_w2x1 || cfg == k64x32_w8) ? 32 : 64;
+}
+
+int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
                          int64_t tiles, hipStream_t s) {
   if (n <= 0 || tiles <= 0) return kOk;
   if (prec == kF64) {
-    if (b_mode == kRowsPadded) return launch_grouped_prec<kF64, 32, kRowsPadded, 4>(stage, table, tile_start, n, tiles, s);
-    if (b_mode == kRowsAligned) return launch_grouped_prec<kF64, 32, kRowsAligned, 4>(stage, table, tile_start, n, tiles, s);
-    return launch_grouped_prec<kF64, 32, kRowsUnaligned, 4>(stage, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsPadded) return launch_grouped_prec<kF64, 32, kRowsPadded, 4>(cfg, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsAligned) return launch_grouped_prec<kF64, 32, kRowsAligned, 4>(cfg, table, tile_start, n, tiles, s);
+    return launch_grouped_prec<kF64, 32, kRowsUnaligned, 4>(cfg, table, tile_start, n, tiles, s);
   }
   if (prec == kF32) {
-    if (b_mode == kRowsPadded) return launch_grouped_prec<kF32, 64, kRowsPadded, 2>(stage, table, tile_start, n, tiles, s);
-    if (b_mode == kRowsAligned) return launch_grouped_prec<kF32, 64, kRowsAligned, 2>(stage, table, tile_start, n, tiles, s);
-    return launch_grouped_prec<kF32, 64, kRowsUnaligned, 2>(stage, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsPadded) return launch_grouped_prec<kF32, 64, kRowsPadded, 2>(cfg, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsAligned) return launch_grouped_prec<kF32, 64, kRowsAligned, 2>(cfg, table, tile_start, n, tiles, s);
+    return launch_grouped_prec<kF32, 64, kRowsUnaligned, 2>(cfg, table, tile_start, n, tiles, s);
   }
   if (prec == kF16x2) {
     HCIPY_REQUIRE(b_mode == kRowsPadded, "split-FP16 stage requires a padded data operand");
-    return launch_grouped_prec<kF16x2, 128, kRowsPadded, 2>(stage, table, tile_start, n, tiles, s);
+    return launch_grouped_prec<kF16x2, 128, kRowsPadded, 2>(cfg, table, tile_start, n, tiles, s);
   }
   set_last_error("unknown precision %d", prec);
   return kInvalidValue;

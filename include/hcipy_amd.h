@@ -123,7 +123,8 @@ int hcipy_mft_execute(hcipy_mft_plan plan, int direction, const void* in, void* 
  * wavelengths / instances of MatrixFourierTransform.forward and .backward
  * (hcipy/fourier/matrix_fourier_transform.py:132-224 called per AgnosticOpticalElement instance,
  * hcipy/optics/optical_element.py:297-380); the stage kernels are the same as hcipy_mft_execute's
- * (fixed tile shapes: 64x64 for stage 1, 64x32 for stage 2).  All items share one precision and
+ * (one tile shape per launch, chosen from the items' longest contraction and total size).  All
+ * items share one precision and
  * live on the group's device; every item has its own workspace (hcipy_mft_workspace_size).
  * A group owns a device table of stage arguments and a pinned host mirror (no allocation in
  * hcipy_mft_execute_grouped).  Calls on one group are ordered on one stream.  Items that read the
