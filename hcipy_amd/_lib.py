@@ -113,6 +113,10 @@ SIGNATURES = {
     'hcipy_filter_nearfield_transfer_function': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                                 ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                                 ctypes.c_double, ctypes.c_int, _vp]),
+    'hcipy_nearfield_impulse_response': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                        ctypes.c_int, _vp, _vp]),
     'hcipy_filter_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_filter_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, _vp, ctypes.c_size_t, _vp]),
 }

@@ -423,6 +423,51 @@ __global__ void nearfield_tf_kernel(int kind, int NX, int NY, double dx, double 
   }
 }
 
+// Impulse responses of the near-field propagators on a spatial grid, FP64, averaged over
+// oversampling^2 dithered sub-grids like the transfer functions above; written as complex128 in
+// grid order.  With r^2 = x^2 + y^2 + z^2:
+//   kind 0: Rayleigh-Sommerfeld  (z / r) / (2 pi) exp(i k r) (1 / r^2 - i k / r)   (angular_spectrum.py:59-64)
+//   kind 1: Fresnel              exp(i k z) / (i lambda z) exp(i k (x^2 + y^2) / (2 z))  (fresnel.py:55-62)
+__global__ void nearfield_ir_kernel(int kind, int NX, int NY, double dx, double zx, double dy, double zy, double k,
+                                    double z, int ovs, double pf_re, double pf_im, double2* __restrict__ out) {
+  const int64_t total = (int64_t)NX * NY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int iy = (int)(i / NX);
+    const int ix = (int)(i % NX);
+    const double x0 = sep_coord(ix, dx, zx), y0 = sep_coord(iy, dy, zy);
+    const double sx = sep_spacing(ix, NX, dx, zx), sy = sep_spacing(iy, NY, dy, zy);
+    double acc_re = 0.0, acc_im = 0.0;
+    const double dd = 1.0 / ovs;
+    for (int a = 0; a < ovs * ovs; ++a) {
+      const double ox = (double)(a % ovs) * dd + (-0.5 + dd / 2);
+      const double oy = (double)(a / ovs) * dd + (-0.5 + dd / 2);
+      const double x = x0 + ox * sx;
+      const double y = y0 + oy * sy;
+      double re, im;
+      if (kind == 0) {
+        const double r2 = x * x + y * y + z * z;
+        const double r = sqrt(r2);
+        const double c0 = (z / r) / (2.0 * M_PI);
+        double s, c;
+        sincos(k * r, &s, &c);
+        // (c + i s) (1 / r^2 - i k / r)
+        const double a_re = 1.0 / r2, a_im = -k / r;
+        re = c0 * (c * a_re - s * a_im);
+        im = c0 * (c * a_im + s * a_re);
+      } else {
+        const double ph = (0.5 * k / z) * (x * x + y * y);
+        double s, c;
+        sincos(ph, &s, &c);
+        re = pf_re * c - pf_im * s;
+        im = pf_re * s + pf_im * c;
+      }
+      acc_re += re;
+      acc_im += im;
+    }
+    out[i] = make_double2(acc_re / (double)(ovs * ovs), acc_im / (double)(ovs * ovs));
+  }
+}
+
 struct FilterPlan {
   int nx, ny, NX, NY, dtype, device;
   int c0x, c0y;
@@ -582,6 +627,25 @@ int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, d
                      static_cast<hipStream_t>(stream), kind, p.NX, p.NY, delta_x, zero_x, delta_y, zero_y, wavenumber,
                      distance, oversampling, pf_re, pf_im, p.tf, p.dtype);
   return check_hip(hipGetLastError(), "nearfield_tf_kernel");
+}
+
+int hcipy_nearfield_impulse_response(int kind, int nx, int ny, double delta_x, double zero_x, double delta_y,
+                                     double zero_y, double wavenumber, double wavelength, double distance,
+                                     int oversampling, void* out, void* stream) {
+  HCIPY_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (angular spectrum) or 1 (Fresnel)");
+  HCIPY_REQUIRE(nx >= 2 && ny >= 2, "grid too small");
+  HCIPY_REQUIRE(oversampling >= 1 && oversampling <= 16, "oversampling out of range");
+  HCIPY_REQUIRE(distance != 0.0 && wavelength > 0.0, "distance must be non-zero and the wavelength positive");
+  HCIPY_REQUIRE(out != nullptr, "NULL output");
+  // Fresnel prefactor exp(i k z) / (i lambda z), in FP64 on the host as the reference computes it
+  const double e_re = cos(wavenumber * distance), e_im = sin(wavenumber * distance);
+  const double d = wavelength * distance;  // exp(i k z) / (i d) = (e_im - i e_re) / d
+  const double pf_re = e_im / d, pf_im = -e_re / d;
+  const int64_t total = (int64_t)nx * ny;
+  hipLaunchKernelGGL(nearfield_ir_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), kind, nx, ny, delta_x, zero_x, delta_y, zero_y, wavenumber,
+                     distance, oversampling, pf_re, pf_im, static_cast<double2*>(out));
+  return check_hip(hipGetLastError(), "nearfield_ir_kernel");
 }
 
 int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* bytes) {

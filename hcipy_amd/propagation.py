@@ -5,9 +5,11 @@
 """
 import numpy as np
 
+import torch
+
 from . import _lib
-from .field import Field
-from .fourier import MatrixFourierTransform, MftGroup
+from .field import Field, default_device
+from .fourier import FastFourierTransform, MatrixFourierTransform, MftGroup, make_fft_grid
 from .optics import AgnosticOpticalElement, Wavefront
 
 __all__ = ['FraunhoferPropagator']
@@ -117,11 +119,32 @@ class _NearFieldPropagator(AgnosticOpticalElement):
             raise ValueError('The input grid must be a regular, Cartesian grid.')
         n = self.evaluate_parameter(self.refractive_index, input_grid, output_grid, wavelength)
         k = 2 * np.pi / wavelength * n
-        if not _nearfield_branch(input_grid, wavelength, self.distance):
-            raise NotImplementedError('the impulse-response transfer function branch (grid spacing below '
-                                      'lambda |z| / L) is not implemented on the device')
-        tf = NearFieldTransferFunction(self.KIND, k, self.distance, self.num_oversampling)
+        if _nearfield_branch(input_grid, wavelength, self.distance):
+            tf = NearFieldTransferFunction(self.KIND, k, self.distance, self.num_oversampling)
+        else:
+            tf = self._impulse_response_tf(input_grid, k, wavelength)
         instance_data.fourier_filter = FourierFilter(input_grid, tf, q=self._q())
+
+    def _impulse_response_tf(self, input_grid, k, wavelength):
+        """Impulse-response branch (angular_spectrum.py:55-69, fresnel.py:50-67), taken when the grid
+        step is below lambda |z| / L: the impulse response, supersampled on the spatial grid of
+        the filter's internal grid (the padded input grid), Fourier transformed onto the internal
+        grid.  The impulse response is generated in FP64 on the device; the transform is the
+        FP64 MFMA transform onto the FFT grid."""
+        internal = make_fft_grid(input_grid, self._q(), 1)
+        enlarged = make_fft_grid(internal)
+        dev = default_device()
+        with torch.cuda.device(dev):
+            h = torch.empty(enlarged.size, dtype=torch.complex128, device=dev)
+            (x0, y0), (dx, dy) = enlarged.zero, enlarged.delta
+            nx, ny = (int(v) for v in enlarged.dims)
+            kind = {'angular_spectrum': 0, 'fresnel': 1}[self.KIND]
+            st = _lib.load().hcipy_nearfield_impulse_response(kind, nx, ny, float(dx), float(x0), float(dy), float(y0),
+                                                             float(k), float(wavelength), float(self.distance),
+                                                             int(self.num_oversampling), _lib.dptr(h),
+                                                             _lib.stream_ptr(dev))
+            _lib.check(st, 'hcipy_nearfield_impulse_response')
+            return FastFourierTransform(enlarged).forward(Field(h, enlarged))
 
     def _q(self):
         return self._zero_padding

@@ -304,6 +304,17 @@ int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, d
                                              double delta_y, double zero_y, double wavenumber, double distance,
                                              int oversampling, void* stream);
 
+/* Impulse response of a near-field propagator on an nx x ny spatial grid (separated coordinates
+ * i * delta + zero), FP64, averaged over oversampling^2 dithered sub-grids, written as complex128
+ * in grid order.  The impulse-response branch of the propagators Fourier transforms it into the
+ * transfer function when the grid step is below lambda |z| / L (hcipy/propagation/angular_spectrum.py:55-69,
+ * hcipy/propagation/fresnel.py:50-67).  kind 0: Rayleigh-Sommerfeld
+ * (z / r) / (2 pi) exp(i k r) (1 / r^2 - i k / r); kind 1: Fresnel
+ * exp(i k z) / (i lambda z) exp(i k (x^2 + y^2) / (2 z)). */
+int hcipy_nearfield_impulse_response(int kind, int nx, int ny, double delta_x, double zero_x, double delta_y,
+                                     double zero_y, double wavenumber, double wavelength, double distance,
+                                     int oversampling, void* out, void* stream);
+
 /* Bytes of workspace hcipy_filter_execute needs for `batch` fields. */
 int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* bytes);
 

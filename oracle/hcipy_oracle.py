@@ -19,8 +19,10 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
 * ``fft_forward``/``fft_backward``/``fourier_filter``: numpy.fft as library routine, the
   MFT on the same grids, and FourierFilter == FFT.backward(FFT.forward(f) * tf)
   (tests/test_fourier.py:273-309);
-* ``angular_spectrum_tf``/``fresnel_tf``: the Fresnel-integral closed form of a rectangular
-  aperture and +-z symmetry (tests/test_propagation.py:64-138);
+* ``angular_spectrum_tf``/``fresnel_tf``/``impulse_response_tf``: the Fresnel-integral closed
+  form of a rectangular aperture and +-z symmetry (tests/test_propagation.py:64-138), at 0.2 of
+  the critical distance (transfer-function branch) and at 5 (impulse-response branch), as the
+  reference's own test does;
 * apertures: the reference's FITS golden rasters (tests/baseline_for_apertures/circular,
   obstructed_circular);
 * ``lyot_forward``: the explicit (1-M) Babinet algebra and the reference thresholds
@@ -473,15 +475,47 @@ def fresnel_tf(internal_grid, wavelength, distance, num_oversampling=2, refracti
     return evaluate_supersampled(tf_native, internal_grid, num_oversampling)
 
 
+def impulse_response_tf(kind, internal_grid, wavelength, distance, num_oversampling=2, refractive_index=1):
+    """Impulse-response branch (hcipy/propagation/angular_spectrum.py:55-69,
+    hcipy/propagation/fresnel.py:50-67): the impulse response supersampled on the FFT grid of the
+    internal grid (the padded spatial grid), then FFT'd onto the internal grid."""
+    k = 2 * np.pi / wavelength * refractive_index
+    enlarged = make_fft_grid(internal_grid)
+    fft = FFT(enlarged)
+
+    if kind == 'angular_spectrum':
+        def ir(g):
+            r_squared = g.x ** 2 + g.y ** 2 + distance ** 2
+            r = np.sqrt(r_squared)
+            cos_theta = distance / r
+            return cos_theta / (2 * np.pi) * np.exp(1j * k * r) * (1 / r_squared - 1j * k / r)
+    else:
+        phase_factor = np.exp(1j * k * distance) / (1j * wavelength * distance)
+
+        def ir(g):
+            return phase_factor * np.exp(0.5j * k / distance * (g.x ** 2 + g.y ** 2))
+
+    return fft.forward(evaluate_supersampled(ir, enlarged, num_oversampling))
+
+
 def angular_spectrum_forward(field, input_grid, wavelength, distance, num_oversampling=2, adjoint=False):
+    """AngularSpectrumPropagator (angular_spectrum.py:45-80): native transfer function, or the
+    impulse-response branch when the grid step is below lambda |z| / L."""
     internal = make_fft_grid(input_grid, 2, 1)
-    tf = angular_spectrum_tf(internal, wavelength, distance, num_oversampling)
+    if angular_spectrum_uses_native_tf(input_grid, wavelength, distance):
+        tf = angular_spectrum_tf(internal, wavelength, distance, num_oversampling)
+    else:
+        tf = impulse_response_tf('angular_spectrum', internal, wavelength, distance, num_oversampling)
     return fourier_filter(field, input_grid, tf, q=2, adjoint=adjoint)
 
 
 def fresnel_forward(field, input_grid, wavelength, distance, num_oversampling=2, zero_padding=2, adjoint=False):
+    """FresnelPropagator (fresnel.py:43-82), with the same branch rule."""
     internal = make_fft_grid(input_grid, zero_padding, 1)
-    tf = fresnel_tf(internal, wavelength, distance, num_oversampling)
+    if angular_spectrum_uses_native_tf(input_grid, wavelength, distance):
+        tf = fresnel_tf(internal, wavelength, distance, num_oversampling)
+    else:
+        tf = impulse_response_tf('fresnel', internal, wavelength, distance, num_oversampling)
     return fourier_filter(field, input_grid, tf, q=zero_padding, adjoint=adjoint)
 
 

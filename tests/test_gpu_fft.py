@@ -41,11 +41,14 @@ def test_fourier_filter_random_tf(n, q, dtype):
 
 @pytest.mark.parametrize('which', ['angular_spectrum', 'fresnel'])
 @pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
-def test_nearfield_propagators_vs_oracle(which, dtype):
+@pytest.mark.parametrize('rel', [0.2, 5])
+def test_nearfield_propagators_vs_oracle(which, dtype, rel):
+    """rel = 0.2: transfer-function branch; rel = 5: impulse-response branch (the grid step is
+    below lambda z / L)."""
     n, wl, a, b = 256, 500e-9, 0.001, 0.001
     g = ha.make_pupil_grid(n, [16 * a, 16 * b])
     zc = np.min(g.delta * np.array([16 * a, 16 * b]) / wl)
-    z = 0.2 * zc
+    z = rel * zc
     ap = o.evaluate_supersampled(lambda gg: o.rectangular_aperture(gg, [2 * a, 2 * b]), og(g), 2)
     cls = ha.AngularSpectrumPropagator if which == 'angular_spectrum' else ha.FresnelPropagator
     prop = cls(g, z)
@@ -64,14 +67,15 @@ def test_nearfield_propagators_vs_oracle(which, dtype):
     assert peak_rel_field(bwd, ref_b) < tol
 
 
-def test_nearfield_rectangle_closed_form():
-    """Fresnel-integral closed form of a rectangle, bar 0.02 at z = 0.2 z_c
-    (tests/test_propagation.py:64-138)."""
+@pytest.mark.parametrize('rel,thr', [(0.2, 0.02), (5, 0.01)])
+def test_nearfield_rectangle_closed_form(rel, thr):
+    """Fresnel-integral closed form of a rectangle, bars 0.02 at z = 0.2 z_c and 0.01 at 5 z_c
+    (impulse-response branch) (tests/test_propagation.py:64-138)."""
     import scipy.special
     n, wl, a, b = 512, 500e-9, 0.001, 0.001
     g = ha.make_pupil_grid(n, [16 * a, 16 * b])
     zc = np.min(g.delta * np.array([16 * a, 16 * b]) / wl)
-    z = 0.2 * zc
+    z = rel * zc
     ap = ha.evaluate_supersampled(ha.make_rectangular_aperture([2 * a, 2 * b]), g, 2)
     for cls in [ha.FresnelPropagator, ha.AngularSpectrumPropagator]:
         img = ha.subsample_field(cls(g, z).forward(ha.Wavefront(ap, wl)).intensity, 8)
@@ -88,7 +92,9 @@ def test_nearfield_rectangle_closed_form():
             return np.abs(r) ** 2
         ref = o.evaluate_supersampled(gen, og(g), 2)
         ref = ref.reshape(n // 8, 8, n // 8, 8).mean(axis=(1, 3)).ravel()
-        assert np.abs(host(img) - ref).max() < 0.02
+        err = np.abs(host(img) - ref).max()
+        say('%s z = %g z_c: closed-form error %.3e' % (cls.__name__, rel, err))
+        assert err < thr
 
 
 def test_angular_spectrum_1024_1km():
