@@ -82,9 +82,9 @@ __device__ inline double poke(const DmGeom& g, double qx, double qy, double mx, 
 
 // surface[b][pixel] = sum_a w[b][a] IF_a(pixel)
 __global__ __launch_bounds__(256) void dm_surface_kernel(DmGeom g, const double* __restrict__ w,
-                                                         double* __restrict__ surface) {
+                                                         double* __restrict__ surface, int64_t b_off) {
   const int64_t npix = g.nx * g.ny;
-  const int64_t b = blockIdx.y;
+  const int64_t b = blockIdx.y + b_off;
   const double* wb = w + b * g.n_act * g.n_act;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = p % g.nx, j = p / g.nx;
@@ -113,10 +113,10 @@ __global__ __launch_bounds__(256) void dm_surface_kernel(DmGeom g, const double*
 // grad_w[b][a] = sum_pixel grad_s[b][pixel] IF_a(pixel): one workgroup per (actuator, batch item),
 // a fixed thread -> pixel assignment and a fixed reduction tree (deterministic).
 __global__ __launch_bounds__(256) void dm_adjoint_kernel(DmGeom g, const double* __restrict__ gs,
-                                                         double* __restrict__ gw) {
+                                                         double* __restrict__ gw, int64_t b_off) {
   __shared__ double red[4];
   const int64_t a = blockIdx.x;
-  const int64_t b = blockIdx.y;
+  const int64_t b = blockIdx.y + b_off;
   const int64_t npix = g.nx * g.ny;
   const double* gsb = gs + b * npix;
   const double mx = g.ax0 + (double)(a % g.n_act) * g.pitch;
@@ -170,28 +170,35 @@ extern "C" {
 int hcipy_dm_gaussian_surface(const hcipy_dm_gaussian_desc* desc, const double* actuators, double* surface,
                               int64_t batch, void* stream) {
   if (int st = check_desc(desc)) return st;
-  HCIPY_REQUIRE(batch >= 0 && batch <= 65535, "bad batch");
+  HCIPY_REQUIRE(batch >= 0, "bad batch");
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(actuators != nullptr && surface != nullptr, "NULL pointer");
   const DmGeom g = make_geom(*desc);
   const int64_t npix = g.nx * g.ny;
-  const dim3 grid((unsigned)std::min<int64_t>(ceil_div(npix, 256), 8192), (unsigned)batch);
-  hipLaunchKernelGGL(dm_surface_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, actuators, surface);
-  return check_hip(hipGetLastError(), "dm_surface_kernel");
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const dim3 grid((unsigned)std::min<int64_t>(ceil_div(npix, 256), 8192), (unsigned)std::min<int64_t>(65535, batch - b0));
+    hipLaunchKernelGGL(dm_surface_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, actuators, surface,
+                       b0);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
 }
 
 int hcipy_dm_gaussian_surface_adjoint(const hcipy_dm_gaussian_desc* desc, const double* grad_surface,
                                       double* grad_actuators, int64_t batch, void* stream) {
   if (int st = check_desc(desc)) return st;
-  HCIPY_REQUIRE(batch >= 0 && batch <= 65535, "bad batch");
+  HCIPY_REQUIRE(batch >= 0, "bad batch");
   HCIPY_REQUIRE(desc->n_act * desc->n_act <= INT32_MAX, "too many actuators");
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(grad_surface != nullptr && grad_actuators != nullptr, "NULL pointer");
   const DmGeom g = make_geom(*desc);
-  const dim3 grid((unsigned)(g.n_act * g.n_act), (unsigned)batch);
-  hipLaunchKernelGGL(dm_adjoint_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, grad_surface,
-                     grad_actuators);
-  return check_hip(hipGetLastError(), "dm_adjoint_kernel");
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const dim3 grid((unsigned)(g.n_act * g.n_act), (unsigned)std::min<int64_t>(65535, batch - b0));
+    hipLaunchKernelGGL(dm_adjoint_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, grad_surface,
+                       grad_actuators, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
 }
 
 }  // extern "C"

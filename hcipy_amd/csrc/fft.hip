@@ -659,7 +659,7 @@ int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* b
 int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, void* out, int64_t batch,
                          void* workspace, size_t workspace_bytes, void* stream) {
   HCIPY_REQUIRE(plan != nullptr, "NULL plan");
-  HCIPY_REQUIRE(batch >= 0 && batch <= 65535, "batch out of range");
+  HCIPY_REQUIRE(batch >= 0, "batch out of range");
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(in != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
   size_t need = 0;
@@ -669,9 +669,19 @@ int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, vo
   HCIPY_HIP(hipGetDevice(&cur));
   HCIPY_REQUIRE(cur == plan->p.device, "plan was created on device %d but device %d is current", plan->p.device, cur);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (plan->p.dtype == HCIPY_DTYPE_C64)
-    return run_filter<float>(plan->p, adjoint ? 1 : 0, in, out, batch, static_cast<char*>(workspace), s);
-  return run_filter<double>(plan->p, adjoint ? 1 : 0, in, out, batch, static_cast<char*>(workspace), s);
+  // fields beyond the 65 535 slices of gridDim.y run as further launches on the same workspace
+  const FilterPlan& p = plan->p;
+  const size_t field_bytes = (size_t)p.nx * p.ny * cbytes(p.dtype);
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const int64_t nb = std::min<int64_t>(65535, batch - b0);
+    const void* ib = static_cast<const char*>(in) + (size_t)b0 * field_bytes;
+    void* ob = static_cast<char*>(out) + (size_t)b0 * field_bytes;
+    const int st = p.dtype == HCIPY_DTYPE_C64
+                       ? run_filter<float>(p, adjoint ? 1 : 0, ib, ob, nb, static_cast<char*>(workspace), s)
+                       : run_filter<double>(p, adjoint ? 1 : 0, ib, ob, nb, static_cast<char*>(workspace), s);
+    if (st != kOk) return st;
+  }
+  return kOk;
 }
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 //
 // All kernels are HBM-bound streaming kernels: one pass, coalesced along the flattened field,
 // a batch of fields on gridDim.y with the modulating array broadcast (stride 0) or per field.
+// Batches beyond the 65 535 slices of gridDim.y run as several launches (field offset b_off).
 #include "common.hpp"
 #include "../../include/hcipy_amd.h"
 
@@ -49,8 +50,8 @@ __device__ inline void store_c<C128>(void* p, int64_t i, double re, double im) {
 // for complex128 fields.
 template <typename C, typename R>
 __global__ void sub_mul_kernel(const C* __restrict__ x, const C* __restrict__ y, const void* __restrict__ a,
-                               int a_dtype, int64_t a_stride, int conj, C* __restrict__ out, int64_t n) {
-  const int64_t b = blockIdx.y;
+                               int a_dtype, int64_t a_stride, int conj, C* __restrict__ out, int64_t n, int64_t b_off) {
+  const int64_t b = blockIdx.y + b_off;
   const C* xb = x + b * n;
   const C* yb = y ? y + b * n : nullptr;
   C* ob = out + b * n;
@@ -76,8 +77,8 @@ __global__ void sub_mul_kernel(const C* __restrict__ x, const C* __restrict__ y,
 // out = x * exp(i * scale * phi), phase evaluated in FP64.
 template <typename C, typename R>
 __global__ void phase_kernel(const C* __restrict__ x, const void* __restrict__ phi, int phi_dtype, int64_t phi_stride,
-                             double scale, C* __restrict__ out, int64_t n) {
-  const int64_t b = blockIdx.y;
+                             double scale, C* __restrict__ out, int64_t n, int64_t b_off) {
+  const int64_t b = blockIdx.y + b_off;
   const C* xb = x + b * n;
   C* ob = out + b * n;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -102,9 +103,9 @@ __global__ void intensity_kernel(const C* __restrict__ x, R* __restrict__ out, i
 template <typename C>
 __global__ __launch_bounds__(256) void power_partial_kernel(const C* __restrict__ x, const void* __restrict__ mask,
                                                             int mask_dtype, int64_t mask_stride, double weight,
-                                                            double* __restrict__ partial, int64_t n) {
+                                                            double* __restrict__ partial, int64_t n, int64_t b_off) {
   __shared__ double red[4];
-  const int64_t b = blockIdx.y;
+  const int64_t b = blockIdx.y + b_off;
   const C* xb = x + b * n;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -135,9 +136,9 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double* __restr
 template <typename R>
 __global__ __launch_bounds__(256) void masked_partial_kernel(const R* __restrict__ x, const void* __restrict__ mask,
                                                              int mask_dtype, int64_t mask_stride,
-                                                             double* __restrict__ partial, int64_t n) {
+                                                             double* __restrict__ partial, int64_t n, int64_t b_off) {
   __shared__ double red[2][4];
-  const int64_t b = blockIdx.y;
+  const int64_t b = blockIdx.y + b_off;
   const R* xb = x + b * n;
   double acc = 0.0, cnt = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -207,8 +208,8 @@ __global__ void phase_adjoint_kernel(const C* __restrict__ y, const C* __restric
 // Adjoint of I = |x|^2: out = 2 g x (g real, broadcast when g_stride is 0).
 template <typename C, typename R>
 __global__ void intensity_adjoint_kernel(const C* __restrict__ x, const void* __restrict__ g, int g_dtype,
-                                         int64_t g_stride, C* __restrict__ out, int64_t n) {
-  const int64_t b = blockIdx.y;
+                                         int64_t g_stride, C* __restrict__ out, int64_t n, int64_t b_off) {
+  const int64_t b = blockIdx.y + b_off;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double gv, dummy;
     load_any(g, g_dtype, b * g_stride + i, gv, dummy);
@@ -221,8 +222,8 @@ __global__ void intensity_adjoint_kernel(const C* __restrict__ x, const void* __
 template <typename R>
 __global__ void masked_mean_adjoint_kernel(const void* __restrict__ mask, int mask_dtype, int64_t mask_stride,
                                            const double* __restrict__ g, const double* __restrict__ count,
-                                           R* __restrict__ out, int64_t n) {
-  const int64_t b = blockIdx.y;
+                                           R* __restrict__ out, int64_t n, int64_t b_off) {
+  const int64_t b = blockIdx.y + b_off;
   const double v = g[b] / count[b];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double m = 1.0, dummy;
@@ -234,10 +235,15 @@ __global__ void masked_mean_adjoint_kernel(const void* __restrict__ mask, int ma
 constexpr int kThreads = 256;
 constexpr int kPowerBlocks = 256;
 
+constexpr int64_t kMaxGridY = 65535;
+
 dim3 grid_for(int64_t n, int64_t batch) {
   const int64_t bx = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kThreads), 4096));
   return dim3((unsigned)bx, (unsigned)batch);
 }
+
+// Number of fields of the launch starting at field b0.
+int64_t chunk(int64_t batch, int64_t b0) { return std::min<int64_t>(kMaxGridY, batch - b0); }
 
 bool is_complex(int d) { return d == HCIPY_DTYPE_C64 || d == HCIPY_DTYPE_C128; }
 bool is_valid(int d) { return d >= HCIPY_DTYPE_C64 && d <= HCIPY_DTYPE_F64; }
@@ -253,36 +259,42 @@ int hcipy_pw_sub_mul(const void* x, const void* y, const void* a, int a_dtype, i
                      void* out, int field_dtype, int64_t n, int64_t batch, void* stream) {
   HCIPY_REQUIRE(field_dtype == HCIPY_DTYPE_C64 || field_dtype == HCIPY_DTYPE_C128, "field must be complex");
   HCIPY_REQUIRE(a == nullptr || is_valid(a_dtype), "bad dtype of the multiplier");
-  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
   if (n == 0 || batch == 0) return kOk;
   HCIPY_REQUIRE(x != nullptr && out != nullptr, "NULL pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 g = grid_for(n, batch);
-  if (field_dtype == HCIPY_DTYPE_C64)
-    hipLaunchKernelGGL((sub_mul_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, (const C64*)y, a, a_dtype,
-                       a_stride, conjugate_a, (C64*)out, n);
-  else
-    hipLaunchKernelGGL((sub_mul_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, (const C128*)y, a,
-                       a_dtype, a_stride, conjugate_a, (C128*)out, n);
-  return check_hip(hipGetLastError(), "sub_mul_kernel");
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const dim3 g = grid_for(n, chunk(batch, b0));
+    if (field_dtype == HCIPY_DTYPE_C64)
+      hipLaunchKernelGGL((sub_mul_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, (const C64*)y, a,
+                         a_dtype, a_stride, conjugate_a, (C64*)out, n, b0);
+    else
+      hipLaunchKernelGGL((sub_mul_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, (const C128*)y, a,
+                         a_dtype, a_stride, conjugate_a, (C128*)out, n, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
 }
 
 int hcipy_pw_phase(const void* x, const void* phi, int phi_dtype, int64_t phi_stride, double scale, void* out,
                    int field_dtype, int64_t n, int64_t batch, void* stream) {
   HCIPY_REQUIRE(field_dtype == HCIPY_DTYPE_C64 || field_dtype == HCIPY_DTYPE_C128, "field must be complex");
   HCIPY_REQUIRE(phi_dtype == HCIPY_DTYPE_F32 || phi_dtype == HCIPY_DTYPE_F64, "phase must be real");
-  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
   if (n == 0 || batch == 0) return kOk;
   HCIPY_REQUIRE(x != nullptr && out != nullptr && phi != nullptr, "NULL pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 g = grid_for(n, batch);
-  if (field_dtype == HCIPY_DTYPE_C64)
-    hipLaunchKernelGGL((phase_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, phi, phi_dtype, phi_stride,
-                       scale, (C64*)out, n);
-  else
-    hipLaunchKernelGGL((phase_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, phi, phi_dtype,
-                       phi_stride, scale, (C128*)out, n);
-  return check_hip(hipGetLastError(), "phase_kernel");
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const dim3 g = grid_for(n, chunk(batch, b0));
+    if (field_dtype == HCIPY_DTYPE_C64)
+      hipLaunchKernelGGL((phase_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, phi, phi_dtype,
+                         phi_stride, scale, (C64*)out, n, b0);
+    else
+      hipLaunchKernelGGL((phase_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, phi, phi_dtype,
+                         phi_stride, scale, (C128*)out, n, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
 }
 
 int hcipy_pw_intensity(const void* x, int field_dtype, void* out, int64_t total, void* stream) {
@@ -309,19 +321,21 @@ int hcipy_pw_power(const void* x, int field_dtype, const void* mask, int mask_dt
                    void* stream) {
   HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
   HCIPY_REQUIRE(mask == nullptr || is_valid(mask_dtype), "bad mask dtype");
-  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
   HCIPY_REQUIRE(workspace_bytes >= (size_t)batch * kPowerBlocks * sizeof(double), "workspace too small");
   if (batch == 0) return kOk;
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* partial = static_cast<double*>(workspace);
-  const dim3 g(kPowerBlocks, (unsigned)batch);
-  if (field_dtype == HCIPY_DTYPE_C64)
-    hipLaunchKernelGGL((power_partial_kernel<C64>), g, dim3(kThreads), 0, s, (const C64*)x, mask, mask_dtype,
-                       mask_stride, weight, partial, n);
-  else
-    hipLaunchKernelGGL((power_partial_kernel<C128>), g, dim3(kThreads), 0, s, (const C128*)x, mask, mask_dtype,
-                       mask_stride, weight, partial, n);
-  HCIPY_HIP(hipGetLastError());
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const dim3 g(kPowerBlocks, (unsigned)chunk(batch, b0));
+    if (field_dtype == HCIPY_DTYPE_C64)
+      hipLaunchKernelGGL((power_partial_kernel<C64>), g, dim3(kThreads), 0, s, (const C64*)x, mask, mask_dtype,
+                         mask_stride, weight, partial, n, b0);
+    else
+      hipLaunchKernelGGL((power_partial_kernel<C128>), g, dim3(kThreads), 0, s, (const C128*)x, mask, mask_dtype,
+                         mask_stride, weight, partial, n, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)batch), dim3(kThreads), 0, s, partial, kPowerBlocks, out);
   return check_hip(hipGetLastError(), "sum_partials_kernel");
 }
@@ -337,20 +351,22 @@ int hcipy_pw_masked_mean(const void* x, int dtype, const void* mask, int mask_dt
                          void* stream) {
   HCIPY_REQUIRE(dtype == HCIPY_DTYPE_F32 || dtype == HCIPY_DTYPE_F64, "values must be real");
   HCIPY_REQUIRE(mask == nullptr || is_valid(mask_dtype), "bad mask dtype");
-  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
   HCIPY_REQUIRE(workspace_bytes >= (size_t)batch * kPowerBlocks * 2 * sizeof(double), "workspace too small");
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(x != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* partial = static_cast<double*>(workspace);
-  const dim3 g(kPowerBlocks, (unsigned)batch);
-  if (dtype == HCIPY_DTYPE_F32)
-    hipLaunchKernelGGL(masked_partial_kernel<float>, g, dim3(kThreads), 0, s, (const float*)x, mask, mask_dtype,
-                       mask_stride, partial, n);
-  else
-    hipLaunchKernelGGL(masked_partial_kernel<double>, g, dim3(kThreads), 0, s, (const double*)x, mask, mask_dtype,
-                       mask_stride, partial, n);
-  HCIPY_HIP(hipGetLastError());
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const dim3 g(kPowerBlocks, (unsigned)chunk(batch, b0));
+    if (dtype == HCIPY_DTYPE_F32)
+      hipLaunchKernelGGL(masked_partial_kernel<float>, g, dim3(kThreads), 0, s, (const float*)x, mask, mask_dtype,
+                         mask_stride, partial, n, b0);
+    else
+      hipLaunchKernelGGL(masked_partial_kernel<double>, g, dim3(kThreads), 0, s, (const double*)x, mask, mask_dtype,
+                         mask_stride, partial, n, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(mean_partials_kernel, dim3((unsigned)batch), dim3(kThreads), 0, s, partial, kPowerBlocks, out,
                      count);
   return check_hip(hipGetLastError(), "mean_partials_kernel");
@@ -361,18 +377,21 @@ int hcipy_pw_masked_mean_adjoint(const void* mask, int mask_dtype, int64_t mask_
                                  void* stream) {
   HCIPY_REQUIRE(out_dtype == HCIPY_DTYPE_F32 || out_dtype == HCIPY_DTYPE_F64, "gradient must be real");
   HCIPY_REQUIRE(mask == nullptr || is_valid(mask_dtype), "bad mask dtype");
-  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
   if (n == 0 || batch == 0) return kOk;
   HCIPY_REQUIRE(g != nullptr && count != nullptr && out != nullptr, "NULL pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 grid = grid_for(n, batch);
-  if (out_dtype == HCIPY_DTYPE_F32)
-    hipLaunchKernelGGL(masked_mean_adjoint_kernel<float>, grid, dim3(kThreads), 0, s, mask, mask_dtype, mask_stride, g,
-                       count, (float*)out, n);
-  else
-    hipLaunchKernelGGL(masked_mean_adjoint_kernel<double>, grid, dim3(kThreads), 0, s, mask, mask_dtype, mask_stride,
-                       g, count, (double*)out, n);
-  return check_hip(hipGetLastError(), "masked_mean_adjoint_kernel");
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const dim3 grid = grid_for(n, chunk(batch, b0));
+    if (out_dtype == HCIPY_DTYPE_F32)
+      hipLaunchKernelGGL(masked_mean_adjoint_kernel<float>, grid, dim3(kThreads), 0, s, mask, mask_dtype, mask_stride,
+                         g, count, (float*)out, n, b0);
+    else
+      hipLaunchKernelGGL(masked_mean_adjoint_kernel<double>, grid, dim3(kThreads), 0, s, mask, mask_dtype,
+                         mask_stride, g, count, (double*)out, n, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
 }
 
 int hcipy_pw_phase_adjoint(const void* y, const void* g, int field_dtype, double scale, double* out,
@@ -398,18 +417,21 @@ int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, in
                                int64_t n, int64_t batch, void* stream) {
   HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
   HCIPY_REQUIRE(g_dtype == HCIPY_DTYPE_F32 || g_dtype == HCIPY_DTYPE_F64, "gradient of the intensity must be real");
-  HCIPY_REQUIRE(n >= 0 && batch >= 0 && batch <= 65535, "bad size");
+  HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
   if (n == 0 || batch == 0) return kOk;
   HCIPY_REQUIRE(x != nullptr && g != nullptr && out != nullptr, "NULL pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 grid = grid_for(n, batch);
-  if (field_dtype == HCIPY_DTYPE_C64)
-    hipLaunchKernelGGL((intensity_adjoint_kernel<C64, float>), grid, dim3(kThreads), 0, s, (const C64*)x, g, g_dtype,
-                       g_stride, (C64*)out, n);
-  else
-    hipLaunchKernelGGL((intensity_adjoint_kernel<C128, double>), grid, dim3(kThreads), 0, s, (const C128*)x, g,
-                       g_dtype, g_stride, (C128*)out, n);
-  return check_hip(hipGetLastError(), "intensity_adjoint_kernel");
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const dim3 grid = grid_for(n, chunk(batch, b0));
+    if (field_dtype == HCIPY_DTYPE_C64)
+      hipLaunchKernelGGL((intensity_adjoint_kernel<C64, float>), grid, dim3(kThreads), 0, s, (const C64*)x, g,
+                         g_dtype, g_stride, (C64*)out, n, b0);
+    else
+      hipLaunchKernelGGL((intensity_adjoint_kernel<C128, double>), grid, dim3(kThreads), 0, s, (const C128*)x, g,
+                         g_dtype, g_stride, (C128*)out, n, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
 }
 
 }  // extern "C"

@@ -145,7 +145,7 @@ class FourierFilter:
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         NY, NX = self.internal_grid.shape
-        if _pow2(NX) and _pow2(NY) and batch <= 65535:
+        if _pow2(NX) and _pow2(NY):
             key = (cdt, t.device.index)
             plan = self._plans.get(key)
             if plan is None:

@@ -318,7 +318,7 @@ int hcipy_nearfield_impulse_response(int kind, int nx, int ny, double delta_x, d
 /* Bytes of workspace hcipy_filter_execute needs for `batch` fields. */
 int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* bytes);
 
-/* out[b] = crop(ifft2(fft2(pad(in[b])) * tf)) (conj(tf) when adjoint != 0); batch <= 65535.
+/* out[b] = crop(ifft2(fft2(pad(in[b])) * tf)) (conj(tf) when adjoint != 0), any batch size.
  * in and out are batch x ny x nx complex arrays of the plan's dtype and must not overlap. */
 int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, void* out, int64_t batch,
                          void* workspace, size_t workspace_bytes, void* stream);
