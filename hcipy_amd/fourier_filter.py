@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, pointwise
 from .field import Field
 from .fourier import MatrixFourierTransform, complex_dtype_for, make_fft_grid
 
@@ -37,7 +37,10 @@ def _pow2(n):
 
 
 class _FilterPlan:
-    def __init__(self, filt, cdt, device):
+    """One device plan: the filter geometry and one scalar transfer function (``tf``, if given, is
+    a centred device tensor used instead of the filter's own transfer function)."""
+
+    def __init__(self, filt, cdt, device, tf=None):
         lib = _lib.load()
         self.device = device
         self.dtype = cdt
@@ -49,7 +52,8 @@ class _FilterPlan:
             stream = _lib.stream_ptr(device)
             _lib.check(lib.hcipy_filter_plan_create(ctypes.byref(h), nx, ny, NX, NY, dcode, stream), 'hcipy_filter_plan_create')
             self.handle = h
-            tf = filt.transfer_function
+            if tf is None:
+                tf = filt.transfer_function
             if isinstance(tf, NearFieldTransferFunction):
                 g = filt.internal_grid
                 st = lib.hcipy_filter_nearfield_transfer_function(
@@ -57,7 +61,9 @@ class _FilterPlan:
                     float(g.zero[1]), tf.wavenumber, tf.distance, tf.oversampling, stream)
                 _lib.check(st, 'hcipy_filter_nearfield_transfer_function')
             else:
-                t = filt.evaluate_transfer_function(device).to(cdt).contiguous()
+                t = tf if isinstance(tf, torch.Tensor) and not isinstance(tf, Field) and tf.dim() == 1 \
+                    else filt.evaluate_transfer_function(device)
+                t = t.to(device=device, dtype=cdt).contiguous()
                 _lib.check(lib.hcipy_filter_set_transfer_function(h, _lib.dptr(t), dcode, stream),
                            'hcipy_filter_set_transfer_function')
                 torch.cuda.current_stream(device).synchronize()
@@ -102,7 +108,12 @@ class FourierFilter:
     """Filter in the Fourier domain with zero padding q (fourier_operations.py:8-151).
 
     ``transfer_function`` is a Field on the internal grid (centred order), a callable that
-    maps the internal grid to such a Field, or a NearFieldTransferFunction.
+    maps the internal grid to such a Field, or a NearFieldTransferFunction.  A transfer function
+    of shape (p, q, internal size) is a matrix field (fourier_operations.py:120-131): the input
+    then carries q components in its second-to-last dimension and the output p, with
+    ``out_i = sum_j filter(tf_ij)(in_j)`` (the adjoint uses the conjugate transpose).  Each
+    (i, j) runs as a scalar filter; the terms are added by fused subtract passes of filters with
+    the negated transfer function.
     """
 
     def __init__(self, input_grid, transfer_function, q=1):
@@ -115,6 +126,7 @@ class FourierFilter:
         self.transfer_function = transfer_function
         self._plans = {}
         self._mft = None
+        self._matrix_tf = None
 
     def evaluate_transfer_function(self, device):
         tf = self.transfer_function
@@ -123,9 +135,47 @@ class FourierFilter:
         if not isinstance(tf, torch.Tensor):
             tf = torch.as_tensor(np.asarray(tf), device=device)
         tf = tf.as_subclass(torch.Tensor) if isinstance(tf, Field) else tf
-        if tf.dim() != 1 or tf.shape[0] != self.internal_grid.size:
-            raise ValueError('only scalar transfer functions on the internal grid are supported')
+        if tf.dim() not in (1, 3) or tf.shape[-1] != self.internal_grid.size:
+            raise ValueError('the transfer function must be a scalar or matrix field on the internal grid')
         return tf.to(device)
+
+    def _is_matrix(self, device):
+        tf = self.transfer_function
+        if isinstance(tf, NearFieldTransferFunction):
+            return False
+        if self._matrix_tf is None:
+            t = self.evaluate_transfer_function(device)
+            self._matrix_tf = t if t.dim() == 3 else False
+        return self._matrix_tf is not False
+
+    def _run_matrix(self, t, adjoint):
+        """Matrix transfer function: out_i = sum_j F_ij(in_j), adjoint in_j = sum_i F_ij^H(out_i)."""
+        tf = self._matrix_tf
+        p, q = int(tf.shape[0]), int(tf.shape[1])
+        n_in, n_out = (p, q) if adjoint else (q, p)
+        if t.dim() < 2 or t.shape[-2] != n_in:
+            raise ValueError('a matrix transfer function of shape (%d, %d) needs %d field components, got shape %s'
+                             % (p, q, n_in, tuple(t.shape)))
+        cdt, dev_ = t.dtype, t.device
+        outs = []
+        for o in range(n_out):
+            acc = None
+            for c in range(n_in):
+                i, j = (c, o) if adjoint else (o, c)
+                neg = acc is not None
+                key = (cdt, dev_.index, i, j, neg)
+                plan = self._plans.get(key)
+                if plan is None:
+                    plan = _FilterPlan(self, cdt, dev_, tf=(-tf[i, j] if neg else tf[i, j]))
+                    self._plans[key] = plan
+                xin = t[..., c, :].contiguous()
+                lead = tuple(xin.shape[:-1])
+                batch = int(np.prod(lead)) if lead else 1
+                term = torch.empty_like(xin)
+                plan.execute(adjoint, xin, term, batch)
+                acc = term if acc is None else pointwise._sub_mul_raw(acc, term, None, 0, False)
+            outs.append(acc)
+        return torch.stack(outs, dim=-2)
 
     def _operation(self, field, adjoint):
         grid = self.input_grid
@@ -145,6 +195,10 @@ class FourierFilter:
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         NY, NX = self.internal_grid.shape
+        if self._is_matrix(t.device):
+            if not (_pow2(NX) and _pow2(NY)):
+                raise ValueError('matrix transfer functions need a power-of-two internal grid')
+            return self._run_matrix(t, adjoint)
         if _pow2(NX) and _pow2(NY):
             key = (cdt, t.device.index)
             plan = self._plans.get(key)

@@ -398,15 +398,33 @@ class FFT:
 
 
 def fourier_filter(field, input_grid, tf_internal, q=1, adjoint=False):
-    """FourierFilter._operation for a scalar transfer function (fourier_operations.py:89-151).
+    """FourierFilter._operation (fourier_operations.py:89-151).
 
     ``tf_internal`` is the transfer function sampled on the internal (FFT output, q, fov=1) grid
     in that grid's natural (centred) order; it is ifftshifted here as in :41-52.
-    ``field`` may carry leading batch dimensions.
+    ``field`` may carry leading batch dimensions.  A transfer function of shape (p, q, size) is a
+    matrix field (:120-131): the field's second-to-last dimension holds its q components and
+    the result has p (the adjoint uses the conjugate transpose).
     """
     fft = FFT(input_grid, q)
     internal_grid = fft.output_grid
-    tf = np.fft.ifftshift(np.asarray(tf_internal).reshape(internal_grid.shape))
+    tf_arr = np.asarray(tf_internal)
+    if tf_arr.ndim == 3:
+        tf = np.fft.ifftshift(tf_arr.reshape(tf_arr.shape[:2] + internal_grid.shape), axes=(-2, -1))
+        if adjoint:
+            tf = np.conj(np.swapaxes(tf, 0, 1))
+        field = np.asarray(field)
+        lead = field.shape[:-2]
+        f = field.reshape(field.shape[:-1] + input_grid.shape)
+        arr = np.zeros(field.shape[:-1] + internal_grid.shape, dtype=np.result_type(field.dtype, np.complex64))
+        c = (slice(None),) * (len(lead) + 1) + (fft.cutout_input or (slice(None), slice(None)))
+        arr[c] = f
+        F = np.fft.fftn(arr, axes=(-2, -1))
+        G = np.einsum('ijyx,...jyx->...iyx', tf.astype(arr.dtype), F)  # field_dot per frequency
+        r = np.fft.ifftn(G, axes=(-2, -1))
+        r = r[(slice(None),) * (len(lead) + 1) + (fft.cutout_input or (slice(None), slice(None)))]
+        return r.reshape(r.shape[:-2] + (-1,)).astype(arr.dtype)
+    tf = np.fft.ifftshift(tf_arr.reshape(internal_grid.shape))
     field = np.asarray(field)
     lead = field.shape[:-1]
     f = field.reshape(lead + input_grid.shape)

@@ -388,3 +388,25 @@ def test_multiscale_levels_add_up_to_the_mask():
     a = np.vdot(U, o.multiscale_forward(E, pg, grids, masks))
     b = np.vdot(o.multiscale_forward(U, pg, grids, masks, adjoint=True), E)
     assert abs(a - b) < 1e-11 * abs(a)
+
+
+@pytest.mark.parametrize('n,q', [(16, 1), (17, 2), (16, 3)])
+def test_matrix_fourier_filter_equals_fft_chain(n, q):
+    """Matrix transfer function (3 x 3): FourierFilter == FFT.backward(field_dot(tf, FFT.forward(f)))
+    and the adjoint with the conjugate transpose (tests/test_fourier.py:273-309)."""
+    g = o.make_pupil_grid(n, 2)
+    fft = o.FFT(g, q)
+    rng = np.random.default_rng(7)
+    tf = rng.standard_normal((3, 3, fft.output_grid.size)) + 1j * rng.standard_normal((3, 3, fft.output_grid.size))
+    f = rng.standard_normal((3, g.size)) + 1j * rng.standard_normal((3, g.size))
+    a = o.fourier_filter(f, g, tf, q)
+    F = np.array([fft.forward(fj) for fj in f])
+    b = np.array([fft.backward(np.sum(tf[i] * F, axis=0)) for i in range(3)])
+    assert np.abs(a - b).max() < 1e-10
+    a = o.fourier_filter(f, g, tf, q, adjoint=True)
+    b = np.array([fft.backward(np.sum(np.conj(tf[:, i]) * F, axis=0)) for i in range(3)])
+    assert np.abs(a - b).max() < 1e-10
+    # a batch of vector fields
+    fb = np.stack([f, 2 * f])
+    ab = o.fourier_filter(fb, g, tf, q)
+    assert np.abs(ab[1] - 2 * o.fourier_filter(f, g, tf, q)).max() < 1e-10
