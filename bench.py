@@ -36,6 +36,8 @@ def parse():
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a plain 256 MiB device-to-device copy 3 times (scale of the HBM byte counters)')
     ap.add_argument('--cpu-seconds', type=float, default=2.0, help='t_max of the CPU timing loop')
+    ap.add_argument('--paths-only', action='store_true',
+                    help='only time the angular-spectrum, Lyot-chain, vortex and mixed-size grouped paths')
     return ap.parse_args()
 
 
@@ -236,6 +238,47 @@ def main():
                                             'levels': len(vc.focal_grids)}
         return out
 
+    def time_mixed_grouped(n_wl=8, batch=4, reps=10):
+        """Mixed pupil sizes (256^2, 512^2, 1024^2 -> 128^2, 256^2, 512^2 focal planes, q = 4), each at
+        n_wl wavelengths with their own matrices and a batch of fields (pokes), f16x2 intensity:
+        every transform as one grouped launch per stage, against one call per transform."""
+        rng = np.random.default_rng(6)
+        calls, per = [], []
+        for npix in (256, 512, 1024):
+            pg = ha.make_pupil_grid(npix)
+            fg = ha.make_focal_grid(4, npix // 16)
+            E = torch.as_tensor((rng.standard_normal((batch, pg.size)) + 1j * rng.standard_normal((batch, pg.size)))
+                                .astype(np.complex64), device=dev)
+            prop = ha.FraunhoferPropagator(pg, fg, precision='f16x2')
+            for wl in np.linspace(0.95, 1.05, n_wl):
+                wf = ha.Wavefront(ha.Field(E, pg), wl)
+                per.append((prop, wf))
+                d = prop.instance(wl)
+                calls.append((d.fourier_transform, _lib.FORWARD, E, complex(pg.weights) * d.norm_factor,
+                              _lib.OUTPUT_INTENSITY))
+        grp = ha.MftGroup(len(calls))
+        gflop = sum(8.0 * b * (ny * nx * nu + nv * ny * nu) for b, ny, nx, nv, nu in
+                    [(batch, n, n, n // 2, n // 2) for n in (256, 512, 1024) for _ in range(n_wl)]) / 1e9
+
+        def grouped():
+            return grp.execute(calls)
+
+        def one_by_one():
+            return [prop.forward_intensity(wf) for prop, wf in per]
+
+        sec_g, _ = time_graph(grouped, n_chunks=reps, chunk=1)
+        sec_p, _ = time_graph(one_by_one, n_chunks=reps, chunk=1)
+        return {'workload': '%d transforms: pupils 256^2/512^2/1024^2 x %d wavelengths x %d fields, f16x2 intensity'
+                            % (len(calls), n_wl, batch),
+                'grouped_ms': round(sec_g * 1e3, 3), 'one_call_per_transform_ms': round(sec_p * 1e3, 3),
+                'grouped_tflops': round(gflop / sec_g / 1e3, 1)}
+
+    if args.paths_only:
+        out = time_other_paths()
+        out['mixed_grouped'] = time_mixed_grouped()
+        print(json.dumps(out))
+        return
+
     log('timing ...')
     main_out = _lib.OUTPUT_INTENSITY
     sec_main, steps_done, kms_main, out_main = time_path(args.precision, args.steps, args.warmup, main_out)
@@ -252,7 +295,10 @@ def main():
                           'peak_rel_intensity_error': err(o), 'kernel_ms': [round(v, 5) for v in kms]}
     fp32_ref = variants.get('fp32_intensity') or variants.get('fp32_field_c64')
     broadband = None if args.no_variants else time_broadband()
-    paths = None if args.no_variants else time_other_paths()
+    paths = None
+    if not args.no_variants:
+        paths = time_other_paths()
+        paths['mixed_grouped'] = time_mixed_grouped()
 
     # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC).
     ny, nx = pupil.shape
