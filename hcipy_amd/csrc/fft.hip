@@ -73,8 +73,12 @@ __device__ void line_fft(typename Cplx<T>::V* line, int tl, const typename Cplx<
   constexpr int B4 = (N / 4 + TPL - 1) / TPL;  // radix-4 butterflies per thread
   constexpr int B2 = (N / 2 + TPL - 1) / TPL;
   const T s = (T)sign;
-  int Ns = 1;
-#pragma unroll 1
+  int Ns = 1;   // span of the sub-transforms done so far (a power of 4)
+  int lg = 0;   // log2(Ns): index arithmetic by shifts and masks
+  // FP32 lines: passes unrolled (index arithmetic folds to constants; measured faster); FP64
+  // lines keep the loop (unrolling raises register pressure and was measured slower)
+  constexpr int UNR = sizeof(T) == 4 ? NR4 : 1;
+#pragma unroll UNR
   for (int p = 0; p < NR4; ++p) {
     V v[B4][4];
 #pragma unroll
@@ -85,7 +89,7 @@ __device__ void line_fft(typename Cplx<T>::V* line, int tl, const typename Cplx<
 #pragma unroll
         for (int r = 0; r < 4; ++r) v[b][r] = line[j + r * (N / 4)];
         if (Ns > 1) {
-          const int step = N / (Ns * 4);
+          const int step = N >> (lg + 2);
 #pragma unroll
           for (int r = 1; r < 4; ++r) {
             V w = tw[(r * k * step) & (N - 1)];
@@ -109,13 +113,14 @@ __device__ void line_fft(typename Cplx<T>::V* line, int tl, const typename Cplx<
       const int j = tl + b * TPL;
       if (j < N / 4) {
         const int k = j & (Ns - 1);
-        const int o = (j / Ns) * Ns * 4 + k;
+        const int o = ((j >> lg) << (lg + 2)) + k;
 #pragma unroll
         for (int r = 0; r < 4; ++r) line[o + r * Ns] = v[b][r];
       }
     }
     __syncthreads();
     Ns *= 4;
+    lg += 2;
   }
   if constexpr (R2) {
     V v[B2][2];
@@ -127,7 +132,7 @@ __device__ void line_fft(typename Cplx<T>::V* line, int tl, const typename Cplx<
         v[b][0] = line[j];
         v[b][1] = line[j + N / 2];
         if (Ns > 1) {
-          V w = tw[(k * (N / (Ns * 2))) & (N - 1)];
+          V w = tw[(k * (N >> (lg + 1))) & (N - 1)];
           w.y *= -s;
           v[b][1] = cmul(v[b][1], w);
         }
@@ -142,7 +147,7 @@ __device__ void line_fft(typename Cplx<T>::V* line, int tl, const typename Cplx<
       const int j = tl + b * TPL;
       if (j < N / 2) {
         const int k = j & (Ns - 1);
-        const int o = (j / Ns) * Ns * 2 + k;
+        const int o = ((j >> lg) << (lg + 1)) + k;
         line[o] = v[b][0];
         line[o + Ns] = v[b][1];
       }
