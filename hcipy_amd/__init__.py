@@ -13,7 +13,8 @@ from .fourier import (ComputationalComplexity, FastFourierTransform, FourierTran
 from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, OpticalSystem, PhaseApodizer, Wavefront
 from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, FresnelPropagator
 from .fourier_filter import FourierFilter, NearFieldTransferFunction
-from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VortexCoronagraph
+from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VectorVortexCoronagraph, VortexCoronagraph
+from .polarization import CircularRetarder, JonesMatrixOpticalElement, LinearRetarder, PhaseRetarder
 from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
 from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, make_actuator_positions,
                                 make_gaussian_influence_functions)

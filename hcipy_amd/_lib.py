@@ -117,6 +117,10 @@ SIGNATURES = {
                                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                         ctypes.c_int, _vp, _vp]),
+    'hcipy_pw_field_dot': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
+                                          ctypes.c_int, _c_int64, _vp, _c_int64, _vp]),
+    'hcipy_pw_stokes': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                       _vp, _c_int64, _vp]),
     'hcipy_filter_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_filter_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, _vp, ctypes.c_size_t, _vp]),
 }

@@ -11,10 +11,12 @@ from .aperture import make_circular_aperture
 from .field import Field, make_focal_grid
 from .fourier import FastFourierTransform, MatrixFourierTransform, MftGroup
 from .fourier_filter import FourierFilter
-from .optics import Apodizer, OpticalElement, Wavefront
+from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, Wavefront
+from .polarization import phase_retarder_jones
 from .propagation import FraunhoferPropagator
 
-__all__ = ['LyotCoronagraph', 'MultiScaleCoronagraph', 'VortexCoronagraph', 'periodic_tukey_window']
+__all__ = ['LyotCoronagraph', 'MultiScaleCoronagraph', 'VortexCoronagraph', 'VectorVortexCoronagraph',
+           'periodic_tukey_window']
 
 
 class LyotCoronagraph(OpticalElement):
@@ -57,21 +59,25 @@ class LyotCoronagraph(OpticalElement):
 
     def forward(self, wavefront):
         wf_foc = self.prop.forward(wavefront)
-        wf_foc = Wavefront(self._fpm_complement(wf_foc, False), wavefront.wavelength)
+        wf_foc = Wavefront(self._fpm_complement(wf_foc, False), wavefront.wavelength,
+                           wavefront.input_stokes_vector)
         lyot = self.prop.backward(wf_foc)
         stop = self._stop()
         if self.lyot_stop is None or stop is not None:
             E = pointwise.sub_mul(wavefront.electric_field, lyot.electric_field, stop)
-            return Wavefront(Field(E, wavefront.grid), wavefront.wavelength)
-        lyot = Wavefront(pointwise.sub_mul(wavefront.electric_field, lyot.electric_field), wavefront.wavelength)
+            return Wavefront(Field(E, wavefront.grid), wavefront.wavelength, wavefront.input_stokes_vector)
+        lyot = Wavefront(pointwise.sub_mul(wavefront.electric_field, lyot.electric_field), wavefront.wavelength,
+                         wavefront.input_stokes_vector)
         return self.lyot_stop.forward(lyot)
 
     def backward(self, wavefront):
         wf = self.lyot_stop.backward(wavefront) if self.lyot_stop is not None else wavefront
         wf_foc = self.prop.forward(wf)
-        wf_foc = Wavefront(self._fpm_complement(wf_foc, True), wavefront.wavelength)
+        wf_foc = Wavefront(self._fpm_complement(wf_foc, True), wavefront.wavelength,
+                           wavefront.input_stokes_vector)
         pup = self.prop.backward(wf_foc)
-        return Wavefront(pointwise.sub_mul(wf.electric_field, pup.electric_field), wavefront.wavelength)
+        return Wavefront(pointwise.sub_mul(wf.electric_field, pup.electric_field), wavefront.wavelength,
+                         wavefront.input_stokes_vector)
 
 
 def periodic_tukey_window(n):
@@ -99,6 +105,10 @@ class MultiScaleCoronagraph(OpticalElement):
     ``backward``).  Without gradients the level MFTs run as two grouped launches (all forward
     transforms, then all adjoint transforms); the adjoint transforms are computed with a
     negated scale so that each level is added by a fused subtract pass.
+
+    ``complex_mask(grid)`` returns a scalar mask (size,) or a Jones-matrix mask (2, 2, size); a
+    Jones mask acts on the first dimension of a polarized field (field_dot), level 0 then
+    being a matrix-transfer-function FourierFilter.
     """
 
     def __init__(self, input_grid, complex_mask, lyot_stop=None, q=1024, scaling_factor=4, window_size=32,
@@ -162,7 +172,7 @@ class MultiScaleCoronagraph(OpticalElement):
             for mask, prop in rest:
                 d = prop.instance(1)
                 F = d.fourier_transform._apply(E, _lib.FORWARD, extra_alpha=d.norm_factor)
-                F = pointwise.multiply(F, mask, conjugate=conjugate)
+                F = self._apply_mask(F, mask, conjugate)
                 neg = d.fourier_transform._apply(F, _lib.BACKWARD, extra_alpha=-1 / d.norm_factor)
                 L = pointwise.sub_mul(L, neg)
             return L
@@ -180,18 +190,25 @@ class MultiScaleCoronagraph(OpticalElement):
         bwd = []
         for F, (mask, prop) in zip(Fs, rest):
             d = prop.instance(1)
-            F = pointwise.multiply(F, mask, conjugate=conjugate)
+            F = self._apply_mask(Field(F, mask.grid), mask, conjugate)
             w_uv = complex(np.asarray(d.uv_grid.weights).ravel()[0]) / (2 * np.pi) ** 2
             bwd.append((d.fourier_transform, _lib.BACKWARD, F, -w_uv / d.norm_factor, _lib.OUTPUT_FIELD))
         for neg in self._group[1].execute(bwd):
             L = pointwise.sub_mul(L, neg)
         return L
 
+    @staticmethod
+    def _apply_mask(F, mask, conjugate):
+        """M F (conj(M) F), or J F (J^H F) for a Jones-matrix mask."""
+        if mask.dim() == 3:
+            return pointwise.field_dot(mask, F, adjoint=conjugate)
+        return pointwise.multiply(F, mask, conjugate=conjugate)
+
     def forward(self, wavefront):
         """Pupil -> Lyot plane (multi_scale.py:108-145)."""
         E = wavefront.electric_field
         L = Field(self._levels(E, False), self.input_grid)
-        lyot = Wavefront(L, wavefront.wavelength)
+        lyot = Wavefront(L, wavefront.wavelength, wavefront.input_stokes_vector)
         if self.lyot_stop is not None:
             lyot = self.lyot_stop.forward(lyot)
         return lyot
@@ -202,7 +219,8 @@ class MultiScaleCoronagraph(OpticalElement):
         if self.lyot_stop is not None:
             wavefront = self.lyot_stop.backward(wavefront)
         E = wavefront.electric_field
-        return Wavefront(Field(self._levels(E, True), self.input_grid), wavefront.wavelength)
+        return Wavefront(Field(self._levels(E, True), self.input_grid), wavefront.wavelength,
+                         wavefront.input_stokes_vector)
 
 
 class VortexCoronagraph(MultiScaleCoronagraph):
@@ -220,3 +238,60 @@ class VortexCoronagraph(MultiScaleCoronagraph):
 
         MultiScaleCoronagraph.__init__(self, input_grid, complex_mask, lyot_stop, q, scaling_factor, window_size,
                                        precision)
+
+
+class VectorVortexCoronagraph(AgnosticOpticalElement):
+    """Vector vortex coronagraph (hcipy/coronagraphy/vortex.py:17-199): on every level of the
+    multi-scale scheme the focal mask is the Jones matrix of a linear retarder with retardance
+    ``phase_retardation`` (a scalar or a function of the wavelength) and fast axis
+    charge / 2 * theta, with the sample at the origin removed.  A scalar wavefront enters as the
+    unpolarized wavefront (input Stokes vector (1, 0, 0, 0)); the instances are built per input
+    grid and wavelength."""
+
+    def __init__(self, charge, lyot_stop=None, phase_retardation=np.pi, q=1024, scaling_factor=4, window_size=32,
+                 precision=None):
+        self.charge = charge
+        if lyot_stop is not None and not isinstance(lyot_stop, OpticalElement):
+            lyot_stop = Apodizer(lyot_stop)
+        self.lyot_stop = lyot_stop
+        self.phase_retardation = phase_retardation
+        self.q = q
+        self.scaling_factor = scaling_factor
+        self.window_size = window_size
+        self.precision = precision
+        AgnosticOpticalElement.__init__(self, grid_dependent=True, wavelength_dependent=True)
+
+    def make_instance(self, instance_data, input_grid, output_grid, wavelength):
+        retardance = float(self.evaluate_parameter(self.phase_retardation, input_grid, output_grid, wavelength))
+        charge = self.charge
+
+        def jones_mask(grid):
+            x, y = (np.asarray(c) for c in grid.coords)
+            theta = np.arctan2(y, x)
+            hole = ((x ** 2 + y ** 2) <= (0.5e-9) ** 2).astype(float)
+            J = phase_retarder_jones(retardance, charge / 2 * theta, 0) * (1 - hole)
+            return torch.as_tensor(np.ascontiguousarray(J))
+
+        instance_data.coronagraph = MultiScaleCoronagraph(input_grid, jones_mask, None, self.q, self.scaling_factor,
+                                                          self.window_size, self.precision)
+
+    def _polarized(self, wavefront):
+        if wavefront.is_polarized:
+            return wavefront
+        return Wavefront(wavefront.electric_field, wavefront.wavelength, (1, 0, 0, 0))
+
+    def forward(self, wavefront):
+        """Pupil -> Lyot plane (vortex.py:160-202)."""
+        d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
+        lyot = d.coronagraph.forward(self._polarized(wavefront))
+        if self.lyot_stop is not None:
+            lyot = self.lyot_stop.forward(lyot)
+        return lyot
+
+    def backward(self, wavefront):
+        """Lyot plane -> pupil (vortex.py:204-246): the stop's adjoint, then the levels with the
+        conjugate Jones matrices (the retarder's Jones matrix is symmetric, so conj(J) = J^H)."""
+        if self.lyot_stop is not None:
+            wavefront = self.lyot_stop.backward(wavefront)
+        d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
+        return d.coronagraph.backward(self._polarized(wavefront))

@@ -232,6 +232,75 @@ __global__ void masked_mean_adjoint_kernel(const void* __restrict__ mask, int ma
   }
 }
 
+// Per-pixel matrix product out[i][k] = sum_j J[i][j] E[j][k] (field_dot of a (p, q) matrix field
+// with a (q, K) field; J^H with adjoint: out[i][k] = sum_j conj(J[j][i]) E[j][k]).  Arithmetic
+// in FP32 for complex64 fields and FP64 for complex128 fields, J of either complex dtype.
+template <typename C, typename R>
+__global__ void field_dot_kernel(const void* __restrict__ J, int j_dtype, int p, int q, int adjoint,
+                                 const C* __restrict__ E, int64_t K, C* __restrict__ out, int64_t n) {
+  const int64_t total = (int64_t)p * K * n;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t px = t % n;
+    const int64_t k = (t / n) % K;
+    const int i = (int)(t / (n * K));
+    R re = 0, im = 0;
+    for (int j = 0; j < q; ++j) {
+      double jr, ji;
+      // adjoint: element (i, j) of J^H is conj(J[j][i]); J is (q x p) then
+      const int64_t ji_idx = adjoint ? ((int64_t)j * p + i) : ((int64_t)i * q + j);
+      load_any(J, j_dtype, ji_idx * n + px, jr, ji);
+      if (adjoint) ji = -ji;
+      const C e = E[((int64_t)j * K + k) * n + px];
+      re += (R)jr * e.re - (R)ji * e.im;
+      im += (R)jr * e.im + (R)ji * e.re;
+    }
+    out[((int64_t)i * K + k) * n + px] = C{re, im};
+  }
+}
+
+// Stokes parameters per pixel from the output coherency matrix C = <e e^H> (2 x 2, Hermitian):
+//   I = C00 + C11, Q = C00 - C11, U = 2 Re C01, V = -2 Im C01.
+// A Jones-vector field e (vector_field != 0, layout [2][n]) has C = e e^H.  A Jones-matrix field
+// J = (x, y; z, w) (layout [2][2][n]) acting on the input Stokes vector (s0, s1, s2, s3) has
+// C = J S J^H with S = 0.5 (s0 + s1, s2 - i s3; s2 + i s3, s0 - s1), formed as T = J S and
+// C_ab = sum_k T_ak conj(J_bk).  component 0..3 writes one of (I, Q, U, V) to out[px];
+// component -1 writes all four to out[c * n + px].  FP64 arithmetic.
+template <typename C, typename R>
+__global__ void stokes_kernel(const C* __restrict__ E, int vector_field, double s0, double s1, double s2, double s3,
+                              int component, R* __restrict__ out, int64_t n) {
+  const double sp = 0.5 * (s0 + s1), sm = 0.5 * (s0 - s1), su = 0.5 * s2, sv = 0.5 * s3;
+  for (int64_t px = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; px < n; px += (int64_t)gridDim.x * blockDim.x) {
+    double c00, c11, c01r, c01i;
+    if (vector_field) {
+      const C ex = E[px], ey = E[n + px];
+      c00 = (double)ex.re * ex.re + (double)ex.im * ex.im;
+      c11 = (double)ey.re * ey.re + (double)ey.im * ey.im;
+      c01r = (double)ex.re * ey.re + (double)ex.im * ey.im;
+      c01i = (double)ex.im * ey.re - (double)ex.re * ey.im;
+    } else {
+      const C x = E[px], y = E[n + px], z = E[2 * n + px], w = E[3 * n + px];
+      const double xr = x.re, xi = x.im, yr = y.re, yi = y.im, zr = z.re, zi = z.im, wr = w.re, wi = w.im;
+      // T = J S:  T_a0 = J_a0 sp + J_a1 (su + i sv),  T_a1 = J_a0 (su - i sv) + J_a1 sm
+      const double t00r = xr * sp + yr * su - yi * sv, t00i = xi * sp + yi * su + yr * sv;
+      const double t01r = xr * su + xi * sv + yr * sm, t01i = xi * su - xr * sv + yi * sm;
+      const double t10r = zr * sp + wr * su - wi * sv, t10i = zi * sp + wi * su + wr * sv;
+      const double t11r = zr * su + zi * sv + wr * sm, t11i = zi * su - zr * sv + wi * sm;
+      // C00 = T00 conj(x) + T01 conj(y),  C11 = T10 conj(z) + T11 conj(w),  C01 = T00 conj(z) + T01 conj(w)
+      c00 = t00r * xr + t00i * xi + t01r * yr + t01i * yi;
+      c11 = t10r * zr + t10i * zi + t11r * wr + t11i * wi;
+      c01r = t00r * zr + t00i * zi + t01r * wr + t01i * wi;
+      c01i = t00i * zr - t00r * zi + t01i * wr - t01r * wi;
+    }
+    const double st[4] = {c00 + c11, c00 - c11, 2.0 * c01r, -2.0 * c01i};
+    if (component >= 0) {
+      out[px] = (R)st[component];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) out[c * n + px] = (R)st[c];
+    }
+  }
+}
+
 constexpr int kThreads = 256;
 constexpr int kPowerBlocks = 256;
 
@@ -432,6 +501,47 @@ int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, in
     HCIPY_HIP(hipGetLastError());
   }
   return kOk;
+}
+
+int hcipy_pw_field_dot(const void* J, int j_dtype, int p, int q, int adjoint, const void* E, int field_dtype, int64_t K,
+                       void* out, int64_t n, void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype) && is_complex(j_dtype), "complex fields and matrices required");
+  HCIPY_REQUIRE(p >= 1 && q >= 1 && p <= 16 && q <= 16 && K >= 1 && n >= 0, "bad size");
+  if (n == 0) return kOk;
+  HCIPY_REQUIRE(J != nullptr && E != nullptr && out != nullptr && out != E, "NULL or aliased pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t total = (int64_t)(adjoint ? q : p) * K * n;
+  const dim3 g((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, kThreads), 16384)));
+  // the kernel's (p, q) are the output and contracted sizes of the product it forms
+  const int po = adjoint ? q : p, qo = adjoint ? p : q;
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((field_dot_kernel<C64, float>), g, dim3(kThreads), 0, s, J, j_dtype, po, qo, adjoint,
+                       (const C64*)E, K, (C64*)out, n);
+  else
+    hipLaunchKernelGGL((field_dot_kernel<C128, double>), g, dim3(kThreads), 0, s, J, j_dtype, po, qo, adjoint,
+                       (const C128*)E, K, (C128*)out, n);
+  return check_hip(hipGetLastError(), "field_dot_kernel");
+}
+
+int hcipy_pw_stokes(const void* E, int field_dtype, int vector_field, const double* stokes, int component,
+                    void* out, int64_t n, void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
+  HCIPY_REQUIRE(vector_field || stokes != nullptr, "a Jones-matrix field needs the input Stokes vector");
+  HCIPY_REQUIRE(component >= -1 && component <= 3, "component must be -1 (all) or 0..3 (I, Q, U, V)");
+  HCIPY_REQUIRE(n >= 0, "bad size");
+  if (n == 0) return kOk;
+  HCIPY_REQUIRE(E != nullptr && out != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double z[4] = {0, 0, 0, 0};
+  const double* sv = vector_field ? z : stokes;
+  const dim3 g = grid_for(n, 1);
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((stokes_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)E, vector_field, sv[0], sv[1],
+                       sv[2], sv[3], component, (float*)out, n);
+  else
+    hipLaunchKernelGGL((stokes_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)E, vector_field, sv[0],
+                       sv[1], sv[2], sv[3], component, (double*)out, n);
+  return check_hip(hipGetLastError(), "stokes_kernel");
 }
 
 }  // extern "C"

@@ -220,8 +220,8 @@ class DeformableMirror(OpticalElement):
 
     def forward(self, wavefront):
         return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self.surface, 2 * wavefront.wavenumber),
-                         wavefront.wavelength)
+                         wavefront.wavelength, wavefront.input_stokes_vector)
 
     def backward(self, wavefront):
         return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self.surface, -2 * wavefront.wavenumber),
-                         wavefront.wavelength)
+                         wavefront.wavelength, wavefront.input_stokes_vector)

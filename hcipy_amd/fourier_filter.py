@@ -109,11 +109,11 @@ class FourierFilter:
 
     ``transfer_function`` is a Field on the internal grid (centred order), a callable that
     maps the internal grid to such a Field, or a NearFieldTransferFunction.  A transfer function
-    of shape (p, q, internal size) is a matrix field (fourier_operations.py:120-131): the input
-    then carries q components in its second-to-last dimension and the output p, with
-    ``out_i = sum_j filter(tf_ij)(in_j)`` (the adjoint uses the conjugate transpose).  Each
-    (i, j) runs as a scalar filter; the terms are added by fused subtract passes of filters with
-    the negated transfer function.
+    of shape (p, q, internal size) is a matrix field (fourier_operations.py:120-131): it
+    contracts with the field's first dimension (q components, as field_dot does; any further
+    dimensions are carried along) and the output has p there, ``out_i = sum_j filter(tf_ij)(in_j)``
+    (the adjoint uses the conjugate transpose).  Each (i, j) runs as a scalar filter; the terms
+    are added by fused subtract passes of filters with the negated transfer function.
     """
 
     def __init__(self, input_grid, transfer_function, q=1):
@@ -153,7 +153,7 @@ class FourierFilter:
         tf = self._matrix_tf
         p, q = int(tf.shape[0]), int(tf.shape[1])
         n_in, n_out = (p, q) if adjoint else (q, p)
-        if t.dim() < 2 or t.shape[-2] != n_in:
+        if t.dim() < 2 or t.shape[0] != n_in:
             raise ValueError('a matrix transfer function of shape (%d, %d) needs %d field components, got shape %s'
                              % (p, q, n_in, tuple(t.shape)))
         cdt, dev_ = t.dtype, t.device
@@ -168,14 +168,14 @@ class FourierFilter:
                 if plan is None:
                     plan = _FilterPlan(self, cdt, dev_, tf=(-tf[i, j] if neg else tf[i, j]))
                     self._plans[key] = plan
-                xin = t[..., c, :].contiguous()
+                xin = t[c].contiguous()
                 lead = tuple(xin.shape[:-1])
                 batch = int(np.prod(lead)) if lead else 1
                 term = torch.empty_like(xin)
                 plan.execute(adjoint, xin, term, batch)
                 acc = term if acc is None else pointwise._sub_mul_raw(acc, term, None, 0, False)
             outs.append(acc)
-        return torch.stack(outs, dim=-2)
+        return torch.stack(outs, dim=0)
 
     def _operation(self, field, adjoint):
         grid = self.input_grid

@@ -22,21 +22,31 @@ def _tensor(f):
 
 
 class Wavefront:
-    """Scalar (or Jones-vector) electric field with a wavelength (wavefront.py:15-360).
+    """Electric field with a wavelength (wavefront.py:15-360): scalar, a Jones-vector field, or a
+    partially polarized wavefront, i.e. the 2 x 2 Jones-matrix field acting on an input Stokes
+    vector (I, Q, U, V) (wavefront.py:44-75).
 
     The electric field is cast to complex64 when it is float32/complex64 and to complex128
-    otherwise (wavefront.py:77-88).
+    otherwise (wavefront.py:77-88).  With a Stokes vector, a scalar field E becomes E times the
+    2 x 2 identity; a field of shape (2, 2, size) is taken as the Jones matrix itself.
     """
 
     def __init__(self, electric_field, wavelength=1, input_stokes_vector=None):
-        if input_stokes_vector is not None:
-            raise NotImplementedError('partially polarised wavefronts are outside this library')
         self.electric_field = electric_field
+        if input_stokes_vector is not None:
+            t = _tensor(self._electric_field)
+            if t.dim() == 1:
+                eye = torch.eye(2, dtype=t.dtype, device=t.device)
+                self._electric_field = Field((eye[:, :, None] * t[None, None, :]).contiguous(), self.grid)
+            elif tuple(t.shape[:-1]) != (2, 2):
+                raise ValueError('When supplying a Stokes vector, the electric field must be either a scalar or 2-tensor field.')
+            input_stokes_vector = np.asarray(input_stokes_vector, dtype=float).reshape(4)
         self.wavelength = wavelength
-        self._input_stokes_vector = None
+        self._input_stokes_vector = input_stokes_vector
 
     def copy(self):
-        return Wavefront(Field(_tensor(self._electric_field).clone(), self._electric_field.grid), self.wavelength)
+        return Wavefront(Field(_tensor(self._electric_field).clone(), self._electric_field.grid), self.wavelength,
+                         self._input_stokes_vector)
 
     @property
     def electric_field(self):
@@ -71,12 +81,62 @@ class Wavefront:
         return self._electric_field.is_scalar_field
 
     @property
+    def is_partially_polarized(self):
+        return self._input_stokes_vector is not None
+
+    @property
+    def is_polarized(self):
+        return self._input_stokes_vector is not None or _tensor(self._electric_field).dim() == 2
+
+    def _stokes(self, component):
+        t = _tensor(self._electric_field)
+        if self._input_stokes_vector is not None:
+            return pointwise.stokes(self._electric_field, self._input_stokes_vector, component)
+        if t.dim() == 2 and t.shape[0] == 2:
+            return pointwise.stokes(self._electric_field, None, component)
+        raise ValueError('Stokes parameters need a scalar, Jones-vector (2, n) or Jones-matrix (2, 2, n) field.')
+
+    @property
     def I(self):  # noqa: E743 (HCIPy name)
-        """|E|^2 summed over vector components (wavefront.py:113-143)."""
+        """Intensity (wavefront.py:113-143): |E|^2 for a scalar field, and the Stokes I of a
+        Jones-vector field or a partially polarized wavefront (``hcipy_pw_stokes``).  Other tensor
+        fields sum |E|^2 over their components."""
+        t = _tensor(self._electric_field)
+        if self._input_stokes_vector is not None or (t.dim() == 2 and t.shape[0] == 2):
+            return self._stokes(0)
         I = pointwise.intensity(self._electric_field)
         if I.dim() > 1:
             I = Field(_tensor(I).sum(dim=tuple(range(I.dim() - 1))), self.grid)
         return I
+
+    def _zeros(self):
+        t = _tensor(self._electric_field)
+        rdt = torch.float32 if t.dtype == torch.complex64 else torch.float64
+        return Field(torch.zeros(t.shape[-1], dtype=rdt, device=t.device), self.grid)
+
+    @property
+    def Q(self):  # noqa: N802 (HCIPy name)
+        """Stokes Q (wavefront.py:145-173); zero for a scalar field."""
+        return self._zeros() if self.is_scalar else self._stokes(1)
+
+    @property
+    def U(self):  # noqa: N802 (HCIPy name)
+        """Stokes U (wavefront.py:175-203); zero for a scalar field."""
+        return self._zeros() if self.is_scalar else self._stokes(2)
+
+    @property
+    def V(self):  # noqa: N802 (HCIPy name)
+        """Stokes V (wavefront.py:205-232); zero for a scalar field."""
+        return self._zeros() if self.is_scalar else self._stokes(3)
+
+    @property
+    def stokes_vector(self):
+        """(4, n) Stokes vector (wavefront.py:234-259); a scalar field is unpolarized (I, 0, 0, 0)."""
+        if self.is_scalar:
+            I = _tensor(self.I)
+            z = torch.zeros_like(I)
+            return Field(torch.stack([I, z, z, z]), self.grid)
+        return self._stokes(-1)
 
     @property
     def intensity(self):
@@ -98,6 +158,8 @@ class Wavefront:
     @property
     def total_power(self):
         """FP64 sum of intensity x weights (wavefront.py:332-360)."""
+        if self._input_stokes_vector is not None:
+            return pointwise.real_sum(self.I) * float(np.asarray(self.grid.weights).ravel()[0])
         return pointwise.total_power(self._electric_field, self.grid.weights)
 
     @total_power.setter
@@ -207,11 +269,11 @@ class Apodizer(OpticalElement):
 
     def forward(self, wavefront):
         a = self._get_apodization(wavefront.wavelength)
-        return Wavefront(pointwise.multiply(wavefront.electric_field, a, conjugate=False), wavefront.wavelength)
+        return Wavefront(pointwise.multiply(wavefront.electric_field, a, conjugate=False), wavefront.wavelength, wavefront.input_stokes_vector)
 
     def backward(self, wavefront):
         a = self._get_apodization(wavefront.wavelength)
-        return Wavefront(pointwise.multiply(wavefront.electric_field, a, conjugate=True), wavefront.wavelength)
+        return Wavefront(pointwise.multiply(wavefront.electric_field, a, conjugate=True), wavefront.wavelength, wavefront.input_stokes_vector)
 
 
 class PhaseApodizer(OpticalElement):
@@ -225,7 +287,7 @@ class PhaseApodizer(OpticalElement):
         return self._phase
 
     def forward(self, wavefront):
-        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self._phase, 1.0), wavefront.wavelength)
+        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self._phase, 1.0), wavefront.wavelength, wavefront.input_stokes_vector)
 
     def backward(self, wavefront):
-        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self._phase, -1.0), wavefront.wavelength)
+        return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self._phase, -1.0), wavefront.wavelength, wavefront.input_stokes_vector)

@@ -221,3 +221,107 @@ def total_power(x, weights):
     if w.size == 1:
         return float(power_sum(x, float(w.ravel()[0])).sum().item())
     return float(power_sum(x, 1.0, mask=torch.as_tensor(w, device=x.device)).sum().item())
+
+
+def real_sum(x):
+    """FP64 sum of a real array (the deterministic two-pass reduction of ``hcipy_pw_masked_mean``
+    without a mask: mean times count), as a Python float."""
+    t = _plain(x).contiguous()
+    _lib.require_device(t)
+    if t.dtype not in (torch.float32, torch.float64):
+        raise ValueError('real_sum needs a real array')
+    t = t.reshape(1, -1)
+    n = t.shape[-1]
+    if n == 0:
+        return 0.0
+    lib = _lib.load()
+    nb = ctypes.c_size_t()
+    _lib.check(lib.hcipy_pw_masked_mean_workspace(1, ctypes.byref(nb)), 'hcipy_pw_masked_mean_workspace')
+    ws = torch.empty(max(nb.value, 8), dtype=torch.uint8, device=t.device)
+    out = torch.empty(1, dtype=torch.float64, device=t.device)
+    cnt = torch.empty(1, dtype=torch.float64, device=t.device)
+    st = lib.hcipy_pw_masked_mean(_lib.dptr(t), _lib.dtype_code(t), None, 0, 0, n, 1, _lib.dptr(out), _lib.dptr(cnt),
+                                  _lib.dptr(ws), ws.numel(), _lib.stream_ptr(t.device))
+    _lib.check(st, 'hcipy_pw_masked_mean')
+    return float(out.item()) * float(cnt.item())
+
+
+def stokes(E, input_stokes_vector=None, component=-1):
+    """Stokes parameters of a polarized field (wavefront.py:112-235): E is a Jones-vector field
+    (2, n), or the (2, 2, n) Jones-matrix field of a partially polarized wavefront acting on
+    ``input_stokes_vector``.  ``component`` 0..3 returns I, Q, U or V (n,); -1 all four (4, n).
+    FP64 arithmetic, float32 results for complex64 fields."""
+    grid = getattr(E, 'grid', None)
+    et = _plain(E).contiguous()
+    _lib.require_device(et)
+    vector = input_stokes_vector is None
+    if tuple(et.shape[:-1]) != ((2,) if vector else (2, 2)):
+        raise ValueError('expected a %s field, got shape %s' % ('(2, n) Jones-vector' if vector else '(2, 2, n) Jones-matrix',
+                                                                 tuple(et.shape)))
+    if not et.is_complex():
+        et = et.to(torch.complex128)
+    n = et.shape[-1]
+    rdt = torch.float32 if et.dtype == torch.complex64 else torch.float64
+    out = torch.empty((n,) if component >= 0 else (4, n), dtype=rdt, device=et.device)
+    sv = None
+    if not vector:
+        sv = (ctypes.c_double * 4)(*[float(v) for v in np.asarray(input_stokes_vector, dtype=float).reshape(4)])
+    st = _lib.load().hcipy_pw_stokes(_lib.dptr(et), _lib.dtype_code(et), int(vector), sv, int(component), _lib.dptr(out),
+                                     n, _lib.stream_ptr(et.device))
+    _lib.check(st, 'hcipy_pw_stokes')
+    return Field(out, grid) if grid is not None else out
+
+
+def _field_dot_raw(jt, et, adjoint):
+    p, q = int(jt.shape[0]), int(jt.shape[1])
+    n = et.shape[-1]
+    rest = tuple(et.shape[1:-1])
+    K = int(np.prod(rest)) if rest else 1
+    o = torch.empty(((q if adjoint else p),) + rest + (n,), dtype=et.dtype, device=et.device)
+    st = _lib.load().hcipy_pw_field_dot(_lib.dptr(jt), _lib.dtype_code(jt), p, q, int(bool(adjoint)), _lib.dptr(et),
+                                        _lib.dtype_code(et), K, _lib.dptr(o), n, _lib.stream_ptr(et.device))
+    _lib.check(st, 'hcipy_pw_field_dot')
+    return o
+
+
+class _FieldDotFn(torch.autograd.Function):
+    """out = J E per pixel (J constant); the gradient with respect to E is J^H g."""
+
+    @staticmethod
+    def forward(ctx, jt, et, adjoint):
+        ctx.save_for_backward(jt)
+        ctx.adjoint = adjoint
+        return _field_dot_raw(jt, et, adjoint)
+
+    @staticmethod
+    def backward(ctx, g):
+        (jt,) = ctx.saved_tensors
+        return None, _field_dot_raw(jt, g.contiguous(), not ctx.adjoint), None
+
+
+def field_dot(J, E, adjoint=False):
+    """Per-pixel matrix product of a (p, q, n) matrix field J with a field E of shape (q, ..., n)
+    (field_dot of hcipy/field/field.py; ``adjoint`` applies J^H), returning (p, ..., n)."""
+    grid = getattr(E, 'grid', None)
+    et = _plain(E).contiguous()
+    _lib.require_device(et)
+    jt = _plain(J)
+    if not isinstance(jt, torch.Tensor):
+        jt = torch.as_tensor(np.asarray(J), device=et.device)
+    jt = jt.to(et.device)
+    if not jt.is_complex():
+        jt = jt.to(torch.complex128)
+    jt = jt.contiguous()
+    if jt.dim() != 3 or jt.shape[-1] != et.shape[-1]:
+        raise ValueError('J must be a (p, q, n) matrix field on the same grid as E')
+    if et.shape[0] != (jt.shape[0] if adjoint else jt.shape[1]):
+        raise ValueError('E has %d components along its first dimension, J needs %d'
+                         % (et.shape[0], jt.shape[0] if adjoint else jt.shape[1]))
+    if needs_grad(et):
+        o = _FieldDotFn.apply(jt, et, bool(adjoint))
+    else:
+        o = _field_dot_raw(jt, et, adjoint)
+    return Field(o, grid) if grid is not None else o
+
+
+__all__ += ['real_sum', 'stokes', 'field_dot']

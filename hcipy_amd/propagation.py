@@ -57,12 +57,12 @@ class FraunhoferPropagator(AgnosticOpticalElement):
     def forward(self, wavefront):
         d = self.instance(wavefront.wavelength)
         U = d.fourier_transform._apply(wavefront.electric_field, _lib.FORWARD, extra_alpha=d.norm_factor)
-        return Wavefront(Field(U, self._output_grid), wavefront.wavelength)
+        return Wavefront(Field(U, self._output_grid), wavefront.wavelength, wavefront.input_stokes_vector)
 
     def backward(self, wavefront):
         d = self.instance(wavefront.wavelength)
         U = d.fourier_transform._apply(wavefront.electric_field, _lib.BACKWARD, extra_alpha=1 / d.norm_factor)
-        return Wavefront(Field(U, self._input_grid), wavefront.wavelength)
+        return Wavefront(Field(U, self._input_grid), wavefront.wavelength, wavefront.input_stokes_vector)
 
     def forward_multiple(self, wavefronts, output='field', group=None):
         """Forward propagation of several wavefronts (for example the wavelengths of a broadband
@@ -84,7 +84,8 @@ class FraunhoferPropagator(AgnosticOpticalElement):
             calls.append((d.fourier_transform, _lib.FORWARD, wf.electric_field, w * d.norm_factor, kind))
         outs = group.execute(calls)
         if output == 'field':
-            return [Wavefront(Field(o, self._output_grid), wf.wavelength) for o, wf in zip(outs, wavefronts)]
+            return [Wavefront(Field(o, self._output_grid), wf.wavelength, wf.input_stokes_vector)
+                    for o, wf in zip(outs, wavefronts)]
         return [Field(o, self._output_grid) for o in outs]
 
     def forward_intensity(self, wavefront, float64=False):
@@ -174,11 +175,13 @@ class _NearFieldPropagator(AgnosticOpticalElement):
 
     def forward(self, wavefront):
         d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
-        return Wavefront(d.fourier_filter.forward(wavefront.electric_field), wavefront.wavelength)
+        return Wavefront(d.fourier_filter.forward(wavefront.electric_field), wavefront.wavelength,
+                         wavefront.input_stokes_vector)
 
     def backward(self, wavefront):
         d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
-        return Wavefront(d.fourier_filter.backward(wavefront.electric_field), wavefront.wavelength)
+        return Wavefront(d.fourier_filter.backward(wavefront.electric_field), wavefront.wavelength,
+                         wavefront.input_stokes_vector)
 
 
 class AngularSpectrumPropagator(_NearFieldPropagator):

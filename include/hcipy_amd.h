@@ -243,6 +243,26 @@ int hcipy_pw_phase_adjoint(const void* y, const void* g, int field_dtype, double
 int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, int g_dtype, int64_t g_stride, void* out,
                                int64_t n, int64_t batch, void* stream);
 
+/* Per-pixel matrix product of a (p x q) matrix field J and a field E of q x K components on n
+ * pixels: out[i][k][px] = sum_j J[i][j][px] E[j][k][px], layouts [row][col][pixel].  With
+ * adjoint != 0 the conjugate transpose J^H is applied (E then has p x K components, out q x K).
+ * Replaces field_dot of a Jones matrix with a Jones-vector or Jones-matrix field
+ * (hcipy/field/field.py field_dot, used by JonesMatrixOpticalElement.forward/backward,
+ * hcipy/optics/polarization.py:44-116, and VectorVortexCoronagraph.forward,
+ * hcipy/coronagraphy/vortex.py:162-199).  J may be complex64 or complex128. */
+int hcipy_pw_field_dot(const void* J, int j_dtype, int p, int q, int adjoint, const void* E, int field_dtype, int64_t K,
+                       void* out, int64_t n, void* stream);
+
+/* Stokes parameters (I, Q, U, V) per pixel of a polarized wavefront: Wavefront.I/.Q/.U/.V
+ * (hcipy/optics/wavefront.py:112-235).  With vector_field != 0, E is a Jones-vector field
+ * (layout [2][n]) and stokes is ignored; otherwise E is the 2 x 2 Jones-matrix field (layout
+ * [2][2][n]) of a partially polarized wavefront acting on the input Stokes vector
+ * stokes[0..3].  component 0..3 writes that parameter to out[n]; -1 writes all four to
+ * out[4][n] (Wavefront.stokes_vector).  FP64 arithmetic; float32 out for complex64 fields and
+ * float64 for complex128. */
+int hcipy_pw_stokes(const void* E, int field_dtype, int vector_field, const double* stokes, int component, void* out,
+                    int64_t n, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Deformable mirror with Gaussian influence functions on a regular pupil grid, evaluated
  * without forming the influence matrix.  Replaces make_gaussian_influence_functions +

@@ -27,6 +27,15 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
   obstructed_circular);
 * ``lyot_forward``: the explicit (1-M) Babinet algebra and the reference thresholds
   (tests/test_coronagraphy.py:272-304);
+* ``multiscale_masks``/``multiscale_forward``/``vortex_mask``/``vector_vortex_mask``: the
+  reference's rejection thresholds for charges 2-8, scalar and vector
+  (tests/test_coronagraphy.py:4-28, 81-105), the unit-mask identity, the adjoint identity, and
+  the circular-basis identity tying the vector vortex to the scalar one;
+* ``stokes_vector``: for pure input states the Mueller rows must agree with the Stokes
+  parameters of the propagated Jones vector J e (an independent closed form), and those four
+  states span every input Stokes vector; Malus's law for a rotated polarizer;
+* ``field_dot``/``phase_retarder_jones``: the half- and quarter-wave plate matrices and the
+  circular-basis identity of the vector vortex;
 * ``gaussian_influence_functions``/``dm_surface``: crosstalk definition (value at the
   neighbouring actuator) and cutoff (hcipy/optics/deformable_mirror.py:78-79).
 """
@@ -403,26 +412,26 @@ def fourier_filter(field, input_grid, tf_internal, q=1, adjoint=False):
     ``tf_internal`` is the transfer function sampled on the internal (FFT output, q, fov=1) grid
     in that grid's natural (centred) order; it is ifftshifted here as in :41-52.
     ``field`` may carry leading batch dimensions.  A transfer function of shape (p, q, size) is a
-    matrix field (:120-131): the field's second-to-last dimension holds its q components and
-    the result has p (the adjoint uses the conjugate transpose).
+    matrix field (:120-131): it contracts with the field's first tensor dimension (q components)
+    as field_dot does, and the result has p there (the adjoint uses the conjugate transpose).
     """
     fft = FFT(input_grid, q)
     internal_grid = fft.output_grid
     tf_arr = np.asarray(tf_internal)
     if tf_arr.ndim == 3:
+        # matrix field: field_dot(tf, F) contracts tf's second index with the field's first
+        # tensor index; the field's other tensor indices are carried along
         tf = np.fft.ifftshift(tf_arr.reshape(tf_arr.shape[:2] + internal_grid.shape), axes=(-2, -1))
         if adjoint:
             tf = np.conj(np.swapaxes(tf, 0, 1))
         field = np.asarray(field)
-        lead = field.shape[:-2]
         f = field.reshape(field.shape[:-1] + input_grid.shape)
         arr = np.zeros(field.shape[:-1] + internal_grid.shape, dtype=np.result_type(field.dtype, np.complex64))
-        c = (slice(None),) * (len(lead) + 1) + (fft.cutout_input or (slice(None), slice(None)))
-        arr[c] = f
+        cut = (slice(None),) * (field.ndim - 1) + (fft.cutout_input or (slice(None), slice(None)))
+        arr[cut] = f
         F = np.fft.fftn(arr, axes=(-2, -1))
-        G = np.einsum('ijyx,...jyx->...iyx', tf.astype(arr.dtype), F)  # field_dot per frequency
-        r = np.fft.ifftn(G, axes=(-2, -1))
-        r = r[(slice(None),) * (len(lead) + 1) + (fft.cutout_input or (slice(None), slice(None)))]
+        G = np.einsum('ijyx,j...yx->i...yx', tf.astype(arr.dtype), F)  # field_dot per frequency
+        r = np.fft.ifftn(G, axes=(-2, -1))[cut]
         return r.reshape(r.shape[:-2] + (-1,)).astype(arr.dtype)
     tf = np.fft.ifftshift(tf_arr.reshape(internal_grid.shape))
     field = np.asarray(field)
@@ -641,7 +650,7 @@ def multiscale_masks(pupil_grid, complex_mask, q=1024, scaling_factor=4, window_
             m = m * (1 - multiscale_window(window_size, g.shape))
         for j in range(i):
             fft = FFT(grids[j])
-            m = m - mft_backward(fft.forward(masks[j]), g, fft.output_grid)
+            m = m - mft_backward(_fft_forward_tensor(fft, masks[j]), g, fft.output_grid)
         grids.append(g)
         masks.append(m)
     return grids, masks
@@ -656,9 +665,85 @@ def multiscale_forward(E, pupil_grid, grids, masks, adjoint=False):
     L = fourier_filter(E, pupil_grid, masks[0], q=2, adjoint=adjoint)
     for g, m in zip(grids[1:], masks[1:]):
         F = fraunhofer_forward(E, pupil_grid, g)
-        F = F * (np.conj(m) if adjoint else m)
+        if np.ndim(m) == 3:  # Jones-matrix mask (vortex.py:188-189, 237-238): field_dot
+            F = field_dot(np.conj(m) if adjoint else m, F)
+        else:
+            F = F * (np.conj(m) if adjoint else m)
         L = L + fraunhofer_backward(F, pupil_grid, g)
     return L
+
+
+def field_dot(J, E):
+    """field_dot of a (p, q, n) matrix field with a (q, ..., n) field (hcipy/field/field.py)."""
+    return np.einsum('ijn,j...n->i...n', np.asarray(J), np.asarray(E))
+
+
+def stokes_vector(E, stokes=None):
+    """Wavefront.I/.Q/.U/.V (hcipy/optics/wavefront.py:112-232) as a (4, n) array.  A Jones-vector
+    field E (2, n) uses the fully polarized forms; a (2, 2, n) Jones-matrix field of a partially
+    polarized wavefront is weighted by the input Stokes vector (a, b, c, d) with the rows of
+    its Mueller matrix as the reference writes them."""
+    E = np.asarray(E)
+    if E.ndim == 2:
+        e0, e1 = E
+        return np.array([abs(e0) ** 2 + abs(e1) ** 2, abs(e0) ** 2 - abs(e1) ** 2,
+                         2 * np.real(e0 * np.conj(e1)), -2 * np.imag(e0 * np.conj(e1))])
+    x, y, z, w = E[0, 0], E[0, 1], E[1, 0], E[1, 1]
+    xr, xi, yr, yi, zr, zi, wr, wi = x.real, x.imag, y.real, y.imag, z.real, z.imag, w.real, w.imag
+    X, Y, Z, W = abs(x) ** 2, abs(y) ** 2, abs(z) ** 2, abs(w) ** 2
+    rows = [
+        (X + Y + Z + W, X - Y + Z - W, 2 * (xr * yr + xi * yi + zr * wr + zi * wi),
+         2 * (-xr * yi + xi * yr - zr * wi + zi * wr)),
+        (X + Y - Z - W, X - Y - Z + W, 2 * (xr * yr + xi * yi - zr * wr - zi * wi),
+         2 * (-xr * yi + xi * yr + zr * wi - zi * wr)),
+        (2 * (xr * zr + xi * zi + yr * wr + yi * wi), 2 * (xr * zr + xi * zi - yr * wr - yi * wi),
+         2 * (xr * wr + xi * wi + yr * zr + yi * zi), 2 * (xi * wr - xr * wi + yr * zi - yi * zr)),
+        (2 * (xr * zi - xi * zr + yr * wi - yi * wr), 2 * (xr * zi - xi * zr - yr * wi + yi * wr),
+         2 * (xr * wi - xi * wr + yr * zi - yi * zr), 2 * (xr * wr + xi * wi - yr * zr - yi * zi)),
+    ]
+    a, b, c, d = stokes
+    return np.array([0.5 * (r[0] * a + r[1] * b + r[2] * c + r[3] * d) for r in rows])
+
+
+def jones_vector_of_stokes(stokes):
+    """A Jones vector e (2,) whose fully polarized Stokes vector is ``stokes`` (s0 = |s|)."""
+    s0, s1, s2, s3 = stokes
+    ex = np.sqrt((s0 + s1) / 2)
+    if ex == 0:
+        return np.array([0, np.sqrt(s0)], dtype=complex)
+    # e0 conj(e1) = (s2 - i s3) / 2  ->  e1 = conj((s2 - i s3) / 2) / ex
+    return np.array([ex, np.conj((s2 - 1j * s3) / 2) / ex], dtype=complex)
+
+
+def phase_retarder_jones(phase_retardation, fast_axis_orientation, circularity):
+    """PhaseRetarder.jones_matrix (hcipy/optics/polarization.py:141-160)."""
+    phi_plus = np.exp(1j * phase_retardation / 2)
+    phi_minus = np.exp(-1j * phase_retardation / 2)
+    c, s = np.cos(fast_axis_orientation), np.sin(fast_axis_orientation)
+    j11 = phi_plus * c ** 2 + phi_minus * s ** 2
+    j12 = (phi_plus - phi_minus) * np.exp(-1j * circularity) * c * s
+    j21 = (phi_plus - phi_minus) * np.exp(1j * circularity) * c * s
+    j22 = phi_plus * s ** 2 + phi_minus * c ** 2
+    j11, j12, j21, j22 = np.broadcast_arrays(j11, j12, j21, j22)
+    return np.array([[j11, j12], [j21, j22]])
+
+
+def vector_vortex_mask(charge, phase_retardation=np.pi):
+    """VectorVortexCoronagraph's level masks before windowing (hcipy/coronagraphy/vortex.py:
+    84-94): LinearRetarder(retardance, charge / 2 theta) Jones matrix, origin sample removed."""
+    def fn(grid):
+        x, y = grid.coords
+        J = phase_retarder_jones(phase_retardation, charge / 2 * np.arctan2(y, x), 0)
+        return J * (1 - circular_aperture(grid, 1e-9))
+    return fn
+
+
+def _fft_forward_tensor(fft, f):
+    """FFT.forward over the leading (tensor) dimensions of f."""
+    f = np.asarray(f)
+    lead = f.shape[:-1]
+    flat = f.reshape((-1, f.shape[-1]))
+    return np.array([fft.forward(v) for v in flat]).reshape(lead + (-1,))
 
 
 def actuator_positions(n_act, pitch):

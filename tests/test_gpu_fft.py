@@ -114,22 +114,23 @@ def test_angular_spectrum_1024_1km():
 @pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
 @pytest.mark.parametrize('n,q', [(16, 1), (32, 2)])
 def test_matrix_transfer_function(n, q, dtype, tol):
-    """A (2 x 3) matrix transfer function (fourier_operations.py:120-131): three input components,
-    two output components, against the oracle; the adjoint maps two back to three."""
+    """A (2 x 3) matrix transfer function (fourier_operations.py:120-131) on a 2-tensor field of
+    3 x 4 components: the first index is contracted (field_dot), against the oracle; the adjoint
+    maps the 2 x 4 result back to 3 x 4."""
     g = ha.make_pupil_grid(n, 2)
     internal = o.make_fft_grid(og(g), q)
     rng = np.random.default_rng(8)
     tf = rng.standard_normal((2, 3, internal.size)) + 1j * rng.standard_normal((2, 3, internal.size))
-    f = (rng.standard_normal((4, 3, g.size)) + 1j * rng.standard_normal((4, 3, g.size))).astype(dtype)
+    f = (rng.standard_normal((3, 4, g.size)) + 1j * rng.standard_normal((3, 4, g.size))).astype(dtype)
     filt = ha.FourierFilter(g, dev(tf), q)
     out = host(filt.forward(ha.Field(dev(f), g)))
     ref = o.fourier_filter(f.astype(np.complex128), og(g), tf, q)
-    assert out.shape == (4, 2, g.size)
+    assert out.shape == (2, 4, g.size)
     say('matrix filter n=%d q=%d %s fwd %.2e' % (n, q, dtype.__name__, peak_rel_field(out, ref)))
     assert peak_rel_field(out, ref) < tol
     back = host(filt.backward(ha.Field(dev(out.astype(dtype)), g)))
     refb = o.fourier_filter(out.astype(np.complex128), og(g), tf, q, adjoint=True)
-    assert back.shape == (4, 3, g.size)
+    assert back.shape == (3, 4, g.size)
     assert peak_rel_field(back, refb) < tol
     if dtype == np.complex128:
         # gradient through the recorded path is the adjoint filter

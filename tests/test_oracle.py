@@ -406,7 +406,110 @@ def test_matrix_fourier_filter_equals_fft_chain(n, q):
     a = o.fourier_filter(f, g, tf, q, adjoint=True)
     b = np.array([fft.backward(np.sum(np.conj(tf[:, i]) * F, axis=0)) for i in range(3)])
     assert np.abs(a - b).max() < 1e-10
-    # a batch of vector fields
-    fb = np.stack([f, 2 * f])
+    # a 2-tensor field (3 x 2 components): field_dot contracts the first tensor index
+    fb = np.stack([f, 2 * f], axis=1)
     ab = o.fourier_filter(fb, g, tf, q)
-    assert np.abs(ab[1] - 2 * o.fourier_filter(f, g, tf, q)).max() < 1e-10
+    assert ab.shape == (3, 2, g.size)
+    assert np.abs(ab[:, 1] - 2 * o.fourier_filter(f, g, tf, q)).max() < 1e-10
+
+
+_PURE_STATES = [(1, 1, 0, 0), (1, -1, 0, 0), (1, 0, 1, 0), (1, 0, -1, 0), (1, 0, 0, 1), (1, 0, 0, -1)]
+
+
+def test_stokes_mueller_rows_match_the_propagated_jones_vector():
+    """For a fully polarized input state s with Jones vector e, the Mueller rows of the
+    partially polarized form (wavefront.py:112-232) must give the Stokes parameters of J e; the
+    pure states span every input Stokes vector, so a wrong sign or index in any row fails."""
+    rng = np.random.default_rng(11)
+    J = rng.standard_normal((2, 2, 50)) + 1j * rng.standard_normal((2, 2, 50))
+    for s in _PURE_STATES:
+        e = o.jones_vector_of_stokes(s)
+        np.testing.assert_allclose(o.stokes_vector(np.array([[e[0]], [e[1]]]))[:, 0], s, atol=1e-15)
+        Je = np.einsum('ijn,j->in', J, e)
+        np.testing.assert_allclose(o.stokes_vector(J, s), o.stokes_vector(Je), rtol=1e-12, atol=1e-12)
+    # linear in the input Stokes vector (unpolarized = mean of two orthogonal pure states)
+    np.testing.assert_allclose(o.stokes_vector(J, (1, 0, 0, 0)),
+                               0.5 * (o.stokes_vector(J, (1, 1, 0, 0)) + o.stokes_vector(J, (1, -1, 0, 0))), atol=1e-12)
+
+
+def test_retarder_matrices():
+    """Half- and quarter-wave plates (polarization.py:141-160): a half-wave plate at angle a
+    mirrors linear polarization about a (Q, U) -> (cos 4a, sin 4a) for horizontal input, a
+    quarter-wave plate at 45 degrees turns horizontal into circular light, a linear polarizer
+    (J = P_a) obeys Malus's law, and every retarder is unitary."""
+    for a in (0.0, 0.3, 1.1):
+        J = o.phase_retarder_jones(np.pi, a, 0)
+        np.testing.assert_allclose(J @ np.conj(J.T), np.eye(2), atol=1e-15)
+        e = J @ np.array([1, 0])
+        st = o.stokes_vector(e.reshape(2, 1))[:, 0]
+        np.testing.assert_allclose(st, [1, np.cos(4 * a), np.sin(4 * a), 0], atol=1e-14)
+        P = np.array([[np.cos(a) ** 2, np.cos(a) * np.sin(a)], [np.cos(a) * np.sin(a), np.sin(a) ** 2]])
+        I = o.stokes_vector(np.repeat(P[:, :, None], 1, axis=2), (1, 1, 0, 0))[0, 0]
+        assert abs(I - np.cos(a) ** 2) < 1e-15
+    st = o.stokes_vector((o.phase_retarder_jones(np.pi / 2, np.pi / 4, 0) @ np.array([1, 0])).reshape(2, 1))[:, 0]
+    np.testing.assert_allclose(np.abs(st), [1, 0, 0, 1], atol=1e-15)
+    # an array of orientations gives a (2, 2, n) field of the same matrices
+    th = np.linspace(0, 3, 7)
+    Jf = o.phase_retarder_jones(np.pi, th, 0)
+    for k in range(7):
+        np.testing.assert_array_equal(Jf[:, :, k], o.phase_retarder_jones(np.pi, th[k], 0))
+    # field_dot against explicit loops
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((3, 2, 5)) + 1j * rng.standard_normal((3, 2, 5))
+    B = rng.standard_normal((2, 4, 5)) + 1j * rng.standard_normal((2, 4, 5))
+    ref = np.zeros((3, 4, 5), complex)
+    for i in range(3):
+        for k in range(4):
+            for n in range(5):
+                ref[i, k, n] = A[i, 0, n] * B[0, k, n] + A[i, 1, n] * B[1, k, n]
+    np.testing.assert_allclose(o.field_dot(A, B), ref, atol=1e-14)
+
+
+def _vector_vortex_lyot(E, pg, charge, adjoint=False):
+    grids, masks = o.multiscale_masks(pg, o.vector_vortex_mask(charge))
+    E2 = np.eye(2)[:, :, None] * E[None, None, :]
+    return o.multiscale_forward(E2, pg, grids, masks, adjoint=adjoint)
+
+
+@pytest.mark.parametrize('charge', [2, 4])
+def test_vector_vortex_reference_thresholds_and_circular_basis(charge):
+    """tests/test_coronagraphy.py:81-118: an unpolarized on-axis source through a charge 2, 4
+    vector vortex and a 0.99 Lyot stop leaves total power < 1e-6 and peak < 1e-8 of the
+    unocculted peak.  In the circular basis v_L = (1, i) / sqrt 2, v_R = (1, -i) / sqrt 2 a
+    half-wave plate with fast axis l theta / 2 maps v_L to i e^(i l theta) v_R, so the levels
+    (linear in the mask) give v_R^H L v_L = i L_vortex(l) and v_L^H L v_R = i L_vortex(-l) with the
+    scalar vortex's Lyot field, and v_R^H L v_R = v_L^H L v_L = 0."""
+    pg, fg, E, lyot = _vortex_setup()
+    L = _vector_vortex_lyot(E, pg, charge)
+    vl = np.array([1, 1j]) / np.sqrt(2)
+    vr = np.array([1, -1j]) / np.sqrt(2)
+    circ = lambda u, v: np.einsum('i,ijn,j->n', np.conj(u), L, v)  # noqa: E731
+    for l, u, v in ((charge, vr, vl), (-charge, vl, vr)):
+        grids, masks = o.multiscale_masks(pg, o.vortex_mask(l))
+        ref = o.multiscale_forward(E, pg, grids, masks)
+        assert np.abs(circ(u, v) - 1j * ref).max() < 1e-12 * np.abs(E).max()
+    assert np.abs(circ(vr, vr)).max() < 1e-12 and np.abs(circ(vl, vl)).max() < 1e-12
+    ref_img = o.fraunhofer_forward(E, pg, fg)
+    img = o.fraunhofer_forward(L * lyot, pg, fg)
+    I = o.stokes_vector(img, (1, 0, 0, 0))[0]
+    assert np.sum(I) * fg.weights < 1e-6
+    assert I.max() / np.abs(ref_img).max() ** 2 < 1e-8
+
+
+def test_vector_vortex_circular_input_flips_handedness():
+    """tests/test_coronagraphy.py:120-147: circularly polarized light (Stokes (1, 0, 0, 1))
+    through a charge 2 vector vortex leaves in the orthogonal state (I = -V), and the backward
+    propagation of that image returns to the original state (I = V)."""
+    pg = o.make_pupil_grid(128)
+    fg = o.make_focal_grid(4, 16)
+    ap = o.evaluate_supersampled(lambda g: o.obstructed_circular_aperture(g, 1, 0.2), pg, 4)
+    lyot = o.evaluate_supersampled(lambda g: o.obstructed_circular_aperture(g, 0.99, 0.2 / 0.99), pg, 4) > 1 - 1e-5
+    E = ap.astype(complex)
+    grids, masks = o.multiscale_masks(pg, o.vector_vortex_mask(2), q=64)
+    L = o.multiscale_forward(np.eye(2)[:, :, None] * E, pg, grids, masks) * lyot
+    img = o.fraunhofer_forward(L, pg, fg)
+    st = o.stokes_vector(img, (1, 0, 0, 1))
+    np.testing.assert_allclose(st[0], -st[3], rtol=1e-9, atol=1e-12 * st[0].max())
+    back = o.multiscale_forward(o.fraunhofer_backward(img, pg, fg), pg, grids, masks, adjoint=True)
+    sb = o.stokes_vector(back, (1, 0, 0, 1))
+    np.testing.assert_allclose(sb[0], sb[3], rtol=1e-9, atol=1e-12 * sb[0].max())
