@@ -9,6 +9,7 @@ from .field import (CartesianGrid, Field, RegularCoords, SeparatedCoords, Unstru
                     evaluate_supersampled, make_focal_grid, make_pupil_grid, make_uniform_grid, set_default_device,
                     subsample_field)
 from .fourier import (ComputationalComplexity, FastFourierTransform, FourierTransform, MatrixFourierTransform, MftGroup,
+                      ZoomFastFourierTransform,
                       get_fft_parameters, is_fft_grid, make_fft_grid, make_fourier_transform)
 from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, OpticalSystem, PhaseApodizer, Wavefront
 from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, FresnelPropagator

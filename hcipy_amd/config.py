@@ -7,6 +7,8 @@ TypeError (hcipy/config/config.py:21-55).  Additions of this package:
 * ``fourier.mft.precision_complex64``: arithmetic of complex64 MFTs, ``'f16x2'`` (split FP16
   MFMA with FP64 flushes, the default) or ``'fp32'`` (plain FP32 MFMA).  complex128 always
   runs in FP64.
+* ``fourier.zfft.precision_complex64``: arithmetic of complex64 zoom FFTs, ``'fp64'`` (FP32
+  loads and stores around FP64 chirps, line FFTs and intermediate; the default) or ``'fp32'``.
 * ``fourier.mft.execution_time_prediction_coefficients``: refitted for this library's MFT on
   one MI355X (microsecond-scale latency dominates small transforms).
 """
@@ -25,6 +27,7 @@ _DEFAULTS = {
             'execution_time_prediction_coefficients': {'a': 1.667, 'b': 0.716, 'c': -4.624},
         },
         'zfft': {
+            'precision_complex64': 'fp64',
             'execution_time_prediction_coefficients': {'a': 4.093, 'b': 0.847, 'c': -2.973},
         },
     },

@@ -509,6 +509,186 @@ class MatrixFourierTransform(FourierTransform):
 
 
 # ----------------------------------------------------------------------------------------------
+# Zoom FFT (Bluestein chirp-z transform per axis)
+
+
+class _ZoomPlan:
+    """Owns one C-ABI zoom-FFT plan (per-axis chirps and filter spectra in one dtype)."""
+
+    def __init__(self, input_grid, output_grid, cdt, precision, device):
+        lib = _lib.load()
+        nd = input_grid.ndim
+        ci = ctypes.c_int * nd
+        cd = ctypes.c_double * nd
+        with torch.cuda.device(device):
+            h = ctypes.c_void_p()
+            st = lib.hcipy_zoom_plan_create(
+                ctypes.byref(h), nd, ci(*[int(v) for v in input_grid.dims]), cd(*[float(v) for v in input_grid.delta]),
+                cd(*[float(v) for v in input_grid.zero]), ci(*[int(v) for v in output_grid.dims]),
+                cd(*[float(v) for v in output_grid.delta]), cd(*[float(v) for v in output_grid.zero]),
+                _lib._DTYPE_CODES[cdt], _lib.PRECISIONS[precision], _lib.stream_ptr(device))
+            _lib.check(st, 'hcipy_zoom_plan_create')
+            self.handle = h
+
+    def workspace_bytes(self, batch):
+        n = ctypes.c_size_t()
+        _lib.check(_lib.load().hcipy_zoom_workspace_size(self.handle, batch, ctypes.byref(n)), 'hcipy_zoom_workspace_size')
+        return n.value
+
+    def execute(self, direction, x, out, batch, alpha, workspace):
+        st = _lib.load().hcipy_zoom_execute(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch,
+                                            float(alpha.real), float(alpha.imag),
+                                            _lib.dptr(workspace) if workspace is not None else None,
+                                            0 if workspace is None else workspace.numel(), _lib.stream_ptr(x.device))
+        _lib.check(st, 'hcipy_zoom_execute')
+
+    def close(self):
+        if getattr(self, 'handle', None) is not None and _lib._lib is not None:
+            _lib.load().hcipy_zoom_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _ZoomFn(torch.autograd.Function):
+    """One zoom-FFT direction as an autograd node: the vector-Jacobian product of
+    alpha sum_j x_j exp(s i p_j q_k) is conj(alpha) sum_k g_k exp(-s i p_j q_k), the opposite
+    direction with the conjugate scale."""
+
+    @staticmethod
+    def forward(ctx, t, zfft, direction, alpha):
+        ctx.zfft, ctx.direction, ctx.alpha = zfft, direction, alpha
+        return zfft._execute_plain(t, direction, alpha)
+
+    @staticmethod
+    def backward(ctx, g):
+        other = _lib.BACKWARD if ctx.direction == _lib.FORWARD else _lib.FORWARD
+        return ctx.zfft._execute_plain(g.contiguous(), other, ctx.alpha.conjugate()), None, None, None
+
+
+class ZoomFastFourierTransform(FourierTransform):
+    """Fourier transform between regular Cartesian grids of any spacing by a Bluestein chirp-z
+    transform along each axis (hcipy/fourier/zoom_fast_fourier_transform.py:10-219).
+
+    ``forward`` computes ``w_in * sum_j f_j exp(-i x_j . u_k)``; ``backward`` computes
+    ``w_out / (2 pi)^ndim * sum_k F_k exp(+i u_k . x_j)``, the same sums as the matrix Fourier
+    transform.  complex128 fields run in FP64.  complex64 fields run with ``precision`` 'fp64'
+    (FP32 loads and stores, FP64 chirps, line FFTs and intermediate; the default from
+    ``Configuration().fourier.zfft.precision_complex64``) or 'fp32' (all FP32, chirps and filter
+    spectra rounded from FP64).  One- and two-dimensional grids; each axis needs
+    ``n_in + n_out - 1 <= 8192`` (one line FFT in LDS per row).
+    """
+
+    def __init__(self, input_grid, output_grid, precision=None):
+        self.check_if_supported(input_grid, output_grid)
+        if precision is not None and precision not in ('fp64', 'fp32'):
+            raise ValueError("precision must be 'fp64' or 'fp32' (complex64 fields)")
+        self.input_grid = input_grid
+        self.output_grid = output_grid
+        self.ndim = input_grid.ndim
+        self.precision = precision
+        self._plans = {}
+        self._workspace = {}
+
+    def precision_for(self, cdt):
+        if cdt == torch.complex128:
+            return 'fp64'
+        return self.precision or Configuration().fourier.zfft.precision_complex64
+
+    def plan_for(self, cdt, device):
+        prec = self.precision_for(cdt)
+        key = (cdt, prec, device.index)
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = _ZoomPlan(self.input_grid, self.output_grid, cdt, prec, device)
+            self._plans[key] = plan
+        return plan
+
+    def _workspace_for(self, plan, batch, device):
+        need = plan.workspace_bytes(batch)
+        if need == 0:
+            return None
+        ws = self._workspace.get(device.index)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(need, dtype=torch.uint8, device=device)
+            self._workspace[device.index] = ws
+        return ws
+
+    def _apply(self, field, direction):
+        in_grid, out_grid = (self.input_grid, self.output_grid) if direction == _lib.FORWARD else (self.output_grid, self.input_grid)
+        t, cdt = _as_field_tensor(field, in_grid)
+        if t.shape[-1] != in_grid.size:
+            raise ValueError('Field size %d does not match the grid size %d' % (t.shape[-1], in_grid.size))
+        w = in_grid.weights if direction == _lib.FORWARD else in_grid.weights / (2 * np.pi) ** self.ndim
+        alpha = complex(float(np.asarray(w).ravel()[0]))
+        if t.requires_grad and torch.is_grad_enabled():
+            return Field(_ZoomFn.apply(t, self, direction, alpha), out_grid)
+        return Field(self._execute_plain(t, direction, alpha), out_grid)
+
+    def _execute_plain(self, t, direction, alpha):
+        out_grid = self.output_grid if direction == _lib.FORWARD else self.input_grid
+        lead = tuple(t.shape[:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        plan = self.plan_for(t.dtype, t.device)
+        out = torch.empty(lead + (out_grid.size,), dtype=t.dtype, device=t.device)
+        plan.execute(direction, t, out, batch, alpha, self._workspace_for(plan, batch, t.device))
+        return out
+
+    def forward(self, field):
+        """Forward transform (zoom_fast_fourier_transform.py:70-92)."""
+        return self._apply(field, _lib.FORWARD)
+
+    def backward(self, field):
+        """Inverse transform with the output weights (zoom_fast_fourier_transform.py:94-116)."""
+        return self._apply(field, _lib.BACKWARD)
+
+    @classmethod
+    def check_if_supported(cls, input_grid, output_grid):
+        """zoom_fast_fourier_transform.py:118-141, and the size limits of the device transform."""
+        if not input_grid.is_regular or not input_grid.is_('cartesian'):
+            raise ValueError('The input grid should be regularly spaced in Cartesian coordinates.')
+        if not output_grid.is_regular or not output_grid.is_('cartesian'):
+            raise ValueError('The output grid should be regularly spaced in Cartesian coordinates.')
+        if input_grid.ndim != output_grid.ndim:
+            raise ValueError('The input_grid must have the same dimensions as the output_grid.')
+        if input_grid.ndim not in (1, 2):
+            raise ValueError('The zoom FFT of this library supports one- and two-dimensional grids.')
+        for n, m in zip(input_grid.dims, output_grid.dims):
+            if int(n) + int(m) - 1 > 8192:
+                raise ValueError('n_in + n_out - 1 = %d exceeds the 8192-point line FFT of the zoom FFT.' % (int(n) + int(m) - 1))
+
+    @classmethod
+    def compute_complexity(cls, input_grid, output_grid):
+        """Operation count of the reference's chirp-z passes (zoom_fast_fourier_transform.py:143-219):
+        the same terms, with the reference's next_fast_len for the convolution length."""
+        import scipy.fft
+        cls.check_if_supported(input_grid, output_grid)
+        cm = input_grid.size
+        ca = 0
+        for i, (n_in, n_out) in enumerate(zip(input_grid.shape, output_grid.shape)):
+            czt_len = scipy.fft.next_fast_len(n_in + n_out - 1)
+            lines = math.prod(output_grid.shape[:i]) * math.prod(input_grid.shape[i + 1:])
+            before = math.prod(output_grid.shape[:i]) * math.prod(input_grid.shape[i:])
+            after = math.prod(output_grid.shape[:i + 1]) * math.prod(input_grid.shape[i + 1:])
+            fft_mults = 0.5 * czt_len * lines * math.log2(czt_len)
+            fft_adds = czt_len * lines * math.log2(czt_len)
+            cm += before + 2 * fft_mults + czt_len * lines + 2 * after
+            ca += 2 * fft_adds
+        num_mult = 4 * cm
+        num_add = 2 * cm + 2 * ca
+        coeffs = Configuration().fourier.zfft.execution_time_prediction_coefficients
+        t = FourierTransform._predict_execution_time(num_mult + num_add, coeffs)
+        return ComputationalComplexity(num_multiplications=num_mult, num_additions=num_add, expected_execution_time=t)
+
+
+__all__ += ['ZoomFastFourierTransform']
+
+
+# ----------------------------------------------------------------------------------------------
 # FastFourierTransform and the planner
 
 

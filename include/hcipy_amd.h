@@ -343,6 +343,33 @@ int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* b
 int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, void* out, int64_t batch,
                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Zoom FFT: the Fourier transform between two regular Cartesian grids of any spacing, as one
+ * Bluestein chirp-z transform per axis.  Replaces ZoomFastFourierTransform
+ * (hcipy/fourier/zoom_fast_fourier_transform.py:10-219) and ChirpZTransform
+ * (hcipy/fourier/chirp_z_transform.py:7-105).
+ *
+ * Grids are given per axis in (x, y) order: sizes, spacings and first coordinates.  Forward maps
+ * a field on the input grid (layout [ny][nx]) to alpha * sum_j f_j exp(-i x_j . u_k) on the
+ * output grid ([my][mx]); backward maps the output grid to the input grid with exp(+i u . x).
+ * The weights are the caller's alpha (the reference multiplies by input_grid.weights forward
+ * and output_grid.weights / (2 pi)^ndim backward).  Each axis needs n_in + n_out - 1 <= 8192.
+ * dtype: HCIPY_DTYPE_C64 or HCIPY_DTYPE_C128 fields.  precision: HCIPY_PREC_FP64 (FP64
+ * chirps, line FFTs and intermediate; for complex64 fields only the loads and stores are FP32)
+ * or, for complex64 fields, HCIPY_PREC_FP32 (all FP32, chirps and filters rounded from FP64).
+ * Set-up (chirps, filter FFTs) runs on `stream`. */
+typedef struct hcipy_zoom_plan_s* hcipy_zoom_plan;
+int hcipy_zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, const double* delta_in,
+                           const double* zero_in, const int* n_out, const double* delta_out, const double* zero_out,
+                           int dtype, int precision, void* stream);
+int hcipy_zoom_plan_destroy(hcipy_zoom_plan plan);
+/* Bytes of workspace (the intermediate of the two axis passes) for `batch` fields; 0 in 1-D. */
+int hcipy_zoom_workspace_size(hcipy_zoom_plan plan, int64_t batch, size_t* bytes);
+/* direction 0: forward (zoom_fast_fourier_transform.py:70-92), 1: backward (:94-116).  `batch`
+ * fields of the plan's dtype, contiguous; in and out must not alias. */
+int hcipy_zoom_execute(hcipy_zoom_plan plan, int direction, const void* in, void* out, int64_t batch, double alpha_re,
+                       double alpha_im, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,10 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
   reference's rejection thresholds for charges 2-8, scalar and vector
   (tests/test_coronagraphy.py:4-28, 81-105), the unit-mask identity, the adjoint identity, and
   the circular-basis identity tying the vector vortex to the scalar one;
+* ``czt``/``zoom_fft``: the dense DFT (``naive_ft``) on regular grids with arbitrary spacings,
+  offsets, odd sizes and output longer than input, in 1-D and 2-D, forward and backward; the
+  chirp-z transform against a direct sum over its z_k (arbitrary complex a and w) and against
+  numpy's FFT when it is one (a = 1, w = exp(-2 pi i / n), m = n);
 * ``stokes_vector``: for pure input states the Mueller rows must agree with the Stokes
   parameters of the propagated Jones vector J e (an independent closed form), and those four
   states span every input Stokes vector; Malus's law for a rotated polarizer;
@@ -260,6 +264,51 @@ def mft_backward(field, input_grid, output_grid):
         res = gemm(alpha, M2.T, inter.T, trans_a=2).T  # (Ny, Nx) = inter . M2^H  (:220)
         out.append(res.reshape(-1))
     return np.asarray(out).reshape(lead + (input_grid.size,))
+
+
+def czt(x, n, m, w, a):
+    """ChirpZTransform.__call__ (hcipy/fourier/chirp_z_transform.py:49-105): X_k = sum_j x_j z_k^-j,
+    z_k = a w^-k, along the last axis by Bluestein's algorithm with a numpy FFT of length
+    next_fast_len(n + m - 1); the chirp filter holds w^-(d^2/2) for d = -(n-1) .. m-1 in that
+    order and the result is read from offset n - 1."""
+    from scipy.fft import next_fast_len
+    w = complex(w)
+    a = complex(a)
+    k = np.arange(max(m, n))
+    wk2 = w ** (k ** 2 / 2)
+    nfft = next_fast_len(n + m - 1)
+    Awk2 = a ** -k[:n] * wk2[:n]
+    Fwk2 = np.fft.fft(1 / np.hstack((wk2[n - 1:0:-1], wk2[:m])), n=nfft)
+    y = np.fft.ifft(np.fft.fft(np.asarray(x) * Awk2, n=nfft) * Fwk2)
+    return y[..., n - 1:n + m - 1] * wk2[:m]
+
+
+def zoom_fft(field, input_grid, output_grid, adjoint=False):
+    """ZoomFastFourierTransform.forward / backward (hcipy/fourier/zoom_fast_fourier_transform.py:
+    42-116): a chirp-z transform along x (the last axis of the shaped field) with
+    w = exp(-i du dx), a = exp(i u0 dx) and the shift exp(-i u x0), then along y; backward
+    swaps the grids, flips the signs and weights with output_grid.weights / (2 pi)^ndim."""
+    f = np.asarray(field)
+    if adjoint:
+        input_grid, output_grid = output_grid, input_grid
+        sgn = 1
+        weight = np.asarray(input_grid.weights).ravel()[0] / (2 * np.pi) ** input_grid.ndim
+    else:
+        sgn = -1
+        weight = np.asarray(input_grid.weights).ravel()[0]
+    lead = f.shape[:-1]
+    g = (f * weight).reshape(lead + tuple(input_grid.shape))
+    nd = input_grid.ndim
+    for i in range(nd):
+        n, m = int(input_grid.dims[i]), int(output_grid.dims[i])
+        dx, du = input_grid.delta[i], output_grid.delta[i]
+        x0, u0 = input_grid.zero[i], output_grid.zero[i]
+        w = np.exp(sgn * 1j * du * dx)
+        a = np.exp(-sgn * 1j * u0 * dx)
+        shift = np.exp(sgn * 1j * output_grid.sep[i] * x0)
+        axis = g.ndim - 1 - i  # x is the last axis
+        g = np.moveaxis(czt(np.moveaxis(g, axis, -1), n, m, w, a) * shift, -1, axis)
+    return g.reshape(lead + (-1,))
 
 
 def naive_ft(field, input_grid, output_grid, adjoint=False):

@@ -61,3 +61,24 @@ def test_kernels_refuse_cpu_tensors():
     mft = MatrixFourierTransform(g, make_focal_grid(2, 2))
     with pytest.raises(_lib.HcipyError):
         mft.forward(torch.zeros(g.size, dtype=torch.complex128))
+
+
+def test_zoom_entry_points_reject_bad_arguments():
+    """Argument checks of the zoom-FFT entry points run before any device work."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    ci = ctypes.c_int * 2
+    cd = ctypes.c_double * 2
+    n, d, z = ci(8, 8), cd(0.1, 0.1), cd(0.0, 0.0)
+    assert lib.hcipy_zoom_plan_create(ctypes.byref(h), 3, n, d, z, n, d, z, 1, 0, None) == 1
+    assert b'one- and two-dimensional' in lib.hcipy_get_last_error()
+    assert lib.hcipy_zoom_plan_create(ctypes.byref(h), 2, n, d, z, n, d, z, 2, 0, None) == 1
+    assert lib.hcipy_zoom_plan_create(ctypes.byref(h), 2, n, d, z, n, d, z, 1, 1, None) == 1  # FP32 for complex128
+    assert b'precision' in lib.hcipy_get_last_error()
+    big = ci(6000, 8)
+    assert lib.hcipy_zoom_plan_create(ctypes.byref(h), 2, big, d, z, big, d, z, 1, 0, None) == 1
+    assert b'8192' in lib.hcipy_get_last_error()
+    size = ctypes.c_size_t()
+    assert lib.hcipy_zoom_workspace_size(None, 1, ctypes.byref(size)) == 1
+    assert lib.hcipy_zoom_execute(None, 0, None, None, 1, 1.0, 0.0, None, 0, None) == 1
+    assert lib.hcipy_zoom_plan_destroy(None) == 0

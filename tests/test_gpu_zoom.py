@@ -1,0 +1,120 @@
+"""Zoom FFT (Bluestein chirp-z per axis) on the device against the oracle, the matrix Fourier
+transform and its own adjoint (GPU).
+
+Tolerances: complex128 runs FP64 line FFTs and FP64 chirps, 1e-12 of the output peak against
+the FP64 dense transform (the oracle's MFT, pinned to the dense DFT).  The oracle's restatement
+of the reference's chirp-z (powers w^(k^2 / 2) of a unit complex number rounded to FP64) drifts
+by ~k^2 * 1e-16 in modulus: 2e-10 at k = 5000, so the device is held to it at 1e-12 plus that
+bound.  complex64 runs FP32 line FFTs of length L <= 8192 with chirps rounded from
+FP64 (precision 'fp32'); its error is a few FP32 roundings per butterfly level relative to the
+field's norm, held here to 2e-6 of the output peak on random fields (whose peak is close to
+their norm) and 1e-6 peak-relative intensity on the 1024^2 -> 512^2 Fraunhofer workload.  The
+default 'fp64' complex64 path rounds only its loads and stores: 1e-6 on random fields (the
+FP32 input rounding) and 1e-7 peak-relative intensity on the Fraunhofer workload."""
+import numpy as np
+import pytest
+import torch
+
+import hcipy_amd as ha
+from oracle import hcipy_oracle as o
+
+from helpers import dev, host, og, peak_rel_field, say
+
+pytestmark = pytest.mark.gpu
+
+_CASES = [
+    ([12, 9], [1.1, 0.8], [0.05, -0.02], [7, 15], [20.0, 30.0], [1.0, -0.5]),
+    ([10], [1.0], [0.0], [23], [40.0], [0.3]),
+    ([64, 48], [1.0, 0.75], [0.0, 0.0], [200, 130], [90.0, 60.0], [0.2, -0.4]),
+    ([5000], [1.0], [0.1], [3000], [300.0], [-2.0]),  # line FFT length 8192
+]
+
+
+def _grids(c):
+    di, ei, ci, do, eo, co = c
+    return ha.make_uniform_grid(di, ei, center=ci), ha.make_uniform_grid(do, eo, center=co)
+
+
+def _crand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize('case', range(len(_CASES)))
+@pytest.mark.parametrize('dtype,precision,tol', [(np.complex128, None, 1e-12), (np.complex64, 'fp64', 1e-6),
+                                                 (np.complex64, 'fp32', 2e-6)])
+def test_zoom_matches_oracle(case, dtype, precision, tol):
+    pg, fg = _grids(_CASES[case])
+    say('zoom case', case, 'dims', tuple(pg.dims), '->', tuple(fg.dims), dtype.__name__, precision)
+    zf = ha.ZoomFastFourierTransform(pg, fg, precision=precision)
+    f = _crand((3, pg.size), case)
+    out = host(zf.forward(ha.Field(dev(f.astype(dtype)), pg)))
+    f128 = f.astype(dtype).astype(np.complex128)
+    ref = o.mft_forward(f128, og(pg), og(fg))
+    assert out.shape == (3, fg.size) and out.dtype == dtype
+    err = peak_rel_field(out, ref)
+    drift = 1e-16 * max(max(pg.dims), max(fg.dims)) ** 2
+    err_zoom = peak_rel_field(out, o.zoom_fft(f128, og(pg), og(fg)))
+    say('  forward error vs dense %.2e, vs the chirp-z restatement %.2e' % (err, err_zoom))
+    assert err < tol
+    assert err_zoom < tol + 10 * drift
+    F = _crand((2, 2, fg.size), case + 10)
+    back = host(zf.backward(ha.Field(dev(F.astype(dtype)), fg)))
+    F128 = F.astype(dtype).astype(np.complex128)
+    refb = o.mft_backward(F128, og(pg), og(fg))
+    assert back.shape == (2, 2, pg.size)
+    assert peak_rel_field(back, refb) < tol
+    assert peak_rel_field(back, o.zoom_fft(F128, og(pg), og(fg), adjoint=True)) < tol + 10 * drift
+
+
+def test_zoom_equals_mft_and_adjoint():
+    """The zoom FFT and the MFT compute the same sums; <Z f, g> = <f, Z^dagger g> with the
+    dagger the backward transform up to the weights; the autograd gradient is that adjoint."""
+    pupil = ha.make_pupil_grid(96)
+    focal = ha.make_focal_grid(3, 10)
+    zf = ha.ZoomFastFourierTransform(pupil, focal)
+    mft = ha.MatrixFourierTransform(pupil, focal)
+    f = _crand(pupil.size, 1)
+    a = host(zf.forward(ha.Field(dev(f), pupil)))
+    b = host(mft.forward(ha.Field(dev(f), pupil)))
+    assert peak_rel_field(a, b) < 1e-12
+    g = _crand(focal.size, 2)
+    assert peak_rel_field(host(zf.backward(ha.Field(dev(g), focal))), host(mft.backward(ha.Field(dev(g), focal)))) < 1e-12
+    wi = float(np.asarray(pupil.weights).ravel()[0])
+    wo = float(np.asarray(focal.weights).ravel()[0]) / (2 * np.pi) ** 2
+    lhs = np.vdot(g, a)
+    rhs = np.vdot(host(zf.backward(ha.Field(dev(g), focal))), f) * wi / wo
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    x = dev(f).requires_grad_(True)
+    y = zf.forward(ha.Field(x, pupil)).as_subclass(torch.Tensor)
+    torch.real(torch.sum(torch.conj(dev(g)) * y)).backward()
+    assert peak_rel_field(host(x.grad), host(zf.backward(ha.Field(dev(g), focal))) * wi / wo) < 1e-12
+
+
+def test_zoom_fraunhofer_1024_to_512():
+    """1024^2 pupil -> 512^2 focal plane (q = 4, 64 lambda/D): complex128 against the FP64 MFT,
+    complex64 at <= 1e-7 peak-relative intensity error."""
+    pupil = ha.make_pupil_grid(1024)
+    focal = ha.make_focal_grid(4, 64)
+    ap = ((pupil.x ** 2 + pupil.y ** 2) <= 0.25).astype(float)
+    rng = np.random.default_rng(3)
+    E = ap * np.exp(1j * 0.2 * rng.standard_normal(pupil.size))
+    ref = host(ha.MatrixFourierTransform(pupil, focal).forward(ha.Field(dev(E), pupil)))
+    zf = ha.ZoomFastFourierTransform(pupil, focal)
+    z128 = host(zf.forward(ha.Field(dev(E), pupil)))
+    assert peak_rel_field(z128, ref) < 1e-12
+    Ir = np.abs(ref) ** 2
+    errs = {}
+    for prec in ('fp64', 'fp32'):
+        z64 = host(ha.ZoomFastFourierTransform(pupil, focal, precision=prec).forward(ha.Field(dev(E.astype(np.complex64)), pupil)))
+        errs[prec] = np.abs(np.abs(z64.astype(np.complex128)) ** 2 - Ir).max() / Ir.max()
+        say('zoom 1024->512 complex64 %s peak-relative intensity error %.3e' % (prec, errs[prec]))
+    assert errs['fp64'] < 1e-7
+    assert errs['fp32'] < 1e-6
+
+
+def test_zoom_empty_batch_and_planner():
+    pg, fg = _grids(_CASES[0])
+    zf = ha.ZoomFastFourierTransform(pg, fg)
+    y = zf.forward(ha.Field(torch.zeros((0, pg.size), dtype=torch.complex64, device='cuda'), pg))
+    assert tuple(y.shape) == (0, fg.size)

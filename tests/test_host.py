@@ -1,5 +1,6 @@
 """Host-side logic of the package that needs no GPU."""
 import numpy as np
+import pytest
 from scipy.signal import windows
 
 from hcipy_amd.coronagraphy import periodic_tukey_window
@@ -8,3 +9,19 @@ from hcipy_amd.coronagraphy import periodic_tukey_window
 def test_periodic_tukey_window_matches_scipy():
     for n in (4, 8, 31, 32, 64):
         np.testing.assert_allclose(periodic_tukey_window(n), windows.tukey(n, 1, False), rtol=0, atol=1e-15)
+
+
+def test_zoom_fft_support_and_complexity():
+    """check_if_supported messages (zoom_fast_fourier_transform.py:118-141) and the planner's
+    estimate at 1024^2 -> 512^2 (the reference's coefficients give ~21 ms, the MFT's CPU model
+    ~20 ms)."""
+    import hcipy_amd as ha
+    pupil = ha.make_pupil_grid(1024)
+    focal = ha.make_focal_grid(4, 64)
+    assert tuple(focal.dims) == (512, 512)
+    c = ha.ZoomFastFourierTransform.compute_complexity(pupil, focal)
+    assert 0.019 < c.expected_execution_time < 0.023
+    with pytest.raises(ValueError, match='same dimensions'):
+        ha.ZoomFastFourierTransform(pupil, ha.make_uniform_grid([8], [1.0]))
+    with pytest.raises(ValueError, match='8192'):
+        ha.ZoomFastFourierTransform(ha.make_uniform_grid([5000], [1.0]), ha.make_uniform_grid([4000], [1.0]))

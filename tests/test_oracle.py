@@ -513,3 +513,37 @@ def test_vector_vortex_circular_input_flips_handedness():
     back = o.multiscale_forward(o.fraunhofer_backward(img, pg, fg), pg, grids, masks, adjoint=True)
     sb = o.stokes_vector(back, (1, 0, 0, 1))
     np.testing.assert_allclose(sb[0], sb[3], rtol=1e-9, atol=1e-12 * sb[0].max())
+
+
+def test_czt_against_direct_sum_and_fft():
+    """chirp_z_transform.py:7-105: Bluestein's result equals the direct sum over z_k = a w^-k for
+    off-unit-circle a and w, and numpy's FFT for a = 1, w = exp(-2 pi i / n), m = n."""
+    rng = np.random.default_rng(0)
+    for n, m in ((13, 7), (5, 21), (16, 16)):
+        x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+        w, a = 0.98 * np.exp(-0.3j), 1.01 * np.exp(0.2j)
+        zk = a * w ** -np.arange(m)
+        ref = np.einsum('bj,kj->bk', x, zk[:, None] ** -np.arange(n)[None, :])
+        assert np.abs(o.czt(x, n, m, w, a) - ref).max() < 1e-13 * np.abs(ref).max()
+    x = rng.standard_normal(13) + 1j * rng.standard_normal(13)
+    assert np.abs(o.czt(x, 13, 13, np.exp(-2j * np.pi / 13), 1) - np.fft.fft(x)).max() < 1e-13
+
+
+@pytest.mark.parametrize('dims_in,dims_out', [((12, 9), (7, 15)), ((10,), (23,)), ((32, 32), (16, 48))])
+def test_zoom_fft_equals_dense_dft(dims_in, dims_out):
+    """zoom_fast_fourier_transform.py:70-116 equals the dense DFT (naive_fourier_transform.py) on
+    regular grids with arbitrary spacings and offsets, forward and backward, and the MFT."""
+    rng = np.random.default_rng(1)
+    nd = len(dims_in)
+    pg = o.make_uniform_grid(list(dims_in), [1.1, 0.8][:nd], center=[0.05, -0.02][:nd])
+    fg = o.make_uniform_grid(list(dims_out), [20.0, 30.0][:nd], center=[1.0, -0.5][:nd])
+    f = rng.standard_normal((2, pg.size)) + 1j * rng.standard_normal((2, pg.size))
+    z = o.zoom_fft(f, pg, fg)
+    ref = np.array([o.naive_ft(v, pg, fg) for v in f])
+    assert z.shape == (2, fg.size)
+    assert np.abs(z - ref).max() < 1e-13 * np.abs(ref).max()
+    F = rng.standard_normal(fg.size) + 1j * rng.standard_normal(fg.size)
+    refb = o.naive_ft(F, pg, fg, adjoint=True)
+    assert np.abs(o.zoom_fft(F, pg, fg, adjoint=True) - refb).max() < 1e-13 * np.abs(refb).max()
+    if nd == 2:
+        assert np.abs(o.mft_forward(f, pg, fg) - z).max() < 1e-13 * np.abs(z).max()
