@@ -15,7 +15,8 @@
 // a linear convolution evaluated as a circular one of length L >= n + m - 1 (a power of two):
 // h_d sits at index d mod L for d in [-(n - 1), m - 1], zero elsewhere, so no wrapped term reaches
 // the outputs k < m.  Per row, in LDS: y = x A (zero-padded to L), FFT, times H = FFT(h) / L,
-// inverse FFT, outputs k < m times alpha C_k.
+// inverse FFT, outputs k < m times alpha C_k; the line FFTs are register-resident radix-16
+// Stockham stages with one LDS round trip per stage.
 //
 // A pass writes its output transposed, out[b][k][row], so that the next axis again reads rows:
 // the 2-D forward transform is pass x (rows of nx -> [mx][ny]) then pass y (rows of ny ->
@@ -118,67 +119,87 @@ __global__ void cast_c128_to_c64_kernel(const double2* __restrict__ in, float2* 
   if (t < n) out[t] = make_float2((float)in[t].x, (float)in[t].y);
 }
 
+// Thread layout of the register-resident line FFT: E = 8 points per thread (measured faster than
+// 16 on the 2048-point lines of a 1024^2 -> 512^2 transform: twice the waves for the few rows a
+// pass has), P = N / E threads per line, workgroups of max(256, P) threads holding LINES lines.
+// FP32 lines read their twiddles from an LDS copy (measured faster); FP64 lines read them from
+// global memory (the LDS copy lowers their occupancy and was measured slower).
+template <typename T, int N>
+struct HrCfg {
+  static constexpr int E = N >= 8 ? 8 : N;
+  static constexpr int P = N / E;
+  static constexpr int WG = P > 256 ? P : 256;
+  static constexpr int LINES = WG / P;
+  static constexpr bool TW_LDS = sizeof(T) == 4;
+  static constexpr size_t LDS_BYTES = (size_t)(LINES + (TW_LDS ? 1 : 0)) * N * 2 * sizeof(T);
+};
+
 // One axis pass over `total_rows` rows of n samples (row r = b * rows + y), output transposed to
-// out[(b * m + k) * rows + y].  Workgroups take LINES consecutive rows; the workgroup index is
-// remapped so that the consecutive row groups of one output cache line run on one XCD (the
-// hardware deals workgroups to the 8 XCDs round robin).
+// out[(b * m + k) * rows + y].  Each row: load x A (zero-padded to L) in registers, FFT, times H,
+// inverse FFT, outputs k < m times alpha C.  Workgroups take LINES consecutive rows; the
+// workgroup index is remapped so that the consecutive row groups of one output cache line run on
+// one XCD (the hardware deals workgroups to the 8 XCDs round robin).
 template <typename TI, typename TO, typename T, int L>
-__global__ __launch_bounds__(256) void czt_rows_kernel(const typename Cplx<TI>::V* __restrict__ in,
-                                                       typename Cplx<TO>::V* __restrict__ out,
-                                                       const typename Cplx<T>::V* __restrict__ A,
-                                                       const typename Cplx<T>::V* __restrict__ H,
-                                                       const typename Cplx<T>::V* __restrict__ C,
-                                                       const typename Cplx<T>::V* __restrict__ tw, int n, int m,
-                                                       int rows, int64_t total_rows, int64_t groups, T alpha_re,
-                                                       T alpha_im) {
+__global__ __launch_bounds__((HrCfg<T, L>::WG)) void czt_rows_kernel(const typename Cplx<TI>::V* __restrict__ in,
+                                                                typename Cplx<TO>::V* __restrict__ out,
+                                                                const typename Cplx<T>::V* __restrict__ A,
+                                                                const typename Cplx<T>::V* __restrict__ H,
+                                                                const typename Cplx<T>::V* __restrict__ C,
+                                                                const typename Cplx<T>::V* __restrict__ tw, int n,
+                                                                int m, int rows, int64_t total_rows, int64_t groups,
+                                                                T alpha_re, T alpha_im) {
   using V = typename Cplx<T>::V;
   using VI = typename Cplx<TI>::V;
   using VO = typename Cplx<TO>::V;
-  constexpr int TPL = CztCfg<L>::TPL;
-  constexpr int LINES = CztCfg<L>::LINES;
+  using Cfg = HrCfg<T, L>;
+  constexpr int E = Cfg::E;
+  constexpr int P = Cfg::P;
+  constexpr int LINES = Cfg::LINES;
   extern __shared__ __align__(16) unsigned char czt_lds_raw[];
   V* lds = reinterpret_cast<V*>(czt_lds_raw);
   const int64_t per_xcd = (groups + 7) / 8;
   const int64_t g = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   if (g >= groups) return;  // uniform over the workgroup, before any barrier
-  const int line = threadIdx.x / TPL;
-  const int tl = threadIdx.x % TPL;
+  const int line = threadIdx.x / P;
+  const int tl = threadIdx.x % P;
   const int64_t r = g * LINES + line;
   const bool valid = r < total_rows;
   V* Lp = lds + (size_t)line * L;
-  const V zero = {0, 0};
+  const V* twp = tw;
+  if constexpr (Cfg::TW_LDS) {
+    V* twl = lds + (size_t)LINES * L;
+    for (int t = threadIdx.x; t < L; t += Cfg::WG) twl[t] = tw[t];
+    __syncthreads();
+    twp = twl;
+  }
   const VI* src = in + (valid ? r : 0) * (int64_t)n;
-  constexpr int PR = L / TPL;  // samples per thread; all loads in flight before the LDS stores
-  V g_[PR];
+  V v[E];
 #pragma unroll
-  for (int i = 0; i < PR; ++i) {
-    const int j = tl + i * TPL;
+  for (int q = 0; q < E; ++q) {
+    const int j = tl + q * P;
+    V x = {0, 0};
     if (valid && j < n) {
-      const VI v = src[j];
-      g_[i] = cmul(V{(T)v.x, (T)v.y}, A[j]);
-    } else {
-      g_[i] = zero;
+      const VI e = src[j];
+      x = cmul(V{(T)e.x, (T)e.y}, A[j]);
     }
+    v[q] = x;
   }
+  hr_fft<T, L, P>(v, Lp, tl, twp, -1);
 #pragma unroll
-  for (int i = 0; i < PR; ++i) Lp[tl + i * TPL] = g_[i];
-  __syncthreads();
-  line_fft<T, L, TPL>(Lp, tl, tw, -1, 0);
-#pragma unroll
-  for (int i = 0; i < PR; ++i) {
-    const int t = tl + i * TPL;
-    Lp[t] = cmul(Lp[t], H[t]);
-  }
-  __syncthreads();
-  line_fft<T, L, TPL>(Lp, tl, tw, +1, 0);
+  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], H[tl + q * P]);
+  hr_fft<T, L, P>(v, Lp, tl, twp, +1);
   if (valid) {
     const V alpha = {alpha_re, alpha_im};
     const int64_t b = r / rows;
     const int64_t y = r - b * rows;
     VO* dst = out + b * (int64_t)m * rows + y;
-    for (int k = tl; k < m; k += TPL) {
-      const V v = cmul(Lp[k], cmul(C[k], alpha));
-      dst[(int64_t)k * rows] = VO{(TO)v.x, (TO)v.y};
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int k = tl + q * P;
+      if (k < m) {
+        const V o = cmul(v[q], cmul(C[k], alpha));
+        dst[(int64_t)k * rows] = VO{(TO)o.x, (TO)o.y};
+      }
     }
   }
 }
@@ -290,8 +311,9 @@ int launch_pass(const CztAxis& a, const void* in, void* out, int rows, int64_t t
   using V = typename Cplx<T>::V;
   using VI = typename Cplx<TI>::V;
   using VO = typename Cplx<TO>::V;
-  constexpr int LINES = CztCfg<L>::LINES;
-  const size_t lds = (size_t)LINES * L * sizeof(V);
+  using Cfg = HrCfg<T, L>;
+  constexpr int LINES = Cfg::LINES;
+  const size_t lds = Cfg::LDS_BYTES;
   auto kern = czt_rows_kernel<TI, TO, T, L>;
   if (lds > 65536)
     HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -304,7 +326,8 @@ int launch_pass(const CztAxis& a, const void* in, void* out, int rows, int64_t t
   const V* H = f32 ? (const V*)a.H32 : (const V*)a.H64;
   const V* C = f32 ? (const V*)a.C32 : (const V*)a.C64;
   const V* tw = f32 ? (const V*)a.tw32 : (const V*)a.tw64;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), lds, s, (const VI*)in, (VO*)out, A, H, C, tw, a.n, a.m,
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(Cfg::WG), lds, s, (const VI*)in, (VO*)out, A, H, C, tw,
+                     a.n, a.m,
                      rows, total_rows, groups, (T)are, (T)aim);
   return check_hip(hipGetLastError(), "czt_rows_kernel");
 }

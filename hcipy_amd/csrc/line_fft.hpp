@@ -145,3 +145,169 @@ __device__ void line_fft(typename Cplx<T>::V* line, int tl, const typename Cplx<
 
 }  // namespace
 }  // namespace hcipy
+
+namespace hcipy {
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Register-resident line FFT.  P threads share one line of N = P * E points; every thread holds
+// E points in registers.  Stockham stages of radix E run as in-register DFTs with one LDS round
+// trip between stages; N = E^a * RL, the last stage has radix RL < E (E / RL butterflies per
+// thread) or is the a-th full stage when RL = 1.  On entry and on return slot q of v holds
+// point tl + q * P (natural order), so a caller loads and stores coalesced rows and can chain
+// transforms in registers.  sign = -1: forward (numpy fft), +1: inverse without 1/N.
+// tw[t] = exp(-2 pi i t / N).  Every LDS write is bracketed by barriers, so the line may be
+// reused between calls.
+
+template <typename V, typename T>
+__device__ inline V cmul_tw(V a, V w, T s) {
+  w.y *= -s;  // tw holds exp(-i..): forward (s = -1) keeps it, inverse conjugates
+  return cmul(a, w);
+}
+
+template <typename V, typename T>
+__device__ inline void dft2(V& a, V& b) {
+  const V t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <typename V, typename T>
+__device__ inline void dft4(V& x0, V& x1, V& x2, V& x3, T s) {
+  const V a = cadd(x0, x2), b = csub(x0, x2), c = cadd(x1, x3), d = cmul_si(csub(x1, x3), s);
+  x0 = cadd(a, c);
+  x2 = csub(a, c);
+  x1 = cadd(b, d);
+  x3 = csub(b, d);
+}
+
+// x[0..7] -> X[0..7] in natural order: two DFT4s and the radix-2 combine with w8^k.
+template <typename V, typename T>
+__device__ inline void dft8(V* x, T s) {
+  dft4<V, T>(x[0], x[2], x[4], x[6], s);
+  dft4<V, T>(x[1], x[3], x[5], x[7], s);
+  const T h = (T)0.70710678118654752440;
+  const V o1 = cmul(x[3], V{h, s * h});   // w8^1 O1
+  const V o2 = cmul_si(x[5], s);          // w8^2 O2
+  const V o3 = cmul(x[7], V{-h, s * h});  // w8^3 O3
+  const V e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6], oo0 = x[1];
+  x[0] = cadd(e0, oo0);
+  x[4] = csub(e0, oo0);
+  x[1] = cadd(e1, o1);
+  x[5] = csub(e1, o1);
+  x[2] = cadd(e2, o2);
+  x[6] = csub(e2, o2);
+  x[3] = cadd(e3, o3);
+  x[7] = csub(e3, o3);
+}
+
+// x[0..15] -> X[0..15] in natural order: n = 4 n1 + n2, DFT4 over n1, twiddles w16^(n2 k1),
+// DFT4 over n2, then X[k1 + 4 k2] is read from slot 4 k1 + k2.
+template <typename V, typename T>
+__device__ inline void dft16(V* x, T s) {
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) dft4<V, T>(x[n2], x[4 + n2], x[8 + n2], x[12 + n2], s);
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+  // slot 4 k1 + n2 *= w16^(n2 k1), w16^m = exp(s 2 pi i m / 16)
+  x[5] = cmul(x[5], V{c1, s * s1});       // m = 1
+  x[6] = cmul(x[6], V{h, s * h});         // m = 2
+  x[7] = cmul(x[7], V{s1, s * c1});       // m = 3
+  x[9] = cmul(x[9], V{h, s * h});         // m = 2
+  x[10] = cmul_si(x[10], s);              // m = 4
+  x[11] = cmul(x[11], V{-h, s * h});      // m = 6
+  x[13] = cmul(x[13], V{s1, s * c1});     // m = 3
+  x[14] = cmul(x[14], V{-h, s * h});      // m = 6
+  x[15] = cmul(x[15], V{-c1, -s * s1});   // m = 9
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) dft4<V, T>(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3], s);
+  V y[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) y[k1 + 4 * k2] = x[4 * k1 + k2];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) x[i] = y[i];
+}
+
+template <typename V, typename T, int R>
+__device__ inline void dft_r(V* x, T s) {
+  if constexpr (R == 2) {
+    dft2<V, T>(x[0], x[1]);
+  } else if constexpr (R == 4) {
+    dft4<V, T>(x[0], x[1], x[2], x[3], s);
+  } else if constexpr (R == 8) {
+    dft8<V, T>(x, s);
+  } else {
+    static_assert(R == 16, "radix must be 2, 4, 8 or 16");
+    dft16<V, T>(x, s);
+  }
+}
+
+template <int N, int P>
+struct HrShape {
+  static constexpr int E = N / P;
+  static constexpr int LOGN = __builtin_ctz(N);
+  static constexpr int LOGE = __builtin_ctz(E);
+  static constexpr int A = LOGN / LOGE;                // full stages of radix E
+  static constexpr int RL = 1 << (LOGN - A * LOGE);    // radix of the last stage (1: none)
+  static_assert(E == 2 || E == 4 || E == 8 || E == 16, "points per thread must be 2, 4, 8 or 16");
+  static_assert(P * E == N, "N must be P * E");
+};
+
+template <typename T, int N, int P>
+__device__ void hr_fft(typename Cplx<T>::V (&v)[N / P], typename Cplx<T>::V* line, int tl,
+                       const typename Cplx<T>::V* __restrict__ tw, int sign) {
+  using V = typename Cplx<T>::V;
+  using S = HrShape<N, P>;
+  constexpr int E = S::E;
+  constexpr int A = S::A;
+  constexpr int RL = S::RL;
+  const T s = (T)sign;
+  // stage 1 (span 1): butterfly tl takes points tl + r P, no twiddles
+  dft_r<V, T, E>(v, s);
+  if constexpr (A == 1 && RL == 1) return;  // N == E: P == 1 and the outputs are in order
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < E; ++r) line[tl * E + r] = v[r];
+  __syncthreads();
+  int Ns = E;
+#pragma unroll
+  for (int st = 1; st < A; ++st) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = line[tl + r * P];
+    const int k = tl & (Ns - 1);
+    const int step = N / (Ns * E);
+#pragma unroll
+    for (int r = 1; r < E; ++r) v[r] = cmul_tw(v[r], tw[(r * k * step) & (N - 1)], s);
+    dft_r<V, T, E>(v, s);
+    if (st == A - 1 && RL == 1) return;  // last stage: k == tl, outputs tl + r P are in order
+    __syncthreads();
+    const int base = (tl - k) * E + k;
+#pragma unroll
+    for (int r = 0; r < E; ++r) line[base + r * Ns] = v[r];
+    __syncthreads();
+    Ns *= E;
+  }
+  if constexpr (RL > 1) {
+    // span N / RL: butterfly j = tl + b P (b < E / RL) takes slots b + r (E / RL), twiddles w^(r j)
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = line[tl + q * P];
+    constexpr int NB = E / RL;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = tl + b * P;
+      V x[RL];
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        x[r] = v[b + r * NB];
+        if (r > 0) x[r] = cmul_tw(x[r], tw[(r * j) & (N - 1)], s);
+      }
+      dft_r<V, T, RL>(x, s);
+#pragma unroll
+      for (int r = 0; r < RL; ++r) v[b + r * NB] = x[r];
+    }
+  }
+}
+
+}  // namespace
+}  // namespace hcipy
