@@ -131,7 +131,8 @@ struct HrCfg {
   static constexpr int WG = P > 256 ? P : 256;
   static constexpr int LINES = WG / P;
   static constexpr bool TW_LDS = sizeof(T) == 4;
-  static constexpr size_t LDS_BYTES = (size_t)(LINES + (TW_LDS ? 1 : 0)) * N * 2 * sizeof(T);
+  static constexpr int LINE = hr_lds_points(N);  // LDS slots per line
+  static constexpr size_t LDS_BYTES = ((size_t)LINES * LINE + (TW_LDS ? N : 0)) * 2 * sizeof(T);
 };
 
 // One axis pass over `total_rows` rows of n samples (row r = b * rows + y), output transposed to
@@ -164,10 +165,10 @@ __global__ __launch_bounds__((HrCfg<T, L>::WG)) void czt_rows_kernel(const typen
   const int tl = threadIdx.x % P;
   const int64_t r = g * LINES + line;
   const bool valid = r < total_rows;
-  V* Lp = lds + (size_t)line * L;
+  V* Lp = lds + (size_t)line * Cfg::LINE;
   const V* twp = tw;
   if constexpr (Cfg::TW_LDS) {
-    V* twl = lds + (size_t)LINES * L;
+    V* twl = lds + (size_t)LINES * Cfg::LINE;
     for (int t = threadIdx.x; t < L; t += Cfg::WG) twl[t] = tw[t];
     __syncthreads();
     twp = twl;

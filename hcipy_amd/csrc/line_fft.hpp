@@ -157,7 +157,12 @@ namespace {
 // point tl + q * P (natural order), so a caller loads and stores coalesced rows and can chain
 // transforms in registers.  sign = -1: forward (numpy fft), +1: inverse without 1/N.
 // tw[t] = exp(-2 pi i t / N).  Every LDS write is bracketed by barriers, so the line may be
-// reused between calls.
+// reused between calls.  Point i of the line sits at LDS index i + i / 16 (one pad slot per 16
+// points): stage 1 writes 16 consecutive points per thread, and without the pad the threads of a
+// wave would hit the same banks; a line needs hr_lds_points(N) slots.
+
+__host__ __device__ constexpr int hr_lds_points(int n) { return n + n / 16; }
+__device__ inline int hr_pad(int i) { return i + (i >> 4); }
 
 template <typename V, typename T>
 __device__ inline V cmul_tw(V a, V w, T s) {
@@ -268,13 +273,13 @@ __device__ void hr_fft(typename Cplx<T>::V (&v)[N / P], typename Cplx<T>::V* lin
   if constexpr (A == 1 && RL == 1) return;  // N == E: P == 1 and the outputs are in order
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < E; ++r) line[tl * E + r] = v[r];
+  for (int r = 0; r < E; ++r) line[hr_pad(tl * E + r)] = v[r];
   __syncthreads();
   int Ns = E;
 #pragma unroll
   for (int st = 1; st < A; ++st) {
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = line[tl + r * P];
+    for (int r = 0; r < E; ++r) v[r] = line[hr_pad(tl + r * P)];
     const int k = tl & (Ns - 1);
     const int step = N / (Ns * E);
 #pragma unroll
@@ -284,14 +289,14 @@ __device__ void hr_fft(typename Cplx<T>::V (&v)[N / P], typename Cplx<T>::V* lin
     __syncthreads();
     const int base = (tl - k) * E + k;
 #pragma unroll
-    for (int r = 0; r < E; ++r) line[base + r * Ns] = v[r];
+    for (int r = 0; r < E; ++r) line[hr_pad(base + r * Ns)] = v[r];
     __syncthreads();
     Ns *= E;
   }
   if constexpr (RL > 1) {
     // span N / RL: butterfly j = tl + b P (b < E / RL) takes slots b + r (E / RL), twiddles w^(r j)
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = line[tl + q * P];
+    for (int q = 0; q < E; ++q) v[q] = line[hr_pad(tl + q * P)];
     constexpr int NB = E / RL;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
