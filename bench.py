@@ -220,6 +220,17 @@ def main():
             'hbm_gbs_compulsory': round(compulsory / sec / 1e9, 1), 'hbm_gbs_pass_bytes': round(passes / sec / 1e9, 1),
             'peak_gbs': 8000.0, 'internal_grid': [NX, NX],
             'gflop': round(2 * 5 * NX * NX * math.log2(NX * NX) / 1e9, 3)}
+        # The same 1024^2 -> 512^2 Fraunhofer transform as a zoom FFT (Bluestein chirp-z per axis):
+        # complex64 fields with FP64 chirps, line FFTs and intermediate, and complex128 fields.
+        d64 = prop64.instance(wavelength)
+        for key, prec, xin in (('zoom_fft_1024_to_512_c64_fp64', 'fp64', x), ('zoom_fft_1024_to_512_c64_fp32', 'fp32', x),
+                               ('zoom_fft_1024_to_512_c128', None, x.to(torch.complex128))):
+            zf = ha.ZoomFastFourierTransform(pupil, d64.uv_grid, precision=prec)
+            zin = ha.Field(xin, pupil)
+            sec, o = time_graph(lambda: zf.forward(zin))
+            o = o.as_subclass(torch.Tensor).to(torch.complex128) * complex(d64.norm_factor)
+            out[key] = {'value': round(1.0 / sec, 1), 'unit': 'transforms/s', 'ms': round(sec * 1e3, 4),
+                        'peak_rel_intensity_error': err(o)}
         # Lyot coronagraph chain: 1024^2 pupil -> 256^2 focal-plane mask -> Lyot plane (stop 0.95 D)
         # -> 512^2 focal intensity, f16x2.
         lp, lfg, lfocal, lE, lfpm, lstop, _ = workloads.lyot_1024()
@@ -236,6 +247,11 @@ def main():
         sec, _ = time_graph(lambda: vc.forward(ha.Wavefront(e64, 1.0)).electric_field)
         out['vortex_1024_charge2_f16x2'] = {'value': round(1.0 / sec, 1), 'unit': 'forward/s', 'ms': round(sec * 1e3, 4),
                                             'levels': len(vc.focal_grids)}
+        # Charge-2 vector vortex on the same pupil: 2 x 2 Jones-matrix field (an unpolarized input).
+        vvc = ha.VectorVortexCoronagraph(2, precision='f16x2')
+        sec, _ = time_graph(lambda: vvc.forward(ha.Wavefront(e64, 1.0)).electric_field)
+        out['vector_vortex_1024_charge2_f16x2'] = {'value': round(1.0 / sec, 1), 'unit': 'forward/s',
+                                                   'ms': round(sec * 1e3, 4)}
         return out
 
     def time_mixed_grouped(n_wl=8, batch=4, reps=10):

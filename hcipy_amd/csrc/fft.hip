@@ -2,17 +2,17 @@
 // 2-D inverse FFT -> crop, in three passes over HBM (reference: hcipy/fourier/fourier_operations.py:89-151).
 //
 //   pass A  rows   : load one input row into the centre of a zeroed line of length NX, forward FFT,
-//                    write the line (ny rows x NX).
-//   pass B  columns: gather C adjacent columns of pass A's output into LDS lines of length NY
-//                    (zero outside the centred ny rows), forward FFT, multiply by the transfer
-//                    function (conjugated for the adjoint), inverse FFT, write back only the
-//                    centred ny rows (the crop along y happens here).
+//                    write the line (ny rows x NX, column-blocked: see blk()).
+//   pass B  columns: load C adjacent columns of pass A's output (contiguous in the blocked
+//                    layout; zero outside the centred ny rows), forward FFT, multiply by the
+//                    transfer function (conjugated for the adjoint), inverse FFT, write back only
+//                    the centred ny rows (the crop along y happens here).
 //   pass C  rows   : inverse FFT of each of the ny rows, write the centred nx samples scaled by
 //                    1 / (NX NY) (numpy.fft.ifftn normalisation) -- the crop along x.
 //
-// Line FFTs are radix-4 Stockham passes (one radix-2 pass when log2 N is odd) in LDS, in place,
-// with all loads of a pass in registers before its stores; twiddles come from an FP64-generated
-// table exp(-2 pi i t / N).
+// Line FFTs run on the register-resident Stockham engine of line_fft.hpp (radix-16 stages in
+// registers, one LDS round trip per stage); twiddles come from an FP64-generated table
+// exp(-2 pi i t / N).
 #include "common.hpp"
 #include "line_fft.hpp"
 #include "../../include/hcipy_amd.h"
@@ -24,172 +24,165 @@ namespace hcipy {
 
 namespace {
 
-template <int N>
+// Column-blocked layout of the intermediates and the transfer function: columns are grouped in
+// blocks of CB = min(8, NX) and each block is stored as a contiguous [rows][CB] array, so that
+// the column pass reads and writes whole contiguous blocks and a row pass that handles two
+// adjacent rows touches full 128-byte lines (2 x 8 complex64, or 8 complex128).
+__host__ __device__ inline int64_t blk(int64_t row, int col, int rows, int NX) {
+  const int cb = NX < 8 ? NX : 8;
+  return ((int64_t)(col / cb) * rows + row) * cb + col % cb;
+}
+
+// Line FFTs run on the register-resident engine (hr_fft): E points per thread, P = N / E
+// threads per line, one LDS round trip per radix-E stage.  Lines of one workgroup are
+// LINE = hr_lds_points(N) + 1 slots apart (the odd slot staggers their banks).
+template <typename T, int N>
 struct RowCfg {
-  static constexpr int TPL = (N / 4 < 256) ? (N / 4 < 1 ? 1 : N / 4) : 256;
-  static constexpr int LINES = 256 / TPL;
+  static constexpr int E = N >= 16 ? 16 : N;
+  static constexpr int P = N / E;
+  static constexpr int WG = P > 256 ? P : 256;
+  static constexpr int LINES = WG / P;
+  static constexpr int LINE = hr_lds_points(N) + 1;
+  static constexpr bool TW_LDS = sizeof(T) == 4;
+  static constexpr size_t LDS_BYTES = ((size_t)LINES * LINE + (TW_LDS ? N : 0)) * 2 * sizeof(T);
 };
 
-// pass A: forward row FFTs of padded input rows.
+template <typename T, int N, int WG>
+__device__ inline const typename Cplx<T>::V* stage_twiddles(const typename Cplx<T>::V* __restrict__ tw,
+                                                            typename Cplx<T>::V* lds_tw, bool to_lds) {
+  if (!to_lds) return tw;
+  for (int t = threadIdx.x; t < N; t += WG) lds_tw[t] = tw[t];
+  __syncthreads();
+  return lds_tw;
+}
+
+// pass A: forward row FFTs of the input rows, centred in zeroed lines of NX points.
 template <typename T, int NX>
-__global__ __launch_bounds__(256) void rows_forward_kernel(const typename Cplx<T>::V* __restrict__ in,
-                                                           typename Cplx<T>::V* __restrict__ w1,
-                                                           const typename Cplx<T>::V* __restrict__ twx, int nx, int ny,
-                                                           int c0x) {
+__global__ __launch_bounds__((RowCfg<T, NX>::WG)) void rows_forward_kernel(const typename Cplx<T>::V* __restrict__ in,
+                                                                           typename Cplx<T>::V* __restrict__ w1,
+                                                                           const typename Cplx<T>::V* __restrict__ twx,
+                                                                           int nx, int ny, int c0x) {
   using V = typename Cplx<T>::V;
-  constexpr int TPL = RowCfg<NX>::TPL;
-  constexpr int LINES = RowCfg<NX>::LINES;
-  __shared__ V lds[LINES][NX];
-  const int line = threadIdx.x / TPL;
-  const int tl = threadIdx.x % TPL;
-  const int y = blockIdx.x * LINES + line;
+  using Cfg = RowCfg<T, NX>;
+  constexpr int E = Cfg::E;
+  constexpr int P = Cfg::P;
+  extern __shared__ __align__(16) unsigned char fft_lds_raw[];
+  V* lds = reinterpret_cast<V*>(fft_lds_raw);
+  const V* tw = stage_twiddles<T, NX, Cfg::WG>(twx, lds + Cfg::LINES * Cfg::LINE, Cfg::TW_LDS);
+  const int line = threadIdx.x / P;
+  const int tl = threadIdx.x % P;
+  const int y = blockIdx.x * Cfg::LINES + line;
   const int64_t b = blockIdx.y;
   const bool valid = y < ny;
-  V* L = lds[line];
-  const V zero = {0, 0};
-  constexpr int PR = NX / TPL;  // samples per thread; all loads in flight before the LDS stores
-  V g[PR];
+  const V* src = in + (b * ny + (valid ? y : 0)) * (int64_t)nx;
+  V v[E];
 #pragma unroll
-  for (int j = 0; j < PR; ++j) {
-    const int xi = tl + j * TPL - c0x;
-    g[j] = (valid && xi >= 0 && xi < nx) ? in[(b * ny + y) * (int64_t)nx + xi] : zero;
+  for (int q = 0; q < E; ++q) {
+    const int xi = tl + q * P - c0x;
+    v[q] = (valid && xi >= 0 && xi < nx) ? src[xi] : V{0, 0};
   }
-#pragma unroll
-  for (int j = 0; j < PR; ++j) L[tl + j * TPL] = g[j];
-  __syncthreads();
-  line_fft<T, NX, TPL>(L, tl, twx, -1, 0);
+  hr_fft<T, NX, P>(v, lds + line * Cfg::LINE, tl, tw, -1);
   if (valid) {
-    V* o = w1 + (b * ny + y) * (int64_t)NX;
-    for (int i = tl; i < NX; i += TPL) o[i] = L[i];
+    V* o = w1 + b * ny * (int64_t)NX;
+#pragma unroll
+    for (int q = 0; q < E; ++q) o[blk(y, tl + q * P, ny, NX)] = v[q];
   }
 }
 
-// pass C: inverse row FFTs, crop to nx, scale.
+// pass C: inverse row FFTs, crop to the centred nx samples, scale.
 template <typename T, int NX>
-__global__ __launch_bounds__(256) void rows_inverse_kernel(const typename Cplx<T>::V* __restrict__ w2,
-                                                           typename Cplx<T>::V* __restrict__ out,
-                                                           const typename Cplx<T>::V* __restrict__ twx, int nx, int ny,
-                                                           int c0x, T scale) {
+__global__ __launch_bounds__((RowCfg<T, NX>::WG)) void rows_inverse_kernel(const typename Cplx<T>::V* __restrict__ w2,
+                                                                           typename Cplx<T>::V* __restrict__ out,
+                                                                           const typename Cplx<T>::V* __restrict__ twx,
+                                                                           int nx, int ny, int c0x, T scale) {
   using V = typename Cplx<T>::V;
-  constexpr int TPL = RowCfg<NX>::TPL;
-  constexpr int LINES = RowCfg<NX>::LINES;
-  __shared__ V lds[LINES][NX];
-  const int line = threadIdx.x / TPL;
-  const int tl = threadIdx.x % TPL;
-  const int y = blockIdx.x * LINES + line;
+  using Cfg = RowCfg<T, NX>;
+  constexpr int E = Cfg::E;
+  constexpr int P = Cfg::P;
+  extern __shared__ __align__(16) unsigned char fft_lds_raw[];
+  V* lds = reinterpret_cast<V*>(fft_lds_raw);
+  const V* tw = stage_twiddles<T, NX, Cfg::WG>(twx, lds + Cfg::LINES * Cfg::LINE, Cfg::TW_LDS);
+  const int line = threadIdx.x / P;
+  const int tl = threadIdx.x % P;
+  const int y = blockIdx.x * Cfg::LINES + line;
   const int64_t b = blockIdx.y;
   const bool valid = y < ny;
-  V* L = lds[line];
-  const V zero = {0, 0};
-  constexpr int PR = NX / TPL;
-  V g[PR];
+  const V* src = w2 + b * ny * (int64_t)NX;
+  V v[E];
 #pragma unroll
-  for (int j = 0; j < PR; ++j) g[j] = valid ? w2[(b * ny + y) * (int64_t)NX + tl + j * TPL] : zero;
-#pragma unroll
-  for (int j = 0; j < PR; ++j) L[tl + j * TPL] = g[j];
-  __syncthreads();
-  line_fft<T, NX, TPL>(L, tl, twx, +1, 0);
+  for (int q = 0; q < E; ++q) v[q] = valid ? src[blk(y, tl + q * P, ny, NX)] : V{0, 0};
+  hr_fft<T, NX, P>(v, lds + line * Cfg::LINE, tl, tw, +1);
   if (valid) {
     V* o = out + (b * ny + y) * (int64_t)nx;
-    for (int i = tl; i < nx; i += TPL) {
-      V v = L[c0x + i];
-      v.x *= scale;
-      v.y *= scale;
-      o[i] = v;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int xi = tl + q * P - c0x;
+      if (xi >= 0 && xi < nx) o[xi] = V{v[q].x * scale, v[q].y * scale};
     }
   }
 }
 
 template <typename T, int NY>
 struct ColCfg {
-  // as many lines as fit in 128 KiB of LDS, at most 16; 512 threads share them
-  static constexpr int NT = 512;
-  static constexpr int CF = 131072 / (NY * 2 * (int)sizeof(T));
-  static constexpr int C = CF > 16 ? 16 : (CF < 1 ? 1 : CF);
-  static constexpr int TPL = NT / C;
-  static constexpr int LDN = NY + 1;  // padded line length
-  // elements of the C x NY block each thread gathers / multiplies (fully unrolled, so that all
-  // global loads of a loop are in flight together)
-  static constexpr int PER = (C * NY + NT - 1) / NT;
-  // the twiddle table sits in LDS next to the lines when both fit in the 160 KiB of a CU
-  static constexpr bool TW_LDS = (size_t)C * LDN * 2 * sizeof(T) + (size_t)NY * 2 * sizeof(T) <= 160 * 1024;
-  static constexpr size_t LDS_BYTES = ((size_t)C * LDN + (TW_LDS ? NY : 0)) * 2 * sizeof(T);
+  // 8 points per thread: at 16 the up to 1024 threads of a workgroup spill registers
+  static constexpr int E = NY >= 8 ? 8 : NY;
+  static constexpr int P = NY / E;
+  static constexpr int LINE = hr_lds_points(NY) + 1;
+  // columns per workgroup: as many lines as fit in 140 KiB of LDS, at most 16, and at most 1024
+  // threads (512 for FP64, whose 8 points per thread then fit the register budget unspilled)
+  static constexpr int CF = (int)(143360 / ((size_t)LINE * 2 * sizeof(T)));
+  static constexpr int CT = (sizeof(T) == 8 ? 512 : 1024) / P;
+  static constexpr int C0 = CF < CT ? CF : CT;
+  static constexpr int C = C0 > 16 ? 16 : (C0 < 1 ? 1 : C0);
+  static constexpr int NT = C * P;
+  static constexpr bool TW_LDS = ((size_t)C * LINE + NY) * 2 * sizeof(T) <= 160 * 1024;
+  static constexpr size_t LDS_BYTES = ((size_t)C * LINE + (TW_LDS ? NY : 0)) * 2 * sizeof(T);
 };
 
-// pass B: column FFT, transfer function, inverse column FFT, crop to the centred ny rows.
-// The global loads of the gather, and those of the transfer function, are each issued as one
-// unrolled batch so that their latencies overlap instead of adding up.
+// pass B: column FFT, transfer function, inverse column FFT, crop to the centred ny rows.  C
+// adjacent columns per workgroup; thread (c, tl) = (tid % C, tid / C), so that the threads of a
+// wave load and store C consecutive samples of each row they touch.
 template <typename T, int NY>
-__global__ __launch_bounds__(512) void columns_kernel(const typename Cplx<T>::V* __restrict__ w1,
-                                                      typename Cplx<T>::V* __restrict__ w2,
-                                                      const typename Cplx<T>::V* __restrict__ tf,
-                                                      const typename Cplx<T>::V* __restrict__ twy, int NX, int ny,
-                                                      int c0y, int conj_tf) {
+__global__ __launch_bounds__((ColCfg<T, NY>::NT)) void columns_kernel(const typename Cplx<T>::V* __restrict__ w1,
+                                                                      typename Cplx<T>::V* __restrict__ w2,
+                                                                      const typename Cplx<T>::V* __restrict__ tf,
+                                                                      const typename Cplx<T>::V* __restrict__ twy,
+                                                                      int NX, int ny, int c0y, int conj_tf) {
   using V = typename Cplx<T>::V;
   using Cf = ColCfg<T, NY>;
   constexpr int C = Cf::C;
-  constexpr int TPL = Cf::TPL;
-  constexpr int LDN = Cf::LDN;
-  constexpr int PER = Cf::PER;
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  V* lds = reinterpret_cast<V*>(smem);
-  const int kx0 = blockIdx.x * C;
+  constexpr int E = Cf::E;
+  constexpr int P = Cf::P;
+  extern __shared__ __align__(16) unsigned char fft_lds_raw[];
+  V* lds = reinterpret_cast<V*>(fft_lds_raw);
+  const V* tw = stage_twiddles<T, NY, Cf::NT>(twy, lds + C * Cf::LINE, Cf::TW_LDS);
+  const int c = threadIdx.x % C;
+  const int tl = threadIdx.x / C;
+  const int kx = blockIdx.x * C + c;
   const int64_t b = blockIdx.y;
-  const V zero = {0, 0};
-  const int tid = threadIdx.x;
-  // gather the ny valid rows (C consecutive columns per row: coalesced), zero the padding rows
-  V g[PER];
+  const bool col = kx < NX;
+  V v[E];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int idx = tid + i * Cf::NT;
-    const int c = idx % C;
-    const int r = idx / C;
-    const int yi = r - c0y;
-    V v = zero;
-    if (idx < C * NY && yi >= 0 && yi < ny && kx0 + c < NX) v = w1[(b * ny + yi) * (int64_t)NX + kx0 + c];
-    g[i] = v;
+  for (int q = 0; q < E; ++q) {
+    const int yi = tl + q * P - c0y;
+    v[q] = (col && yi >= 0 && yi < ny) ? w1[b * ny * (int64_t)NX + blk(yi, kx, ny, NX)] : V{0, 0};
   }
-  const V* tw = twy;
-  if constexpr (Cf::TW_LDS) {
-    V* tws = lds + C * LDN;
-    for (int i = tid; i < NY; i += Cf::NT) tws[i] = twy[i];
-    tw = tws;
-  }
+  V* line = lds + c * Cf::LINE;
+  hr_fft<T, NY, P>(v, line, tl, tw, -1);
+  // transfer function (already in FFT order, column-blocked)
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int idx = tid + i * Cf::NT;
-    if (idx < C * NY) lds[(idx % C) * LDN + idx / C] = g[i];
+  for (int q = 0; q < E; ++q) {
+    V t = col ? tf[blk(tl + q * P, kx, NY, NX)] : V{0, 0};
+    if (conj_tf) t.y = -t.y;
+    v[q] = cmul(v[q], t);
   }
-  __syncthreads();
-  const int line = tid / TPL;
-  const int tl = tid % TPL;
-  V* L = lds + line * LDN;
-  line_fft<T, NY, TPL>(L, tl, tw, -1, 0);
-  // transfer function (already in FFT order): tf[ky][kx]; all loads issued before the multiplies
-  V t[PER];
+  hr_fft<T, NY, P>(v, line, tl, tw, +1);
+  if (col) {
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int idx = tid + i * Cf::NT;
-    const int c = idx % C;
-    const int r = idx / C;
-    t[i] = (idx < C * NY && kx0 + c < NX) ? tf[(int64_t)r * NX + kx0 + c] : zero;
-  }
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int idx = tid + i * Cf::NT;
-    if (idx < C * NY) {
-      V tv = t[i];
-      if (conj_tf) tv.y = -tv.y;
-      V* e = &lds[(idx % C) * LDN + idx / C];
-      *e = cmul(*e, tv);
+    for (int q = 0; q < E; ++q) {
+      const int yi = tl + q * P - c0y;
+      if (yi >= 0 && yi < ny) w2[b * ny * (int64_t)NX + blk(yi, kx, ny, NX)] = v[q];
     }
-  }
-  __syncthreads();
-  line_fft<T, NY, TPL>(L, tl, tw, +1, 0);
-  for (int idx = tid; idx < C * ny; idx += Cf::NT) {
-    const int c = idx % C;
-    const int yi = idx / C;
-    if (kx0 + c < NX) w2[(b * ny + yi) * (int64_t)NX + kx0 + c] = lds[c * LDN + c0y + yi];
   }
 }
 
@@ -204,7 +197,8 @@ __global__ void twiddle_kernel(int n, int dtype, void* out) {
     static_cast<double2*>(out)[t] = make_double2(c, s);
 }
 
-// tf_out[ky][kx] = tf_centred[(ky + NY/2) % NY][(kx + NX/2) % NX]  (numpy ifftshift), cast.
+// tf_out[ky][kx] = tf_centred[(ky + NY/2) % NY][(kx + NX/2) % NX]  (numpy ifftshift), cast, stored
+// column-blocked.
 __global__ void ifftshift_cast_kernel(const void* in, int in_dtype, void* out, int out_dtype, int NX, int NY) {
   const int64_t total = (int64_t)NX * NY;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
@@ -221,10 +215,11 @@ __global__ void ifftshift_cast_kernel(const void* in, int in_dtype, void* out, i
       re = v.x;
       im = v.y;
     }
+    const int64_t o = blk(ky, kx, NY, NX);
     if (out_dtype == HCIPY_DTYPE_C64)
-      static_cast<float2*>(out)[i] = make_float2((float)re, (float)im);
+      static_cast<float2*>(out)[o] = make_float2((float)re, (float)im);
     else
-      static_cast<double2*>(out)[i] = make_double2(re, im);
+      static_cast<double2*>(out)[o] = make_double2(re, im);
   }
 }
 
@@ -284,10 +279,10 @@ __global__ void nearfield_tf_kernel(int kind, int NX, int NY, double dx, double 
     }
     acc_re /= (double)(ovs * ovs);
     acc_im /= (double)(ovs * ovs);
-    // stored in FFT order (ifftshift of the centred internal grid)
+    // stored in FFT order (ifftshift of the centred internal grid), column-blocked
     const int oxi = (ix + NX - NX / 2) % NX;
     const int oyi = (iy + NY - NY / 2) % NY;
-    const int64_t o = (int64_t)oyi * NX + oxi;
+    const int64_t o = blk(oyi, oxi, NY, NX);
     if (out_dtype == HCIPY_DTYPE_C64)
       static_cast<float2*>(out)[o] = make_float2((float)acc_re, (float)acc_im);
     else
@@ -352,33 +347,43 @@ size_t cbytes(int dtype) { return dtype == HCIPY_DTYPE_C64 ? 8 : 16; }
 bool pow2(int n) { return n >= 4 && n <= 4096 && (n & (n - 1)) == 0; }
 
 template <typename T, int N>
-void launch_rows_fwd(const FilterPlan& p, const void* in, void* w1, int64_t batch, hipStream_t s) {
+int launch_rows_fwd(const FilterPlan& p, const void* in, void* w1, int64_t batch, hipStream_t s) {
   using V = typename Cplx<T>::V;
-  constexpr int LINES = RowCfg<N>::LINES;
-  dim3 g((unsigned)ceil_div(p.ny, LINES), (unsigned)batch);
-  hipLaunchKernelGGL((rows_forward_kernel<T, N>), g, dim3(256), 0, s, (const V*)in, (V*)w1, (const V*)p.twx, p.nx,
-                     p.ny, p.c0x);
+  using Cfg = RowCfg<T, N>;
+  auto kern = rows_forward_kernel<T, N>;
+  if (Cfg::LDS_BYTES > 65536)
+    HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)Cfg::LDS_BYTES));
+  dim3 g((unsigned)ceil_div(p.ny, Cfg::LINES), (unsigned)batch);
+  hipLaunchKernelGGL(kern, g, dim3(Cfg::WG), Cfg::LDS_BYTES, s, (const V*)in, (V*)w1, (const V*)p.twx, p.nx, p.ny,
+                     p.c0x);
+  return kOk;
 }
 template <typename T, int N>
-void launch_rows_inv(const FilterPlan& p, const void* w2, void* out, int64_t batch, hipStream_t s) {
+int launch_rows_inv(const FilterPlan& p, const void* w2, void* out, int64_t batch, hipStream_t s) {
   using V = typename Cplx<T>::V;
-  constexpr int LINES = RowCfg<N>::LINES;
-  dim3 g((unsigned)ceil_div(p.ny, LINES), (unsigned)batch);
+  using Cfg = RowCfg<T, N>;
+  auto kern = rows_inverse_kernel<T, N>;
+  if (Cfg::LDS_BYTES > 65536)
+    HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)Cfg::LDS_BYTES));
+  dim3 g((unsigned)ceil_div(p.ny, Cfg::LINES), (unsigned)batch);
   const T scale = (T)(1.0 / ((double)p.NX * (double)p.NY));
-  hipLaunchKernelGGL((rows_inverse_kernel<T, N>), g, dim3(256), 0, s, (const V*)w2, (V*)out, (const V*)p.twx, p.nx,
-                     p.ny, p.c0x, scale);
+  hipLaunchKernelGGL(kern, g, dim3(Cfg::WG), Cfg::LDS_BYTES, s, (const V*)w2, (V*)out, (const V*)p.twx, p.nx, p.ny,
+                     p.c0x, scale);
+  return kOk;
 }
 template <typename T, int N>
 int launch_cols(const FilterPlan& p, const void* w1, void* w2, int conj, int64_t batch, hipStream_t s) {
   using V = typename Cplx<T>::V;
   using Cf = ColCfg<T, N>;
-  const size_t lds = Cf::LDS_BYTES;
   auto kern = columns_kernel<T, N>;
-  HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
+  if (Cf::LDS_BYTES > 65536)
+    HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)Cf::LDS_BYTES));
   dim3 g((unsigned)ceil_div(p.NX, Cf::C), (unsigned)batch);
-  hipLaunchKernelGGL(kern, g, dim3(Cf::NT), lds, s, (const V*)w1, (V*)w2, (const V*)p.tf, (const V*)p.twy, p.NX,
-                     p.ny, p.c0y, conj);
+  hipLaunchKernelGGL(kern, g, dim3(Cf::NT), Cf::LDS_BYTES, s, (const V*)w1, (V*)w2, (const V*)p.tf, (const V*)p.twy,
+                     p.NX, p.ny, p.c0y, conj);
   return kOk;
 }
 
@@ -403,19 +408,21 @@ int run_filter(const FilterPlan& p, int adjoint, const void* in, void* out, int6
   const size_t lbytes = (size_t)batch * p.ny * p.NX * cbytes(p.dtype);
   void* w1 = ws;
   void* w2 = ws + round_up((int64_t)lbytes, 256);
-#define ROWS_F(n) launch_rows_fwd<T, n>(p, in, w1, batch, s)
+  int st = kOk;
+#define ROWS_F(n) st = launch_rows_fwd<T, n>(p, in, w1, batch, s)
   HCIPY_POW2_SWITCH(p.NX, ROWS_F)
 #undef ROWS_F
+  if (st) return st;
   HCIPY_HIP(hipGetLastError());
-  int st = kOk;
 #define COLS(n) st = launch_cols<T, n>(p, w1, w2, adjoint, batch, s)
   HCIPY_POW2_SWITCH(p.NY, COLS)
 #undef COLS
   if (st) return st;
   HCIPY_HIP(hipGetLastError());
-#define ROWS_I(n) launch_rows_inv<T, n>(p, w2, out, batch, s)
+#define ROWS_I(n) st = launch_rows_inv<T, n>(p, w2, out, batch, s)
   HCIPY_POW2_SWITCH(p.NX, ROWS_I)
 #undef ROWS_I
+  if (st) return st;
   return check_hip(hipGetLastError(), "fourier filter launch");
 }
 
