@@ -128,15 +128,17 @@ struct ColCfg {
   static constexpr int E = NY >= 8 ? 8 : NY;
   static constexpr int P = NY / E;
   static constexpr int LINE = hr_lds_points(NY) + 1;
-  // columns per workgroup: as many lines as fit in 140 KiB of LDS, at most 16, and at most 1024
-  // threads (512 for FP64, whose 8 points per thread then fit the register budget unspilled)
-  static constexpr int CF = (int)(143360 / ((size_t)LINE * 2 * sizeof(T)));
+  // columns per workgroup: as many lines as fit in 70 KiB of LDS (two workgroups per CU), at most
+  // 16, and at most 1024 threads (512 for FP64, whose 8 points per thread then fit the register
+  // budget unspilled).  Twiddles are read from global memory: an LDS copy would take the room of
+  // the second workgroup, which was measured faster.
+  static constexpr int CF = (int)(71680 / ((size_t)LINE * 2 * sizeof(T)));
   static constexpr int CT = (sizeof(T) == 8 ? 512 : 1024) / P;
   static constexpr int C0 = CF < CT ? CF : CT;
   static constexpr int C = C0 > 16 ? 16 : (C0 < 1 ? 1 : C0);
   static constexpr int NT = C * P;
-  static constexpr bool TW_LDS = ((size_t)C * LINE + NY) * 2 * sizeof(T) <= 160 * 1024;
-  static constexpr size_t LDS_BYTES = ((size_t)C * LINE + (TW_LDS ? NY : 0)) * 2 * sizeof(T);
+  static constexpr bool TW_LDS = false;
+  static constexpr size_t LDS_BYTES = (size_t)C * LINE * 2 * sizeof(T);
 };
 
 // pass B: column FFT, transfer function, inverse column FFT, crop to the centred ny rows.  C
