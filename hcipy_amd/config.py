@@ -9,8 +9,9 @@ TypeError (hcipy/config/config.py:21-55).  Additions of this package:
   runs in FP64.
 * ``fourier.zfft.precision_complex64``: arithmetic of complex64 zoom FFTs, ``'fp64'`` (FP32
   loads and stores around FP64 chirps, line FFTs and intermediate; the default) or ``'fp32'``.
-* ``fourier.mft.execution_time_prediction_coefficients``: refitted for this library's MFT on
-  one MI355X (microsecond-scale latency dominates small transforms).
+* ``execution_time_prediction_coefficients`` (fft, mft, zfft): the reference's values, so that
+  ``make_fourier_transform(planner='estimate')`` makes the reference's choice;
+  ``planner='measure'`` times the candidates on the device instead.
 """
 import copy
 

@@ -7,6 +7,7 @@ Protocol (hcipy/fourier/fourier_transform.py:17-183): ``forward(field) -> Field`
 """
 import ctypes
 import math
+import time
 from collections.abc import Iterable
 from dataclasses import dataclass
 
@@ -752,21 +753,65 @@ class FastFourierTransform(FourierTransform):
         return ComputationalComplexity(num_multiplications=num_mult, num_additions=num_add, expected_execution_time=t)
 
 
-def make_fourier_transform(input_grid, output_grid=None, q=1, fov=1, shift=0, planner='estimate', precision=None):
-    """Fourier transform between two grids (hcipy/fourier/fourier_transform.py:217-300).
+def _time_forward(ft, input_grid, repeats=10):
+    """Seconds per forward transform of a zero complex128 field on the current device."""
+    a = torch.zeros(input_grid.size, dtype=torch.complex128, device=torch.device('cuda', torch.cuda.current_device()))
+    f = Field(a, input_grid)
+    for _ in range(2):
+        ft.forward(f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        ft.forward(f)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / repeats
 
-    With no output grid the FFT grid of (q, fov, shift) is used.  Every transform of this
-    library between separated Cartesian grids runs on the MFMA matrix Fourier transform, so the
-    planner returns a MatrixFourierTransform whenever the grids are separated and raises
-    ValueError otherwise (unstructured grids have no device implementation here).
+
+def make_fourier_transform(input_grid, output_grid=None, q=1, fov=1, shift=0, planner='estimate', precision=None):
+    """Fourier transform between two grids, the fastest of the candidate methods
+    (hcipy/fourier/fourier_transform.py:217-300).
+
+    Candidates: 'fft' (FastFourierTransform, when the output grid is an FFT grid of the input),
+    'mft' (MatrixFourierTransform) and 'zft' (ZoomFastFourierTransform), each where it supports
+    the grids.  ``planner='estimate'`` ranks them by ``compute_complexity`` with the time-model
+    coefficients of the configuration (the reference's, so the choice is the reference's);
+    ``'measure'`` times one forward transform of each on the device.  The reference's
+    NaiveFourierTransform has no device implementation here, so unstructured grids raise
+    ValueError.  ``precision`` is passed to the method that takes it ('f16x2'/'fp32' for the
+    MFT-based methods, 'fp64'/'fp32' for the zoom FFT).
     """
     if planner not in ('estimate', 'measure'):
         raise ValueError("planner must be 'estimate' or 'measure'")
-    if output_grid is None:
+    options = {'fft': FastFourierTransform, 'mft': MatrixFourierTransform, 'zft': ZoomFastFourierTransform}
+    if output_grid is not None:
+        try:
+            q, fov, shift = get_fft_parameters(output_grid, input_grid)
+        except ValueError:
+            del options['fft']
+    else:
         output_grid = make_fft_grid(input_grid, q, fov, shift)
-    if MatrixFourierTransform.is_supported(input_grid, output_grid):
-        return MatrixFourierTransform(input_grid, output_grid, precision=precision)
-    raise ValueError('No Fourier transform of this library supports these grids (separated Cartesian grids are required).')
+
+    def build(method):
+        if method == 'fft':
+            return FastFourierTransform(input_grid, q, fov, shift,
+                                        precision=precision if precision in ('f16x2', 'fp32') else None)
+        if method == 'mft':
+            return MatrixFourierTransform(input_grid, output_grid,
+                                          precision=precision if precision in ('f16x2', 'fp32') else None)
+        return ZoomFastFourierTransform(input_grid, output_grid,
+                                        precision=precision if precision in ('fp64', 'fp32') else None)
+
+    expected = {}
+    for method, ft in options.items():
+        if not ft.is_supported(input_grid, output_grid):
+            continue
+        if planner == 'estimate':
+            expected[method] = ft.compute_complexity(input_grid, output_grid).expected_execution_time
+        else:
+            expected[method] = _time_forward(build(method), input_grid)
+    if not expected:
+        raise ValueError('No Fourier transform of this library supports these grids (separated Cartesian grids are required).')
+    return build(min(expected, key=expected.get))
 
 
 __all__ += ['FastFourierTransform', 'make_fourier_transform']

@@ -118,3 +118,16 @@ def test_zoom_empty_batch_and_planner():
     zf = ha.ZoomFastFourierTransform(pg, fg)
     y = zf.forward(ha.Field(torch.zeros((0, pg.size), dtype=torch.complex64, device='cuda'), pg))
     assert tuple(y.shape) == (0, fg.size)
+
+
+def test_planner_measure():
+    """planner='measure' times each candidate on the device and returns one that computes the
+    same transform."""
+    pupil = ha.make_pupil_grid(128)
+    uv = ha.FraunhoferPropagator(pupil, ha.make_focal_grid(4, 8)).instance(1.0).uv_grid
+    ft = ha.make_fourier_transform(pupil, uv, planner='measure')
+    say('planner measure chose', type(ft).__name__)
+    f = _crand(pupil.size, 4)
+    a = host(ft.forward(ha.Field(dev(f), pupil)))
+    b = host(ha.MatrixFourierTransform(pupil, uv).forward(ha.Field(dev(f), pupil)))
+    assert peak_rel_field(a, b) < 1e-12

@@ -25,3 +25,23 @@ def test_zoom_fft_support_and_complexity():
         ha.ZoomFastFourierTransform(pupil, ha.make_uniform_grid([8], [1.0]))
     with pytest.raises(ValueError, match='8192'):
         ha.ZoomFastFourierTransform(ha.make_uniform_grid([5000], [1.0]), ha.make_uniform_grid([4000], [1.0]))
+
+
+def test_planner_estimates_and_choice():
+    """make_fourier_transform(planner='estimate') (fourier_transform.py:217-300): at 1024^2 ->
+    512^2 (q = 4, 64 lambda/D) the reference's time models predict 78.5 ms (FFT), 20.1 ms (MFT)
+    and 21.0 ms (zoom FFT), so the MFT is chosen; a 4096^2 -> 4000^2 non-FFT output grid goes to
+    the zoom FFT; an FFT grid given by q goes to the FFT."""
+    import hcipy_amd as ha
+    p = ha.make_pupil_grid(1024)
+    uv = ha.FraunhoferPropagator(p, ha.make_focal_grid(4, 64)).instance(1.0).uv_grid
+    est = {k: c.compute_complexity(p, uv).expected_execution_time * 1e3
+           for k, c in (('fft', ha.FastFourierTransform), ('mft', ha.MatrixFourierTransform),
+                        ('zft', ha.ZoomFastFourierTransform))}
+    assert abs(est['fft'] - 78.5) < 0.1 and abs(est['mft'] - 20.1) < 0.1 and abs(est['zft'] - 21.0) < 0.1
+    assert type(ha.make_fourier_transform(p, uv)) is ha.MatrixFourierTransform
+    big = ha.make_pupil_grid(4096)
+    assert type(ha.make_fourier_transform(big, ha.make_uniform_grid([4000, 4000], [300, 300]))) is ha.ZoomFastFourierTransform
+    assert type(ha.make_fourier_transform(p, q=2)) is ha.FastFourierTransform
+    with pytest.raises(ValueError):
+        ha.make_fourier_transform(p, uv, planner='guess')
