@@ -16,6 +16,7 @@ from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, Fresne
 from .fourier_filter import FourierFilter, NearFieldTransferFunction
 from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VectorVortexCoronagraph, VortexCoronagraph
 from .polarization import CircularRetarder, JonesMatrixOpticalElement, LinearRetarder, PhaseRetarder
+from .spectral_noise import SpectralNoiseFFT, SpectralNoiseFactoryFFT, make_power_law_error
 from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
 from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, make_actuator_positions,
                                 make_gaussian_influence_functions)

@@ -119,6 +119,9 @@ SIGNATURES = {
                                                         ctypes.c_int, _vp, _vp]),
     'hcipy_pw_field_dot': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                                           ctypes.c_int, _c_int64, _vp, _c_int64, _vp]),
+    'hcipy_pw_phase_ramp': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp,
+                                           _c_int64, _vp]),
     'hcipy_pw_stokes': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                        _vp, _c_int64, _vp]),
     'hcipy_filter_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),

@@ -301,6 +301,25 @@ __global__ void stokes_kernel(const C* __restrict__ E, int vector_field, double 
   }
 }
 
+// out = x exp(i (a_x u + a_y v)) on a regular grid, u = zero_x + ix delta_x (x fastest),
+// v = zero_y + iy delta_y; the phase in FP64.  Batch b0 + blockIdx.y; out may alias x.
+template <typename C, typename R>
+__global__ void phase_ramp_kernel(const C* x, int nx, int64_t n, double zero_x, double delta_x, double zero_y,
+                                  double delta_y, double ax, double ay, C* out, int64_t b0) {
+  const int64_t off = (b0 + blockIdx.y) * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t iy = i / nx;
+    const int64_t ix = i - iy * nx;
+    const double ph = ax * (zero_x + (double)ix * delta_x) + ay * (zero_y + (double)iy * delta_y);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const C v = x[off + i];
+    const double re = (double)v.re * cs - (double)v.im * sn;
+    const double im = (double)v.re * sn + (double)v.im * cs;
+    out[off + i] = C{(R)re, (R)im};
+  }
+}
+
 constexpr int kThreads = 256;
 constexpr int kPowerBlocks = 256;
 
@@ -542,6 +561,27 @@ int hcipy_pw_stokes(const void* E, int field_dtype, int vector_field, const doub
     hipLaunchKernelGGL((stokes_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)E, vector_field, sv[0],
                        sv[1], sv[2], sv[3], component, (double*)out, n);
   return check_hip(hipGetLastError(), "stokes_kernel");
+}
+
+int hcipy_pw_phase_ramp(const void* x, int field_dtype, int nx, int ny, double zero_x, double delta_x, double zero_y,
+                        double delta_y, double ax, double ay, void* out, int64_t batch, void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
+  HCIPY_REQUIRE(nx >= 1 && ny >= 1 && batch >= 0, "bad size");
+  if (batch == 0) return kOk;
+  HCIPY_REQUIRE(x != nullptr && out != nullptr, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t n = (int64_t)nx * ny;
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
+    const dim3 g = grid_for(n, chunk(batch, b0));
+    if (field_dtype == HCIPY_DTYPE_C64)
+      hipLaunchKernelGGL((phase_ramp_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, nx, n, zero_x,
+                         delta_x, zero_y, delta_y, ax, ay, (C64*)out, b0);
+    else
+      hipLaunchKernelGGL((phase_ramp_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, nx, n, zero_x,
+                         delta_x, zero_y, delta_y, ax, ay, (C128*)out, b0);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
 }
 
 }  // extern "C"

@@ -272,6 +272,31 @@ def stokes(E, input_stokes_vector=None, component=-1):
     return Field(out, grid) if grid is not None else out
 
 
+def phase_ramp(E, grid, ax, ay=0.0, out=None):
+    """E * exp(i (ax x + ay y)) on the regular grid of E (any leading dimensions), the phase in
+    FP64 on the device; ``out`` may be E itself (in place)."""
+    et = _plain(E).contiguous()
+    _lib.require_device(et)
+    if not et.is_complex():
+        et = et.to(torch.complex128)
+    if not grid.is_regular:
+        raise ValueError('phase_ramp needs a regular grid')
+    n = et.shape[-1]
+    if n != grid.size:
+        raise ValueError('field size %d does not match the grid size %d' % (n, grid.size))
+    batch = int(np.prod(et.shape[:-1])) if et.dim() > 1 else 1
+    dims = [int(d) for d in grid.dims]
+    delta = [float(d) for d in grid.delta]
+    zero = [float(z) for z in grid.zero]
+    nx, ny = dims[0], (dims[1] if grid.ndim == 2 else 1)
+    zy, dy = (zero[1], delta[1]) if grid.ndim == 2 else (0.0, 0.0)
+    o = et if out is not None and _plain(out).data_ptr() == et.data_ptr() else torch.empty_like(et)
+    st = _lib.load().hcipy_pw_phase_ramp(_lib.dptr(et), _lib.dtype_code(et), nx, ny, zero[0], delta[0], zy, dy,
+                                         float(ax), float(ay), _lib.dptr(o), batch, _lib.stream_ptr(et.device))
+    _lib.check(st, 'hcipy_pw_phase_ramp')
+    return o
+
+
 def _field_dot_raw(jt, et, adjoint):
     p, q = int(jt.shape[0]), int(jt.shape[1])
     n = et.shape[-1]
@@ -324,4 +349,4 @@ def field_dot(J, E, adjoint=False):
     return Field(o, grid) if grid is not None else o
 
 
-__all__ += ['real_sum', 'stokes', 'field_dot']
+__all__ += ['real_sum', 'stokes', 'field_dot', 'phase_ramp']

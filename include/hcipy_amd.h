@@ -253,6 +253,13 @@ int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, in
 int hcipy_pw_field_dot(const void* J, int j_dtype, int p, int q, int adjoint, const void* E, int field_dtype, int64_t K,
                        void* out, int64_t n, void* stream);
 
+/* out = x exp(i (ax u + ay v)) for `batch` fields on a regular nx x ny grid with coordinates
+ * u = zero_x + ix delta_x, v = zero_y + iy delta_y (x fastest), phase in FP64; out may equal x.
+ * Replaces the shift of an FFT spectral-noise realization, C *= exp(-1j * (s_x u + s_y v))
+ * (hcipy/util/spectral_noise.py:177-187; ax = -s_x, ay = -s_y).  A 1-D grid has ny = 1, ay = 0. */
+int hcipy_pw_phase_ramp(const void* x, int field_dtype, int nx, int ny, double zero_x, double delta_x, double zero_y,
+                        double delta_y, double ax, double ay, void* out, int64_t batch, void* stream);
+
 /* Stokes parameters (I, Q, U, V) per pixel of a polarized wavefront: Wavefront.I/.Q/.U/.V
  * (hcipy/optics/wavefront.py:112-235).  With vector_field != 0, E is a Jones-vector field
  * (layout [2][n]) and stokes is ignored; otherwise E is the 2 x 2 Jones-matrix field (layout

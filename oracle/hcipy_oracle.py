@@ -35,6 +35,10 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
   offsets, odd sizes and output longer than input, in 1-D and 2-D, forward and backward; the
   chirp-z transform against a direct sum over its z_k (arbitrary complex a and w) and against
   numpy's FFT when it is one (a = 1, w = exp(-2 pi i / n), m = n);
+* ``spectral_noise_fft``/``power_law_error``: the ensemble variance of the screens equals the
+  integral of the PSD over the FFT grid (sum psd w / (2 pi)^2: the (2 pi)^ndim and weight
+  factors), a one-pixel shift is np.roll of the screen by one pixel along that axis, two half
+  shifts compose, and the reference's peak-to-valley thresholds (tests/test_optics.py:823-838);
 * ``stokes_vector``: for pure input states the Mueller rows must agree with the Stokes
   parameters of the propagated Jones vector J e (an independent closed form), and those four
   states span every input Stokes vector; Malus's law for a rotated polarizer;
@@ -793,6 +797,41 @@ def _fft_forward_tensor(fft, f):
     lead = f.shape[:-1]
     flat = f.reshape((-1, f.shape[-1]))
     return np.array([fft.forward(v) for v in flat]).reshape(lead + (-1,))
+
+
+def spectral_noise_fft(psd, output_grid, seed, oversample=1, shift=None):
+    """SpectralNoiseFactoryFFT(psd, output_grid, oversample).make_random(seed), optionally
+    shifted, evaluated (hcipy/util/spectral_noise.py:99-201): C = sqrt(psd / w (2 pi)^ndim) on the
+    FFT grid times complex white noise from default_rng(seed), times exp(-i shift . k), then
+    the real part of the FFT backward onto the output grid.  ``shift[i]`` moves along axis i."""
+    fft = FFT(output_grid, q=oversample)
+    g = fft.output_grid
+    C = np.sqrt(np.asarray(psd(g), dtype=float) / g.weights * (2 * np.pi) ** g.ndim)
+    rng = np.random.default_rng(seed)
+    C = C * (rng.standard_normal(g.size) + 1j * rng.standard_normal(g.size))
+    if shift is not None:
+        C = C * np.exp(-1j * sum(shift[i] * g.coords[i] for i in range(g.ndim)))
+    return fft.backward(C).real
+
+
+def power_law_psd(exponent):
+    """The PSD of make_power_law_error (hcipy/optics/aberration.py:37-41): r^exponent, 0 at r = 0."""
+    def psd(grid):
+        r = np.hypot(*grid.coords) if grid.ndim == 2 else np.abs(grid.coords[0])
+        with np.errstate(divide='ignore'):
+            res = r ** exponent
+        res[r == 0] = 0
+        return res
+    return psd
+
+
+def power_law_error(pupil_grid, ptv, diameter, exponent=-2.5, aperture=None, seed=None):
+    """make_power_law_error without mode removal (hcipy/optics/aberration.py:11-54)."""
+    if aperture is None:
+        aperture = circular_aperture(pupil_grid, diameter)
+    screen = spectral_noise_fft(power_law_psd(exponent), pupil_grid, seed)
+    screen = screen * (ptv / np.ptp(screen[np.asarray(aperture) != 0]))
+    return screen * aperture
 
 
 def actuator_positions(n_act, pitch):

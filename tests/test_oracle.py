@@ -547,3 +547,38 @@ def test_zoom_fft_equals_dense_dft(dims_in, dims_out):
     assert np.abs(o.zoom_fft(F, pg, fg, adjoint=True) - refb).max() < 1e-13 * np.abs(refb).max()
     if nd == 2:
         assert np.abs(o.mft_forward(f, pg, fg) - z).max() < 1e-13 * np.abs(z).max()
+
+
+def test_spectral_noise_variance_matches_psd_integral():
+    """spectral_noise.py:99-201: Var(screen) = 1/2 sum E|C_k|^2 (w/(2 pi)^2)^2 = sum psd w / (2 pi)^2
+    over the FFT grid.  A wrong (2 pi)^ndim or weight factor changes it by orders of magnitude."""
+    g = o.make_pupil_grid(32, 2.0)
+    psd = o.power_law_psd(-1.0)
+    fg = o.FFT(g).output_grid
+    expected = np.sum(psd(fg) * fg.weights) / (2 * np.pi) ** 2
+    var = np.mean([np.mean(o.spectral_noise_fft(psd, g, seed) ** 2) for seed in range(400)])
+    assert abs(var / expected - 1) < 0.05
+
+
+def test_spectral_noise_shift_is_translation():
+    """A shift of one pixel along x (y) is np.roll of the screen by one pixel along x (y) at
+    oversample = 1 (the FFT grid makes the screen periodic over the grid); half shifts compose."""
+    g = o.make_uniform_grid([24, 16], [2.4, 1.6])
+    psd = o.power_law_psd(-2.5)
+    base = o.spectral_noise_fft(psd, g, 3).reshape(g.shape)
+    sx = o.spectral_noise_fft(psd, g, 3, shift=(g.delta[0], 0)).reshape(g.shape)
+    sy = o.spectral_noise_fft(psd, g, 3, shift=(0, 2 * g.delta[1])).reshape(g.shape)
+    scale = np.abs(base).max()
+    assert np.abs(sx - np.roll(base, 1, axis=1)).max() < 1e-12 * scale
+    assert np.abs(sy - np.roll(base, 2, axis=0)).max() < 1e-12 * scale
+    half = o.spectral_noise_fft(psd, g, 3, shift=(0.5 * g.delta[0], 0))
+    assert np.abs(half.reshape(g.shape) - base).max() > 1e-3 * scale
+
+
+@pytest.mark.parametrize('exponent', [-2.5, -1.0])
+def test_power_law_error_reference_ptv(exponent):
+    """tests/test_optics.py:823-838: the peak-to-valley over the (default, 2 D) aperture is 0.1."""
+    g = o.make_pupil_grid(64, 2.0)
+    large = o.evaluate_supersampled(lambda gg: o.circular_aperture(gg, 2.0), g, 4)
+    screen = o.power_law_error(g, 0.1, 2.0, exponent=exponent, seed=5)
+    assert abs((np.ptp(screen[large > 0]) - 0.1) / 0.1) < 1e-10
