@@ -18,7 +18,8 @@ from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VectorVortexCo
 from .polarization import CircularRetarder, JonesMatrixOpticalElement, LinearRetarder, PhaseRetarder
 from .spectral_noise import SpectralNoiseFFT, SpectralNoiseFactoryFFT, make_power_law_error
 from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
-from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, make_actuator_positions,
+from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, XineticsInfluenceFunctions,
+                                make_xinetics_influence_functions, make_actuator_positions,
                                 make_gaussian_influence_functions)
 from .metrics import get_mean_intensity_in_roi, get_mean_raw_contrast, get_strehl_from_focal, get_strehl_from_pupil
 from . import pointwise

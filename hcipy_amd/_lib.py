@@ -60,6 +60,17 @@ class DmGaussianDesc(ctypes.Structure):
                 ('y_tilt', ctypes.c_double), ('z_tilt', ctypes.c_double), ('oversampling', ctypes.c_int64)]
 
 
+class DmTableDesc(ctypes.Structure):
+    """hcipy_dm_table_desc of include/hcipy_amd.h."""
+    _fields_ = [('nx', ctypes.c_int64), ('ny', ctypes.c_int64), ('x0', ctypes.c_double), ('y0', ctypes.c_double),
+                ('dx', ctypes.c_double), ('dy', ctypes.c_double), ('n_act', ctypes.c_int64),
+                ('act_x0', ctypes.c_double), ('act_y0', ctypes.c_double), ('pitch', ctypes.c_double),
+                ('x_tilt', ctypes.c_double), ('y_tilt', ctypes.c_double), ('z_tilt', ctypes.c_double),
+                ('table_nx', ctypes.c_int64), ('table_ny', ctypes.c_int64), ('table_x0', ctypes.c_double),
+                ('table_y0', ctypes.c_double), ('table_dx', ctypes.c_double), ('table_dy', ctypes.c_double),
+                ('table', ctypes.c_void_p)]
+
+
 _c_int64 = ctypes.c_int64
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -124,6 +135,8 @@ SIGNATURES = {
                                            _c_int64, _vp]),
     'hcipy_pw_stokes': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                        _vp, _c_int64, _vp]),
+    'hcipy_dm_table_surface': (ctypes.c_int, [ctypes.POINTER(DmTableDesc), _vp, _vp, _c_int64, _vp]),
+    'hcipy_dm_table_surface_adjoint': (ctypes.c_int, [ctypes.POINTER(DmTableDesc), _vp, _vp, _c_int64, _vp]),
     'hcipy_filter_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_filter_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, _vp, ctypes.c_size_t, _vp]),
     'hcipy_zoom_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int),

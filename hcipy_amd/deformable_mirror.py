@@ -1,13 +1,16 @@
-"""Deformable mirror with Gaussian influence functions (hcipy/optics/deformable_mirror.py).
+"""Deformable mirror with Gaussian or Xinetics influence functions
+(hcipy/optics/deformable_mirror.py).
 
-The influence functions are never stored as a matrix: ``GaussianInfluenceFunctions`` describes
-them (actuator grid, Gaussian width, cutoff, tilts, oversampling) and evaluates linear
-combinations and their adjoint with the ``hcipy_dm_gaussian_*`` kernels, in FP64.  It offers the
-parts of the reference's ModeBasis protocol the mirror uses: ``len``, ``grid``, iteration and
-indexing (one influence function as a Field), ``linear_combination`` and ``is_sparse``.
+The influence functions are never stored as a matrix: ``GaussianInfluenceFunctions`` and
+``XineticsInfluenceFunctions`` describe them (actuator grid, shape, tilts) and evaluate linear
+combinations and their adjoint with the ``hcipy_dm_gaussian_*`` / ``hcipy_dm_table_*`` kernels,
+in FP64.  They offer the parts of the reference's ModeBasis protocol the mirror uses: ``len``,
+``grid``, iteration and indexing (one influence function as a Field), ``linear_combination``
+and ``is_sparse``.
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 import torch
@@ -17,7 +20,7 @@ from .field import Field, default_device, make_uniform_grid
 from .optics import OpticalElement, Wavefront
 
 __all__ = ['make_actuator_positions', 'make_gaussian_influence_functions', 'GaussianInfluenceFunctions',
-           'DeformableMirror']
+           'make_xinetics_influence_functions', 'XineticsInfluenceFunctions', 'DeformableMirror']
 
 
 def make_actuator_positions(num_actuators_across_pupil, actuator_spacing, x_tilt=0, y_tilt=0, z_tilt=0):
@@ -42,37 +45,10 @@ class _SurfaceFn(torch.autograd.Function):
         return ctx.basis.linear_combination_adjoint(g.contiguous()), None
 
 
-class GaussianInfluenceFunctions:
-    """Truncated Gaussian influence functions of an n x n actuator grid on a regular 2-D pupil
-    grid (make_gaussian_influence_functions, deformable_mirror.py:43-93, with
-    make_gaussian_pokes, mode_basis/gaussian_pokes.py:5-54)."""
+class _InfluenceFunctions:
+    """Shared ModeBasis-like interface of the kernel-evaluated influence functions."""
 
     is_sparse = True
-
-    def __init__(self, pupil_grid, num_actuators_across_pupil, actuator_spacing, crosstalk=0.15, cutoff=3,
-                 x_tilt=0, y_tilt=0, z_tilt=0, oversampling=1):
-        if pupil_grid.ndim != 2 or not pupil_grid.is_regular:
-            raise ValueError('Gaussian influence functions need a regular two-dimensional pupil grid.')
-        n = int(num_actuators_across_pupil)
-        pos = make_actuator_positions(n, actuator_spacing)
-        self.grid = pupil_grid
-        self.num_actuators_across_pupil = n
-        self.actuator_spacing = float(actuator_spacing)
-        self.sigma = actuator_spacing / math.sqrt(-2 * math.log(crosstalk))
-        # cutoff in units of sigma as make_gaussian_pokes receives it, times sigma
-        self.radius = (actuator_spacing / self.sigma * cutoff) * self.sigma
-        d = _lib.DmGaussianDesc()
-        d.nx, d.ny = (int(v) for v in pupil_grid.dims)
-        d.x0, d.y0 = (float(v) for v in pupil_grid.zero)
-        d.dx, d.dy = (float(v) for v in pupil_grid.delta)
-        d.n_act = n
-        d.act_x0, d.act_y0 = (float(v) for v in pos.zero)
-        d.pitch = float(pos.delta[0])
-        d.sigma = self.sigma
-        d.radius = self.radius
-        d.x_tilt, d.y_tilt, d.z_tilt = float(x_tilt), float(y_tilt), float(z_tilt)
-        d.oversampling = int(round(oversampling))
-        self._desc = d
 
     def __len__(self):
         return self.num_actuators_across_pupil ** 2
@@ -104,9 +80,10 @@ class GaussianInfluenceFunctions:
         lead = tuple(c.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         out = torch.empty(lead + (self.grid.size,), dtype=torch.float64, device=c.device)
-        st = _lib.load().hcipy_dm_gaussian_surface(ctypes.byref(self._desc), _lib.dptr(c), _lib.dptr(out), batch,
-                                                   _lib.stream_ptr(c.device))
-        _lib.check(st, 'hcipy_dm_gaussian_surface')
+        fn, _ = self._entry_points
+        st = getattr(_lib.load(), fn)(ctypes.byref(self._desc), _lib.dptr(c), _lib.dptr(out), batch,
+                                      _lib.stream_ptr(c.device))
+        _lib.check(st, fn)
         return out
 
     def linear_combination_adjoint(self, field):
@@ -119,9 +96,10 @@ class GaussianInfluenceFunctions:
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         out = torch.empty(lead + (len(self),), dtype=torch.float64, device=t.device)
-        st = _lib.load().hcipy_dm_gaussian_surface_adjoint(ctypes.byref(self._desc), _lib.dptr(t), _lib.dptr(out),
-                                                           batch, _lib.stream_ptr(t.device))
-        _lib.check(st, 'hcipy_dm_gaussian_surface_adjoint')
+        _, fn = self._entry_points
+        st = getattr(_lib.load(), fn)(ctypes.byref(self._desc), _lib.dptr(t), _lib.dptr(out), batch,
+                                      _lib.stream_ptr(t.device))
+        _lib.check(st, fn)
         return out
 
     def __getitem__(self, i):
@@ -144,12 +122,124 @@ class GaussianInfluenceFunctions:
         return self.linear_combination(eye).as_subclass(torch.Tensor).T
 
 
+
+
+class GaussianInfluenceFunctions(_InfluenceFunctions):
+    """Truncated Gaussian influence functions of an n x n actuator grid on a regular 2-D pupil
+    grid (make_gaussian_influence_functions, deformable_mirror.py:43-93, with
+    make_gaussian_pokes, mode_basis/gaussian_pokes.py:5-54)."""
+
+    _entry_points = ('hcipy_dm_gaussian_surface', 'hcipy_dm_gaussian_surface_adjoint')
+
+    def __init__(self, pupil_grid, num_actuators_across_pupil, actuator_spacing, crosstalk=0.15, cutoff=3,
+                 x_tilt=0, y_tilt=0, z_tilt=0, oversampling=1):
+        if pupil_grid.ndim != 2 or not pupil_grid.is_regular:
+            raise ValueError('Gaussian influence functions need a regular two-dimensional pupil grid.')
+        n = int(num_actuators_across_pupil)
+        pos = make_actuator_positions(n, actuator_spacing)
+        self.grid = pupil_grid
+        self.num_actuators_across_pupil = n
+        self.actuator_spacing = float(actuator_spacing)
+        self.sigma = actuator_spacing / math.sqrt(-2 * math.log(crosstalk))
+        # cutoff in units of sigma as make_gaussian_pokes receives it, times sigma
+        self.radius = (actuator_spacing / self.sigma * cutoff) * self.sigma
+        d = _lib.DmGaussianDesc()
+        d.nx, d.ny = (int(v) for v in pupil_grid.dims)
+        d.x0, d.y0 = (float(v) for v in pupil_grid.zero)
+        d.dx, d.dy = (float(v) for v in pupil_grid.delta)
+        d.n_act = n
+        d.act_x0, d.act_y0 = (float(v) for v in pos.zero)
+        d.pitch = float(pos.delta[0])
+        d.sigma = self.sigma
+        d.radius = self.radius
+        d.x_tilt, d.y_tilt, d.z_tilt = float(x_tilt), float(y_tilt), float(z_tilt)
+        d.oversampling = int(round(oversampling))
+        self._desc = d
+
 def make_gaussian_influence_functions(pupil_grid, num_actuators_across_pupil, actuator_spacing, crosstalk=0.15,
                                       cutoff=3, x_tilt=0, y_tilt=0, z_tilt=0, oversampling=1):
     """Gaussian influence functions (deformable_mirror.py:43-93): sigma = spacing /
     sqrt(-2 ln crosstalk), truncated at ``cutoff`` actuator spacings."""
     return GaussianInfluenceFunctions(pupil_grid, num_actuators_across_pupil, actuator_spacing, crosstalk, cutoff,
                                       x_tilt, y_tilt, z_tilt, oversampling)
+
+
+def _read_fits_f64(path):
+    """The primary array of a plain FITS file with BITPIX -64 (big-endian FP64), as an array of
+    shape (NAXISn, ..., NAXIS1) with the length-1 axes squeezed."""
+    with open(path, 'rb') as fh:
+        raw = fh.read()
+    cards = {}
+    pos = 0
+    while True:
+        block = raw[pos:pos + 2880]
+        pos += 2880
+        end = False
+        for k in range(0, 2880, 80):
+            card = block[k:k + 80].decode('ascii', 'replace')
+            key = card[:8].strip()
+            if key == 'END':
+                end = True
+                break
+            if card[8:10] == '= ':
+                cards[key] = card[10:].split('/')[0].strip()
+        if end:
+            break
+    if int(cards['BITPIX']) != -64:
+        raise ValueError('expected a BITPIX -64 FITS array')
+    naxis = int(cards['NAXIS'])
+    shape = tuple(int(cards['NAXIS%d' % (i + 1)]) for i in range(naxis))[::-1]
+    n = int(np.prod(shape))
+    data = np.frombuffer(raw, dtype='>f8', count=n, offset=pos).astype(np.float64).reshape(shape)
+    return np.squeeze(data)
+
+
+_XINETICS_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data', 'influence_dm5v2.fits')
+
+
+class XineticsInfluenceFunctions(_InfluenceFunctions):
+    """Influence functions of a Xinetics deformable mirror (make_xinetics_influence_functions,
+    deformable_mirror.py:95-138): the measured DM5V2 actuator shape (91 x 91 samples at a tenth
+    of the actuator spacing, normalised to a peak of 1), bilinearly interpolated at the tilted
+    pupil point relative to each actuator, zero outside the sampled extent, times
+    cos(y_tilt) / cos(x_tilt)."""
+
+    _entry_points = ('hcipy_dm_table_surface', 'hcipy_dm_table_surface_adjoint')
+
+    def __init__(self, pupil_grid, num_actuators_across_pupil, actuator_spacing, x_tilt=0, y_tilt=0, z_tilt=0):
+        if pupil_grid.ndim != 2 or not pupil_grid.is_regular:
+            raise ValueError('Xinetics influence functions need a regular two-dimensional pupil grid.')
+        n = int(num_actuators_across_pupil)
+        pos = make_actuator_positions(n, actuator_spacing)
+        self.grid = pupil_grid
+        self.num_actuators_across_pupil = n
+        self.actuator_spacing = float(actuator_spacing)
+        actuator = _read_fits_f64(_XINETICS_TABLE)
+        actuator = actuator / actuator.max()
+        # the actuator grid: make_uniform_grid(shape, shape * spacing / 10) (x extent from the
+        # first axis of the array's shape, as the reference passes it)
+        shape = np.array(actuator.shape)
+        tgrid = make_uniform_grid(actuator.shape, shape * actuator_spacing / 10.0)
+        self._table = torch.as_tensor(np.ascontiguousarray(actuator), device=default_device())
+        d = _lib.DmTableDesc()
+        d.nx, d.ny = (int(v) for v in pupil_grid.dims)
+        d.x0, d.y0 = (float(v) for v in pupil_grid.zero)
+        d.dx, d.dy = (float(v) for v in pupil_grid.delta)
+        d.n_act = n
+        d.act_x0, d.act_y0 = (float(v) for v in pos.zero)
+        d.pitch = float(pos.delta[0])
+        d.x_tilt, d.y_tilt, d.z_tilt = float(x_tilt), float(y_tilt), float(z_tilt)
+        d.table_nx, d.table_ny = (int(v) for v in tgrid.dims)
+        d.table_x0, d.table_y0 = (float(v) for v in tgrid.zero)
+        d.table_dx, d.table_dy = (float(v) for v in tgrid.delta)
+        d.table = _lib.dptr(self._table)
+        self._desc = d
+
+
+def make_xinetics_influence_functions(pupil_grid, num_actuators_across_pupil, actuator_spacing, x_tilt=0, y_tilt=0,
+                                      z_tilt=0):
+    """Xinetics influence functions (deformable_mirror.py:95-138)."""
+    return XineticsInfluenceFunctions(pupil_grid, num_actuators_across_pupil, actuator_spacing, x_tilt, y_tilt, z_tilt)
 
 
 class DeformableMirror(OpticalElement):

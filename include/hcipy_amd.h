@@ -302,6 +302,31 @@ int hcipy_dm_gaussian_surface(const hcipy_dm_gaussian_desc* desc, const double* 
 int hcipy_dm_gaussian_surface_adjoint(const hcipy_dm_gaussian_desc* desc, const double* grad_surface,
                                       double* grad_actuators, int64_t batch, void* stream);
 
+/* Deformable mirror whose influence function is a sampled actuator shape: the Xinetics DM5V2
+ * shape of make_xinetics_influence_functions (hcipy/optics/deformable_mirror.py:95-138).
+ * IF_a(p) = bilinear interpolation of `table` at q - m_a (zero outside the table's extent), with
+ * q = R(-z_tilt) . diag(1/cos(y_tilt), 1/cos(x_tilt)) . p, times cos(y_tilt) / cos(x_tilt) (the
+ * reference's "/ cos(x_tilt) * cos(y_tilt)").  The table is a device array [table_ny][table_nx]
+ * on the coordinates table_x0 + i table_dx, table_y0 + j table_dy, at most 128 x 128 samples. */
+typedef struct {
+  int64_t nx, ny;
+  double x0, y0, dx, dy;
+  int64_t n_act;
+  double act_x0, act_y0;
+  double pitch;
+  double x_tilt, y_tilt, z_tilt;
+  int64_t table_nx, table_ny;
+  double table_x0, table_y0, table_dx, table_dy;
+  const double* table;
+} hcipy_dm_table_desc;
+
+/* surface[b][pixel] = sum_a actuators[b][a] IF_a(pixel), FP64. */
+int hcipy_dm_table_surface(const hcipy_dm_table_desc* desc, const double* actuators, double* surface, int64_t batch,
+                           void* stream);
+/* grad_actuators[b][a] = sum_pixel grad_surface[b][pixel] IF_a(pixel) (transposed influence matrix). */
+int hcipy_dm_table_surface_adjoint(const hcipy_dm_table_desc* desc, const double* grad_surface, double* grad_actuators,
+                                   int64_t batch, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Fourier filter: zero-pad -> 2-D FFT -> transfer function -> inverse 2-D FFT -> crop.
  * Replaces FourierFilter (hcipy/fourier/fourier_operations.py:8-151) for scalar transfer

@@ -39,6 +39,9 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
   integral of the PSD over the FFT grid (sum psd w / (2 pi)^2: the (2 pi)^ndim and weight
   factors), a one-pixel shift is np.roll of the screen by one pixel along that axis, two half
   shifts compose, and the reference's peak-to-valley thresholds (tests/test_optics.py:823-838);
+* ``xinetics_influence_functions``: at zero tilt, on a pupil grid whose samples fall on the
+  table's samples, each influence function equals the table entries (the shift, axis order
+  and normalisation); the reference's tilted-centroid test (tests/test_optics.py:119-152);
 * ``stokes_vector``: for pure input states the Mueller rows must agree with the Stokes
   parameters of the propagated Jones vector J e (an independent closed form), and those four
   states span every input Stokes vector; Malus's law for a rotated polarizer;
@@ -873,6 +876,32 @@ def gaussian_influence_functions(pupil_grid, n_act, pitch, crosstalk=0.15, cutof
     return out / ovs ** 2
 
 
+def xinetics_influence_functions(pupil_grid, n_act, pitch, table, x_tilt=0, y_tilt=0, z_tilt=0):
+    """make_xinetics_influence_functions (hcipy/optics/deformable_mirror.py:95-138): the actuator
+    shape ``table`` ([ny][nx], the DM5V2 FITS array) normalised to its maximum, on
+    make_uniform_grid(shape, shape * pitch / 10), interpolated linearly (scipy
+    RegularGridInterpolator, fill 0 outside, field/interpolation linear.py:5-50) at the pupil
+    points scaled by 1/cos([y_tilt, x_tilt]), rotated by -z_tilt and shifted by -m_a, times
+    cos(y_tilt) / cos(x_tilt) (the reference's "/ cos(x_tilt) * cos(y_tilt)").  Returns a dense
+    (n_act^2, Npix) array."""
+    from scipy.interpolate import RegularGridInterpolator
+    actuator = np.asarray(table, dtype=float)
+    actuator = actuator / actuator.max()
+    shape = np.array(actuator.shape)
+    tg = make_uniform_grid(list(actuator.shape), shape * pitch / 10.0)
+    interp = RegularGridInterpolator((tg.sep[0], tg.sep[1]), actuator.T, 'linear', False, 0)
+    xs = pupil_grid.x / np.cos(y_tilt)
+    ys = pupil_grid.y / np.cos(x_tilt)
+    ca, sa = np.cos(-z_tilt), np.sin(-z_tilt)
+    qx = ca * xs - sa * ys
+    qy = sa * xs + ca * ys
+    px, py = actuator_positions(n_act, pitch).coords
+    out = np.zeros((len(px), pupil_grid.size))
+    for i, (mx, my) in enumerate(zip(px, py)):
+        out[i] = interp(np.stack([qx - mx, qy - my], axis=-1)) / np.cos(x_tilt) * np.cos(y_tilt)
+    return out
+
+
 def dm_surface(influence, actuators):
     """ModeBasis.linear_combination (mode_basis/mode_basis.py:239-258)."""
     return np.asarray(actuators) @ influence
@@ -932,11 +961,13 @@ def power_law_phase(shape, exponent, rms, mask, seed):
 
 
 def read_fits_image(path):
-    """Reads the primary image of a (gzipped) FITS file: 2880-byte header blocks, big-endian
-    data of BITPIX -64 / -32 / 8 / 16 / 32."""
+    """Reads the primary image of a (gzipped or plain) FITS file: 2880-byte header blocks,
+    big-endian data of BITPIX -64 / -32 / 8 / 16 / 32."""
     import gzip
-    with gzip.open(path, 'rb') as fh:
+    with open(path, 'rb') as fh:
         raw = fh.read()
+    if raw[:2] == b'\x1f\x8b':
+        raw = gzip.decompress(raw)
     hdr = {}
     pos = 0
     while True:

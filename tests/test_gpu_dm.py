@@ -1,5 +1,5 @@
-"""Deformable mirror (matrix-free Gaussian influence functions) and the region-of-interest mean
-against the FP64 oracle (GPU).
+"""Deformable mirror (matrix-free Gaussian and Xinetics influence functions) and the
+region-of-interest mean against the FP64 oracle (GPU).
 
 Tolerances: surface and its adjoint 1e-12 of the peak (FP64 on both sides, different summation
 order); ROI mean 1e-12 relative.
@@ -7,6 +7,8 @@ order); ROI mean 1e-12 relative.
 import numpy as np
 import pytest
 import torch
+
+import os
 
 import hcipy_amd as ha
 from oracle import hcipy_oracle as o
@@ -47,16 +49,45 @@ def test_surface_and_adjoint_vs_oracle(npix, extent, n_act, tilts, ovs):
     assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(s) * np.abs(g))
 
 
+XINETICS_TABLE = np.squeeze(o.read_fits_image(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                          'hcipy_amd', 'data', 'influence_dm5v2.fits')))
+
+
+@pytest.mark.parametrize('npix,extent,n_act,tilts', [(64, 1.1, 8, (0.0, 0.0, 0.0)),
+                                                     (96, 1.5, 12, (np.radians(15), np.radians(10), np.radians(20)))])
+def test_xinetics_surface_and_adjoint_vs_oracle(npix, extent, n_act, tilts):
+    grid = ha.make_pupil_grid(npix, extent)
+    pitch = 1.0 / n_act
+    IFo = o.xinetics_influence_functions(og(grid), n_act, pitch, XINETICS_TABLE, *tilts)
+    IF = ha.make_xinetics_influence_functions(grid, n_act, pitch, x_tilt=tilts[0], y_tilt=tilts[1], z_tilt=tilts[2])
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((2, n_act * n_act)) * 1e-8
+    say('xinetics dm', npix, n_act, tilts)
+    s = host(IF.linear_combination(a))
+    s_ref = a @ IFo
+    assert np.abs(s - s_ref).max() <= 1e-12 * np.abs(s_ref).max()
+    g = rng.standard_normal((2, grid.size))
+    ga = host(IF.linear_combination_adjoint(dev(g)))
+    ga_ref = g @ IFo.T
+    assert np.abs(ga - ga_ref).max() <= 1e-12 * np.abs(ga_ref).max()
+    lhs = np.sum(s * g)
+    rhs = np.sum(a * ga)
+    assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(s) * np.abs(g))
+
+
+@pytest.mark.parametrize('kind', ['gaussian', 'xinetics'])
 @pytest.mark.parametrize('n_act', [12, 16])
-def test_reference_mirror_behaviour(n_act):
-    """The reference's DM test (tests/test_optics.py:119-193) on the GPU mirror."""
+def test_reference_mirror_behaviour(n_act, kind):
+    """The reference's DM test (tests/test_optics.py:119-193) on the GPU mirror, for both
+    influence-function families it covers."""
     x_tilt, y_tilt, z_tilt = np.radians(15), np.radians(10), np.radians(20)
     grid = ha.make_pupil_grid(256, 1.5)
     pitch = 1.0 / n_act
     pos = ha.make_actuator_positions(n_act, pitch)
     assert np.allclose(pos.delta, pitch)
     pos_t = ha.make_actuator_positions(n_act, pitch, x_tilt=x_tilt, y_tilt=y_tilt, z_tilt=z_tilt)
-    IF = ha.make_gaussian_influence_functions(grid, n_act, pitch, x_tilt=x_tilt, y_tilt=y_tilt, z_tilt=z_tilt)
+    make = ha.make_gaussian_influence_functions if kind == 'gaussian' else ha.make_xinetics_influence_functions
+    IF = make(grid, n_act, pitch, x_tilt=x_tilt, y_tilt=y_tilt, z_tilt=z_tilt)
     dm = ha.DeformableMirror(IF)
     assert len(IF) == n_act ** 2 and dm.num_actuators == n_act ** 2
     assert IF.is_sparse

@@ -10,6 +10,8 @@ import scipy.special
 
 from oracle import hcipy_oracle as o
 
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
 GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
 
 
@@ -582,3 +584,54 @@ def test_power_law_error_reference_ptv(exponent):
     large = o.evaluate_supersampled(lambda gg: o.circular_aperture(gg, 2.0), g, 4)
     screen = o.power_law_error(g, 0.1, 2.0, exponent=exponent, seed=5)
     assert abs((np.ptp(screen[large > 0]) - 0.1) / 0.1) < 1e-10
+
+
+def _xinetics_table():
+    return np.squeeze(o.read_fits_image(os.path.join(ROOT, 'hcipy_amd', 'data', 'influence_dm5v2.fits')))
+
+
+def test_xinetics_table_is_the_reference_file():
+    """The packaged actuator shape is the reference's data file, byte for byte (when the
+    reference tree is present)."""
+    ref = '/root/reference/hcipy/optics/influence_dm5v2.fits'
+    if not os.path.exists(ref):
+        pytest.skip('reference tree not present')
+    with open(ref, 'rb') as a, open(os.path.join(ROOT, 'hcipy_amd', 'data', 'influence_dm5v2.fits'), 'rb') as b:
+        assert a.read() == b.read()
+
+
+def test_xinetics_influence_on_table_samples():
+    """Untilted, on a pupil grid whose samples coincide with the table's samples around each
+    actuator, an influence function is the normalised table itself (shift and normalisation),
+    and zero beyond the table's extent."""
+    T = _xinetics_table()
+    pitch = 0.1
+    # one actuator at the origin; pupil grid = the table grid extended by 10 samples each side
+    g = o.make_uniform_grid([111, 111], [111 * pitch / 10] * 2)
+    IF = o.xinetics_influence_functions(g, 1, pitch, T)[0].reshape(111, 111)
+    # the outermost table samples sit on the table's edge, where rounding decides between the
+    # sample and the zero fill (as in the reference): compare the interior
+    assert np.abs(IF[11:100, 11:100] - T[1:90, 1:90] / T.max()).max() < 1e-14
+    assert np.abs(IF[:9]).max() == 0 and np.abs(IF[:, 102:]).max() == 0
+    # two actuators across: the second one's function is the first one's shifted by one pitch
+    IF2 = o.xinetics_influence_functions(g, 2, pitch, T)
+    a = IF2[0].reshape(111, 111)
+    b = IF2[1].reshape(111, 111)
+    assert np.abs(b[:, 12:] - a[:, 2:-10]).max() < 1e-14
+
+
+@pytest.mark.parametrize('n_act', [12, 16])
+def test_xinetics_tilted_centroids(n_act):
+    """tests/test_optics.py:119-152 for the Xinetics functions: every tilted influence function's
+    centroid lies within 0.02 of the tilted actuator position."""
+    x_tilt, y_tilt, z_tilt = np.radians(15), np.radians(10), np.radians(20)
+    grid = o.make_pupil_grid(128, 1.5)
+    pitch = 1.0 / n_act
+    IF = o.xinetics_influence_functions(grid, n_act, pitch, _xinetics_table(), x_tilt, y_tilt, z_tilt)
+    px, py = o.actuator_positions(n_act, pitch).coords
+    px, py = px * np.cos(y_tilt), py * np.cos(x_tilt)
+    c, s = np.cos(z_tilt), np.sin(z_tilt)
+    px, py = c * px - s * py, s * px + c * py
+    for k in range(n_act * n_act):
+        w = IF[k]
+        assert np.allclose([(w * grid.x).sum() / w.sum(), (w * grid.y).sum() / w.sum()], [px[k], py[k]], atol=0.02)
