@@ -131,3 +131,81 @@ def test_planner_measure():
     a = host(ft.forward(ha.Field(dev(f), pupil)))
     b = host(ha.MatrixFourierTransform(pupil, uv).forward(ha.Field(dev(f), pupil)))
     assert peak_rel_field(a, b) < 1e-12
+
+
+def _all_transforms(g, go):
+    return [ha.MatrixFourierTransform(g, go), ha.FastFourierTransform(g, *ha.get_fft_parameters(go, g)),
+            ha.ZoomFastFourierTransform(g, go)]
+
+
+@pytest.mark.parametrize('dims,shift_in,scale,shift_out', [([9, 18], [0.1, 0.1], 2, [0.1, 0.1]), ([8, 16], [0, 0], 1, [0, 0]),
+                                                          ([65], [0.1], 2, [0.1])])
+@pytest.mark.parametrize('dtype', ['complex128', 'complex64'])
+def test_energy_conservation_across_methods(dims, shift_in, scale, shift_out, dtype):
+    """tests/test_fourier.py:22-96 with the zoom FFT among the methods: at q = 1, fov = 1 every
+    forward (MFT, FFT, zoom FFT) followed by every backward is the identity and conserves
+    energy (c128 1e-12; c64 1e-6 pattern, 1e-5 energy)."""
+    g = ha.make_uniform_grid(dims, 1, has_center=True).shifted(shift_in).scaled(scale)
+    go = ha.make_fft_grid(g, 1, 1, shift_out)
+    f = np.random.default_rng(0).standard_normal(g.size).astype(dtype)
+    e_in = np.sum(np.abs(f) ** 2) * np.asarray(g.weights).ravel()[0]
+    fts = _all_transforms(g, go)
+    for a in fts:
+        F = a.forward(ha.Field(dev(f), g))
+        for b in fts:
+            back = host(b.backward(F))
+            assert back.dtype == np.dtype(dtype)
+            e_out = np.sum(np.abs(back) ** 2) * np.asarray(g.weights).ravel()[0]
+            pat = np.std(np.abs(back - f)) / np.abs(f).mean()
+            if dtype == 'complex128':
+                assert abs(e_out / e_in - 1) < 1e-12 and pat < 1e-12, (type(a).__name__, type(b).__name__)
+            else:
+                assert abs(e_out / e_in - 1) < 1e-5 and pat < 1e-6, (type(a).__name__, type(b).__name__)
+
+
+@pytest.mark.parametrize('dtype', ['complex128', 'complex64'])
+def test_zoom_symmetry_and_linearity(dtype):
+    """tests/test_fourier.py:98-144 for the zoom FFT: even inputs give real, odd inputs imaginary
+    transforms on a symmetric grid; forward is linear."""
+    tol = 1e-12 if dtype == 'complex128' else 1e-6
+    g = ha.make_uniform_grid([9, 9], 1, has_center=False).scaled(2)
+    go = ha.make_fft_grid(g, 3, 0.8)
+    f = np.random.default_rng(0).standard_normal(g.size)
+    fe = ((f + f[::-1]) / 2).astype(dtype)
+    fo = ((f - f[::-1]) / 2).astype(dtype)
+    zf = ha.ZoomFastFourierTransform(g, go)
+    Fe = host(zf.forward(ha.Field(dev(fe), g)))
+    Fo = host(zf.forward(ha.Field(dev(fo), g)))
+    assert np.abs(Fe.imag).max() < tol * max(1, np.abs(Fe).max())
+    assert np.abs(Fo.real).max() < tol * max(1, np.abs(Fo).max())
+    Fb = host(zf.forward(ha.Field(dev(f.astype(dtype)), g)))
+    assert np.allclose(Fb, Fe + Fo, atol=tol * np.abs(Fb).max())
+    assert np.allclose(host(zf.forward(ha.Field(dev((2 * f).astype(dtype)), g))), 2 * Fb, atol=tol * np.abs(Fb).max())
+
+
+def test_zoom_1d_matches_scipy_czt():
+    """The 1-D zoom FFT is a chirp-z transform times the output phase shift
+    (zoom_fast_fourier_transform.py:42-68): against the dense sum at 1e-12 and against
+    scipy.signal.CZT (the reference's own yardstick, tests/test_fourier.py:389-424) within
+    scipy's k^2 drift, for random lengths and spacings."""
+    import scipy.signal
+    rng = np.random.default_rng(1)
+    for _ in range(4):
+        n = int(rng.integers(50, 3000))
+        m = int(rng.integers(50, 3000))
+        dx, du = rng.uniform(0.5, 2.0), rng.uniform(0.001, 0.02)
+        x0, u0 = rng.uniform(-3, 3), rng.uniform(-5, 5)
+        g = ha.make_uniform_grid([n], [n * dx], center=[x0 + (n - 1) * dx / 2])
+        go = ha.make_uniform_grid([m], [m * du], center=[u0 + (m - 1) * du / 2])
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        out = host(ha.ZoomFastFourierTransform(g, go).forward(ha.Field(dev(x), g)))
+        w = np.exp(-1j * du * dx)
+        a = np.exp(1j * go.zero[0] * dx)
+        ref = scipy.signal.CZT(n, m, w, a)(x) * np.exp(-1j * go.separated_coords[0] * g.zero[0]) * dx
+        dense = np.exp(-1j * np.outer(go.separated_coords[0], g.separated_coords[0])) @ x * dx
+        peak = np.abs(dense).max()
+        # phases u x reach max|u| max|x| radians, rounded at 1e-16 of that in both the device's
+        # chirps and the dense sum; scipy forms w^(k^2/2) by powers and drifts by ~k^2 * 1e-16
+        phase = np.abs(go.separated_coords[0]).max() * np.abs(g.separated_coords[0]).max()
+        assert np.abs(out - dense).max() < (1e-12 + 1e-15 * phase) * peak, (n, m)
+        assert np.abs(out - ref).max() < (1e-12 + 1e-15 * (n + m) ** 2) * peak, (n, m)

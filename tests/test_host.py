@@ -45,3 +45,23 @@ def test_planner_estimates_and_choice():
     assert type(ha.make_fourier_transform(p, q=2)) is ha.FastFourierTransform
     with pytest.raises(ValueError):
         ha.make_fourier_transform(p, uv, planner='guess')
+
+
+def test_planner_reference_choices():
+    """tests/test_fourier.py:146-182: an FFT grid (q = 1, fov = 1, shift 0.1) goes to the FFT,
+    q = 8 / fov = 0.3 to the MFT, a 1024^2 pupil at q = 8 / fov = 0.1 to the zoom FFT; an
+    unstructured output grid has no device transform here (ValueError)."""
+    import hcipy_amd as ha
+    g = ha.make_pupil_grid(128)
+    ft = ha.make_fourier_transform(g, q=1, fov=1, shift=0.1)
+    assert type(ft) is ha.FastFourierTransform and ft.input_grid is g
+    assert type(ha.make_fourier_transform(g, ha.make_fft_grid(g, q=1, fov=1, shift=0.1))) is ha.FastFourierTransform
+    assert type(ha.make_fourier_transform(g, q=8, fov=0.3, shift=0.1)) is ha.MatrixFourierTransform
+    assert type(ha.make_fourier_transform(g, ha.make_fft_grid(g, q=8, fov=0.3, shift=0.1))) is ha.MatrixFourierTransform
+    big = ha.make_pupil_grid(1024)
+    assert type(ha.make_fourier_transform(big, ha.make_fft_grid(big, q=8, fov=0.1))) is ha.ZoomFastFourierTransform
+    from hcipy_amd.field import CartesianGrid, UnstructuredCoords
+    rng = np.random.default_rng(0)
+    un = CartesianGrid(UnstructuredCoords([rng.standard_normal(100), rng.standard_normal(100)]))
+    with pytest.raises(ValueError):
+        ha.make_fourier_transform(big, un)

@@ -635,3 +635,22 @@ def test_xinetics_tilted_centroids(n_act):
     for k in range(n_act * n_act):
         w = IF[k]
         assert np.allclose([(w * grid.x).sum() / w.sum(), (w * grid.y).sum() / w.sum()], [px[k], py[k]], atol=0.02)
+
+
+@pytest.mark.parametrize('dtype', ['complex128', 'complex64'])
+def test_czt_matches_scipy(dtype):
+    """tests/test_fourier.py:389-424: the chirp-z restatement equals scipy.signal.CZT for random
+    lengths (exponential, mean 1000) and random unit-circle w, a (rtol 1e-13 in complex128,
+    1e-4 after rounding the input to complex64)."""
+    import scipy.signal
+    rng = np.random.default_rng(0)
+    ns = rng.exponential(1000, size=6).astype(int) + 1
+    ms = rng.exponential(1000, size=6).astype(int) + 1
+    for n, m in zip(ns, ms):
+        x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(dtype)
+        w = np.exp(1j * rng.uniform(0, 2 * np.pi))
+        a = np.exp(1j * rng.uniform(0, 2 * np.pi))
+        y = o.czt(x.astype(complex), n, m, w, a)
+        ref = scipy.signal.CZT(n, m, w, a)(x.astype(complex))
+        rtol = 1e-13 if dtype == 'complex128' else 1e-4
+        assert np.allclose(y, ref, rtol=rtol)
