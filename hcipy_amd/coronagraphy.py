@@ -161,9 +161,12 @@ class MultiScaleCoronagraph(OpticalElement):
         self._group = None
         # grouped: the level transforms run as two grouped launches; False runs them one by one
         self.grouped = True
+        self._masks_h = None
 
     def _levels(self, E, conjugate):
         """sum over levels >= 1 of P_i^dagger (M_i P_i E), accumulated onto the level-0 result."""
+        if (self.focal_masks[0].dim() == 3 and E.dim() == 1 and self.grouped and not pointwise.needs_grad(E)):
+            return self._levels_scalar_jones(E, conjugate)
         L = self.props[0].backward(E) if conjugate else self.props[0].forward(E)
         rest = list(zip(self.focal_masks[1:], self.props[1:]))
         if not rest:
@@ -193,6 +196,43 @@ class MultiScaleCoronagraph(OpticalElement):
             F = self._apply_mask(Field(F, mask.grid), mask, conjugate)
             w_uv = complex(np.asarray(d.uv_grid.weights).ravel()[0]) / (2 * np.pi) ** 2
             bwd.append((d.fourier_transform, _lib.BACKWARD, F, -w_uv / d.norm_factor, _lib.OUTPUT_FIELD))
+        for neg in self._group[1].execute(bwd):
+            L = pointwise.sub_mul(L, neg)
+        return L
+
+    def _levels_scalar_jones(self, E, conjugate):
+        """Jones masks acting on a scalar field E, i.e. on E times the 2 x 2 identity: every
+        level is linear, so level 0 is each element filter applied to E (T[i][k] = F_ik(E),
+        adjoint F_ki^H(E)), and level i forms P_i E once and the masked field J P_i E (J^H
+        for the adjoint) per component.  The same sums as the tensor field E (x) I, without the
+        transforms of its zero components."""
+        f0 = self.props[0]
+        if conjugate:
+            L = f0.apply_elements(E, adjoint=True).as_subclass(torch.Tensor).transpose(0, 1).contiguous()
+        else:
+            L = f0.apply_elements(E).as_subclass(torch.Tensor)
+        rest = list(zip(self.focal_masks[1:], self.props[1:]))
+        if not rest:
+            return L
+        if self._group is None or self._group[0].device != E.device:
+            self._group = (MftGroup(max(16, len(rest)), E.device), MftGroup(max(16, len(rest)), E.device))
+        if conjugate and self._masks_h is None:
+            self._masks_h = [torch.conj(m.transpose(0, 1)).contiguous() for m in self.focal_masks]
+        w_in = complex(np.asarray(self.input_grid.weights).ravel()[0])
+        fwd = []
+        for mask, prop in rest:
+            d = prop.instance(1)
+            fwd.append((d.fourier_transform, _lib.FORWARD, E, w_in * d.norm_factor, _lib.OUTPUT_FIELD))
+        Fs = self._group[0].execute(fwd)
+        bwd = []
+        for k, (F, (mask, prop)) in enumerate(zip(Fs, rest)):
+            d = prop.instance(1)
+            J = self._masks_h[k + 1] if conjugate else mask
+            # (J F)[i][k] = J_ik F: the masked E (x) I, in the field's dtype
+            F4 = F.as_subclass(torch.Tensor).expand((2, 2) + tuple(F.shape)).contiguous()
+            G = pointwise.multiply(F4, J)
+            w_uv = complex(np.asarray(d.uv_grid.weights).ravel()[0]) / (2 * np.pi) ** 2
+            bwd.append((d.fourier_transform, _lib.BACKWARD, G, -w_uv / d.norm_factor, _lib.OUTPUT_FIELD))
         for neg in self._group[1].execute(bwd):
             L = pointwise.sub_mul(L, neg)
         return L
@@ -280,10 +320,21 @@ class VectorVortexCoronagraph(AgnosticOpticalElement):
             return wavefront
         return Wavefront(wavefront.electric_field, wavefront.wavelength, (1, 0, 0, 0))
 
+    def _run(self, wavefront, conjugate):
+        """The levels on the wavefront; a scalar wavefront is the unpolarized E (x) I, whose
+        levels are computed from E itself."""
+        d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
+        coro = d.coronagraph
+        E = wavefront.electric_field
+        if wavefront.is_scalar and not pointwise.needs_grad(E):
+            L = coro._levels(E.as_subclass(torch.Tensor), conjugate)
+            return Wavefront(Field(L, coro.input_grid), wavefront.wavelength, (1, 0, 0, 0))
+        wf = self._polarized(wavefront)
+        return coro.backward(wf) if conjugate else coro.forward(wf)
+
     def forward(self, wavefront):
         """Pupil -> Lyot plane (vortex.py:160-202)."""
-        d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
-        lyot = d.coronagraph.forward(self._polarized(wavefront))
+        lyot = self._run(wavefront, False)
         if self.lyot_stop is not None:
             lyot = self.lyot_stop.forward(lyot)
         return lyot
@@ -293,5 +344,4 @@ class VectorVortexCoronagraph(AgnosticOpticalElement):
         conjugate Jones matrices (the retarder's Jones matrix is symmetric, so conj(J) = J^H)."""
         if self.lyot_stop is not None:
             wavefront = self.lyot_stop.backward(wavefront)
-        d = self.get_instance_data(wavefront.grid, None, wavefront.wavelength)
-        return d.coronagraph.backward(self._polarized(wavefront))
+        return self._run(wavefront, True)

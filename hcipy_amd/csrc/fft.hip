@@ -124,16 +124,17 @@ __global__ __launch_bounds__((RowCfg<T, NX>::WG)) void rows_inverse_kernel(const
 
 template <typename T, int NY>
 struct ColCfg {
-  // 8 points per thread: at 16 the up to 1024 threads of a workgroup spill registers
-  static constexpr int E = NY >= 8 ? 8 : NY;
+  // 16 points per thread (one LDS round trip fewer than 8), with at most 512 threads per
+  // workgroup so that they fit the register budget unspilled (measured faster than 8 points
+  // with 1024 threads)
+  static constexpr int E = NY >= 16 ? 16 : NY;
   static constexpr int P = NY / E;
   static constexpr int LINE = hr_lds_points(NY) + 1;
   // columns per workgroup: as many lines as fit in 70 KiB of LDS (two workgroups per CU), at most
-  // 16, and at most 1024 threads (512 for FP64, whose 8 points per thread then fit the register
-  // budget unspilled).  Twiddles are read from global memory: an LDS copy would take the room of
-  // the second workgroup, which was measured faster.
+  // 16, and at most 512 threads.  Twiddles are read from global memory: an LDS copy would take
+  // the room of the second workgroup, which was measured faster.
   static constexpr int CF = (int)(71680 / ((size_t)LINE * 2 * sizeof(T)));
-  static constexpr int CT = (sizeof(T) == 8 ? 512 : 1024) / P;
+  static constexpr int CT = 512 / P > 0 ? 512 / P : 1;
   static constexpr int C0 = CF < CT ? CF : CT;
   static constexpr int C = C0 > 16 ? 16 : (C0 < 1 ? 1 : C0);
   static constexpr int NT = C * P;

@@ -884,7 +884,6 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_f16x2_kernel(StageArgs 
   constexpr int NMF = 6 * TMC * TN;       // MFMAs per k-vector
   constexpr int NDR = 2 * (TMC + TN);     // fragment reads per k-vector
   constexpr int NST = 2 * (AL + BL);      // LDS stores (= global loads) per step
-  constexpr int ST_PER_KV = (NST + KV_PER_WAVE - 1) / KV_PER_WAVE;
   auto interleave = [&]() __attribute__((always_inline)) {
     // one LDS store and one global load after each of the first NST MFMAs, fragment reads of
     // the next k-vector in between

@@ -177,6 +177,32 @@ class FourierFilter:
             outs.append(acc)
         return torch.stack(outs, dim=0)
 
+    def apply_elements(self, field, adjoint=False):
+        """For a matrix transfer function (p, q): every element filter applied to the same scalar
+        field, T[i][j] = F_ij(E) of shape (p, q, n), or with ``adjoint`` the adjoint element
+        filters F_ij^H(E).  This is the matrix filter of E times the identity (E on the diagonal
+        of a q x q tensor field) without the filters of its zero components."""
+        t = field.as_subclass(torch.Tensor) if isinstance(field, Field) else field
+        _lib.require_device(t)
+        if not self._is_matrix(t.device):
+            raise ValueError('apply_elements needs a matrix transfer function')
+        cdt = complex_dtype_for(t.dtype)
+        t = t.to(cdt).contiguous()
+        tf = self._matrix_tf
+        p, q = int(tf.shape[0]), int(tf.shape[1])
+        lead = tuple(t.shape[:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        out = torch.empty((p, q) + tuple(t.shape), dtype=cdt, device=t.device)
+        for i in range(p):
+            for j in range(q):
+                key = (cdt, t.device.index, i, j, False)
+                plan = self._plans.get(key)
+                if plan is None:
+                    plan = _FilterPlan(self, cdt, t.device, tf=tf[i, j])
+                    self._plans[key] = plan
+                plan.execute(adjoint, t, out[i, j], batch)
+        return Field(out, self.input_grid)
+
     def _operation(self, field, adjoint):
         grid = self.input_grid
         if not isinstance(field, torch.Tensor):
