@@ -130,6 +130,7 @@ SIGNATURES = {
                                                         ctypes.c_int, _vp, _vp]),
     'hcipy_pw_field_dot': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                                           ctypes.c_int, _c_int64, _vp, _c_int64, _vp]),
+    'hcipy_pw_sub_many': (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int, _vp, ctypes.c_int, _c_int64, _vp]),
     'hcipy_pw_phase_ramp': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp,
                                            _c_int64, _vp]),

@@ -196,9 +196,7 @@ class MultiScaleCoronagraph(OpticalElement):
             F = self._apply_mask(Field(F, mask.grid), mask, conjugate)
             w_uv = complex(np.asarray(d.uv_grid.weights).ravel()[0]) / (2 * np.pi) ** 2
             bwd.append((d.fourier_transform, _lib.BACKWARD, F, -w_uv / d.norm_factor, _lib.OUTPUT_FIELD))
-        for neg in self._group[1].execute(bwd):
-            L = pointwise.sub_mul(L, neg)
-        return L
+        return pointwise.sub_many(L, self._group[1].execute(bwd))
 
     def _levels_scalar_jones(self, E, conjugate):
         """Jones masks acting on a scalar field E, i.e. on E times the 2 x 2 identity: every
@@ -233,9 +231,7 @@ class MultiScaleCoronagraph(OpticalElement):
             G = pointwise.multiply(F4, J)
             w_uv = complex(np.asarray(d.uv_grid.weights).ravel()[0]) / (2 * np.pi) ** 2
             bwd.append((d.fourier_transform, _lib.BACKWARD, G, -w_uv / d.norm_factor, _lib.OUTPUT_FIELD))
-        for neg in self._group[1].execute(bwd):
-            L = pointwise.sub_mul(L, neg)
-        return L
+        return pointwise.sub_many(L, self._group[1].execute(bwd))
 
     @staticmethod
     def _apply_mask(F, mask, conjugate):

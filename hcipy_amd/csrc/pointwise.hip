@@ -320,6 +320,26 @@ __global__ void phase_ramp_kernel(const C* x, int nx, int64_t n, double zero_x, 
   }
 }
 
+// out = x - sum_j y_j over k <= 16 arrays of n complex values, subtracted in order j = 0, 1, ...
+// (the rounding of k chained x - y passes); out may alias x.
+constexpr int kMaxSubMany = 16;
+struct PtrList {
+  const void* p[kMaxSubMany];
+};
+
+template <typename C, typename R>
+__global__ void sub_many_kernel(const C* x, PtrList ys, int k, C* out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    C a = x[i];
+    for (int j = 0; j < k; ++j) {
+      const C y = static_cast<const C*>(ys.p[j])[i];
+      a.re -= y.re;
+      a.im -= y.im;
+    }
+    out[i] = a;
+  }
+}
+
 constexpr int kThreads = 256;
 constexpr int kPowerBlocks = 256;
 
@@ -582,6 +602,27 @@ int hcipy_pw_phase_ramp(const void* x, int field_dtype, int nx, int ny, double z
     HCIPY_HIP(hipGetLastError());
   }
   return kOk;
+}
+
+int hcipy_pw_sub_many(const void* x, const void* const* ys, int k, void* out, int field_dtype, int64_t n,
+                      void* stream) {
+  HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
+  HCIPY_REQUIRE(k >= 0 && k <= kMaxSubMany, "between 0 and %d subtrahends", kMaxSubMany);
+  HCIPY_REQUIRE(n >= 0, "bad size");
+  if (n == 0) return kOk;
+  HCIPY_REQUIRE(x != nullptr && out != nullptr && (k == 0 || ys != nullptr), "NULL pointer");
+  PtrList l{};
+  for (int j = 0; j < k; ++j) {
+    HCIPY_REQUIRE(ys[j] != nullptr, "NULL subtrahend %d", j);
+    l.p[j] = ys[j];
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 g = grid_for(n, 1);
+  if (field_dtype == HCIPY_DTYPE_C64)
+    hipLaunchKernelGGL((sub_many_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, l, k, (C64*)out, n);
+  else
+    hipLaunchKernelGGL((sub_many_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, l, k, (C128*)out, n);
+  return check_hip(hipGetLastError(), "sub_many_kernel");
 }
 
 }  // extern "C"

@@ -272,6 +272,32 @@ def stokes(E, input_stokes_vector=None, component=-1):
     return Field(out, grid) if grid is not None else out
 
 
+def sub_many(x, ys):
+    """x - ys[0] - ys[1] - ... in one pass (the rounding of the chained subtractions); the
+    operands share x's shape and dtype."""
+    grid = getattr(x, 'grid', None)
+    xt = _plain(x).contiguous()
+    _lib.require_device(xt)
+    out = torch.empty_like(xt)
+    done = 0
+    src = xt
+    ys = [_plain(y).contiguous() for y in ys]
+    for y in ys:
+        if y.shape != xt.shape or y.dtype != xt.dtype:
+            raise ValueError('sub_many operands must match the field in shape and dtype')
+    while True:
+        chunk = ys[done:done + 16]
+        arr = (ctypes.c_void_p * max(1, len(chunk)))(*[_lib.dptr(y) for y in chunk])
+        st = _lib.load().hcipy_pw_sub_many(_lib.dptr(src), arr, len(chunk), _lib.dptr(out), _lib.dtype_code(xt),
+                                           xt.numel(), _lib.stream_ptr(xt.device))
+        _lib.check(st, 'hcipy_pw_sub_many')
+        done += len(chunk)
+        src = out
+        if done >= len(ys):
+            break
+    return Field(out, grid) if grid is not None else out
+
+
 def phase_ramp(E, grid, ax, ay=0.0, out=None):
     """E * exp(i (ax x + ay y)) on the regular grid of E (any leading dimensions), the phase in
     FP64 on the device; ``out`` may be E itself (in place)."""
@@ -349,4 +375,4 @@ def field_dot(J, E, adjoint=False):
     return Field(o, grid) if grid is not None else o
 
 
-__all__ += ['real_sum', 'stokes', 'field_dot', 'phase_ramp']
+__all__ += ['real_sum', 'stokes', 'field_dot', 'phase_ramp', 'sub_many']

@@ -253,6 +253,13 @@ int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, in
 int hcipy_pw_field_dot(const void* J, int j_dtype, int p, int q, int adjoint, const void* E, int field_dtype, int64_t K,
                        void* out, int64_t n, void* stream);
 
+/* out = x - sum_j ys[j] (j = 0 .. k-1, k <= 16, subtracted in that order) over n complex values;
+ * ys is a host array of device pointers and out may equal x.  Adds the level terms of
+ * MultiScaleCoronagraph.forward/backward (hcipy/coronagraphy/multi_scale.py:108-182) onto the
+ * Lyot-plane field in one pass (the level transforms are computed with a negated scale). */
+int hcipy_pw_sub_many(const void* x, const void* const* ys, int k, void* out, int field_dtype, int64_t n,
+                      void* stream);
+
 /* out = x exp(i (ax u + ay v)) for `batch` fields on a regular nx x ny grid with coordinates
  * u = zero_x + ix delta_x, v = zero_y + iy delta_y (x fastest), phase in FP64; out may equal x.
  * Replaces the shift of an FFT spectral-noise realization, C *= exp(-1j * (s_x u + s_y v))
