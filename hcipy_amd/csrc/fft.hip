@@ -189,6 +189,61 @@ __global__ __launch_bounds__((ColCfg<T, NY>::NT)) void columns_kernel(const type
   }
 }
 
+// pass B for k transfer functions applied to the same input: the column FFT once, then per
+// transfer function t: multiply, inverse FFT, crop, write to w2s.p[t] (same layout as w2).
+constexpr int kMaxMultiTf = 8;
+struct TfList {
+  const void* tf[kMaxMultiTf];
+  void* w2[kMaxMultiTf];
+};
+
+template <typename T, int NY>
+__global__ __launch_bounds__((ColCfg<T, NY>::NT)) void columns_multi_kernel(const typename Cplx<T>::V* __restrict__ w1,
+                                                                            TfList l, int k,
+                                                                            const typename Cplx<T>::V* __restrict__ twy,
+                                                                            int NX, int ny, int c0y, int conj_tf) {
+  using V = typename Cplx<T>::V;
+  using Cf = ColCfg<T, NY>;
+  constexpr int C = Cf::C;
+  constexpr int E = Cf::E;
+  constexpr int P = Cf::P;
+  extern __shared__ __align__(16) unsigned char fft_lds_raw[];
+  V* lds = reinterpret_cast<V*>(fft_lds_raw);
+  const V* tw = stage_twiddles<T, NY, Cf::NT>(twy, lds + C * Cf::LINE, Cf::TW_LDS);
+  const int c = threadIdx.x % C;
+  const int tl = threadIdx.x / C;
+  const int kx = blockIdx.x * C + c;
+  const int64_t b = blockIdx.y;
+  const bool col = kx < NX;
+  V v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int yi = tl + q * P - c0y;
+    v[q] = (col && yi >= 0 && yi < ny) ? w1[b * ny * (int64_t)NX + blk(yi, kx, ny, NX)] : V{0, 0};
+  }
+  V* line = lds + c * Cf::LINE;
+  hr_fft<T, NY, P>(v, line, tl, tw, -1);
+  for (int t = 0; t < k; ++t) {
+    const V* tf = static_cast<const V*>(l.tf[t]);
+    V u[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      V tv = col ? tf[blk(tl + q * P, kx, NY, NX)] : V{0, 0};
+      if (conj_tf) tv.y = -tv.y;
+      u[q] = cmul(v[q], tv);
+    }
+    hr_fft<T, NY, P>(u, line, tl, tw, +1);
+    if (col) {
+      V* w2 = static_cast<V*>(l.w2[t]);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int yi = tl + q * P - c0y;
+        if (yi >= 0 && yi < ny) w2[b * ny * (int64_t)NX + blk(yi, kx, ny, NX)] = u[q];
+      }
+    }
+  }
+}
+
 __global__ void twiddle_kernel(int n, int dtype, void* out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -390,6 +445,21 @@ int launch_cols(const FilterPlan& p, const void* w1, void* w2, int conj, int64_t
   return kOk;
 }
 
+template <typename T, int N>
+int launch_cols_multi(const FilterPlan& p, const void* w1, const TfList& l, int k, int conj, int64_t batch,
+                      hipStream_t s) {
+  using V = typename Cplx<T>::V;
+  using Cf = ColCfg<T, N>;
+  auto kern = columns_multi_kernel<T, N>;
+  if (Cf::LDS_BYTES > 65536)
+    HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)Cf::LDS_BYTES));
+  dim3 g((unsigned)ceil_div(p.NX, Cf::C), (unsigned)batch);
+  hipLaunchKernelGGL(kern, g, dim3(Cf::NT), Cf::LDS_BYTES, s, (const V*)w1, l, k, (const V*)p.twy, p.NX, p.ny, p.c0y,
+                     conj);
+  return kOk;
+}
+
 #define HCIPY_POW2_SWITCH(N, CALL) \
   switch (N) {                     \
     case 4: CALL(4); break;        \
@@ -427,6 +497,39 @@ int run_filter(const FilterPlan& p, int adjoint, const void* in, void* out, int6
 #undef ROWS_I
   if (st) return st;
   return check_hip(hipGetLastError(), "fourier filter launch");
+}
+
+// k filters of one input: pass A once into w1, pass B (multi) into the k slices of w2s, pass C
+// per output.
+template <typename T>
+int run_filter_multi(const FilterPlan* const* ps, int k, int adjoint, const void* in, void* const* outs,
+                     int64_t batch, char* ws, hipStream_t s) {
+  const FilterPlan& p = *ps[0];
+  const size_t lbytes = (size_t)round_up((int64_t)((size_t)batch * p.ny * p.NX * cbytes(p.dtype)), 256);
+  void* w1 = ws;
+  TfList l{};
+  for (int t = 0; t < k; ++t) {
+    l.tf[t] = ps[t]->tf;
+    l.w2[t] = ws + (size_t)(t + 1) * lbytes;
+  }
+  int st = kOk;
+#define ROWS_F(n) st = launch_rows_fwd<T, n>(p, in, w1, batch, s)
+  HCIPY_POW2_SWITCH(p.NX, ROWS_F)
+#undef ROWS_F
+  if (st) return st;
+  HCIPY_HIP(hipGetLastError());
+#define COLS(n) st = launch_cols_multi<T, n>(p, w1, l, k, adjoint, batch, s)
+  HCIPY_POW2_SWITCH(p.NY, COLS)
+#undef COLS
+  if (st) return st;
+  HCIPY_HIP(hipGetLastError());
+  for (int t = 0; t < k; ++t) {
+#define ROWS_I(n) st = launch_rows_inv<T, n>(p, l.w2[t], outs[t], batch, s)
+    HCIPY_POW2_SWITCH(p.NX, ROWS_I)
+#undef ROWS_I
+    if (st) return st;
+  }
+  return check_hip(hipGetLastError(), "fourier filter (multi) launch");
 }
 
 }  // namespace
@@ -561,6 +664,56 @@ int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, vo
     const int st = p.dtype == HCIPY_DTYPE_C64
                        ? run_filter<float>(p, adjoint ? 1 : 0, ib, ob, nb, static_cast<char*>(workspace), s)
                        : run_filter<double>(p, adjoint ? 1 : 0, ib, ob, nb, static_cast<char*>(workspace), s);
+    if (st != kOk) return st;
+  }
+  return kOk;
+}
+
+int hcipy_filter_workspace_size_multi(const hcipy_filter_plan* plans, int k, int64_t batch, size_t* bytes) {
+  HCIPY_REQUIRE(plans != nullptr && bytes != nullptr && batch >= 0, "bad argument");
+  HCIPY_REQUIRE(k >= 1 && k <= kMaxMultiTf, "between 1 and %d filters", kMaxMultiTf);
+  HCIPY_REQUIRE(plans[0] != nullptr, "NULL plan");
+  const FilterPlan& p = plans[0]->p;
+  const size_t lbytes = (size_t)batch * p.ny * p.NX * cbytes(p.dtype);
+  *bytes = (size_t)(k + 1) * (size_t)round_up((int64_t)lbytes, 256);
+  return kOk;
+}
+
+int hcipy_filter_execute_multi(const hcipy_filter_plan* plans, int k, int adjoint, const void* in, void* const* outs,
+                               int64_t batch, void* workspace, size_t workspace_bytes, void* stream) {
+  HCIPY_REQUIRE(plans != nullptr && outs != nullptr, "NULL argument");
+  HCIPY_REQUIRE(k >= 1 && k <= kMaxMultiTf, "between 1 and %d filters", kMaxMultiTf);
+  HCIPY_REQUIRE(batch >= 0, "batch out of range");
+  const FilterPlan* ps[kMaxMultiTf];
+  for (int t = 0; t < k; ++t) {
+    HCIPY_REQUIRE(plans[t] != nullptr && outs[t] != nullptr, "NULL plan or output %d", t);
+    ps[t] = &plans[t]->p;
+    const FilterPlan& a = *ps[0];
+    const FilterPlan& b = *ps[t];
+    HCIPY_REQUIRE(a.nx == b.nx && a.ny == b.ny && a.NX == b.NX && a.NY == b.NY && a.dtype == b.dtype &&
+                      a.device == b.device,
+                  "the plans of one multi-filter call must share grids, dtype and device");
+  }
+  if (batch == 0) return kOk;
+  HCIPY_REQUIRE(in != nullptr && workspace != nullptr, "NULL pointer");
+  for (int t = 0; t < k; ++t) HCIPY_REQUIRE(outs[t] != in, "output %d aliases the input", t);
+  size_t need = 0;
+  hcipy_filter_workspace_size_multi(plans, k, std::min<int64_t>(batch, 65535), &need);
+  HCIPY_REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+  int cur = -1;
+  HCIPY_HIP(hipGetDevice(&cur));
+  HCIPY_REQUIRE(cur == ps[0]->device, "plans were created on device %d but device %d is current", ps[0]->device, cur);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const FilterPlan& p = *ps[0];
+  const size_t field_bytes = (size_t)p.nx * p.ny * cbytes(p.dtype);
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const int64_t nb = std::min<int64_t>(65535, batch - b0);
+    const void* ib = static_cast<const char*>(in) + (size_t)b0 * field_bytes;
+    void* ob[kMaxMultiTf];
+    for (int t = 0; t < k; ++t) ob[t] = static_cast<char*>(outs[t]) + (size_t)b0 * field_bytes;
+    const int st = p.dtype == HCIPY_DTYPE_C64
+                       ? run_filter_multi<float>(ps, k, adjoint ? 1 : 0, ib, ob, nb, static_cast<char*>(workspace), s)
+                       : run_filter_multi<double>(ps, k, adjoint ? 1 : 0, ib, ob, nb, static_cast<char*>(workspace), s);
     if (st != kOk) return st;
   }
   return kOk;

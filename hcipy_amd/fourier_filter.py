@@ -82,6 +82,24 @@ class _FilterPlan:
                                               _lib.dptr(ws), ws.numel(), _lib.stream_ptr(x.device))
         _lib.check(st, 'hcipy_filter_execute')
 
+    @staticmethod
+    def execute_multi(plans, adjoint, x, outs, batch, ws_holder):
+        """The filters of several plans (same grids) applied to one input: one row pass over the
+        input and one forward column FFT per column shared by all of them."""
+        k = len(plans)
+        lib = _lib.load()
+        hs = (ctypes.c_void_p * k)(*[pl.handle.value for pl in plans])
+        n = ctypes.c_size_t()
+        _lib.check(lib.hcipy_filter_workspace_size_multi(hs, k, batch, ctypes.byref(n)), 'hcipy_filter_workspace_size_multi')
+        ws = ws_holder.get('ws')
+        if ws is None or ws.numel() < n.value or ws.device != x.device:
+            ws = torch.empty(max(n.value, 256), dtype=torch.uint8, device=x.device)
+            ws_holder['ws'] = ws
+        op = (ctypes.c_void_p * k)(*[_lib.dptr(o) for o in outs])
+        st = lib.hcipy_filter_execute_multi(hs, k, int(adjoint), _lib.dptr(x), op, batch, _lib.dptr(ws), ws.numel(),
+                                            _lib.stream_ptr(x.device))
+        _lib.check(st, 'hcipy_filter_execute_multi')
+
     def __del__(self):
         try:
             if getattr(self, 'handle', None) is not None and _lib._lib is not None:
@@ -125,6 +143,7 @@ class FourierFilter:
         self.shape_in = input_grid.shape
         self.transfer_function = transfer_function
         self._plans = {}
+        self._multi_ws = {}
         self._mft = None
         self._matrix_tf = None
 
@@ -193,6 +212,7 @@ class FourierFilter:
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         out = torch.empty((p, q) + tuple(t.shape), dtype=cdt, device=t.device)
+        plans = []
         for i in range(p):
             for j in range(q):
                 key = (cdt, t.device.index, i, j, False)
@@ -200,7 +220,11 @@ class FourierFilter:
                 if plan is None:
                     plan = _FilterPlan(self, cdt, t.device, tf=tf[i, j])
                     self._plans[key] = plan
-                plan.execute(adjoint, t, out[i, j], batch)
+                plans.append(plan)
+        outs = [out[i, j] for i in range(p) for j in range(q)]
+        # at most 8 transfer functions per shared pass
+        for k0 in range(0, len(plans), 8):
+            _FilterPlan.execute_multi(plans[k0:k0 + 8], adjoint, t, outs[k0:k0 + 8], batch, self._multi_ws)
         return Field(out, self.input_grid)
 
     def _operation(self, field, adjoint):

@@ -382,6 +382,16 @@ int hcipy_filter_workspace_size(hcipy_filter_plan plan, int64_t batch, size_t* b
 int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, void* out, int64_t batch,
                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* k <= 8 filters (plans with the same grids, dtype and device, each with its transfer function)
+ * of the same input: outs[t] = plan t's filter of `in`, sharing the padded row FFTs and the
+ * forward column FFTs.  The element filters of a matrix transfer function applied to one field
+ * (FourierFilter with a matrix transfer function, hcipy/fourier/fourier_operations.py:120-131,
+ * on a field times the identity, as VectorVortexCoronagraph's level 0 sees an unpolarized
+ * wavefront, hcipy/coronagraphy/vortex.py:178-185).  Outputs must not alias the input. */
+int hcipy_filter_workspace_size_multi(const hcipy_filter_plan* plans, int k, int64_t batch, size_t* bytes);
+int hcipy_filter_execute_multi(const hcipy_filter_plan* plans, int k, int adjoint, const void* in, void* const* outs,
+                               int64_t batch, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Zoom FFT: the Fourier transform between two regular Cartesian grids of any spacing, as one
  * Bluestein chirp-z transform per axis.  Replaces ZoomFastFourierTransform

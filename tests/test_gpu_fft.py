@@ -139,3 +139,24 @@ def test_matrix_transfer_function(n, q, dtype, tol):
         u = dev(rng.standard_normal(y.shape) + 1j * rng.standard_normal(y.shape))
         torch.real(torch.sum(torch.conj(u) * y)).backward()
         assert peak_rel_field(host(x.grad), host(filt.backward(ha.Field(u, g)))) < 1e-12
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
+def test_matrix_filter_elements_shared_passes(dtype, tol):
+    """FourierFilter.apply_elements (hcipy_filter_execute_multi: one row pass and one forward
+    column FFT shared by every element filter) against each element filter of the oracle, both
+    directions, with a batch; more than 8 elements run as several shared passes."""
+    g = ha.make_pupil_grid(32, 2)
+    internal = o.make_fft_grid(og(g), 2)
+    rng = np.random.default_rng(9)
+    for p_, q_ in ((2, 2), (3, 3)):
+        tf = rng.standard_normal((p_, q_, internal.size)) + 1j * rng.standard_normal((p_, q_, internal.size))
+        f = (rng.standard_normal((2, g.size)) + 1j * rng.standard_normal((2, g.size))).astype(dtype)
+        filt = ha.FourierFilter(g, dev(tf), 2)
+        for adjoint in (False, True):
+            out = host(filt.apply_elements(ha.Field(dev(f), g), adjoint=adjoint))
+            assert out.shape == (p_, q_, 2, g.size)
+            for i in range(p_):
+                for j in range(q_):
+                    ref = o.fourier_filter(f.astype(np.complex128), og(g), tf[i, j], q=2, adjoint=adjoint)
+                    assert peak_rel_field(out[i, j], ref) < tol, (p_, q_, i, j, adjoint)
