@@ -144,6 +144,10 @@ SIGNATURES = {
                                                           ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_filter_execute_multi': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp,
                                                    ctypes.POINTER(_vp), _c_int64, _vp, ctypes.c_size_t, _vp]),
+    'hcipy_filter_workspace_size_matrix': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int64,
+                                                           ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_filter_execute_matrix': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    _vp, _vp, _c_int64, _vp, ctypes.c_size_t, _vp]),
     'hcipy_zoom_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),

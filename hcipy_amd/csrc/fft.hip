@@ -244,6 +244,87 @@ __global__ __launch_bounds__((ColCfg<T, NY>::NT)) void columns_multi_kernel(cons
   }
 }
 
+// pass B of a matrix transfer function (p x q <= 4 x 4): out_i = IFFT(sum_j TF_ij FFT(in_j)).
+// Inputs are the Q fields j * batch + b of w1, outputs the p fields i * batch + b of w2; the
+// q forward column FFTs are held in registers, so each output costs one multiply-add pass and
+// one inverse FFT.  tf[i * q + j] (conjugated with conj_tf).
+constexpr int kMaxMimo = 4;
+struct TfMatrix {
+  const void* tf[kMaxMimo * kMaxMimo];
+};
+
+template <typename T, int NY, int Q>
+struct MimoCfg {
+  // the Q held transforms take Q * E * 2 * sizeof(T) / 4 VGPRs: 16 points per thread while that
+  // stays within 128, else 8
+  static constexpr int EMAX = Q * sizeof(T) <= 16 ? 16 : 8;
+  static constexpr int E = NY >= EMAX ? EMAX : NY;
+  static constexpr int P = NY / E;
+  static constexpr int LINE = hr_lds_points(NY) + 1;
+  static constexpr int CF = (int)(71680 / ((size_t)LINE * 2 * sizeof(T)));
+  // the Q held transforms take registers: at most 256 threads (512 for one input)
+  static constexpr int CT = (Q == 1 ? 512 : 256) / P > 0 ? (Q == 1 ? 512 : 256) / P : 1;
+  static constexpr int C0 = CF < CT ? CF : CT;
+  static constexpr int C = C0 > 16 ? 16 : (C0 < 1 ? 1 : C0);
+  static constexpr int NT = C * P;
+  static constexpr size_t LDS_BYTES = (size_t)C * LINE * 2 * sizeof(T);
+};
+
+template <typename T, int NY, int Q>
+__global__ __launch_bounds__((MimoCfg<T, NY, Q>::NT)) void columns_mimo_kernel(
+    const typename Cplx<T>::V* __restrict__ w1, typename Cplx<T>::V* __restrict__ w2, TfMatrix m, int p, int64_t batch,
+    const typename Cplx<T>::V* __restrict__ tw, int NX, int ny, int c0y, int conj_tf) {
+  using V = typename Cplx<T>::V;
+  using Cf = MimoCfg<T, NY, Q>;
+  constexpr int C = Cf::C;
+  constexpr int E = Cf::E;
+  constexpr int P = Cf::P;
+  extern __shared__ __align__(16) unsigned char fft_lds_raw[];
+  V* lds = reinterpret_cast<V*>(fft_lds_raw);
+  const int c = threadIdx.x % C;
+  const int tl = threadIdx.x / C;
+  const int kx = blockIdx.x * C + c;
+  const int64_t b = blockIdx.y;
+  const bool col = kx < NX;
+  const int64_t field = (int64_t)ny * NX;
+  V* line = lds + c * Cf::LINE;
+  V F[Q][E];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const V* src = w1 + (j * batch + b) * field;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int yi = tl + q * P - c0y;
+      F[j][q] = (col && yi >= 0 && yi < ny) ? src[blk(yi, kx, ny, NX)] : V{0, 0};
+    }
+    hr_fft<T, NY, P>(F[j], line, tl, tw, -1);
+  }
+  for (int i = 0; i < p; ++i) {
+    V u[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) u[q] = V{0, 0};
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      const V* tf = static_cast<const V*>(m.tf[i * Q + j]);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        V t = col ? tf[blk(tl + q * P, kx, NY, NX)] : V{0, 0};
+        if (conj_tf) t.y = -t.y;
+        u[q] = cadd(u[q], cmul(F[j][q], t));
+      }
+    }
+    hr_fft<T, NY, P>(u, line, tl, tw, +1);
+    if (col) {
+      V* dst = w2 + (i * batch + b) * field;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int yi = tl + q * P - c0y;
+        if (yi >= 0 && yi < ny) dst[blk(yi, kx, ny, NX)] = u[q];
+      }
+    }
+  }
+}
+
 __global__ void twiddle_kernel(int n, int dtype, void* out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -460,6 +541,21 @@ int launch_cols_multi(const FilterPlan& p, const void* w1, const TfList& l, int 
   return kOk;
 }
 
+template <typename T, int N, int Q>
+int launch_cols_mimo(const FilterPlan& p, const void* w1, void* w2, const TfMatrix& m, int n_out, int64_t batch,
+                     int conj, hipStream_t s) {
+  using V = typename Cplx<T>::V;
+  using Cf = MimoCfg<T, N, Q>;
+  auto kern = columns_mimo_kernel<T, N, Q>;
+  if (Cf::LDS_BYTES > 65536)
+    HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)Cf::LDS_BYTES));
+  dim3 g((unsigned)ceil_div(p.NX, Cf::C), (unsigned)batch);
+  hipLaunchKernelGGL(kern, g, dim3(Cf::NT), Cf::LDS_BYTES, s, (const V*)w1, (V*)w2, m, n_out, batch, (const V*)p.twy,
+                     p.NX, p.ny, p.c0y, conj);
+  return kOk;
+}
+
 #define HCIPY_POW2_SWITCH(N, CALL) \
   switch (N) {                     \
     case 4: CALL(4); break;        \
@@ -530,6 +626,57 @@ int run_filter_multi(const FilterPlan* const* ps, int k, int adjoint, const void
     if (st) return st;
   }
   return check_hip(hipGetLastError(), "fourier filter (multi) launch");
+}
+
+// matrix filter: pass A over the n_in input fields of each batch item (inputs [n_in][batch]),
+// pass B (MIMO) into n_out fields, pass C per output field (outputs [n_out][batch]).
+template <typename T, int Q>
+int run_filter_mimo_q(const FilterPlan& p, const TfMatrix& m, int n_out, int conj, const void* in, void* out,
+                      int64_t batch, char* ws, hipStream_t s) {
+  const size_t lbytes = (size_t)round_up((int64_t)((size_t)Q * batch * p.ny * p.NX * cbytes(p.dtype)), 256);
+  void* w1 = ws;
+  void* w2 = ws + lbytes;
+  const size_t fin = (size_t)p.ny * p.nx * cbytes(p.dtype);   // one input/output field
+  const size_t fint = (size_t)p.ny * p.NX * cbytes(p.dtype);  // one intermediate field
+  int st = kOk;
+  // the row passes cover Q * batch (n_out * batch) fields, in grid chunks of 65535
+  for (int64_t f0 = 0; f0 < (int64_t)Q * batch; f0 += 65535) {
+    const int64_t nf = std::min<int64_t>(65535, (int64_t)Q * batch - f0);
+    const void* src = static_cast<const char*>(in) + f0 * fin;
+    void* dst = static_cast<char*>(w1) + f0 * fint;
+#define ROWS_F(n) st = launch_rows_fwd<T, n>(p, src, dst, nf, s)
+    HCIPY_POW2_SWITCH(p.NX, ROWS_F)
+#undef ROWS_F
+    if (st) return st;
+    HCIPY_HIP(hipGetLastError());
+  }
+#define COLS(n) st = launch_cols_mimo<T, n, Q>(p, w1, w2, m, n_out, batch, conj, s)
+  HCIPY_POW2_SWITCH(p.NY, COLS)
+#undef COLS
+  if (st) return st;
+  HCIPY_HIP(hipGetLastError());
+  for (int64_t f0 = 0; f0 < (int64_t)n_out * batch; f0 += 65535) {
+    const int64_t nf = std::min<int64_t>(65535, (int64_t)n_out * batch - f0);
+    const void* src = static_cast<const char*>(w2) + f0 * fint;
+    void* dst = static_cast<char*>(out) + f0 * fin;
+#define ROWS_I(n) st = launch_rows_inv<T, n>(p, src, dst, nf, s)
+    HCIPY_POW2_SWITCH(p.NX, ROWS_I)
+#undef ROWS_I
+    if (st) return st;
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+
+template <typename T>
+int run_filter_mimo(const FilterPlan& p, const TfMatrix& m, int n_out, int n_in, int conj, const void* in, void* out,
+                    int64_t batch, char* ws, hipStream_t s) {
+  switch (n_in) {
+    case 1: return run_filter_mimo_q<T, 1>(p, m, n_out, conj, in, out, batch, ws, s);
+    case 2: return run_filter_mimo_q<T, 2>(p, m, n_out, conj, in, out, batch, ws, s);
+    case 3: return run_filter_mimo_q<T, 3>(p, m, n_out, conj, in, out, batch, ws, s);
+    default: return run_filter_mimo_q<T, 4>(p, m, n_out, conj, in, out, batch, ws, s);
+  }
 }
 
 }  // namespace
@@ -717,6 +864,51 @@ int hcipy_filter_execute_multi(const hcipy_filter_plan* plans, int k, int adjoin
     if (st != kOk) return st;
   }
   return kOk;
+}
+
+int hcipy_filter_workspace_size_matrix(hcipy_filter_plan plan, int n_out, int n_in, int64_t batch, size_t* bytes) {
+  HCIPY_REQUIRE(plan != nullptr && bytes != nullptr && batch >= 0, "bad argument");
+  HCIPY_REQUIRE(n_out >= 1 && n_out <= kMaxMimo && n_in >= 1 && n_in <= kMaxMimo, "matrix of at most %d x %d",
+                kMaxMimo, kMaxMimo);
+  const FilterPlan& p = plan->p;
+  const int64_t b = std::min<int64_t>(batch, 65535);
+  const size_t fb = (size_t)p.ny * p.NX * cbytes(p.dtype);
+  *bytes = (size_t)round_up((int64_t)((size_t)n_in * b * fb), 256) + (size_t)round_up((int64_t)((size_t)n_out * b * fb), 256);
+  return kOk;
+}
+
+int hcipy_filter_execute_matrix(const hcipy_filter_plan* plans, int n_out, int n_in, int conj, const void* in, void* out,
+                                int64_t batch, void* workspace, size_t workspace_bytes, void* stream) {
+  HCIPY_REQUIRE(plans != nullptr, "NULL plans");
+  HCIPY_REQUIRE(n_out >= 1 && n_out <= kMaxMimo && n_in >= 1 && n_in <= kMaxMimo, "matrix of at most %d x %d",
+                kMaxMimo, kMaxMimo);
+  HCIPY_REQUIRE(batch >= 0, "batch out of range");
+  TfMatrix m{};
+  for (int t = 0; t < n_out * n_in; ++t) {
+    HCIPY_REQUIRE(plans[t] != nullptr, "NULL plan %d", t);
+    const FilterPlan& a = plans[0]->p;
+    const FilterPlan& b = plans[t]->p;
+    HCIPY_REQUIRE(a.nx == b.nx && a.ny == b.ny && a.NX == b.NX && a.NY == b.NY && a.dtype == b.dtype &&
+                      a.device == b.device,
+                  "the plans of one matrix filter must share grids, dtype and device");
+    m.tf[t] = b.tf;
+  }
+  if (batch == 0) return kOk;
+  HCIPY_REQUIRE(in != nullptr && out != nullptr && workspace != nullptr && in != out, "NULL or aliased pointer");
+  size_t need = 0;
+  hcipy_filter_workspace_size_matrix(plans[0], n_out, n_in, batch, &need);
+  HCIPY_REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+  int cur = -1;
+  HCIPY_HIP(hipGetDevice(&cur));
+  const FilterPlan& p = plans[0]->p;
+  HCIPY_REQUIRE(cur == p.device, "plans were created on device %d but device %d is current", p.device, cur);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // the column pass takes one batch item per grid row; inputs are [n_in][batch] and outputs
+  // [n_out][batch], so a larger batch is split by the caller (a chunk is a strided subset)
+  HCIPY_REQUIRE(batch <= 65535, "matrix filters take at most 65535 fields per component per call");
+  return p.dtype == HCIPY_DTYPE_C64
+             ? run_filter_mimo<float>(p, m, n_out, n_in, conj ? 1 : 0, in, out, batch, static_cast<char*>(workspace), s)
+             : run_filter_mimo<double>(p, m, n_out, n_in, conj ? 1 : 0, in, out, batch, static_cast<char*>(workspace), s);
 }
 
 }  // extern "C"

@@ -100,6 +100,23 @@ class _FilterPlan:
                                             _lib.stream_ptr(x.device))
         _lib.check(st, 'hcipy_filter_execute_multi')
 
+    @staticmethod
+    def execute_matrix(plans, n_out, n_in, conj, x, out, batch, ws_holder):
+        """Matrix filter: ``x`` is [n_in][batch] fields, ``out`` [n_out][batch];
+        plans[o * n_in + c] filters input c into output o, with conj(tf) if ``conj``."""
+        lib = _lib.load()
+        hs = (ctypes.c_void_p * len(plans))(*[pl.handle.value for pl in plans])
+        n = ctypes.c_size_t()
+        _lib.check(lib.hcipy_filter_workspace_size_matrix(hs[0], n_out, n_in, batch, ctypes.byref(n)),
+                   'hcipy_filter_workspace_size_matrix')
+        ws = ws_holder.get('ws')
+        if ws is None or ws.numel() < n.value or ws.device != x.device:
+            ws = torch.empty(max(n.value, 256), dtype=torch.uint8, device=x.device)
+            ws_holder['ws'] = ws
+        st = lib.hcipy_filter_execute_matrix(hs, n_out, n_in, int(conj), _lib.dptr(x), _lib.dptr(out), batch,
+                                             _lib.dptr(ws), ws.numel(), _lib.stream_ptr(x.device))
+        _lib.check(st, 'hcipy_filter_execute_matrix')
+
     def __del__(self):
         try:
             if getattr(self, 'handle', None) is not None and _lib._lib is not None:
@@ -130,8 +147,9 @@ class FourierFilter:
     of shape (p, q, internal size) is a matrix field (fourier_operations.py:120-131): it
     contracts with the field's first dimension (q components, as field_dot does; any further
     dimensions are carried along) and the output has p there, ``out_i = sum_j filter(tf_ij)(in_j)``
-    (the adjoint uses the conjugate transpose).  Each (i, j) runs as a scalar filter; the terms
-    are added by fused subtract passes of filters with the negated transfer function.
+    (the adjoint uses the conjugate transpose).  By linearity this is
+    ``IFFT(sum_j tf_ij FFT(in_j))``: each input is transformed once and each output
+    inverse-transformed once (``hcipy_filter_execute_matrix``).
     """
 
     def __init__(self, input_grid, transfer_function, q=1):
@@ -167,8 +185,20 @@ class FourierFilter:
             self._matrix_tf = t if t.dim() == 3 else False
         return self._matrix_tf is not False
 
+    def _element_plan(self, cdt, device, i, j):
+        key = (cdt, device.index, i, j, False)
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = _FilterPlan(self, cdt, device, tf=self._matrix_tf[i, j])
+            self._plans[key] = plan
+        return plan
+
     def _run_matrix(self, t, adjoint):
-        """Matrix transfer function: out_i = sum_j F_ij(in_j), adjoint in_j = sum_i F_ij^H(out_i)."""
+        """Matrix transfer function: out_i = sum_j F_ij(in_j), adjoint in_j = sum_i F_ij^H(out_i).
+
+        Up to 4 x 4 runs as one matrix filter (each input transformed once, each output
+        inverse-transformed once); larger matrices run in 4 x 4 blocks whose partial sums are
+        added."""
         tf = self._matrix_tf
         p, q = int(tf.shape[0]), int(tf.shape[1])
         n_in, n_out = (p, q) if adjoint else (q, p)
@@ -176,25 +206,34 @@ class FourierFilter:
             raise ValueError('a matrix transfer function of shape (%d, %d) needs %d field components, got shape %s'
                              % (p, q, n_in, tuple(t.shape)))
         cdt, dev_ = t.dtype, t.device
-        outs = []
-        for o in range(n_out):
+        lead = tuple(t.shape[1:-1])
+        batch = int(np.prod(lead)) if lead else 1
+        x = t.reshape(n_in, batch, t.shape[-1])
+        out = torch.empty((n_out, batch, t.shape[-1]), dtype=cdt, device=dev_)
+        B = 4
+        for o0 in range(0, n_out, B):
+            no = min(B, n_out - o0)
             acc = None
-            for c in range(n_in):
-                i, j = (c, o) if adjoint else (o, c)
-                neg = acc is not None
-                key = (cdt, dev_.index, i, j, neg)
-                plan = self._plans.get(key)
-                if plan is None:
-                    plan = _FilterPlan(self, cdt, dev_, tf=(-tf[i, j] if neg else tf[i, j]))
-                    self._plans[key] = plan
-                xin = t[c].contiguous()
-                lead = tuple(xin.shape[:-1])
-                batch = int(np.prod(lead)) if lead else 1
-                term = torch.empty_like(xin)
-                plan.execute(adjoint, xin, term, batch)
-                acc = term if acc is None else pointwise._sub_mul_raw(acc, term, None, 0, False)
-            outs.append(acc)
-        return torch.stack(outs, dim=0)
+            for c0 in range(0, n_in, B):
+                nc = min(B, n_in - c0)
+                # plan of (output o, input c): element (o, c) of the matrix, or (c, o) for the adjoint
+                plans = [self._element_plan(cdt, dev_, *((c, o) if adjoint else (o, c)))
+                         for o in range(o0, o0 + no) for c in range(c0, c0 + nc)]
+                dst = out[o0:o0 + no] if acc is None else torch.empty((no, batch, t.shape[-1]), dtype=cdt, device=dev_)
+                for b0 in range(0, batch, 65535):
+                    nb = min(65535, batch - b0)
+                    xs = x[c0:c0 + nc, b0:b0 + nb]
+                    xs = xs if (nb == batch and xs.is_contiguous()) else xs.contiguous()
+                    whole = nb == batch
+                    ds = dst[:, b0:b0 + nb] if whole else torch.empty((no, nb, t.shape[-1]), dtype=cdt, device=dev_)
+                    _FilterPlan.execute_matrix(plans, no, nc, adjoint, xs, ds, nb, self._multi_ws)
+                    if not whole:
+                        dst[:, b0:b0 + nb] = ds
+                if acc is None:
+                    acc = dst
+                else:
+                    acc.add_(dst)
+        return out.reshape((n_out,) + lead + (t.shape[-1],))
 
     def apply_elements(self, field, adjoint=False):
         """For a matrix transfer function (p, q): every element filter applied to the same scalar
@@ -212,15 +251,7 @@ class FourierFilter:
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
         out = torch.empty((p, q) + tuple(t.shape), dtype=cdt, device=t.device)
-        plans = []
-        for i in range(p):
-            for j in range(q):
-                key = (cdt, t.device.index, i, j, False)
-                plan = self._plans.get(key)
-                if plan is None:
-                    plan = _FilterPlan(self, cdt, t.device, tf=tf[i, j])
-                    self._plans[key] = plan
-                plans.append(plan)
+        plans = [self._element_plan(cdt, t.device, i, j) for i in range(p) for j in range(q)]
         outs = [out[i, j] for i in range(p) for j in range(q)]
         # at most 8 transfer functions per shared pass
         for k0 in range(0, len(plans), 8):

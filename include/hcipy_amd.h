@@ -392,6 +392,16 @@ int hcipy_filter_workspace_size_multi(const hcipy_filter_plan* plans, int k, int
 int hcipy_filter_execute_multi(const hcipy_filter_plan* plans, int k, int adjoint, const void* in, void* const* outs,
                                int64_t batch, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Matrix transfer function (n_out x n_in <= 4 x 4 plans with the same grids, dtype and device,
+ * plans[i * n_in + j] holding element (i, j)): out[i][b] = sum_j filter_ij(in[j][b]), with
+ * conj(tf) when conj != 0 (the adjoint passes the transposed plan matrix).  in is
+ * [n_in][batch][ny][nx], out [n_out][batch][ny][nx], batch <= 65535.  Each input is
+ * transformed once and each output inverse-transformed once.  Replaces FourierFilter with a
+ * matrix transfer function, field_dot per frequency (hcipy/fourier/fourier_operations.py:120-131). */
+int hcipy_filter_workspace_size_matrix(hcipy_filter_plan plan, int n_out, int n_in, int64_t batch, size_t* bytes);
+int hcipy_filter_execute_matrix(const hcipy_filter_plan* plans, int n_out, int n_in, int conj, const void* in, void* out,
+                                int64_t batch, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Zoom FFT: the Fourier transform between two regular Cartesian grids of any spacing, as one
  * Bluestein chirp-z transform per axis.  Replaces ZoomFastFourierTransform

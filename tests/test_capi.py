@@ -82,3 +82,16 @@ def test_zoom_entry_points_reject_bad_arguments():
     assert lib.hcipy_zoom_workspace_size(None, 1, ctypes.byref(size)) == 1
     assert lib.hcipy_zoom_execute(None, 0, None, None, 1, 1.0, 0.0, None, 0, None) == 1
     assert lib.hcipy_zoom_plan_destroy(None) == 0
+
+
+def test_matrix_filter_entry_points_reject_bad_arguments():
+    """Argument checks of the matrix filter entry points run before any device work."""
+    lib = _lib.load()
+    size = ctypes.c_size_t()
+    hs = (ctypes.c_void_p * 25)()
+    assert lib.hcipy_filter_workspace_size_matrix(None, 2, 2, 1, ctypes.byref(size)) == 1
+    assert lib.hcipy_filter_execute_matrix(None, 2, 2, 0, None, None, 1, None, 0, None) == 1
+    assert lib.hcipy_filter_execute_matrix(hs, 5, 2, 0, None, None, 1, None, 0, None) == 1
+    assert b'at most 4 x 4' in lib.hcipy_get_last_error()
+    assert lib.hcipy_filter_execute_matrix(hs, 1, 1, 0, None, None, 1, None, 0, None) == 1
+    assert b'NULL plan 0' in lib.hcipy_get_last_error()

@@ -142,6 +142,60 @@ def test_matrix_transfer_function(n, q, dtype, tol):
 
 
 @pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
+@pytest.mark.parametrize('shape', [(1, 4), (4, 1), (4, 4), (5, 3), (6, 6)])
+def test_matrix_transfer_function_shapes(shape, dtype, tol):
+    """Matrix transfer functions from 1 x 4 to 6 x 6 against the oracle, both directions, with a
+    batch dimension: up to 4 x 4 is one hcipy_filter_execute_matrix call (each input transformed
+    once), larger ones run in 4 x 4 blocks whose partial sums are added."""
+    p_, q_ = shape
+    g = ha.make_pupil_grid(32, 2)
+    internal = o.make_fft_grid(og(g), 2)
+    rng = np.random.default_rng(10 + p_ * 7 + q_)
+    tf = rng.standard_normal((p_, q_, internal.size)) + 1j * rng.standard_normal((p_, q_, internal.size))
+    f = (rng.standard_normal((q_, 3, g.size)) + 1j * rng.standard_normal((q_, 3, g.size))).astype(dtype)
+    filt = ha.FourierFilter(g, dev(tf), 2)
+    out = host(filt.forward(ha.Field(dev(f), g)))
+    ref = o.fourier_filter(f.astype(np.complex128), og(g), tf, 2)
+    assert out.shape == (p_, 3, g.size)
+    e = peak_rel_field(out, ref)
+    u = (rng.standard_normal((p_, 3, g.size)) + 1j * rng.standard_normal((p_, 3, g.size))).astype(dtype)
+    back = host(filt.backward(ha.Field(dev(u), g)))
+    refb = o.fourier_filter(u.astype(np.complex128), og(g), tf, 2, adjoint=True)
+    assert back.shape == (q_, 3, g.size)
+    eb = peak_rel_field(back, refb)
+    say('matrix filter %dx%d %s fwd %.2e adj %.2e' % (p_, q_, dtype.__name__, e, eb))
+    assert e < tol and eb < tol
+
+
+def test_matrix_filter_capi_vs_element_filters():
+    """hcipy_filter_execute_matrix through the C-ABI (2 x 2, 1024 x 1024 on the 2048 x 2048
+    grid, a batch of 2) against the sums of the scalar element filters of the same plans, and
+    with conj against the scalar adjoints."""
+    from hcipy_amd.fourier_filter import _FilterPlan
+    g = ha.make_pupil_grid(1024, 2)
+    internal = ha.make_fft_grid(g, 2, 1)
+    rng = np.random.default_rng(12)
+    tf = np.exp(2j * np.pi * rng.random((2, 2, internal.size)))
+    filt = ha.FourierFilter(g, dev(tf), 2)
+    filt._is_matrix(dev(np.zeros(1)).device)
+    x = dev(rng.standard_normal((2, 2, g.size)) + 1j * rng.standard_normal((2, 2, g.size)))
+    plans = [filt._element_plan(torch.complex128, x.device, i, j) for i in range(2) for j in range(2)]
+    for conj in (0, 1):
+        out = torch.empty_like(x)
+        _FilterPlan.execute_matrix(plans, 2, 2, conj, x, out, 2, {})
+        for i in range(2):
+            acc = torch.zeros_like(x[0])
+            for j in range(2):
+                # element (i, j) forward, or for conj the adjoint of element (i, j)
+                t = torch.empty_like(x[0])
+                plans[i * 2 + j].execute(conj, x[j].contiguous(), t, 2)
+                acc += t
+            e = peak_rel_field(host(out[i]), host(acc))
+            say('matrix C-ABI conj=%d output %d: %.2e' % (conj, i, e))
+            assert e < 1e-12
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
 def test_matrix_filter_elements_shared_passes(dtype, tol):
     """FourierFilter.apply_elements (hcipy_filter_execute_multi: one row pass and one forward
     column FFT shared by every element filter) against each element filter of the oracle, both
