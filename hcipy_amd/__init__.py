@@ -11,13 +11,16 @@ from .field import (CartesianGrid, Field, RegularCoords, SeparatedCoords, Unstru
 from .fourier import (ComputationalComplexity, FastFourierTransform, FourierTransform, MatrixFourierTransform, MftGroup,
                       ZoomFastFourierTransform,
                       get_fft_parameters, is_fft_grid, make_fft_grid, make_fourier_transform)
-from .optics import AgnosticOpticalElement, Apodizer, OpticalElement, OpticalSystem, PhaseApodizer, Wavefront
+from .optics import (AgnosticOpticalElement, Apodizer, ComplexSurfaceApodizer, OpticalElement, OpticalSystem, PhaseApodizer,
+                     SurfaceApodizer, Wavefront)
 from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, FresnelPropagator
 from .fourier_filter import FourierFilter, NearFieldTransferFunction
 from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VectorVortexCoronagraph, VortexCoronagraph
 from .polarization import CircularRetarder, JonesMatrixOpticalElement, LinearRetarder, PhaseRetarder
 from .spectral_noise import SpectralNoiseFFT, SpectralNoiseFactoryFFT, make_power_law_error
-from .aperture import make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider
+from .aperture import (make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider,
+                       make_super_gaussian_aperture)
+from .aberration import SurfaceAberration, SurfaceAberrationAtDistance, make_high_pass_power_law_error
 from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, XineticsInfluenceFunctions,
                                 make_xinetics_influence_functions, make_actuator_positions,
                                 make_gaussian_influence_functions)

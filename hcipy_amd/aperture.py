@@ -1,4 +1,4 @@
-"""Aperture field generators (hcipy/aperture/generic.py:11-46, 301-348, 394-429).
+"""Aperture field generators (hcipy/aperture/generic.py:11-46, 301-348, 394-429, 870-895).
 
 Generators are evaluated on the host in FP64 (set-up work) and return Fields on the default
 device.
@@ -7,7 +7,8 @@ import numpy as np
 
 from .field import Field
 
-__all__ = ['make_circular_aperture', 'make_rectangular_aperture', 'make_spider', 'make_obstructed_circular_aperture']
+__all__ = ['make_circular_aperture', 'make_rectangular_aperture', 'make_spider', 'make_obstructed_circular_aperture',
+           'make_super_gaussian_aperture']
 
 
 def _xy(grid):
@@ -71,4 +72,17 @@ def make_obstructed_circular_aperture(pupil_diameter, central_obscuration_ratio,
         for angle in np.linspace(0, 2 * np.pi, num_spiders, endpoint=False):
             spiders = spiders * _spider_mask(grid, (0, 0), (pupil_diameter * np.cos(angle), pupil_diameter * np.sin(angle)), spider_width)
         return Field((outer - inner) * spiders, grid)
+    return func
+
+
+def make_super_gaussian_aperture(full_width_half_maximum, shape_parameter=5, center=None):
+    """exp(-(r / sigma)^(2 p) / 2) with sigma = fwhm / 2 / (2 ln 2)^(1 / (2 p)), so that the value
+    is 1/2 at r = fwhm / 2; p = 1 is a Gaussian (generic.py:870-895)."""
+    shift = np.zeros(2) if center is None else center * np.ones(2)
+
+    def func(grid):
+        x, y = _xy(grid)
+        sigma = 0.5 * full_width_half_maximum / (2 * np.log(2)) ** (1 / (2 * shape_parameter))
+        r = np.hypot(x - shift[0], y - shift[1]) / sigma
+        return Field(np.exp(-0.5 * np.abs(r) ** (2 * shape_parameter)).ravel(), grid)
     return func

@@ -14,7 +14,8 @@ import torch
 from . import pointwise
 from .field import Field, field_dtype
 
-__all__ = ['Wavefront', 'OpticalElement', 'AgnosticOpticalElement', 'OpticalSystem', 'Apodizer', 'PhaseApodizer']
+__all__ = ['Wavefront', 'OpticalElement', 'AgnosticOpticalElement', 'OpticalSystem', 'Apodizer', 'PhaseApodizer',
+           'SurfaceApodizer', 'ComplexSurfaceApodizer']
 
 
 def _tensor(f):
@@ -274,6 +275,58 @@ class Apodizer(OpticalElement):
     def backward(self, wavefront):
         a = self._get_apodization(wavefront.wavelength)
         return Wavefront(pointwise.multiply(wavefront.electric_field, a, conjugate=True), wavefront.wavelength, wavefront.input_stokes_vector)
+
+
+class SurfaceApodizer(OpticalElement):
+    """Thin sagged plate: E exp(i (n - 1) k sag) forward, E exp(-i (n - 1) k sag) backward, with
+    the refractive index n a scalar or a function of wavelength (apodization.py:64-102).  The
+    phase is formed in FP64 by the phase kernel."""
+
+    def __init__(self, surface_sag, refractive_index):
+        self.surface_sag = surface_sag
+        self.refractive_index = refractive_index
+
+    def _get_refractive_index(self, wavelength):
+        return self.refractive_index(wavelength) if callable(self.refractive_index) else self.refractive_index
+
+    def _scale(self, wavefront):
+        return float((self._get_refractive_index(wavefront.wavelength) - 1) * wavefront.wavenumber)
+
+    def forward(self, wavefront):
+        E = pointwise.multiply_phase(wavefront.electric_field, self.surface_sag, self._scale(wavefront))
+        return Wavefront(E, wavefront.wavelength, wavefront.input_stokes_vector)
+
+    def backward(self, wavefront):
+        E = pointwise.multiply_phase(wavefront.electric_field, self.surface_sag, -self._scale(wavefront))
+        return Wavefront(E, wavefront.wavelength, wavefront.input_stokes_vector)
+
+
+class ComplexSurfaceApodizer(SurfaceApodizer):
+    """Sagged plate with an amplitude coating: E a exp(i (n - 1) k surface) forward, E conj(a)
+    exp(-i (n - 1) k surface) backward; a may be a function of wavelength
+    (apodization.py:104-150)."""
+
+    def __init__(self, amplitude, surface, refractive_index):
+        SurfaceApodizer.__init__(self, surface, refractive_index)
+        self.amplitude = amplitude
+
+    @property
+    def surface(self):
+        return self.surface_sag
+
+    def _get_amplitude(self, wavelength):
+        a = self.amplitude
+        return a(wavelength) if callable(a) and not isinstance(a, torch.Tensor) else a
+
+    def forward(self, wavefront):
+        E = pointwise.multiply(wavefront.electric_field, self._get_amplitude(wavefront.wavelength))
+        E = pointwise.multiply_phase(E, self.surface_sag, self._scale(wavefront))
+        return Wavefront(E, wavefront.wavelength, wavefront.input_stokes_vector)
+
+    def backward(self, wavefront):
+        E = pointwise.multiply(wavefront.electric_field, self._get_amplitude(wavefront.wavelength), conjugate=True)
+        E = pointwise.multiply_phase(E, self.surface_sag, -self._scale(wavefront))
+        return Wavefront(E, wavefront.wavelength, wavefront.input_stokes_vector)
 
 
 class PhaseApodizer(OpticalElement):

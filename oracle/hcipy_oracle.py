@@ -837,6 +837,30 @@ def power_law_error(pupil_grid, ptv, diameter, exponent=-2.5, aperture=None, see
     return screen * aperture
 
 
+def super_gaussian_aperture(grid, fwhm, shape_parameter=5, center=(0, 0)):
+    """make_super_gaussian_aperture (hcipy/aperture/generic.py:870-895): exp(-(r/sigma)^(2p)/2),
+    sigma = fwhm / 2 / (2 ln 2)^(1/(2p))."""
+    x = grid.sep[0][np.newaxis, :] - center[0]
+    y = grid.sep[1][:, np.newaxis] - center[1]
+    sigma = 0.5 * fwhm / (2 * np.log(2)) ** (1 / (2 * shape_parameter))
+    r = np.hypot(x, y) / sigma
+    return np.exp(-0.5 * r ** (2 * shape_parameter)).ravel()
+
+
+def high_pass_power_law_error(pupil_grid, ptv, diameter, cutoff_frequency, exponent=-2.5, aperture=None,
+                              filter_shape_parameter=15, seed=None):
+    """make_high_pass_power_law_error (hcipy/optics/aberration.py:56-92): the power-law error
+    filtered by 1 - super_gaussian(2 cutoff) with FourierFilter(q=2), real part, times the mask
+    aperture > 0."""
+    internal = make_fft_grid(pupil_grid, 2)
+    tf = 1 - super_gaussian_aperture(internal, 2 * cutoff_frequency, filter_shape_parameter)
+    if aperture is None:
+        aperture = circular_aperture(pupil_grid, diameter)
+    mask = (np.asarray(aperture) > 0).astype(float)
+    screen = power_law_error(pupil_grid, ptv, diameter, exponent, aperture=None, seed=seed)
+    return fourier_filter(screen.astype(complex), pupil_grid, tf, q=2).real * mask
+
+
 def actuator_positions(n_act, pitch):
     """make_actuator_positions without tilts (deformable_mirror.py:11-41)."""
     return make_uniform_grid(n_act, [pitch * n_act, pitch * n_act])

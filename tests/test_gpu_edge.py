@@ -36,7 +36,7 @@ def test_empty_batch(precision, dtype):
 
 def test_batch_beyond_one_launch():
     """70 000 fields: more than the 65 535 slices of one launch's grid.y / grid.z, so the stage-2,
-    filter, pointwise and deformable-mirror launches run in chunks."""
+    filter, matrix-filter, pointwise and deformable-mirror launches run in chunks."""
     pupil = ha.make_pupil_grid(8)
     focal = ha.make_focal_grid(2, 2)
     n = 70000
@@ -52,6 +52,17 @@ def test_batch_beyond_one_launch():
     tf = _rand(o.make_fft_grid(og(pupil), 2).size, 2)
     ref = o.fourier_filter(x[sel], og(pupil), tf, q=2)
     assert peak_rel_field(f[sel], ref) < 1e-12
+    # a 2 x 2 matrix transfer function over the same 70 000 fields per component: the batch is
+    # split into calls of 65 535 and the row passes of each call into launches of 65 535 fields
+    xm = np.stack([x, _rand((n, pupil.size), 5)])
+    tfm = _rand((2, 2, o.make_fft_grid(og(pupil), 2).size), 6)
+    filt_m = ha.FourierFilter(pupil, dev(tfm), 2)
+    fm = host(filt_m.forward(ha.Field(dev(xm), pupil)))
+    refm = o.fourier_filter(xm[:, sel], og(pupil), tfm, q=2)
+    assert peak_rel_field(fm[:, sel], refm) < 1e-12
+    fb = host(filt_m.backward(ha.Field(dev(xm), pupil)))
+    refb = o.fourier_filter(xm[:, sel], og(pupil), tfm, q=2, adjoint=True)
+    assert peak_rel_field(fb[:, sel], refb) < 1e-12
     # pointwise kernels: broadcast and per-field operands, intensity, FP64 power
     a = _rand(pupil.size, 3)
     m = host(pointwise.multiply(ha.Field(dev(x), pupil), dev(a)))

@@ -654,3 +654,64 @@ def test_czt_matches_scipy(dtype):
         ref = scipy.signal.CZT(n, m, w, a)(x.astype(complex))
         rtol = 1e-13 if dtype == 'complex128' else 1e-4
         assert np.allclose(y, ref, rtol=rtol)
+
+
+# --- super-Gaussian aperture and the high-pass power-law error --------------------------------
+
+@pytest.mark.parametrize('path', sorted(glob.glob(os.path.join(GOLDEN, 'apertures', 'super_gaussian', '*.fits.gz'))))
+def test_super_gaussian_golden(path):
+    """The reference's rasters (tests/test_aperture.py:99-114): 256^2 over the unit square,
+    supersampled 8x."""
+    name = os.path.basename(path)
+    fwhm = 0.5 if '_small_' in name else 1.0
+    p = 1.0 if 'smallshape' in name else 10.0
+    center = (0.25, -0.3) if 'shifted' in name else (0.0, 0.0)
+    g = o.make_uniform_grid(256, [1, 1])
+    f = o.evaluate_supersampled(lambda gg: o.super_gaussian_aperture(gg, fwhm, p, center), g, 8)
+    assert np.allclose(f, o.read_fits_image(path).ravel())
+
+
+def test_super_gaussian_half_maximum_and_gaussian():
+    """1/2 at r = fwhm/2 for any shape parameter; p = 1 is the Gaussian of that FWHM,
+    sigma = fwhm / (2 sqrt(2 ln 2))."""
+    g = o.Grid(separated=[np.array([-0.3, 0.0, 0.35]), np.array([0.0])])
+    for p in (0.5, 1.0, 3.0, 15.0):
+        f = o.super_gaussian_aperture(g, 0.7, p)
+        # r = fwhm/2 gives 1/2, r = 0 gives 1, and in between 2^-((r / (fwhm/2))^(2p))
+        assert abs(f[2] - 0.5) < 1e-15
+        assert abs(f[1] - 1) < 1e-15
+        assert abs(f[0] - 2 ** -((0.3 / 0.35) ** (2 * p))) < 1e-14
+    gg = o.make_pupil_grid(32, 2.0)
+    x, y = gg.sep[0][np.newaxis, :], gg.sep[1][:, np.newaxis]
+    sigma = 0.8 / (2 * np.sqrt(2 * np.log(2)))
+    assert np.allclose(o.super_gaussian_aperture(gg, 0.8, 1.0), np.exp(-(x ** 2 + y ** 2) / (2 * sigma ** 2)).ravel())
+
+
+def test_high_pass_power_law_error_reference_spectrum():
+    """The reference's check (tests/test_optics.py:823-852): inside the cutoff circle the power
+    spectrum of the filtered screen is flat to 1e-6, with and without an aperture."""
+    D = 1.0
+    g = o.make_pupil_grid(64, 2.0)
+    cutoff = 2 * np.pi * 10 / D
+    fft = o.FFT(g)
+    fmask = o.circular_aperture(fft.output_grid, 2 * cutoff)
+    ap = o.evaluate_supersampled(lambda gg: o.circular_aperture(gg, D), g, 4)
+    for aperture in (None, ap):
+        screen = o.high_pass_power_law_error(g, 0.1, 2 * D, cutoff, aperture=aperture, seed=3)
+        fk = np.abs(fft.forward(screen.astype(complex))) ** 2
+        assert np.ptp(fk[fmask > 0]) < 1e-6
+
+
+def test_high_pass_power_law_error_dc_only_filter():
+    """With a cutoff far below one frequency sample, 1 - super-Gaussian removes only the DC
+    sample of the 2x padded spectrum: the result is the screen minus its sum over the padded
+    grid, (screen - sum / (2n)^2), times the mask.  Pins the filter's centring and the q = 2
+    padding."""
+    n = 32
+    g = o.make_pupil_grid(n, 2.0)
+    internal = o.make_fft_grid(g, 2)
+    cutoff = 0.05 * internal.delta[0]
+    got = o.high_pass_power_law_error(g, 0.1, 1.5, cutoff, seed=4)
+    screen = o.power_law_error(g, 0.1, 1.5, seed=4)
+    mask = o.circular_aperture(g, 1.5) > 0
+    assert np.abs(got - (screen - screen.sum() / (2 * n) ** 2) * mask).max() < 1e-15
