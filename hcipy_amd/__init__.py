@@ -17,7 +17,8 @@ from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, Fresne
 from .fourier_filter import FourierFilter, NearFieldTransferFunction
 from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VectorVortexCoronagraph, VortexCoronagraph
 from .polarization import CircularRetarder, JonesMatrixOpticalElement, LinearRetarder, PhaseRetarder
-from .spectral_noise import SpectralNoiseFFT, SpectralNoiseFactoryFFT, make_power_law_error
+from .spectral_noise import (SpectralNoiseFFT, SpectralNoiseFactoryFFT, SpectralNoiseFactoryMultiscale,
+                             SpectralNoiseMultiscale, make_power_law_error)
 from .aperture import (make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider,
                        make_super_gaussian_aperture)
 from .aberration import SurfaceAberration, SurfaceAberrationAtDistance, make_high_pass_power_law_error

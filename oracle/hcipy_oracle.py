@@ -39,6 +39,10 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
   integral of the PSD over the FFT grid (sum psd w / (2 pi)^2: the (2 pi)^ndim and weight
   factors), a one-pixel shift is np.roll of the screen by one pixel along that axis, two half
   shifts compose, and the reference's peak-to-valley thresholds (tests/test_optics.py:823-838);
+* ``spectral_noise_multiscale``: the ensemble variance equals the PSD integral over g1 outside
+  the boundary plus over g2 inside it (the masks, their sides and both amplitude factors); a
+  one-pixel shift moves the screen by one pixel (the periodic FFT part and the non-periodic
+  MFT part alike, so only the overlapping samples are compared);
 * ``xinetics_influence_functions``: at zero tilt, on a pupil grid whose samples fall on the
   table's samples, each influence function equals the table entries (the shift, axis order
   and normalisation); the reference's tilted-centroid test (tests/test_optics.py:119-152);
@@ -815,6 +819,36 @@ def spectral_noise_fft(psd, output_grid, seed, oversample=1, shift=None):
     if shift is not None:
         C = C * np.exp(-1j * sum(shift[i] * g.coords[i] for i in range(g.ndim)))
     return fft.backward(C).real
+
+
+def _radius(g):
+    return np.hypot(*g.coords) if g.ndim == 2 else np.abs(g.coords[0])
+
+
+def spectral_noise_multiscale(psd, output_grid, seed, oversampling, shift=None):
+    """SpectralNoiseFactoryMultiscale(psd, output_grid, oversampling).make_random(seed),
+    optionally shifted, evaluated (hcipy/util/spectral_noise.py:203-330).  The high frequencies
+    sit on the FFT grid g1 of the output grid with |k| < boundary zeroed; the low ones on
+    g2 = g1 scaled by 1/oversampling with |k| >= boundary zeroed, boundary = max |x| of g2.
+    Noise is drawn for g1 (real, imaginary), then for g2; the screen is the real part of the
+    FFT backward of the first plus the MFT backward (output grid <- g2) of the second.
+    ``shift[i]`` moves along axis i."""
+    fft = FFT(output_grid)
+    g1 = fft.output_grid
+    g2 = g1.scaled(1.0 / oversampling)
+    boundary = np.abs(g2.coords[0]).max()
+    nd = g1.ndim
+    C1 = np.sqrt(np.asarray(psd(g1), dtype=float) / g1.weights * (2 * np.pi) ** nd)
+    C1[_radius(g1) < boundary] = 0
+    C2 = np.sqrt(np.asarray(psd(g2), dtype=float) / g2.weights * (2 * np.pi) ** nd)
+    C2[_radius(g2) >= boundary] = 0
+    rng = np.random.default_rng(seed)
+    C1 = C1 * (rng.standard_normal(g1.size) + 1j * rng.standard_normal(g1.size))
+    C2 = C2 * (rng.standard_normal(g2.size) + 1j * rng.standard_normal(g2.size))
+    if shift is not None:
+        C1 = C1 * np.exp(-1j * sum(shift[i] * g1.coords[i] for i in range(nd)))
+        C2 = C2 * np.exp(-1j * sum(shift[i] * g2.coords[i] for i in range(nd)))
+    return fft.backward(C1).real + mft_backward(C2, output_grid, g2).real
 
 
 def power_law_psd(exponent):

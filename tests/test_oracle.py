@@ -715,3 +715,42 @@ def test_high_pass_power_law_error_dc_only_filter():
     screen = o.power_law_error(g, 0.1, 1.5, seed=4)
     mask = o.circular_aperture(g, 1.5) > 0
     assert np.abs(got - (screen - screen.sum() / (2 * n) ** 2) * mask).max() < 1e-15
+
+
+def _smooth_psd(grid):
+    return (np.hypot(*grid.coords) ** 2 + 100.0) ** -1.0
+
+
+def test_spectral_noise_multiscale_variance():
+    """Ensemble variance of the multi-scale screens = sum of psd w / (2 pi)^2 over the FFT grid
+    outside the boundary plus over the finer grid inside it (spectral_noise.py:224-241): fixes
+    both amplitude factors and which side of the boundary each part keeps."""
+    g = o.make_pupil_grid(16, 1.0)
+    # finite at k = 0 (the scaled grid's centre sample can sit at +-1 ulp instead of 0, as the
+    # reference computes it) and spread over many modes, so 400 screens estimate the variance
+    psd = _smooth_psd
+    ovs = 4
+    g1 = o.FFT(g).output_grid
+    g2 = g1.scaled(1.0 / ovs)
+    b = np.abs(g2.coords[0]).max()
+    r1, r2 = np.hypot(*g1.coords), np.hypot(*g2.coords)
+    expect = (np.sum(psd(g1)[r1 >= b]) * g1.weights + np.sum(psd(g2)[r2 < b]) * g2.weights) / (2 * np.pi) ** 2
+    # the two parts alone: a swapped mask would move most of the power between them
+    assert np.sum(psd(g2)[r2 < b]) * g2.weights > 0.2 * expect * (2 * np.pi) ** 2
+    scr = np.array([o.spectral_noise_multiscale(psd, g, s, ovs) for s in range(400)])
+    var = np.mean(scr ** 2)
+    assert abs(var / expect - 1) < 0.05, var / expect
+
+
+def test_spectral_noise_multiscale_one_pixel_shift():
+    """A shift of one pixel along x (y) moves the screen by one pixel along that axis; the MFT
+    part is not periodic on the output grid, so only the overlapping samples are compared."""
+    g = o.make_pupil_grid([24, 16], [2.4, 1.6])
+    psd = _smooth_psd
+    base = o.spectral_noise_multiscale(psd, g, 8, 3).reshape(g.shape)
+    scale = np.abs(base).max()
+    sx = o.spectral_noise_multiscale(psd, g, 8, 3, shift=(g.delta[0], 0)).reshape(g.shape)
+    assert np.abs(sx[:, 1:] - base[:, :-1]).max() < 1e-12 * scale
+    sy = o.spectral_noise_multiscale(psd, g, 8, 3, shift=(0, g.delta[1])).reshape(g.shape)
+    assert np.abs(sy[1:, :] - base[:-1, :]).max() < 1e-12 * scale
+    assert np.abs(sx - base).max() > 1e-3 * scale
