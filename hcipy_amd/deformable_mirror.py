@@ -19,7 +19,7 @@ from . import _lib, pointwise
 from .field import Field, default_device, make_uniform_grid
 from .optics import OpticalElement, Wavefront
 
-__all__ = ['make_actuator_positions', 'make_gaussian_influence_functions', 'GaussianInfluenceFunctions',
+__all__ = ['TipTiltMirror', 'make_actuator_positions', 'make_gaussian_influence_functions', 'GaussianInfluenceFunctions',
            'make_xinetics_influence_functions', 'XineticsInfluenceFunctions', 'DeformableMirror']
 
 
@@ -315,3 +315,55 @@ class DeformableMirror(OpticalElement):
     def backward(self, wavefront):
         return Wavefront(pointwise.multiply_phase(wavefront.electric_field, self.surface, -2 * wavefront.wavenumber),
                          wavefront.wavelength, wavefront.input_stokes_vector)
+
+
+class TipTiltMirror(OpticalElement):
+    """A tip-tilt mirror: a deformable mirror whose two modes are the x and y coordinates
+    (tip_tilt_mirror.py:6-23), actuators in meters of surface per meter of x / y.  forward
+    multiplies the field by exp(2 i k (a0 x + a1 y)), backward by the conjugate; on a regular
+    grid the tilt phase is formed in FP64 by the phase-ramp kernel, with no surface array."""
+
+    def __init__(self, input_grid):
+        self.input_grid = input_grid
+        self.actuators = np.zeros((2,))
+
+    @property
+    def num_actuators(self):
+        return 2
+
+    @property
+    def actuators(self):
+        return self._actuators
+
+    @actuators.setter
+    def actuators(self, actuators):
+        self._actuators = np.asarray(actuators, dtype=np.float64).reshape(2)
+
+    @property
+    def surface(self):
+        x, y = self.input_grid.coords
+        return Field(torch.as_tensor(self._actuators[0] * x + self._actuators[1] * y, device=default_device()),
+                     self.input_grid)
+
+    @property
+    def opd(self):
+        return 2 * self.surface
+
+    def flatten(self):
+        self._actuators = np.zeros(2)
+
+    def _apply(self, wavefront, sign):
+        E = wavefront.electric_field
+        k2 = 2 * sign * wavefront.wavenumber
+        if self.input_grid.is_regular and not pointwise.needs_grad(E):
+            out = pointwise.phase_ramp(E, self.input_grid, k2 * self._actuators[0], k2 * self._actuators[1])
+            out = Field(out, self.input_grid)
+        else:
+            out = pointwise.multiply_phase(E, self.surface, k2)
+        return Wavefront(out, wavefront.wavelength, wavefront.input_stokes_vector)
+
+    def forward(self, wavefront):
+        return self._apply(wavefront, 1)
+
+    def backward(self, wavefront):
+        return self._apply(wavefront, -1)

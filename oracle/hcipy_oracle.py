@@ -51,6 +51,11 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
   states span every input Stokes vector; Malus's law for a rotated polarizer;
 * ``field_dot``/``phase_retarder_jones``: the half- and quarter-wave plate matrices and the
   circular-basis identity of the vector vortex;
+* ``pyramid_wfs_forward``/``modulated_pyramid_wfs_forward``: a flat pupil makes four equal
+  pupil images centred at (+-sep/2, +-sep/2) (the surface's scale); a pupil tilted into the
+  (+, +) facet lands in the (-, -) quadrant (the sign chain of two forward MFTs and the
+  surface); backward is the weighted adjoint of forward; the reference's slow/fast modulation
+  agreement (tests/test_wavefront_sensing.py:29-52);
 * ``gaussian_influence_functions``/``dm_surface``: crosstalk definition (value at the
   neighbouring actuator) and cutoff (hcipy/optics/deformable_mirror.py:78-79).
 """
@@ -893,6 +898,73 @@ def high_pass_power_law_error(pupil_grid, ptv, diameter, cutoff_frequency, expon
     mask = (np.asarray(aperture) > 0).astype(float)
     screen = power_law_error(pupil_grid, ptv, diameter, exponent, aperture=None, seed=seed)
     return fourier_filter(screen.astype(complex), pupil_grid, tf, q=2).real * mask
+
+
+# ----------------------------------------------------------------------------------------------
+# Pyramid wavefront-sensor optics (hcipy/wavefront_sensing/pyramid.py:10-200)
+
+
+class PyramidGeometry:
+    """The focal grid, spatial filter and pyramid surface of PyramidWavefrontSensorOptics
+    (pyramid.py:118-168)."""
+
+    def __init__(self, input_grid, output_grid, separation=None, pupil_diameter=None, wavelength_0=1, q=None,
+                 num_airy=None, refractive_index=lambda w: 1.5):
+        ext_in = np.asarray(input_grid.delta) * np.asarray(input_grid.dims)
+        ext_out = np.asarray(output_grid.delta) * np.asarray(output_grid.dims)
+        self.pupil_diameter = np.max(ext_in) if pupil_diameter is None else pupil_diameter
+        self.separation = self.pupil_diameter if separation is None else separation
+        qmin = np.ceil(max(np.max(ext_out / ext_in), 2))
+        q = qmin if q is None else q
+        self.num_airy = np.max(input_grid.shape) / 2 if num_airy is None else num_airy
+        num_pixels = 2 * int(self.num_airy * q)
+        self.focal_grid = make_pupil_grid(num_pixels, 2 * wavelength_0 / self.pupil_diameter * self.num_airy)
+        self.filter = circular_aperture(self.focal_grid, 2 * self.num_airy * wavelength_0 / self.pupil_diameter)
+        self.refractive_index = refractive_index
+        self.wavelength_0 = wavelength_0
+        self.input_grid, self.output_grid = input_grid, output_grid
+
+    def surface(self, grid):
+        x = grid.sep[0][np.newaxis, :]
+        y = grid.sep[1][:, np.newaxis]
+        n0 = self.refractive_index(self.wavelength_0)
+        return (-self.separation / (2 * (n0 - 1)) * (np.abs(x) + np.abs(y))).ravel()
+
+
+def pyramid_wfs_forward(E, geom, wavelength=1.0, adjoint=False):
+    """PyramidWavefrontSensorOptics.forward / backward (pyramid.py:170-200): Fraunhofer to the
+    focal grid, the spatial filter, exp(+-i (n - 1) k surface), Fraunhofer to the output grid
+    (backward: the adjoint chain in reverse)."""
+    k = 2 * np.pi / wavelength
+    n = geom.refractive_index(wavelength)
+    ph = np.exp((-1j if adjoint else 1j) * (n - 1) * k * geom.surface(geom.focal_grid))
+    if not adjoint:
+        F = fraunhofer_forward(E, geom.input_grid, geom.focal_grid, wavelength)
+        return fraunhofer_forward(F * geom.filter * ph, geom.focal_grid, geom.output_grid, wavelength)
+    F = fraunhofer_backward(E, geom.focal_grid, geom.output_grid, wavelength)
+    return fraunhofer_backward(F * geom.filter * ph, geom.input_grid, geom.focal_grid, wavelength)
+
+
+def modulated_pyramid_wfs_forward(E, geom, modulation, num_steps=12, fast=False, wavelength=1.0):
+    """ModulatedPyramidWavefrontSensorOptics.forward (pyramid.py:29-89): ``num_steps`` positions
+    p on a circle of radius modulation/2.  Slow: the input times the tip-tilt mirror's
+    exp(2 i k (p_x x + p_y y)), then the pyramid optics.  Fast: the filtered focal field times
+    exp(i (n - 1) k s_p) with s_p the surface on the focal grid shifted by 2 p, supersampled 4x."""
+    theta = np.linspace(0, 2 * np.pi, num_steps, endpoint=False)
+    pts = np.stack([modulation / 2 * np.cos(theta), modulation / 2 * np.sin(theta)], axis=1)
+    k = 2 * np.pi / wavelength
+    out = []
+    if fast:
+        F = fraunhofer_forward(E, geom.input_grid, geom.focal_grid, wavelength) * geom.filter
+        n = geom.refractive_index(wavelength)
+        for p in pts:
+            s = evaluate_supersampled(geom.surface, geom.focal_grid.shifted(2 * p), 4)
+            out.append(fraunhofer_forward(F * np.exp(1j * (n - 1) * k * s), geom.focal_grid, geom.output_grid, wavelength))
+        return out
+    x, y = geom.input_grid.coords
+    for p in pts:
+        out.append(pyramid_wfs_forward(E * np.exp(2j * k * (p[0] * x + p[1] * y)), geom, wavelength))
+    return out
 
 
 def actuator_positions(n_act, pitch):
