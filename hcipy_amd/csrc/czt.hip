@@ -29,6 +29,7 @@
 #include "../../include/hcipy_amd.h"
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 
 namespace hcipy {
@@ -45,8 +46,26 @@ struct CztCfg {
 
 // Chirps of one axis pass in FP64 (formulas in the header comment).  One thread per index of
 // the longest array; h is written in FFT order (index d mod L).
+// exp(i s pi (t^2 mod 2N) / N) = exp(i s D t^2 / 2) for D = 2 pi / N exactly: the quadratic
+// chirp phase of a transform onto the FFT grid of N points, reduced in integers, so that its
+// rounding does not grow with t^2.
+__device__ inline double2 dft_chirp(int64_t t, int fft_n, double s) {
+  const int64_t r = (t * t) % (2 * (int64_t)fft_n);
+  double sn, cs;
+  sincospi(s * (double)r / (double)fft_n, &sn, &cs);
+  return make_double2(cs, sn);
+}
+
+__device__ inline double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// fft_n > 0: the grids are an FFT pair, d_in * d_out = 2 pi / fft_n; the quadratic chirp terms
+// use that exact value (the sum is then the reference FFT's exact DFT) and are multiplied by
+// the linear terms, whose phases stay below pi n m / fft_n.  fft_n = 0: one FP64 phase per term.
 __global__ void czt_chirp_kernel(int n, int m, int L, double d_in, double z_in, double d_out, double z_out, int sign,
-                                 double2* __restrict__ A, double2* __restrict__ h, double2* __restrict__ C) {
+                                 int fft_n, double2* __restrict__ A, double2* __restrict__ h,
+                                 double2* __restrict__ C) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= L) return;
   const double s = (double)sign;
@@ -54,13 +73,23 @@ __global__ void czt_chirp_kernel(int n, int m, int L, double d_in, double z_in, 
   double sn, cs;
   if (t < n) {
     const double j = (double)t;
-    sincos(s * (j * d_in * z_out + 0.5 * D * (j * j)), &sn, &cs);
-    A[t] = make_double2(cs, sn);
+    if (fft_n > 0) {
+      sincos(s * (j * d_in * z_out), &sn, &cs);
+      A[t] = cmul2(make_double2(cs, sn), dft_chirp(t, fft_n, s));
+    } else {
+      sincos(s * (j * d_in * z_out + 0.5 * D * (j * j)), &sn, &cs);
+      A[t] = make_double2(cs, sn);
+    }
   }
   if (t < m) {
     const double k = (double)t;
-    sincos(s * (0.5 * D * (k * k) + z_in * (z_out + k * d_out)), &sn, &cs);
-    C[t] = make_double2(cs, sn);
+    if (fft_n > 0) {
+      sincos(s * (z_in * (z_out + k * d_out)), &sn, &cs);
+      C[t] = cmul2(make_double2(cs, sn), dft_chirp(t, fft_n, s));
+    } else {
+      sincos(s * (0.5 * D * (k * k) + z_in * (z_out + k * d_out)), &sn, &cs);
+      C[t] = make_double2(cs, sn);
+    }
   }
   // h at index t: d = t for t < m, d = t - L for t >= L - (n - 1), zero in between
   double2 hv = make_double2(0.0, 0.0);
@@ -74,9 +103,13 @@ __global__ void czt_chirp_kernel(int n, int m, int L, double d_in, double z_in, 
     on = true;
   }
   if (on) {
-    const double dd = (double)d;
-    sincos(-s * 0.5 * D * (dd * dd), &sn, &cs);
-    hv = make_double2(cs, sn);
+    if (fft_n > 0) {
+      hv = dft_chirp(d, fft_n, -s);
+    } else {
+      const double dd = (double)d;
+      sincos(-s * 0.5 * D * (dd * dd), &sn, &cs);
+      hv = make_double2(cs, sn);
+    }
   }
   h[t] = hv;
 }
@@ -264,8 +297,8 @@ int launch_filter(const double2* h, const double2* tw64, double2* H64, float2* H
   return check_hip(hipGetLastError(), "czt_filter_kernel");
 }
 
-int axis_setup(CztAxis& a, int n, int m, double d_in, double z_in, double d_out, double z_out, int sign, int dtype,
-               double2* h, hipStream_t s) {
+int axis_setup(CztAxis& a, int n, int m, double d_in, double z_in, double d_out, double z_out, int sign, int fft_n,
+               int dtype, double2* h, hipStream_t s) {
   a.n = n;
   a.m = m;
   a.L = next_pow2(n + m - 1);
@@ -281,8 +314,8 @@ int axis_setup(CztAxis& a, int n, int m, double d_in, double z_in, double d_out,
     HCIPY_HIP(hipMalloc(&a.tw32, (size_t)L * 8));
   }
   const unsigned gb = (unsigned)ceil_div(L, 256);
-  hipLaunchKernelGGL(czt_chirp_kernel, dim3(gb), dim3(256), 0, s, n, m, L, d_in, z_in, d_out, z_out, sign, a.A64, h,
-                     a.C64);
+  hipLaunchKernelGGL(czt_chirp_kernel, dim3(gb), dim3(256), 0, s, n, m, L, d_in, z_in, d_out, z_out, sign, fft_n,
+                     a.A64, h, a.C64);
   HCIPY_HIP(hipGetLastError());
   hipLaunchKernelGGL(czt_twiddle_kernel, dim3(gb), dim3(256), 0, s, L, a.tw64, a.tw32);
   HCIPY_HIP(hipGetLastError());
@@ -354,9 +387,9 @@ struct hcipy_zoom_plan_s {
 
 extern "C" {
 
-int hcipy_zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, const double* delta_in,
-                           const double* zero_in, const int* n_out, const double* delta_out, const double* zero_out,
-                           int dtype, int precision, void* stream) {
+static int zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, const double* delta_in,
+                            const double* zero_in, const int* n_out, const double* delta_out, const double* zero_out,
+                            const int* fft_n, int dtype, int precision, void* stream) {
   HCIPY_REQUIRE(plan != nullptr, "plan pointer is NULL");
   *plan = nullptr;
   HCIPY_REQUIRE(ndim == 1 || ndim == 2, "the zoom FFT supports one- and two-dimensional grids");
@@ -369,6 +402,12 @@ int hcipy_zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, con
     HCIPY_REQUIRE(n_in[d] + n_out[d] - 1 <= kCztMaxL,
                   "axis %d: n_in + n_out - 1 = %d exceeds the %d-point line FFT of the zoom FFT", d,
                   n_in[d] + n_out[d] - 1, kCztMaxL);
+    if (fft_n) {
+      HCIPY_REQUIRE(fft_n[d] >= 1, "axis %d: fft_n must be positive", d);
+      const double r = delta_in[d] * delta_out[d] * (double)fft_n[d] / (2.0 * M_PI);
+      HCIPY_REQUIRE(std::fabs(r - 1.0) < 1e-9, "axis %d: delta_in * delta_out = %.17g is not 2 pi / %d", d,
+                    delta_in[d] * delta_out[d], fft_n[d]);
+    }
   }
   hcipy_zoom_plan_s* hp = new (std::nothrow) hcipy_zoom_plan_s();
   HCIPY_REQUIRE(hp != nullptr, "out of host memory");
@@ -386,11 +425,12 @@ int hcipy_zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, con
   for (int d = 0; d < ndim && !st; ++d) {
     // forward: input grid -> output grid, exp(-i p q); backward: output grid -> input grid, exp(+i p q)
     const int tdt = precision == HCIPY_PREC_FP32 ? HCIPY_DTYPE_C64 : HCIPY_DTYPE_C128;  // table precision
-    st = axis_setup(p.ax[0][d], n_in[d], n_out[d], delta_in[d], zero_in[d], delta_out[d], zero_out[d], -1, tdt,
+    const int fn = fft_n ? fft_n[d] : 0;
+    st = axis_setup(p.ax[0][d], n_in[d], n_out[d], delta_in[d], zero_in[d], delta_out[d], zero_out[d], -1, fn, tdt,
                     static_cast<double2*>(p.scratch), s);
     // the FP64 h buffer is reused: stream order keeps each filter kernel before the next chirp kernel
     if (!st)
-      st = axis_setup(p.ax[1][d], n_out[d], n_in[d], delta_out[d], zero_out[d], delta_in[d], zero_in[d], +1, tdt,
+      st = axis_setup(p.ax[1][d], n_out[d], n_in[d], delta_out[d], zero_out[d], delta_in[d], zero_in[d], +1, fn, tdt,
                       static_cast<double2*>(p.scratch), s);
   }
   if (st) {
@@ -399,6 +439,21 @@ int hcipy_zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, con
   }
   *plan = hp;
   return kOk;
+}
+
+int hcipy_zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, const double* delta_in,
+                           const double* zero_in, const int* n_out, const double* delta_out, const double* zero_out,
+                           int dtype, int precision, void* stream) {
+  return zoom_plan_create(plan, ndim, n_in, delta_in, zero_in, n_out, delta_out, zero_out, nullptr, dtype, precision,
+                          stream);
+}
+
+int hcipy_zoom_plan_create_fft(hcipy_zoom_plan* plan, int ndim, const int* n_in, const double* delta_in,
+                               const double* zero_in, const int* n_out, const double* delta_out,
+                               const double* zero_out, const int* fft_n, int dtype, int precision, void* stream) {
+  HCIPY_REQUIRE(fft_n != nullptr, "NULL fft_n");
+  return zoom_plan_create(plan, ndim, n_in, delta_in, zero_in, n_out, delta_out, zero_out, fft_n, dtype, precision,
+                          stream);
 }
 
 int hcipy_zoom_plan_destroy(hcipy_zoom_plan plan) {

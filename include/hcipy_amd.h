@@ -421,6 +421,15 @@ typedef struct hcipy_zoom_plan_s* hcipy_zoom_plan;
 int hcipy_zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, const double* delta_in,
                            const double* zero_in, const int* n_out, const double* delta_out, const double* zero_out,
                            int dtype, int precision, void* stream);
+/* The same plan for an FFT pair of grids: per axis delta_in * delta_out = 2 pi / fft_n[d] (to
+ * 1e-9, else an error), fft_n the FFT length of the internal grid (the output grid is that FFT
+ * grid, cropped and shifted as make_fft_grid makes it).  The quadratic chirp phases then use the
+ * exact 2 pi / fft_n, reduced in integers, so the result is the exact DFT of FastFourierTransform
+ * with its shift ramps (fourier/fast_fourier_transform.py:280-406) and its rounding does not grow
+ * with the chirp's t^2.  Execution, workspace and destruction as for the general plan. */
+int hcipy_zoom_plan_create_fft(hcipy_zoom_plan* plan, int ndim, const int* n_in, const double* delta_in,
+                               const double* zero_in, const int* n_out, const double* delta_out,
+                               const double* zero_out, const int* fft_n, int dtype, int precision, void* stream);
 int hcipy_zoom_plan_destroy(hcipy_zoom_plan plan);
 /* Bytes of workspace (the intermediate of the two axis passes) for `batch` fields; 0 in 1-D. */
 int hcipy_zoom_workspace_size(hcipy_zoom_plan plan, int64_t batch, size_t* bytes);

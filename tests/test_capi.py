@@ -95,3 +95,20 @@ def test_matrix_filter_entry_points_reject_bad_arguments():
     assert b'at most 4 x 4' in lib.hcipy_get_last_error()
     assert lib.hcipy_filter_execute_matrix(hs, 1, 1, 0, None, None, 1, None, 0, None) == 1
     assert b'NULL plan 0' in lib.hcipy_get_last_error()
+
+
+def test_zoom_fft_pair_entry_point_checks_the_grids():
+    """hcipy_zoom_plan_create_fft: fft_n is required, positive, and must satisfy
+    delta_in * delta_out = 2 pi / fft_n; the checks run before any device work."""
+    import math
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    ci = ctypes.c_int * 1
+    cd = ctypes.c_double * 1
+    n, z = ci(8), cd(0.0)
+    d_in, d_out = cd(0.25), cd(2 * math.pi / (16 * 0.25))
+    assert lib.hcipy_zoom_plan_create_fft(ctypes.byref(h), 1, n, d_in, z, n, d_out, z, None, 1, 0, None) == 1
+    assert b'fft_n' in lib.hcipy_get_last_error()
+    assert lib.hcipy_zoom_plan_create_fft(ctypes.byref(h), 1, n, d_in, z, n, d_out, z, ci(0), 1, 0, None) == 1
+    assert lib.hcipy_zoom_plan_create_fft(ctypes.byref(h), 1, n, d_in, z, n, d_out, z, ci(17), 1, 0, None) == 1
+    assert b'is not 2 pi / 17' in lib.hcipy_get_last_error()

@@ -209,3 +209,27 @@ def test_zoom_1d_matches_scipy_czt():
         phase = np.abs(go.separated_coords[0]).max() * np.abs(g.separated_coords[0]).max()
         assert np.abs(out - dense).max() < (1e-12 + 1e-15 * phase) * peak, (n, m)
         assert np.abs(out - ref).max() < (1e-12 + 1e-15 * (n + m) ** 2) * peak, (n, m)
+
+
+@pytest.mark.parametrize('n,q,fov,shift', [(1024, 2, 1, 0), (512, 2, 0.5, 0.1), (96, 3, 0.7, 0.0), (2048, 1, 1, 0)])
+def test_fft_class_on_zoom_backend_matches_oracle(n, q, fov, shift):
+    """FastFourierTransform (fast_fourier_transform.py:188-406) on the zoom-FFT backend against the
+    oracle's numpy FFT with the reference's pad, shift and crop: complex128 at 1e-12 of the
+    peak, complex64 (FP64 internal) at 1e-6; backward against the oracle too.  The split-FP16
+    MFT backend gives the same transform within its own bound."""
+    g = ha.make_pupil_grid(n, 1.0)
+    ft = ha.FastFourierTransform(g, q, fov, shift)
+    assert ft.method == 'zft'
+    ref_ft = o.FFT(og(g), q, fov, shift)
+    f = _crand(g.size, 7)
+    ref = ref_ft.forward(f)
+    for dtype, tol in ((np.complex128, 1e-12), (np.complex64, 1e-6)):
+        out = host(ft.forward(ha.Field(dev(f.astype(dtype)), g)))
+        e = peak_rel_field(out, ref)
+        say('FFT on zoom n=%d q=%g fov=%g %s: %.2e' % (n, q, fov, dtype.__name__, e))
+        assert e < tol
+    back = host(ft.backward(ha.Field(dev(ref), ft.output_grid)))
+    assert peak_rel_field(back, ref_ft.backward(ref)) < 1e-12
+    if n <= 512:
+        m = host(ha.FastFourierTransform(g, q, fov, shift, precision='f16x2').forward(ha.Field(dev(f.astype(np.complex64)), g)))
+        assert peak_rel_field(m, ref) < 2e-6

@@ -65,3 +65,14 @@ def test_planner_reference_choices():
     un = CartesianGrid(UnstructuredCoords([rng.standard_normal(100), rng.standard_normal(100)]))
     with pytest.raises(ValueError):
         ha.make_fourier_transform(big, un)
+
+
+def test_fft_backend_choice():
+    """FastFourierTransform evaluates its sum with the zoom FFT where each axis fits the 8192-point
+    line FFT, with the MFT beyond that or when the split-FP16 MFT is asked for."""
+    import hcipy_amd as ha
+    g = ha.make_pupil_grid(256)
+    assert ha.FastFourierTransform(g, q=2).method == 'zft'
+    assert ha.FastFourierTransform(g, q=2, fov=0.5, shift=0.1, precision='fp32').method == 'zft'
+    assert ha.FastFourierTransform(g, precision='f16x2').method == 'mft'
+    assert ha.FastFourierTransform(ha.make_uniform_grid([5000], [1.0])).method == 'mft'
