@@ -252,6 +252,39 @@ def main():
         sec, _ = time_graph(lambda: vvc.forward(ha.Wavefront(e64, 1.0)).electric_field)
         out['vector_vortex_1024_charge2_f16x2'] = {'value': round(1.0 / sec, 1), 'unit': 'forward/s',
                                                    'ms': round(sec * 1e3, 4)}
+        # 2 x 2 Jones-matrix transfer function (random unit-modulus elements) on a Jones-vector field
+        # of the 1024^2 pupil, q = 2: one forward FFT per input component, one inverse per output.
+        rng = np.random.default_rng(5)
+        jint = ha.make_fft_grid(asp_grid, 2, 1)
+        jtf = torch.as_tensor(np.exp(2j * np.pi * rng.random((2, 2, jint.size))), device=dev)
+        jf = ha.FourierFilter(asp_grid, jtf, 2)
+        jx128 = ha.Field(torch.stack([x, x.flip(0)]).to(torch.complex128), asp_grid)
+        jref = jf.forward(jx128).as_subclass(torch.Tensor)
+        for key, xin in (('jones_filter_1024_q2_c64', jx128.as_subclass(torch.Tensor).to(torch.complex64)),
+                         ('jones_filter_1024_q2_c128', jx128.as_subclass(torch.Tensor))):
+            jin = ha.Field(xin, asp_grid)
+            sec, o = time_graph(lambda: jf.forward(jin))
+            o = o.as_subclass(torch.Tensor).to(torch.complex128)
+            fb = 8 if xin.dtype == torch.complex64 else 16
+            # rows: 2 fields in, 2 intermediates out; columns: 2 in, 4 transfer functions, 2 out;
+            # rows: 2 in, 2 fields out
+            nbytes = fb * (2 * 1024 ** 2 + 2 * 2048 * 1024 + 2 * 2048 * 1024 + 4 * 2048 ** 2 + 2 * 2048 * 1024
+                           + 2 * 2048 * 1024 + 2 * 1024 ** 2)
+            out[key] = {'value': round(1.0 / sec, 1), 'unit': 'filters/s', 'ms': round(sec * 1e3, 4),
+                        'hbm_gbs_pass_bytes': round(nbytes / sec / 1e9, 1),
+                        'peak_rel_field_error_vs_c128': (o - jref).abs().max().item() / jref.abs().max().item()}
+        # FastFourierTransform of a 2048^2 field (q = 1) on the zoom-FFT backend (exact 2 pi / N chirps).
+        fgrid = ha.make_pupil_grid(2048, 1.0)
+        fft = ha.FastFourierTransform(fgrid)
+        fx = torch.as_tensor(rng.standard_normal(fgrid.size) + 1j * rng.standard_normal(fgrid.size), device=dev)
+        fref = ha.MatrixFourierTransform(fgrid, fft.output_grid).forward(ha.Field(fx, fgrid)).as_subclass(torch.Tensor)
+        for key, xin in (('fft_2048_c64', fx.to(torch.complex64)), ('fft_2048_c128', fx)):
+            fin = ha.Field(xin, fgrid)
+            sec, o = time_graph(lambda: fft.forward(fin))
+            o = o.as_subclass(torch.Tensor).to(torch.complex128)
+            out[key] = {'value': round(1.0 / sec, 1), 'unit': 'transforms/s', 'ms': round(sec * 1e3, 4),
+                        'method': fft.method,
+                        'peak_rel_field_error_vs_fp64_mft': (o - fref).abs().max().item() / fref.abs().max().item()}
         return out
 
     def time_mixed_grouped(n_wl=8, batch=4, reps=10):
