@@ -101,9 +101,12 @@ enum BMode : int {
 // XCD-aware tile order.  Consecutive workgroups are dispatched round-robin over the 8 XCDs; the
 // hardware id h is remapped so that XCD x owns a contiguous chunk of the logical tile order, and
 // that order walks GM tile rows at a time, so each XCD's L2 sees a compact block of A and B rows.
-__device__ inline void tile_of(int h, int gx, int gy, int& bx, int& by) {
+// slot_off is the XCD slot of the workgroup with h = 0 (nonzero inside a grouped launch, where an
+// item's first workgroup need not sit on a multiple of 8); within each run of 8 consecutive h the
+// slots (h + slot_off) mod 8 are all distinct, so the map stays a bijection on [0, T).
+__device__ inline void tile_of(int h, int gx, int gy, int& bx, int& by, int slot_off = 0) {
   const int T = gx * gy;
-  const int L = (T % 8 == 0) ? (h % 8) * (T / 8) + h / 8 : h;
+  const int L = (T % 8 == 0) ? ((h + slot_off) & 7) * (T / 8) + (h >> 3) : h;
   constexpr int GM = 4;
   const int per_group = GM * gy;
   const int group = L / per_group;
@@ -151,7 +154,7 @@ __device__ inline u32x4_t expand_odd(u32x4_t v) {
 // the MFMAs of tile t run.
 // Body of the stage kernel for workgroup tile h (in the XCD-aware order) of batch slice z.
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
-__device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z) {
+__device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int slot_off = 0) {
   using TR = PrecTraits<P>;
   using T = typename TR::T;
   constexpr int MT = TR::MT;
@@ -194,7 +197,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z) {
   const int gx = (p.M + BM - 1) / BM;
   const int gy = (p.N + BN - 1) / BN;
   int bx, by;
-  tile_of(h, gx, gy, bx, by);
+  tile_of(h, gx, gy, bx, by, slot_off);
   const int m0 = bx * BM;
   const int mc0 = m0 / 2;
   const int n0 = by * BN;
@@ -658,7 +661,9 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_grouped_kernel(co
   const StageArgs p = table[lo];
   const int64_t tiles_xy = (int64_t)((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
   const int64_t local = b - tile_start[lo];
-  stage_body<P, BM, BN, BK, WM, WN, WK, BMODE, PF>(p, (int)(local % tiles_xy), (int)(local / tiles_xy));
+  const int h = (int)(local % tiles_xy);
+  // the XCD slot of this item's tile h = 0 in this batch slice
+  stage_body<P, BM, BN, BK, WM, WN, WK, BMODE, PF>(p, h, (int)(local / tiles_xy), (int)((b - h) & 7));
 }
 
 // Split-FP16 stage kernel with the constant operand kept compact.

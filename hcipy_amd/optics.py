@@ -192,9 +192,15 @@ def wavelength_key(wavelength):
 
 class AgnosticOpticalElement(OpticalElement):
     """Element whose per-(grid, wavelength) instance data is built on first use and kept in a
-    FIFO cache of at most ``max_in_cache`` entries (optical_element.py:261-380, 600-644)."""
+    FIFO cache of at most ``max_in_cache`` entries (optical_element.py:261-380, 600-644).
 
-    def __init__(self, grid_dependent=True, wavelength_dependent=True, max_in_cache=11):
+    The default keeps 64 instances where the reference keeps 11: instance data (Fourier
+    matrices, filter plans) lives in device memory, and a FIFO of 11 under a cycle of more
+    wavelengths (32-wavelength broadband or Jacobian runs) misses on every access, rebuilding
+    every instance on every pass.  64 instances of a 1024^2 -> 512^2 propagator hold about
+    1 GB."""
+
+    def __init__(self, grid_dependent=True, wavelength_dependent=True, max_in_cache=64):
         self.grid_dependent = grid_dependent
         self.wavelength_dependent = wavelength_dependent
         self.max_in_cache = max_in_cache
