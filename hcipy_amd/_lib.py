@@ -152,6 +152,9 @@ SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int, _vp]),
+    'hcipy_nearfield_transfer_function': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     'hcipy_zoom_plan_create_fft': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),

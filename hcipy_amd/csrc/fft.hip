@@ -376,7 +376,8 @@ __device__ inline double sep_spacing(int i, int n, double delta, double zero) {
 }
 
 __global__ void nearfield_tf_kernel(int kind, int NX, int NY, double dx, double zx, double dy, double zy, double k,
-                                    double z, int ovs, double pf_re, double pf_im, void* out, int out_dtype) {
+                                    double z, int ovs, double pf_re, double pf_im, void* out, int out_dtype,
+                                    int centred) {
   const int64_t total = (int64_t)NX * NY;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int iy = (int)(i / NX);
@@ -418,10 +419,11 @@ __global__ void nearfield_tf_kernel(int kind, int NX, int NY, double dx, double 
     }
     acc_re /= (double)(ovs * ovs);
     acc_im /= (double)(ovs * ovs);
-    // stored in FFT order (ifftshift of the centred internal grid), column-blocked
+    // filter plans: FFT order (ifftshift of the centred internal grid), column-blocked;
+    // centred: grid order
     const int oxi = (ix + NX - NX / 2) % NX;
     const int oyi = (iy + NY - NY / 2) % NY;
-    const int64_t o = blk(oyi, oxi, NY, NX);
+    const int64_t o = centred ? i : blk(oyi, oxi, NY, NX);
     if (out_dtype == HCIPY_DTYPE_C64)
       static_cast<float2*>(out)[o] = make_float2((float)acc_re, (float)acc_im);
     else
@@ -757,7 +759,22 @@ int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, d
   const int64_t total = (int64_t)p.NX * p.NY;
   hipLaunchKernelGGL(nearfield_tf_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), kind, p.NX, p.NY, delta_x, zero_x, delta_y, zero_y, wavenumber,
-                     distance, oversampling, pf_re, pf_im, p.tf, p.dtype);
+                     distance, oversampling, pf_re, pf_im, p.tf, p.dtype, 0);
+  return check_hip(hipGetLastError(), "nearfield_tf_kernel");
+}
+
+int hcipy_nearfield_transfer_function(int kind, int nx, int ny, double delta_x, double zero_x, double delta_y,
+                                      double zero_y, double wavenumber, double distance, int oversampling, void* out,
+                                      void* stream) {
+  HCIPY_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (angular spectrum) or 1 (Fresnel)");
+  HCIPY_REQUIRE(oversampling >= 1 && oversampling <= 16, "oversampling out of range");
+  HCIPY_REQUIRE(nx >= 1 && ny >= 1, "grid sizes must be positive");
+  HCIPY_REQUIRE(out != nullptr, "NULL output");
+  const double pf_re = cos(wavenumber * distance), pf_im = sin(wavenumber * distance);
+  const int64_t total = (int64_t)nx * ny;
+  hipLaunchKernelGGL(nearfield_tf_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), kind, nx, ny, delta_x, zero_x, delta_y, zero_y, wavenumber,
+                     distance, oversampling, pf_re, pf_im, out, (int)HCIPY_DTYPE_C128, 1);
   return check_hip(hipGetLastError(), "nearfield_tf_kernel");
 }
 

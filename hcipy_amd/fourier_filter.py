@@ -2,10 +2,11 @@
 (hcipy/fourier/fourier_operations.py:8-151).
 
 Power-of-two internal grids run on the library's three-pass Stockham FFT kernels
-(``hcipy_filter_*``).  Other sizes run the same operator as two matrix Fourier transforms on
-MFMA (input grid -> internal frequency grid -> cropped input grid); on the FFT grid the two
-give identical results up to rounding, since the weights multiply to 1/(NX NY) and the
-kernel exp(i u (x' - x)) is N-periodic.
+(``hcipy_filter_*``).  Other sizes run the same operator as FastFourierTransform forward onto
+the internal frequency grid, the transfer function, and its backward onto the input grid (the
+zoom FFT with exact 2 pi / N chirps, or the MFT beyond its line length); on the FFT grid this
+is the padded FFT filter, since the weights multiply to 1/(NX NY) and the kernel
+exp(i u (x' - x)) is N-periodic.
 """
 import ctypes
 
@@ -14,7 +15,7 @@ import torch
 
 from . import _lib, pointwise
 from .field import Field
-from .fourier import MatrixFourierTransform, complex_dtype_for, make_fft_grid
+from .fourier import FastFourierTransform, complex_dtype_for, make_fft_grid
 
 __all__ = ['FourierFilter', 'NearFieldTransferFunction']
 
@@ -162,11 +163,23 @@ class FourierFilter:
         self.transfer_function = transfer_function
         self._plans = {}
         self._multi_ws = {}
-        self._mft = None
+        self._fft = None
+        self._tf_cache = {}
         self._matrix_tf = None
 
     def evaluate_transfer_function(self, device):
         tf = self.transfer_function
+        if isinstance(tf, NearFieldTransferFunction):
+            # generated on the device in FP64, centred order, on the internal grid
+            g = self.internal_grid
+            out = torch.empty(g.size, dtype=torch.complex128, device=device)
+            with torch.cuda.device(device):
+                st = _lib.load().hcipy_nearfield_transfer_function(
+                    NearFieldTransferFunction.KINDS[tf.kind], int(g.dims[0]), int(g.dims[1]), float(g.delta[0]),
+                    float(g.zero[0]), float(g.delta[1]), float(g.zero[1]), tf.wavenumber, tf.distance, tf.oversampling,
+                    _lib.dptr(out), _lib.stream_ptr(device))
+                _lib.check(st, 'hcipy_nearfield_transfer_function')
+            return out
         if callable(tf) and not isinstance(tf, torch.Tensor):
             tf = tf(self.internal_grid)
         if not isinstance(tf, torch.Tensor):
@@ -289,17 +302,22 @@ class FourierFilter:
             out = torch.empty_like(t)
             plan.execute(adjoint, t, out, batch)
             return out
-        return self._mft_operation(t, adjoint, cdt).as_subclass(torch.Tensor)
+        return self._fft_operation(t, adjoint, cdt).as_subclass(torch.Tensor)
 
-    def _mft_operation(self, t, adjoint, cdt):
-        if self._mft is None:
-            self._mft = MatrixFourierTransform(self.input_grid, self.internal_grid)
-        F = self._mft.forward(Field(t, self.input_grid))
-        tf = self.evaluate_transfer_function(t.device).to(cdt)
-        if adjoint:
-            tf = tf.conj()
-        F = Field(F.as_subclass(torch.Tensor) * tf, self.internal_grid)
-        return self._mft.backward(F)
+    def _fft_operation(self, t, adjoint, cdt):
+        """Non-power-of-two internal grids: FastFourierTransform onto the internal grid, the
+        transfer function (conjugated for the adjoint; evaluated once per device and dtype),
+        and the backward transform."""
+        if self._fft is None:
+            self._fft = FastFourierTransform(self.input_grid, self.q)
+        key = (t.device, cdt)
+        tf = self._tf_cache.get(key)
+        if tf is None:
+            tf = self.evaluate_transfer_function(t.device).to(cdt).contiguous()
+            self._tf_cache[key] = tf
+        F = self._fft.forward(Field(t, self.input_grid))
+        F = Field(pointwise.multiply(F, tf, conjugate=adjoint), self.internal_grid)
+        return self._fft.backward(F)
 
     def forward(self, field):
         return self._operation(field, adjoint=False)

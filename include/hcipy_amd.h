@@ -363,6 +363,14 @@ int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, d
                                              double delta_y, double zero_y, double wavenumber, double distance,
                                              int oversampling, void* stream);
 
+/* The same transfer function on an nx x ny frequency grid (separated coordinates i * delta + zero),
+ * written as complex128 in grid (centred) order: for filters whose padded grid is not a power
+ * of two, which transform with the zoom FFT or the MFT and multiply by it pointwise
+ * (angular_spectrum.py:71-78, fresnel.py:69-80). */
+int hcipy_nearfield_transfer_function(int kind, int nx, int ny, double delta_x, double zero_x, double delta_y,
+                                      double zero_y, double wavenumber, double distance, int oversampling, void* out,
+                                      void* stream);
+
 /* Impulse response of a near-field propagator on an nx x ny spatial grid (separated coordinates
  * i * delta + zero), FP64, averaged over oversampling^2 dithered sub-grids, written as complex128
  * in grid order.  The impulse-response branch of the propagators Fourier transforms it into the

@@ -42,10 +42,12 @@ def test_fourier_filter_random_tf(n, q, dtype):
 @pytest.mark.parametrize('which', ['angular_spectrum', 'fresnel'])
 @pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
 @pytest.mark.parametrize('rel', [0.2, 5])
-def test_nearfield_propagators_vs_oracle(which, dtype, rel):
+@pytest.mark.parametrize('n', [256, 96])
+def test_nearfield_propagators_vs_oracle(which, dtype, rel, n):
     """rel = 0.2: transfer-function branch; rel = 5: impulse-response branch (the grid step is
-    below lambda z / L)."""
-    n, wl, a, b = 256, 500e-9, 0.001, 0.001
+    below lambda z / L).  n = 96 pads to 192^2, not a power of two: the filter then runs on
+    FastFourierTransform with the device-generated transfer function in grid order."""
+    wl, a, b = 500e-9, 0.001, 0.001
     g = ha.make_pupil_grid(n, [16 * a, 16 * b])
     zc = np.min(g.delta * np.array([16 * a, 16 * b]) / wl)
     z = rel * zc
