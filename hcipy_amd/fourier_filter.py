@@ -291,7 +291,7 @@ class FourierFilter:
         NY, NX = self.internal_grid.shape
         if self._is_matrix(t.device):
             if not (_pow2(NX) and _pow2(NY)):
-                raise ValueError('matrix transfer functions need a power-of-two internal grid')
+                return self._fft_matrix_operation(t, adjoint, cdt).as_subclass(torch.Tensor)
             return self._run_matrix(t, adjoint)
         if _pow2(NX) and _pow2(NY):
             key = (cdt, t.device.index)
@@ -318,6 +318,27 @@ class FourierFilter:
         F = self._fft.forward(Field(t, self.input_grid))
         F = Field(pointwise.multiply(F, tf, conjugate=adjoint), self.internal_grid)
         return self._fft.backward(F)
+
+    def _fft_matrix_operation(self, t, adjoint, cdt):
+        """Matrix transfer function on a non-power-of-two internal grid: each input component
+        transformed onto the internal grid, the per-frequency matrix product (field_dot; J^H for
+        the adjoint), each output component transformed back."""
+        tf = self._matrix_tf
+        p, q = int(tf.shape[0]), int(tf.shape[1])
+        n_in = p if adjoint else q
+        if t.dim() < 2 or t.shape[0] != n_in:
+            raise ValueError('a matrix transfer function of shape (%d, %d) needs %d field components, got shape %s'
+                             % (p, q, n_in, tuple(t.shape)))
+        if self._fft is None:
+            self._fft = FastFourierTransform(self.input_grid, self.q)
+        key = (t.device, cdt, 'matrix')
+        J = self._tf_cache.get(key)
+        if J is None:
+            J = tf.to(device=t.device, dtype=cdt).contiguous()
+            self._tf_cache[key] = J
+        F = self._fft.forward(Field(t, self.input_grid))
+        G = pointwise.field_dot(J, F.as_subclass(torch.Tensor).contiguous(), adjoint=adjoint)
+        return self._fft.backward(Field(G.as_subclass(torch.Tensor).contiguous(), self.internal_grid))
 
     def forward(self, field):
         return self._operation(field, adjoint=False)

@@ -216,3 +216,26 @@ def test_matrix_filter_elements_shared_passes(dtype, tol):
                 for j in range(q_):
                     ref = o.fourier_filter(f.astype(np.complex128), og(g), tf[i, j], q=2, adjoint=adjoint)
                     assert peak_rel_field(out[i, j], ref) < tol, (p_, q_, i, j, adjoint)
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
+@pytest.mark.parametrize('dims,q', [([17, 17], 2), ([24, 16], 3)])
+def test_matrix_transfer_function_non_pow2(dims, q, dtype, tol):
+    """A (2 x 3) matrix transfer function on padded grids that are not powers of two (34^2,
+    72 x 48): FastFourierTransform per component, the per-frequency field_dot, the backward
+    transform; forward and adjoint against the oracle, with a batch dimension."""
+    g = ha.make_uniform_grid(dims, [1.0, dims[1] / dims[0]])
+    internal = o.make_fft_grid(og(g), q)
+    rng = np.random.default_rng(13)
+    tf = rng.standard_normal((2, 3, internal.size)) + 1j * rng.standard_normal((2, 3, internal.size))
+    filt = ha.FourierFilter(g, dev(tf), q)
+    f = (rng.standard_normal((3, 2, g.size)) + 1j * rng.standard_normal((3, 2, g.size))).astype(dtype)
+    out = host(filt.forward(ha.Field(dev(f), g)))
+    ref = o.fourier_filter(f.astype(np.complex128), og(g), tf, q)
+    assert out.shape == (2, 2, g.size)
+    u = (rng.standard_normal((2, 2, g.size)) + 1j * rng.standard_normal((2, 2, g.size))).astype(dtype)
+    back = host(filt.backward(ha.Field(dev(u), g)))
+    refb = o.fourier_filter(u.astype(np.complex128), og(g), tf, q, adjoint=True)
+    e, eb = peak_rel_field(out, ref), peak_rel_field(back, refb)
+    say('matrix filter non-pow2 %s q=%d %s: fwd %.2e adj %.2e' % (dims, q, dtype.__name__, e, eb))
+    assert e < tol and eb < tol
