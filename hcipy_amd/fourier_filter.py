@@ -263,6 +263,23 @@ class FourierFilter:
         p, q = int(tf.shape[0]), int(tf.shape[1])
         lead = tuple(t.shape[:-1])
         batch = int(np.prod(lead)) if lead else 1
+        NY, NX = self.internal_grid.shape
+        if not (_pow2(NX) and _pow2(NY)):
+            # one forward transform of E, every element's product in one pass (conj(tf) for the
+            # adjoint), one batched backward transform
+            if self._fft is None:
+                self._fft = FastFourierTransform(self.input_grid, self.q)
+            F = self._fft.forward(Field(t, self.input_grid)).as_subclass(torch.Tensor)
+            n_int = F.shape[-1]
+            Fb = F.reshape((1, 1) + tuple(F.shape)).expand((p, q) + tuple(F.shape)).contiguous()
+            key = (t.device, cdt, 'elements')
+            J = self._tf_cache.get(key)
+            if J is None:
+                J = tf.to(device=t.device, dtype=cdt).contiguous()
+                self._tf_cache[key] = J
+            Jb = J.reshape((p, q) + (1,) * len(lead) + (n_int,)).expand(Fb.shape).contiguous()
+            G = pointwise.multiply(Fb, Jb, conjugate=adjoint)
+            return Field(self._fft.backward(Field(G, self.internal_grid)).as_subclass(torch.Tensor), self.input_grid)
         out = torch.empty((p, q) + tuple(t.shape), dtype=cdt, device=t.device)
         plans = [self._element_plan(cdt, t.device, i, j) for i in range(p) for j in range(q)]
         outs = [out[i, j] for i in range(p) for j in range(q)]

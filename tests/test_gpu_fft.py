@@ -198,6 +198,26 @@ def test_matrix_filter_capi_vs_element_filters():
 
 
 @pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
+def test_matrix_filter_elements_non_pow2(dtype, tol):
+    """FourierFilter.apply_elements on a 34^2 padded grid (not a power of two): one forward
+    transform, the element products, one batched backward transform; each element filter against
+    the oracle, both directions, with a batch."""
+    g = ha.make_pupil_grid(17, 2)
+    internal = o.make_fft_grid(og(g), 2)
+    rng = np.random.default_rng(14)
+    tf = rng.standard_normal((2, 2, internal.size)) + 1j * rng.standard_normal((2, 2, internal.size))
+    f = (rng.standard_normal((3, g.size)) + 1j * rng.standard_normal((3, g.size))).astype(dtype)
+    filt = ha.FourierFilter(g, dev(tf), 2)
+    for adjoint in (False, True):
+        out = host(filt.apply_elements(ha.Field(dev(f), g), adjoint=adjoint))
+        assert out.shape == (2, 2, 3, g.size)
+        for i in range(2):
+            for j in range(2):
+                ref = o.fourier_filter(f.astype(np.complex128), og(g), tf[i, j], q=2, adjoint=adjoint)
+                assert peak_rel_field(out[i, j], ref) < tol, (i, j, adjoint)
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
 def test_matrix_filter_elements_shared_passes(dtype, tol):
     """FourierFilter.apply_elements (hcipy_filter_execute_multi: one row pass and one forward
     column FFT shared by every element filter) against each element filter of the oracle, both
