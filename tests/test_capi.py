@@ -112,3 +112,27 @@ def test_zoom_fft_pair_entry_point_checks_the_grids():
     assert lib.hcipy_zoom_plan_create_fft(ctypes.byref(h), 1, n, d_in, z, n, d_out, z, ci(0), 1, 0, None) == 1
     assert lib.hcipy_zoom_plan_create_fft(ctypes.byref(h), 1, n, d_in, z, n, d_out, z, ci(17), 1, 0, None) == 1
     assert b'is not 2 pi / 17' in lib.hcipy_get_last_error()
+
+
+@pytest.mark.parametrize('dims,q,fov,shift', [([64, 64], 2, 1, 0), ([24, 16], 1.234, 0.6, 0.1), ([65], 3, 0.8, 0)])
+def test_fft_class_hands_the_zoom_plan_its_fft_lengths(dims, q, fov, shift):
+    """FastFourierTransform on the zoom backend passes the internal grid's dims as fft_n, and
+    those grids pass the plan's delta_in * delta_out = 2 pi / fft_n check (here the call then
+    stops at device selection, with no GPU, not at the grid check)."""
+    import hcipy_amd as ha
+    g = ha.make_uniform_grid(dims, 1.0)
+    ft = ha.FastFourierTransform(g, q, fov, shift)
+    assert ft.method == 'zft'
+    assert ft._impl._fft_n == [int(n) for n in ft.internal_grid.dims]
+    lib = _lib.load()
+    nd = g.ndim
+    ci, cd = ctypes.c_int * nd, ctypes.c_double * nd
+    out = ft.output_grid
+    h = ctypes.c_void_p()
+    st = lib.hcipy_zoom_plan_create_fft(ctypes.byref(h), nd, ci(*[int(v) for v in g.dims]), cd(*g.delta), cd(*g.zero),
+                                        ci(*[int(v) for v in out.dims]), cd(*out.delta), cd(*out.zero),
+                                        ci(*ft._impl._fft_n), 1, 0, None)
+    if st != 0:
+        assert b'is not 2 pi' not in lib.hcipy_get_last_error()
+    else:
+        lib.hcipy_zoom_plan_destroy(h)
