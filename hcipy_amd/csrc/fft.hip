@@ -45,6 +45,7 @@ struct RowCfg {
   static constexpr int LINE = hr_lds_points(N) + 1;
   static constexpr bool TW_LDS = sizeof(T) == 4;
   static constexpr size_t LDS_BYTES = ((size_t)LINES * LINE + (TW_LDS ? N : 0)) * 2 * sizeof(T);
+  static_assert(LDS_BYTES <= 160 * 1024, "row pass exceeds the 160 KiB of LDS per CU");
 };
 
 template <typename T, int N, int WG>
@@ -140,6 +141,7 @@ struct ColCfg {
   static constexpr int NT = C * P;
   static constexpr bool TW_LDS = false;
   static constexpr size_t LDS_BYTES = (size_t)C * LINE * 2 * sizeof(T);
+  static_assert(LDS_BYTES <= 160 * 1024, "column pass exceeds the 160 KiB of LDS per CU");
 };
 
 // pass B: column FFT, transfer function, inverse column FFT, crop to the centred ny rows.  C
@@ -268,6 +270,7 @@ struct MimoCfg {
   static constexpr int C = C0 > 16 ? 16 : (C0 < 1 ? 1 : C0);
   static constexpr int NT = C * P;
   static constexpr size_t LDS_BYTES = (size_t)C * LINE * 2 * sizeof(T);
+  static_assert(LDS_BYTES <= 160 * 1024, "column pass exceeds the 160 KiB of LDS per CU");
 };
 
 template <typename T, int NY, int Q>
@@ -485,7 +488,7 @@ struct FilterPlan {
 };
 
 size_t cbytes(int dtype) { return dtype == HCIPY_DTYPE_C64 ? 8 : 16; }
-bool pow2(int n) { return n >= 4 && n <= 4096 && (n & (n - 1)) == 0; }
+bool pow2(int n) { return n >= 4 && n <= 8192 && (n & (n - 1)) == 0; }
 
 template <typename T, int N>
 int launch_rows_fwd(const FilterPlan& p, const void* in, void* w1, int64_t batch, hipStream_t s) {
@@ -571,6 +574,7 @@ int launch_cols_mimo(const FilterPlan& p, const void* w1, void* w2, const TfMatr
     case 1024: CALL(1024); break;  \
     case 2048: CALL(2048); break;  \
     case 4096: CALL(4096); break;  \
+    case 8192: CALL(8192); break;  \
     default: break;                \
   }
 
@@ -696,7 +700,7 @@ int hcipy_filter_plan_create(hcipy_filter_plan* plan, int nx, int ny, int NX, in
   HCIPY_REQUIRE(plan != nullptr, "plan pointer is NULL");
   *plan = nullptr;
   HCIPY_REQUIRE(dtype == HCIPY_DTYPE_C64 || dtype == HCIPY_DTYPE_C128, "filter dtype must be complex");
-  HCIPY_REQUIRE(pow2(NX) && pow2(NY), "internal grid %d x %d: FFT kernels need powers of two in [4, 4096]", NX, NY);
+  HCIPY_REQUIRE(pow2(NX) && pow2(NY), "internal grid %d x %d: FFT kernels need powers of two in [4, 8192]", NX, NY);
   HCIPY_REQUIRE(nx >= 1 && ny >= 1 && nx <= NX && ny <= NY, "input %d x %d does not fit the internal grid", nx, ny);
   hcipy_filter_plan_s* h = new (std::nothrow) hcipy_filter_plan_s();
   HCIPY_REQUIRE(h != nullptr, "out of host memory");

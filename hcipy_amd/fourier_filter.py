@@ -34,7 +34,7 @@ class NearFieldTransferFunction:
 
 
 def _pow2(n):
-    return 4 <= n <= 4096 and (n & (n - 1)) == 0
+    return 4 <= n <= 8192 and (n & (n - 1)) == 0
 
 
 class _FilterPlan:

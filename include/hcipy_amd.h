@@ -344,7 +344,7 @@ int hcipy_dm_table_surface_adjoint(const hcipy_dm_table_desc* desc, const double
 
 typedef struct hcipy_filter_plan_s* hcipy_filter_plan;
 
-/* nx, ny: field size; NX, NY: internal (padded) size, powers of two in [4, 4096].
+/* nx, ny: field size; NX, NY: internal (padded) size, powers of two in [4, 8192].
  * dtype: HCIPY_DTYPE_C64 or HCIPY_DTYPE_C128 (fields and transfer function). */
 int hcipy_filter_plan_create(hcipy_filter_plan* plan, int nx, int ny, int NX, int NY, int dtype, void* stream);
 

@@ -259,3 +259,28 @@ def test_matrix_transfer_function_non_pow2(dims, q, dtype, tol):
     e, eb = peak_rel_field(out, ref), peak_rel_field(back, refb)
     say('matrix filter non-pow2 %s q=%d %s: fwd %.2e adj %.2e' % (dims, q, dtype.__name__, e, eb))
     assert e < tol and eb < tol
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
+@pytest.mark.parametrize('dims', [[4096, 8], [8, 4096]])
+def test_filter_8192_point_lines(dims, dtype, tol):
+    """8192-point row (4096 x 8 input, q = 2: 8192 x 16) and column (8 x 4096: 16 x 8192) passes of
+    the filter kernels: a random scalar transfer function and a 2 x 2 matrix one against the
+    oracle, forward and adjoint."""
+    g = ha.make_uniform_grid(dims, [1.0, dims[1] / dims[0]])
+    internal = o.make_fft_grid(og(g), 2)
+    rng = np.random.default_rng(15)
+    f = (rng.standard_normal((2, g.size)) + 1j * rng.standard_normal((2, g.size))).astype(dtype)
+    tf = rng.standard_normal(internal.size) + 1j * rng.standard_normal(internal.size)
+    filt = ha.FourierFilter(g, lambda gg: ha.Field(dev(tf), gg), 2)
+    for adjoint in (False, True):
+        out = host(filt.backward(ha.Field(dev(f), g)) if adjoint else filt.forward(ha.Field(dev(f), g)))
+        ref = o.fourier_filter(f.astype(np.complex128), og(g), tf, q=2, adjoint=adjoint)
+        assert peak_rel_field(out, ref) < tol, adjoint
+    tfm = rng.standard_normal((2, 2, internal.size)) + 1j * rng.standard_normal((2, 2, internal.size))
+    fm = ha.FourierFilter(g, dev(tfm), 2)
+    out = host(fm.forward(ha.Field(dev(f), g)))
+    ref = o.fourier_filter(f.astype(np.complex128), og(g), tfm, q=2)
+    e = peak_rel_field(out, ref)
+    say('8192-point filter %s %s: matrix %.2e' % (dims, dtype.__name__, e))
+    assert e < tol
