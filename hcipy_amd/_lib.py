@@ -89,6 +89,7 @@ SIGNATURES = {
     'hcipy_mft_group_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
     'hcipy_mft_group_destroy': (ctypes.c_int, [_vp]),
     'hcipy_mft_execute_grouped': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    'hcipy_mft_group_release_captures': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     'hcipy_mft_autotune': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_int, _vp, ctypes.c_size_t,
                                           ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp]),
     'hcipy_mft_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,

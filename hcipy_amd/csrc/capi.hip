@@ -26,6 +26,14 @@ int check_hip(hipError_t e, const char* what) {
   return kHipError;
 }
 
+int refuse_under_capture(hipStream_t stream, const char* what) {
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  HCIPY_HIP(hipStreamIsCapturing(stream, &capture));
+  HCIPY_REQUIRE(capture == hipStreamCaptureStatusNone,
+                "%s: the stream is capturing a graph; create the plan before the capture starts", what);
+  return kOk;
+}
+
 int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s);
 int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
                          int64_t tiles, hipStream_t s);
@@ -331,7 +339,8 @@ struct hcipy_mft_plan_s {
 // Layout: StageArgs[2 * cap] | int64 tile starts[2 * (cap + 4)].
 // Launches recorded under stream capture copy their table from a snapshot of their own (the copy
 // node reads the host buffer at every replay, so a shared mirror would hand every replay the
-// last table written): `snaps` holds `snap_count` tables, each used by one captured call.
+// last table written): `snaps` holds `snap_count` tables; captured calls take them in turn until
+// hcipy_mft_group_release_captures hands them all back.
 struct hcipy_mft_group_s {
   int cap = 0;
   int device = 0;
@@ -375,6 +384,7 @@ int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const d
   }
   const int64_t lim = (int64_t)1 << 24;
   HCIPY_REQUIRE(nx < lim && ny < lim && nu < lim && nv < lim, "grid dimension too large");
+  if (int st = refuse_under_capture(static_cast<hipStream_t>(stream), "hcipy_mft_plan_create")) return st;
 
   hcipy_mft_plan_s* h = new (std::nothrow) hcipy_mft_plan_s();
   HCIPY_REQUIRE(h != nullptr, "out of host memory");
@@ -564,6 +574,13 @@ int hcipy_mft_group_destroy(hcipy_mft_group group) {
   return kOk;
 }
 
+int hcipy_mft_group_release_captures(hcipy_mft_group group, int* held) {
+  HCIPY_REQUIRE(group != nullptr, "NULL group");
+  if (held != nullptr) *held = group->snap_used;
+  group->snap_used = 0;
+  return kOk;
+}
+
 int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item* items, int n_items, void* stream) {
   HCIPY_REQUIRE(group != nullptr && items != nullptr, "NULL argument");
   HCIPY_REQUIRE(n_items > 0 && n_items <= group->cap, "%d items for a group of capacity %d", n_items, group->cap);
@@ -607,7 +624,9 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   char* src = group->host;
   if (capture != hipStreamCaptureStatusNone) {
     HCIPY_REQUIRE(group->snap_used < group->snap_count,
-                  "this group has served its %d launches under stream capture; use another group", group->snap_count);
+                  "all %d capture snapshots of this group are taken; call hcipy_mft_group_release_captures once "
+                  "the graphs that recorded this group are no longer replayed, or use another group",
+                  group->snap_count);
     src = group->snaps + (size_t)group->snap_used * group->bytes;
     ++group->snap_used;
   } else if (group->pending && hipEventQuery(group->copied) == hipErrorNotReady) {

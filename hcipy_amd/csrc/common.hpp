@@ -31,6 +31,10 @@ void set_last_error(const char* fmt, ...);
 // Returns kHipError (and records the message) if `e` is not hipSuccess.
 int check_hip(hipError_t e, const char* what);
 
+// Plan creation allocates device memory and waits on the stream, which a graph capture cannot
+// record: returns kInvalidValue with a message naming `what` when `stream` is capturing.
+int refuse_under_capture(hipStream_t stream, const char* what);
+
 #define HCIPY_HIP(expr)                                        \
   do {                                                         \
     int _st = ::hcipy::check_hip((expr), #expr);               \

@@ -409,6 +409,7 @@ static int zoom_plan_create(hcipy_zoom_plan* plan, int ndim, const int* n_in, co
                     delta_in[d] * delta_out[d], fft_n[d]);
     }
   }
+  if (int st0 = refuse_under_capture(static_cast<hipStream_t>(stream), "hcipy_zoom_plan_create")) return st0;
   hcipy_zoom_plan_s* hp = new (std::nothrow) hcipy_zoom_plan_s();
   HCIPY_REQUIRE(hp != nullptr, "out of host memory");
   ZoomPlan& p = hp->p;

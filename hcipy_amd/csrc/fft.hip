@@ -485,6 +485,7 @@ struct FilterPlan {
   void* tf = nullptr;
   void* twx = nullptr;
   void* twy = nullptr;
+  bool tf_set = false;  // the transfer function is undefined until set (no memset at creation)
 };
 
 size_t cbytes(int dtype) { return dtype == HCIPY_DTYPE_C64 ? 8 : 16; }
@@ -702,6 +703,7 @@ int hcipy_filter_plan_create(hcipy_filter_plan* plan, int nx, int ny, int NX, in
   HCIPY_REQUIRE(dtype == HCIPY_DTYPE_C64 || dtype == HCIPY_DTYPE_C128, "filter dtype must be complex");
   HCIPY_REQUIRE(pow2(NX) && pow2(NY), "internal grid %d x %d: FFT kernels need powers of two in [4, 8192]", NX, NY);
   HCIPY_REQUIRE(nx >= 1 && ny >= 1 && nx <= NX && ny <= NY, "input %d x %d does not fit the internal grid", nx, ny);
+  if (int st0 = refuse_under_capture(static_cast<hipStream_t>(stream), "hcipy_filter_plan_create")) return st0;
   hcipy_filter_plan_s* h = new (std::nothrow) hcipy_filter_plan_s();
   HCIPY_REQUIRE(h != nullptr, "out of host memory");
   FilterPlan& p = h->p;
@@ -723,7 +725,6 @@ int hcipy_filter_plan_create(hcipy_filter_plan* plan, int nx, int ny, int NX, in
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)ceil_div(NY, 256)), dim3(256), 0, s, NY, dtype, p.twy);
     st = check_hip(hipGetLastError(), "twiddle_kernel");
   }
-  if (!st) st = check_hip(hipMemsetAsync(p.tf, 0, (size_t)NX * NY * cbytes(dtype), s), "hipMemsetAsync");
   if (st) {
     hcipy_filter_plan_destroy(h);
     return st;
@@ -744,11 +745,13 @@ int hcipy_filter_plan_destroy(hcipy_filter_plan plan) {
 int hcipy_filter_set_transfer_function(hcipy_filter_plan plan, const void* tf_centred, int tf_dtype, void* stream) {
   HCIPY_REQUIRE(plan != nullptr && tf_centred != nullptr, "NULL argument");
   HCIPY_REQUIRE(tf_dtype == HCIPY_DTYPE_C64 || tf_dtype == HCIPY_DTYPE_C128, "transfer function must be complex");
-  const FilterPlan& p = plan->p;
+  FilterPlan& p = plan->p;
   const int64_t total = (int64_t)p.NX * p.NY;
   hipLaunchKernelGGL(ifftshift_cast_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256),
                      0, static_cast<hipStream_t>(stream), tf_centred, tf_dtype, p.tf, p.dtype, p.NX, p.NY);
-  return check_hip(hipGetLastError(), "ifftshift_cast_kernel");
+  HCIPY_HIP(hipGetLastError());
+  p.tf_set = true;
+  return kOk;
 }
 
 int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, double delta_x, double zero_x,
@@ -757,14 +760,16 @@ int hcipy_filter_nearfield_transfer_function(hcipy_filter_plan plan, int kind, d
   HCIPY_REQUIRE(plan != nullptr, "NULL plan");
   HCIPY_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (angular spectrum) or 1 (Fresnel)");
   HCIPY_REQUIRE(oversampling >= 1 && oversampling <= 16, "oversampling out of range");
-  const FilterPlan& p = plan->p;
+  FilterPlan& p = plan->p;
   // Fresnel phase factor exp(i k z), computed on the host in FP64 as the reference does.
   const double pf_re = cos(wavenumber * distance), pf_im = sin(wavenumber * distance);
   const int64_t total = (int64_t)p.NX * p.NY;
   hipLaunchKernelGGL(nearfield_tf_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), kind, p.NX, p.NY, delta_x, zero_x, delta_y, zero_y, wavenumber,
                      distance, oversampling, pf_re, pf_im, p.tf, p.dtype, 0);
-  return check_hip(hipGetLastError(), "nearfield_tf_kernel");
+  HCIPY_HIP(hipGetLastError());
+  p.tf_set = true;
+  return kOk;
 }
 
 int hcipy_nearfield_transfer_function(int kind, int nx, int ny, double delta_x, double zero_x, double delta_y,
@@ -813,6 +818,7 @@ int hcipy_filter_execute(hcipy_filter_plan plan, int adjoint, const void* in, vo
                          void* workspace, size_t workspace_bytes, void* stream) {
   HCIPY_REQUIRE(plan != nullptr, "NULL plan");
   HCIPY_REQUIRE(batch >= 0, "batch out of range");
+  HCIPY_REQUIRE(plan->p.tf_set, "the filter plan has no transfer function yet");
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(in != nullptr && out != nullptr && workspace != nullptr, "NULL pointer");
   size_t need = 0;
@@ -855,6 +861,7 @@ int hcipy_filter_execute_multi(const hcipy_filter_plan* plans, int k, int adjoin
   const FilterPlan* ps[kMaxMultiTf];
   for (int t = 0; t < k; ++t) {
     HCIPY_REQUIRE(plans[t] != nullptr && outs[t] != nullptr, "NULL plan or output %d", t);
+    HCIPY_REQUIRE(plans[t]->p.tf_set, "filter plan %d has no transfer function yet", t);
     ps[t] = &plans[t]->p;
     const FilterPlan& a = *ps[0];
     const FilterPlan& b = *ps[t];
@@ -907,6 +914,7 @@ int hcipy_filter_execute_matrix(const hcipy_filter_plan* plans, int n_out, int n
   TfMatrix m{};
   for (int t = 0; t < n_out * n_in; ++t) {
     HCIPY_REQUIRE(plans[t] != nullptr, "NULL plan %d", t);
+    HCIPY_REQUIRE(plans[t]->p.tf_set, "filter plan %d has no transfer function yet", t);
     const FilterPlan& a = plans[0]->p;
     const FilterPlan& b = plans[t]->p;
     HCIPY_REQUIRE(a.nx == b.nx && a.ny == b.ny && a.NX == b.NX && a.NY == b.NY && a.dtype == b.dtype &&

@@ -288,6 +288,14 @@ class MftGroup:
         _lib.check(st, 'hcipy_mft_execute_grouped')
         return outs
 
+    def release_captures(self):
+        """Returns the group's capture snapshots (``hcipy_mft_group_release_captures``); graphs that
+        recorded this group earlier must not be replayed afterwards.  Returns how many were held."""
+        held = ctypes.c_int()
+        _lib.check(_lib.load().hcipy_mft_group_release_captures(self.handle, ctypes.byref(held)),
+                   'hcipy_mft_group_release_captures')
+        return held.value
+
     def close(self):
         if getattr(self, 'handle', None) is not None and _lib._lib is not None:
             _lib.load().hcipy_mft_group_destroy(self.handle)

@@ -130,8 +130,10 @@ int hcipy_mft_execute(hcipy_mft_plan plan, int direction, const void* in, void* 
  * hcipy_mft_execute_grouped).  Calls on one group are ordered on one stream.  Items that read the
  * same input with the same rows and contraction length share one data split (split-FP16).
  * A call recorded under stream capture copies its table from a pinned snapshot of its own, so every
- * replay sees the table of its capture; a group holds up to 64 such snapshots over its lifetime
- * (fewer for very large capacities) and then refuses further captured calls. */
+ * replay sees the table of its capture.  A group holds 64 snapshots (fewer for very large
+ * capacities), taken in turn by captured calls; once all are taken a captured call fails until
+ * hcipy_mft_group_release_captures returns them.  Graphs that recorded calls of a group must not be
+ * replayed after its snapshots are released or the group is destroyed (they read its tables). */
 typedef struct hcipy_mft_group_s* hcipy_mft_group;
 typedef struct {
   hcipy_mft_plan plan;
@@ -147,6 +149,10 @@ typedef struct {
 int hcipy_mft_group_create(hcipy_mft_group* group, int capacity);
 int hcipy_mft_group_destroy(hcipy_mft_group group);
 int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item* items, int n_items, void* stream);
+/* Hands every capture snapshot of the group back (held, may be NULL: how many were taken).  The
+ * caller states that no graph recorded before this call is replayed again (destroy or re-capture
+ * them).  Host-only; no counterpart in the reference, whose loop over instances is not captured. */
+int hcipy_mft_group_release_captures(hcipy_mft_group group, int* held);
 
 /* Output kinds of hcipy_mft_execute_ex. */
 #define HCIPY_OUTPUT_FIELD 0         /* complex field in the plan's dtype (complex128 / complex64) */

@@ -170,3 +170,80 @@ def test_zoom_filter_and_vector_vortex_replay_in_graphs():
         say('zoom/filter/vector-vortex replay', r)
         for a, b in zip(got, ref):
             assert torch.equal(a, b)
+
+
+def test_plan_creation_under_capture_raises_cleanly():
+    """Plan creation allocates device memory and waits on the stream, which a graph cannot record:
+    the MFT, filter and zoom-FFT plan constructors refuse a capturing stream with a clear error
+    before touching the device, and the same objects then build and run eagerly."""
+    from hcipy_amd import _lib
+    from hcipy_amd.fourier import _MftPlan, _ZoomPlan
+    from hcipy_amd.fourier_filter import _FilterPlan
+
+    dev = torch.device('cuda', 0)
+    pupil = ha.make_pupil_grid(64)
+    focal = ha.make_focal_grid(4, 8)
+    tf = torch.exp(1j * torch.linspace(0, 1, 128 * 128, dtype=torch.float64)).to(torch.complex64).cuda()
+    filt = ha.FourierFilter(pupil, tf, q=2)
+    torch.cuda.synchronize()
+    makers = [('mft', lambda: _MftPlan(pupil, focal, 'f16x2', dev)),
+              ('zoom', lambda: _ZoomPlan(pupil, focal, torch.complex64, 'fp64', dev)),
+              ('filter', lambda: _FilterPlan(filt, torch.complex64, dev, tf=tf))]
+    for name, make in makers:
+        say('capture refusal', name)
+        g = torch.cuda.CUDAGraph()
+        with pytest.raises(_lib.HcipyError, match='before the capture starts'):
+            with torch.cuda.graph(g):
+                make()
+        del g
+        torch.cuda.synchronize()
+        plan = make()  # eager creation still works
+        torch.cuda.synchronize()
+        del plan
+    torch.cuda.synchronize()
+    x = torch.ones(pupil.size, dtype=torch.complex64, device='cuda')
+    y = ha.MatrixFourierTransform(pupil, focal, precision='f16x2').forward(ha.Field(x, pupil))
+    torch.cuda.synchronize()
+    assert torch.isfinite(torch.view_as_real(y.as_subclass(torch.Tensor))).all()
+
+
+def test_group_capture_snapshots_are_released_on_request():
+    """A group serves 64 captured calls, refuses the next one with a clear error, and serves again
+    after release_captures(); every graph replays equal to the eager result."""
+    from hcipy_amd import _lib
+    from hcipy_amd.fourier import MftGroup
+
+    pupil = ha.make_pupil_grid(64)
+    focal = ha.make_focal_grid(4, 8)
+    mft = ha.MatrixFourierTransform(pupil, focal, precision='f16x2')
+    g0 = torch.Generator().manual_seed(3)
+    x = torch.complex(torch.randn(2, pupil.size, generator=g0), torch.randn(2, pupil.size, generator=g0)).cuda()
+    group = MftGroup(capacity=2)
+    calls = [(mft, _lib.FORWARD, x[0], 1.0, _lib.OUTPUT_FIELD), (mft, _lib.FORWARD, x[1], 0.5j, _lib.OUTPUT_INTENSITY)]
+    ref = [o.clone() for o in group.execute(calls)]
+    torch.cuda.synchronize()
+
+    def capture_and_check(i):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            outs = group.execute(calls)
+        for _ in range(2):
+            g.replay()
+            torch.cuda.synchronize()
+            for a, b in zip(outs, ref):
+                assert torch.equal(a, b), i
+        del g, outs
+
+    for i in range(64):
+        capture_and_check(i)
+    say('64 captured group calls replayed')
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(_lib.HcipyError, match='capture snapshots'):
+        with torch.cuda.graph(g):
+            group.execute(calls)
+    del g
+    torch.cuda.synchronize()
+    assert group.release_captures() == 64
+    for i in range(6):
+        capture_and_check(64 + i)
+    group.close()
