@@ -96,6 +96,8 @@ SIGNATURES = {
                                          _vp, ctypes.c_size_t, _vp]),
     'hcipy_mft_execute_ex': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, _vp, ctypes.c_size_t, _vp]),
+    'hcipy_mft_execute_noalloc': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_int, _vp]),
     'hcipy_mft_profile': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                          _vp]),

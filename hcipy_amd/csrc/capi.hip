@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -354,7 +355,34 @@ struct hcipy_mft_group_s {
   bool pending = false;
 };
 
+// Workspace of hcipy_mft_execute_noalloc: a file-scope device array, one copy per device that
+// loads the library, held until the process ends.  Host code never uses its name as an address:
+// each device's copy is looked up once with hipGetSymbolAddress (with that device current) and
+// only that pointer is handed to kernels.  Every call on one device shares that device's copy, so
+// such calls must be ordered on one stream (overlapping calls race).  The stage kernels and the
+// split write every workspace element they later read, so nothing needs clearing between calls.
+static __device__ __align__(256) unsigned char scratch[HCIPY_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on first use per device).
+int scratch_on_device(int device, void** ptr) {
+  HCIPY_REQUIRE(device >= 0 && device < kMaxDevices, "device %d beyond the %d the library tracks", device,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] == nullptr) {
+    void* p = nullptr;
+    HCIPY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HCIPY_REQUIRE(p != nullptr, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return kOk;
+}
+
 // Tables kept for launches recorded under stream capture (at most 64 MiB per group).
 constexpr size_t kGroupCaptureSlots = 64;
 
@@ -501,6 +529,30 @@ int hcipy_mft_execute_ex(hcipy_mft_plan plan, int direction_, const void* in, vo
   HCIPY_HIP(hipGetDevice(&cur));
   HCIPY_REQUIRE(cur == pl.device, "plan was created on device %d but device %d is current", pl.device, cur);
   return run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace),
+                       static_cast<hipStream_t>(stream), nullptr, output);
+}
+
+int hcipy_mft_execute_noalloc(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch,
+                              double alpha_re, double alpha_im, int output, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
+  HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
+  HCIPY_REQUIRE(batch >= 0, "negative batch");
+  if (batch == 0) return kOk;
+  HCIPY_REQUIRE(in != nullptr && out != nullptr, "NULL data pointer");
+  const Plan& pl = plan->p;
+  const Ws w = workspace_layout(pl, direction(pl, direction_), batch);
+  if (w.total > (size_t)HCIPY_NOALLOC_SCRATCH_BYTES) {
+    set_last_error("hcipy_mft_execute_noalloc: %lld fields need %zu bytes of workspace, above the %d-byte static "
+                   "scratch; use hcipy_mft_execute_ex with a caller workspace or a smaller batch",
+                   (long long)batch, w.total, (int)HCIPY_NOALLOC_SCRATCH_BYTES);
+    return kOutOfRange;
+  }
+  int cur = -1;
+  HCIPY_HIP(hipGetDevice(&cur));
+  HCIPY_REQUIRE(cur == pl.device, "plan was created on device %d but device %d is current", pl.device, cur);
+  void* ws = nullptr;
+  if (int st = scratch_on_device(cur, &ws)) return st;
+  return run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(ws),
                        static_cast<hipStream_t>(stream), nullptr, output);
 }
 

@@ -169,6 +169,20 @@ int hcipy_mft_execute_ex(hcipy_mft_plan plan, int direction, const void* in, voi
                          double alpha_re, double alpha_im, int output, void* workspace, size_t workspace_bytes,
                          void* stream);
 
+/* Bytes of the static device scratch behind hcipy_mft_execute_noalloc (32 MiB per device). */
+#define HCIPY_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+
+/* hcipy_mft_execute_ex without a workspace argument and without any allocation: the split data
+ * and the intermediate go to a static device array of HCIPY_NOALLOC_SCRATCH_BYTES that every
+ * device loading the library holds until the process ends.  No counterpart in the reference; it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Fails with HCIPY_ERROR_OUT_OF_RANGE, launching nothing, when
+ * hcipy_mft_workspace_size(plan, batch) exceeds the scratch (e.g. 1024^2 -> 512^2 split-FP16 fits
+ * batch 2).  All calls on one device share its scratch: calls that overlap on one device race, so
+ * order them on one stream.  The first call on a device looks the scratch up (no allocation). */
+int hcipy_mft_execute_noalloc(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch,
+                              double alpha_re, double alpha_im, int output, void* stream);
+
 /* Runs hcipy_mft_execute_ex once on `stream`, then launches each of its kernels alone `repeats`
  * times back to back between two HIP events, waits, and stores the mean duration per launch in
  * milliseconds of: kernel_ms[0] the data split (split-FP16 only, else ~0), kernel_ms[1] stage 1,
