@@ -2,13 +2,15 @@
 """Throughput of the MFMA matrix Fourier transform: FraunhoferPropagator forward to the focal
 intensity, 1024^2 pupil -> 512^2 focal plane (q = 4, 64 lambda/D), complex64 input, batch 1.
 
-One step = one propagation (data split + two MFMA GEMM stages, |E|^2 formed in the epilogue)
-of a field already resident in HBM.  The timed steps are replayed from HIP graphs of `--chunk`
-propagations each.  Also reported: the same propagation with complex64 / complex128 field
-output and the plain-FP32 MFMA path, each with its peak-relative intensity error against this
-library's FP64 MFMA path (itself parity-tested against the FP64 oracle); the dominant kernel's
-roofline from HIP events; and the CPU baseline (numpy/scipy restatement of the reference's two
-cgemm calls, timed on the host cores).
+One step = one replay of a HIP graph holding `--chunk` propagations (each: data split + two
+MFMA GEMM stages, |E|^2 formed in the epilogue) of a field already resident in HBM, so that the
+driver's short runs still time tens of milliseconds of GPU work; `value` is propagations/s.
+Also reported: the same propagation with complex64 / complex128 field output and the plain-FP32
+MFMA path, each with its peak-relative intensity error against this library's FP64 MFMA path
+and, in the CPU leg, against the FP64 oracle; the dominant kernel's roofline from HIP events;
+the other paths of the hot path (angular spectrum, zoom FFT, Lyot chain with its error, vortex,
+Jones filter, FFT, pointwise and deformable-mirror kernels in GB/s); and the CPU baseline
+(numpy/scipy restatement of the reference's two cgemm calls, timed on the host cores).
 """
 import argparse
 import math
@@ -25,10 +27,10 @@ PEAK_TFLOPS = {'f16x2': 2500.0, 'fp32': 157.3, 'fp64': 78.6}  # MI355X dense MFM
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=2000)
-    ap.add_argument('--warmup', type=int, default=50)
+    ap.add_argument('--steps', type=int, default=200, help='graph replays timed (each --chunk propagations)')
+    ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--precision', default='f16x2', choices=['f16x2', 'fp32'])
-    ap.add_argument('--chunk', type=int, default=50, help='propagations per captured HIP graph')
+    ap.add_argument('--chunk', type=int, default=64, help='propagations per step (one captured HIP graph)')
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-autotune', action='store_true', help='keep the default tile shapes')
@@ -73,6 +75,7 @@ def main():
     wavelength = 1.0
 
     tiles = {}
+    lyot_outs = {}
 
     def setup(precision):
         prop = ha.FraunhoferPropagator(pupil, focal, precision=precision)
@@ -87,42 +90,41 @@ def main():
         return prop, plan, ws, alpha
 
     def time_path(precision, steps, warmup, output):
+        """Seconds per propagation over `steps` steps of `chunk` propagations each (one graph
+        replay per step), after `warmup` untimed steps; HIP-event durations of the three kernels."""
         prop, plan, ws, alpha = setup(precision)
         odt = {_lib.OUTPUT_FIELD: torch.complex64, _lib.OUTPUT_INTENSITY: torch.float32,
                _lib.OUTPUT_FIELD_C128: torch.complex128}[output]
         out = torch.empty(focal.size, dtype=odt, device=dev)
+        chunk = max(1, args.chunk)
 
-        def step():
+        def prop_once():
             plan.execute(_lib.FORWARD, x, out, 1, alpha, ws, output)
 
-        for _ in range(warmup):
-            step()
+        prop_once()
         torch.cuda.synchronize()
-        chunk = max(1, min(args.chunk, steps))
-        graph = None
-        if not args.no_graph:
+        if args.no_graph:
+            def step():
+                for _ in range(chunk):
+                    prop_once()
+        else:
             s = torch.cuda.Stream()
             s.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(s):
-                step()
+                prop_once()
             torch.cuda.current_stream().wait_stream(s)
             torch.cuda.synchronize()
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph):
                 for _ in range(chunk):
-                    step()
-            graph.replay()
-            torch.cuda.synchronize()
-        n_rep = max(1, steps // chunk)
-        steps_done = n_rep * chunk if graph is not None else steps
+                    prop_once()
+            step = graph.replay
+        for _ in range(warmup):
+            step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        if graph is not None:
-            for _ in range(n_rep):
-                graph.replay()
-        else:
-            for _ in range(steps):
-                step()
+        for _ in range(steps):
+            step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         # per-kernel durations from HIP events on the launching stream
@@ -131,7 +133,7 @@ def main():
                                            float(alpha.real), float(alpha.imag), output, _lib.dptr(ws), ws.numel(),
                                            200, kms, _lib.stream_ptr(dev))
         _lib.check(st, 'hcipy_mft_profile')
-        return (t1 - t0) / steps_done, steps_done, [kms[0], kms[1], kms[2]], out.clone()
+        return (t1 - t0) / (steps * chunk), [kms[0], kms[1], kms[2]], out.clone()
 
     # Reference intensity: this library's FP64 MFMA path on the same complex64-rounded input.
     prop64 = ha.FraunhoferPropagator(pupil, focal)
@@ -247,15 +249,43 @@ def main():
             out[key] = {'value': round(1.0 / sec, 1), 'unit': 'transforms/s', 'ms': round(sec * 1e3, 4),
                         'peak_rel_intensity_error': err(o)}
         # Lyot coronagraph chain: 1024^2 pupil -> 256^2 focal-plane mask -> Lyot plane (stop 0.95 D)
-        # -> 512^2 focal intensity, f16x2.
+        # -> 512^2 focal intensity, split-FP16, plain FP32 and FP64 (complex128); the error of each
+        # against the oracle's FP64 chain relative to the unocculted peak is added in the CPU leg.
         lp, lfg, lfocal, lE, lfpm, lstop, _ = workloads.lyot_1024()
-        coro = ha.LyotCoronagraph(lp, ha.Field(torch.as_tensor(lfpm, device=dev), lfg),
-                                  ha.Field(torch.as_tensor(lstop, device=dev), lp), precision='f16x2')
-        lprop = ha.FraunhoferPropagator(lp, lfocal, precision='f16x2')
-        lx = ha.Field(torch.as_tensor(lE.astype(np.complex64), device=dev), lp)
-        sec, _ = time_graph(lambda: lprop.forward_intensity(coro.forward(ha.Wavefront(lx, 1.0))))
-        out['lyot_chain_1024_f16x2'] = {'value': round(1.0 / sec, 1), 'unit': 'chains/s', 'ms': round(sec * 1e3, 4),
-                                        'gflop_per_chain': 11.81}
+        for key, prec, cdt in (('lyot_chain_1024_f16x2', 'f16x2', torch.complex64),
+                               ('lyot_chain_1024_fp32', 'fp32', torch.complex64),
+                               ('lyot_chain_1024_fp64', None, torch.complex128)):
+            coro = ha.LyotCoronagraph(lp, ha.Field(torch.as_tensor(lfpm, device=dev), lfg),
+                                      ha.Field(torch.as_tensor(lstop, device=dev), lp), precision=prec)
+            lprop = ha.FraunhoferPropagator(lp, lfocal, precision=prec)
+            lx = ha.Field(torch.as_tensor(lE.astype(np.complex64), device=dev).to(cdt), lp)
+            sec, lo = time_graph(lambda: lprop.forward_intensity(coro.forward(ha.Wavefront(lx, 1.0))),
+                                 n_chunks=5 if prec is None else 10)
+            lyot_outs[key] = lo.as_subclass(torch.Tensor).clone()
+            out[key] = {'value': round(1.0 / sec, 1), 'unit': 'chains/s', 'ms': round(sec * 1e3, 4),
+                        'gflop_per_chain': 11.81}
+            del coro, lprop
+        # HBM-bound kernels of the chain on the 1024^2 complex64 field (apodizer / mask product,
+        # phase screen, intensity) and the 48 x 48 deformable-mirror surface: GB/s of the bytes
+        # each kernel must read and write.
+        amp = torch.as_tensor(np.random.default_rng(8).random(asp_grid.size), device=dev)
+        phi = torch.as_tensor(np.random.default_rng(9).standard_normal(asp_grid.size), device=dev)
+        xo = torch.empty_like(x)
+        n = x.numel()
+        kernels = {
+            'sub_mul_1024_c64_f64mask': (lambda: ha.pointwise.sub_mul(x, None, amp, out=xo), n * (8 + 8 + 8)),
+            'phase_1024_c64_f64phase': (lambda: ha.pointwise.multiply_phase(x, phi, 0.7), n * (8 + 8 + 8)),
+            'intensity_1024_c64': (lambda: ha.pointwise.intensity(x), n * (8 + 4)),
+        }
+        IFb = ha.make_gaussian_influence_functions(asp_grid, 48, 1.0 / 48)
+        dmb = ha.DeformableMirror(IFb)
+        acts = torch.as_tensor(np.random.default_rng(10).standard_normal(48 * 48) * 1e-9, device=dev)
+        kernels['dm_surface_1024_48x48_f64'] = (lambda: IFb.linear_combination(acts), n * 8 + 48 * 48 * 8)
+        for key, (fn, nbytes) in kernels.items():
+            sec, _ = time_graph(fn, n_chunks=10, chunk=20)
+            out[key] = {'us': round(sec * 1e6, 2), 'gbs': round(nbytes / sec / 1e9, 1), 'bytes': nbytes,
+                        'frac_of_8tbs': round(nbytes / sec / 8e12, 3)}
+        del dmb
         # Charge-2 vortex coronagraph on the 1024^2 pupil: FFT filter (2048^2) + 5 levels of MFTs
         # 1024^2 <-> 128^2 as two grouped launches, f16x2.
         vc = ha.VortexCoronagraph(asp_grid, 2, precision='f16x2')
@@ -345,18 +375,20 @@ def main():
 
     log('timing ...')
     main_out = _lib.OUTPUT_INTENSITY
-    sec_main, steps_done, kms_main, out_main = time_path(args.precision, args.steps, args.warmup, main_out)
+    sec_main, kms_main, out_main = time_path(args.precision, args.steps, args.warmup, main_out)
     err_main = err(out_main)
     variants = {}
+    outs_for_oracle = {'main': out_main}
     for name, prec, output in [('%s_field_c64' % args.precision, args.precision, _lib.OUTPUT_FIELD),
                                ('%s_field_c128' % args.precision, args.precision, _lib.OUTPUT_FIELD_C128),
                                ('fp32_field_c64', 'fp32', _lib.OUTPUT_FIELD),
                                ('fp32_intensity', 'fp32', _lib.OUTPUT_INTENSITY)]:
         if args.no_variants or (prec == args.precision and output == main_out):
             continue
-        sec, _, kms, o = time_path(prec, max(200, args.steps // 4), 20, output)
-        variants[name] = {'value': round(1.0 / sec, 2), 'ms_per_step': round(sec * 1e3, 5),
+        sec, kms, o = time_path(prec, max(20, args.steps // 4), 5, output)
+        variants[name] = {'value': round(1.0 / sec, 2), 'ms_per_propagation': round(sec * 1e3, 5),
                           'peak_rel_intensity_error': err(o), 'kernel_ms': [round(v, 5) for v in kms]}
+        outs_for_oracle[name] = o
     fp32_ref = variants.get('fp32_intensity') or variants.get('fp32_field_c64')
     broadband = None if args.no_variants else time_broadband()
     paths = None
@@ -393,19 +425,50 @@ def main():
                 'algorithmic_flop_per_launch': flops_stage1}
 
     cpu = None
+    oracle_err = None
     if not args.no_cpu_baseline:
         log('cpu baseline ...')
         from oracle import hcipy_oracle as o
+        from threadpoolctl import threadpool_info
+        og_p = o.Grid(pupil.delta, tuple(int(v) for v in pupil.dims), pupil.zero)
+        og_f = o.Grid(focal.delta, tuple(int(v) for v in focal.dims), focal.zero)
+        # error of every timed output against the oracle's FP64 transform of the same
+        # complex64-rounded input (peak-relative intensity)
+        I_or = np.abs(o.fraunhofer_forward(E64.astype(np.complex128), og_p, og_f)) ** 2
+        oracle_err = {}
+        for name, out_t in outs_for_oracle.items():
+            a = out_t.cpu().numpy()
+            Ia = a.astype(np.float64) if not np.iscomplexobj(a) else np.abs(a.astype(np.complex128)) ** 2
+            oracle_err[name] = float(np.abs(Ia - I_or).max() / I_or.max())
         mcpu = o.MftCpu(o.Grid(pupil.delta, tuple(int(v) for v in pupil.dims), pupil.zero),
                         o.Grid(focal.scaled(2 * np.pi).delta, tuple(int(v) for v in focal.dims),
                                focal.scaled(2 * np.pi).zero), 'complex64')
         f = E64.copy()
         t = o.time_it(lambda: mcpu.forward(f), t_max=args.cpu_seconds)
-        cores = len(os.sched_getaffinity(0))
-        cpu = {'value': round(1.0 / t, 3), 'unit': 'propagations/s', 'cores': int(os.environ.get('OPENBLAS_NUM_THREADS', cores)),
+        blas = [i for i in threadpool_info() if i.get('user_api') == 'blas']
+        b0 = blas[0] if blas else {}
+        cpu = {'value': round(1.0 / t, 3), 'unit': 'propagations/s',
+               'cores': int(b0.get('num_threads', len(os.sched_getaffinity(0)))),
                'kind': 'port',
                'sample': 'same 1024^2->512^2 complex64 MFT: scipy cgemm x2 with the reference transpose pattern, '
-                         'precomputed matrices, median of 5 runs of >= %.1f s (reference _time_it)' % args.cpu_seconds}
+                         'precomputed matrices, median of 5 runs of >= %.1f s (reference _time_it)' % args.cpu_seconds,
+               'blas': '%s %s (%s)' % (b0.get('internal_api', '?'), b0.get('version', '?'),
+                                       os.path.basename(b0.get('filepath', '?'))),
+               'affinity_cores': len(os.sched_getaffinity(0)),
+               'OPENBLAS_NUM_THREADS': os.environ.get('OPENBLAS_NUM_THREADS')}
+        if paths is not None and 'lyot_chain_1024_f16x2' in paths:
+            # Lyot-chain error against the oracle's FP64 chain, relative to the unocculted peak
+            lp, lfg, lfocal, lE, lfpm, lstop, lap = workloads.lyot_1024()
+            ogp = o.Grid(lp.delta, tuple(int(v) for v in lp.dims), lp.zero)
+            ogm = o.Grid(lfg.delta, tuple(int(v) for v in lfg.dims), lfg.zero)
+            ogf = o.Grid(lfocal.delta, tuple(int(v) for v in lfocal.dims), lfocal.zero)
+            lE64 = lE.astype(np.complex64).astype(np.complex128)
+            I_l = np.abs(o.fraunhofer_forward(o.lyot_forward(lE64, ogp, ogm, lfpm, lstop), ogp, ogf)) ** 2
+            I_un = np.abs(o.fraunhofer_forward(lE64 * lstop, ogp, ogf)) ** 2
+            for key, It in lyot_outs.items():
+                paths[key]['peak_rel_error_vs_oracle_unocculted'] = float(
+                    np.abs(It.cpu().numpy().astype(np.float64) - I_l).max() / I_un.max())
+            paths['lyot_chain_1024_f16x2']['unocculted_peak_over_coronagraphic_peak'] = float(I_un.max() / I_l.max())
 
     value = 1.0 / sec_main
     res = {
@@ -413,9 +476,11 @@ def main():
         'value': round(value, 2),
         'unit': 'propagations/s',
         'n_gpus': 1,
-        'steps': steps_done,
+        'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': round(sec_main * 1e3, 5),
+        'ms_per_step': round(sec_main * max(1, args.chunk) * 1e3, 5),
+        'propagations_per_step': max(1, args.chunk),
+        'ms_per_propagation': round(sec_main * 1e3, 5),
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
@@ -424,9 +489,11 @@ def main():
         'config': {'workload': 'fraunhofer_mft_1024_to_512_complex64_focal_intensity', 'pupil': [1024, 1024],
                    'focal': [512, 512], 'q': 4, 'num_airy': 64, 'batch': 1, 'precision': args.precision,
                    'output': 'focal intensity (float32) from the FP64 epilogue value',
-                   'graph_chunk': None if args.no_graph else args.chunk,
+                   'step': '%d propagations replayed from one HIP graph' % max(1, args.chunk)
+                           if not args.no_graph else '%d eager propagations' % max(1, args.chunk),
                    'tile_shapes': {k: list(v) for k, v in tiles.items()} or 'default'},
         'peak_rel_intensity_error': err_main,
+        'peak_rel_intensity_error_vs_oracle': oracle_err,
         'speedup_vs_fp32': round(value / fp32_ref['value'], 3) if fp32_ref else None,
         'variants': variants,
         'broadband_grouped': broadband,

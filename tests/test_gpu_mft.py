@@ -75,6 +75,23 @@ def test_fraunhofer_256_to_128_complex128():
     assert peak_rel_intensity(a, b) < 1e-12
 
 
+def test_fraunhofer_1024_to_512_complex128():
+    """1024^2 -> 512^2 (4 spider vanes, 0.05 rad phase), complex128 on the FP64 MFMA path against
+    the oracle's FP64 transform: 1e-12 of the peak field and intensity."""
+    pupil, focal, E = workloads.plain_mft_1024()
+    prop = ha.FraunhoferPropagator(pupil, focal)
+    say('fraunhofer 1024->512 c128')
+    wf = ha.Wavefront(ha.Field(dev(E), pupil), 1.0)
+    a = host(prop(wf).electric_field)
+    assert a.dtype == np.complex128
+    b = o.fraunhofer_forward(E, og(pupil), og(focal))
+    ef, ei = peak_rel_field(a, b), peak_rel_intensity(a, b)
+    say('  field %.3e  intensity %.3e' % (ef, ei))
+    assert ef < 1e-12 and ei < 1e-12
+    back = host(prop.backward(ha.Wavefront(ha.Field(dev(b), focal), 1.0)).electric_field)
+    assert peak_rel_field(back, o.fraunhofer_backward(b, og(pupil), og(focal))) < 1e-12
+
+
 @pytest.mark.parametrize('precision', ['f16x2', 'fp32'])
 def test_fraunhofer_1024_to_512_complex64(precision):
     """1024^2 -> 512^2 complex64 against the FP64 result on the same (complex64) input.
