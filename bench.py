@@ -21,7 +21,7 @@ import sys
 import time
 
 METRIC = 'MFT propagations/s 1024²→512² (split-prec MFMA vs FP32) at ≤1e-7 peak-rel err'
-PEAK_TFLOPS = {'f16x2': 2500.0, 'fp32': 157.3, 'fp64': 78.6}  # MI355X dense MFMA peaks
+PEAK_TFLOPS = {'f16x2': 2500.0, 'bf16x3': 2500.0, 'fp32': 157.3, 'fp64': 78.6}  # MI355X dense MFMA peaks
 
 
 def parse():
@@ -29,7 +29,7 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=200, help='graph replays timed (each --chunk propagations)')
     ap.add_argument('--warmup', type=int, default=10)
-    ap.add_argument('--precision', default='f16x2', choices=['f16x2', 'fp32'])
+    ap.add_argument('--precision', default='f16x2', choices=['f16x2', 'bf16x3', 'fp32'])
     ap.add_argument('--chunk', type=int, default=64, help='propagations per step (one captured HIP graph)')
     ap.add_argument('--no-graph', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
@@ -253,6 +253,7 @@ def main():
         # against the oracle's FP64 chain relative to the unocculted peak is added in the CPU leg.
         lp, lfg, lfocal, lE, lfpm, lstop, _ = workloads.lyot_1024()
         for key, prec, cdt in (('lyot_chain_1024_f16x2', 'f16x2', torch.complex64),
+                               ('lyot_chain_1024_bf16x3', 'bf16x3', torch.complex64),
                                ('lyot_chain_1024_fp32', 'fp32', torch.complex64),
                                ('lyot_chain_1024_fp64', None, torch.complex128)):
             coro = ha.LyotCoronagraph(lp, ha.Field(torch.as_tensor(lfpm, device=dev), lfg),
@@ -381,6 +382,8 @@ def main():
     outs_for_oracle = {'main': out_main}
     for name, prec, output in [('%s_field_c64' % args.precision, args.precision, _lib.OUTPUT_FIELD),
                                ('%s_field_c128' % args.precision, args.precision, _lib.OUTPUT_FIELD_C128),
+                               ('bf16x3_intensity', 'bf16x3', _lib.OUTPUT_INTENSITY),
+                               ('bf16x3_field_c128', 'bf16x3', _lib.OUTPUT_FIELD_C128),
                                ('fp32_field_c64', 'fp32', _lib.OUTPUT_FIELD),
                                ('fp32_intensity', 'fp32', _lib.OUTPUT_INTENSITY)]:
         if args.no_variants or (prec == args.precision and output == main_out):
@@ -484,7 +487,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'fp16x2' if args.precision == 'f16x2' else 'fp32',
+        'dtype': {'f16x2': 'fp16x2', 'bf16x3': 'bf16x3', 'fp32': 'fp32'}[args.precision],
         'data': 'synthetic',
         'config': {'workload': 'fraunhofer_mft_1024_to_512_complex64_focal_intensity', 'pupil': [1024, 1024],
                    'focal': [512, 512], 'q': 4, 'num_airy': 64, 'batch': 1, 'precision': args.precision,

@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(_HERE, 'libhcipy_amd.so')
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_F16X2 = 2
-PRECISIONS = {'fp64': PREC_FP64, 'fp32': PREC_FP32, 'f16x2': PREC_F16X2}
+PREC_BF16X3 = 3
+PRECISIONS = {'fp64': PREC_FP64, 'fp32': PREC_FP32, 'f16x2': PREC_F16X2, 'bf16x3': PREC_BF16X3}
 FORWARD = 0
 BACKWARD = 1
 OUTPUT_FIELD = 0
@@ -40,7 +41,8 @@ TILES_D64X64_W4 = 12
 TILES_D64X64_W8 = 13
 TILES_D64X32_W4 = 14
 TILES_D64X32_W8 = 15
-TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(16))}
+TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(16)),
+             'bf16x3': (0, 1, 4, 7)}
 
 
 class MftGroupItem(ctypes.Structure):

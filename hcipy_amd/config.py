@@ -5,8 +5,8 @@
 TypeError (hcipy/config/config.py:21-55).  Additions of this package:
 
 * ``fourier.mft.precision_complex64``: arithmetic of complex64 MFTs, ``'f16x2'`` (split FP16
-  MFMA with FP64 flushes, the default) or ``'fp32'`` (plain FP32 MFMA).  complex128 always
-  runs in FP64.
+  MFMA with FP64 flushes, the default), ``'bf16x3'`` (three-way split BF16 MFMA) or ``'fp32'``
+  (plain FP32 MFMA).  complex128 always runs in FP64.
 * ``fourier.zfft.precision_complex64``: arithmetic of complex64 zoom FFTs, ``'fp64'`` (FP32
   loads and stores around FP64 chirps, line FFTs and intermediate; the default) or ``'fp32'``.
 * ``execution_time_prediction_coefficients`` (fft, mft, zfft): the reference's values, so that

@@ -41,10 +41,10 @@ int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, 
 int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64);
 void grouped_tile_shape(int cfg, int* bm, int* bn);
 constexpr int kModePadded = 0, kModeAligned = 1, kModeUnaligned = 2;  // BMode of mft_gemm.hip
-int launch_split_rows(const float* in, int64_t ld_in, int K, int64_t rows, _Float16* hi, _Float16* lo,
-                      int64_t ld_out, float* row_scale, hipStream_t s);
-int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, void* out,
-                      void* out_lo, int64_t lda, int rows_pad, hipStream_t s);
+int launch_split_rows(int prec, const float* in, int64_t ld_in, int K, int64_t rows, SplitOut o, int64_t ld_out,
+                      float* row_scale, hipStream_t s);
+int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, SplitOut out,
+                      int64_t lda, int rows_pad, hipStream_t s);
 
 namespace {
 
@@ -57,6 +57,7 @@ int64_t padded_row(int64_t k) { return round_up(k, kRowPad); }
 constexpr int kMaxGridZ = 65535;
 
 size_t elem_size(int prec) { return prec == kF64 ? 8 : (prec == kF32 ? 4 : 2); }
+int n_arrays(int prec) { return split_arrays(prec); }
 
 int ilog2_ceil(int64_t v) {
   int e = 0;
@@ -68,6 +69,7 @@ int ilog2_ceil(int64_t v) {
 struct FMatrix {
   void* hi = nullptr;
   void* lo = nullptr;
+  void* lo2 = nullptr;
   int nc = 0, nk = 0;
   int64_t lda = 0;
   int rows_pad = 0;
@@ -91,19 +93,19 @@ int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, 
   m.lda = padded_row(2 * (int64_t)nk);
   m.rows_pad = (int)round_up((int64_t)nc, 64);
   const size_t bytes = (size_t)m.rows_pad * m.lda * elem_size(pl.prec);
-  HCIPY_HIP(hipMalloc(&m.hi, bytes));
-  pl.matrix_bytes += bytes;
-  if (pl.prec == kF16x2) {
-    HCIPY_HIP(hipMalloc(&m.lo, bytes));
+  void** arrs[3] = {&m.hi, &m.lo, &m.lo2};
+  for (int i = 0; i < n_arrays(pl.prec); ++i) {
+    HCIPY_HIP(hipMalloc(arrs[i], bytes));
     pl.matrix_bytes += bytes;
   }
-  return launch_gen_matrix(a, nc, b, nk, sign, pl.prec, m.hi, m.lo, m.lda, m.rows_pad, s);
+  return launch_gen_matrix(a, nc, b, nk, sign, pl.prec, SplitOut{{m.hi, m.lo, m.lo2}}, m.lda, m.rows_pad, s);
 }
 
 void free_matrix(FMatrix& m) {
   if (m.hi) (void)hipFree(m.hi);
   if (m.lo) (void)hipFree(m.lo);
-  m.hi = m.lo = nullptr;
+  if (m.lo2) (void)hipFree(m.lo2);
+  m.hi = m.lo = m.lo2 = nullptr;
 }
 
 // Geometry of one direction of a 2-D (or 1-D) transform.
@@ -135,7 +137,7 @@ Dir direction(const Plan& pl, int dir) {
 
 // Workspace carve-up for one direction.
 struct Ws {
-  size_t split_hi = 0, split_lo = 0, row_scale = 0, t = 0, z_hi = 0, z_lo = 0, total = 0;
+  size_t split_hi = 0, split_lo = 0, split_lo2 = 0, row_scale = 0, t = 0, z_hi = 0, z_lo = 0, z_lo2 = 0, total = 0;
   int64_t ld1 = 0, n1_pad = 0;
 };
 
@@ -148,10 +150,11 @@ Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
     return at;
   };
   const int64_t rows1 = batch * (pl.ndim == 2 ? d.n1 : 1);
-  if (pl.prec == kF16x2) {
+  if (is_split(pl.prec)) {
     w.ld1 = padded_row(2 * d.k1);
     w.split_hi = take((size_t)rows1 * w.ld1 * 2);
     w.split_lo = take((size_t)rows1 * w.ld1 * 2);
+    if (pl.prec == kBF16x3) w.split_lo2 = take((size_t)rows1 * w.ld1 * 2);
     w.row_scale = take((size_t)rows1 * 4);
     w.t = take((size_t)batch * 4);
   }
@@ -159,7 +162,8 @@ Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
     w.n1_pad = padded_row(2 * d.n1) / 2;  // complex elements per intermediate row (k-tile multiple)
     const size_t zelems = (size_t)batch * d.c1 * 2 * w.n1_pad;  // reals
     w.z_hi = take(zelems * elem_size(pl.prec));
-    if (pl.prec == kF16x2) w.z_lo = take(zelems * 2);
+    if (is_split(pl.prec)) w.z_lo = take(zelems * 2);
+    if (pl.prec == kBF16x3) w.z_lo2 = take(zelems * 2);
   }
   w.total = off;
   return w;
@@ -221,22 +225,25 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   float* row_scale = nullptr;
   float* t = nullptr;
   if (int st0 = mark(0)) return st0;
-  if (pl.prec == kF16x2) {
-    _Float16* hi = reinterpret_cast<_Float16*>(ws + w.split_hi);
-    _Float16* lo = reinterpret_cast<_Float16*>(ws + w.split_lo);
+  if (is_split(pl.prec)) {
+    void* hi = ws + w.split_hi;
+    void* lo = ws + w.split_lo;
+    void* lo2 = pl.prec == kBF16x3 ? ws + w.split_lo2 : nullptr;
     row_scale = reinterpret_cast<float*>(ws + w.row_scale);
     t = reinterpret_cast<float*>(ws + w.t);
     if (shared_split != nullptr) {
-      hi = static_cast<_Float16*>(const_cast<void*>(shared_split->a1.b));
-      lo = static_cast<_Float16*>(const_cast<void*>(shared_split->a1.b_lo));
+      hi = const_cast<void*>(shared_split->a1.b);
+      lo = const_cast<void*>(shared_split->a1.b_lo);
+      lo2 = const_cast<void*>(shared_split->a1.b_lo2);
       row_scale = const_cast<float*>(shared_split->a1.row_scale);
     } else if (only < 0 || only == 0) {
-      int st = launch_split_rows(static_cast<const float*>(in), 2 * d.k1, (int)(2 * d.k1), rows1, hi, lo, w.ld1,
-                                 row_scale, s);
+      int st = launch_split_rows(pl.prec, static_cast<const float*>(in), 2 * d.k1, (int)(2 * d.k1), rows1,
+                                 SplitOut{{hi, lo, lo2}}, w.ld1, row_scale, s);
       if (st != kOk) return st;
     }
     a1.b = hi;
     a1.b_lo = lo;
+    a1.b_lo2 = lo2;
     a1.ldb = w.ld1;
     a1.row_scale = row_scale;
   } else {
@@ -246,6 +253,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   }
   a1.a = d.m1->hi;
   a1.a_lo = d.m1->lo;
+  a1.a_lo2 = d.m1->lo2;
   a1.lda = d.m1->lda;
   a1.M = (int)(2 * d.c1);
   a1.N = (int)rows1;
@@ -274,11 +282,12 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
 
   // Stage 1 writes the intermediate, one (c1 x n1_pad) block per field.
   a1.out = ws + w.z_hi;
-  a1.out_lo = pl.prec == kF16x2 ? ws + w.z_lo : nullptr;
+  a1.out_lo = is_split(pl.prec) ? ws + w.z_lo : nullptr;
+  a1.out_lo2 = pl.prec == kBF16x3 ? ws + w.z_lo2 : nullptr;
   a1.ldo = w.n1_pad;
   a1.fold = (int)d.n1;
   a1.fold_stride = d.c1 * w.n1_pad;
-  a1.out_mode = pl.prec == kF16x2 ? kOutSplit : (pl.prec == kF64 ? kOutC128 : kOutC64);
+  a1.out_mode = is_split(pl.prec) ? kOutSplit : (pl.prec == kF64 ? kOutC128 : kOutC64);
   a1.t_out = t;
   a1.alpha_re = 1.0;
   a1.alpha_im = 0.0;
@@ -291,9 +300,11 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
   a2.tile_cfg = pl.tiles[dir][1];
   a2.a = d.m2->hi;
   a2.a_lo = d.m2->lo;
+  a2.a_lo2 = d.m2->lo2;
   a2.lda = d.m2->lda;
   a2.b = ws + w.z_hi;
-  a2.b_lo = pl.prec == kF16x2 ? ws + w.z_lo : nullptr;
+  a2.b_lo = is_split(pl.prec) ? ws + w.z_lo : nullptr;
+  a2.b_lo2 = pl.prec == kBF16x3 ? ws + w.z_lo2 : nullptr;
   a2.ldb = 2 * w.n1_pad;
   a2.b_zstride = d.c1 * 2 * w.n1_pad;
   a2.M = (int)(2 * d.c2);
@@ -319,6 +330,7 @@ int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t ba
     StageArgs a = a2;
     a.b = static_cast<const char*>(a2.b) + b0 * a2.b_zstride * es;
     if (a2.b_lo) a.b_lo = static_cast<const char*>(a2.b_lo) + b0 * a2.b_zstride * es;
+    if (a2.b_lo2) a.b_lo2 = static_cast<const char*>(a2.b_lo2) + b0 * a2.b_zstride * es;
     a.out = static_cast<char*>(out) + b0 * a2.out_zstride * out_es;
     a.z_scale = t ? t + b0 : nullptr;
     st = launch_stage(pl.prec, a, nz, kModePadded, s);
@@ -403,7 +415,8 @@ int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const d
   HCIPY_REQUIRE(plan != nullptr, "plan pointer is NULL");
   *plan = nullptr;
   HCIPY_REQUIRE(ndim == 1 || ndim == 2, "The input_grid must be one- or two-dimensional (got ndim=%d).", ndim);
-  HCIPY_REQUIRE(precision == kF64 || precision == kF32 || precision == kF16x2, "unknown precision %d", precision);
+  HCIPY_REQUIRE(precision == kF64 || precision == kF32 || precision == kF16x2 || precision == kBF16x3,
+                "unknown precision %d", precision);
   HCIPY_REQUIRE(nx > 0 && nu > 0 && x_in != nullptr && u_out != nullptr, "empty grid");
   if (ndim == 1) {
     HCIPY_REQUIRE(ny == 1 && nv == 1, "1-D transform requires ny == nv == 1");
@@ -656,7 +669,7 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
     // same input as an earlier item, with the same rows, contraction length and split row pitch
     // (for example one pupil field sent to several focal grids), reuses that item's split.
     const Collected* shared = nullptr;
-    if (prec == kF16x2) {
+    if (is_split(prec)) {
       for (int j = 0; j < i && shared == nullptr; ++j) {
         const hcipy_mft_group_item& o = items[j];
         const Plan& pj = o.plan->p;

@@ -9,6 +9,8 @@
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
@@ -58,6 +60,13 @@ inline int64_t ceil_div(int64_t v, int64_t m) { return (v + m - 1) / m; }
 __device__ inline _Float16 f32_to_f16(float f) {
   __asm__ volatile("" : "+v"(f));
   return (_Float16)f;
+}
+
+// float -> bfloat16, round to nearest even (v_cvt_pk_bf16_f32), with the float made opaque as in
+// f32_to_f16 so that a preceding double -> float rounding stays a separate step.
+__device__ inline __bf16 f32_to_bf16(float f) {
+  __asm__ volatile("" : "+v"(f));
+  return (__bf16)f;
 }
 
 // Wave-level reductions over the 64 lanes of a CDNA wavefront.

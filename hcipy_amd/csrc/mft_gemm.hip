@@ -163,7 +163,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   constexpr int NTHR = 64 * WM * WN * WK;
   constexpr int VEC = 16 / (int)sizeof(T);
   constexpr int LDK = BK + VEC;  // 16-byte row padding against LDS bank conflicts
-  constexpr int NARR = TR::SPLIT ? 2 : 1;
+  constexpr int NARR = TR::NARR;
   constexpr int BMC = BM / 2;    // compact constant rows per tile
   constexpr int WTM = BM / WM;
   constexpr int WTN = BN / WN;
@@ -204,10 +204,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   const int M = p.M, N = p.N, K = p.K;
   const int nk = (K + BK - 1) / BK;
 
-  const T* Ag[2] = {static_cast<const T*>(p.a), static_cast<const T*>(p.a_lo)};
-  const T* Bg[2] = {static_cast<const T*>(p.b), static_cast<const T*>(p.b_lo)};
-  if constexpr (NARR == 2) Bg[1] += (int64_t)z * p.b_zstride;
-  Bg[0] += (int64_t)z * p.b_zstride;
+  const T* Ag[3] = {static_cast<const T*>(p.a), static_cast<const T*>(p.a_lo), static_cast<const T*>(p.a_lo2)};
+  const T* Bg[3] = {static_cast<const T*>(p.b), static_cast<const T*>(p.b_lo), static_cast<const T*>(p.b_lo2)};
+#pragma unroll
+  for (int arr = 0; arr < NARR; ++arr) Bg[arr] += (int64_t)z * p.b_zstride;
 
   const bool split_out = (p.out_mode == kOutSplit);
   int item0 = 0;
@@ -306,6 +306,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   // cross products Ah.Bl and Al.Bh in separate accumulators: the three MFMAs of a k-vector
   // are independent and issue back to back
   AccT accx[NXA][TR::SPLIT ? TM : 1][TR::SPLIT ? TN : 1];
+  // second-order cross products A0.B2 + A1.B1 + A2.B0 of the three-way BF16 split
+  constexpr bool THREE = (NARR == 3);
+  AccT accy[THREE ? TM : 1][THREE ? TN : 1];
   double acc64[TR::SPLIT ? TM : 1][TR::SPLIT ? TN : 1][TR::SPLIT ? NACC : 1];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -320,6 +323,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
           for (int q = 0; q < NXA; ++q) accx[q][i][j][r] = 0;
           acc64[i][j][r] = 0.0;
         }
+        if constexpr (THREE) accy[i][j][r] = 0;
       }
     }
 
@@ -331,19 +335,24 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   // buffer while the MFMAs of k-vector q run.
   auto compute_tile = [&](auto qc, const T* base) __attribute__((always_inline)) {
     constexpr int Q = decltype(qc)::value;
-    if constexpr (P == kF16x2) {
-      half8_t fa[2][2][TM], fb[2][2][TN];  // [buffer][hi/lo][tile]
+    if constexpr (P == kF16x2 || P == kBF16x3) {
+      using FragT = typename std::conditional<P == kF16x2, half8_t, bf16x8_t>::type;
+      FragT fa[2][NARR][TM], fb[2][NARR][TN];  // [buffer][split array][tile]
+      auto mfma = [](FragT a, FragT b, AccT c) __attribute__((always_inline)) {
+        if constexpr (P == kF16x2) return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+        else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+      };
       auto read_frags = [&](int buf, int q) __attribute__((always_inline)) {
         const int kofs = (q * WK + wk) * KVEC + (lane / MT) * KPL;
 #pragma unroll
-        for (int arr = 0; arr < 2; ++arr) {
+        for (int arr = 0; arr < NARR; ++arr) {
           const T* sA = base + arr * BM * LDK;
           const T* sB = base + ABUF + arr * BN * LDK;
 #pragma unroll
-          for (int i = 0; i < TM; ++i) fa[buf][arr][i] = *reinterpret_cast<const half8_t*>(&sA[arow[i] * LDK + kofs]);
+          for (int i = 0; i < TM; ++i) fa[buf][arr][i] = *reinterpret_cast<const FragT*>(&sA[arow[i] * LDK + kofs]);
 #pragma unroll
           for (int j = 0; j < TN; ++j)
-            fb[buf][arr][j] = *reinterpret_cast<const half8_t*>(&sB[(wn * WTN + j * MT + lane % MT) * LDK + kofs]);
+            fb[buf][arr][j] = *reinterpret_cast<const FragT*>(&sB[(wn * WTN + j * MT + lane % MT) * LDK + kofs]);
         }
       };
       read_frags(0, 0);
@@ -355,10 +364,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            acc[Q][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cb][0][i], fb[cb][0][j], acc[Q][i][j], 0, 0, 0);
-            accx[0][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cb][0][i], fb[cb][1][j], accx[0][i][j], 0, 0, 0);
-            accx[NXA - 1][i][j] =
-                __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cb][1][i], fb[cb][0][j], accx[NXA - 1][i][j], 0, 0, 0);
+            acc[Q][i][j] = mfma(fa[cb][0][i], fb[cb][0][j], acc[Q][i][j]);
+            accx[0][i][j] = mfma(fa[cb][0][i], fb[cb][1][j], accx[0][i][j]);
+            accx[NXA - 1][i][j] = mfma(fa[cb][1][i], fb[cb][0][j], accx[NXA - 1][i][j]);
+            if constexpr (THREE) {
+              accy[i][j] = mfma(fa[cb][0][i], fb[cb][NARR - 1][j], accy[i][j]);
+              accy[i][j] = mfma(fa[cb][1][i], fb[cb][1][j], accy[i][j]);
+              accy[i][j] = mfma(fa[cb][NARR - 1][i], fb[cb][0][j], accy[i][j]);
+            }
           }
       }
     } else if constexpr (P == kF32) {
@@ -526,7 +539,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
         for (int r = 0; r < NACC; ++r) {
           double x = (double)accx[0][i][j][r];
           if constexpr (NXA == 2) x += (double)accx[1][i][j][r];
-          acc64[i][j][r] += x * kLoScale;
+          if constexpr (THREE) acc64[i][j][r] += x + (double)accy[i][j][r];
+          else acc64[i][j][r] += x * kLoScale;
         }
   }
 
@@ -621,6 +635,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
           reinterpret_cast<float*>(p.out)[idx] = (float)(orr * orr + oii * oii);
         } else if (p.out_mode == kOutI64) {
           reinterpret_cast<double*>(p.out)[idx] = orr * orr + oii * oii;
+        } else if constexpr (P == kBF16x3) {
+          // b0 through FP32 (a double rounding moves it by one ulp at most); b1 and b2 take the
+          // exact FP64 remainders
+          const double xr = orr * (double)t;
+          const double xi = oii * (double)t;
+          const __bf16 r0 = f32_to_bf16((float)xr), i0 = f32_to_bf16((float)xi);
+          const double xr1 = xr - (double)(float)r0, xi1 = xi - (double)(float)i0;
+          const __bf16 r1 = f32_to_bf16((float)xr1), i1 = f32_to_bf16((float)xi1);
+          const __bf16 r2 = f32_to_bf16((float)(xr1 - (double)(float)r1));
+          const __bf16 i2 = f32_to_bf16((float)(xi1 - (double)(float)i1));
+          reinterpret_cast<bf16x2_t*>(p.out)[idx] = bf16x2_t{r0, i0};
+          reinterpret_cast<bf16x2_t*>(p.out_lo)[idx] = bf16x2_t{r1, i1};
+          reinterpret_cast<bf16x2_t*>(p.out_lo2)[idx] = bf16x2_t{r2, i2};
         } else {
           // hi through FP32 (native conversions; a double rounding only moves hi by one ulp, and
           // lo below takes the exact remainder either way)
@@ -1359,29 +1386,56 @@ __global__ __launch_bounds__(64 * WN * WK) void mft_f16x2_dma_kernel(StageArgs p
   }
 }
 
-// Splits rows of interleaved complex64 data into FP16 hi/lo arrays, with a power-of-two row scale
-// s = 2^-e chosen so that max|x| * s < 1.  One workgroup per row, 8 consecutive floats per thread
-// (two 16-byte loads, two 16-byte stores); row padding up to ld_out is zero-filled.
+// Split formats of the data operand: FP16 hi/lo (x = hi + lo * 2^-11) or BF16 b0/b1/b2
+// (x = b0 + b1 + b2, each the bfloat16 rounding of the remaining remainder).
+enum SplitFmt : int { kSplitF16x2 = 0, kSplitBF16x3 = 1 };
+
+// Splits 4 consecutive floats (already multiplied by the row scale) and stores them at element k
+// of each split row (8-byte stores).
+template <int FMT>
+__device__ inline void split_store4(floatx4_t v, void* const* outs, int64_t row_off, int k) {
+  if constexpr (FMT == kSplitF16x2) {
+    typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+    half4_t vh, vl;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const _Float16 q = (_Float16)v[e];
+      vh[e] = q;
+      vl[e] = (_Float16)((v[e] - (float)q) * kLoInv);
+    }
+    *reinterpret_cast<half4_t*>(static_cast<_Float16*>(outs[0]) + row_off + k) = vh;
+    *reinterpret_cast<half4_t*>(static_cast<_Float16*>(outs[1]) + row_off + k) = vl;
+  } else {
+    typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
+    bf4_t b0, b1, b2;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      // v - b0 and r - b1 are exact in FP32 (b0, b1 keep the leading 8 bits of their operands)
+      const __bf16 q0 = (__bf16)v[e];
+      const float r = v[e] - (float)q0;
+      const __bf16 q1 = (__bf16)r;
+      b0[e] = q0;
+      b1[e] = q1;
+      b2[e] = (__bf16)(r - (float)q1);
+    }
+    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[0]) + row_off + k) = b0;
+    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[1]) + row_off + k) = b1;
+    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[2]) + row_off + k) = b2;
+  }
+}
+
+// Splits rows of interleaved complex64 data into the split arrays of format FMT, with a
+// power-of-two row scale s = 2^-e chosen so that max|x| * s < 1.  One workgroup per row, any
+// length and alignment; row padding up to ld_out is zero-filled.
+template <int FMT>
 __global__ __launch_bounds__(256) void split_rows_kernel(const float* __restrict__ in, int64_t ld_in, int K,
-                                                         _Float16* __restrict__ hi, _Float16* __restrict__ lo,
-                                                         int64_t ld_out, float* __restrict__ row_scale) {
+                                                         SplitOut o, int64_t ld_out, float* __restrict__ row_scale) {
   __shared__ float red[4];
   const int64_t row = blockIdx.x;
   const float* x = in + row * ld_in;
   const int tid = threadIdx.x;
-  const bool vec_ok = ((ld_in & 3) == 0) && ((reinterpret_cast<uintptr_t>(in) & 15) == 0);
   float mx = 0.0f;
-  for (int k0 = tid * 8; k0 < K; k0 += 256 * 8) {
-    if (vec_ok && k0 + 8 <= K) {
-      const float4 a = *reinterpret_cast<const float4*>(x + k0);
-      const float4 b = *reinterpret_cast<const float4*>(x + k0 + 4);
-      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))));
-      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fmaxf(fabsf(b.z), fabsf(b.w))));
-    } else {
-      for (int j = 0; j < 8; ++j)
-        if (k0 + j < K) mx = fmaxf(mx, fabsf(x[k0 + j]));
-    }
-  }
+  for (int k = tid; k < K; k += 256) mx = fmaxf(mx, fabsf(x[k]));
   mx = wave_max(mx);
   if ((tid & 63) == 0) red[tid >> 6] = mx;
   __syncthreads();
@@ -1392,30 +1446,12 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const float* __restrict
     frexpf(mx, &e);  // mx = f * 2^e, f in [0.5, 1)  =>  mx * 2^-e < 1
     s = ldexpf(1.0f, -e);
   }
-  _Float16* h = hi + row * ld_out;
-  _Float16* l = lo + row * ld_out;
-  for (int k0 = tid * 8; k0 < ld_out; k0 += 256 * 8) {
-    float v[8];
-    if (vec_ok && k0 + 8 <= K) {
-      const float4 a = *reinterpret_cast<const float4*>(x + k0);
-      const float4 b = *reinterpret_cast<const float4*>(x + k0 + 4);
-      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else {
+  // ld_out is a multiple of 64 elements, so k0 + 4 <= ld_out
+  for (int k0 = tid * 4; k0 < ld_out; k0 += 256 * 4) {
+    floatx4_t v;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = (k0 + j < K) ? x[k0 + j] : 0.0f;
-    }
-    half8_t vh, vl;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float sv = v[j] * s;
-      const _Float16 q = (_Float16)sv;
-      vh[j] = q;
-      vl[j] = (_Float16)((sv - (float)q) * kLoInv);
-    }
-    // ld_out is a multiple of 64 halves, so k0 + 8 <= ld_out
-    *reinterpret_cast<half8_t*>(h + k0) = vh;
-    *reinterpret_cast<half8_t*>(l + k0) = vl;
+    for (int j = 0; j < 4; ++j) v[j] = (k0 + j < K) ? x[k0 + j] * s : 0.0f;
+    split_store4<FMT>(v, o.p, row * ld_out, k0);
   }
   if (tid == 0) row_scale[row] = s;
 }
@@ -1423,11 +1459,10 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const float* __restrict
 // Fast path of the split: one wave per row, the whole row (K <= 64 * PL floats) in registers,
 // 16-byte loads, 8-byte stores, a wave-only max reduction.  Needs 16-byte aligned rows with
 // K and ld_in multiples of 4.
-template <int PL>
+template <int PL, int FMT>
 __global__ __launch_bounds__(256) void split_rows_wave_kernel(const float* __restrict__ in, int64_t ld_in, int K,
-                                                              _Float16* __restrict__ hi, _Float16* __restrict__ lo,
-                                                              int64_t ld_out, float* __restrict__ row_scale,
-                                                              int64_t rows) {
+                                                              SplitOut o, int64_t ld_out,
+                                                              float* __restrict__ row_scale, int64_t rows) {
   constexpr int NV = PL / 4;  // float4 chunks per lane
   const int lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1448,40 +1483,22 @@ __global__ __launch_bounds__(256) void split_rows_wave_kernel(const float* __res
     frexpf(mx, &e);
     s = ldexpf(1.0f, -e);
   }
-  typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-  _Float16* h = hi + row * ld_out;
-  _Float16* l = lo + row * ld_out;
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     const int k = (j * 64 + lane) * 4;
-    if (k < ld_out) {
-      half4_t vh, vl;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float sv = v[j][e] * s;
-        const _Float16 q = (_Float16)sv;
-        vh[e] = q;
-        vl[e] = (_Float16)((sv - (float)q) * kLoInv);
-      }
-      *reinterpret_cast<half4_t*>(h + k) = vh;
-      *reinterpret_cast<half4_t*>(l + k) = vl;
-    }
+    if (k < ld_out) split_store4<FMT>(v[j] * s, o.p, row * ld_out, k);
   }
   // zero padding beyond the register-held chunks
-  for (int k = NV * 256 + lane * 4; k < ld_out; k += 256) {
-    typedef _Float16 half4z_t __attribute__((ext_vector_type(4)));
-    const half4z_t zq = {0, 0, 0, 0};
-    *reinterpret_cast<half4z_t*>(h + k) = zq;
-    *reinterpret_cast<half4z_t*>(l + k) = zq;
-  }
+  for (int k = NV * 256 + lane * 4; k < ld_out; k += 256) split_store4<FMT>(floatx4_t{0.f, 0.f, 0.f, 0.f}, o.p, row * ld_out, k);
   if (lane == 0) row_scale[row] = s;
 }
 
 // Compact Fourier matrix: m(c, k) = exp(i * sign * a[c] * b[k]), phase product and sincos in FP64,
-// stored as interleaved (re, im) rows of lda reals (the stage kernel expands it per fragment).
+// stored as interleaved (re, im) rows of lda reals (the stage kernel expands it per fragment), in
+// the operand format of `prec` (split precisions write their 2 or 3 arrays).
 // One thread per complex entry; entries outside (nc, nk) are 0.
 __global__ void gen_matrix_kernel(const double* __restrict__ a, int nc, const double* __restrict__ b, int nk,
-                                  double sign, int prec, void* out, void* out_lo, int64_t lda, int rows_pad) {
+                                  double sign, int prec, SplitOut out, int64_t lda, int rows_pad) {
   const int64_t kc = lda / 2;
   const int64_t total = (int64_t)rows_pad * kc;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -1498,21 +1515,32 @@ __global__ void gen_matrix_kernel(const double* __restrict__ a, int nc, const do
     }
     const int64_t o = (int64_t)c * lda + 2 * k;
     if (prec == kF64) {
-      double* q = static_cast<double*>(out);
+      double* q = static_cast<double*>(out.p[0]);
       q[o] = mr;
       q[o + 1] = mi;
     } else if (prec == kF32) {
-      float* q = static_cast<float*>(out);
+      float* q = static_cast<float*>(out.p[0]);
       q[o] = (float)mr;
       q[o + 1] = (float)mi;
-    } else {
-      _Float16* oh = static_cast<_Float16*>(out);
-      _Float16* ol = static_cast<_Float16*>(out_lo);
+    } else if (prec == kF16x2) {
+      _Float16* oh = static_cast<_Float16*>(out.p[0]);
+      _Float16* ol = static_cast<_Float16*>(out.p[1]);
       const _Float16 hr = (_Float16)mr, hi = (_Float16)mi;
       oh[o] = hr;
       oh[o + 1] = hi;
       ol[o] = (_Float16)((mr - (double)hr) * (double)kLoInv);
       ol[o + 1] = (_Float16)((mi - (double)hi) * (double)kLoInv);
+    } else {
+      const double v[2] = {mr, mi};
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const __bf16 q0 = f32_to_bf16((float)v[e]);
+        const double r = v[e] - (double)(float)q0;
+        const __bf16 q1 = f32_to_bf16((float)r);
+        static_cast<__bf16*>(out.p[0])[o + e] = q0;
+        static_cast<__bf16*>(out.p[1])[o + e] = q1;
+        static_cast<__bf16*>(out.p[2])[o + e] = f32_to_bf16((float)(r - (double)(float)q1));
+      }
     }
   }
 }
@@ -1524,9 +1552,9 @@ template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int 
 static int launch_stage_t(const StageArgs& a, int nz, hipStream_t s) {
   using T = typename PrecTraits<P>::T;
   constexpr int VEC = 16 / (int)sizeof(T);
-  constexpr int NARR = PrecTraits<P>::SPLIT ? 2 : 1;
+  constexpr int NARR = PrecTraits<P>::NARR;
   constexpr size_t lds = 2 * (size_t)NARR * (BM + BN) * (BK + VEC) * sizeof(T);
-  static_assert(lds <= 160 * 1024, "LDS budget of one CU");
+  static_assert(lds <= 156 * 1024, "LDS budget of one CU (with the kernel's static arrays)");
   auto kern = mft_stage_kernel<P, BM, BN, BK, WM, WN, WK, BMODE, PF>;
   static bool configured = false;  // per instantiation
   if (!configured) {
@@ -1561,6 +1589,8 @@ static int launch_f16x2_dma_t(const StageArgs& a, int nz, hipStream_t s) {
 
 bool tile_cfg_supported(int prec, int cfg) {
   if (cfg < 0 || cfg >= kNumTileCfg) return false;
+  // BF16x3: three operand arrays per matrix; the shapes whose LDS buffers and thread split fit
+  if (prec == kBF16x3) return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8;
   if (cfg >= kC64x64_w4) return prec == kF16x2;  // compact-constant shapes: split-FP16 only
   // FP64: 64 x 64 wave tiles need more LDS for the k-split reduction than the operand buffers
   // hold, and its 32-deep k-tiles have too few k-vectors for a 4-way split
@@ -1613,12 +1643,28 @@ static int launch_prec(const StageArgs& a, int nz, hipStream_t s) {
   return kInvalidValue;
 }
 
+// BF16x3 shapes: 64-deep k-tiles for the 4-wave shapes (three operand arrays per matrix), the
+// 8-wave 64x32 shape keeps 128-deep k-tiles (its 512 threads need 16 chunks per tile row).
+static int launch_bf16x3(const StageArgs& a, int nz, hipStream_t s) {
+  const int cfg = a.tile_cfg == kTileAuto ? k64x32_w8 : a.tile_cfg;
+  switch (cfg) {
+    case k64x64_w2x2: return launch_stage_t<kBF16x3, 64, 64, 64, 2, 2, 1, kRowsPadded, 2>(a, nz, s);
+    case k64x32_w2x1: return launch_stage_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRowsPadded, 2>(a, nz, s);
+    case k128x64: return launch_stage_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRowsPadded, 2>(a, nz, s);
+    case k64x32_w8: return launch_stage_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRowsPadded, 2>(a, nz, s);
+    default: break;
+  }
+  set_last_error("tile configuration %d is not available for BF16x3", cfg);
+  return kInvalidValue;
+}
+
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
 static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, int n, int64_t tiles, hipStream_t s) {
   using T = typename PrecTraits<P>::T;
   constexpr int VEC = 16 / (int)sizeof(T);
-  constexpr int NARR = PrecTraits<P>::SPLIT ? 2 : 1;
+  constexpr int NARR = PrecTraits<P>::NARR;
   constexpr size_t lds = 2 * (size_t)NARR * (BM + BN) * (BK + VEC) * sizeof(T);
+  static_assert(lds <= 156 * 1024, "LDS budget of one CU (with the kernel's static arrays)");
   auto kern = mft_stage_grouped_kernel<P, BM, BN, BK, WM, WN, WK, BMODE, PF>;
   static bool configured = false;
   if (!configured) {
@@ -1655,6 +1701,10 @@ static int launch_grouped_prec(int cfg, const StageArgs* table, const int64_t* t
 // shapes, 64x64 when that still gives two workgroups per CU.
 int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64) {
   if (prec == kF64) return stage == 0 ? k64x64_w2x2 : k64x32_w2x1;
+  if (prec == kBF16x3) {
+    if (max_k <= 512) return stage == 0 ? k64x32_w2x1 : k128x64;
+    return k64x32_w8;
+  }
   if (max_k <= 512) return stage == 0 ? k64x32_w2x1 : k128x64;
   if (stage == 0 && tiles64 >= 512) return k64x64_w8;
   return k64x32_w8;
@@ -1682,6 +1732,14 @@ int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, 
     HCIPY_REQUIRE(b_mode == kRowsPadded, "split-FP16 stage requires a padded data operand");
     return launch_grouped_prec<kF16x2, 128, kRowsPadded, 2>(cfg, table, tile_start, n, tiles, s);
   }
+  if (prec == kBF16x3) {
+    HCIPY_REQUIRE(b_mode == kRowsPadded, "split-BF16 stage requires a padded data operand");
+    if (cfg == k64x32_w2x1) return launch_grouped_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRowsPadded, 2>(table, tile_start, n, tiles, s);
+    if (cfg == k128x64) return launch_grouped_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRowsPadded, 2>(table, tile_start, n, tiles, s);
+    if (cfg == k64x32_w8) return launch_grouped_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRowsPadded, 2>(table, tile_start, n, tiles, s);
+    set_last_error("tile configuration %d is not available to grouped BF16x3 launches", cfg);
+    return kInvalidValue;
+  }
   set_last_error("unknown precision %d", prec);
   return kInvalidValue;
 }
@@ -1703,6 +1761,12 @@ int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s
       return kInvalidValue;
     }
     st = launch_prec<kF16x2, 128, kRowsPadded, 2>(a, nz, s);
+  } else if (prec == kBF16x3) {
+    if (b_mode != kRowsPadded) {
+      set_last_error("split-BF16 stage requires a padded data operand");
+      return kInvalidValue;
+    }
+    st = launch_bf16x3(a, nz, s);
   } else {
     set_last_error("unknown precision %d", prec);
     return kInvalidValue;
@@ -1711,29 +1775,36 @@ int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s
   return check_hip(hipGetLastError(), "mft_stage_kernel launch");
 }
 
-int launch_split_rows(const float* in, int64_t ld_in, int K, int64_t rows, _Float16* hi, _Float16* lo,
-                      int64_t ld_out, float* row_scale, hipStream_t s) {
+int launch_split_rows(int prec, const float* in, int64_t ld_in, int K, int64_t rows, SplitOut o, int64_t ld_out,
+                      float* row_scale, hipStream_t s) {
   if (rows <= 0) return kOk;
   const bool vec = ((reinterpret_cast<uintptr_t>(in) & 15) == 0) && (ld_in % 4 == 0) && (K % 4 == 0);
   const dim3 g((unsigned)ceil_div(rows, 4));
-  if (vec && K <= 64 * 16 && ld_out <= 64 * 16 + 256)
-    hipLaunchKernelGGL(split_rows_wave_kernel<16>, g, dim3(256), 0, s, in, ld_in, K, hi, lo, ld_out, row_scale, rows);
-  else if (vec && K <= 64 * 32)
-    hipLaunchKernelGGL(split_rows_wave_kernel<32>, g, dim3(256), 0, s, in, ld_in, K, hi, lo, ld_out, row_scale, rows);
-  else if (vec && K <= 64 * 64)
-    hipLaunchKernelGGL(split_rows_wave_kernel<64>, g, dim3(256), 0, s, in, ld_in, K, hi, lo, ld_out, row_scale, rows);
-  else
-    hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)rows), dim3(256), 0, s, in, ld_in, K, hi, lo, ld_out,
+#define HCIPY_SPLIT_LAUNCH(FMT)                                                                                     \
+  if (vec && K <= 64 * 16 && ld_out <= 64 * 16 + 256)                                                              \
+    hipLaunchKernelGGL((split_rows_wave_kernel<16, FMT>), g, dim3(256), 0, s, in, ld_in, K, o, ld_out, row_scale, rows); \
+  else if (vec && K <= 64 * 32)                                                                                    \
+    hipLaunchKernelGGL((split_rows_wave_kernel<32, FMT>), g, dim3(256), 0, s, in, ld_in, K, o, ld_out, row_scale, rows); \
+  else if (vec && K <= 64 * 64)                                                                                    \
+    hipLaunchKernelGGL((split_rows_wave_kernel<64, FMT>), g, dim3(256), 0, s, in, ld_in, K, o, ld_out, row_scale, rows); \
+  else                                                                                                             \
+    hipLaunchKernelGGL((split_rows_kernel<FMT>), dim3((unsigned)rows), dim3(256), 0, s, in, ld_in, K, o, ld_out,     \
                        row_scale);
+  if (prec == kF16x2) {
+    HCIPY_SPLIT_LAUNCH(kSplitF16x2)
+  } else {
+    HCIPY_SPLIT_LAUNCH(kSplitBF16x3)
+  }
+#undef HCIPY_SPLIT_LAUNCH
   return check_hip(hipGetLastError(), "split_rows_kernel launch");
 }
 
-int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, void* out,
-                      void* out_lo, int64_t lda, int rows_pad, hipStream_t s) {
+int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, SplitOut out,
+                      int64_t lda, int rows_pad, hipStream_t s) {
   const int64_t total = (int64_t)rows_pad * (lda / 2);
   const int64_t blocks = std::min<int64_t>(ceil_div(total, 256), 65536);
-  hipLaunchKernelGGL(gen_matrix_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, nc, b, nk, sign, prec, out,
-                     out_lo, lda, rows_pad);
+  hipLaunchKernelGGL(gen_matrix_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, nc, b, nk, sign, prec, out, lda,
+                     rows_pad);
   return check_hip(hipGetLastError(), "gen_matrix_kernel launch");
 }
 

@@ -15,27 +15,37 @@
 //   * the MFMA accumulator layout puts rows (2c, 2c+1) of one column in the same lane, so the
 //     epilogue holds complete complex values and writes them interleaved, coalesced along n.
 //
-// Three arithmetic variants share the tiling:
-//   kF64   : v_mfma_f64_16x16x4_f64   (complex128 fields; parity with the FP64 reference)
-//   kF32   : v_mfma_f32_32x32x2_f32   (plain FP32; single-level FP32 accumulation)
-//   kF16x2 : v_mfma_f32_32x32x16_f16  on hi/lo FP16 splits of both operands:
-//              x = hi + lo * 2^-11,  A.B ~= Ah.Bh + 2^-11 (Ah.Bl + Al.Bh)
-//            The Ah.Bh accumulator is flushed into FP64 every BK (=64) real k; the cross
-//            accumulator is scaled by 2^-11 at each flush.  Data rows carry a power-of-two scale
-//            so that every split value lies in [-1, 1].
+// Four arithmetic variants share the tiling:
+//   kF64    : v_mfma_f64_16x16x4_f64   (complex128 fields; parity with the FP64 reference)
+//   kF32    : v_mfma_f32_32x32x2_f32   (plain FP32; single-level FP32 accumulation)
+//   kF16x2  : v_mfma_f32_32x32x16_f16  on hi/lo FP16 splits of both operands:
+//               x = hi + lo * 2^-11,  A.B ~= Ah.Bh + 2^-11 (Ah.Bl + Al.Bh)
+//             The Ah.Bh accumulator is flushed into FP64 every kFlushK real k; the cross
+//             accumulators are added (times 2^-11) at the end.  Data rows carry a power-of-two
+//             scale so that every split value lies in [-1, 1].
+//   kBF16x3 : v_mfma_f32_32x32x16_bf16 on three-way BF16 splits of both operands:
+//               x = b0 + b1 + b2 (each the bfloat16 rounding of the remainder, |x - sum| <= 2^-27 |x|)
+//               A.B ~= A0.B0 + (A0.B1 + A1.B0) + (A0.B2 + A1.B1 + A2.B0)   (six MFMAs per k-vector)
+//             A0.B0 is flushed into FP64 like the FP16 hi.hi products; the two cross sums
+//             (2^-9 and 2^-18 of it) run in FP32 over the whole contraction.
 #pragma once
 
 #include "common.hpp"
 
 namespace hcipy {
 
-enum Prec : int { kF64 = 0, kF32 = 1, kF16x2 = 2 };
+enum Prec : int { kF64 = 0, kF32 = 1, kF16x2 = 2, kBF16x3 = 3 };
+
+// True for the split precisions (data rows are split before stage 1, the intermediate is
+// written split, the operand arrays are 2 or 3 per matrix).
+inline bool is_split(int prec) { return prec == kF16x2 || prec == kBF16x3; }
+inline int split_arrays(int prec) { return prec == kF16x2 ? 2 : (prec == kBF16x3 ? 3 : 1); }
 
 // Output formats written by the epilogue.
 enum OutMode : int {
   kOutC128 = 0,   // interleaved complex128
   kOutC64 = 1,    // interleaved complex64
-  kOutSplit = 2,  // interleaved hi/lo FP16 pair arrays (feeds the next kF16x2 stage)
+  kOutSplit = 2,  // interleaved split arrays (FP16 hi/lo or BF16 b0/b1/b2) feeding the next split stage
   kOutI32 = 3,    // |z|^2 as float32, evaluated from the FP64 result
   kOutI64 = 4,    // |z|^2 as float64
 };
@@ -49,6 +59,7 @@ template <> struct PrecTraits<kF64> {
   static constexpr int KVEC = GROUPS * KPL;
   static constexpr int NACC = 4;     // accumulator registers per lane per MFMA tile
   static constexpr bool SPLIT = false;
+  static constexpr int NARR = 1;     // operand arrays per matrix
 };
 template <> struct PrecTraits<kF32> {
   using T = float;
@@ -58,6 +69,7 @@ template <> struct PrecTraits<kF32> {
   static constexpr int KVEC = GROUPS * KPL;
   static constexpr int NACC = 16;
   static constexpr bool SPLIT = false;
+  static constexpr int NARR = 1;
 };
 template <> struct PrecTraits<kF16x2> {
   using T = _Float16;
@@ -67,6 +79,17 @@ template <> struct PrecTraits<kF16x2> {
   static constexpr int KVEC = GROUPS * KPL;
   static constexpr int NACC = 16;
   static constexpr bool SPLIT = true;
+  static constexpr int NARR = 2;
+};
+template <> struct PrecTraits<kBF16x3> {
+  using T = __bf16;
+  static constexpr int MT = 32;
+  static constexpr int GROUPS = 2;
+  static constexpr int KPL = 8;
+  static constexpr int KVEC = GROUPS * KPL;
+  static constexpr int NACC = 16;
+  static constexpr bool SPLIT = true;
+  static constexpr int NARR = 3;
 };
 
 // Scale of the lo part of a split value: x = hi + lo * kLoScale.
@@ -79,10 +102,12 @@ struct StageArgs {
   // Constant operand (expanded Fourier matrix), padded so that tile loads never leave it.
   const void* a;
   const void* a_lo;
+  const void* a_lo2;            // third split array (kBF16x3)
   int64_t lda;
   // Data operand.
   const void* b;
   const void* b_lo;
+  const void* b_lo2;
   int64_t ldb;
   int64_t b_zstride;            // elements between batch items along gridDim.z
   // Problem size: M = 2 * (number of constant rows), N = data rows, K = real contraction length.
@@ -90,6 +115,7 @@ struct StageArgs {
   // Output.
   void* out;
   void* out_lo;
+  void* out_lo2;
   int64_t ldo;                  // complex elements between consecutive constant rows
   int64_t out_zstride;          // complex elements between batch items along gridDim.z
   int out_mode;
@@ -143,6 +169,11 @@ enum TileCfg : int {
   kD64x32_w4 = 14,
   kD64x32_w8 = 15,
   kNumTileCfg = 16,
+};
+
+// Up to three split arrays (device pointers), passed by value to the split and matrix kernels.
+struct SplitOut {
+  void* p[3];
 };
 
 // True if `cfg` is instantiated for precision `prec`.

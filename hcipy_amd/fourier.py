@@ -333,7 +333,7 @@ class MatrixFourierTransform(FourierTransform):
     of the field are one batched launch (the reference loops over them,
     fourier_transform.py:302-321).
 
-    ``precision`` picks the complex64 arithmetic ('f16x2' or 'fp32'; default from
+    ``precision`` picks the complex64 arithmetic ('f16x2', 'bf16x3' or 'fp32'; default from
     ``Configuration().fourier.mft.precision_complex64``).  complex128 always runs in FP64.
     """
 
@@ -347,8 +347,8 @@ class MatrixFourierTransform(FourierTransform):
         cfg = Configuration().fourier.mft
         self.precompute_matrices = cfg.precompute_matrices if precompute_matrices is None else precompute_matrices
         self.allocate_intermediate = cfg.allocate_intermediate if allocate_intermediate is None else allocate_intermediate
-        if precision is not None and precision not in ('f16x2', 'fp32'):
-            raise ValueError("precision must be 'f16x2' or 'fp32' (complex64 fields)")
+        if precision is not None and precision not in ('f16x2', 'bf16x3', 'fp32'):
+            raise ValueError("precision must be 'f16x2', 'bf16x3' or 'fp32' (complex64 fields)")
         self.precision = precision
         self._plans = {}
         self._workspace = {}
@@ -718,8 +718,8 @@ class FastFourierTransform(FourierTransform):
     Fourier sum onto ``make_fft_grid(input_grid, q, fov, shift)``.  This class evaluates that sum
     with the Bluestein zoom FFT (O(N^2 log N); measured faster than the MFT at every size from
     128^2 up, 13.6x at 2048^2 in FP64) whenever each axis fits its 8192-point line FFT, and with
-    the matrix Fourier transform otherwise, or when ``precision='f16x2'`` asks for the split-FP16
-    MFT.  ``precision`` None / 'fp32' selects the complex64 arithmetic of the zoom FFT (FP64
+    the matrix Fourier transform otherwise, or when ``precision`` 'f16x2' / 'bf16x3' asks for a
+    split-precision MFT.  ``precision`` None / 'fp32' selects the complex64 arithmetic of the zoom FFT (FP64
     internal / FP32) or of the MFT.  ``emulate_fftshifts`` is accepted for interface
     compatibility (it only changes the reference's rounding).
     """
@@ -745,7 +745,7 @@ class FastFourierTransform(FourierTransform):
         self.shape_out = self.output_grid.shape
         self.internal_shape = self.internal_grid.shape
         self._impl = None
-        if precision != 'f16x2':
+        if precision not in ('f16x2', 'bf16x3'):
             try:
                 self._impl = ZoomFastFourierTransform(input_grid, self.output_grid, precision=precision)
                 self._impl._fft_n = [int(n) for n in self.internal_grid.dims]
@@ -828,10 +828,10 @@ def make_fourier_transform(input_grid, output_grid=None, q=1, fov=1, shift=0, pl
     def build(method):
         if method == 'fft':
             return FastFourierTransform(input_grid, q, fov, shift,
-                                        precision=precision if precision in ('f16x2', 'fp32') else None)
+                                        precision=precision if precision in ('f16x2', 'bf16x3', 'fp32') else None)
         if method == 'mft':
             return MatrixFourierTransform(input_grid, output_grid,
-                                          precision=precision if precision in ('f16x2', 'fp32') else None)
+                                          precision=precision if precision in ('f16x2', 'bf16x3', 'fp32') else None)
         return ZoomFastFourierTransform(input_grid, output_grid,
                                         precision=precision if precision in ('fp64', 'fp32') else None)
 

@@ -20,7 +20,7 @@ class FraunhoferPropagator(AgnosticOpticalElement):
 
     forward: ``MFT_{pupil -> uv}(E) / (i f lambda)`` with ``uv = focal * 2 pi / (f lambda)``;
     backward: ``MFT^dagger(E) * (i f lambda)`` (fraunhofer.py:32-38, 56-88).
-    ``precision`` selects the complex64 arithmetic ('f16x2' or 'fp32').
+    ``precision`` selects the complex64 arithmetic ('f16x2', 'bf16x3' or 'fp32').
     """
 
     def __init__(self, input_grid, output_grid, focal_length=1, precision=None):

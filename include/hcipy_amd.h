@@ -27,10 +27,12 @@ extern "C" {
 #define HCIPY_ERROR_UNSUPPORTED 4
 
 /* Arithmetic of the MFT kernels.  The dtype contract does not depend on it:
- * FP64 reads/writes complex128, FP32 and F16X2 read/write complex64. */
-#define HCIPY_PREC_FP64 0  /* FP64 MFMA (complex128 fields)                                 */
-#define HCIPY_PREC_FP32 1  /* FP32 MFMA, single-level FP32 accumulation (complex64 fields)  */
-#define HCIPY_PREC_F16X2 2 /* hi/lo FP16 split MFMA, FP64 flush of partial sums (complex64) */
+ * FP64 reads/writes complex128, FP32, F16X2 and BF16X3 read/write complex64. */
+#define HCIPY_PREC_FP64 0   /* FP64 MFMA (complex128 fields)                                  */
+#define HCIPY_PREC_FP32 1   /* FP32 MFMA, single-level FP32 accumulation (complex64 fields)   */
+#define HCIPY_PREC_F16X2 2  /* hi/lo FP16 split MFMA, FP64 flush of partial sums (complex64)  */
+#define HCIPY_PREC_BF16X3 3 /* three-way BF16 split MFMA (six products per k), FP64 flush of
+                               the leading products (complex64)                                */
 
 #define HCIPY_FORWARD 0
 #define HCIPY_BACKWARD 1

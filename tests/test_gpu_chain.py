@@ -82,7 +82,7 @@ def test_lyot_chain_small_vs_oracle():
     assert host(prop(cor(wf)).power).max() / norm < 5e-3
 
 
-@pytest.mark.parametrize('precision,tol', [('f16x2', 1e-9), ('fp32', 1e-7)])
+@pytest.mark.parametrize('precision,tol', [('f16x2', 1e-9), ('bf16x3', 1e-9), ('fp32', 1e-7)])
 def test_lyot_1024_chain_contrast_error(precision, tol):
     """Full 1024^2 Lyot chain (256^2 mask plane, 512^2 final plane) in complex64: intensity
     error as a fraction of the unocculted peak, against the FP64 oracle on the same input."""

@@ -19,6 +19,7 @@ def _rand(shape, seed, dtype):
 
 
 @pytest.mark.parametrize('precision,dtype,tol', [(None, np.complex128, 1e-12), ('f16x2', np.complex64, 5e-7),
+                                                 ('bf16x3', np.complex64, 5e-7),
                                                  ('fp32', np.complex64, 3e-6)])
 def test_grouped_matches_individual(precision, dtype, tol):
     mfts, calls, refs = [], [], []

@@ -92,13 +92,13 @@ def test_fraunhofer_1024_to_512_complex128():
     assert peak_rel_field(back, o.fraunhofer_backward(b, og(pupil), og(focal))) < 1e-12
 
 
-@pytest.mark.parametrize('precision', ['f16x2', 'fp32'])
+@pytest.mark.parametrize('precision', ['f16x2', 'bf16x3', 'fp32'])
 def test_fraunhofer_1024_to_512_complex64(precision):
     """1024^2 -> 512^2 complex64 against the FP64 result on the same (complex64) input.
 
-    Bars (peak-relative): f16x2 field 1e-7, fused intensity 1e-7, intensity of the complex64
-    field 2.5e-7 (rounding re/im to FP32 alone moves |E|^2 by up to 2^-23 = 1.2e-7 at the peak);
-    fp32 field and intensities 3e-6.
+    Bars (peak-relative): f16x2 and bf16x3 field 1e-7, fused intensity 1e-7, intensity of the
+    complex64 field 2.5e-7 (rounding re/im to FP32 alone moves |E|^2 by up to 2^-23 = 1.2e-7 at
+    the peak); fp32 field and intensities 3e-6.
     """
     pupil, focal, E = workloads.plain_mft_1024()
     E64 = E.astype(np.complex64)
@@ -115,13 +115,13 @@ def test_fraunhofer_1024_to_512_complex64(precision):
     ei_field = peak_rel_intensity(a, b)
     ei = np.abs(I.astype(np.float64) - Ib).max() / Ib.max()
     say('  field %.3e  intensity(fused) %.3e  intensity(from c64 field) %.3e' % (ef, ei, ei_field))
-    if precision == 'f16x2':
+    if precision in ('f16x2', 'bf16x3'):
         assert ef < 1e-7 and ei < 1e-7 and ei_field < 2.5e-7
     else:
         assert ef < 3e-6 and ei < 3e-6 and ei_field < 3e-6
 
 
-@pytest.mark.parametrize('precision,tol', [('f16x2', 5e-7), ('fp32', 3e-6)])
+@pytest.mark.parametrize('precision,tol', [('f16x2', 5e-7), ('bf16x3', 5e-7), ('fp32', 3e-6)])
 def test_backward_256_to_1024_complex64(precision, tol):
     """Lyot return leg: 256^2 focal (q=32, 4 lambda/D) -> 1024^2 pupil, adjoint MFT, on random
     complex data.  Random data has no coherent peak, so the error is set by the 22-bit hi/lo
@@ -142,11 +142,12 @@ def test_f16x2_handles_wide_dynamic_range():
     gi = ha.make_pupil_grid(64)
     go = ha.make_focal_grid(4, 8)
     f = _rand(gi.size, 9)
-    mft = ha.MatrixFourierTransform(gi, go, precision='f16x2')
     b = o.mft_forward(f, og(gi), og(go))
-    for scale in [1e-12, 1.0, 1e9]:
-        a = host(mft.forward(ha.Field(dev(f * scale).to(torch.complex64), gi))) / scale
-        assert peak_rel_field(a, b) < 2e-6, scale
+    for prec in ('f16x2', 'bf16x3'):
+        mft = ha.MatrixFourierTransform(gi, go, precision=prec)
+        for scale in [1e-12, 1.0, 1e9]:
+            a = host(mft.forward(ha.Field(dev(f * scale).to(torch.complex64), gi))) / scale
+            assert peak_rel_field(a, b) < 2e-6, (prec, scale)
 
 
 def test_precompute_and_intermediate_flags():
@@ -223,7 +224,7 @@ def test_fraunhofer_airy_closed_form():
 
 
 @pytest.mark.parametrize('precision,cdt,tol', [('fp64', np.complex128, 1e-12), ('fp32', np.complex64, 3e-6),
-                                               ('f16x2', np.complex64, 5e-7)])
+                                               ('f16x2', np.complex64, 5e-7), ('bf16x3', np.complex64, 5e-7)])
 def test_every_tile_shape_matches_oracle(precision, cdt, tol):
     """Each workgroup tile shape (pinned on the plan for both stages and directions) gives the
     oracle's result: ragged sizes exercise the k tails, the row clamps and the k-split sums."""
@@ -242,9 +243,9 @@ def test_every_tile_shape_matches_oracle(precision, cdt, tol):
             for stage in (0, 1):
                 plan.set_tiles(d, stage, cfg)
                 assert plan.get_tiles(d, stage) == cfg
-        say(precision, 'tile shape', cfg)
         a = host(mft.forward(ha.Field(dev(f), gi)))
         b = host(mft.backward(ha.Field(dev(F), go)))
+        say(precision, 'tile shape', cfg, 'errors %.3e %.3e' % (peak_rel_field(a, ref_f), peak_rel_field(b, ref_b)))
         assert peak_rel_field(a, ref_f) < tol, cfg
         assert peak_rel_field(b, ref_b) < tol, cfg
 
