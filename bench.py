@@ -40,6 +40,8 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=2.0, help='t_max of the CPU timing loop')
     ap.add_argument('--paths-only', action='store_true',
                     help='only time the angular-spectrum, Lyot-chain, vortex and mixed-size grouped paths')
+    ap.add_argument('--path', default=None, choices=['poke_jacobian', 'mixed_grouped', 'broadband_grouped'],
+                    help='only time this one workload (for profiler runs)')
     return ap.parse_args()
 
 
@@ -368,9 +370,47 @@ def main():
                 'grouped_ms': round(sec_g * 1e3, 3), 'one_call_per_transform_ms': round(sec_p * 1e3, 3),
                 'grouped_tflops': round(gflop / sec_g / 1e3, 1)}
 
+    poke_sample = {}
+
+    def time_poke_jacobian(reps=5):
+        """Broadband DM-poke batch (workloads.poke_jacobian): 32 wavelengths x 1024 single-actuator
+        pokes of a 32 x 32 Gaussian DM over 256^2 / 512^2 / 1024^2 pupils -> 128^2 / 256^2 / 512^2
+        focal intensities, complex64, split FP16: one poke-set execution (base transforms + windowed
+        poke transforms as grouped launches) replayed from a HIP graph."""
+        wl_items, h, n_act = workloads.poke_jacobian()
+        IFs, props, items = {}, {}, []
+        for pg, fg, wl, E in wl_items:
+            n = pg.shape[0]
+            if n not in IFs:
+                IFs[n] = ha.make_gaussian_influence_functions(pg, n_act, 1.0 / n_act)
+                props[n] = ha.FraunhoferPropagator(pg, fg, precision='f16x2')
+            items.append((props[n], wl, torch.as_tensor(E.astype(np.complex64), device=dev), IFs[n]))
+        ps = ha.PokeSet(items, h, autotune=True)
+        sec, _ = time_graph(ps.execute, n_chunks=reps, chunk=1)
+        k = 2  # a 1024^2 item: one poke kept for the oracle check of the CPU leg
+        poke_sample.update(item=wl_items[k], act=527, h=h, n_act=n_act, base=ps.base[k].cpu().numpy(),
+                           poke=ps.pokes[k][527].cpu().numpy())
+        res = {'workload': '32 wavelengths x 1024 pokes (32x32 Gaussian DM, 1e-3 rad), pupils 256^2/512^2/1024^2 '
+                           '-> focal 128^2/256^2/512^2, complex64, f16x2, one poke-set execution',
+               'pokes': ps.num_pokes, 'ms': round(sec * 1e3, 3), 'pokes_per_s': round(ps.num_pokes / sec, 1),
+               'executed_tflops': round(ps.flops_windowed / sec / 1e12, 1),
+               'full_transform_equivalent_tflops': round(ps.flops_full / sec / 1e12, 1),
+               'executed_tflop': round(ps.flops_windowed / 1e12, 3), 'full_transform_tflop': round(ps.flops_full / 1e12, 3),
+               'device_gb': round(ps.nbytes / 1e9, 1), 'tile_shapes': list(ps.tiles)}
+        ps.close()
+        del ps, items
+        torch.cuda.empty_cache()
+        return res
+
+    if args.path is not None:
+        fn = {'poke_jacobian': time_poke_jacobian, 'mixed_grouped': time_mixed_grouped,
+              'broadband_grouped': time_broadband}[args.path]
+        print(json.dumps({args.path: fn()}))
+        return
     if args.paths_only:
         out = time_other_paths()
         out['mixed_grouped'] = time_mixed_grouped()
+        out['poke_jacobian'] = time_poke_jacobian()
         print(json.dumps(out))
         return
 
@@ -398,6 +438,7 @@ def main():
     if not args.no_variants:
         paths = time_other_paths()
         paths['mixed_grouped'] = time_mixed_grouped()
+        paths['poke_jacobian'] = time_poke_jacobian()
 
     # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC).
     ny, nx = pupil.shape
@@ -472,6 +513,18 @@ def main():
                 paths[key]['peak_rel_error_vs_oracle_unocculted'] = float(
                     np.abs(It.cpu().numpy().astype(np.float64) - I_l).max() / I_un.max())
             paths['lyot_chain_1024_f16x2']['unocculted_peak_over_coronagraphic_peak'] = float(I_un.max() / I_l.max())
+        if poke_sample:
+            # one 1024^2 poke of the poke batch against the oracle's poke-and-propagate
+            pg, fg, wl, E = poke_sample['item']
+            ogp = o.Grid(pg.delta, tuple(int(v) for v in pg.dims), pg.zero)
+            ogf = o.Grid(fg.delta, tuple(int(v) for v in fg.dims), fg.zero)
+            E64 = E.astype(np.complex64).astype(np.complex128)
+            IF = o.gaussian_influence_functions(ogp, poke_sample['n_act'], 1.0 / poke_sample['n_act'],
+                                                only=[poke_sample['act']])[0]
+            Ib = np.abs(o.fraunhofer_forward(E64, ogp, ogf, wl)) ** 2
+            Ip = np.abs(o.fraunhofer_forward(o.dm_forward(E64, poke_sample['h'] * IF, wl), ogp, ogf, wl)) ** 2
+            paths['poke_jacobian']['peak_rel_error_vs_oracle_sample'] = float(
+                max(np.abs(poke_sample['base'] - Ib).max(), np.abs(poke_sample['poke'] - Ip).max()) / Ib.max())
 
     value = 1.0 / sec_main
     res = {

@@ -23,7 +23,8 @@ from .spectral_noise import (SpectralNoiseFFT, SpectralNoiseFactoryFFT, Spectral
 from .aperture import (make_circular_aperture, make_obstructed_circular_aperture, make_rectangular_aperture, make_spider,
                        make_super_gaussian_aperture)
 from .aberration import SurfaceAberration, SurfaceAberrationAtDistance, make_high_pass_power_law_error
-from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, TipTiltMirror, XineticsInfluenceFunctions,
+from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, PokeSet, TipTiltMirror,
+                                XineticsInfluenceFunctions,
                                 make_xinetics_influence_functions, make_actuator_positions,
                                 make_gaussian_influence_functions)
 from .metrics import get_mean_intensity_in_roi, get_mean_raw_contrast, get_strehl_from_focal, get_strehl_from_pupil

@@ -73,6 +73,13 @@ class DmTableDesc(ctypes.Structure):
                 ('table', ctypes.c_void_p)]
 
 
+class PokeItem(ctypes.Structure):
+    """hcipy_poke_item of include/hcipy_amd.h."""
+    _fields_ = [('plan', ctypes.c_void_p), ('field', ctypes.c_void_p), ('alpha_re', ctypes.c_double),
+                ('alpha_im', ctypes.c_double), ('dm', DmGaussianDesc), ('phase_scale', ctypes.c_double),
+                ('base_out', ctypes.c_void_p), ('poke_out', ctypes.c_void_p)]
+
+
 _c_int64 = ctypes.c_int64
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -122,6 +129,13 @@ SIGNATURES = {
                                                   _vp]),
     'hcipy_dm_gaussian_surface': (ctypes.c_int, [_vp, _vp, _vp, _c_int64, _vp]),
     'hcipy_dm_gaussian_surface_adjoint': (ctypes.c_int, [_vp, _vp, _vp, _c_int64, _vp]),
+    'hcipy_poke_set_create': (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _vp]),
+    'hcipy_poke_set_bytes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_poke_set_execute': (ctypes.c_int, [_vp, _vp]),
+    'hcipy_poke_set_autotune': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp]),
+    'hcipy_poke_set_info': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)]),
+    'hcipy_poke_set_destroy': (ctypes.c_int, [_vp]),
     'hcipy_filter_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_int, _vp]),
     'hcipy_filter_plan_destroy': (ctypes.c_int, [_vp]),

@@ -9,6 +9,7 @@
 
 #include "../../include/hcipy_amd.h"
 #include "mft_gemm.hpp"
+#include "pokes.hpp"
 
 namespace hcipy {
 
@@ -36,9 +37,10 @@ int refuse_under_capture(hipStream_t stream, const char* what) {
 }
 
 int launch_stage(int prec, const StageArgs& a, int nz, int b_mode, hipStream_t s);
-int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
-                         int64_t tiles, hipStream_t s);
+int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, const int64_t* tile_start,
+                         const int* tile_item, int n, int64_t tiles, hipStream_t s);
 int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64);
+bool grouped_cfg_supported(int prec, int cfg);
 void grouped_tile_shape(int cfg, int* bm, int* bn);
 constexpr int kModePadded = 0, kModeAligned = 1, kModeUnaligned = 2;  // BMode of mft_gemm.hip
 int launch_split_rows(int prec, const float* in, int64_t ld_in, int K, int64_t rows, SplitOut o, int64_t ld_out,
@@ -401,6 +403,183 @@ constexpr size_t kGroupCaptureSlots = 64;
 size_t group_table_bytes(int cap) {
   return round_up((int64_t)(2 * (size_t)cap * sizeof(StageArgs)), 64) + 2 * ((size_t)cap + 4) * sizeof(int64_t);
 }
+
+// One grouped launch: `n` consecutive entries of a stage-argument table starting at `first`, with
+// their tile prefix sums at `start_off` of the tile-start array.
+struct GroupedLaunch {
+  int stage, b_mode, cfg, first, n;
+  int64_t start_off, tiles;
+};
+
+// Writes the stage arguments of `col` to hargs (stage-1 entries ordered by data mode, one launch
+// per mode, then the stage-2 entries) and their tile prefix sums to hstart; appends the launches.
+// hargs holds 2 * col.size() entries, hstart 2 * col.size() + 4.
+void build_grouped_launches(int prec, const std::vector<Collected>& col, StageArgs* hargs, int64_t* hstart,
+                            std::vector<GroupedLaunch>& launches, const int* force_cfg = nullptr) {
+  const int n_items = (int)col.size();
+  int na = 0;
+  int64_t ns = 0;
+  for (int stage = 0; stage < 2; ++stage) {
+    for (int mode = 0; mode < 3; ++mode) {
+      if (stage == 1 && mode != kModePadded) continue;
+      auto member = [&](const Collected& c) { return stage == 0 ? c.b_mode1 == mode : c.nz2 != 0; };
+      // tile shape of this launch from its longest contraction and its size in 64x64 tiles
+      int max_k = 0;
+      int64_t tiles64 = 0;
+      for (int i = 0; i < n_items; ++i) {
+        const Collected& c = col[(size_t)i];
+        if (!member(c)) continue;
+        const StageArgs& a = stage == 0 ? c.a1 : c.a2;
+        max_k = std::max(max_k, a.K);
+        tiles64 += ceil_div(a.M, 64) * ceil_div(a.N, 64) * (stage == 0 ? 1 : c.nz2);
+      }
+      const int cfg = (force_cfg != nullptr && force_cfg[stage] >= 0) ? force_cfg[stage]
+                                                                      : grouped_tile_cfg(prec, stage, max_k, tiles64);
+      int bm, bn;
+      grouped_tile_shape(cfg, &bm, &bn);
+      GroupedLaunch L{stage, mode, cfg, na, 0, ns, 0};
+      int64_t acc = 0;
+      for (int i = 0; i < n_items; ++i) {
+        const Collected& c = col[(size_t)i];
+        if (!member(c)) continue;
+        const StageArgs& a = stage == 0 ? c.a1 : c.a2;
+        const int64_t nz = stage == 0 ? 1 : c.nz2;
+        hargs[na++] = a;
+        hstart[ns++] = acc;
+        acc += ceil_div(a.M, bm) * ceil_div(a.N, bn) * nz;
+        ++L.n;
+      }
+      if (L.n == 0) continue;
+      hstart[ns++] = acc;
+      L.tiles = acc;
+      launches.push_back(L);
+    }
+  }
+}
+}  // namespace
+
+// Poke set: the base transform of every item and the windowed transform of every single-actuator
+// poke, as fixed grouped launches over tables built once at creation (no allocation or copy per
+// execution, so executions can be captured).  One device allocation holds the workspaces, the
+// base focal fields, the window descriptors and the tables.
+struct hcipy_poke_set_s {
+  int prec = 0, device = 0;
+  char* mem = nullptr;
+  size_t mem_bytes = 0;
+  int64_t total_rows = 0;
+  int max_wx = 0;
+  int n_items = 0;
+  int64_t n_pokes = 0;
+  double flops_windowed = 0.0, flops_full = 0.0;  // real flops (8 per complex MAC) per execution
+  void* win_items = nullptr;
+  struct Item {
+    const Plan* pl;
+    const void* field;
+    char* ws;        // base transform workspace
+    double2* base;   // base focal field (complex128, nv x nu)
+    void* base_out;  // base intensity (float32 / float64) or NULL
+    int64_t n_focal;
+  };
+  std::vector<Item> items;
+  const StageArgs* base_tab = nullptr;
+  const int64_t* base_start = nullptr;
+  std::vector<GroupedLaunch> base_launches;
+  const StageArgs* poke_tab = nullptr;
+  const int64_t* poke_start = nullptr;
+  std::vector<GroupedLaunch> poke_launches;
+  // item of every workgroup of each poke launch (one dependent load instead of a binary search
+  // over tens of thousands of tile starts)
+  int* map_mem = nullptr;
+  size_t map_cap = 0;  // entries of map_mem (the tile count of the smallest tile shape)
+  std::vector<const int*> poke_maps;
+  std::vector<Collected> pcol;  // stage arguments of every poke (host copy, for re-tiling)
+  StageArgs* ptab_dev = nullptr;
+  int64_t* pst_dev = nullptr;
+  int cfg[2] = {-1, -1};  // tile shapes of the poke launches (-1: size heuristic)
+};
+
+namespace {
+// Builds the poke launches with tile shapes `cfg` and uploads their table, tile starts and
+// tile-to-item maps.
+int upload_poke_tables(hcipy_poke_set_s* h, const int* cfg) {
+  const size_t n = h->pcol.size();
+  std::vector<StageArgs> tab(2 * n);
+  std::vector<int64_t> start(2 * n + 4);
+  std::vector<GroupedLaunch> launches;
+  build_grouped_launches(h->prec, h->pcol, tab.data(), start.data(), launches, cfg);
+  int64_t total_tiles = 0;
+  for (const GroupedLaunch& L : launches) total_tiles += L.tiles;
+  HCIPY_REQUIRE((size_t)total_tiles <= h->map_cap, "tile map of %lld entries exceeds %zu", (long long)total_tiles,
+                h->map_cap);
+  std::vector<int> map((size_t)total_tiles);
+  std::vector<const int*> maps;
+  int64_t off_t = 0;
+  for (const GroupedLaunch& L : launches) {
+    maps.push_back(h->map_mem + off_t);
+    for (int i = 0; i < L.n; ++i)
+      for (int64_t t = start[(size_t)(L.start_off + i)]; t < start[(size_t)(L.start_off + i + 1)]; ++t)
+        map[(size_t)(off_t + t)] = i;
+    off_t += L.tiles;
+  }
+  HCIPY_HIP(hipMemcpy(h->ptab_dev, tab.data(), 2 * n * sizeof(StageArgs), hipMemcpyHostToDevice));
+  HCIPY_HIP(hipMemcpy(h->pst_dev, start.data(), (2 * n + 4) * sizeof(int64_t), hipMemcpyHostToDevice));
+  HCIPY_HIP(hipMemcpy(h->map_mem, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+  h->poke_launches = launches;
+  h->poke_maps = maps;
+  h->poke_tab = h->ptab_dev;
+  h->poke_start = h->pst_dev;
+  return kOk;
+}
+}  // namespace
+
+namespace {
+__global__ __launch_bounds__(256) void c128_intensity_kernel(const double2* __restrict__ z, void* __restrict__ out,
+                                                             int f64, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = z[i];
+    const double I = v.x * v.x + v.y * v.y;
+    if (f64) static_cast<double*>(out)[i] = I;
+    else static_cast<float*>(out)[i] = (float)I;
+  }
+}
+
+// Window of one poke along one axis: [lo, hi] the support pixels (hi < lo when empty), n the grid
+// size.  The window is w pixels from an origin that is a multiple of 8 (16-byte aligned matrix
+// columns), within the matrix's padded contraction length round_up(n, 64).
+int64_t window_origin(int64_t lo, int64_t hi, int64_t n, int64_t w) {
+  const int64_t lim = round_up(n, 64);
+  if (hi < lo || w >= lim) return 0;
+  return std::min(lo / 8 * 8, lim - w);
+}
+
+// Workspace of one item's pokes (bytes from the item's base), each array indexed by poke:
+// windows [poke][wy][2 wx], row scales [poke][wy], intermediate scales [poke], intermediates
+// [poke][nu][2 wy] (one array per operand array of the precision).
+struct PokeWs {
+  size_t win[3] = {0, 0, 0}, row_scale = 0, t = 0, z[3] = {0, 0, 0}, total = 0;
+  size_t win_pitch = 0, z_pitch = 0;  // bytes per poke
+};
+PokeWs poke_ws(int prec, int64_t n_pokes, int64_t wx, int64_t wy, int64_t nu) {
+  PokeWs p;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (size_t)round_up((int64_t)bytes, 256);
+    return at;
+  };
+  const int na = n_arrays(prec);
+  const size_t es = elem_size(prec);
+  p.win_pitch = (size_t)wy * 2 * wx * es;
+  p.z_pitch = (size_t)nu * 2 * wy * es;
+  for (int q = 0; q < na; ++q) p.win[q] = take(p.win_pitch * (size_t)n_pokes);
+  if (is_split(prec)) {
+    p.row_scale = take((size_t)n_pokes * wy * 4);
+    p.t = take((size_t)n_pokes * 4);
+  }
+  for (int q = 0; q < na; ++q) p.z[q] = take(p.z_pitch * (size_t)n_pokes);
+  p.total = off;
+  return p;
+}
 }  // namespace
 
 extern "C" {
@@ -701,48 +880,8 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   // Table: stage-1 items ordered by data mode (one launch per mode), then stage-2 items.
   StageArgs* hargs = reinterpret_cast<StageArgs*>(src);
   int64_t* hstart = reinterpret_cast<int64_t*>(src + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
-  struct Launch {
-    int stage, b_mode, cfg, first, n;
-    int64_t start_off, tiles;
-  };
-  std::vector<Launch> launches;
-  int na = 0;
-  int64_t ns = 0;
-  for (int stage = 0; stage < 2; ++stage) {
-    for (int mode = 0; mode < 3; ++mode) {
-      if (stage == 1 && mode != kModePadded) continue;
-      auto member = [&](const Collected& c) { return stage == 0 ? c.b_mode1 == mode : c.nz2 != 0; };
-      // tile shape of this launch from its longest contraction and its size in 64x64 tiles
-      int max_k = 0;
-      int64_t tiles64 = 0;
-      for (int i = 0; i < n_items; ++i) {
-        const Collected& c = col[(size_t)i];
-        if (!member(c)) continue;
-        const StageArgs& a = stage == 0 ? c.a1 : c.a2;
-        max_k = std::max(max_k, a.K);
-        tiles64 += ceil_div(a.M, 64) * ceil_div(a.N, 64) * (stage == 0 ? 1 : c.nz2);
-      }
-      const int cfg = grouped_tile_cfg(prec, stage, max_k, tiles64);
-      int bm, bn;
-      grouped_tile_shape(cfg, &bm, &bn);
-      Launch L{stage, mode, cfg, na, 0, ns, 0};
-      int64_t acc = 0;
-      for (int i = 0; i < n_items; ++i) {
-        const Collected& c = col[(size_t)i];
-        if (!member(c)) continue;
-        const StageArgs& a = stage == 0 ? c.a1 : c.a2;
-        const int64_t nz = stage == 0 ? 1 : c.nz2;
-        hargs[na++] = a;
-        hstart[ns++] = acc;
-        acc += ceil_div(a.M, bm) * ceil_div(a.N, bn) * nz;
-        ++L.n;
-      }
-      if (L.n == 0) continue;
-      hstart[ns++] = acc;
-      L.tiles = acc;
-      launches.push_back(L);
-    }
-  }
+  std::vector<GroupedLaunch> launches;
+  build_grouped_launches(prec, col, hargs, hstart, launches);
   HCIPY_HIP(hipMemcpyAsync(group->dev, src, group->bytes, hipMemcpyHostToDevice, s));
   if (capture == hipStreamCaptureStatusNone) {
     HCIPY_HIP(hipEventRecord(group->copied, s));
@@ -750,8 +889,9 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   }
   const StageArgs* dargs = reinterpret_cast<const StageArgs*>(group->dev);
   const int64_t* dstart = reinterpret_cast<const int64_t*>(group->dev + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
-  for (const Launch& L : launches) {
-    const int st = launch_stage_grouped(prec, L.cfg, L.b_mode, dargs + L.first, dstart + L.start_off, L.n, L.tiles, s);
+  for (const GroupedLaunch& L : launches) {
+    const int st =
+        launch_stage_grouped(prec, L.cfg, L.b_mode, dargs + L.first, dstart + L.start_off, nullptr, L.n, L.tiles, s);
     if (st != kOk) return st;
   }
   return kOk;
@@ -785,6 +925,333 @@ int hcipy_mft_autotune(hcipy_mft_plan plan, int direction_, const void* in, void
     if (chosen != nullptr) chosen[stage] = best;
   }
   if (chosen != nullptr && nstage == 1) chosen[1] = kTileAuto;
+  return kOk;
+}
+
+int hcipy_poke_set_create(hcipy_poke_set* set, const hcipy_poke_item* items, int n_items, void* stream) {
+  HCIPY_REQUIRE(set != nullptr && items != nullptr && n_items > 0, "bad argument");
+  *set = nullptr;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (int st = refuse_under_capture(s, "hcipy_poke_set_create")) return st;
+  int cur = 0;
+  HCIPY_HIP(hipGetDevice(&cur));
+  const int prec = items[0].plan != nullptr ? items[0].plan->p.prec : -1;
+  // geometry of every item: window size and per-actuator origins
+  struct Geo {
+    int64_t wx, wy, n_act2, nu, nv;
+    std::vector<int> org;
+  };
+  std::vector<Geo> geo((size_t)n_items);
+  int64_t total_pokes = 0;
+  for (int k = 0; k < n_items; ++k) {
+    const hcipy_poke_item& it = items[k];
+    HCIPY_REQUIRE(it.plan != nullptr && it.field != nullptr && it.poke_out != nullptr, "item %d: NULL pointer", k);
+    const Plan& pl = it.plan->p;
+    HCIPY_REQUIRE(pl.prec == prec, "item %d: all items of a poke set share one precision", k);
+    HCIPY_REQUIRE(pl.ndim == 2 && pl.device == cur, "item %d: needs a 2-D plan on the current device", k);
+    if (int st = check_gaussian_desc(&it.dm)) return st;
+    HCIPY_REQUIRE(it.dm.nx == pl.nx && it.dm.ny == pl.ny, "item %d: influence functions on a %lld x %lld grid, plan "
+                  "input %lld x %lld", k, (long long)it.dm.nx, (long long)it.dm.ny, (long long)pl.nx, (long long)pl.ny);
+    Geo& g = geo[(size_t)k];
+    g.n_act2 = it.dm.n_act * it.dm.n_act;
+    g.nu = pl.nu;
+    g.nv = pl.nv;
+    int64_t need_x = 1, need_y = 1;
+    std::vector<int64_t> box((size_t)g.n_act2 * 4);
+    for (int64_t a = 0; a < g.n_act2; ++a) {
+      int64_t* b = &box[(size_t)a * 4];
+      gaussian_support_window(it.dm, a, &b[0], &b[1], &b[2], &b[3]);
+      need_x = std::max(need_x, b[1] - b[0] + 1);
+      need_y = std::max(need_y, b[3] - b[2] + 1);
+    }
+    g.wx = std::min(round_up(need_x + 8, 64), round_up(pl.nx, 64));
+    g.wy = std::min(round_up(need_y + 8, 64), round_up(pl.ny, 64));
+    HCIPY_REQUIRE(g.wx <= kPokeWindowMax && g.wy <= kPokeWindowMax,
+                  "item %d: influence-function windows of %lld x %lld pixels exceed %d", k, (long long)g.wx,
+                  (long long)g.wy, kPokeWindowMax);
+    g.org.resize((size_t)g.n_act2 * 2);
+    for (int64_t a = 0; a < g.n_act2; ++a) {
+      const int64_t* b = &box[(size_t)a * 4];
+      g.org[(size_t)a * 2] = (int)window_origin(b[0], b[1], pl.nx, g.wx);
+      g.org[(size_t)a * 2 + 1] = (int)window_origin(b[2], b[3], pl.ny, g.wy);
+    }
+    total_pokes += g.n_act2;
+  }
+  HCIPY_REQUIRE(total_pokes <= (int64_t)1 << 24, "too many pokes (%lld)", (long long)total_pokes);
+
+  // memory layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (size_t)round_up((int64_t)bytes, 256);
+    return at;
+  };
+  std::vector<size_t> o_base_ws((size_t)n_items), o_base((size_t)n_items), o_org((size_t)n_items), o_poke((size_t)n_items);
+  std::vector<PokeWs> pws((size_t)n_items);
+  for (int k = 0; k < n_items; ++k) {
+    const Plan& pl = items[k].plan->p;
+    const Geo& g = geo[(size_t)k];
+    o_base_ws[(size_t)k] = take(std::max<size_t>(workspace_layout(pl, direction(pl, HCIPY_FORWARD), 1).total, 256));
+    o_base[(size_t)k] = take((size_t)g.nu * g.nv * 16);
+    o_org[(size_t)k] = take(g.org.size() * sizeof(int));
+    pws[(size_t)k] = poke_ws(prec, g.n_act2, g.wx, g.wy, g.nu);
+    o_poke[(size_t)k] = take(pws[(size_t)k].total);
+  }
+  const size_t o_win = take((size_t)poke_window_bytes(n_items));
+  const size_t o_btab = take(2 * (size_t)n_items * sizeof(StageArgs));
+  const size_t o_bst = take((2 * (size_t)n_items + 4) * sizeof(int64_t));
+  const size_t o_ptab = take(2 * (size_t)total_pokes * sizeof(StageArgs));
+  const size_t o_pst = take((2 * (size_t)total_pokes + 4) * sizeof(int64_t));
+
+  hcipy_poke_set_s* h = new (std::nothrow) hcipy_poke_set_s();
+  HCIPY_REQUIRE(h != nullptr, "out of host memory");
+  h->prec = prec;
+  h->device = cur;
+  h->n_items = n_items;
+  h->mem_bytes = off;
+  auto fail = [&](int st) {
+    hcipy_poke_set_destroy(h);
+    return st;
+  };
+  if (int st = check_hip(hipMalloc(&h->mem, off), "hipMalloc(poke set)")) return fail(st);
+  char* mem = h->mem;
+
+  std::vector<Collected> bcol((size_t)n_items), pcol((size_t)total_pokes);
+  std::vector<PokeWinItem> win((size_t)n_items);
+  const size_t es = elem_size(prec);
+  const int out_mode = prec == kF64 ? kOutI64Base : kOutI32Base;
+  int64_t row0 = 0, pk = 0;
+  for (int k = 0; k < n_items; ++k) {
+    const hcipy_poke_item& it = items[k];
+    const Plan& pl = it.plan->p;
+    const Geo& g = geo[(size_t)k];
+    hcipy_poke_set_s::Item bi{&pl, it.field, mem + o_base_ws[(size_t)k], reinterpret_cast<double2*>(mem + o_base[(size_t)k]),
+                              it.base_out, g.nu * g.nv};
+    h->items.push_back(bi);
+    // base transform: stage arguments only (its split is launched by each execution)
+    int st = run_direction(pl, HCIPY_FORWARD, it.field, bi.base, 1, it.alpha_re, it.alpha_im, bi.ws, s, nullptr,
+                           HCIPY_OUTPUT_FIELD_C128, 1, &bcol[(size_t)k], nullptr);
+    if (st) return fail(st);
+    int* org_dev = reinterpret_cast<int*>(mem + o_org[(size_t)k]);
+    if ((st = check_hip(hipMemcpy(org_dev, g.org.data(), g.org.size() * sizeof(int), hipMemcpyHostToDevice),
+                        "copy poke window origins")))
+      return fail(st);
+    const PokeWs& w = pws[(size_t)k];
+    char* p0 = mem + o_poke[(size_t)k];
+    PokeWinItem& wi = win[(size_t)k];
+    wi.dm = it.dm;
+    wi.field = it.field;
+    wi.phase_scale = it.phase_scale;
+    wi.prec = prec;
+    wi.wx = (int)g.wx;
+    wi.wy = (int)g.wy;
+    wi.row0 = row0;
+    wi.org = org_dev;
+    wi.row_scale = is_split(prec) ? reinterpret_cast<float*>(p0 + w.row_scale) : nullptr;
+    // the window kernel writes actuator a's rows at out[q] + a * wy * 2 wx elements
+    for (int q = 0; q < 3; ++q) wi.out[q] = q < n_arrays(prec) ? p0 + w.win[q] : nullptr;
+    row0 += g.n_act2 * g.wy;
+    const FMatrix* m1 = &pl.f1;
+    const FMatrix* m2 = &pl.f2;
+    for (int64_t a = 0; a < g.n_act2; ++a, ++pk) {
+      char* win_a[3], *z_a[3];
+      for (int q = 0; q < 3; ++q) {
+        win_a[q] = q < n_arrays(prec) ? p0 + w.win[q] + (size_t)a * w.win_pitch : nullptr;
+        z_a[q] = q < n_arrays(prec) ? p0 + w.z[q] + (size_t)a * w.z_pitch : nullptr;
+      }
+      const float* rs_a = is_split(prec) ? reinterpret_cast<const float*>(p0 + w.row_scale) + a * g.wy : nullptr;
+      float* t_a = is_split(prec) ? reinterpret_cast<float*>(p0 + w.t) + a : nullptr;
+      const int64_t x0 = g.org[(size_t)a * 2], y0 = g.org[(size_t)a * 2 + 1];
+      Collected& c = pcol[(size_t)pk];
+      StageArgs& a1 = c.a1;
+      a1.a = static_cast<const char*>(m1->hi) + 2 * x0 * es;
+      a1.a_lo = m1->lo ? static_cast<const char*>(m1->lo) + 2 * x0 * es : nullptr;
+      a1.a_lo2 = m1->lo2 ? static_cast<const char*>(m1->lo2) + 2 * x0 * es : nullptr;
+      a1.lda = m1->lda;
+      a1.b = win_a[0];
+      a1.b_lo = win_a[1];
+      a1.b_lo2 = win_a[2];
+      a1.ldb = 2 * g.wx;
+      a1.M = (int)(2 * g.nu);
+      a1.N = (int)g.wy;
+      a1.K = (int)(2 * g.wx);
+      a1.t_log2k = ilog2_ceil(a1.K);
+      a1.out = z_a[0];
+      a1.out_lo = z_a[1];
+      a1.out_lo2 = z_a[2];
+      a1.ldo = g.wy;
+      a1.fold = (int)g.wy;
+      a1.fold_stride = g.nu * g.wy;
+      a1.out_mode = is_split(prec) ? kOutSplit : (prec == kF64 ? kOutC128 : kOutC64);
+      a1.row_scale = rs_a;
+      a1.t_out = t_a;
+      a1.alpha_re = 1.0;
+      a1.alpha_im = 0.0;
+      a1.tile_cfg = kTileAuto;
+      StageArgs& a2 = c.a2;
+      a2.a = static_cast<const char*>(m2->hi) + 2 * y0 * es;
+      a2.a_lo = m2->lo ? static_cast<const char*>(m2->lo) + 2 * y0 * es : nullptr;
+      a2.a_lo2 = m2->lo2 ? static_cast<const char*>(m2->lo2) + 2 * y0 * es : nullptr;
+      a2.lda = m2->lda;
+      a2.b = z_a[0];
+      a2.b_lo = a1.out_lo;
+      a2.b_lo2 = a1.out_lo2;
+      a2.ldb = 2 * g.wy;
+      a2.b_zstride = g.nu * 2 * g.wy;
+      a2.M = (int)(2 * g.nv);
+      a2.N = (int)g.nu;
+      a2.K = (int)(2 * g.wy);
+      a2.out = static_cast<char*>(it.poke_out) + (size_t)a * g.nu * g.nv * (prec == kF64 ? 8 : 4);
+      a2.ldo = g.nu;
+      a2.out_zstride = g.nu * g.nv;
+      a2.out_mode = out_mode;
+      a2.base = bi.base;
+      a2.z_scale = t_a;
+      a2.alpha_re = it.alpha_re;
+      a2.alpha_im = it.alpha_im;
+      a2.tile_cfg = kTileAuto;
+      c.b_mode1 = kModePadded;
+      c.nz2 = 1;
+    }
+  }
+  h->total_rows = row0;
+  h->n_pokes = total_pokes;
+  for (const Geo& g : geo) h->max_wx = std::max(h->max_wx, (int)g.wx);
+  for (int k = 0; k < n_items; ++k) {
+    const Plan& pl = items[k].plan->p;
+    const Geo& g = geo[(size_t)k];
+    const double full = 8.0 * ((double)pl.ny * pl.nx * pl.nu + (double)pl.nv * pl.ny * pl.nu);
+    const double win = 8.0 * ((double)g.wy * g.wx * g.nu + (double)g.nv * g.wy * g.nu);
+    h->flops_full += full * (double)(g.n_act2 + 1);
+    h->flops_windowed += full + win * (double)g.n_act2;
+  }
+  std::vector<char> host((size_t)poke_window_bytes(n_items));
+  h->win_items = mem + o_win;
+  int st = upload_poke_items(win.data(), n_items, host.data(), h->win_items, s);
+  if (!st) st = check_hip(hipStreamSynchronize(s), "poke set descriptors");
+  if (st) return fail(st);
+  // tables
+  std::vector<StageArgs> tab(2 * (size_t)std::max<int64_t>(total_pokes, n_items));
+  std::vector<int64_t> start(2 * (size_t)std::max<int64_t>(total_pokes, n_items) + 4);
+  build_grouped_launches(prec, bcol, tab.data(), start.data(), h->base_launches);
+  if ((st = check_hip(hipMemcpy(mem + o_btab, tab.data(), 2 * (size_t)n_items * sizeof(StageArgs), hipMemcpyHostToDevice),
+                      "copy base table")))
+    return fail(st);
+  if ((st = check_hip(hipMemcpy(mem + o_bst, start.data(), (2 * (size_t)n_items + 4) * sizeof(int64_t),
+                                hipMemcpyHostToDevice), "copy base tile starts")))
+    return fail(st);
+  // tile maps sized for the smallest tile shape of any grouped launch (64 x 32)
+  for (const Collected& c : pcol)
+    h->map_cap += (size_t)(ceil_div(c.a1.M, 64) * ceil_div(c.a1.N, 32) + ceil_div(c.a2.M, 64) * ceil_div(c.a2.N, 32));
+  if ((st = check_hip(hipMalloc(&h->map_mem, std::max<size_t>(h->map_cap, 1) * sizeof(int)), "hipMalloc(tile map)")))
+    return fail(st);
+  h->pcol = std::move(pcol);
+  h->ptab_dev = reinterpret_cast<StageArgs*>(mem + o_ptab);
+  h->pst_dev = reinterpret_cast<int64_t*>(mem + o_pst);
+  if ((st = upload_poke_tables(h, h->cfg))) return fail(st);
+  h->base_tab = reinterpret_cast<const StageArgs*>(mem + o_btab);
+  h->base_start = reinterpret_cast<const int64_t*>(mem + o_bst);
+  *set = h;
+  return kOk;
+}
+
+int hcipy_poke_set_bytes(hcipy_poke_set set, size_t* bytes) {
+  HCIPY_REQUIRE(set != nullptr && bytes != nullptr, "NULL argument");
+  *bytes = set->mem_bytes;
+  return kOk;
+}
+
+int hcipy_poke_set_info(hcipy_poke_set set, int64_t* n_pokes, double* flops_windowed, double* flops_full) {
+  HCIPY_REQUIRE(set != nullptr, "NULL poke set");
+  if (n_pokes) *n_pokes = set->n_pokes;
+  if (flops_windowed) *flops_windowed = set->flops_windowed;
+  if (flops_full) *flops_full = set->flops_full;
+  return kOk;
+}
+
+int hcipy_poke_set_execute(hcipy_poke_set set, void* stream) {
+  HCIPY_REQUIRE(set != nullptr, "NULL poke set");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int cur = 0;
+  HCIPY_HIP(hipGetDevice(&cur));
+  HCIPY_REQUIRE(cur == set->device, "the poke set was created on device %d, current device is %d", set->device, cur);
+  const int prec = set->prec;
+  if (is_split(prec)) {
+    for (const auto& it : set->items) {
+      const Plan& pl = *it.pl;
+      const Ws w = workspace_layout(pl, direction(pl, HCIPY_FORWARD), 1);
+      SplitOut o{{it.ws + w.split_hi, it.ws + w.split_lo, prec == kBF16x3 ? it.ws + w.split_lo2 : nullptr}};
+      if (int st = launch_split_rows(prec, static_cast<const float*>(it.field), 2 * pl.nx, (int)(2 * pl.nx), pl.ny, o,
+                                     w.ld1, reinterpret_cast<float*>(it.ws + w.row_scale), s))
+        return st;
+    }
+  }
+  if (int st = launch_poke_windows(set->win_items, set->n_items, set->total_rows, set->max_wx, s)) return st;
+  for (const GroupedLaunch& L : set->base_launches)
+    if (int st = launch_stage_grouped(prec, L.cfg, L.b_mode, set->base_tab + L.first, set->base_start + L.start_off,
+                                      nullptr, L.n, L.tiles, s))
+      return st;
+  for (const GroupedLaunch& L : set->poke_launches)
+    if (int st = launch_stage_grouped(prec, L.cfg, L.b_mode, set->poke_tab + L.first, set->poke_start + L.start_off,
+                                      set->poke_maps[&L - set->poke_launches.data()], L.n, L.tiles, s))
+      return st;
+  for (const auto& it : set->items) {
+    if (it.base_out == nullptr) continue;
+    hipLaunchKernelGGL(c128_intensity_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(it.n_focal, 256), 4096)),
+                       dim3(256), 0, s, it.base, it.base_out, prec == kF64 ? 1 : 0, it.n_focal);
+    HCIPY_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+
+int hcipy_poke_set_autotune(hcipy_poke_set set, int repeats, int* chosen, void* stream) {
+  HCIPY_REQUIRE(set != nullptr && repeats > 0, "bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (int st = refuse_under_capture(s, "hcipy_poke_set_autotune")) return st;
+  hipEvent_t ev[2];
+  HCIPY_HIP(hipEventCreate(&ev[0]));
+  if (int st = check_hip(hipEventCreate(&ev[1]), "hipEventCreate")) {
+    (void)hipEventDestroy(ev[0]);
+    return st;
+  }
+  int st = kOk;
+  for (int stage = 0; stage < 2 && !st; ++stage) {
+    int best = -1;
+    float best_ms = 0.0f;
+    for (int cfg = 0; cfg < kNumTileCfg && !st; ++cfg) {
+      if (!grouped_cfg_supported(set->prec, cfg)) continue;
+      int trial[2] = {set->cfg[0], set->cfg[1]};
+      trial[stage] = cfg;
+      if ((st = upload_poke_tables(set, trial))) break;
+      if ((st = hcipy_poke_set_execute(set, stream))) break;  // warm-up
+      if ((st = check_hip(hipEventRecord(ev[0], s), "hipEventRecord"))) break;
+      for (int r = 0; r < repeats && !st; ++r) st = hcipy_poke_set_execute(set, stream);
+      if (!st) st = check_hip(hipEventRecord(ev[1], s), "hipEventRecord");
+      if (!st) st = check_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
+      float ms = 0.0f;
+      if (!st) st = check_hip(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
+      if (!st && (best < 0 || ms < best_ms)) {
+        best = cfg;
+        best_ms = ms;
+      }
+    }
+    if (!st) set->cfg[stage] = best;
+  }
+  if (!st) st = upload_poke_tables(set, set->cfg);
+  if (!st && chosen != nullptr) {
+    chosen[0] = set->cfg[0];
+    chosen[1] = set->cfg[1];
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return st;
+}
+
+int hcipy_poke_set_destroy(hcipy_poke_set set) {
+  if (set == nullptr) return kOk;
+  if (set->mem) (void)hipFree(set->mem);
+  if (set->map_mem) (void)hipFree(set->map_mem);
+  delete set;
   return kOk;
 }
 

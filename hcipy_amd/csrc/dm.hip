@@ -17,9 +17,11 @@
 //   IF_a(p) = mean over the oversampling dithers of g, divided by cos(x_tilt) cos(y_tilt)
 // with dithers ((k + 0.5) / ovs - 0.5) * delta per axis (evaluate_supersampled,
 // hcipy/field/util.py:606-698).
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
+#include "pokes.hpp"
 #include "../../include/hcipy_amd.h"
 
 namespace hcipy {
@@ -300,6 +302,140 @@ __global__ __launch_bounds__(256) void dm_table_adjoint_kernel(DmTableGeom g, co
   if (threadIdx.x == 0) gw[b * g.n_act * g.n_act + a] = (red[0] + red[1] + red[2] + red[3]) * g.norm;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Poke windows: for every single-actuator poke of a poke set, the change of the pupil field
+//   dE = E (exp(i s IF_a) - 1),   s = 2 k h (phase of the poke height h),
+// evaluated only on the wx x wy pixel window that holds IF_a's support (dE is exactly 0 outside it),
+// and written as rows of the stage-1 data operand in the plan's format: FP16 hi/lo or BF16
+// b0/b1/b2 splits with a power-of-two row scale, or raw complex64 / complex128.  exp(i phi) - 1 is
+// formed as (-2 sin^2(phi/2), sin(phi)) so that small phases keep their relative accuracy.
+// One wave per window row; the row's values stay in registers (wx <= kPokeMaxW) for the row
+// maximum of the split formats.
+
+constexpr int kPokeMaxW = kPokeWindowMax;
+
+struct PokeWinItemDev {
+  DmGeom g;
+  const void* field;     // base pupil field, complex64 (complex128 when prec is FP64)
+  double phase_scale;    // s
+  int prec;              // 0 FP64, 1 FP32, 2 FP16 hi/lo, 3 BF16 three-way (the Prec of mft_gemm.hpp)
+  int wx, wy;            // window width and height in pixels (multiples of 64, wx <= kPokeMaxW)
+  int64_t row0;          // first global window row of this item
+  int64_t n_rows;        // n_act^2 * wy
+  const int* org;        // window origin (x0, y0) per actuator
+  void* out[3];          // [actuator][wy][2 wx] values of each output array
+  float* row_scale;      // [actuator][wy] (split formats)
+};
+
+template <int NPL>  // window pixels per lane at most (wx <= 64 NPL for every item)
+__global__ __launch_bounds__(256) void poke_window_kernel(const PokeWinItemDev* __restrict__ items, int n_items,
+                                                          int64_t total_rows) {
+  const int lane = threadIdx.x & 63;
+  const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (R >= total_rows) return;
+  int lo = 0, hi = n_items - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].row0 <= R) lo = mid;
+    else hi = mid - 1;
+  }
+  const PokeWinItemDev& it = items[lo];
+  const DmGeom& g = it.g;
+  const int w = it.wx;
+  const int64_t local = R - it.row0;
+  const int64_t a = local / it.wy;
+  const int r = (int)(local - a * it.wy);
+  const int64_t x0 = it.org[2 * a], y0 = it.org[2 * a + 1];
+  const int64_t j = y0 + r;
+  const double mx = g.ax0 + (double)(a % g.n_act) * g.pitch;
+  const double my = g.ay0 + (double)(a / g.n_act) * g.pitch;
+  // the row's values stay in registers: pixel c = lane + 64 t of the window
+  double2 val[NPL];
+  double mxv = 0.0;
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    const int c = lane + 64 * t;
+    const int64_t i = x0 + c;
+    double2 d = make_double2(0.0, 0.0);
+    if (c < w && i < g.nx && j < g.ny) {
+      double ifv = 0.0;
+      for (int v = 0; v < g.ovs; ++v)
+        for (int u = 0; u < g.ovs; ++u) {
+          double qx, qy;
+          eval_point(g, i, j, u, v, qx, qy);
+          ifv += poke(g, qx, qy, mx, my);
+        }
+      if (ifv != 0.0) {
+        // exp(i phi) - 1 = (-2 sin^2(phi/2), 2 sin(phi/2) cos(phi/2)): one sincos, no cancellation
+        double sh, ch;
+        sincos(0.5 * it.phase_scale * ifv * g.norm, &sh, &ch);
+        const double er = -2.0 * sh * sh, ei = 2.0 * sh * ch;
+        double fr, fi;
+        if (it.prec == 0) {
+          const double2 e = static_cast<const double2*>(it.field)[j * g.nx + i];
+          fr = e.x;
+          fi = e.y;
+        } else {
+          const float2 e = static_cast<const float2*>(it.field)[j * g.nx + i];
+          fr = e.x;
+          fi = e.y;
+        }
+        d = make_double2(fr * er - fi * ei, fr * ei + fi * er);
+        if (it.prec != 0) {
+          // complex64 values of the poke change, as the split of a complex64 field would see them
+          d = make_double2((double)(float)d.x, (double)(float)d.y);
+          mxv = fmax(mxv, fmax(fabs(d.x), fabs(d.y)));
+        }
+      }
+    }
+    val[t] = d;
+  }
+  const int64_t row_off = (a * it.wy + r) * (int64_t)(2 * w);  // reals
+  if (it.prec == 0 || it.prec == 1) {
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) {
+      const int c = lane + 64 * t;
+      if (c >= w) continue;
+      if (it.prec == 0) static_cast<double2*>(it.out[0])[row_off / 2 + c] = val[t];
+      else static_cast<float2*>(it.out[0])[row_off / 2 + c] = make_float2((float)val[t].x, (float)val[t].y);
+    }
+    return;
+  }
+  const float m = wave_max((float)mxv);
+  float s = 1.0f;
+  if (m > 0.0f && m <= 3.0e38f) {
+    int e;
+    frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)  =>  m 2^-e < 1
+    s = ldexpf(1.0f, -e);
+  }
+  if (lane == 0) it.row_scale[a * it.wy + r] = s;
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    const int c = lane + 64 * t;
+    if (c >= w) continue;
+    const float vr = (float)val[t].x * s, vi = (float)val[t].y * s;
+    if (it.prec == 2) {
+      const _Float16 hr = (_Float16)vr, hi = (_Float16)vi;
+      static_cast<half2_t*>(it.out[0])[row_off / 2 + c] = half2_t{hr, hi};
+      static_cast<half2_t*>(it.out[1])[row_off / 2 + c] =
+          half2_t{(_Float16)((vr - (float)hr) * 2048.0f), (_Float16)((vi - (float)hi) * 2048.0f)};
+    } else {
+      const float v2[2] = {vr, vi};
+      __bf16 q0[2], q1[2], q2[2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        q0[e] = (__bf16)v2[e];
+        const float rr = v2[e] - (float)q0[e];
+        q1[e] = (__bf16)rr;
+        q2[e] = (__bf16)(rr - (float)q1[e]);
+      }
+      static_cast<bf16x2_t*>(it.out[0])[row_off / 2 + c] = bf16x2_t{q0[0], q0[1]};
+      static_cast<bf16x2_t*>(it.out[1])[row_off / 2 + c] = bf16x2_t{q1[0], q1[1]};
+      static_cast<bf16x2_t*>(it.out[2])[row_off / 2 + c] = bf16x2_t{q2[0], q2[1]};
+    }
+  }
+}
+
 int check_table_desc(const hcipy_dm_table_desc* d) {
   HCIPY_REQUIRE(d != nullptr, "NULL descriptor");
   HCIPY_REQUIRE(d->nx > 0 && d->ny > 0 && d->n_act > 0, "bad grid or actuator count");
@@ -321,6 +457,61 @@ int check_desc(const hcipy_dm_gaussian_desc* d) {
 }
 
 }  // namespace
+
+int check_gaussian_desc(const hcipy_dm_gaussian_desc* d) { return check_desc(d); }
+
+// Pixel window [i0, i1] x [j0, j1] (inclusive, clipped to the grid) that holds the support of
+// actuator a's influence function, widened by one pixel for the dithers.
+void gaussian_support_window(const hcipy_dm_gaussian_desc& d, int64_t a, int64_t* i0, int64_t* i1, int64_t* j0,
+                             int64_t* j1) {
+  const DmGeom g = make_geom(d);
+  const double mx = g.ax0 + (double)(a % g.n_act) * g.pitch;
+  const double my = g.ay0 + (double)(a / g.n_act) * g.pitch;
+  const double cx = (g.cz * mx - g.sz * my) / g.sx;
+  const double cy = (g.sz * mx + g.cz * my) / g.sy;
+  *i0 = std::max((int64_t)0, (int64_t)std::floor((cx - g.radius - g.x0) / g.dx) - 1);
+  *i1 = std::min(g.nx - 1, (int64_t)std::ceil((cx + g.radius - g.x0) / g.dx) + 1);
+  *j0 = std::max((int64_t)0, (int64_t)std::floor((cy - g.radius - g.y0) / g.dy) - 1);
+  *j1 = std::min(g.ny - 1, (int64_t)std::ceil((cy + g.radius - g.y0) / g.dy) + 1);
+}
+
+int poke_window_bytes(int n_items) { return n_items * (int)sizeof(PokeWinItemDev); }
+
+// Writes the device descriptors of `n` items to `dev_items` (device memory of poke_window_bytes)
+// through the pinned host buffer `host` and returns; the copy is ordered on `s`.
+int upload_poke_items(const PokeWinItem* items, int n, void* host, void* dev_items, hipStream_t s) {
+  PokeWinItemDev* h = static_cast<PokeWinItemDev*>(host);
+  for (int k = 0; k < n; ++k) {
+    h[k].g = make_geom(items[k].dm);
+    h[k].field = items[k].field;
+    h[k].phase_scale = items[k].phase_scale;
+    h[k].prec = items[k].prec;
+    h[k].wx = items[k].wx;
+    h[k].wy = items[k].wy;
+    h[k].row0 = items[k].row0;
+    h[k].n_rows = items[k].dm.n_act * items[k].dm.n_act * (int64_t)items[k].wy;
+    h[k].org = items[k].org;
+    for (int q = 0; q < 3; ++q) h[k].out[q] = items[k].out[q];
+    h[k].row_scale = items[k].row_scale;
+  }
+  HCIPY_HIP(hipMemcpyAsync(dev_items, host, (size_t)n * sizeof(PokeWinItemDev), hipMemcpyHostToDevice, s));
+  return kOk;
+}
+
+int launch_poke_windows(const void* dev_items, int n_items, int64_t total_rows, int max_wx, hipStream_t s) {
+  if (total_rows <= 0) return kOk;
+  HCIPY_REQUIRE(ceil_div(total_rows, 4) <= INT32_MAX, "too many poke window rows");
+  HCIPY_REQUIRE(max_wx >= 1 && max_wx <= kPokeMaxW, "poke window width %d", max_wx);
+  const dim3 g((unsigned)ceil_div(total_rows, 4));
+  const auto* it = static_cast<const PokeWinItemDev*>(dev_items);
+  if (max_wx <= 64) hipLaunchKernelGGL(poke_window_kernel<1>, g, dim3(256), 0, s, it, n_items, total_rows);
+  else if (max_wx <= 128) hipLaunchKernelGGL(poke_window_kernel<2>, g, dim3(256), 0, s, it, n_items, total_rows);
+  else if (max_wx <= 256) hipLaunchKernelGGL(poke_window_kernel<4>, g, dim3(256), 0, s, it, n_items, total_rows);
+  else if (max_wx <= 512) hipLaunchKernelGGL(poke_window_kernel<8>, g, dim3(256), 0, s, it, n_items, total_rows);
+  else hipLaunchKernelGGL(poke_window_kernel<16>, g, dim3(256), 0, s, it, n_items, total_rows);
+  return check_hip(hipGetLastError(), "poke_window_kernel launch");
+}
+
 }  // namespace hcipy
 
 using namespace hcipy;

@@ -635,6 +635,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
           reinterpret_cast<float*>(p.out)[idx] = (float)(orr * orr + oii * oii);
         } else if (p.out_mode == kOutI64) {
           reinterpret_cast<double*>(p.out)[idx] = orr * orr + oii * oii;
+        } else if (p.out_mode == kOutI32Base || p.out_mode == kOutI64Base) {
+          const double2 bz = static_cast<const double2*>(p.base)[idx];
+          const double xr = orr + bz.x, xi = oii + bz.y;
+          if (p.out_mode == kOutI32Base) reinterpret_cast<float*>(p.out)[idx] = (float)(xr * xr + xi * xi);
+          else reinterpret_cast<double*>(p.out)[idx] = xr * xr + xi * xi;
         } else if constexpr (P == kBF16x3) {
           // b0 through FP32 (a double rounding moves it by one ulp at most); b1 and b2 take the
           // exact FP64 remainders
@@ -677,13 +682,19 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_kernel(StageArgs 
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
 __global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_grouped_kernel(const StageArgs* __restrict__ table,
                                                                               const int64_t* __restrict__ tile_start,
+                                                                              const int* __restrict__ tile_item,
                                                                               int n_items) {
   const int64_t b = blockIdx.x;
-  int lo = 0, hi = n_items - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tile_start[mid] <= b) lo = mid;
-    else hi = mid - 1;
+  int lo = 0;
+  if (tile_item != nullptr) {
+    lo = tile_item[b];  // item of every workgroup, tabulated by the caller
+  } else {
+    int hi = n_items - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= b) lo = mid;
+      else hi = mid - 1;
+    }
   }
   const StageArgs p = table[lo];
   const int64_t tiles_xy = (int64_t)((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
@@ -1659,7 +1670,7 @@ static int launch_bf16x3(const StageArgs& a, int nz, hipStream_t s) {
 }
 
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
-static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, int n, int64_t tiles, hipStream_t s) {
+static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, const int* tile_item, int n, int64_t tiles, hipStream_t s) {
   using T = typename PrecTraits<P>::T;
   constexpr int VEC = 16 / (int)sizeof(T);
   constexpr int NARR = PrecTraits<P>::NARR;
@@ -1673,22 +1684,22 @@ static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, i
     configured = true;
   }
   HCIPY_REQUIRE(tiles > 0 && tiles <= INT32_MAX, "grouped launch of %lld workgroups", (long long)tiles);
-  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * WM * WN * WK), lds, s, table, tile_start, n);
+  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * WM * WN * WK), lds, s, table, tile_start, tile_item, n);
   return check_hip(hipGetLastError(), "mft_stage_grouped_kernel launch");
 }
 
 template <int P, int BK, int BMODE, int PF>
-static int launch_grouped_prec(int cfg, const StageArgs* table, const int64_t* tile_start, int n, int64_t tiles,
+static int launch_grouped_prec(int cfg, const StageArgs* table, const int64_t* tile_start, const int* tile_item, int n, int64_t tiles,
                                hipStream_t s) {
   switch (cfg) {
-    case k64x64_w2x2: return launch_grouped_t<P, 64, 64, BK, 2, 2, 1, BMODE, PF>(table, tile_start, n, tiles, s);
-    case k64x32_w2x1: return launch_grouped_t<P, 64, 32, BK, 2, 1, 2, BMODE, PF>(table, tile_start, n, tiles, s);
-    case k64x64_w8: return launch_grouped_t<P, 64, 64, BK, 2, 2, 2, BMODE, PF>(table, tile_start, n, tiles, s);
+    case k64x64_w2x2: return launch_grouped_t<P, 64, 64, BK, 2, 2, 1, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
+    case k64x32_w2x1: return launch_grouped_t<P, 64, 32, BK, 2, 1, 2, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
+    case k64x64_w8: return launch_grouped_t<P, 64, 64, BK, 2, 2, 2, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
     default: break;
   }
   if constexpr (P != kF64) {
-    if (cfg == k64x32_w8) return launch_grouped_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(table, tile_start, n, tiles, s);
-    if (cfg == k128x64) return launch_grouped_t<P, 128, 64, BK / 2, 2, 2, 1, BMODE, PF>(table, tile_start, n, tiles, s);
+    if (cfg == k64x32_w8) return launch_grouped_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
+    if (cfg == k128x64) return launch_grouped_t<P, 128, 64, BK / 2, 2, 2, 1, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
   }
   set_last_error("tile configuration %d is not available to grouped launches", cfg);
   return kInvalidValue;
@@ -1710,33 +1721,40 @@ int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64) {
   return k64x32_w8;
 }
 
+// Tile shapes launch_stage_grouped instantiates for each precision.
+bool grouped_cfg_supported(int prec, int cfg) {
+  if (prec == kF64) return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8;
+  if (prec == kBF16x3) return cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8;
+  return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8 || cfg == k64x32_w8 || cfg == k128x64;
+}
+
 void grouped_tile_shape(int cfg, int* bm, int* bn) {
   *bm = cfg == k128x64 ? 128 : 64;
   *bn = (cfg == k64x32_w2x1 || cfg == k64x32_w8) ? 32 : 64;
 }
 
-int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, const int64_t* tile_start, int n,
-                         int64_t tiles, hipStream_t s) {
+int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, const int64_t* tile_start,
+                         const int* tile_item, int n, int64_t tiles, hipStream_t s) {
   if (n <= 0 || tiles <= 0) return kOk;
   if (prec == kF64) {
-    if (b_mode == kRowsPadded) return launch_grouped_prec<kF64, 32, kRowsPadded, 4>(cfg, table, tile_start, n, tiles, s);
-    if (b_mode == kRowsAligned) return launch_grouped_prec<kF64, 32, kRowsAligned, 4>(cfg, table, tile_start, n, tiles, s);
-    return launch_grouped_prec<kF64, 32, kRowsUnaligned, 4>(cfg, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsPadded) return launch_grouped_prec<kF64, 32, kRowsPadded, 4>(cfg, table, tile_start, tile_item, n, tiles, s);
+    if (b_mode == kRowsAligned) return launch_grouped_prec<kF64, 32, kRowsAligned, 4>(cfg, table, tile_start, tile_item, n, tiles, s);
+    return launch_grouped_prec<kF64, 32, kRowsUnaligned, 4>(cfg, table, tile_start, tile_item, n, tiles, s);
   }
   if (prec == kF32) {
-    if (b_mode == kRowsPadded) return launch_grouped_prec<kF32, 64, kRowsPadded, 2>(cfg, table, tile_start, n, tiles, s);
-    if (b_mode == kRowsAligned) return launch_grouped_prec<kF32, 64, kRowsAligned, 2>(cfg, table, tile_start, n, tiles, s);
-    return launch_grouped_prec<kF32, 64, kRowsUnaligned, 2>(cfg, table, tile_start, n, tiles, s);
+    if (b_mode == kRowsPadded) return launch_grouped_prec<kF32, 64, kRowsPadded, 2>(cfg, table, tile_start, tile_item, n, tiles, s);
+    if (b_mode == kRowsAligned) return launch_grouped_prec<kF32, 64, kRowsAligned, 2>(cfg, table, tile_start, tile_item, n, tiles, s);
+    return launch_grouped_prec<kF32, 64, kRowsUnaligned, 2>(cfg, table, tile_start, tile_item, n, tiles, s);
   }
   if (prec == kF16x2) {
     HCIPY_REQUIRE(b_mode == kRowsPadded, "split-FP16 stage requires a padded data operand");
-    return launch_grouped_prec<kF16x2, 128, kRowsPadded, 2>(cfg, table, tile_start, n, tiles, s);
+    return launch_grouped_prec<kF16x2, 128, kRowsPadded, 2>(cfg, table, tile_start, tile_item, n, tiles, s);
   }
   if (prec == kBF16x3) {
     HCIPY_REQUIRE(b_mode == kRowsPadded, "split-BF16 stage requires a padded data operand");
-    if (cfg == k64x32_w2x1) return launch_grouped_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRowsPadded, 2>(table, tile_start, n, tiles, s);
-    if (cfg == k128x64) return launch_grouped_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRowsPadded, 2>(table, tile_start, n, tiles, s);
-    if (cfg == k64x32_w8) return launch_grouped_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRowsPadded, 2>(table, tile_start, n, tiles, s);
+    if (cfg == k64x32_w2x1) return launch_grouped_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRowsPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+    if (cfg == k128x64) return launch_grouped_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRowsPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+    if (cfg == k64x32_w8) return launch_grouped_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRowsPadded, 2>(table, tile_start, tile_item, n, tiles, s);
     set_last_error("tile configuration %d is not available to grouped BF16x3 launches", cfg);
     return kInvalidValue;
   }

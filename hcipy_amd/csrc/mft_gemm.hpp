@@ -48,6 +48,8 @@ enum OutMode : int {
   kOutSplit = 2,  // interleaved split arrays (FP16 hi/lo or BF16 b0/b1/b2) feeding the next split stage
   kOutI32 = 3,    // |z|^2 as float32, evaluated from the FP64 result
   kOutI64 = 4,    // |z|^2 as float64
+  kOutI32Base = 5,  // |z + base|^2 as float32 (base: complex128 array of the output's shape)
+  kOutI64Base = 6,  // |z + base|^2 as float64
 };
 
 template <int P> struct PrecTraits;
@@ -119,6 +121,7 @@ struct StageArgs {
   int64_t ldo;                  // complex elements between consecutive constant rows
   int64_t out_zstride;          // complex elements between batch items along gridDim.z
   int out_mode;
+  const void* base;             // complex128 field added before |.|^2 (kOutI32Base / kOutI64Base)
   int nc_layout;                // 1: out[n * ldo + c] instead of out[c * ldo + n]
   // Folding of several fields into the N dimension: n = item * fold + y is written to
   // out[item * fold_stride + c * ldo + y].  fold = 0 disables folding.

@@ -20,7 +20,7 @@ from .field import Field, default_device, make_uniform_grid
 from .optics import OpticalElement, Wavefront
 
 __all__ = ['TipTiltMirror', 'make_actuator_positions', 'make_gaussian_influence_functions', 'GaussianInfluenceFunctions',
-           'make_xinetics_influence_functions', 'XineticsInfluenceFunctions', 'DeformableMirror']
+           'make_xinetics_influence_functions', 'XineticsInfluenceFunctions', 'DeformableMirror', 'PokeSet']
 
 
 def make_actuator_positions(num_actuators_across_pupil, actuator_spacing, x_tilt=0, y_tilt=0, z_tilt=0):
@@ -367,3 +367,100 @@ class TipTiltMirror(OpticalElement):
 
     def backward(self, wavefront):
         return self._apply(wavefront, -1)
+
+
+class PokeSet:
+    """Focal-plane intensities of every single-actuator poke of Gaussian deformable mirrors, for
+    several propagation items at once (``hcipy_poke_set_*``): the DM-Jacobian workload of
+    broadband wavefront control, where the reference pokes each actuator
+    (DeformableMirror.actuators, hcipy/optics/deformable_mirror.py:200-263) and propagates every
+    poked wavefront through the per-wavelength Fraunhofer instance
+    (hcipy/optics/optical_element.py:297-380).
+
+    ``items``: sequence of ``(propagator, wavelength, field, influence_functions)`` with a
+    FraunhoferPropagator, the base pupil field (complex64, or complex128 for FP64) on its input
+    grid and GaussianInfluenceFunctions on that grid.  ``poke_height`` is the surface height of
+    one poke (meters; the poke multiplies the field by exp(2 i k h IF_a)).  All items share one
+    precision (the propagators' complex64 arithmetic, or FP64 for complex128 fields).
+
+    ``execute()`` recomputes, from the current contents of the field tensors (``self.fields``,
+    which may be overwritten in place), ``self.base[k]`` = |E_focal|^2 of item k and
+    ``self.pokes[k][a]`` = |E_focal|^2 with actuator a poked, and returns both lists.  Memory
+    (workspaces, launch tables) is allocated once here; executions allocate nothing and can be
+    captured in HIP graphs.  ``autotune=True`` times every grouped tile shape for the two poke
+    launches here and keeps the fastest (``self.tiles``)."""
+
+    def __init__(self, items, poke_height, device=None, autotune=False):
+        self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
+        lib = _lib.load()
+        items = list(items)
+        if not items:
+            raise ValueError('a poke set needs at least one item')
+        citems = (_lib.PokeItem * len(items))()
+        self.fields, self.base, self.pokes, self._keep = [], [], [], []
+        for k, (prop, wavelength, field, infl) in enumerate(items):
+            if not isinstance(infl, GaussianInfluenceFunctions):
+                raise ValueError('item %d: poke sets take Gaussian influence functions' % k)
+            d = prop.instance(wavelength)
+            mft = d.fourier_transform
+            t = field.as_subclass(torch.Tensor) if isinstance(field, torch.Tensor) else torch.as_tensor(np.asarray(field))
+            t = t.to(self.device)
+            if not t.is_complex():
+                t = t.to(torch.complex128)
+            t = t.contiguous()
+            if t.numel() != mft.input_grid.size:
+                raise ValueError('item %d: field of %d samples on a %d-sample pupil grid' % (k, t.numel(), mft.input_grid.size))
+            w = np.asarray(mft.input_grid.weights).ravel()
+            if not np.all(w == w[0]):
+                raise ValueError('item %d: poke sets need scalar pupil-grid weights' % k)
+            plan = mft.plan_for(t.dtype, self.device)
+            alpha = complex(float(w[0])) * complex(d.norm_factor)
+            fp64 = t.dtype == torch.complex128
+            odt = torch.float64 if fp64 else torch.float32
+            n_focal = mft.output_grid.size
+            base = torch.empty(n_focal, dtype=odt, device=self.device)
+            pokes = torch.empty((len(infl), n_focal), dtype=odt, device=self.device)
+            k_wave = 2 * np.pi / wavelength
+            citems[k] = _lib.PokeItem(plan.handle.value, t.data_ptr(), alpha.real, alpha.imag, infl._desc,
+                                      2 * k_wave * float(poke_height), base.data_ptr(), pokes.data_ptr())
+            self.fields.append(t)
+            self.base.append(base)
+            self.pokes.append(pokes)
+            self._keep.append((plan, mft, prop))
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            st = lib.hcipy_poke_set_create(ctypes.byref(h), ctypes.cast(citems, ctypes.c_void_p), len(items),
+                                           _lib.stream_ptr(self.device))
+        _lib.check(st, 'hcipy_poke_set_create')
+        self.handle = h
+        n = ctypes.c_size_t()
+        _lib.check(lib.hcipy_poke_set_bytes(h, ctypes.byref(n)), 'hcipy_poke_set_bytes')
+        self.nbytes = n.value
+        npk, fw, ff = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _lib.check(lib.hcipy_poke_set_info(h, ctypes.byref(npk), ctypes.byref(fw), ctypes.byref(ff)), 'hcipy_poke_set_info')
+        self.num_pokes = npk.value
+        self.flops_windowed = fw.value  # real flops of one execution as run
+        self.flops_full = ff.value      # the same results by full transforms of every poked field
+        self.tiles = (_lib.TILES_AUTO, _lib.TILES_AUTO)
+        if autotune:
+            chosen = (ctypes.c_int * 2)()
+            with torch.cuda.device(self.device):
+                _lib.check(lib.hcipy_poke_set_autotune(h, 2, chosen, _lib.stream_ptr(self.device)), 'hcipy_poke_set_autotune')
+            self.tiles = (chosen[0], chosen[1])
+
+    def execute(self):
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.load().hcipy_poke_set_execute(self.handle, _lib.stream_ptr(self.device)),
+                       'hcipy_poke_set_execute')
+        return self.base, self.pokes
+
+    def close(self):
+        if getattr(self, 'handle', None) is not None and _lib._lib is not None:
+            _lib.load().hcipy_poke_set_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -9,6 +9,13 @@
 * ``lyot_1024``       : 1024^2 circular pupil, power-law phase (PSD exponent -2, 0.01 rad RMS,
   seed 3); focal-plane mask 1 - circ(6 lambda/D) on a 256^2 grid (q = 32, 4 lambda/D);
   Lyot stop circ(0.95 D); final focal grid q = 4, 64 lambda/D (512^2).
+* ``poke_jacobian``    : broadband DM-poke batch.  32 wavelengths lambda_0 (0.95 ... 1.05),
+  lambda_0 = 500 nm, D = 1 m, f = 1 m; wavelength i uses the pupil size (256, 512, 1024)[i % 3]
+  (focal grid q = 4 at lambda_0, n/16 lambda_0/D radius: 128^2, 256^2, 512^2, fixed in physical
+  units, so every wavelength has its own Fourier matrices); circular pupil with the smoothed
+  0.05 rad phase (seed 4); 32 x 32 Gaussian actuators (crosstalk 0.15, the reference's sigma =
+  pitch / sqrt(-2 ln 0.15)); every actuator poked by a surface height of 1e-3 rad of phase at
+  lambda_0.
 """
 import numpy as np
 
@@ -90,3 +97,27 @@ def lyot_1024():
     stop = _circ(pupil, 0.95)
     focal = make_focal_grid(4, 64)
     return pupil, fpm_grid, focal, ap * np.exp(1j * ph), fpm, stop, ap
+
+
+POKE_LAMBDA0 = 500e-9
+POKE_PUPILS = (256, 512, 1024)
+
+
+def poke_jacobian(n_wavelengths=32, n_act=32, pupils=POKE_PUPILS, seed=4):
+    """Geometry and base fields of the broadband DM-poke batch: a list of
+    (pupil_grid, focal_grid, wavelength, field) with the focal grid in meters (f = 1 m), and the
+    poke surface height h (2 k_0 h = 1e-3 rad) and the actuator count across."""
+    lam0 = POKE_LAMBDA0
+    wls = lam0 * np.linspace(0.95, 1.05, n_wavelengths)
+    fields = {}
+    for n in pupils:
+        pg = make_pupil_grid(n, 1.0)
+        ap = _circ(pg, 1.0)
+        fields[n] = (pg, make_focal_grid(4, n // 16, spatial_resolution=lam0),
+                     ap * np.exp(1j * smoothed_phase(pg.shape, 4.0 * n / 1024, 0.05, ap, seed)))
+    items = []
+    for i, wl in enumerate(wls):
+        pg, fg, E = fields[pupils[i % len(pupils)]]
+        items.append((pg, fg, wl, E))
+    h = 1e-3 / (2 * 2 * np.pi / lam0)
+    return items, h, n_act

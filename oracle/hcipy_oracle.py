@@ -998,17 +998,20 @@ def actuator_positions(n_act, pitch):
 
 
 def gaussian_influence_functions(pupil_grid, n_act, pitch, crosstalk=0.15, cutoff=3, x_tilt=0, y_tilt=0, z_tilt=0,
-                                 oversampling=1):
+                                 oversampling=1, only=None):
     """make_gaussian_influence_functions (deformable_mirror.py:43-93) with make_gaussian_pokes
     (mode_basis/gaussian_pokes.py:5-54) and evaluate_supersampled (field/util.py:606-698):
     each poke is evaluated on the pupil grid scaled by 1/cos([y_tilt, x_tilt]) and rotated by
     -z_tilt (R = [[cos, -sin], [sin, cos]] on the points, cartesian_grid.py:12-32), divided by
     cos(x_tilt) cos(y_tilt), and averaged over the oversampling dithers ((k + 0.5)/ovs - 0.5)
-    times the grid step.  Returns a dense (n_act^2, Npix) array."""
+    times the grid step.  Returns a dense (n_act^2, Npix) array, or the rows of the actuator
+    indices ``only``."""
     pos = actuator_positions(n_act, pitch)
     sigma = pitch / np.sqrt(-2 * np.log(crosstalk))
     cut = pitch / sigma * cutoff
     px, py = pos.coords
+    if only is not None:
+        px, py = px[np.asarray(only)], py[np.asarray(only)]
     x0 = pupil_grid.x
     y0 = pupil_grid.y
     dx, dy = pupil_grid.delta
