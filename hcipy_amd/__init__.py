@@ -29,6 +29,7 @@ from .deformable_mirror import (DeformableMirror, GaussianInfluenceFunctions, Po
                                 make_gaussian_influence_functions)
 from .metrics import get_mean_intensity_in_roi, get_mean_raw_contrast, get_strehl_from_focal, get_strehl_from_pupil
 from .wavefront_sensing import ModulatedPyramidWavefrontSensorOptics, PyramidWavefrontSensorOptics
+from .matrix_inversion import inverse_tikhonov
 from . import pointwise
 
 __version__ = '0.1.0'

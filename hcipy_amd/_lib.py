@@ -77,7 +77,7 @@ class PokeItem(ctypes.Structure):
     """hcipy_poke_item of include/hcipy_amd.h."""
     _fields_ = [('plan', ctypes.c_void_p), ('field', ctypes.c_void_p), ('alpha_re', ctypes.c_double),
                 ('alpha_im', ctypes.c_double), ('dm', DmGaussianDesc), ('phase_scale', ctypes.c_double),
-                ('base_out', ctypes.c_void_p), ('poke_out', ctypes.c_void_p)]
+                ('base_out', ctypes.c_void_p), ('poke_out', ctypes.c_void_p), ('output', ctypes.c_int)]
 
 
 _c_int64 = ctypes.c_int64
@@ -136,6 +136,9 @@ SIGNATURES = {
     'hcipy_poke_set_info': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double)]),
     'hcipy_poke_set_destroy': (ctypes.c_int, [_vp]),
+    'hcipy_inverse_tikhonov_workspace': (ctypes.c_int, [_c_int64, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_inverse_tikhonov': (ctypes.c_int, [_vp, _c_int64, _c_int64, ctypes.c_double, _vp,
+                                              ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp]),
     'hcipy_filter_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_int, _vp]),
     'hcipy_filter_plan_destroy': (ctypes.c_int, [_vp]),

@@ -946,6 +946,8 @@ int hcipy_poke_set_create(hcipy_poke_set* set, const hcipy_poke_item* items, int
   for (int k = 0; k < n_items; ++k) {
     const hcipy_poke_item& it = items[k];
     HCIPY_REQUIRE(it.plan != nullptr && it.field != nullptr && it.poke_out != nullptr, "item %d: NULL pointer", k);
+    HCIPY_REQUIRE(it.output == HCIPY_POKE_INTENSITY || it.output == HCIPY_POKE_FIELD_CHANGE, "item %d: output %d", k,
+                  it.output);
     const Plan& pl = it.plan->p;
     HCIPY_REQUIRE(pl.prec == prec, "item %d: all items of a poke set share one precision", k);
     HCIPY_REQUIRE(pl.ndim == 2 && pl.device == cur, "item %d: needs a 2-D plan on the current device", k);
@@ -1101,11 +1103,13 @@ int hcipy_poke_set_create(hcipy_poke_set* set, const hcipy_poke_item* items, int
       a2.M = (int)(2 * g.nv);
       a2.N = (int)g.nu;
       a2.K = (int)(2 * g.wy);
-      a2.out = static_cast<char*>(it.poke_out) + (size_t)a * g.nu * g.nv * (prec == kF64 ? 8 : 4);
+      const bool field_change = it.output == HCIPY_POKE_FIELD_CHANGE;
+      const size_t out_es = field_change ? (prec == kF64 ? 16 : 8) : (prec == kF64 ? 8 : 4);
+      a2.out = static_cast<char*>(it.poke_out) + (size_t)a * g.nu * g.nv * out_es;
       a2.ldo = g.nu;
       a2.out_zstride = g.nu * g.nv;
-      a2.out_mode = out_mode;
-      a2.base = bi.base;
+      a2.out_mode = field_change ? (prec == kF64 ? kOutC128 : kOutC64) : out_mode;
+      a2.base = field_change ? nullptr : bi.base;
       a2.z_scale = t_a;
       a2.alpha_re = it.alpha_re;
       a2.alpha_im = it.alpha_im;

@@ -385,18 +385,24 @@ class PokeSet:
 
     ``execute()`` recomputes, from the current contents of the field tensors (``self.fields``,
     which may be overwritten in place), ``self.base[k]`` = |E_focal|^2 of item k and
-    ``self.pokes[k][a]`` = |E_focal|^2 with actuator a poked, and returns both lists.  Memory
+    ``self.pokes[k][a]`` = |E_focal|^2 with actuator a poked, and returns both lists.  With
+    ``output='field_change'`` ``self.pokes[k][a]`` is instead the change of the complex focal
+    field caused by the poke (the Jacobian column of actuator a times the poke height).  Memory
     (workspaces, launch tables) is allocated once here; executions allocate nothing and can be
     captured in HIP graphs.  ``autotune=True`` times every grouped tile shape for the two poke
     launches here and keeps the fastest (``self.tiles``)."""
 
-    def __init__(self, items, poke_height, device=None, autotune=False):
+    def __init__(self, items, poke_height, device=None, autotune=False, output='intensity'):
         self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
         lib = _lib.load()
         items = list(items)
         if not items:
             raise ValueError('a poke set needs at least one item')
         citems = (_lib.PokeItem * len(items))()
+        if output not in ('intensity', 'field_change'):
+            raise ValueError("output must be 'intensity' or 'field_change'")
+        kind = 0 if output == 'intensity' else 1
+        self.output = output
         self.fields, self.base, self.pokes, self._keep = [], [], [], []
         for k, (prop, wavelength, field, infl) in enumerate(items):
             if not isinstance(infl, GaussianInfluenceFunctions):
@@ -419,10 +425,11 @@ class PokeSet:
             odt = torch.float64 if fp64 else torch.float32
             n_focal = mft.output_grid.size
             base = torch.empty(n_focal, dtype=odt, device=self.device)
-            pokes = torch.empty((len(infl), n_focal), dtype=odt, device=self.device)
+            pdt = odt if output == 'intensity' else (torch.complex128 if fp64 else torch.complex64)
+            pokes = torch.empty((len(infl), n_focal), dtype=pdt, device=self.device)
             k_wave = 2 * np.pi / wavelength
             citems[k] = _lib.PokeItem(plan.handle.value, t.data_ptr(), alpha.real, alpha.imag, infl._desc,
-                                      2 * k_wave * float(poke_height), base.data_ptr(), pokes.data_ptr())
+                                      2 * k_wave * float(poke_height), base.data_ptr(), pokes.data_ptr(), kind)
             self.fields.append(t)
             self.base.append(base)
             self.pokes.append(pokes)

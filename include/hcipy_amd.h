@@ -358,8 +358,14 @@ typedef struct {
   hcipy_dm_gaussian_desc dm;   /* influence functions on the plan's input grid               */
   double phase_scale;          /* phase of the poke on a unit influence function: 2 k h      */
   void* base_out;              /* base intensity (nv * nu) or NULL                           */
-  void* poke_out;              /* n_act^2 intensities [actuator][nv * nu]                    */
+  void* poke_out;              /* n_act^2 outputs [actuator][nv * nu]                        */
+  int output;                  /* HCIPY_POKE_INTENSITY or HCIPY_POKE_FIELD_CHANGE            */
 } hcipy_poke_item;
+/* Outputs of a poke: the poked focal intensity (float32, float64 for FP64), or the change of the
+ * focal field alpha P_w(dE_a) (complex64, complex128 for FP64) -- the columns of the DM Jacobian
+ * times the poke height, as an electric-field-conjugation loop uses them. */
+#define HCIPY_POKE_INTENSITY 0
+#define HCIPY_POKE_FIELD_CHANGE 1
 int hcipy_poke_set_create(hcipy_poke_set* set, const hcipy_poke_item* items, int n_items, void* stream);
 int hcipy_poke_set_bytes(hcipy_poke_set set, size_t* bytes);
 /* Pokes of the set and the real flops (8 per complex multiply-add) of one execution as run (base
@@ -372,6 +378,19 @@ int hcipy_poke_set_execute(hcipy_poke_set set, void* stream);
  * refuses a capturing stream.  No counterpart in the reference. */
 int hcipy_poke_set_autotune(hcipy_poke_set set, int repeats, int* chosen, void* stream);
 int hcipy_poke_set_destroy(hcipy_poke_set set);
+
+/* Tikhonov-regularised inverse of a real m x n matrix M (row-major, device, FP64):
+ *   out (n x m, row-major) = V diag(s / (s^2 + (rcond s_max)^2)) U^T = (M^T M + lam^2 I)^-1 M^T,
+ * lam = rcond * s_max, s_max the largest singular value (written to *s_max, host, may be NULL).
+ * Replaces inverse_tikhonov (hcipy/util/matrix_inversion.py:71-100), which takes an SVD: here
+ * M^T M runs on the FP64 MFMA stage kernel, s_max^2 comes from power iteration on it, and the
+ * regularised normal equations are solved by a blocked device Cholesky factorisation.  Relative
+ * accuracy about 1e-16 / rcond^2 (the conditioning of the normal equations); rcond must keep
+ * M^T M + lam^2 I positive definite in FP64.  Synchronises with the host (convergence checks).
+ * The workspace holds hcipy_inverse_tikhonov_workspace(m, n) bytes. */
+int hcipy_inverse_tikhonov_workspace(int64_t m, int64_t n, size_t* bytes);
+int hcipy_inverse_tikhonov(const double* M, int64_t m, int64_t n, double rcond, double* out, double* s_max,
+                           void* workspace, size_t workspace_bytes, void* stream);
 
 /* Deformable mirror whose influence function is a sampled actuator shape: the Xinetics DM5V2
  * shape of make_xinetics_influence_functions (hcipy/optics/deformable_mirror.py:95-138).

@@ -971,6 +971,14 @@ def modulated_pyramid_wfs_forward(E, geom, modulation, num_steps=12, fast=False,
     return out
 
 
+def inverse_tikhonov(M, rcond=1e-15):
+    """inverse_tikhonov (hcipy/util/matrix_inversion.py:71-100): from the SVD M = U S V^T,
+    V diag(s / (s^2 + (rcond max s)^2)) U^T."""
+    U, S, Vt = np.linalg.svd(np.asarray(M, dtype=float), full_matrices=False)
+    S_inv = S / (S ** 2 + (rcond * S.max()) ** 2)
+    return (Vt.T * S_inv).dot(U.T)
+
+
 def perfect_coronagraph(E, aperture, grid, order=2, coeffs=None):
     """PerfectCoronagraph.forward (hcipy/coronagraphy/perfect_coronagraph.py:38-88): E minus its
     projection onto the QR-orthogonalized modes aperture * x^j y^(i-j), i < order/2 (signs from

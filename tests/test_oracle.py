@@ -859,3 +859,26 @@ def test_perfect_coronagraph_coefficients():
     E = ap.astype(complex)
     assert np.abs(o.perfect_coronagraph(E, ap, g, coeffs=[0, 1, 1]) - E).max() < 1e-12
     assert np.abs(o.perfect_coronagraph(E, ap, g, coeffs=[1, 1, 1])).max() < 1e-12
+
+
+# --- inverse_tikhonov (hcipy/util/matrix_inversion.py:71-100) --------------------------------
+
+def test_inverse_tikhonov_pins():
+    """The restatement against what the mathematics fixes: for rcond -> 0 it is numpy's
+    pseudo-inverse; for any rcond it equals the normal-equations form (M^T M + lam^2 I)^-1 M^T with
+    lam = rcond * s_max (computed here by solve, not by SVD); for a diagonal M it is
+    s / (s^2 + lam^2) elementwise; a transposed operand or a dropped square would fail these."""
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((40, 12)) @ np.diag(np.logspace(0, -3, 12)) @ rng.standard_normal((12, 12))
+    assert np.allclose(o.inverse_tikhonov(M, 1e-14), np.linalg.pinv(M), rtol=1e-6, atol=1e-8)
+    for rcond in (1e-3, 1e-1, 0.7):
+        lam = rcond * np.linalg.svd(M, compute_uv=False).max()
+        ref = np.linalg.solve(M.T @ M + lam ** 2 * np.eye(12), M.T)
+        got = o.inverse_tikhonov(M, rcond)
+        assert got.shape == (12, 40)
+        assert np.abs(got - ref).max() < 1e-9 * np.abs(ref).max()
+    d = np.array([3.0, 1.0, 0.5])
+    D = np.zeros((5, 3))
+    D[:3, :3] = np.diag(d)
+    lam = 0.2 * 3.0
+    assert np.allclose(o.inverse_tikhonov(D, 0.2)[:, :3], np.diag(d / (d ** 2 + lam ** 2)), rtol=1e-14, atol=0)
