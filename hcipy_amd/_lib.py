@@ -186,6 +186,15 @@ SIGNATURES = {
     'hcipy_zoom_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_zoom_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double, _vp,
                                           ctypes.c_size_t, _vp]),
+    'hcipy_fft_plan_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                             ctypes.c_int, ctypes.c_int, _vp]),
+    'hcipy_fft_plan_destroy': (ctypes.c_int, [_vp]),
+    'hcipy_fft_workspace_size': (ctypes.c_int, [_vp, _c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    'hcipy_fft_execute': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double, _vp,
+                                         ctypes.c_size_t, _vp]),
 }
 
 DTYPE_C64 = 0

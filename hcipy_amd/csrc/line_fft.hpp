@@ -259,7 +259,37 @@ struct HrShape {
   static_assert(P * E == N, "N must be P * E");
 };
 
-template <typename T, int N, int P>
+// One LDS round trip of a line: every thread stores its E points at windex(r), and after a
+// barrier loads slot r from hr_pad(tl + r P).  SPLIT moves the real parts and then the imaginary
+// parts through a line of T (half the LDS, two more barriers); the line then holds
+// hr_lds_points(N) values of T instead of complex values.
+template <bool SPLIT, typename T, int E, int P, typename WI>
+__device__ inline void hr_exchange(typename Cplx<T>::V (&v)[E], typename Cplx<T>::V* line, int tl, WI windex) {
+  if constexpr (!SPLIT) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) line[windex(r)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = line[hr_pad(tl + r * P)];
+  } else {
+    T* l = reinterpret_cast<T*>(line);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) l[windex(r)] = v[r].x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r].x = l[hr_pad(tl + r * P)];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) l[windex(r)] = v[r].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r].y = l[hr_pad(tl + r * P)];
+  }
+}
+
+template <typename T, int N, int P, bool SPLIT = false>
 __device__ void hr_fft(typename Cplx<T>::V (&v)[N / P], typename Cplx<T>::V* line, int tl,
                        const typename Cplx<T>::V* __restrict__ tw, int sign) {
   using V = typename Cplx<T>::V;
@@ -271,32 +301,23 @@ __device__ void hr_fft(typename Cplx<T>::V (&v)[N / P], typename Cplx<T>::V* lin
   // stage 1 (span 1): butterfly tl takes points tl + r P, no twiddles
   dft_r<V, T, E>(v, s);
   if constexpr (A == 1 && RL == 1) return;  // N == E: P == 1 and the outputs are in order
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < E; ++r) line[hr_pad(tl * E + r)] = v[r];
-  __syncthreads();
+  hr_exchange<SPLIT, T, E, P>(v, line, tl, [&](int r) { return hr_pad(tl * E + r); });
   int Ns = E;
 #pragma unroll
   for (int st = 1; st < A; ++st) {
-#pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = line[hr_pad(tl + r * P)];
     const int k = tl & (Ns - 1);
     const int step = N / (Ns * E);
 #pragma unroll
     for (int r = 1; r < E; ++r) v[r] = cmul_tw(v[r], tw[(r * k * step) & (N - 1)], s);
     dft_r<V, T, E>(v, s);
     if (st == A - 1 && RL == 1) return;  // last stage: k == tl, outputs tl + r P are in order
-    __syncthreads();
     const int base = (tl - k) * E + k;
-#pragma unroll
-    for (int r = 0; r < E; ++r) line[hr_pad(base + r * Ns)] = v[r];
-    __syncthreads();
+    hr_exchange<SPLIT, T, E, P>(v, line, tl, [&](int r) { return hr_pad(base + r * Ns); });
     Ns *= E;
   }
   if constexpr (RL > 1) {
-    // span N / RL: butterfly j = tl + b P (b < E / RL) takes slots b + r (E / RL), twiddles w^(r j)
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = line[hr_pad(tl + q * P)];
+    // span N / RL: butterfly j = tl + b P (b < E / RL) takes slots b + r (E / RL), twiddles w^(r j);
+    // the last exchange left slot q = point tl + q P in v
     constexpr int NB = E / RL;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {

@@ -522,10 +522,14 @@ class MatrixFourierTransform(FourierTransform):
 
 
 class _ZoomPlan:
-    """Owns one C-ABI zoom-FFT plan (per-axis chirps and filter spectra in one dtype)."""
+    """Owns one C-ABI zoom-FFT plan (per-axis chirps and filter spectra in one dtype), or with
+    ``direct`` a direct power-of-two FFT plan (per-axis twiddles and phase ramps), which takes
+    the same execute arguments."""
 
-    def __init__(self, input_grid, output_grid, cdt, precision, device, fft_n=None):
+    def __init__(self, input_grid, output_grid, cdt, precision, device, fft_n=None, direct=False):
         lib = _lib.load()
+        self._api = 'fft' if direct else 'zoom'
+
         nd = input_grid.ndim
         ci = ctypes.c_int * nd
         cd = ctypes.c_double * nd
@@ -534,7 +538,12 @@ class _ZoomPlan:
                  cd(*[float(v) for v in output_grid.delta]), cd(*[float(v) for v in output_grid.zero]))
         with torch.cuda.device(device):
             h = ctypes.c_void_p()
-            if fft_n is None:
+            if direct:
+                st = lib.hcipy_fft_plan_create(ctypes.byref(h), nd, *grids, ci(*[int(v) for v in fft_n]),
+                                               _lib._DTYPE_CODES[cdt], _lib.PRECISIONS[precision],
+                                               _lib.stream_ptr(device))
+                _lib.check(st, 'hcipy_fft_plan_create')
+            elif fft_n is None:
                 st = lib.hcipy_zoom_plan_create(ctypes.byref(h), nd, *grids, _lib._DTYPE_CODES[cdt],
                                                 _lib.PRECISIONS[precision], _lib.stream_ptr(device))
                 _lib.check(st, 'hcipy_zoom_plan_create')
@@ -547,19 +556,21 @@ class _ZoomPlan:
 
     def workspace_bytes(self, batch):
         n = ctypes.c_size_t()
-        _lib.check(_lib.load().hcipy_zoom_workspace_size(self.handle, batch, ctypes.byref(n)), 'hcipy_zoom_workspace_size')
+        name = 'hcipy_%s_workspace_size' % self._api
+        _lib.check(getattr(_lib.load(), name)(self.handle, batch, ctypes.byref(n)), name)
         return n.value
 
     def execute(self, direction, x, out, batch, alpha, workspace):
-        st = _lib.load().hcipy_zoom_execute(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch,
-                                            float(alpha.real), float(alpha.imag),
-                                            _lib.dptr(workspace) if workspace is not None else None,
-                                            0 if workspace is None else workspace.numel(), _lib.stream_ptr(x.device))
-        _lib.check(st, 'hcipy_zoom_execute')
+        name = 'hcipy_%s_execute' % self._api
+        st = getattr(_lib.load(), name)(self.handle, direction, _lib.dptr(x), _lib.dptr(out), batch,
+                                        float(alpha.real), float(alpha.imag),
+                                        _lib.dptr(workspace) if workspace is not None else None,
+                                        0 if workspace is None else workspace.numel(), _lib.stream_ptr(x.device))
+        _lib.check(st, name)
 
     def close(self):
         if getattr(self, 'handle', None) is not None and _lib._lib is not None:
-            _lib.load().hcipy_zoom_plan_destroy(self.handle)
+            getattr(_lib.load(), 'hcipy_%s_plan_destroy' % self._api)(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -609,6 +620,7 @@ class ZoomFastFourierTransform(FourierTransform):
         # FFT length per axis when the grids are an FFT pair (set by FastFourierTransform): the
         # chirps then use the exact 2 pi / N
         self._fft_n = None
+        self._direct = False
         self._plans = {}
         self._workspace = {}
 
@@ -622,7 +634,8 @@ class ZoomFastFourierTransform(FourierTransform):
         key = (cdt, prec, device.index)
         plan = self._plans.get(key)
         if plan is None:
-            plan = _ZoomPlan(self.input_grid, self.output_grid, cdt, prec, device, fft_n=self._fft_n)
+            plan = _ZoomPlan(self.input_grid, self.output_grid, cdt, prec, device, fft_n=self._fft_n,
+                             direct=self._direct)
             self._plans[key] = plan
         return plan
 
@@ -710,18 +723,42 @@ __all__ += ['ZoomFastFourierTransform']
 # FastFourierTransform and the planner
 
 
+class _DirectFft(ZoomFastFourierTransform):
+    """FastFourierTransform's power-of-two path: one line FFT of the internal length per row and
+    per column of the unpadded field, with the FFT grid's phase ramps on load and store
+    (``hcipy_fft_execute``).  Same sums, weights and autograd as the zoom FFT; no zoom-FFT size
+    limit (each internal length is a power of two in [4, 8192])."""
+
+    def __init__(self, input_grid, output_grid, fft_n, precision=None):
+        if precision is not None and precision not in ('fp64', 'fp32'):
+            raise ValueError("precision must be 'fp64' or 'fp32' (complex64 fields)")
+        self.input_grid = input_grid
+        self.output_grid = output_grid
+        self.ndim = input_grid.ndim
+        self.precision = precision
+        self._fft_n = [int(n) for n in fft_n]
+        self._direct = True
+        self._plans = {}
+        self._workspace = {}
+
+    @staticmethod
+    def supports(fft_n):
+        return all(4 <= int(n) <= 8192 and int(n) & (int(n) - 1) == 0 for n in fft_n)
+
+
 class FastFourierTransform(FourierTransform):
     """Zero-padded / cropped / shifted Fourier transform onto an FFT grid
     (hcipy/fourier/fast_fourier_transform.py:188-482).
 
     The reference's FFT output (with its input/output phase-ramp corrections) equals the
     Fourier sum onto ``make_fft_grid(input_grid, q, fov, shift)``.  This class evaluates that sum
-    with the Bluestein zoom FFT (O(N^2 log N); measured faster than the MFT at every size from
-    128^2 up, 13.6x at 2048^2 in FP64) whenever each axis fits its 8192-point line FFT, and with
-    the matrix Fourier transform otherwise, or when ``precision`` 'f16x2' / 'bf16x3' asks for a
-    split-precision MFT.  ``precision`` None / 'fp32' selects the complex64 arithmetic of the zoom FFT (FP64
-    internal / FP32) or of the MFT.  ``emulate_fftshifts`` is accepted for interface
-    compatibility (it only changes the reference's rounding).
+    with a direct FFT of the internal length (``method`` 'fft') when every internal length is a
+    power of two in [4, 8192]; otherwise with the Bluestein zoom FFT ('zft'; measured faster than
+    the MFT at every size from 128^2 up) when each axis fits its 8192-point line FFT, and with
+    the matrix Fourier transform ('mft') beyond that, or when ``precision`` 'f16x2' / 'bf16x3'
+    asks for a split-precision MFT.  ``precision`` None / 'fp32' selects the complex64 arithmetic
+    of the FFTs (FP64 internal / FP32) or of the MFT.  ``emulate_fftshifts`` is accepted for
+    interface compatibility (it only changes the reference's rounding).
     """
 
     def __init__(self, input_grid, q=1, fov=1, shift=0, emulate_fftshifts=None, precision=None):
@@ -745,7 +782,10 @@ class FastFourierTransform(FourierTransform):
         self.shape_out = self.output_grid.shape
         self.internal_shape = self.internal_grid.shape
         self._impl = None
-        if precision not in ('f16x2', 'bf16x3'):
+        fft_n = [int(n) for n in self.internal_grid.dims]
+        if precision not in ('f16x2', 'bf16x3') and input_grid.ndim in (1, 2) and _DirectFft.supports(fft_n):
+            self._impl = _DirectFft(input_grid, self.output_grid, fft_n, precision=precision)
+        elif precision not in ('f16x2', 'bf16x3'):
             try:
                 self._impl = ZoomFastFourierTransform(input_grid, self.output_grid, precision=precision)
                 self._impl._fft_n = [int(n) for n in self.internal_grid.dims]
@@ -756,7 +796,9 @@ class FastFourierTransform(FourierTransform):
 
     @property
     def method(self):
-        """'zft' or 'mft': the transform that evaluates this FFT."""
+        """'fft', 'zft' or 'mft': the transform that evaluates this FFT."""
+        if isinstance(self._impl, _DirectFft):
+            return 'fft'
         return 'zft' if isinstance(self._impl, ZoomFastFourierTransform) else 'mft'
 
     def forward(self, field):

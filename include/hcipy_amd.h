@@ -529,6 +529,32 @@ int hcipy_zoom_workspace_size(hcipy_zoom_plan plan, int64_t batch, size_t* bytes
 int hcipy_zoom_execute(hcipy_zoom_plan plan, int direction, const void* in, void* out, int64_t batch, double alpha_re,
                        double alpha_im, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Direct FFT between a grid and its FFT grid for a power-of-two internal length (per axis
+ * fft_n[d] in [4, 8192]).  Replaces FastFourierTransform.forward / .backward
+ * (hcipy/fourier/fast_fourier_transform.py:280-406): the reference zero-pads the field into an
+ * fft_n buffer, fftshifts, runs numpy's FFT and multiplies shift ramps; this plan runs one
+ * length-fft_n line FFT per row and per column of the unpadded field, with the input ramp
+ * exp(-i u0 dx n) applied on load and the output ramps exp(-i u0 x0) exp(-i du x0 k) on store,
+ * keeping only the fft_n-grid bins the output grid holds.  Arguments as for
+ * hcipy_zoom_plan_create_fft (same grids, delta_in * delta_out = 2 pi / fft_n[d] to 1e-9); the
+ * output grid's first coordinate may sit anywhere on the FFT lattice.  precision: HCIPY_PREC_FP64
+ * (FP64 line FFTs and intermediate; complex64 fields load and store FP32) or, for complex64
+ * fields, HCIPY_PREC_FP32.  Set-up runs on `stream` and is refused under stream capture. */
+typedef struct hcipy_fft_plan_s* hcipy_fft_plan;
+int hcipy_fft_plan_create(hcipy_fft_plan* plan, int ndim, const int* n_in, const double* delta_in,
+                          const double* zero_in, const int* n_out, const double* delta_out, const double* zero_out,
+                          const int* fft_n, int dtype, int precision, void* stream);
+int hcipy_fft_plan_destroy(hcipy_fft_plan plan);
+/* Bytes of workspace (the column-blocked intermediate of the row pass) for `batch` fields in
+ * either direction; 0 in 1-D. */
+int hcipy_fft_workspace_size(hcipy_fft_plan plan, int64_t batch, size_t* bytes);
+/* direction 0: forward, alpha * sum_n f[n] exp(-i x_n . u_k) on the output grid; 1: backward,
+ * alpha * sum_k F[k] exp(+i u_k . x_n) on the input grid.  The reference's weights are the
+ * caller's alpha.  `batch` fields of the plan's dtype, contiguous; in and out must not alias. */
+int hcipy_fft_execute(hcipy_fft_plan plan, int direction, const void* in, void* out, int64_t batch, double alpha_re,
+                      double alpha_im, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

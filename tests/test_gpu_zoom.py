@@ -211,11 +211,12 @@ def test_zoom_1d_matches_scipy_czt():
         assert np.abs(out - ref).max() < (1e-12 + 1e-15 * (n + m) ** 2) * peak, (n, m)
 
 
-@pytest.mark.parametrize('n,q,fov,shift', [(1024, 2, 1, 0), (512, 2, 0.5, 0.1), (96, 3, 0.7, 0.0), (2048, 1, 1, 0)])
+@pytest.mark.parametrize('n,q,fov,shift', [(1000, 2, 1, 0), (500, 2, 0.5, 0.1), (96, 3, 0.7, 0.0), (2000, 1, 1, 0)])
 def test_fft_class_on_zoom_backend_matches_oracle(n, q, fov, shift):
     """FastFourierTransform (fast_fourier_transform.py:188-406) on the zoom-FFT backend against the
     oracle's numpy FFT with the reference's pad, shift and crop: complex128 at 1e-12 of the
-    peak, complex64 (FP64 internal) at 1e-6; backward against the oracle too.  The split-FP16
+    peak plus the oracle's own phase rounding, complex64 (FP64 internal) at 1e-6; backward
+    against the oracle too.  The split-FP16
     MFT backend gives the same transform within its own bound."""
     g = ha.make_pupil_grid(n, 1.0)
     ft = ha.FastFourierTransform(g, q, fov, shift)
@@ -223,13 +224,15 @@ def test_fft_class_on_zoom_backend_matches_oracle(n, q, fov, shift):
     ref_ft = o.FFT(og(g), q, fov, shift)
     f = _crand(g.size, 7)
     ref = ref_ft.forward(f)
-    for dtype, tol in ((np.complex128, 1e-12), (np.complex64, 1e-6)):
+    # the oracle's fftshift ramp rounds phases of up to pi N / 2 per axis at 1e-16
+    slack = 2e-16 * sum(np.pi * int(d) / 2 for d in ft.internal_grid.dims)
+    for dtype, tol in ((np.complex128, 1e-12 + slack), (np.complex64, 1e-6)):
         out = host(ft.forward(ha.Field(dev(f.astype(dtype)), g)))
         e = peak_rel_field(out, ref)
         say('FFT on zoom n=%d q=%g fov=%g %s: %.2e' % (n, q, fov, dtype.__name__, e))
         assert e < tol
     back = host(ft.backward(ha.Field(dev(ref), ft.output_grid)))
-    assert peak_rel_field(back, ref_ft.backward(ref)) < 1e-12
+    assert peak_rel_field(back, ref_ft.backward(ref)) < 1e-12 + slack
     if n <= 512:
         m = host(ha.FastFourierTransform(g, q, fov, shift, precision='f16x2').forward(ha.Field(dev(f.astype(np.complex64)), g)))
         assert peak_rel_field(m, ref) < 2e-6

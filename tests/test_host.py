@@ -68,11 +68,17 @@ def test_planner_reference_choices():
 
 
 def test_fft_backend_choice():
-    """FastFourierTransform evaluates its sum with the zoom FFT where each axis fits the 8192-point
-    line FFT, with the MFT beyond that or when the split-FP16 MFT is asked for."""
+    """FastFourierTransform evaluates its sum with the direct FFT where every internal length is
+    a power of two in [4, 8192], with the zoom FFT where each axis fits the 8192-point line FFT,
+    with the MFT beyond that or when a split-precision MFT is asked for."""
     import hcipy_amd as ha
     g = ha.make_pupil_grid(256)
-    assert ha.FastFourierTransform(g, q=2).method == 'zft'
-    assert ha.FastFourierTransform(g, q=2, fov=0.5, shift=0.1, precision='fp32').method == 'zft'
+    assert ha.FastFourierTransform(g, q=2).method == 'fft'
+    assert ha.FastFourierTransform(g, q=2, fov=0.5, shift=0.1, precision='fp32').method == 'fft'
+    assert ha.FastFourierTransform(ha.make_uniform_grid([4096], [1.0]), q=2).method == 'fft'
+    assert ha.FastFourierTransform(ha.make_pupil_grid(200), q=2).method == 'zft'
+    assert ha.FastFourierTransform(ha.make_pupil_grid(200), q=2, precision='fp32').method == 'zft'
     assert ha.FastFourierTransform(g, precision='f16x2').method == 'mft'
+    assert ha.FastFourierTransform(g, precision='bf16x3').method == 'mft'
     assert ha.FastFourierTransform(ha.make_uniform_grid([5000], [1.0])).method == 'mft'
+    assert ha.FastFourierTransform(ha.make_uniform_grid([4096], [1.0]), q=4).method == 'mft'
