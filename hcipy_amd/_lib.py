@@ -41,8 +41,13 @@ TILES_D64X64_W4 = 12
 TILES_D64X64_W8 = 13
 TILES_D64X32_W4 = 14
 TILES_D64X32_W8 = 15
+TILES_R64X64_W4 = 16
+TILES_R64X32_W4 = 17
+TILES_R32X64_W4 = 18
 TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(16)),
              'bf16x3': (0, 1, 4, 7)}
+# tile shapes of the parity form (hcipy_mft_plan_set_parity)
+PARITY_TILE_CFGS = {'f16x2': (0, 1, 4, 5, 6, 7, 16, 17, 18), 'bf16x3': (0, 1, 4, 7)}
 
 
 class MftGroupItem(ctypes.Structure):
@@ -95,6 +100,8 @@ SIGNATURES = {
     'hcipy_mft_plan_bytes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_mft_plan_set_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     'hcipy_mft_plan_get_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    'hcipy_mft_plan_set_parity': (ctypes.c_int, [_vp, ctypes.c_int]),
+    'hcipy_mft_plan_get_parity': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     'hcipy_mft_group_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
     'hcipy_mft_group_destroy': (ctypes.c_int, [_vp]),
     'hcipy_mft_execute_grouped': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),

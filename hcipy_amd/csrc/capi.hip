@@ -1,4 +1,5 @@
 // C-ABI of the library: error state and the matrix-Fourier-transform plan.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -47,6 +48,14 @@ int launch_split_rows(int prec, const float* in, int64_t ld_in, int K, int64_t r
                       float* row_scale, hipStream_t s);
 int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, SplitOut out,
                       int64_t lda, int rows_pad, hipStream_t s);
+bool real_cfg_supported(int prec, int cfg);
+int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s);
+int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis& ax, const ParAxis& ay,
+                        int64_t batch, SplitOut o, int64_t ld_out, float* row_scale, hipStream_t s);
+int launch_gen_parity(const double* c, int nc, const double* k, int nk, int prec, SplitOut out, int64_t lda,
+                      int rows_pad, hipStream_t s);
+int launch_parity_combine(const double* H, const ParAxis& au, const ParAxis& av, int nu, int nv, double sign, double ar,
+                          double ai, int out_mode, void* out, int64_t batch, hipStream_t s);
 
 namespace {
 
@@ -78,6 +87,14 @@ struct FMatrix {
   size_t bytes() const { return hi == nullptr ? 0 : (size_t)rows_pad * lda; }
 };
 
+// Real parity matrices of one stage: plane 0 cos, plane 1 sin, rows_pad x lda reals each.
+struct PMatrix {
+  void* p[3] = {nullptr, nullptr, nullptr};
+  int64_t lda = 0;
+  int rows_pad = 0;
+  int64_t plane() const { return (int64_t)rows_pad * lda; }
+};
+
 struct Plan {
   int ndim = 0, prec = 0, device = 0;
   int64_t nx = 1, ny = 1, nu = 1, nv = 1;
@@ -85,6 +102,12 @@ struct Plan {
   double* coords = nullptr;
   size_t matrix_bytes = 0;
   int tiles[2][2] = {{kTileAuto, kTileAuto}, {kTileAuto, kTileAuto}};  // [direction][stage]
+  // Parity form (grids symmetric about 0, split precisions): available, and in use for single
+  // executions.
+  bool parity_ok = false, parity = false;
+  PMatrix pf1, pf2, pb1, pb2;  // forward stage 1/2, backward stage 1/2
+  ParAxis px{}, py{}, pu{}, pv{};  // axis tables (device: plus, minus, P)
+  void* ptables = nullptr;
 };
 
 int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, int nk, double sign,
@@ -108,6 +131,92 @@ void free_matrix(FMatrix& m) {
   if (m.lo) (void)hipFree(m.lo);
   if (m.lo2) (void)hipFree(m.lo2);
   m.hi = m.lo = m.lo2 = nullptr;
+}
+
+// Parity matrices of a stage with nc rows (positive half of the output coordinates c) and nk
+// contraction values (positive half of k); rows padded to whole 128-row tiles.
+int make_parity(Plan& pl, PMatrix& m, const double* c, int nc, const double* k, int nk, hipStream_t s) {
+  m.lda = padded_row(nk);
+  m.rows_pad = (int)round_up((int64_t)nc, 128);
+  const size_t bytes = (size_t)2 * m.plane() * elem_size(pl.prec);
+  for (int i = 0; i < n_arrays(pl.prec); ++i) {
+    HCIPY_HIP(hipMalloc(&m.p[i], bytes));
+    pl.matrix_bytes += bytes;
+  }
+  return launch_gen_parity(c, nc, k, nk, pl.prec, SplitOut{{m.p[0], m.p[1], m.p[2]}}, m.lda, m.rows_pad, s);
+}
+
+void free_parity(PMatrix& m) {
+  for (auto& q : m.p) {
+    if (q) (void)hipFree(q);
+    q = nullptr;
+  }
+}
+
+// Parity description of one axis (host): magnitudes P[j] >= 0 with the indices of the samples
+// at +P[j] / -P[j] (-1 if absent), from pairing x[i] with the x[k] = -x[i] (to 1e-12 of max |x|).
+// False when the axis is not symmetric enough to gain: more than n / 2 + 1 magnitudes, or
+// repeated coordinates.
+struct HostAxis {
+  std::vector<double> P;
+  std::vector<int> plus, minus;
+};
+
+bool parity_axis(const double* x, int64_t n, HostAxis& a) {
+  if (n < 2) return false;
+  std::vector<int> idx((size_t)n);
+  for (int64_t i = 0; i < n; ++i) idx[(size_t)i] = (int)i;
+  std::sort(idx.begin(), idx.end(), [&](int i, int k) { return x[i] < x[k]; });
+  double mx = 0.0;
+  for (int64_t i = 0; i < n; ++i) mx = std::max(mx, std::fabs(x[i]));
+  if (!(mx > 0.0) || !(mx < INFINITY)) return false;
+  const double tol = 1e-12 * mx;
+  for (int64_t i = 1; i < n; ++i)
+    if (x[idx[(size_t)i]] - x[idx[(size_t)i - 1]] <= tol) return false;
+  a = HostAxis{};
+  int64_t lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    const double s = x[idx[(size_t)lo]] + x[idx[(size_t)hi]];
+    if (lo == hi) {
+      const double v = x[idx[(size_t)lo]];
+      if (std::fabs(v) <= tol) {
+        a.P.push_back(0.0);
+        a.plus.push_back(idx[(size_t)lo]);
+        a.minus.push_back(-1);
+      } else if (v > 0) {
+        a.P.push_back(v);
+        a.plus.push_back(idx[(size_t)lo]);
+        a.minus.push_back(-1);
+      } else {
+        a.P.push_back(-v);
+        a.plus.push_back(-1);
+        a.minus.push_back(idx[(size_t)lo]);
+      }
+      break;
+    }
+    if (std::fabs(s) <= tol) {  // a mirror pair
+      a.P.push_back(x[idx[(size_t)hi]]);
+      a.plus.push_back(idx[(size_t)hi]);
+      a.minus.push_back(idx[(size_t)lo]);
+      ++lo;
+      --hi;
+    } else if (s < 0) {  // the most negative sample has no mirror
+      a.P.push_back(-x[idx[(size_t)lo]]);
+      a.plus.push_back(-1);
+      a.minus.push_back(idx[(size_t)lo]);
+      ++lo;
+    } else {
+      a.P.push_back(x[idx[(size_t)hi]]);
+      a.plus.push_back(idx[(size_t)hi]);
+      a.minus.push_back(-1);
+      --hi;
+    }
+  }
+  // smallest magnitude first: on regular grids plus[] then ascends and minus[] descends by one
+  std::reverse(a.P.begin(), a.P.end());
+  std::reverse(a.plus.begin(), a.plus.end());
+  std::reverse(a.minus.begin(), a.minus.end());
+  return (int64_t)a.P.size() <= n / 2 + 1;
 }
 
 // Geometry of one direction of a 2-D (or 1-D) transform.
@@ -171,6 +280,47 @@ Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
   return w;
 }
 
+// Workspace of the parity form of one direction: parity split F (4 items per field, rows
+// (y', re / im), x' contiguous), its row scales and per-item scales, stage-1 result G (rows
+// (u', re / im), y' contiguous) and the FP64 products H (rows v', complex u').  H is written by
+// stage 2, after stage 1 has read the last of F: the two share one region.
+struct ParWs {
+  size_t f[3] = {0, 0, 0}, rs = 0, t = 0, g[3] = {0, 0, 0}, h = 0, total = 0;
+  int64_t ldf = 0, ldg = 0;
+};
+
+ParWs parity_layout(const Plan& pl, const Dir& d, int64_t batch) {
+  ParWs w;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (size_t)round_up((int64_t)bytes, kAlign);
+    return at;
+  };
+  const bool fw = d.m1 == &pl.f1;
+  const int64_t hk = (fw ? pl.px : pl.pu).h, hn = (fw ? pl.py : pl.pv).h;
+  const int64_t hc1 = (fw ? pl.pu : pl.px).h, hc2 = (fw ? pl.pv : pl.py).h;
+  const int na = n_arrays(pl.prec);
+  w.ldf = padded_row(hk);
+  const size_t fbytes = (size_t)round_up((int64_t)batch * 4 * 2 * hn * w.ldf * 2, kAlign);
+  const size_t hbytes = (size_t)batch * 4 * hc2 * 2 * hc1 * 8;
+  const size_t shared = take(std::max((size_t)na * fbytes, hbytes));
+  for (int i = 0; i < na; ++i) w.f[i] = shared + (size_t)i * fbytes;
+  w.h = shared;
+  w.rs = take((size_t)batch * 4 * 2 * hn * 4);
+  w.t = take((size_t)batch * 4 * 4);
+  w.ldg = padded_row(hn);
+  for (int i = 0; i < na; ++i) w.g[i] = take((size_t)batch * 4 * 2 * hc1 * w.ldg * 2);
+  w.total = off;
+  return w;
+}
+
+// Workspace bytes of one direction as a single execution runs it.
+size_t direction_ws_bytes(const Plan& pl, int dir, int64_t batch) {
+  const Dir d = direction(pl, dir);
+  return pl.parity ? parity_layout(pl, d, batch).total : workspace_layout(pl, d, batch).total;
+}
+
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 // ev (optional, 4 events): recorded before the split, before stage 1, before stage 2, after stage 2.
@@ -200,6 +350,109 @@ struct Collected {
   int64_t nz2 = 0;  // batch slices of stage 2 (0: 1-D plan, no stage 2)
 };
 
+// Runs one direction in the parity form: parity split, stage 1 (real half-size constants, four
+// items per field), stage 2 (likewise, FP64 products), recombination into the four quadrants.
+// `only` >= 0 launches one step (0 split, 1 stage 1, 2 stage 2 and recombination).
+int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai, char* ws,
+               hipStream_t s, hipEvent_t* ev, int output, int only) {
+  auto mark = [&](int i) -> int {
+    if (ev != nullptr) return check_hip(hipEventRecord(ev[i], s), "hipEventRecord");
+    return kOk;
+  };
+  const Dir d = direction(pl, dir);
+  const bool fw = dir == HCIPY_FORWARD;
+  const PMatrix& p1 = fw ? pl.pf1 : pl.pb1;
+  const PMatrix& p2 = fw ? pl.pf2 : pl.pb2;
+  const ParAxis& ak = fw ? pl.px : pl.pu;   // input x (contraction of stage 1)
+  const ParAxis& an = fw ? pl.py : pl.pv;   // input y (contraction of stage 2)
+  const ParAxis& ac1 = fw ? pl.pu : pl.px;  // output x
+  const ParAxis& ac2 = fw ? pl.pv : pl.py;  // output y
+  const int out_mode = out_mode_for(pl, output);
+  HCIPY_REQUIRE(out_mode >= 0, "unknown output kind %d", output);
+  const ParWs w = parity_layout(pl, d, batch);
+  const int64_t hn = an.h, hk = ak.h, hc1 = ac1.h, hc2 = ac2.h;
+  const int na = n_arrays(pl.prec);
+  float* rs = reinterpret_cast<float*>(ws + w.rs);
+  float* t = reinterpret_cast<float*>(ws + w.t);
+  double* H = reinterpret_cast<double*>(ws + w.h);
+  int st = mark(0);
+  if (!st && (only < 0 || only == 0)) {
+    SplitOut fo{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
+    st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch, fo, w.ldf, rs,
+                             s);
+  }
+  if (!st) st = mark(1);
+  if (st) return st;
+  constexpr int64_t kChunk = kMaxGridZ / 4;  // fields per launch (four grid.z slices each)
+  for (int64_t b0 = 0; b0 < batch && (only < 0 || only == 1); b0 += kChunk) {
+    const int nb = (int)std::min<int64_t>(kChunk, batch - b0);
+    StageArgs a{};
+    a.tile_cfg = pl.tiles[dir][0];
+    a.a = p1.p[0];
+    a.a_lo = p1.p[1];
+    a.a_lo2 = p1.p[2];
+    a.lda = p1.lda;
+    a.a_zstride = p1.plane();
+    a.a_zshift = 0;
+    const int64_t fz = 2 * hn * w.ldf;  // elements per item of F
+    a.b = ws + w.f[0] + (size_t)b0 * 4 * fz * 2;
+    a.b_lo = na > 1 ? ws + w.f[1] + (size_t)b0 * 4 * fz * 2 : nullptr;
+    a.b_lo2 = na > 2 ? ws + w.f[2] + (size_t)b0 * 4 * fz * 2 : nullptr;
+    a.ldb = w.ldf;
+    a.b_zstride = fz;
+    a.M = (int)hc1;
+    a.N = (int)(2 * hn);
+    a.K = (int)hk;
+    a.row_scale = rs + b0 * 4 * 2 * hn;
+    a.rs_zstride = 2 * hn;
+    a.out_mode = kOutSplitT;
+    const int64_t gz = 2 * hc1 * w.ldg;  // elements per item of G
+    a.out = ws + w.g[0] + (size_t)b0 * 4 * gz * 2;
+    a.out_lo = na > 1 ? ws + w.g[1] + (size_t)b0 * 4 * gz * 2 : nullptr;
+    a.out_lo2 = na > 2 ? ws + w.g[2] + (size_t)b0 * 4 * gz * 2 : nullptr;
+    a.ldo = w.ldg;
+    a.out_zstride = gz;
+    a.fold = (int)(2 * hn);
+    a.t_out = t + b0 * 4;
+    a.t_zstride = 1;
+    a.t_log2k = ilog2_ceil(hk);
+    a.alpha_re = 1.0;
+    if ((st = launch_stage_real(pl.prec, a, 4 * nb, s))) return st;
+  }
+  if ((st = mark(2))) return st;
+  if (only >= 0 && only != 2) return mark(3);
+  for (int64_t b0 = 0; b0 < batch; b0 += kChunk) {
+    const int nb = (int)std::min<int64_t>(kChunk, batch - b0);
+    StageArgs a{};
+    a.tile_cfg = pl.tiles[dir][1];
+    a.a = p2.p[0];
+    a.a_lo = p2.p[1];
+    a.a_lo2 = p2.p[2];
+    a.lda = p2.lda;
+    a.a_zstride = p2.plane();
+    a.a_zshift = 1;
+    const int64_t gz = 2 * hc1 * w.ldg;
+    a.b = ws + w.g[0] + (size_t)b0 * 4 * gz * 2;
+    a.b_lo = na > 1 ? ws + w.g[1] + (size_t)b0 * 4 * gz * 2 : nullptr;
+    a.b_lo2 = na > 2 ? ws + w.g[2] + (size_t)b0 * 4 * gz * 2 : nullptr;
+    a.ldb = w.ldg;
+    a.b_zstride = gz;
+    a.M = (int)hc2;
+    a.N = (int)(2 * hc1);
+    a.K = (int)hn;
+    a.z_scale = t + b0 * 4;
+    a.out_mode = kOutR64;
+    a.out = H + b0 * 4 * hc2 * 2 * hc1;
+    a.ldo = 2 * hc1;
+    a.out_zstride = hc2 * 2 * hc1;
+    a.alpha_re = 1.0;
+    if ((st = launch_stage_real(pl.prec, a, 4 * nb, s))) return st;
+  }
+  st = launch_parity_combine(H, ac1, ac2, (int)d.c1, (int)d.c2, fw ? -1.0 : 1.0, ar, ai, out_mode, out, batch, s);
+  if (st) return st;
+  return mark(3);
+}
+
 // Runs one direction: data split (split-FP16 only), stage 1, stage 2.  `only` >= 0 launches just
 // that kernel (0 split, 1 stage 1, 2 stage 2) on a workspace a full run has filled (timing).
 // With `col`, the split is launched and the stage arguments are returned instead of launched.
@@ -208,6 +461,8 @@ struct Collected {
 int run_direction(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai,
                   char* ws, hipStream_t s, hipEvent_t* ev = nullptr, int output = HCIPY_OUTPUT_FIELD, int only = -1,
                   Collected* col = nullptr, const Collected* shared_split = nullptr) {
+  if (pl.parity && col == nullptr && shared_split == nullptr)
+    return run_parity(pl, dir, in, out, batch, ar, ai, ws, s, ev, output, only);
   auto mark = [&](int i) -> int {
     if (ev != nullptr) return check_hip(hipEventRecord(ev[i], s), "hipEventRecord");
     return kOk;
@@ -646,6 +901,71 @@ int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const d
     st = make_matrix(pl, pl.f2, dv, (int)nv, dy, (int)ny, -1.0, s);
     if (!st) st = make_matrix(pl, pl.b2, dy, (int)ny, dv, (int)nv, +1.0, s);
   }
+  // parity form: 2-D split precisions between grids symmetric about 0 on every axis
+  HostAxis hx, hy, hu, hv;
+  pl.parity_ok = !st && ndim == 2 && is_split(precision) && parity_axis(x_in, nx, hx) && parity_axis(y_in, ny, hy) &&
+                 parity_axis(u_out, nu, hu) && parity_axis(v_out, nv, hv);
+  if (pl.parity_ok) {
+    // one device block: per axis plus[h], minus[h] (int), then per axis P[h] (double)
+    const HostAxis* ha[4] = {&hx, &hy, &hu, &hv};
+    ParAxis* pa[4] = {&pl.px, &pl.py, &pl.pu, &pl.pv};
+    size_t ints = 0, dbls = 0;
+    for (const HostAxis* a : ha) {
+      ints += 2 * a->P.size();
+      dbls += a->P.size();
+    }
+    const size_t ibytes = round_up((int64_t)(ints * sizeof(int)), 16);
+    std::vector<char> host(ibytes + dbls * sizeof(double));
+    int* hi = reinterpret_cast<int*>(host.data());
+    double* hd = reinterpret_cast<double*>(host.data() + ibytes);
+    st = check_hip(hipMalloc(&pl.ptables, host.size()), "hipMalloc(parity tables)");
+    if (!st) {
+      size_t io = 0, dof = 0;
+      for (int i = 0; i < 4; ++i) {
+        const HostAxis& a = *ha[i];
+        const size_t h = a.P.size();
+        std::copy(a.plus.begin(), a.plus.end(), hi + io);
+        std::copy(a.minus.begin(), a.minus.end(), hi + io + h);
+        std::copy(a.P.begin(), a.P.end(), hd + dof);
+        pa[i]->plus = static_cast<const int*>(pl.ptables) + io;
+        pa[i]->minus = static_cast<const int*>(pl.ptables) + io + h;
+        pa[i]->P = reinterpret_cast<const double*>(static_cast<char*>(pl.ptables) + ibytes) + dof;
+        pa[i]->h = (int)h;
+        pa[i]->n = (int)(i == 0 ? nx : i == 1 ? ny : i == 2 ? nu : nv);
+        // affine description of the tables when it reproduces them exactly
+        int p0 = 0, m0 = 0, mlo = 0;
+        for (size_t j = 0; j < h; ++j)
+          if (a.plus[j] >= 0) {
+            p0 = a.plus[j] - (int)j;
+            break;
+          }
+        for (size_t j = 0; j < h; ++j)
+          if (a.minus[j] >= 0) {
+            m0 = a.minus[j] + (int)j;
+            mlo = (int)j;
+            break;
+          }
+        bool aff = true;
+        for (size_t j = 0; j < h && aff; ++j) {
+          const int pj = p0 + (int)j < pa[i]->n ? p0 + (int)j : -1;
+          const int mj = ((int)j >= mlo && m0 - (int)j >= 0) ? m0 - (int)j : -1;
+          aff = (pj == a.plus[j] || (a.plus[j] < 0 && pj < 0)) && mj == a.minus[j] && p0 + (int)j >= 0;
+        }
+        pa[i]->affine = aff ? 1 : 0;
+        pa[i]->p0 = p0;
+        pa[i]->m0 = m0;
+        pa[i]->mlo = mlo;
+        io += 2 * h;
+        dof += h;
+      }
+      st = check_hip(hipMemcpy(pl.ptables, host.data(), host.size(), hipMemcpyHostToDevice), "copy parity tables");
+    }
+    if (!st) st = make_parity(pl, pl.pf1, pl.pu.P, pl.pu.h, pl.px.P, pl.px.h, s);
+    if (!st) st = make_parity(pl, pl.pf2, pl.pv.P, pl.pv.h, pl.py.P, pl.py.h, s);
+    if (!st) st = make_parity(pl, pl.pb1, pl.px.P, pl.px.h, pl.pu.P, pl.pu.h, s);
+    if (!st) st = make_parity(pl, pl.pb2, pl.py.P, pl.py.h, pl.pv.P, pl.pv.h, s);
+    pl.parity = true;
+  }
   if (!st) st = check_hip(hipStreamSynchronize(s), "plan creation");
   if (st) return fail(st);
   *plan = h;
@@ -659,6 +979,11 @@ int hcipy_mft_plan_destroy(hcipy_mft_plan plan) {
   free_matrix(pl.f2);
   free_matrix(pl.b1);
   free_matrix(pl.b2);
+  free_parity(pl.pf1);
+  free_parity(pl.pf2);
+  free_parity(pl.pb1);
+  free_parity(pl.pb2);
+  if (pl.ptables) (void)hipFree(pl.ptables);
   if (pl.coords) (void)hipFree(pl.coords);
   delete plan;
   return kOk;
@@ -671,6 +996,31 @@ int hcipy_mft_workspace_size(hcipy_mft_plan plan, int64_t batch, size_t* bytes) 
   const Ws f = workspace_layout(pl, direction(pl, HCIPY_FORWARD), batch);
   const Ws b = workspace_layout(pl, direction(pl, HCIPY_BACKWARD), batch);
   *bytes = std::max(f.total, b.total);
+  // enough for the parity form too, so a workspace stays valid when it is switched on or off
+  if (pl.parity_ok) {
+    *bytes = std::max(*bytes, parity_layout(pl, direction(pl, HCIPY_FORWARD), batch).total);
+    *bytes = std::max(*bytes, parity_layout(pl, direction(pl, HCIPY_BACKWARD), batch).total);
+  }
+  return kOk;
+}
+
+int hcipy_mft_plan_set_parity(hcipy_mft_plan plan, int enable) {
+  HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
+  Plan& pl = plan->p;
+  HCIPY_REQUIRE(!enable || pl.parity_ok,
+                "the parity form needs a two-dimensional split-precision plan between grids whose axes are all "
+                "symmetric about 0 (every sample but at most two paired with its mirror image)");
+  if (pl.parity != (enable != 0)) {
+    pl.parity = enable != 0;
+    for (auto& d : pl.tiles) d[0] = d[1] = kTileAuto;  // tile shapes of the other form do not carry over
+  }
+  return kOk;
+}
+
+int hcipy_mft_plan_get_parity(hcipy_mft_plan plan, int* available, int* enabled) {
+  HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
+  if (available) *available = plan->p.parity_ok ? 1 : 0;
+  if (enabled) *enabled = plan->p.parity ? 1 : 0;
   return kOk;
 }
 
@@ -684,8 +1034,10 @@ int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction_, int stage, int
   HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
   HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "unknown direction %d", direction_);
   HCIPY_REQUIRE(stage == 0 || stage == 1, "stage must be 0 or 1, got %d", stage);
-  HCIPY_REQUIRE(tile_cfg == HCIPY_TILES_AUTO || tile_cfg_supported(plan->p.prec, tile_cfg),
-                "tile configuration %d is not available for precision %d", tile_cfg, plan->p.prec);
+  HCIPY_REQUIRE(tile_cfg == HCIPY_TILES_AUTO || (plan->p.parity ? real_cfg_supported(plan->p.prec, tile_cfg)
+                                                                 : tile_cfg_supported(plan->p.prec, tile_cfg)),
+                "tile configuration %d is not available for precision %d%s", tile_cfg, plan->p.prec,
+                plan->p.parity ? " in the parity form" : "");
   plan->p.tiles[direction_][stage] = tile_cfg;
   return kOk;
 }
@@ -713,9 +1065,9 @@ int hcipy_mft_execute_ex(hcipy_mft_plan plan, int direction_, const void* in, vo
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(in != nullptr && out != nullptr, "NULL data pointer");
   const Plan& pl = plan->p;
-  const Ws w = workspace_layout(pl, direction(pl, direction_), batch);
-  HCIPY_REQUIRE(workspace_bytes >= w.total, "workspace too small: %zu < %zu bytes", workspace_bytes, w.total);
-  HCIPY_REQUIRE(w.total == 0 || workspace != nullptr, "workspace is NULL");
+  const size_t need = direction_ws_bytes(pl, direction_, batch);
+  HCIPY_REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+  HCIPY_REQUIRE(need == 0 || workspace != nullptr, "workspace is NULL");
   HCIPY_REQUIRE(aligned16(workspace), "workspace must be 16-byte aligned");
   int cur = -1;
   HCIPY_HIP(hipGetDevice(&cur));
@@ -732,11 +1084,11 @@ int hcipy_mft_execute_noalloc(hcipy_mft_plan plan, int direction_, const void* i
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(in != nullptr && out != nullptr, "NULL data pointer");
   const Plan& pl = plan->p;
-  const Ws w = workspace_layout(pl, direction(pl, direction_), batch);
-  if (w.total > (size_t)HCIPY_NOALLOC_SCRATCH_BYTES) {
+  const size_t need = direction_ws_bytes(pl, direction_, batch);
+  if (need > (size_t)HCIPY_NOALLOC_SCRATCH_BYTES) {
     set_last_error("hcipy_mft_execute_noalloc: %lld fields need %zu bytes of workspace, above the %d-byte static "
                    "scratch; use hcipy_mft_execute_ex with a caller workspace or a smaller batch",
-                   (long long)batch, w.total, (int)HCIPY_NOALLOC_SCRATCH_BYTES);
+                   (long long)batch, need, (int)HCIPY_NOALLOC_SCRATCH_BYTES);
     return kOutOfRange;
   }
   int cur = -1;
@@ -755,8 +1107,8 @@ int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void*
   HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
   HCIPY_REQUIRE(batch > 0 && in != nullptr && out != nullptr, "bad data arguments");
   const Plan& pl = plan->p;
-  const Ws w = workspace_layout(pl, direction(pl, direction_), batch);
-  HCIPY_REQUIRE(workspace_bytes >= w.total, "workspace too small: %zu < %zu bytes", workspace_bytes, w.total);
+  const size_t need = direction_ws_bytes(pl, direction_, batch);
+  HCIPY_REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
   HCIPY_REQUIRE(aligned16(workspace), "workspace must be 16-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipEvent_t ev[2];
@@ -907,7 +1259,7 @@ int hcipy_mft_autotune(hcipy_mft_plan plan, int direction_, const void* in, void
     int best = kTileAuto;
     float best_ms = 0.0f;
     for (int cfg = 0; cfg < kNumTileCfg; ++cfg) {
-      if (!tile_cfg_supported(pl.prec, cfg)) continue;
+      if (!(pl.parity ? real_cfg_supported(pl.prec, cfg) : tile_cfg_supported(pl.prec, cfg))) continue;
       pl.tiles[direction_][stage] = cfg;
       float ms[3] = {0.0f, 0.0f, 0.0f};
       const int st = hcipy_mft_profile(plan, direction_, in, out, batch, 1.0, 0.0, output, workspace, workspace_bytes,

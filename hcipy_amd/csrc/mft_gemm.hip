@@ -95,7 +95,9 @@ __device__ inline u32x4_t load_masked(const T* src, int valid) {
 enum BMode : int {
   kRowsPadded = 0,   // rows padded to a multiple of 64 elements: 16-byte loads never leave the row
   kRowsAligned = 1,  // 16-byte aligned rows of exactly K elements: vector loads inside K, scalar tail
-  kRowsUnaligned = 2 // element loads only
+  kRowsUnaligned = 2,// element loads only
+  kRealPadded = 3    // padded data rows, and a real constant of BM plain rows per tile (no 2x2
+                     // expansion): C[m][n] is real and the epilogue writes single values
 };
 
 // XCD-aware tile order.  Consecutive workgroups are dispatched round-robin over the 8 XCDs; the
@@ -164,7 +166,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   constexpr int VEC = 16 / (int)sizeof(T);
   constexpr int LDK = BK + VEC;  // 16-byte row padding against LDS bank conflicts
   constexpr int NARR = TR::NARR;
-  constexpr int BMC = BM / 2;    // compact constant rows per tile
+  constexpr bool REAL = (BMODE == kRealPadded);
+  constexpr int BMC = REAL ? BM : BM / 2;  // constant rows loaded per tile (compact complex, or real)
   constexpr int WTM = BM / WM;
   constexpr int WTN = BN / WN;
   constexpr int TM = WTM / MT;
@@ -206,10 +209,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
 
   const T* Ag[3] = {static_cast<const T*>(p.a), static_cast<const T*>(p.a_lo), static_cast<const T*>(p.a_lo2)};
   const T* Bg[3] = {static_cast<const T*>(p.b), static_cast<const T*>(p.b_lo), static_cast<const T*>(p.b_lo2)};
+  const int64_t a_plane = (int64_t)((z >> p.a_zshift) & 1) * p.a_zstride;
 #pragma unroll
-  for (int arr = 0; arr < NARR; ++arr) Bg[arr] += (int64_t)z * p.b_zstride;
+  for (int arr = 0; arr < NARR; ++arr) {
+    Ag[arr] += a_plane;
+    Bg[arr] += (int64_t)z * p.b_zstride;
+  }
 
-  const bool split_out = (p.out_mode == kOutSplit);
+  const bool split_out = (p.out_mode == kOutSplit || p.out_mode == kOutSplitT);
   int item0 = 0;
 
   // Per-thread chunk geometry (fixed over the k loop).
@@ -232,7 +239,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   const int nk_full = K / BK;  // k-tiles with no element beyond K
   // Largest k offset a full-vector load may start at: padded rows allow every chunk of the last
   // k-tile; exact rows (K a multiple of VEC) are clamped to the last chunk.
-  const int kmax_vec = (BMODE == kRowsPadded) ? (nk * BK - VEC) : (K - VEC);
+  const int kmax_vec = (BMODE == kRowsPadded || REAL) ? (nk * BK - VEC) : (K - VEC);
 
   u32x4_t ra[PF][NARR][AL];
   u32x4_t rb[PF][NARR][BL];
@@ -245,8 +252,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
 #pragma unroll
       for (int i = 0; i < AL; ++i) {
         const int ch = tid + i * NTHR;
-        ra[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Ag[arr] + (int64_t)(mc0 + ch / CPR) * p.lda + k0 +
-                                                          (ch % CPR) * VEC);
+        ra[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Ag[arr] + (int64_t)((REAL ? m0 : mc0) + ch / CPR) * p.lda +
+                                                          k0 + (ch % CPR) * VEC);
       }
 #pragma unroll
       for (int i = 0; i < BL; ++i) {
@@ -275,10 +282,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
       T* sB = base + ABUF + arr * BN * LDK;
 #pragma unroll
       for (int i = 0; i < AL; ++i) {
-        // compact row r -> expanded rows 2r (re, -im) and 2r+1 (im, re)
-        const int o = a_off[i] + (a_off[i] / LDK) * LDK;
-        *reinterpret_cast<u32x4_t*>(&sA[o]) = expand_even<T>(ra[S][arr][i]);
-        *reinterpret_cast<u32x4_t*>(&sA[o + LDK]) = expand_odd<T>(ra[S][arr][i]);
+        if constexpr (REAL) {
+          *reinterpret_cast<u32x4_t*>(&sA[a_off[i]]) = ra[S][arr][i];
+        } else {
+          // compact row r -> expanded rows 2r (re, -im) and 2r+1 (im, re)
+          const int o = a_off[i] + (a_off[i] / LDK) * LDK;
+          *reinterpret_cast<u32x4_t*>(&sA[o]) = expand_even<T>(ra[S][arr][i]);
+          *reinterpret_cast<u32x4_t*>(&sA[o + LDK]) = expand_odd<T>(ra[S][arr][i]);
+        }
       }
 #pragma unroll
       for (int i = 0; i < BL; ++i) {
@@ -296,9 +307,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   // has accumulated (FE k-tiles).  With FE = 1 and a single MFMA tile per wave, two sets
   // alternate over k-tiles so that the flush of one overlaps the MFMAs of the other.
   constexpr int KPW = KV_PER_WAVE * KVEC;  // real k per wave per k-tile
-  constexpr int FE = (KPW >= kFlushK) ? 1 : kFlushK / KPW;
-  static_assert(FE * KPW == kFlushK || FE == 1, "flush interval must be a whole number of k-tiles");
-  constexpr int NSET = (TR::SPLIT && FE == 1 && TM * TN < 4) ? 2 : 1;
+  // The parity form flushes twice as often: its contiguous k ranges are coherent (the
+  // products near the peak add up with one sign), so FP32 partial sums grow faster.
+  constexpr int FLUSHK = REAL ? kFlushK / 2 : kFlushK;
+  constexpr int FE = (KPW >= FLUSHK) ? 1 : FLUSHK / KPW;
+  static_assert(FE * KPW == FLUSHK || FE == 1, "flush interval must be a whole number of k-tiles");
+  constexpr int NSET = (TR::SPLIT && FE == 1 && TM * TN < 4 && !REAL) ? 2 : 1;
   // Cross products go to two accumulators for a single tile (independent MFMAs back to back),
   // to one when the wave has several tiles.
   constexpr int NXA = (TR::SPLIT && TM * TN < 2) ? 2 : 1;
@@ -484,7 +498,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
     item0 = n0 / p.fold;
     const int nitems = (nend - 1) / p.fold - item0 + 1;
     for (int it = 0; it < nitems; ++it) {
-      const float* rs = p.row_scale + (int64_t)(item0 + it) * p.fold;
+      const float* rs = p.row_scale + (int64_t)z * p.rs_zstride + (int64_t)(item0 + it) * p.fold;
       float mn = FLT_MAX;
       int r = tid;
       for (; r + 3 * NTHR < p.fold; r += 4 * NTHR)
@@ -499,7 +513,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
         for (int w = 1; w < NTHR / 64; ++w) m = fminf(m, s_red[w]);
         const float t = ldexpf(m, -p.t_log2k);
         s_t[it] = t;
-        if (bx == 0 && p.t_out != nullptr) p.t_out[item0 + it] = t;
+        if (bx == 0 && p.t_out != nullptr) p.t_out[(int64_t)z * p.t_zstride + item0 + it] = t;
       }
       __syncthreads();
     }
@@ -583,8 +597,50 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
     }
   }
 
-  // Epilogue: complex values (rows 2c, 2c+1) of column n.
   const double inv_z = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[z] : 1.0;
+  if constexpr (REAL) {
+    // Epilogue of a real constant: one real value per accumulator register.
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int n = n0 + wn * WTN + j * MT + lane % MT;
+        if (n >= N) continue;
+        double sc = inv_z;
+        if (p.row_scale != nullptr) sc /= (double)p.row_scale[(int64_t)z * p.rs_zstride + n];
+        if (split_out) sc *= (double)s_t[n / p.fold - item0];
+#pragma unroll
+        for (int r = 0; r < NACC; ++r) {
+          const int m = m0 + wm * WTM + i * MT + acc_row<MT>(r, lane);
+          if (m >= M) continue;
+          double v;
+          if constexpr (TR::SPLIT) v = acc64[i][j][r];
+          else v = (double)acc[0][i][j][r];
+          v *= sc;
+          if (p.out_mode == kOutR64) {
+            static_cast<double*>(p.out)[(int64_t)z * p.out_zstride + (int64_t)m * p.ldo + n] = v;
+          } else if constexpr (TR::SPLIT) {
+            const int64_t idx = (int64_t)z * p.out_zstride + (int64_t)(2 * m + (n & 1)) * p.ldo + (n >> 1);
+            if constexpr (P == kBF16x3) {
+              const __bf16 q0 = f32_to_bf16((float)v);
+              const double r1 = v - (double)(float)q0;
+              const __bf16 q1 = f32_to_bf16((float)r1);
+              static_cast<__bf16*>(p.out)[idx] = q0;
+              static_cast<__bf16*>(p.out_lo)[idx] = q1;
+              static_cast<__bf16*>(p.out_lo2)[idx] = f32_to_bf16((float)(r1 - (double)(float)q1));
+            } else {
+              const _Float16 h = f32_to_f16((float)v);
+              static_cast<_Float16*>(p.out)[idx] = h;
+              static_cast<_Float16*>(p.out_lo)[idx] = f32_to_f16((float)((v - (double)h) * (double)kLoInv));
+            }
+          }
+        }
+      }
+    }
+    return;
+  }
+
+  // Epilogue: complex values (rows 2c, 2c+1) of column n.
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
 #pragma unroll
@@ -1557,6 +1613,247 @@ __global__ void gen_matrix_kernel(const double* __restrict__ a, int nc, const do
 }
 
 // ---------------------------------------------------------------------------------------------
+// Parity form of the 2-D transform between grids that are symmetric about 0 on every axis.
+//
+// Each axis is described by magnitudes P[j] >= 0 and the grid indices of the samples at +P[j]
+// and -P[j] (either may be absent: the origin has no separate mirror, and an FFT-style grid's
+// first sample -n/2 d has none).  With y', x' indexing the magnitudes of the input axes, the
+// four mirror samples p = E(+y', +x'), q = E(+y', -x'), r = E(-y', +x'), s = E(-y', -x') (absent
+// samples count as 0) give
+//   F_ee = p + q + r + s,  F_eo = p - q + r - s,  F_oe = p + q - r - s,  F_oo = p - q - r + s
+// (first letter the y parity, second the x parity), and with C = cos(u x), S = sin(u x) per axis
+//   H_ab[v'][u'] = sum_y' Y_a[v'][y'] sum_x' X_b[u'][x'] F_ab[y'][x']      (Y_e/X_e = C, Y_o/X_o = S)
+//   O(sv v', su u') = H_ee + i s su H_eo + i s sv H_oe - su sv H_oo      (s = -1 forward, +1 backward)
+// The eight real half-size products do about a quarter of the multiply-adds of the complex form.
+
+// Splits 4 FP64 values (already scaled into (-1, 1)) into the split arrays of format FMT at element
+// k of a row (8-byte stores): hi = fp16(v), lo = fp16((v - hi) * 2^11), or the three bfloat16
+// parts of the remainders, all taken from the FP64 value.
+template <int FMT>
+__device__ inline void split_store4d(const double* v, void* const* outs, int64_t row_off, int k) {
+  if constexpr (FMT == kSplitF16x2) {
+    typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+    half4_t vh, vl;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const _Float16 q = f32_to_f16((float)v[e]);
+      vh[e] = q;
+      vl[e] = f32_to_f16((float)((v[e] - (double)q) * (double)kLoInv));
+    }
+    *reinterpret_cast<half4_t*>(static_cast<_Float16*>(outs[0]) + row_off + k) = vh;
+    *reinterpret_cast<half4_t*>(static_cast<_Float16*>(outs[1]) + row_off + k) = vl;
+  } else {
+    typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
+    bf4_t b0, b1, b2;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const __bf16 q0 = f32_to_bf16((float)v[e]);
+      const double r = v[e] - (double)(float)q0;
+      const __bf16 q1 = f32_to_bf16((float)r);
+      b0[e] = q0;
+      b1[e] = q1;
+      b2[e] = f32_to_bf16((float)(r - (double)(float)q1));
+    }
+    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[0]) + row_off + k) = b0;
+    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[1]) + row_off + k) = b1;
+    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[2]) + row_off + k) = b2;
+  }
+}
+
+// Parity split of `batch` c64 fields [ny][nx]: item z = 4 b + parity (ee, eo, oe, oo) holds rows
+// 2 y' (real parts) and 2 y' + 1 (imaginary parts) over x' < ax.h, each multiplied by a
+// power-of-two scale (max |value| * scale < 1) and split; row padding up to ld_out is zero.  The
+// parity sums are formed in FP64 (exact for FP32 inputs) and split from FP64.
+// A workgroup of 256 threads takes two y' (128 threads each); a thread holds the four mirror
+// samples of XPT consecutive x' in registers (XPT * 128 >= ld_out), so the field is read once.
+template <int FMT, int XPT>
+__global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restrict__ in, int nx, int ny, ParAxis ax,
+                                                           ParAxis ay, SplitOut o, int64_t ld_out,
+                                                           float* __restrict__ row_scale) {
+  static_assert(XPT % 4 == 0, "x' per thread: whole 4-element stores");
+  __shared__ double red[2][2][8];
+  const int team = threadIdx.x >> 7;
+  const int tt = threadIdx.x & 127;
+  const int yp = blockIdx.x * 2 + team;
+  const int64_t b = blockIdx.y;
+  const int hx = ax.h, hy = ay.h;
+  const bool live = yp < hy;
+  const int jp = live ? par_plus(ay, yp) : -1, jm = live ? par_minus(ay, yp) : -1;
+  const float2* rp = in + (b * ny + (jp >= 0 ? jp : 0)) * (int64_t)nx;
+  const float2* rm = in + (b * ny + (jm >= 0 ? jm : 0)) * (int64_t)nx;
+  const int x0 = tt * XPT;
+  float2 P[XPT], Q[XPT], R[XPT], W[XPT];
+  const float2 zero = make_float2(0.f, 0.f);
+  // indices first, then every load unconditionally at a clamped index (so that they issue back to
+  // back), then the absent samples are zeroed by selects
+  int cps[XPT], cms[XPT];
+#pragma unroll
+  for (int j = 0; j < XPT; ++j) {
+    const int xp = min(x0 + j, hx - 1);
+    cps[j] = x0 + j < hx ? par_plus(ax, xp) : -1;
+    cms[j] = x0 + j < hx ? par_minus(ax, xp) : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < XPT; ++j) {
+    const int cp = max(cps[j], 0), cm = max(cms[j], 0);
+    P[j] = rp[cp];
+    Q[j] = rp[cm];
+    R[j] = rm[cp];
+    W[j] = rm[cm];
+  }
+#pragma unroll
+  for (int j = 0; j < XPT; ++j) {
+    const bool vp = cps[j] >= 0, vm = cms[j] >= 0;
+    P[j] = (jp >= 0 && vp) ? P[j] : zero;
+    Q[j] = (jp >= 0 && vm) ? Q[j] : zero;
+    R[j] = (jm >= 0 && vp) ? R[j] : zero;
+    W[j] = (jm >= 0 && vm) ? W[j] : zero;
+  }
+  // the eight values (parity ab, part c) at index 2 ab + c, in FP64
+  auto combos = [&](int j, double* v) __attribute__((always_inline)) {
+    const double pqx = (double)P[j].x + Q[j].x, pqy = (double)P[j].y + Q[j].y;
+    const double pmx = (double)P[j].x - Q[j].x, pmy = (double)P[j].y - Q[j].y;
+    const double rwx = (double)R[j].x + W[j].x, rwy = (double)R[j].y + W[j].y;
+    const double rmx = (double)R[j].x - W[j].x, rmy = (double)R[j].y - W[j].y;
+    v[0] = pqx + rwx; v[1] = pqy + rwy;  // ee
+    v[2] = pmx + rmx; v[3] = pmy + rmy;  // eo
+    v[4] = pqx - rwx; v[5] = pqy - rwy;  // oe
+    v[6] = pmx - rmx; v[7] = pmy - rmy;  // oo
+  };
+  double mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < XPT; ++j) {
+    double v[8];
+    combos(j, v);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) mx[e] = fmax(mx[e], fabs(v[e]));
+  }
+  const int wv = tt >> 6;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    double m = mx[e];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((tt & 63) == 0) red[team][wv][e] = m;
+  }
+  __syncthreads();
+  if (!live) return;
+  double sc[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const double m = fmax(red[team][0][e], red[team][1][e]);
+    double s = 1.0;
+    if (m > 0.0 && m <= DBL_MAX) s = ldexp(1.0, -(ilogb(m) + 1));  // m * s in [0.5, 1)
+    sc[e] = s;
+  }
+  if (tt < 8) {
+    double v = 1.0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (e == tt) v = sc[e];  // a static register index, no private-memory array
+    row_scale[((b * 4 + (tt >> 1)) * hy + yp) * 2 + (tt & 1)] = (float)v;
+  }
+  if (x0 >= ld_out) return;
+#pragma unroll
+  for (int j0 = 0; j0 < XPT; j0 += 4) {
+    if (x0 + j0 >= ld_out) break;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      double w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double v[8];
+        combos(j0 + j, v);  // recomputed per row: a few adds, fewer live registers
+        w[j] = v[e] * sc[e];
+      }
+      const int64_t row = ((b * 4 + (e >> 1)) * hy + yp) * 2 + (e & 1);
+      split_store4d<FMT>(w, o.p, row * ld_out, x0 + j0);
+    }
+  }
+}
+
+// Real parity matrices of one stage, two planes of rows_pad x lda: plane 0 cos(c[i] k[j]),
+// plane 1 sin(c[i] k[j]) for i < nc, j < nk (c, k: the positive halves of the row and
+// contraction coordinates), zero elsewhere; phase and sincos in FP64, stored in the operand
+// format of `prec`.
+__global__ void gen_parity_kernel(const double* __restrict__ c, int nc, const double* __restrict__ k, int nk, int prec,
+                                  SplitOut out, int64_t lda, int rows_pad) {
+  const int64_t plane = (int64_t)rows_pad * lda;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < 2 * plane;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int pl = (int)(idx / plane);
+    const int64_t rem = idx - pl * plane;
+    const int i = (int)(rem / lda);
+    const int j = (int)(rem - (int64_t)i * lda);
+    double v = 0.0;
+    if (i < nc && j < nk) {
+      double sn, cs;
+      sincos(c[i] * k[j], &sn, &cs);
+      v = pl ? sn : cs;
+    }
+    if (prec == kF64) {
+      static_cast<double*>(out.p[0])[idx] = v;
+    } else if (prec == kF32) {
+      static_cast<float*>(out.p[0])[idx] = (float)v;
+    } else if (prec == kF16x2) {
+      const _Float16 h = (_Float16)v;
+      static_cast<_Float16*>(out.p[0])[idx] = h;
+      static_cast<_Float16*>(out.p[1])[idx] = (_Float16)((v - (double)h) * (double)kLoInv);
+    } else {
+      const __bf16 q0 = f32_to_bf16((float)v);
+      const double r = v - (double)(float)q0;
+      const __bf16 q1 = f32_to_bf16((float)r);
+      static_cast<__bf16*>(out.p[0])[idx] = q0;
+      static_cast<__bf16*>(out.p[1])[idx] = q1;
+      static_cast<__bf16*>(out.p[2])[idx] = f32_to_bf16((float)(r - (double)(float)q1));
+    }
+  }
+}
+
+// Recombination of the four parity products H (item 4 b + ab: au.h ... rows av.h of au.h complex
+// FP64 values) into output field b ([nv][nu]) at the grid indices of +-v', +-u' that exist, times
+// alpha, in the output format.
+__global__ __launch_bounds__(256) void parity_combine_kernel(const double2* __restrict__ H, ParAxis au, ParAxis av,
+                                                             int nu, int nv, double sign, double ar, double ai,
+                                                             int out_mode, void* __restrict__ out, int64_t batch) {
+  const int hu = au.h, hv = av.h;
+  const int64_t per = (int64_t)hu * hv;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= batch * per) return;
+  const int64_t b = idx / per;
+  const int64_t rem = idx - b * per;
+  const int vp = (int)(rem / hu);
+  const int up = (int)(rem - (int64_t)vp * hu);
+  const double2* Hb = H + b * 4 * per + rem;
+  const double2 ee = Hb[0], eo = Hb[per], oe = Hb[2 * per], oo = Hb[3 * per];
+  const int64_t ob = b * (int64_t)nv * nu;
+  const int vis[2] = {par_plus(av, vp), par_minus(av, vp)};
+  const int uis[2] = {par_plus(au, up), par_minus(au, up)};
+#pragma unroll
+  for (int qv = 0; qv < 2; ++qv) {
+    const double sv = qv ? -1.0 : 1.0;
+    const int vi = vis[qv];
+    if (vi < 0) continue;
+#pragma unroll
+    for (int qu = 0; qu < 2; ++qu) {
+      const double su = qu ? -1.0 : 1.0;
+      const int ui = uis[qu];
+      if (ui < 0) continue;
+      // ee + i s su eo + i s sv oe - su sv oo
+      const double a = sign * su, c = sign * sv, d = su * sv;
+      const double orr = ee.x - a * eo.y - c * oe.y - d * oo.x;
+      const double oii = ee.y + a * eo.x + c * oe.x - d * oo.y;
+      const double zr = orr * ar - oii * ai, zi = orr * ai + oii * ar;
+      const int64_t o = ob + (int64_t)vi * nu + ui;
+      if (out_mode == kOutC64) static_cast<float2*>(out)[o] = make_float2((float)zr, (float)zi);
+      else if (out_mode == kOutC128) static_cast<double2*>(out)[o] = make_double2(zr, zi);
+      else if (out_mode == kOutI32) static_cast<float*>(out)[o] = (float)(zr * zr + zi * zi);
+      else static_cast<double*>(out)[o] = zr * zr + zi * zi;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Host-side launchers.
 
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
@@ -1599,7 +1896,7 @@ static int launch_f16x2_dma_t(const StageArgs& a, int nz, hipStream_t s) {
 }
 
 bool tile_cfg_supported(int prec, int cfg) {
-  if (cfg < 0 || cfg >= kNumTileCfg) return false;
+  if (cfg < 0 || cfg >= kR64x64_w4) return false;
   // BF16x3: three operand arrays per matrix; the shapes whose LDS buffers and thread split fit
   if (prec == kBF16x3) return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8;
   if (cfg >= kC64x64_w4) return prec == kF16x2;  // compact-constant shapes: split-FP16 only
@@ -1824,6 +2121,97 @@ int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double s
   hipLaunchKernelGGL(gen_matrix_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, nc, b, nk, sign, prec, out, lda,
                      rows_pad);
   return check_hip(hipGetLastError(), "gen_matrix_kernel launch");
+}
+
+// Parity-form launchers.
+bool real_cfg_supported(int prec, int cfg) {
+  if (prec == kF16x2)
+    return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8 || cfg == k64x32_w8 || cfg == k128x64 ||
+           cfg == k64x128 || cfg == kR64x64_w4 || cfg == kR64x32_w4 || cfg == kR32x64_w4;
+  if (prec == kBF16x3) return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8;
+  return false;
+}
+
+int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
+  if (a.M <= 0 || a.N <= 0 || a.K <= 0 || nz <= 0) return kOk;
+  const int64_t tiles64 = ceil_div(a.M, 64) * ceil_div(a.N, 64) * nz;
+  int st = kInvalidValue;
+  if (prec == kF16x2) {
+    // 64-deep k-tiles, two workgroups per CU: measured fastest on both parity stages at
+    // 1024^2 -> 512^2 (K = 512); the 8-wave shapes for small grids
+    const int cfg = a.tile_cfg == kTileAuto ? (tiles64 >= 128 ? kR64x64_w4 : k64x32_w8) : a.tile_cfg;
+    switch (cfg) {
+      case k64x64_w2x2: st = launch_stage_t<kF16x2, 64, 64, 128, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
+      case k64x32_w2x1: st = launch_stage_t<kF16x2, 64, 32, 128, 2, 1, 2, kRealPadded, 2>(a, nz, s); break;
+      case k64x64_w8: st = launch_stage_t<kF16x2, 64, 64, 128, 2, 2, 2, kRealPadded, 2>(a, nz, s); break;
+      case k64x32_w8: st = launch_stage_t<kF16x2, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(a, nz, s); break;
+      case k128x64: st = launch_stage_t<kF16x2, 128, 64, 64, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
+      case k64x128: st = launch_stage_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
+      case kR64x64_w4: st = launch_stage_t<kF16x2, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
+      case kR64x32_w4: st = launch_stage_t<kF16x2, 64, 32, 64, 2, 1, 2, kRealPadded, 2>(a, nz, s); break;
+      case kR32x64_w4: st = launch_stage_t<kF16x2, 32, 64, 64, 1, 2, 2, kRealPadded, 2>(a, nz, s); break;
+      default: set_last_error("tile configuration %d is not available to the parity form", cfg); return kInvalidValue;
+    }
+  } else if (prec == kBF16x3) {
+    const int cfg = a.tile_cfg == kTileAuto ? k64x32_w8 : a.tile_cfg;
+    switch (cfg) {
+      case k64x64_w2x2: st = launch_stage_t<kBF16x3, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
+      case k64x32_w2x1: st = launch_stage_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRealPadded, 2>(a, nz, s); break;
+      case k128x64: st = launch_stage_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
+      case k64x32_w8: st = launch_stage_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(a, nz, s); break;
+      default: set_last_error("tile configuration %d is not available to the parity form", cfg); return kInvalidValue;
+    }
+  } else {
+    set_last_error("the parity form runs the split precisions only");
+    return kInvalidValue;
+  }
+  if (st) return st;
+  return check_hip(hipGetLastError(), "mft_stage_kernel (parity) launch");
+}
+
+int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis& ax, const ParAxis& ay,
+                        int64_t batch, SplitOut o, int64_t ld_out, float* row_scale, hipStream_t s) {
+  HCIPY_REQUIRE(ld_out <= 128 * 16, "parity split of %d-sample half rows (at most 2048)", ax.h);
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const int64_t nb = std::min<int64_t>(65535, batch - b0);
+    const float2* ib = reinterpret_cast<const float2*>(in) + b0 * (int64_t)nx * ny;
+    const int64_t rows = b0 * 4 * 2 * (int64_t)ay.h;  // split rows of the fields before this chunk
+    SplitOut ob = o;
+    for (int i = 0; i < 3; ++i)
+      if (ob.p[i]) ob.p[i] = static_cast<char*>(ob.p[i]) + (size_t)rows * ld_out * 2;
+    const dim3 g((unsigned)ceil_div(ay.h, 2), (unsigned)nb);
+#define HCIPY_PSPLIT(FMT, XPT)                                                                                   \
+  hipLaunchKernelGGL((parity_split_kernel<FMT, XPT>), g, dim3(256), 0, s, ib, nx, ny, ax, ay, ob, ld_out, row_scale + rows)
+    if (prec == kF16x2) {
+      if (ld_out <= 512) HCIPY_PSPLIT(kSplitF16x2, 4);
+      else if (ld_out <= 1024) HCIPY_PSPLIT(kSplitF16x2, 8);
+      else HCIPY_PSPLIT(kSplitF16x2, 16);
+    } else {
+      if (ld_out <= 512) HCIPY_PSPLIT(kSplitBF16x3, 4);
+      else if (ld_out <= 1024) HCIPY_PSPLIT(kSplitBF16x3, 8);
+      else HCIPY_PSPLIT(kSplitBF16x3, 16);
+    }
+#undef HCIPY_PSPLIT
+  }
+  return check_hip(hipGetLastError(), "parity_split_kernel launch");
+}
+
+int launch_gen_parity(const double* c, int nc, const double* k, int nk, int prec, SplitOut out, int64_t lda,
+                      int rows_pad, hipStream_t s) {
+  const int64_t total = 2 * (int64_t)rows_pad * lda;
+  const int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 65535);
+  hipLaunchKernelGGL(gen_parity_kernel, dim3(blocks), dim3(256), 0, s, c, nc, k, nk, prec, out, lda, rows_pad);
+  return check_hip(hipGetLastError(), "gen_parity_kernel launch");
+}
+
+int launch_parity_combine(const double* H, const ParAxis& au, const ParAxis& av, int nu, int nv, double sign, double ar,
+                          double ai, int out_mode, void* out, int64_t batch, hipStream_t s) {
+  const int64_t n = batch * (int64_t)au.h * av.h;
+  if (n <= 0) return kOk;
+  HCIPY_REQUIRE(ceil_div(n, 256) <= INT32_MAX, "parity recombination of %lld points", (long long)n);
+  hipLaunchKernelGGL(parity_combine_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(H), au, av, nu, nv, sign, ar, ai, out_mode, out, batch);
+  return check_hip(hipGetLastError(), "parity_combine_kernel launch");
 }
 
 }  // namespace hcipy

@@ -50,6 +50,9 @@ enum OutMode : int {
   kOutI64 = 4,    // |z|^2 as float64
   kOutI32Base = 5,  // |z + base|^2 as float32 (base: complex128 array of the output's shape)
   kOutI64Base = 6,  // |z + base|^2 as float64
+  // Real-constant (parity) stages, one real value per accumulator:
+  kOutSplitT = 7,   // split arrays, element (m, n) at row 2m + (n & 1), column n >> 1 (re / im rows)
+  kOutR64 = 8,      // float64, element (m, n) at m * ldo + n
 };
 
 template <int P> struct PrecTraits;
@@ -134,6 +137,13 @@ struct StageArgs {
   int t_log2k;                  // ceil(log2(K)): bound |C| <= K / min(row_scale)
   double alpha_re, alpha_im;
   int tile_cfg;                 // kTileAuto or one of TileCfg
+  // Per batch slice z: the constant operand is plane (z >> a_zshift) & 1 of a two-plane array
+  // (a_zstride elements apart; 0: one plane), row scales and split-output scales of slice z start
+  // rs_zstride / t_zstride entries further on.
+  int64_t a_zstride;
+  int a_zshift;
+  int64_t rs_zstride;
+  int t_zstride;
 };
 
 // Workgroup tile shapes of the stage kernel (256 threads = 4 waves in every shape):
@@ -171,8 +181,34 @@ enum TileCfg : int {
   kD64x64_w8 = 13,
   kD64x32_w4 = 14,
   kD64x32_w8 = 15,
-  kNumTileCfg = 16,
+  // Parity form only (real constant), 64-deep k-tiles: 74 KB of LDS, two workgroups per CU
+  kR64x64_w4 = 16,  // 64 x 64 tile, 2 x 2 waves of 32 x 32
+  kR64x32_w4 = 17,  // 64 x 32 tile, 2 x 1 waves of 32 x 32, k-vectors split over 2
+  kR32x64_w4 = 18,  // 32 x 64 tile, 1 x 2 waves of 32 x 32, k-vectors split over 2
+  kNumTileCfg = 19,
 };
+
+// One axis of a grid in the parity form: h magnitudes P[j] >= 0; the sample at +P[j] is grid
+// index plus[j], the one at -P[j] is minus[j] (-1: no such sample, e.g. the origin's mirror or
+// the unpaired first sample of an FFT-style grid).  Device tables.
+struct ParAxis {
+  const int* plus;
+  const int* minus;
+  const double* P;
+  int h;
+  // Regular grids: plus[j] = p0 + j where that is < n, minus[j] = m0 - j for j >= mlo where that
+  // is >= 0 (affine != 0; the tables then need not be read).
+  int affine, p0, m0, mlo, n;
+};
+
+__device__ inline int par_plus(const ParAxis& a, int j) {
+  if (a.affine) return a.p0 + j < a.n ? a.p0 + j : -1;
+  return a.plus[j];
+}
+__device__ inline int par_minus(const ParAxis& a, int j) {
+  if (a.affine) return (j >= a.mlo && a.m0 - j >= 0) ? a.m0 - j : -1;
+  return a.minus[j];
+}
 
 // Up to three split arrays (device pointers), passed by value to the split and matrix kernels.
 struct SplitOut {

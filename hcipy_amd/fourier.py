@@ -195,6 +195,26 @@ class _MftPlan:
                    'hcipy_mft_plan_get_tiles')
         return v.value
 
+    @property
+    def parity_available(self):
+        """True for 2-D split-precision plans between centred grids (even sizes, axes symmetric
+        about 0): their transforms can run in the parity form (real half-size cosine and sine
+        products of the field's four parity components)."""
+        a = ctypes.c_int()
+        _lib.check(_lib.load().hcipy_mft_plan_get_parity(self.handle, ctypes.byref(a), None), 'hcipy_mft_plan_get_parity')
+        return bool(a.value)
+
+    @property
+    def parity(self):
+        """Whether single transforms run in the parity form (on by default where available)."""
+        e = ctypes.c_int()
+        _lib.check(_lib.load().hcipy_mft_plan_get_parity(self.handle, None, ctypes.byref(e)), 'hcipy_mft_plan_get_parity')
+        return bool(e.value)
+
+    @parity.setter
+    def parity(self, enable):
+        _lib.check(_lib.load().hcipy_mft_plan_set_parity(self.handle, 1 if enable else 0), 'hcipy_mft_plan_set_parity')
+
     def autotune(self, direction, x, out, batch, workspace, output=_lib.OUTPUT_FIELD, repeats=20):
         """Times every tile shape of each stage on these buffers and pins the fastest.
 

@@ -94,8 +94,28 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
 #define HCIPY_TILES_D64X64_W8 13 /* 32x64 compact tile, 8 waves */
 #define HCIPY_TILES_D64X32_W4 14 /* 32x32 compact tile, 4 waves */
 #define HCIPY_TILES_D64X32_W8 15 /* 32x32 compact tile, 8 waves */
+/* F16X2 parity form only (hcipy_mft_plan_set_parity), 64-deep k-tiles, two workgroups per CU: */
+#define HCIPY_TILES_R64X64_W4 16 /* 64x64 tile, 2x2 waves of 32x32 */
+#define HCIPY_TILES_R64X32_W4 17 /* 64x32 tile, 2x1 waves, k-vectors split over 2 */
+#define HCIPY_TILES_R32X64_W4 18 /* 32x64 tile, 1x2 waves, k-vectors split over 2 */
 int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction, int stage, int tile_cfg);
 int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction, int stage, int* tile_cfg);
+
+/* Parity form of a 2-D split-precision plan (HCIPY_PREC_F16X2 / BF16X3) whose four coordinate
+ * axes are each symmetric about 0 (every sample x paired with a sample at -x, to 1e-12 of
+ * max |x|, with at most one unpaired sample besides the origin: make_pupil_grid and
+ * make_focal_grid grids of any size without shifts).  The field is split into its four x/y
+ * parity components and each stage multiplies them by the real half-size cosine and sine
+ * matrices: the same sum as the complex form (matrix_fourier_transform.py:171-172, 219-220) with
+ * about a quarter of the multiply-adds; a final pass recombines the four quadrants.  Such plans
+ * start with the parity form on; single executions
+ * (hcipy_mft_execute / _ex / _noalloc, hcipy_mft_profile, hcipy_mft_autotune) use it, grouped
+ * executions and poke sets keep the complex form.  set_parity(plan, 0) selects the complex form
+ * (tile choices reset to HCIPY_TILES_AUTO on a switch; the parity form takes the shapes
+ * 64X64_W2X2, 64X32_W2X1, 64X64_W8, 64X32_W8, 128X64 and, F16X2 only, 64X128 and the R* shapes); enabling it on an
+ * ineligible plan is an error.  No counterpart in the reference. */
+int hcipy_mft_plan_set_parity(hcipy_mft_plan plan, int enable);
+int hcipy_mft_plan_get_parity(hcipy_mft_plan plan, int* available, int* enabled);
 
 /* Autotuner: for each stage of `direction`, times every tile shape available for the plan's
  * precision on the caller's buffers (`repeats` runs each, HIP events around the stage kernel),
@@ -179,8 +199,8 @@ int hcipy_mft_execute_ex(hcipy_mft_plan plan, int direction, const void* in, voi
  * device loading the library holds until the process ends.  No counterpart in the reference; it
  * is for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  Fails with HCIPY_ERROR_OUT_OF_RANGE, launching nothing, when
- * hcipy_mft_workspace_size(plan, batch) exceeds the scratch (e.g. 1024^2 -> 512^2 split-FP16 fits
- * batch 2).  All calls on one device share its scratch: calls that overlap on one device race, so
+ * the workspace this direction needs (at most hcipy_mft_workspace_size(plan, batch)) exceeds the
+ * scratch (e.g. 1024^2 -> 512^2 split-FP16 forward fits batch 2).  All calls on one device share its scratch: calls that overlap on one device race, so
  * order them on one stream.  The first call on a device looks the scratch up (no allocation). */
 int hcipy_mft_execute_noalloc(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch,
                               double alpha_re, double alpha_im, int output, void* stream);

@@ -33,8 +33,7 @@ def _run_pair(pupil, focal, precision, batch, output, seed):
         odt = torch.float64 if precision == 'fp64' else torch.float32
     out_ws = torch.empty((batch, focal.size), dtype=odt, device='cuda')
     out_na = torch.full((batch, focal.size), float('nan'), dtype=odt, device='cuda')
-    nbytes = plan.workspace_bytes(batch)
-    assert nbytes <= 32 * 1024 * 1024
+    nbytes = plan.workspace_bytes(batch)  # either direction; the noalloc call checks its own need
     # caller workspace full of NaN bytes: any read before write would show in the output
     ws = torch.full((nbytes,), 0xFF, dtype=torch.uint8, device='cuda')
     alpha = complex(0.7, -0.2)
