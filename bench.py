@@ -77,24 +77,29 @@ def main():
     wavelength = 1.0
 
     tiles = {}
+    forms = {}
     lyot_outs = {}
 
-    def setup(precision):
+    def setup(precision, parity=True):
         prop = ha.FraunhoferPropagator(pupil, focal, precision=precision)
         d = prop.instance(wavelength)
         mft = d.fourier_transform
         alpha = complex(pupil.weights) * d.norm_factor
         cdt = torch.complex64 if precision != 'fp64' else torch.complex128
         plan = mft.plan_for(cdt, dev)
+        if not parity and plan.parity_available:
+            plan.parity = False  # the complex form, for comparison
+        forms[(precision, parity)] = 'parity' if plan.parity_available and plan.parity else 'complex'
         if not args.no_autotune:
             tiles[precision] = mft.autotune(ha.Field(x, pupil), repeats=10)['forward']
         ws = mft._workspace_for(plan, 1, dev)
         return prop, plan, ws, alpha
 
-    def time_path(precision, steps, warmup, output):
+    def time_path(precision, steps, warmup, output, parity=True):
         """Seconds per propagation over `steps` steps of `chunk` propagations each (one graph
-        replay per step), after `warmup` untimed steps; HIP-event durations of the three kernels."""
-        prop, plan, ws, alpha = setup(precision)
+        replay per step), after `warmup` untimed steps; HIP-event durations of the three steps
+        (split, stage 1, stage 2 with the parity recombination)."""
+        prop, plan, ws, alpha = setup(precision, parity)
         odt = {_lib.OUTPUT_FIELD: torch.complex64, _lib.OUTPUT_INTENSITY: torch.float32,
                _lib.OUTPUT_FIELD_C128: torch.complex128}[output]
         out = torch.empty(focal.size, dtype=odt, device=dev)
@@ -432,16 +437,19 @@ def main():
     err_main = err(out_main)
     variants = {}
     outs_for_oracle = {'main': out_main}
-    for name, prec, output in [('%s_field_c64' % args.precision, args.precision, _lib.OUTPUT_FIELD),
-                               ('%s_field_c128' % args.precision, args.precision, _lib.OUTPUT_FIELD_C128),
-                               ('bf16x3_intensity', 'bf16x3', _lib.OUTPUT_INTENSITY),
-                               ('bf16x3_field_c128', 'bf16x3', _lib.OUTPUT_FIELD_C128),
-                               ('fp32_field_c64', 'fp32', _lib.OUTPUT_FIELD),
-                               ('fp32_intensity', 'fp32', _lib.OUTPUT_INTENSITY)]:
-        if args.no_variants or (prec == args.precision and output == main_out):
+    for name, prec, output, parity in [
+            ('%s_field_c64' % args.precision, args.precision, _lib.OUTPUT_FIELD, True),
+            ('%s_field_c128' % args.precision, args.precision, _lib.OUTPUT_FIELD_C128, True),
+            ('%s_intensity_complex_form' % args.precision, args.precision, _lib.OUTPUT_INTENSITY, False),
+            ('bf16x3_intensity', 'bf16x3', _lib.OUTPUT_INTENSITY, True),
+            ('bf16x3_field_c128', 'bf16x3', _lib.OUTPUT_FIELD_C128, True),
+            ('fp32_field_c64', 'fp32', _lib.OUTPUT_FIELD, True),
+            ('fp32_intensity', 'fp32', _lib.OUTPUT_INTENSITY, True)]:
+        if args.no_variants or (prec == args.precision and output == main_out and parity):
             continue
-        sec, kms, o = time_path(prec, max(20, args.steps // 4), 5, output)
+        sec, kms, o = time_path(prec, max(20, args.steps // 4), 5, output, parity)
         variants[name] = {'value': round(1.0 / sec, 2), 'ms_per_propagation': round(sec * 1e3, 5),
+                          'form': forms[(prec, parity)],
                           'peak_rel_intensity_error': err(o), 'kernel_ms': [round(v, 5) for v in kms]}
         outs_for_oracle[name] = o
     fp32_ref = variants.get('fp32_intensity') or variants.get('fp32_field_c64')
@@ -452,7 +460,8 @@ def main():
         paths['mixed_grouped'] = time_mixed_grouped()
         paths['poke_jacobian'] = time_poke_jacobian()
 
-    # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC).
+    # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC of the
+    # reference's product, the per-unit count the headline is credited with).
     ny, nx = pupil.shape
     nv, nu = focal.shape
     flops_stage1 = 8.0 * ny * nx * nu
@@ -461,6 +470,15 @@ def main():
     achieved = flops_stage1 / (ms1 * 1e-3) / 1e12
     peak_tf = PEAK_TFLOPS[args.precision]
     t0_cfg = tiles.get(args.precision, (-1, -1))[0]
+    main_form = forms.get((args.precision, True), 'complex')
+    if main_form == 'parity':
+        # the parity form's own stage 1: four real products of (u' halves) x (2 y' halves) x
+        # (x' halves), magnitudes counted as the plan pairs them (the FFT-style focal axis has one
+        # unpaired sample, so 257 u' values for 512 samples)
+        hu, hy, hx = nu // 2 + (1 if nu % 2 == 0 else 0), ny // 2, nx // 2
+        flops_exec1 = 2.0 * 4 * hu * (2 * hy) * hx
+    else:
+        flops_exec1 = flops_stage1
     # HBM bytes per stage-1 launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # (copy-calibrated), valid for the tile shape they were measured with
     traffic, traffic_src = None, None
@@ -468,13 +486,27 @@ def main():
     if args.precision == 'f16x2' and os.path.exists(tpath):
         with open(tpath) as fh:
             tj = json.load(fh)
-        cfg_used = t0_cfg if t0_cfg >= 0 else _lib.TILES_64X64_W8
-        if tj.get('tile_cfg_stage1') == cfg_used:
+        default_cfg = _lib.TILES_R64X64_W4 if main_form == 'parity' else _lib.TILES_64X64_W8
+        cfg_used = t0_cfg if t0_cfg >= 0 else default_cfg
+        if tj.get('tile_cfg_stage1') == cfg_used and tj.get('form', 'complex') == main_form:
             traffic = tj['fetch_bytes_per_launch'] + tj['write_bytes_per_launch']
             traffic_src = 'profiles/mft_stage1_hbm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)' 
-    kname = 'mft_f16x2_kernel' if t0_cfg >= _lib.TILES_C64X64_W4 else 'mft_stage_kernel'
-    roofline = {'bound': 'mfma', 'kernel': '%s (stage 1, E.M2)' % kname, 'achieved': round(achieved, 2),
+    if main_form == 'parity':
+        kname = 'mft_stage_kernel (parity stage 1, X_b . F_ab)'
+    elif _lib.TILES_C64X64_W4 <= t0_cfg < _lib.TILES_R64X64_W4:
+        kname = 'mft_f16x2_kernel (stage 1, E.M2)'
+    else:
+        kname = 'mft_stage_kernel (stage 1, E.M2)'
+    split_products = {'f16x2': 3, 'bf16x3': 6}.get(args.precision, 1)
+    roofline = {'bound': 'mfma', 'kernel': kname, 'form': main_form, 'achieved': round(achieved, 2),
                 'peak': peak_tf, 'unit': 'TFLOP/s', 'frac': round(achieved / peak_tf, 4), 'traffic': traffic,
+                # flops the launched kernel's own algorithm needs, and the MFMA flops it issues
+                # (split products included): the parity form needs about a quarter of the
+                # reference's count above
+                'form_flop_per_launch': flops_exec1,
+                'form_achieved': round(flops_exec1 / (ms1 * 1e-3) / 1e12, 2),
+                'form_frac': round(flops_exec1 / (ms1 * 1e-3) / 1e12 / peak_tf, 4),
+                'mfma_issued_tflops': round(split_products * flops_exec1 / (ms1 * 1e-3) / 1e12, 2),
                 'traffic_unit': 'bytes per launch', 'traffic_source': traffic_src,
                 'kernel_ms': {'split': round(kms_main[0], 5), 'stage1': round(kms_main[1], 5),
                               'stage2': round(kms_main[2], 5)},

@@ -1626,27 +1626,27 @@ __global__ void gen_matrix_kernel(const double* __restrict__ a, int nc, const do
 //   O(sv v', su u') = H_ee + i s su H_eo + i s sv H_oe - su sv H_oo      (s = -1 forward, +1 backward)
 // The eight real half-size products do about a quarter of the multiply-adds of the complex form.
 
-// Splits 4 FP64 values (already scaled into (-1, 1)) into the split arrays of format FMT at element
-// k of a row (8-byte stores): hi = fp16(v), lo = fp16((v - hi) * 2^11), or the three bfloat16
-// parts of the remainders, all taken from the FP64 value.
-template <int FMT>
-__device__ inline void split_store4d(const double* v, void* const* outs, int64_t row_off, int k) {
+// Splits NV (2 or 4) FP64 values (already scaled into (-1, 1)) into the split arrays of format FMT
+// at element k of a row (one vector store per array): hi = fp16(v), lo = fp16((v - hi) * 2^11), or
+// the three bfloat16 parts of the remainders, all taken from the FP64 value.
+template <int FMT, int NV>
+__device__ inline void split_store_d(const double* v, void* const* outs, int64_t row_off, int k) {
   if constexpr (FMT == kSplitF16x2) {
-    typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-    half4_t vh, vl;
+    typedef _Float16 hv_t __attribute__((ext_vector_type(NV)));
+    hv_t vh, vl;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < NV; ++e) {
       const _Float16 q = f32_to_f16((float)v[e]);
       vh[e] = q;
       vl[e] = f32_to_f16((float)((v[e] - (double)q) * (double)kLoInv));
     }
-    *reinterpret_cast<half4_t*>(static_cast<_Float16*>(outs[0]) + row_off + k) = vh;
-    *reinterpret_cast<half4_t*>(static_cast<_Float16*>(outs[1]) + row_off + k) = vl;
+    *reinterpret_cast<hv_t*>(static_cast<_Float16*>(outs[0]) + row_off + k) = vh;
+    *reinterpret_cast<hv_t*>(static_cast<_Float16*>(outs[1]) + row_off + k) = vl;
   } else {
-    typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
-    bf4_t b0, b1, b2;
+    typedef __bf16 bv_t __attribute__((ext_vector_type(NV)));
+    bv_t b0, b1, b2;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < NV; ++e) {
       const __bf16 q0 = f32_to_bf16((float)v[e]);
       const double r = v[e] - (double)(float)q0;
       const __bf16 q1 = f32_to_bf16((float)r);
@@ -1654,9 +1654,9 @@ __device__ inline void split_store4d(const double* v, void* const* outs, int64_t
       b1[e] = q1;
       b2[e] = f32_to_bf16((float)(r - (double)(float)q1));
     }
-    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[0]) + row_off + k) = b0;
-    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[1]) + row_off + k) = b1;
-    *reinterpret_cast<bf4_t*>(static_cast<__bf16*>(outs[2]) + row_off + k) = b2;
+    *reinterpret_cast<bv_t*>(static_cast<__bf16*>(outs[0]) + row_off + k) = b0;
+    *reinterpret_cast<bv_t*>(static_cast<__bf16*>(outs[1]) + row_off + k) = b1;
+    *reinterpret_cast<bv_t*>(static_cast<__bf16*>(outs[2]) + row_off + k) = b2;
   }
 }
 
@@ -1664,17 +1664,20 @@ __device__ inline void split_store4d(const double* v, void* const* outs, int64_t
 // 2 y' (real parts) and 2 y' + 1 (imaginary parts) over x' < ax.h, each multiplied by a
 // power-of-two scale (max |value| * scale < 1) and split; row padding up to ld_out is zero.  The
 // parity sums are formed in FP64 (exact for FP32 inputs) and split from FP64.
-// A workgroup of 256 threads takes two y' (128 threads each); a thread holds the four mirror
-// samples of XPT consecutive x' in registers (XPT * 128 >= ld_out), so the field is read once.
-template <int FMT, int XPT>
+// A workgroup of 256 threads takes 256 / TPR values of y' (TPR threads each); a thread holds the
+// four mirror samples of XPT consecutive x' in registers (XPT * TPR >= ld_out), so the field is
+// read once.
+template <int FMT, int XPT, int TPR>
 __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restrict__ in, int nx, int ny, ParAxis ax,
                                                            ParAxis ay, SplitOut o, int64_t ld_out,
                                                            float* __restrict__ row_scale) {
-  static_assert(XPT % 4 == 0, "x' per thread: whole 4-element stores");
-  __shared__ double red[2][2][8];
-  const int team = threadIdx.x >> 7;
-  const int tt = threadIdx.x & 127;
-  const int yp = blockIdx.x * 2 + team;
+  constexpr int SV = XPT % 4 == 0 ? 4 : 2;  // elements per store
+  static_assert(XPT % SV == 0 && (TPR == 64 || TPR == 128 || TPR == 256), "thread layout");
+  constexpr int TEAMS = 256 / TPR, WPT = TPR / 64;
+  __shared__ double red[TEAMS][WPT][8];
+  const int team = threadIdx.x / TPR;
+  const int tt = threadIdx.x % TPR;
+  const int yp = blockIdx.x * TEAMS + team;
   const int64_t b = blockIdx.y;
   const int hx = ax.h, hy = ay.h;
   const bool live = yp < hy;
@@ -1741,7 +1744,9 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
   double sc[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const double m = fmax(red[team][0][e], red[team][1][e]);
+    double m = red[team][0][e];
+#pragma unroll
+    for (int w = 1; w < WPT; ++w) m = fmax(m, red[team][w][e]);
     double s = 1.0;
     if (m > 0.0 && m <= DBL_MAX) s = ldexp(1.0, -(ilogb(m) + 1));  // m * s in [0.5, 1)
     sc[e] = s;
@@ -1755,19 +1760,19 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
   }
   if (x0 >= ld_out) return;
 #pragma unroll
-  for (int j0 = 0; j0 < XPT; j0 += 4) {
+  for (int j0 = 0; j0 < XPT; j0 += SV) {
     if (x0 + j0 >= ld_out) break;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      double w[4];
+      double w[SV];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < SV; ++j) {
         double v[8];
         combos(j0 + j, v);  // recomputed per row: a few adds, fewer live registers
         w[j] = v[e] * sc[e];
       }
       const int64_t row = ((b * 4 + (e >> 1)) * hy + yp) * 2 + (e & 1);
-      split_store4d<FMT>(w, o.p, row * ld_out, x0 + j0);
+      split_store_d<FMT, SV>(w, o.p, row * ld_out, x0 + j0);
     }
   }
 }
@@ -2179,18 +2184,21 @@ int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis
     SplitOut ob = o;
     for (int i = 0; i < 3; ++i)
       if (ob.p[i]) ob.p[i] = static_cast<char*>(ob.p[i]) + (size_t)rows * ld_out * 2;
-    const dim3 g((unsigned)ceil_div(ay.h, 2), (unsigned)nb);
-#define HCIPY_PSPLIT(FMT, XPT)                                                                                   \
-  hipLaunchKernelGGL((parity_split_kernel<FMT, XPT>), g, dim3(256), 0, s, ib, nx, ny, ax, ay, ob, ld_out, row_scale + rows)
+    // more threads per y' for more loads in flight: XPT * TPR covers the padded half row
+#define HCIPY_PSPLIT(FMT, XPT, TPR)                                                                          \
+  hipLaunchKernelGGL((parity_split_kernel<FMT, XPT, TPR>), dim3((unsigned)ceil_div(ay.h, 256 / TPR), (unsigned)nb), \
+                     dim3(256), 0, s, ib, nx, ny, ax, ay, ob, ld_out, row_scale + rows)
+#define HCIPY_PSPLIT_ALL(FMT)                             \
+  if (ld_out <= 256) HCIPY_PSPLIT(FMT, 2, 128);           \
+  else if (ld_out <= 512) HCIPY_PSPLIT(FMT, 2, 256);      \
+  else if (ld_out <= 1024) HCIPY_PSPLIT(FMT, 4, 256);     \
+  else HCIPY_PSPLIT(FMT, 8, 256);
     if (prec == kF16x2) {
-      if (ld_out <= 512) HCIPY_PSPLIT(kSplitF16x2, 4);
-      else if (ld_out <= 1024) HCIPY_PSPLIT(kSplitF16x2, 8);
-      else HCIPY_PSPLIT(kSplitF16x2, 16);
+      HCIPY_PSPLIT_ALL(kSplitF16x2)
     } else {
-      if (ld_out <= 512) HCIPY_PSPLIT(kSplitBF16x3, 4);
-      else if (ld_out <= 1024) HCIPY_PSPLIT(kSplitBF16x3, 8);
-      else HCIPY_PSPLIT(kSplitBF16x3, 16);
+      HCIPY_PSPLIT_ALL(kSplitBF16x3)
     }
+#undef HCIPY_PSPLIT_ALL
 #undef HCIPY_PSPLIT
   }
   return check_hip(hipGetLastError(), "parity_split_kernel launch");
