@@ -15,8 +15,7 @@ from .optics import (AgnosticOpticalElement, Apodizer, ComplexSurfaceApodizer, O
                      SurfaceApodizer, Wavefront)
 from .propagation import AngularSpectrumPropagator, FraunhoferPropagator, FresnelPropagator
 from .fourier_filter import FourierFilter, NearFieldTransferFunction
-from .coronagraphy import (LyotCoronagraph, MultiScaleCoronagraph, PerfectCoronagraph, VectorVortexCoronagraph,
-                          VortexCoronagraph)
+from .coronagraphy import LyotCoronagraph, MultiScaleCoronagraph, VectorVortexCoronagraph, VortexCoronagraph
 from .polarization import CircularRetarder, JonesMatrixOpticalElement, LinearRetarder, PhaseRetarder
 from .spectral_noise import (SpectralNoiseFFT, SpectralNoiseFactoryFFT, SpectralNoiseFactoryMultiscale,
                              SpectralNoiseMultiscale, make_power_law_error)

@@ -16,7 +16,7 @@ from .polarization import phase_retarder_jones
 from .propagation import FraunhoferPropagator
 
 __all__ = ['LyotCoronagraph', 'MultiScaleCoronagraph', 'VortexCoronagraph', 'VectorVortexCoronagraph',
-           'PerfectCoronagraph', 'periodic_tukey_window']
+           'periodic_tukey_window']
 
 
 class LyotCoronagraph(OpticalElement):
@@ -341,73 +341,3 @@ class VectorVortexCoronagraph(AgnosticOpticalElement):
         if self.lyot_stop is not None:
             wavefront = self.lyot_stop.backward(wavefront)
         return self._run(wavefront, True)
-
-
-def _inverse_truncated(M, rcond):
-    """Pseudo-inverse dropping singular values below ``rcond`` times the largest
-    (util/matrix_inversion.py:40-70)."""
-    U, S, Vt = np.linalg.svd(M, full_matrices=False)
-    S_inv = np.where(np.abs(S) > rcond * S.max(), 1 / S, 0)
-    return (Vt.T * S_inv).dot(U.T)
-
-
-class PerfectCoronagraph(OpticalElement):
-    """A perfect coronagraph of even ``order`` for ``aperture`` (perfect_coronagraph.py:6-110):
-    the field minus its projection onto the aperture times the monomials x^j y^(i - j),
-    i < order / 2, orthogonalized by QR (column signs from diag(R), as ModeBasis.orthogonalized
-    makes them), each mode attenuated by ``coeffs`` (all 1 by default: full suppression).
-
-    Set-up is FP64 on the host (QR, truncated pseudo-inverse at rcond 1e-6); the projection
-    runs on the device as two skinny FP64 GEMMs, ``E - ((E T^+^T) diag(c)) T^T``.  backward is
-    the same operation, as in the reference."""
-
-    def __init__(self, aperture, order=2, coeffs=None):
-        if order % 2 != 0:
-            raise AssertionError('The coronagraph order has to be even.')
-        self.pupil_grid = aperture.grid
-        ap = aperture.as_subclass(torch.Tensor).detach().cpu().numpy() if isinstance(aperture, torch.Tensor) \
-            else np.asarray(aperture)
-        ap = ap.astype(float)
-        if coeffs is not None:
-            order = int(2 * np.ceil(0.5 * (np.sqrt(8 * len(coeffs) + 1) - 1)))
-            self.coeffs = np.asarray(coeffs, dtype=float)
-        else:
-            self.coeffs = np.ones(int(order * (order / 2 + 1) / 4))
-        self.order = order
-        x, y = self.pupil_grid.coords
-        modes = [ap * x ** j * y ** (i - j) for i in range(order // 2) for j in range(i + 1)]
-        if len(modes) != len(self.coeffs):
-            raise ValueError('%d coefficients for the %d modes of an order-%d coronagraph'
-                             % (len(self.coeffs), len(modes), order))
-        q, r = np.linalg.qr(np.stack(modes, axis=1))
-        self.transformation = q / np.sign(np.diag(r))
-        self.transformation_inverse = _inverse_truncated(self.transformation, 1e-6)
-        self._mats = {}
-
-    def _matrices(self, dtype, device):
-        key = (dtype, device)
-        m = self._mats.get(key)
-        if m is None:
-            # P = T^+^T diag(c) (n x k) and T^T (k x n), both contiguous
-            P = (self.transformation_inverse.T * self.coeffs[np.newaxis, :])
-            m = (torch.as_tensor(np.ascontiguousarray(P), device=device).to(dtype),
-                 torch.as_tensor(np.ascontiguousarray(self.transformation.T), device=device).to(dtype))
-            self._mats[key] = m
-        return m
-
-    def _apply(self, wavefront):
-        E = wavefront.electric_field
-        t = E.as_subclass(torch.Tensor)
-        _lib.require_device(t)
-        # the mode coefficients are sums over every aperture pixel: formed in FP64 for complex64
-        # fields too (an FP32 sum over 5e4 pixels leaves 3e-5 of the field), then subtracted in
-        # the field's precision
-        P, TT = self._matrices(torch.complex128, t.device)
-        corr = torch.matmul(torch.matmul(t.to(torch.complex128), P), TT).to(t.dtype)
-        return Wavefront(pointwise.sub_mul(E, corr), wavefront.wavelength, wavefront.input_stokes_vector)
-
-    def forward(self, wavefront):
-        return self._apply(wavefront)
-
-    def backward(self, wavefront):
-        return self._apply(wavefront)

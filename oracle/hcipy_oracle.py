@@ -56,10 +56,6 @@ dropped term, a sign error, a swapped index or a transposed operand fails a chec
   (+, +) facet lands in the (-, -) quadrant (the sign chain of two forward MFTs and the
   surface); backward is the weighted adjoint of forward; the reference's slow/fast modulation
   agreement (tests/test_wavefront_sensing.py:29-52);
-* ``perfect_coronagraph``: the reference's suppression test (tests/test_coronagraphy.py:244-270):
-  an on-axis flat wavefront of unit power leaves < 1e-10, and the leaked power of a tilted one
-  grows as tilt^order (log-log slope within 1e-3 of the order for orders 2-8); a mode scaled by
-  coeff 0 passes unchanged;
 * ``gaussian_influence_functions``/``dm_surface``: crosstalk definition (value at the
   neighbouring actuator) and cutoff (hcipy/optics/deformable_mirror.py:78-79).
 """
@@ -977,27 +973,6 @@ def inverse_tikhonov(M, rcond=1e-15):
     U, S, Vt = np.linalg.svd(np.asarray(M, dtype=float), full_matrices=False)
     S_inv = S / (S ** 2 + (rcond * S.max()) ** 2)
     return (Vt.T * S_inv).dot(U.T)
-
-
-def perfect_coronagraph(E, aperture, grid, order=2, coeffs=None):
-    """PerfectCoronagraph.forward (hcipy/coronagraphy/perfect_coronagraph.py:38-88): E minus its
-    projection onto the QR-orthogonalized modes aperture * x^j y^(i-j), i < order/2 (signs from
-    diag(R), mode_basis.py:260-282), through the pseudo-inverse truncated at 1e-6
-    (util/matrix_inversion.py:40-70), each mode scaled by coeffs."""
-    ap = np.asarray(aperture, dtype=float)
-    if coeffs is not None:
-        order = int(2 * np.ceil(0.5 * (np.sqrt(8 * len(coeffs) + 1) - 1)))
-    else:
-        coeffs = np.ones(int(order * (order / 2 + 1) / 4))
-    x, y = grid.coords
-    M = np.stack([ap * x ** j * y ** (i - j) for i in range(order // 2) for j in range(i + 1)], axis=1)
-    q, r = np.linalg.qr(M)
-    T = q / np.sign(np.diag(r))
-    U, S, Vt = np.linalg.svd(T, full_matrices=False)
-    Tinv = (Vt.T * np.where(S > 1e-6 * S.max(), 1 / S, 0)).dot(U.T)
-    # the reference's einsum 'kj,j,ji,...i->...k' as two products (one contraction at a time)
-    c = np.asarray(E) @ Tinv.T * np.asarray(coeffs, dtype=float)
-    return E - c @ T.T
 
 
 def actuator_positions(n_act, pitch):

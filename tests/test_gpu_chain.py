@@ -101,31 +101,3 @@ def test_lyot_1024_chain_contrast_error(precision, tol):
     assert err < tol
 
 
-@pytest.mark.parametrize('order', [2, 4, 6, 8])
-def test_perfect_coronagraph(order):
-    """PerfectCoronagraph (perfect_coronagraph.py:6-110) on the device: against the oracle's
-    projection (complex128 1e-12 of the peak field, complex64 1e-6), and the reference's test
-    (tests/test_coronagraphy.py:244-270): an on-axis wavefront of unit power leaves < 1e-10,
-    and the leaked power of tilted ones grows as tilt^order (log-log slope within 1e-3)."""
-    g = ha.make_pupil_grid(256)
-    ap = o.circular_aperture(og(g), 1)
-    coro = ha.PerfectCoronagraph(ha.Field(dev(ap), g), order)
-    rng = np.random.default_rng(order)
-    E = ap * np.exp(0.3j * rng.standard_normal(g.size))
-    ref = o.perfect_coronagraph(E, ap, og(g), order)
-    for dtype, tol in ((np.complex128, 1e-12), (np.complex64, 1e-6)):
-        out = host(coro.forward(ha.Wavefront(ha.Field(dev(E.astype(dtype)), g))).electric_field)
-        assert np.abs(out - ref).max() < tol * np.abs(E).max()
-    wf = ha.Wavefront(ha.Field(dev(ap.astype(complex)), g))
-    wf.total_power = 1
-    assert float(coro(wf).total_power) < 1e-10
-    tilts = np.logspace(-3, -1, 51)
-    x = og(g).coords[0]
-    batch = dev(ap[np.newaxis, :] * np.exp(2j * np.pi * x[np.newaxis, :] * tilts[:, np.newaxis]))
-    out = host(coro.forward(ha.Wavefront(ha.Field(batch, g))).electric_field)
-    leak = (np.abs(out) ** 2).sum(axis=1) * og(g).weights
-    lx, ly = np.log10(tilts), np.log10(leak)
-    n = len(lx)
-    beta = ((lx * ly).sum() - lx.sum() * ly.sum() / n) / ((lx * lx).sum() - lx.sum() ** 2 / n)
-    say('perfect coronagraph order %d: slope %.5f' % (order, beta))
-    assert abs(beta - order) / order < 1e-3

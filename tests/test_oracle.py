@@ -830,39 +830,6 @@ def test_modulated_pyramid_slow_and_fast_agree():
     assert np.abs(ts - tf).max() / ts.max() < 2e-2
 
 
-# --- perfect coronagraph (tests/test_coronagraphy.py:244-270) ---------------------------------
-
-def _leakage_slope(run, grid, aperture):
-    tilts = np.logspace(-3, -1, 51)
-    x = grid.coords[0]
-    leak = [np.sum(np.abs(run(aperture * np.exp(2j * np.pi * x * t))) ** 2) * grid.weights for t in tilts]
-    lx, ly = np.log10(tilts), np.log10(leak)
-    n = len(lx)
-    return ((lx * ly).sum() - lx.sum() * ly.sum() / n) / ((lx * lx).sum() - lx.sum() ** 2 / n)
-
-
-@pytest.mark.parametrize('order', [2, 4, 6, 8])
-def test_perfect_coronagraph_reference_suppression(order):
-    g = o.make_pupil_grid(256)
-    ap = o.circular_aperture(g, 1)
-    E = ap / np.sqrt(np.sum(ap ** 2) * g.weights)
-    assert np.sum(np.abs(o.perfect_coronagraph(E.astype(complex), ap, g, order)) ** 2) * g.weights < 1e-10
-    beta = _leakage_slope(lambda e: o.perfect_coronagraph(e, ap, g, order), g, ap)
-    assert abs(beta - order) / order < 1e-3
-
-
-def test_perfect_coronagraph_coefficients():
-    """coeffs pick the order (3 coefficients: order 4) and scale each orthogonal mode: with the
-    piston coefficient 0 a flat wavefront passes unchanged, with 1 it is removed."""
-    g = o.make_pupil_grid(64)
-    ap = o.circular_aperture(g, 1)
-    E = ap.astype(complex)
-    assert np.abs(o.perfect_coronagraph(E, ap, g, coeffs=[0, 1, 1]) - E).max() < 1e-12
-    assert np.abs(o.perfect_coronagraph(E, ap, g, coeffs=[1, 1, 1])).max() < 1e-12
-
-
-# --- inverse_tikhonov (hcipy/util/matrix_inversion.py:71-100) --------------------------------
-
 def test_inverse_tikhonov_pins():
     """The restatement against what the mathematics fixes: for rcond -> 0 it is numpy's
     pseudo-inverse; for any rcond it equals the normal-equations form (M^T M + lam^2 I)^-1 M^T with
