@@ -51,7 +51,8 @@ int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double s
 bool real_cfg_supported(int prec, int cfg);
 int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s);
 int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis& ax, const ParAxis& ay,
-                        int64_t batch, SplitOut o, int64_t ld_out, float* row_scale, hipStream_t s);
+                        int64_t batch, SplitOut o, int64_t ld_out, float* row_scale, const double* nyq_coef,
+                        double* nyq_out, hipStream_t s);
 int launch_gen_parity(const double* c, int nc, const double* k, int nk, int prec, SplitOut out, int64_t lda,
                       int rows_pad, hipStream_t s);
 int launch_parity_combine(const double* H, const ParAxis& au, const ParAxis& av, int nu, int nv, double sign, double ar,
@@ -108,6 +109,12 @@ struct Plan {
   PMatrix pf1, pf2, pb1, pb2;  // forward stage 1/2, backward stage 1/2
   ParAxis px{}, py{}, pu{}, pv{};  // axis tables (device: plus, minus, P)
   void* ptables = nullptr;
+  // Per direction: the last magnitude of the stage-1 output axis has one sample only (the -n/2
+  // sample of an FFT-style axis) and dropping its row saves a row tile.  The parity split then
+  // forms that row of the stage-1 product in FP64 from nyq_coef (cosines, then sines, of that
+  // magnitude times the contraction magnitudes) and stage 1 runs one row fewer.
+  bool nyq[2] = {false, false};
+  double* nyq_coef[2] = {nullptr, nullptr};
 };
 
 int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, int nk, double sign,
@@ -219,6 +226,14 @@ bool parity_axis(const double* x, int64_t n, HostAxis& a) {
   return (int64_t)a.P.size() <= n / 2 + 1;
 }
 
+// Whether the stage-1 rows of output axis `a` end in a magnitude with one sample only whose row
+// alone fills a row tile of 32 (the Nyquist sample -n/2 d of an FFT-style axis of even n).
+bool nyq_row_worth(const HostAxis& a) {
+  const size_t h = a.P.size();
+  if (h < 33 || (h - 1) % 32 != 0) return false;
+  return (a.plus.back() < 0) != (a.minus.back() < 0) && a.P.back() > 0.0;
+}
+
 // Geometry of one direction of a 2-D (or 1-D) transform.
 struct Dir {
   const FMatrix* m1;  // stage 1 constant (rows c1, contraction k1)
@@ -285,7 +300,7 @@ Ws workspace_layout(const Plan& pl, const Dir& d, int64_t batch) {
 // (u', re / im), y' contiguous) and the FP64 products H (rows v', complex u').  H is written by
 // stage 2, after stage 1 has read the last of F: the two share one region.
 struct ParWs {
-  size_t f[3] = {0, 0, 0}, rs = 0, t = 0, g[3] = {0, 0, 0}, h = 0, total = 0;
+  size_t f[3] = {0, 0, 0}, rs = 0, t = 0, g[3] = {0, 0, 0}, h = 0, nyq = 0, total = 0;
   int64_t ldf = 0, ldg = 0;
 };
 
@@ -311,6 +326,7 @@ ParWs parity_layout(const Plan& pl, const Dir& d, int64_t batch) {
   w.t = take((size_t)batch * 4 * 4);
   w.ldg = padded_row(hn);
   for (int i = 0; i < na; ++i) w.g[i] = take((size_t)batch * 4 * 2 * hc1 * w.ldg * 2);
+  if (pl.nyq[fw ? 0 : 1]) w.nyq = take((size_t)batch * 4 * 2 * hn * sizeof(double));
   w.total = off;
   return w;
 }
@@ -375,11 +391,13 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
   float* rs = reinterpret_cast<float*>(ws + w.rs);
   float* t = reinterpret_cast<float*>(ws + w.t);
   double* H = reinterpret_cast<double*>(ws + w.h);
+  const bool nyq = pl.nyq[fw ? 0 : 1];
+  double* nyq_vals = nyq ? reinterpret_cast<double*>(ws + w.nyq) : nullptr;
   int st = mark(0);
   if (!st && (only < 0 || only == 0)) {
     SplitOut fo{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
     st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch, fo, w.ldf, rs,
-                             s);
+                             nyq ? pl.nyq_coef[fw ? 0 : 1] : nullptr, nyq_vals, s);
   }
   if (!st) st = mark(1);
   if (st) return st;
@@ -400,7 +418,12 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
     a.b_lo2 = na > 2 ? ws + w.f[2] + (size_t)b0 * 4 * fz * 2 : nullptr;
     a.ldb = w.ldf;
     a.b_zstride = fz;
-    a.M = (int)hc1;
+    a.M = (int)(nyq ? hc1 - 1 : hc1);
+    if (nyq) {
+      a.nyq = nyq_vals + b0 * 4 * 2 * hn;
+      a.nyq_zstride = 2 * hn;
+      a.nyq_row = (int)(hc1 - 1);
+    }
     a.N = (int)(2 * hn);
     a.K = (int)hk;
     a.row_scale = rs + b0 * 4 * 2 * hn;
@@ -964,6 +987,21 @@ int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const d
     if (!st) st = make_parity(pl, pl.pf2, pl.pv.P, pl.pv.h, pl.py.P, pl.py.h, s);
     if (!st) st = make_parity(pl, pl.pb1, pl.px.P, pl.px.h, pl.pu.P, pl.pu.h, s);
     if (!st) st = make_parity(pl, pl.pb2, pl.py.P, pl.py.h, pl.pv.P, pl.pv.h, s);
+    // Nyquist rows of stage 1: forward output u (contraction x), backward output x (contraction u)
+    const HostAxis* c1[2] = {&hu, &hx};
+    const ParAxis* pc1[2] = {&pl.pu, &pl.px};
+    const ParAxis* pk[2] = {&pl.px, &pl.pu};
+    for (int dir = 0; dir < 2 && !st; ++dir) {
+      if (!nyq_row_worth(*c1[dir])) continue;
+      const int hk = pk[dir]->h;
+      st = check_hip(hipMalloc(&pl.nyq_coef[dir], 2 * (size_t)hk * sizeof(double)), "hipMalloc(nyquist row)");
+      if (!st) {
+        pl.matrix_bytes += 2 * (size_t)hk * sizeof(double);
+        st = launch_gen_parity(pc1[dir]->P + (pc1[dir]->h - 1), 1, pk[dir]->P, hk, kF64,
+                               SplitOut{{pl.nyq_coef[dir], nullptr, nullptr}}, hk, 1, s);
+      }
+      if (!st) pl.nyq[dir] = true;
+    }
     pl.parity = true;
   }
   if (!st) st = check_hip(hipStreamSynchronize(s), "plan creation");
@@ -984,6 +1022,8 @@ int hcipy_mft_plan_destroy(hcipy_mft_plan plan) {
   free_parity(pl.pb1);
   free_parity(pl.pb2);
   if (pl.ptables) (void)hipFree(pl.ptables);
+  for (double* c : pl.nyq_coef)
+    if (c) (void)hipFree(c);
   if (pl.coords) (void)hipFree(pl.coords);
   delete plan;
   return kOk;

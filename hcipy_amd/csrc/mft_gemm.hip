@@ -520,6 +520,31 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   }
   __syncthreads();
 
+  // The extra output row (real split output only): stored by the first row tile with the same
+  // scales and split as the product rows, while the k loop of this workgroup is still ahead.
+  if constexpr (REAL && TR::SPLIT) {
+    if (p.nyq != nullptr && m0 == 0 && p.out_mode == kOutSplitT) {
+      const double zs = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[z] : 1.0;
+      const int nend = min(n0 + BN, N);
+      for (int n = n0 + tid; n < nend; n += NTHR) {
+        const double v = p.nyq[(int64_t)z * p.nyq_zstride + n] * zs * (double)s_t[n / p.fold - item0];
+        const int64_t idx = (int64_t)z * p.out_zstride + (int64_t)(2 * p.nyq_row + (n & 1)) * p.ldo + (n >> 1);
+        if constexpr (P == kBF16x3) {
+          const __bf16 q0 = f32_to_bf16((float)v);
+          const double r1 = v - (double)(float)q0;
+          const __bf16 q1 = f32_to_bf16((float)r1);
+          static_cast<__bf16*>(p.out)[idx] = q0;
+          static_cast<__bf16*>(p.out_lo)[idx] = q1;
+          static_cast<__bf16*>(p.out_lo2)[idx] = f32_to_bf16((float)(r1 - (double)(float)q1));
+        } else {
+          const _Float16 hq = f32_to_f16((float)v);
+          static_cast<_Float16*>(p.out)[idx] = hq;
+          static_cast<_Float16*>(p.out_lo)[idx] = f32_to_f16((float)((v - (double)hq) * (double)kLoInv));
+        }
+      }
+    }
+  }
+
   // Main loop: every tile it stores (up to kb + PF) is a full k-tile.
   int kb = 0;
   for (; kb + PF < nk_full; kb += PF) {
@@ -1626,6 +1651,34 @@ __global__ void gen_matrix_kernel(const double* __restrict__ a, int nc, const do
 //   O(sv v', su u') = H_ee + i s su H_eo + i s sv H_oe - su sv H_oo      (s = -1 forward, +1 backward)
 // The eight real half-size products do about a quarter of the multiply-adds of the complex form.
 
+// Sum (MAX false) or maximum (MAX true) of eight values per lane over the 64 lanes of a wave, by
+// halving exchanges: lanes keep half of their values at each of the offsets 32, 16, 8 and send the
+// other half, then the one value left is reduced over offsets 4, 2, 1 (ten exchanges instead of
+// 48).  Afterwards every lane l holds the result for value index l >> 3.
+template <bool MAX>
+__device__ inline double wave_reduce8(const double (&v)[8], int lane) {
+  auto op = [](double a, double b) __attribute__((always_inline)) { return MAX ? fmax(a, b) : a + b; };
+  const bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0;
+  double k4[4], k2[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = b5 ? v[4 + i] : v[i];
+    const double send = b5 ? v[i] : v[4 + i];
+    k4[i] = op(keep, __shfl_xor(send, 32));
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = b4 ? k4[2 + i] : k4[i];
+    const double send = b4 ? k4[i] : k4[2 + i];
+    k2[i] = op(keep, __shfl_xor(send, 16));
+  }
+  double k1 = op(b3 ? k2[1] : k2[0], __shfl_xor(b3 ? k2[0] : k2[1], 8));
+  k1 = op(k1, __shfl_xor(k1, 4));
+  k1 = op(k1, __shfl_xor(k1, 2));
+  k1 = op(k1, __shfl_xor(k1, 1));
+  return k1;
+}
+
 // Splits NV (2 or 4) FP64 values (already scaled into (-1, 1)) into the split arrays of format FMT
 // at element k of a row (one vector store per array): hi = fp16(v), lo = fp16((v - hi) * 2^11), or
 // the three bfloat16 parts of the remainders, all taken from the FP64 value.
@@ -1664,17 +1717,23 @@ __device__ inline void split_store_d(const double* v, void* const* outs, int64_t
 // 2 y' (real parts) and 2 y' + 1 (imaginary parts) over x' < ax.h, each multiplied by a
 // power-of-two scale (max |value| * scale < 1) and split; row padding up to ld_out is zero.  The
 // parity sums are formed in FP64 (exact for FP32 inputs) and split from FP64.
+// With nyq_out set, the split also forms one row of the stage-1 product in FP64: for every row
+// (z, 2 y' + c), sum_x' X_b[x'] F_ab[y'][x'] with X_e = nyq_coef[x'] (cosines) and X_o =
+// nyq_coef[ax.h + x'] (sines), written unscaled to nyq_out[z * 2 ay.h + 2 y' + c].
 // A workgroup of 256 threads takes 256 / TPR values of y' (TPR threads each); a thread holds the
 // four mirror samples of XPT consecutive x' in registers (XPT * TPR >= ld_out), so the field is
 // read once.
 template <int FMT, int XPT, int TPR>
 __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restrict__ in, int nx, int ny, ParAxis ax,
                                                            ParAxis ay, SplitOut o, int64_t ld_out,
-                                                           float* __restrict__ row_scale) {
+                                                           float* __restrict__ row_scale,
+                                                           const double* __restrict__ nyq_coef,
+                                                           double* __restrict__ nyq_out) {
   constexpr int SV = XPT % 4 == 0 ? 4 : 2;  // elements per store
   static_assert(XPT % SV == 0 && (TPR == 64 || TPR == 128 || TPR == 256), "thread layout");
   constexpr int TEAMS = 256 / TPR, WPT = TPR / 64;
   __shared__ double red[TEAMS][WPT][8];
+  __shared__ double red_n[TEAMS][WPT][8];
   const int team = threadIdx.x / TPR;
   const int tt = threadIdx.x % TPR;
   const int yp = blockIdx.x * TEAMS + team;
@@ -1704,6 +1763,15 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
     R[j] = rm[cp];
     W[j] = rm[cm];
   }
+  // coefficients of the extra stage-1 row (issued with the data loads)
+  const bool nyq = nyq_out != nullptr;
+  double ncs[XPT], nsn[XPT];
+#pragma unroll
+  for (int j = 0; j < XPT; ++j) {
+    const int xj = min(x0 + j, hx - 1);
+    ncs[j] = nyq ? nyq_coef[xj] : 0.0;
+    nsn[j] = nyq ? nyq_coef[hx + xj] : 0.0;
+  }
 #pragma unroll
   for (int j = 0; j < XPT; ++j) {
     const bool vp = cps[j] >= 0, vm = cms[j] >= 0;
@@ -1732,15 +1800,30 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
     for (int e = 0; e < 8; ++e) mx[e] = fmax(mx[e], fabs(v[e]));
   }
   const int wv = tt >> 6;
+  const int lane = tt & 63;
+  {
+    const double m = wave_reduce8<true>(mx, lane);
+    if ((lane & 7) == 0) red[team][wv][lane >> 3] = m;
+  }
+  if (nyq) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    double m = mx[e];
+    for (int j = 0; j < XPT; ++j) {
+      double v[8];
+      combos(j, v);
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((tt & 63) == 0) red[team][wv][e] = m;
+      for (int e = 0; e < 8; ++e) acc[e] += v[e] * (((e >> 1) & 1) ? nsn[j] : ncs[j]);  // x parity: cos / sin
+    }
+    const double a = wave_reduce8<false>(acc, lane);
+    if ((lane & 7) == 0) red_n[team][wv][lane >> 3] = a;
   }
   __syncthreads();
   if (!live) return;
+  if (nyq && tt < 8) {
+    double a = red_n[team][0][tt];
+    for (int w = 1; w < WPT; ++w) a += red_n[team][w][tt];
+    nyq_out[(b * 4 + (tt >> 1)) * 2 * (int64_t)hy + 2 * yp + (tt & 1)] = a;
+  }
   double sc[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
@@ -2175,7 +2258,8 @@ int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
 }
 
 int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis& ax, const ParAxis& ay,
-                        int64_t batch, SplitOut o, int64_t ld_out, float* row_scale, hipStream_t s) {
+                        int64_t batch, SplitOut o, int64_t ld_out, float* row_scale, const double* nyq_coef,
+                        double* nyq_out, hipStream_t s) {
   HCIPY_REQUIRE(ld_out <= 128 * 16, "parity split of %d-sample half rows (at most 2048)", ax.h);
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     const int64_t nb = std::min<int64_t>(65535, batch - b0);
@@ -2187,7 +2271,8 @@ int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis
     // more threads per y' for more loads in flight: XPT * TPR covers the padded half row
 #define HCIPY_PSPLIT(FMT, XPT, TPR)                                                                          \
   hipLaunchKernelGGL((parity_split_kernel<FMT, XPT, TPR>), dim3((unsigned)ceil_div(ay.h, 256 / TPR), (unsigned)nb), \
-                     dim3(256), 0, s, ib, nx, ny, ax, ay, ob, ld_out, row_scale + rows)
+                     dim3(256), 0, s, ib, nx, ny, ax, ay, ob, ld_out, row_scale + rows, nyq_coef,            \
+                     nyq_out != nullptr ? nyq_out + b0 * 4 * 2 * (int64_t)ay.h : nullptr)
 #define HCIPY_PSPLIT_ALL(FMT)                             \
   if (ld_out <= 256) HCIPY_PSPLIT(FMT, 2, 128);           \
   else if (ld_out <= 512) HCIPY_PSPLIT(FMT, 2, 256);      \

@@ -144,6 +144,12 @@ struct StageArgs {
   int a_zshift;
   int64_t rs_zstride;
   int t_zstride;
+  // Real split output (kOutSplitT) with one more output row than the product has: when nyq is
+  // set, the workgroups of the first row tile also store row nyq_row from the FP64 values
+  // nyq[z * nyq_zstride + n] (unscaled results, formed by the kernel that split the data).
+  const double* nyq;
+  int64_t nyq_zstride;
+  int nyq_row;
 };
 
 // Workgroup tile shapes of the stage kernel (256 threads = 4 waves in every shape):

@@ -62,8 +62,24 @@ def test_parity_matches_oracle_random_fields(precision, dims_in, dims_out):
     """Centred rectangular grids (odd sizes hold the origin, which has no separate mirror),
     batch 3, forward and backward, field outputs, against the oracle and against the complex
     form of the same plan."""
-    gi = ha.make_uniform_grid(dims_in, [1.0, 0.8])
-    go = ha.make_uniform_grid(dims_out, [40.0, 30.0])
+    _check_against_oracle(ha.make_uniform_grid(dims_in, [1.0, 0.8]), ha.make_uniform_grid(dims_out, [40.0, 30.0]),
+                          precision, '%s->%s' % (dims_in, dims_out))
+
+
+@pytest.mark.parametrize('precision', ['f16x2', 'bf16x3'])
+@pytest.mark.parametrize('dims_in,dims_out', [([128, 96], [64, 90]), ([1024, 1024], [512, 512])])
+def test_parity_fft_style_grids_with_nyquist_rows(precision, dims_in, dims_out):
+    """FFT-style grids (a sample at the origin and an unpaired -n/2 sample) whose unpaired
+    magnitude would take a stage-1 row tile of its own (x axes of 128 and 1024 samples, u axes
+    of 64 and 512): the parity split forms that row of the stage-1 product in FP64 instead, in
+    both directions; the small case runs every parity tile shape."""
+    gi = ha.make_uniform_grid(dims_in, [1.0, 0.8], has_center=True)
+    go = ha.make_uniform_grid(dims_out, [40.0, 30.0], has_center=True)
+    _check_against_oracle(gi, go, precision, 'FFT-style %s->%s' % (dims_in, dims_out),
+                          all_tiles=dims_in[0] == 128)
+
+
+def _check_against_oracle(gi, go, precision, label, all_tiles=False):
     f = _rand(gi.size, 3, (3,)).astype(np.complex64)
     F = _rand(go.size, 4, (3,)).astype(np.complex64)
     ref_f = o.mft_forward(f.astype(np.complex128), og(gi), og(go))
@@ -78,10 +94,20 @@ def test_parity_matches_oracle_random_fields(precision, dims_in, dims_out):
         bb = host(mft.backward(ha.Field(dev(F), go)))
         out[parity] = (a, bb)
         ea, eb = peak_rel_field(a, ref_f), peak_rel_field(bb, ref_b)
-        say('%s %s->%s parity=%s: forward %.3e backward %.3e' % (precision, dims_in, dims_out, parity, ea, eb))
+        say('%s %s parity=%s: forward %.3e backward %.3e' % (precision, label, parity, ea, eb))
         assert ea < 5e-7 and eb < 5e-7
     assert peak_rel_field(out[True][0], out[False][0]) < 5e-7
     assert peak_rel_field(out[True][1], out[False][1]) < 5e-7
+    if all_tiles:
+        plan.parity = True
+        for cfg in _lib.PARITY_TILE_CFGS[precision]:
+            for d in (_lib.FORWARD, _lib.BACKWARD):
+                for stage in (0, 1):
+                    plan.set_tiles(d, stage, cfg)
+            ea = peak_rel_field(host(mft.forward(ha.Field(dev(f), gi))), ref_f)
+            eb = peak_rel_field(host(mft.backward(ha.Field(dev(F), go))), ref_b)
+            say('  tile shape %d: forward %.3e backward %.3e' % (cfg, ea, eb))
+            assert ea < 5e-7 and eb < 5e-7, cfg
 
 
 @pytest.mark.parametrize('precision', ['f16x2', 'bf16x3'])
