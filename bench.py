@@ -91,7 +91,7 @@ def main():
             plan.parity = False  # the complex form, for comparison
         forms[(precision, parity)] = 'parity' if plan.parity_available and plan.parity else 'complex'
         if not args.no_autotune:
-            tiles[precision] = mft.autotune(ha.Field(x, pupil), repeats=10)['forward']
+            tiles[(precision, forms[(precision, parity)])] = mft.autotune(ha.Field(x, pupil), repeats=10)['forward']
         ws = mft._workspace_for(plan, 1, dev)
         return prop, plan, ws, alpha
 
@@ -134,12 +134,20 @@ def main():
             step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        # per-kernel durations from HIP events on the launching stream
+        # per-kernel durations from HIP events on the launching stream: each step inside whole
+        # propagations (events between the launches of one propagation, 200 propagations), and
+        # each kernel alone over 200 back-to-back launches
         kms = (ctypes.c_float * 3)()
+        st = _lib.load().hcipy_mft_profile_pipeline(plan.handle, _lib.FORWARD, _lib.dptr(x), _lib.dptr(out), 1,
+                                                    float(alpha.real), float(alpha.imag), output, _lib.dptr(ws),
+                                                    ws.numel(), 200, kms, _lib.stream_ptr(dev))
+        _lib.check(st, 'hcipy_mft_profile_pipeline')
+        kiso = (ctypes.c_float * 3)()
         st = _lib.load().hcipy_mft_profile(plan.handle, _lib.FORWARD, _lib.dptr(x), _lib.dptr(out), 1,
                                            float(alpha.real), float(alpha.imag), output, _lib.dptr(ws), ws.numel(),
-                                           200, kms, _lib.stream_ptr(dev))
+                                           200, kiso, _lib.stream_ptr(dev))
         _lib.check(st, 'hcipy_mft_profile')
+        time_path.isolated[(precision, output, parity)] = [kiso[0], kiso[1], kiso[2]]
         return (t1 - t0) / (steps * chunk), [kms[0], kms[1], kms[2]], out.clone()
 
     # Reference intensity: this library's FP64 MFMA path on the same complex64-rounded input.
@@ -433,7 +441,9 @@ def main():
 
     log('timing ...')
     main_out = _lib.OUTPUT_INTENSITY
+    time_path.isolated = {}
     sec_main, kms_main, out_main = time_path(args.precision, args.steps, args.warmup, main_out)
+    kms_iso = time_path.isolated[(args.precision, main_out, True)]
     err_main = err(out_main)
     variants = {}
     outs_for_oracle = {'main': out_main}
@@ -469,13 +479,14 @@ def main():
     ms1 = kms_main[1]
     achieved = flops_stage1 / (ms1 * 1e-3) / 1e12
     peak_tf = PEAK_TFLOPS[args.precision]
-    t0_cfg = tiles.get(args.precision, (-1, -1))[0]
     main_form = forms.get((args.precision, True), 'complex')
+    t0_cfg = tiles.get((args.precision, main_form), (-1, -1))[0]
     if main_form == 'parity':
         # the parity form's own stage 1: four real products of (u' halves) x (2 y' halves) x
         # (x' halves), magnitudes counted as the plan pairs them (the FFT-style focal axis has one
-        # unpaired sample, so 257 u' values for 512 samples)
-        hu, hy, hx = nu // 2 + (1 if nu % 2 == 0 else 0), ny // 2, nx // 2
+        # unpaired sample, so 257 u' values for 512 samples; the parity split forms the unpaired
+        # value's row, so stage 1 runs the 256 paired ones)
+        hu, hy, hx = nu // 2, ny // 2, nx // 2
         flops_exec1 = 2.0 * 4 * hu * (2 * hy) * hx
     else:
         flops_exec1 = flops_stage1
@@ -508,8 +519,12 @@ def main():
                 'form_frac': round(flops_exec1 / (ms1 * 1e-3) / 1e12 / peak_tf, 4),
                 'mfma_issued_tflops': round(split_products * flops_exec1 / (ms1 * 1e-3) / 1e12, 2),
                 'traffic_unit': 'bytes per launch', 'traffic_source': traffic_src,
+                # HIP-event durations inside whole propagations (the achieved figures above), and
+                # of each kernel alone over back-to-back launches
                 'kernel_ms': {'split': round(kms_main[0], 5), 'stage1': round(kms_main[1], 5),
                               'stage2': round(kms_main[2], 5)},
+                'kernel_ms_isolated': {'split': round(kms_iso[0], 5), 'stage1': round(kms_iso[1], 5),
+                                       'stage2': round(kms_iso[2], 5)},
                 'algorithmic_flop_per_launch': flops_stage1}
 
     cpu = None
@@ -591,7 +606,7 @@ def main():
                    'output': 'focal intensity (float32) from the FP64 epilogue value',
                    'step': '%d propagations replayed from one HIP graph' % max(1, args.chunk)
                            if not args.no_graph else '%d eager propagations' % max(1, args.chunk),
-                   'tile_shapes': {k: list(v) for k, v in tiles.items()} or 'default'},
+                   'tile_shapes': {'%s %s form' % k: list(v) for k, v in tiles.items()} or 'default'},
         'peak_rel_intensity_error': err_main,
         'peak_rel_intensity_error_vs_oracle': oracle_err,
         'speedup_vs_fp32': round(value / fp32_ref['value'], 3) if fp32_ref else None,

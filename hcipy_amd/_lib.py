@@ -44,10 +44,14 @@ TILES_D64X32_W8 = 15
 TILES_R64X64_W4 = 16
 TILES_R64X32_W4 = 17
 TILES_R32X64_W4 = 18
+TILES_RC32X32_W4 = 19
+TILES_RC32X32_H = 20
 TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(16)),
              'bf16x3': (0, 1, 4, 7)}
 # tile shapes of the parity form (hcipy_mft_plan_set_parity)
 PARITY_TILE_CFGS = {'f16x2': (0, 1, 4, 5, 6, 7, 16, 17, 18), 'bf16x3': (0, 1, 4, 7)}
+# stage 2 of the parity form only: the recombination runs in the stage's epilogue
+PARITY_STAGE2_TILE_CFGS = {'f16x2': (19, 20), 'bf16x3': (20,)}
 
 
 class MftGroupItem(ctypes.Structure):
@@ -117,6 +121,9 @@ SIGNATURES = {
     'hcipy_mft_profile': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                          _vp]),
+    'hcipy_mft_profile_pipeline': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _c_int64, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_int, _vp, ctypes.c_size_t, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_float), _vp]),
     'hcipy_pw_sub_mul': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _c_int64, ctypes.c_int, _vp, ctypes.c_int,
                                         _c_int64, _c_int64, _vp]),
     'hcipy_pw_phase': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _vp, ctypes.c_int, _c_int64,

@@ -43,12 +43,17 @@ int launch_stage_grouped(int prec, int cfg, int b_mode, const StageArgs* table, 
 int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64);
 bool grouped_cfg_supported(int prec, int cfg);
 void grouped_tile_shape(int cfg, int* bm, int* bn);
+int grouped_real_cfg(int prec, int stage, int64_t tiles64);
+void real_tile_shape(int cfg, int* bm, int* bn);
+int launch_stage_grouped_real(int prec, int cfg, const StageArgs* table, const int64_t* tile_start,
+                              const int* tile_item, int n, int64_t tiles, hipStream_t s);
 constexpr int kModePadded = 0, kModeAligned = 1, kModeUnaligned = 2;  // BMode of mft_gemm.hip
 int launch_split_rows(int prec, const float* in, int64_t ld_in, int K, int64_t rows, SplitOut o, int64_t ld_out,
                       float* row_scale, hipStream_t s);
 int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double sign, int prec, SplitOut out,
                       int64_t lda, int rows_pad, hipStream_t s);
-bool real_cfg_supported(int prec, int cfg);
+bool real_cfg_supported(int prec, int cfg, int stage);
+bool is_comb_cfg(int cfg);
 int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s);
 int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis& ax, const ParAxis& ay,
                         int64_t batch, SplitOut o, int64_t ld_out, float* row_scale, const double* nyq_coef,
@@ -364,6 +369,7 @@ struct Collected {
   StageArgs a1{}, a2{};
   int b_mode1 = 0;
   int64_t nz2 = 0;  // batch slices of stage 2 (0: 1-D plan, no stage 2)
+  int64_t nz1 = 1;  // batch slices of stage 1 (the parity form: four items per field)
 };
 
 // Runs one direction in the parity form: parity split, stage 1 (real half-size constants, four
@@ -444,6 +450,44 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
   }
   if ((st = mark(2))) return st;
   if (only >= 0 && only != 2) return mark(3);
+  // default stage-2 shape of the parity form: the recombining one
+  const int cfg2 = pl.tiles[dir][1] != kTileAuto ? pl.tiles[dir][1] : (pl.prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h);
+  if (is_comb_cfg(cfg2)) {
+    // stage 2 with the recombination in its epilogue: one workgroup computes all four items of a
+    // tile and writes the output quadrants
+    const size_t ob = out_elem_bytes(out_mode) * (size_t)d.c1 * d.c2;
+    for (int64_t b0 = 0; b0 < batch; b0 += kChunk) {
+      const int nb = (int)std::min<int64_t>(kChunk, batch - b0);
+      StageArgs a{};
+      a.tile_cfg = cfg2;
+      a.a = p2.p[0];
+      a.a_lo = p2.p[1];
+      a.a_lo2 = p2.p[2];
+      a.lda = p2.lda;
+      a.a_zstride = p2.plane();
+      const int64_t gz = 2 * hc1 * w.ldg;
+      a.b = ws + w.g[0] + (size_t)b0 * 4 * gz * 2;
+      a.b_lo = na > 1 ? ws + w.g[1] + (size_t)b0 * 4 * gz * 2 : nullptr;
+      a.b_lo2 = na > 2 ? ws + w.g[2] + (size_t)b0 * 4 * gz * 2 : nullptr;
+      a.ldb = w.ldg;
+      a.b_zstride = gz;
+      a.M = (int)hc2;
+      a.N = (int)(2 * hc1);
+      a.K = (int)hn;
+      a.z_scale = t + b0 * 4;
+      a.out_mode = out_mode;
+      a.out = static_cast<char*>(out) + (size_t)b0 * ob;
+      a.cu = ac1;
+      a.cv = ac2;
+      a.c_nu = (int)d.c1;
+      a.c_nv = (int)d.c2;
+      a.c_sign = fw ? -1.0 : 1.0;
+      a.alpha_re = ar;
+      a.alpha_im = ai;
+      if ((st = launch_stage_real(pl.prec, a, nb, s))) return st;
+    }
+    return mark(3);
+  }
   for (int64_t b0 = 0; b0 < batch; b0 += kChunk) {
     const int nb = (int)std::min<int64_t>(kChunk, batch - b0);
     StageArgs a{};
@@ -474,6 +518,106 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
   st = launch_parity_combine(H, ac1, ac2, (int)d.c1, (int)d.c2, fw ? -1.0 : 1.0, ar, ai, out_mode, out, batch, s);
   if (st) return st;
   return mark(3);
+}
+
+// Stage arguments of one direction in the parity form for a grouped launch (stage 1 with the
+// real constant, recombining stage 2; no extra Nyquist row).  The parity split is launched here
+// unless `shared` names an earlier item that split the same input with the same axis pairing;
+// stage 1 then reads that item's split data and row scales.
+int collect_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai, char* ws,
+                   hipStream_t s, int output, Collected* col, const Collected* shared) {
+  const Dir d = direction(pl, dir);
+  const bool fw = dir == HCIPY_FORWARD;
+  const PMatrix& p1 = fw ? pl.pf1 : pl.pb1;
+  const PMatrix& p2 = fw ? pl.pf2 : pl.pb2;
+  const ParAxis& ak = fw ? pl.px : pl.pu;
+  const ParAxis& an = fw ? pl.py : pl.pv;
+  const ParAxis& ac1 = fw ? pl.pu : pl.px;
+  const ParAxis& ac2 = fw ? pl.pv : pl.py;
+  const int out_mode = out_mode_for(pl, output);
+  HCIPY_REQUIRE(out_mode >= 0, "unknown output kind %d", output);
+  const ParWs w = parity_layout(pl, d, batch);
+  const int64_t hn = an.h, hk = ak.h, hc1 = ac1.h, hc2 = ac2.h;
+  const int na = n_arrays(pl.prec);
+  StageArgs a{};
+  if (shared != nullptr) {
+    a.b = shared->a1.b;
+    a.b_lo = shared->a1.b_lo;
+    a.b_lo2 = shared->a1.b_lo2;
+    a.row_scale = shared->a1.row_scale;
+  } else {
+    SplitOut fo{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
+    float* rs = reinterpret_cast<float*>(ws + w.rs);
+    if (int st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch, fo,
+                                     w.ldf, rs, nullptr, nullptr, s))
+      return st;
+    a.b = fo.p[0];
+    a.b_lo = fo.p[1];
+    a.b_lo2 = fo.p[2];
+    a.row_scale = rs;
+  }
+  float* t = reinterpret_cast<float*>(ws + w.t);
+  a.a = p1.p[0];
+  a.a_lo = p1.p[1];
+  a.a_lo2 = p1.p[2];
+  a.lda = p1.lda;
+  a.a_zstride = p1.plane();
+  a.a_zshift = 0;
+  a.ldb = w.ldf;
+  a.b_zstride = 2 * hn * w.ldf;
+  a.M = (int)hc1;
+  a.N = (int)(2 * hn);
+  a.K = (int)hk;
+  a.rs_zstride = 2 * hn;
+  a.out_mode = kOutSplitT;
+  const int64_t gz = 2 * hc1 * w.ldg;
+  a.out = ws + w.g[0];
+  a.out_lo = na > 1 ? ws + w.g[1] : nullptr;
+  a.out_lo2 = na > 2 ? ws + w.g[2] : nullptr;
+  a.ldo = w.ldg;
+  a.out_zstride = gz;
+  a.fold = (int)(2 * hn);
+  a.t_out = t;
+  a.t_zstride = 1;
+  a.t_log2k = ilog2_ceil(hk);
+  a.alpha_re = 1.0;
+  col->a1 = a;
+  col->nz1 = 4 * batch;
+  StageArgs c{};
+  c.a = p2.p[0];
+  c.a_lo = p2.p[1];
+  c.a_lo2 = p2.p[2];
+  c.lda = p2.lda;
+  c.a_zstride = p2.plane();
+  c.b = a.out;
+  c.b_lo = a.out_lo;
+  c.b_lo2 = a.out_lo2;
+  c.ldb = w.ldg;
+  c.b_zstride = gz;
+  c.M = (int)hc2;
+  c.N = (int)(2 * hc1);
+  c.K = (int)hn;
+  c.z_scale = t;
+  c.out_mode = out_mode;
+  c.out = out;
+  c.cu = ac1;
+  c.cv = ac2;
+  c.c_nu = (int)d.c1;
+  c.c_nv = (int)d.c2;
+  c.c_sign = fw ? -1.0 : 1.0;
+  c.alpha_re = ar;
+  c.alpha_im = ai;
+  col->a2 = c;
+  col->nz2 = batch;
+  return kOk;
+}
+
+// Same pairing of the input axes (the parity split depends on the grid indices of the mirror
+// samples only).
+bool same_pairing(const ParAxis& a, const ParAxis& b) {
+  if (a.h != b.h || a.n != b.n || a.affine != b.affine) return false;
+  if (a.affine) return a.p0 == b.p0 && a.m0 == b.m0 && a.mlo == b.mlo;
+  return a.plus == b.plus && a.minus == b.minus;
 }
 
 // Runs one direction: data split (split-FP16 only), stage 1, stage 2.  `only` >= 0 launches just
@@ -687,7 +831,40 @@ size_t group_table_bytes(int cap) {
 struct GroupedLaunch {
   int stage, b_mode, cfg, first, n;
   int64_t start_off, tiles;
+  bool real = false;  // parity-form launch (launch_stage_grouped_real)
 };
+
+// Parity-form grouped launches: stage 1 of every item, then the recombining stage 2 of every item.
+void build_parity_launches(int prec, const std::vector<Collected>& col, StageArgs* hargs, int64_t* hstart,
+                           std::vector<GroupedLaunch>& launches) {
+  int na = 0;
+  int64_t ns = 0;
+  for (int stage = 0; stage < 2; ++stage) {
+    int64_t tiles64 = 0;
+    for (const Collected& c : col) {
+      const StageArgs& a = stage == 0 ? c.a1 : c.a2;
+      tiles64 += ceil_div(a.M, 64) * ceil_div(a.N, 64) * (stage == 0 ? c.nz1 : c.nz2);
+    }
+    const int cfg = grouped_real_cfg(prec, stage, tiles64);
+    int bm, bn;
+    real_tile_shape(cfg, &bm, &bn);
+    GroupedLaunch L{stage, 0, cfg, na, 0, ns, 0};
+    L.real = true;
+    int64_t acc = 0;
+    for (const Collected& c : col) {
+      const StageArgs& a = stage == 0 ? c.a1 : c.a2;
+      hargs[na] = a;
+      hargs[na].tile_cfg = cfg;
+      ++na;
+      hstart[ns++] = acc;
+      acc += ceil_div(a.M, bm) * ceil_div(a.N, bn) * (stage == 0 ? c.nz1 : c.nz2);
+      ++L.n;
+    }
+    hstart[ns++] = acc;
+    L.tiles = acc;
+    launches.push_back(L);
+  }
+}
 
 // Writes the stage arguments of `col` to hargs (stage-1 entries ordered by data mode, one launch
 // per mode, then the stage-2 entries) and their tile prefix sums to hstart; appends the launches.
@@ -1074,7 +1251,7 @@ int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction_, int stage, int
   HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
   HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "unknown direction %d", direction_);
   HCIPY_REQUIRE(stage == 0 || stage == 1, "stage must be 0 or 1, got %d", stage);
-  HCIPY_REQUIRE(tile_cfg == HCIPY_TILES_AUTO || (plan->p.parity ? real_cfg_supported(plan->p.prec, tile_cfg)
+  HCIPY_REQUIRE(tile_cfg == HCIPY_TILES_AUTO || (plan->p.parity ? real_cfg_supported(plan->p.prec, tile_cfg, stage)
                                                                  : tile_cfg_supported(plan->p.prec, tile_cfg)),
                 "tile configuration %d is not available for precision %d%s", tile_cfg, plan->p.prec,
                 plan->p.parity ? " in the parity form" : "");
@@ -1174,6 +1351,45 @@ int hcipy_mft_profile(hcipy_mft_plan plan, int direction_, const void* in, void*
   return st;
 }
 
+int hcipy_mft_profile_pipeline(hcipy_mft_plan plan, int direction_, const void* in, void* out, int64_t batch,
+                               double alpha_re, double alpha_im, int output, void* workspace,
+                               size_t workspace_bytes, int repeats, float* kernel_ms, void* stream) {
+  HCIPY_REQUIRE(plan != nullptr && kernel_ms != nullptr && repeats > 0, "bad argument");
+  HCIPY_REQUIRE(direction_ == HCIPY_FORWARD || direction_ == HCIPY_BACKWARD, "bad direction %d", direction_);
+  HCIPY_REQUIRE(batch > 0 && in != nullptr && out != nullptr, "bad data arguments");
+  const Plan& pl = plan->p;
+  const size_t need = direction_ws_bytes(pl, direction_, batch);
+  HCIPY_REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+  HCIPY_REQUIRE(aligned16(workspace), "workspace must be 16-byte aligned");
+  if (int st = refuse_under_capture(static_cast<hipStream_t>(stream), "hcipy_mft_profile_pipeline")) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t ev[2];
+  int created = 0;
+  int st = kOk;
+  for (; created < 2 && !st; ++created) st = check_hip(hipEventCreate(&ev[created]), "hipEventCreate");
+  if (!st)
+    st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s, nullptr,
+                       output);
+  // prefix k: `repeats` executions of steps 0..k back to back between two events; step k's time
+  // is the difference of consecutive prefixes
+  float prev = 0.0f;
+  for (int k = 0; k < 3 && !st; ++k) {
+    st = check_hip(hipEventRecord(ev[0], s), "hipEventRecord");
+    for (int r = 0; r < repeats && !st; ++r)
+      for (int j = 0; j <= k && !st; ++j)
+        st = run_direction(pl, direction_, in, out, batch, alpha_re, alpha_im, static_cast<char*>(workspace), s,
+                           nullptr, output, j);
+    if (!st) st = check_hip(hipEventRecord(ev[1], s), "hipEventRecord");
+    if (!st) st = check_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
+    float ms = 0.0f;
+    if (!st) st = check_hip(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
+    kernel_ms[k] = (ms - prev) / (float)repeats;
+    prev = ms;
+  }
+  for (int i = 0; i < created; ++i) (void)hipEventDestroy(ev[i]);
+  return st;
+}
+
 int hcipy_mft_group_create(hcipy_mft_group* group, int capacity) {
   HCIPY_REQUIRE(group != nullptr && capacity > 0 && capacity <= (1 << 20), "bad argument");
   *group = nullptr;
@@ -1226,7 +1442,35 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   HCIPY_REQUIRE(cur == group->device, "the group was created on device %d, current device is %d", group->device, cur);
   const int prec = items[0].plan != nullptr ? items[0].plan->p.prec : -1;
   std::vector<Collected> col((size_t)n_items);
-  for (int i = 0; i < n_items; ++i) {
+  // the parity form when every item's plan runs it (2-D split plans between symmetric grids)
+  bool parity = true;
+  for (int i = 0; i < n_items && parity; ++i) parity = items[i].plan != nullptr && items[i].plan->p.parity;
+  for (int i = 0; i < n_items && parity; ++i) {
+    const hcipy_mft_group_item& it = items[i];
+    HCIPY_REQUIRE(it.in != nullptr && it.out != nullptr && it.batch > 0, "bad item %d", i);
+    const Plan& pl = it.plan->p;
+    HCIPY_REQUIRE(pl.prec == prec, "item %d: all items of a grouped launch share one precision", i);
+    HCIPY_REQUIRE(pl.device == cur, "item %d: plan lives on device %d", i, pl.device);
+    HCIPY_REQUIRE(it.direction == HCIPY_FORWARD || it.direction == HCIPY_BACKWARD, "item %d: bad direction", i);
+    const Dir di = direction(pl, it.direction);
+    HCIPY_REQUIRE(it.workspace_bytes >= parity_layout(pl, di, it.batch).total && aligned16(it.workspace),
+                  "item %d: workspace too small", i);
+    const bool fw = it.direction == HCIPY_FORWARD;
+    const Collected* shared = nullptr;
+    for (int j = 0; j < i && shared == nullptr; ++j) {
+      const hcipy_mft_group_item& o = items[j];
+      const Plan& pj = o.plan->p;
+      const bool fj = o.direction == HCIPY_FORWARD;
+      if (o.in == it.in && o.batch == it.batch && same_pairing(fw ? pl.px : pl.pu, fj ? pj.px : pj.pu) &&
+          same_pairing(fw ? pl.py : pl.pv, fj ? pj.py : pj.pv) &&
+          parity_layout(pj, direction(pj, o.direction), o.batch).ldf == parity_layout(pl, di, it.batch).ldf)
+        shared = &col[(size_t)j];
+    }
+    const int st = collect_parity(pl, it.direction, it.in, it.out, it.batch, it.alpha_re, it.alpha_im,
+                                  static_cast<char*>(it.workspace), s, it.output, &col[(size_t)i], shared);
+    if (st != kOk) return st;
+  }
+  for (int i = 0; i < n_items && !parity; ++i) {
     const hcipy_mft_group_item& it = items[i];
     HCIPY_REQUIRE(it.plan != nullptr && it.in != nullptr && it.out != nullptr && it.batch > 0, "bad item %d", i);
     const Plan& pl = it.plan->p;
@@ -1273,7 +1517,8 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   StageArgs* hargs = reinterpret_cast<StageArgs*>(src);
   int64_t* hstart = reinterpret_cast<int64_t*>(src + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
   std::vector<GroupedLaunch> launches;
-  build_grouped_launches(prec, col, hargs, hstart, launches);
+  if (parity) build_parity_launches(prec, col, hargs, hstart, launches);
+  else build_grouped_launches(prec, col, hargs, hstart, launches);
   HCIPY_HIP(hipMemcpyAsync(group->dev, src, group->bytes, hipMemcpyHostToDevice, s));
   if (capture == hipStreamCaptureStatusNone) {
     HCIPY_HIP(hipEventRecord(group->copied, s));
@@ -1282,8 +1527,10 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   const StageArgs* dargs = reinterpret_cast<const StageArgs*>(group->dev);
   const int64_t* dstart = reinterpret_cast<const int64_t*>(group->dev + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
   for (const GroupedLaunch& L : launches) {
-    const int st =
-        launch_stage_grouped(prec, L.cfg, L.b_mode, dargs + L.first, dstart + L.start_off, nullptr, L.n, L.tiles, s);
+    const int st = L.real ? launch_stage_grouped_real(prec, L.cfg, dargs + L.first, dstart + L.start_off, nullptr, L.n,
+                                                      L.tiles, s)
+                          : launch_stage_grouped(prec, L.cfg, L.b_mode, dargs + L.first, dstart + L.start_off, nullptr,
+                                                 L.n, L.tiles, s);
     if (st != kOk) return st;
   }
   return kOk;
@@ -1299,7 +1546,7 @@ int hcipy_mft_autotune(hcipy_mft_plan plan, int direction_, const void* in, void
     int best = kTileAuto;
     float best_ms = 0.0f;
     for (int cfg = 0; cfg < kNumTileCfg; ++cfg) {
-      if (!(pl.parity ? real_cfg_supported(pl.prec, cfg) : tile_cfg_supported(pl.prec, cfg))) continue;
+      if (!(pl.parity ? real_cfg_supported(pl.prec, cfg, stage) : tile_cfg_supported(pl.prec, cfg))) continue;
       pl.tiles[direction_][stage] = cfg;
       float ms[3] = {0.0f, 0.0f, 0.0f};
       const int st = hcipy_mft_profile(plan, direction_, in, out, batch, 1.0, 0.0, output, workspace, workspace_bytes,

@@ -96,9 +96,43 @@ enum BMode : int {
   kRowsPadded = 0,   // rows padded to a multiple of 64 elements: 16-byte loads never leave the row
   kRowsAligned = 1,  // 16-byte aligned rows of exactly K elements: vector loads inside K, scalar tail
   kRowsUnaligned = 2,// element loads only
-  kRealPadded = 3    // padded data rows, and a real constant of BM plain rows per tile (no 2x2
+  kRealPadded = 3,   // padded data rows, and a real constant of BM plain rows per tile (no 2x2
                      // expansion): C[m][n] is real and the epilogue writes single values
+  kRealComb = 4      // parity stage 2 with recombination: BM = 64 constant rows (32 of each plane),
+                     // BN = 128 data rows (32 of each parity item), wave w computes item w
 };
+
+// Parity recombination of one entry (v', u'): the four products of the parity items (ee, eo, oe,
+// oo; x = real part, y = imaginary part) give the outputs at the up to four mirror positions
+//   O(sv v', su u') = ee + i s su eo + i s sv oe - su sv oo     (s = -1 forward, +1 backward)
+// times alpha, stored in out_mode at out[ob + v * nu + u].
+__device__ inline void parity_store(double2 ee, double2 eo, double2 oe, double2 oo, int vp, int up, const ParAxis& au,
+                                    const ParAxis& av, int nu, double sign, double ar, double ai, int out_mode,
+                                    void* __restrict__ out, int64_t ob) {
+  const int vis[2] = {par_plus(av, vp), par_minus(av, vp)};
+  const int uis[2] = {par_plus(au, up), par_minus(au, up)};
+#pragma unroll
+  for (int qv = 0; qv < 2; ++qv) {
+    const double sv = qv ? -1.0 : 1.0;
+    const int vi = vis[qv];
+    if (vi < 0) continue;
+#pragma unroll
+    for (int qu = 0; qu < 2; ++qu) {
+      const double su = qu ? -1.0 : 1.0;
+      const int ui = uis[qu];
+      if (ui < 0) continue;
+      const double a = sign * su, c = sign * sv, d = su * sv;
+      const double orr = ee.x - a * eo.y - c * oe.y - d * oo.x;
+      const double oii = ee.y + a * eo.x + c * oe.x - d * oo.y;
+      const double zr = orr * ar - oii * ai, zi = orr * ai + oii * ar;
+      const int64_t o = ob + (int64_t)vi * nu + ui;
+      if (out_mode == kOutC64) static_cast<float2*>(out)[o] = make_float2((float)zr, (float)zi);
+      else if (out_mode == kOutC128) static_cast<double2*>(out)[o] = make_double2(zr, zi);
+      else if (out_mode == kOutI32) static_cast<float*>(out)[o] = (float)(zr * zr + zi * zi);
+      else static_cast<double*>(out)[o] = zr * zr + zi * zi;
+    }
+  }
+}
 
 // XCD-aware tile order.  Consecutive workgroups are dispatched round-robin over the 8 XCDs; the
 // hardware id h is remapped so that XCD x owns a contiguous chunk of the logical tile order, and
@@ -108,7 +142,14 @@ enum BMode : int {
 // slots (h + slot_off) mod 8 are all distinct, so the map stays a bijection on [0, T).
 __device__ inline void tile_of(int h, int gx, int gy, int& bx, int& by, int slot_off = 0) {
   const int T = gx * gy;
-  const int L = (T % 8 == 0) ? ((h + slot_off) & 7) * (T / 8) + (h >> 3) : h;
+  int L = h;
+  if (T % 8 == 0) {
+    L = ((h + slot_off) & 7) * (T / 8) + (h >> 3);
+  } else if (slot_off == 0) {
+    // chunks of q + 1 (the first r XCDs) or q logical tiles: still a bijection on [0, T)
+    const int q = T >> 3, r = T & 7, x = h & 7;
+    L = x * q + min(x, r) + (h >> 3);
+  }
   constexpr int GM = 4;
   const int per_group = GM * gy;
   const int group = L / per_group;
@@ -166,10 +207,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   constexpr int VEC = 16 / (int)sizeof(T);
   constexpr int LDK = BK + VEC;  // 16-byte row padding against LDS bank conflicts
   constexpr int NARR = TR::NARR;
-  constexpr bool REAL = (BMODE == kRealPadded);
+  constexpr bool COMB = (BMODE == kRealComb);
+  constexpr bool REAL = (BMODE == kRealPadded || COMB);
+  static_assert(!COMB || (BM == 64 && BN == 128 && WM == 2 && WN == 2 && WK == 1 && TR::SPLIT),
+                "recombining stage: 2 planes x 32 rows, 4 items x 32 rows, one wave per item");
   constexpr int BMC = REAL ? BM : BM / 2;  // constant rows loaded per tile (compact complex, or real)
   constexpr int WTM = BM / WM;
-  constexpr int WTN = BN / WN;
+  constexpr int WTN = COMB ? 32 : BN / WN;
   constexpr int TM = WTM / MT;
   constexpr int TN = WTN / MT;
   constexpr int CPR = BK / VEC;  // 16-byte chunks per tile row
@@ -195,25 +239,29 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wk = wave / (WM * WN);
-  const int wm = (wave % (WM * WN)) / WN;
-  const int wn = wave % WN;
-  const int gx = (p.M + BM - 1) / BM;
-  const int gy = (p.N + BN - 1) / BN;
+  // recombining stage: wave w takes constant plane w >> 1 (cosines for ee / eo, sines for oe / oo)
+  // and parity item w
+  const int wm = COMB ? (wave >> 1) : (wave % (WM * WN)) / WN;
+  const int wn = COMB ? wave : wave % WN;
+  constexpr int TBM = COMB ? 32 : BM;  // output rows and columns per workgroup
+  constexpr int TBN = COMB ? 32 : BN;
+  const int gx = (p.M + TBM - 1) / TBM;
+  const int gy = (p.N + TBN - 1) / TBN;
   int bx, by;
   tile_of(h, gx, gy, bx, by, slot_off);
-  const int m0 = bx * BM;
+  const int m0 = bx * TBM;
   const int mc0 = m0 / 2;
-  const int n0 = by * BN;
+  const int n0 = by * TBN;
   const int M = p.M, N = p.N, K = p.K;
   const int nk = (K + BK - 1) / BK;
 
   const T* Ag[3] = {static_cast<const T*>(p.a), static_cast<const T*>(p.a_lo), static_cast<const T*>(p.a_lo2)};
   const T* Bg[3] = {static_cast<const T*>(p.b), static_cast<const T*>(p.b_lo), static_cast<const T*>(p.b_lo2)};
-  const int64_t a_plane = (int64_t)((z >> p.a_zshift) & 1) * p.a_zstride;
+  const int64_t a_plane = COMB ? 0 : (int64_t)((z >> p.a_zshift) & 1) * p.a_zstride;
 #pragma unroll
   for (int arr = 0; arr < NARR; ++arr) {
     Ag[arr] += a_plane;
-    Bg[arr] += (int64_t)z * p.b_zstride;
+    Bg[arr] += (int64_t)z * (COMB ? 4 : 1) * p.b_zstride;
   }
 
   const bool split_out = (p.out_mode == kOutSplit || p.out_mode == kOutSplitT);
@@ -221,20 +269,27 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
 
   // Per-thread chunk geometry (fixed over the k loop).
   int a_off[AL];
+  int64_t a_goff[AL];
   int b_row[BL];
   int64_t b_off[BL];
   int b_kc[BL];
 #pragma unroll
   for (int i = 0; i < AL; ++i) {
     const int ch = tid + i * NTHR;
-    a_off[i] = (ch / CPR) * LDK + (ch % CPR) * VEC;  // LDS offset of the compact chunk's row (x2 below)
+    const int r = ch / CPR;
+    a_off[i] = r * LDK + (ch % CPR) * VEC;  // LDS offset of the compact chunk's row (x2 below)
+    if constexpr (COMB) a_goff[i] = (int64_t)(r >> 5) * p.a_zstride + (int64_t)(m0 + (r & 31)) * p.lda;
+    else a_goff[i] = (int64_t)((REAL ? m0 : mc0) + r) * p.lda;
+    a_goff[i] += (ch % CPR) * VEC;
   }
 #pragma unroll
   for (int i = 0; i < BL; ++i) {
     const int ch = tid + i * NTHR;
-    b_row[i] = n0 + ch / CPR;
+    const int r = ch / CPR;
+    b_row[i] = n0 + (COMB ? (r & 31) : r);
     b_kc[i] = (ch % CPR) * VEC;
     b_off[i] = (int64_t)min(b_row[i], N - 1) * p.ldb;
+    if constexpr (COMB) b_off[i] += (int64_t)(r >> 5) * p.b_zstride;
   }
   const int nk_full = K / BK;  // k-tiles with no element beyond K
   // Largest k offset a full-vector load may start at: padded rows allow every chunk of the last
@@ -250,11 +305,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
 #pragma unroll
     for (int arr = 0; arr < NARR; ++arr) {
 #pragma unroll
-      for (int i = 0; i < AL; ++i) {
-        const int ch = tid + i * NTHR;
-        ra[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Ag[arr] + (int64_t)((REAL ? m0 : mc0) + ch / CPR) * p.lda +
-                                                          k0 + (ch % CPR) * VEC);
-      }
+      for (int i = 0; i < AL; ++i) ra[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Ag[arr] + a_goff[i] + k0);
 #pragma unroll
       for (int i = 0; i < BL; ++i) {
         if constexpr (BMODE == kRowsUnaligned) {
@@ -622,6 +673,34 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
     }
   }
 
+  if constexpr (COMB) {
+    // Epilogue of the recombining stage: each wave's 32 x 32 products (item w, scaled back) go to
+    // LDS, then every thread recombines entries (v', u') of the four items into the output.
+    __syncthreads();
+    constexpr int HLD = 33;  // padded row of doubles
+    static_assert((size_t)4 * 32 * HLD * sizeof(double) <= 2 * (size_t)BUF * sizeof(T), "products fit in LDS");
+    double* hs = reinterpret_cast<double*>(smem_raw);
+    const double zs = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[4 * z + wave] : 1.0;
+#pragma unroll
+    for (int r = 0; r < NACC; ++r) hs[(wave * 32 + acc_row<MT>(r, lane)) * HLD + lane % MT] = acc64[0][0][r] * zs;
+    __syncthreads();
+    const int64_t ob = (int64_t)z * p.c_nu * p.c_nv;
+    for (int e = tid; e < 32 * 16; e += NTHR) {
+      const int mi = e >> 4, ui = e & 15;
+      const int vp = m0 + mi, up = (n0 >> 1) + ui;
+      if (vp >= M || 2 * up >= N) continue;
+      double2 q[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double* row = hs + (c * 32 + mi) * HLD + 2 * ui;
+        q[c] = make_double2(row[0], row[1]);
+      }
+      parity_store(q[0], q[1], q[2], q[3], vp, up, p.cu, p.cv, p.c_nu, p.c_sign, p.alpha_re, p.alpha_im, p.out_mode,
+                   p.out, ob);
+    }
+    return;
+  }
+
   const double inv_z = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[z] : 1.0;
   if constexpr (REAL) {
     // Epilogue of a real constant: one real value per accumulator register.
@@ -778,7 +857,9 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_grouped_kernel(co
     }
   }
   const StageArgs p = table[lo];
-  const int64_t tiles_xy = (int64_t)((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
+  constexpr int TBM = BMODE == kRealComb ? 32 : BM;  // output tile of a workgroup
+  constexpr int TBN = BMODE == kRealComb ? 32 : BN;
+  const int64_t tiles_xy = (int64_t)((p.M + TBM - 1) / TBM) * ((p.N + TBN - 1) / TBN);
   const int64_t local = b - tile_start[lo];
   const int h = (int)(local % tiles_xy);
   // the XCD slot of this item's tile h = 0 in this batch slice
@@ -1736,7 +1817,12 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
   __shared__ double red_n[TEAMS][WPT][8];
   const int team = threadIdx.x / TPR;
   const int tt = threadIdx.x % TPR;
-  const int yp = blockIdx.x * TEAMS + team;
+  // XCD-aware row order: consecutive workgroups go round-robin over the 8 XCDs, and XCD x takes
+  // the x-th eighth of the y' rows, the rows the stage-1 workgroups on that XCD read (their tile
+  // order gives each XCD a contiguous range of data rows)
+  const int nblk = gridDim.x;
+  const int rb = (nblk % 8 == 0) ? (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+  const int yp = rb * TEAMS + team;
   const int64_t b = blockIdx.y;
   const int hx = ax.h, hy = ay.h;
   const bool live = yp < hy;
@@ -1913,32 +1999,8 @@ __global__ __launch_bounds__(256) void parity_combine_kernel(const double2* __re
   const int vp = (int)(rem / hu);
   const int up = (int)(rem - (int64_t)vp * hu);
   const double2* Hb = H + b * 4 * per + rem;
-  const double2 ee = Hb[0], eo = Hb[per], oe = Hb[2 * per], oo = Hb[3 * per];
-  const int64_t ob = b * (int64_t)nv * nu;
-  const int vis[2] = {par_plus(av, vp), par_minus(av, vp)};
-  const int uis[2] = {par_plus(au, up), par_minus(au, up)};
-#pragma unroll
-  for (int qv = 0; qv < 2; ++qv) {
-    const double sv = qv ? -1.0 : 1.0;
-    const int vi = vis[qv];
-    if (vi < 0) continue;
-#pragma unroll
-    for (int qu = 0; qu < 2; ++qu) {
-      const double su = qu ? -1.0 : 1.0;
-      const int ui = uis[qu];
-      if (ui < 0) continue;
-      // ee + i s su eo + i s sv oe - su sv oo
-      const double a = sign * su, c = sign * sv, d = su * sv;
-      const double orr = ee.x - a * eo.y - c * oe.y - d * oo.x;
-      const double oii = ee.y + a * eo.x + c * oe.x - d * oo.y;
-      const double zr = orr * ar - oii * ai, zi = orr * ai + oii * ar;
-      const int64_t o = ob + (int64_t)vi * nu + ui;
-      if (out_mode == kOutC64) static_cast<float2*>(out)[o] = make_float2((float)zr, (float)zi);
-      else if (out_mode == kOutC128) static_cast<double2*>(out)[o] = make_double2(zr, zi);
-      else if (out_mode == kOutI32) static_cast<float*>(out)[o] = (float)(zr * zr + zi * zi);
-      else static_cast<double*>(out)[o] = zr * zr + zi * zi;
-    }
-  }
+  parity_store(Hb[0], Hb[per], Hb[2 * per], Hb[3 * per], vp, up, au, av, nu, sign, ar, ai, out_mode, out,
+               b * (int64_t)nv * nu);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1958,8 +2020,10 @@ static int launch_stage_t(const StageArgs& a, int nz, hipStream_t s) {
                                   (int)lds));
     configured = true;
   }
-  const int gx = (int)ceil_div(a.M, BM);
-  const int gy = (int)ceil_div(a.N, BN);
+  constexpr int TBM = BMODE == kRealComb ? 32 : BM;  // output tile of a workgroup
+  constexpr int TBN = BMODE == kRealComb ? 32 : BN;
+  const int gx = (int)ceil_div(a.M, TBM);
+  const int gy = (int)ceil_div(a.N, TBN);
   hipLaunchKernelGGL(kern, dim3((unsigned)(gx * gy), 1, (unsigned)nz), dim3(64 * WM * WN * WK), lds, s, a);
   return kOk;
 }
@@ -2113,6 +2177,49 @@ bool grouped_cfg_supported(int prec, int cfg) {
   return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8 || cfg == k64x32_w8 || cfg == k128x64;
 }
 
+bool is_comb_cfg(int cfg);
+
+// Grouped launches of the parity form: stage 1 (real constant) and the recombining stage 2.
+int grouped_real_cfg(int prec, int stage, int64_t tiles64) {
+  if (stage == 1) return prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h;
+  if (tiles64 < 128) return k64x32_w8;
+  return prec == kF16x2 ? kR64x64_w4 : k64x64_w2x2;
+}
+
+// Output rows x columns per workgroup of the parity shapes grouped launches use.
+void real_tile_shape(int cfg, int* bm, int* bn) {
+  if (is_comb_cfg(cfg)) {
+    *bm = 32;
+    *bn = 32;
+  } else {
+    *bm = 64;
+    *bn = cfg == k64x32_w8 ? 32 : 64;
+  }
+}
+
+int launch_stage_grouped_real(int prec, int cfg, const StageArgs* table, const int64_t* tile_start,
+                              const int* tile_item, int n, int64_t tiles, hipStream_t s) {
+  if (n <= 0 || tiles <= 0) return kOk;
+  if (prec == kF16x2) {
+    switch (cfg) {
+      case kR64x64_w4: return launch_grouped_t<kF16x2, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+      case k64x32_w8: return launch_grouped_t<kF16x2, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+      case kRC32x32_w4: return launch_grouped_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
+      case kRC32x32_h: return launch_grouped_t<kF16x2, 64, 128, 32, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
+      default: break;
+    }
+  } else if (prec == kBF16x3) {
+    switch (cfg) {
+      case k64x64_w2x2: return launch_grouped_t<kBF16x3, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+      case k64x32_w8: return launch_grouped_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+      case kRC32x32_h: return launch_grouped_t<kBF16x3, 64, 128, 32, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
+      default: break;
+    }
+  }
+  set_last_error("tile configuration %d is not available to grouped parity launches", cfg);
+  return kInvalidValue;
+}
+
 void grouped_tile_shape(int cfg, int* bm, int* bn) {
   *bm = cfg == k128x64 ? 128 : 64;
   *bn = (cfg == k64x32_w2x1 || cfg == k64x32_w8) ? 32 : 64;
@@ -2212,13 +2319,18 @@ int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double s
 }
 
 // Parity-form launchers.
-bool real_cfg_supported(int prec, int cfg) {
+bool real_cfg_supported(int prec, int cfg, int stage) {
   if (prec == kF16x2)
     return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8 || cfg == k64x32_w8 || cfg == k128x64 ||
-           cfg == k64x128 || cfg == kR64x64_w4 || cfg == kR64x32_w4 || cfg == kR32x64_w4;
-  if (prec == kBF16x3) return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8;
+           cfg == k64x128 || cfg == kR64x64_w4 || cfg == kR64x32_w4 || cfg == kR32x64_w4 ||
+           (stage == 1 && (cfg == kRC32x32_w4 || cfg == kRC32x32_h));
+  if (prec == kBF16x3)
+    return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8 ||
+           (stage == 1 && cfg == kRC32x32_h);
   return false;
 }
+
+bool is_comb_cfg(int cfg) { return cfg == kRC32x32_w4 || cfg == kRC32x32_h; }
 
 int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
   if (a.M <= 0 || a.N <= 0 || a.K <= 0 || nz <= 0) return kOk;
@@ -2238,15 +2350,20 @@ int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
       case kR64x64_w4: st = launch_stage_t<kF16x2, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
       case kR64x32_w4: st = launch_stage_t<kF16x2, 64, 32, 64, 2, 1, 2, kRealPadded, 2>(a, nz, s); break;
       case kR32x64_w4: st = launch_stage_t<kF16x2, 32, 64, 64, 1, 2, 2, kRealPadded, 2>(a, nz, s); break;
+      case kRC32x32_w4: st = launch_stage_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
+      case kRC32x32_h: st = launch_stage_t<kF16x2, 64, 128, 32, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
       default: set_last_error("tile configuration %d is not available to the parity form", cfg); return kInvalidValue;
     }
   } else if (prec == kBF16x3) {
-    const int cfg = a.tile_cfg == kTileAuto ? k64x32_w8 : a.tile_cfg;
+    // 64 x 64 tiles with 64-deep k-tiles (one workgroup per CU) on grids of 128 such tiles or
+    // more: twice as fast as the 8-wave 64 x 32 shape on stage 1 at 1024^2 -> 512^2
+    const int cfg = a.tile_cfg == kTileAuto ? (tiles64 >= 128 ? k64x64_w2x2 : k64x32_w8) : a.tile_cfg;
     switch (cfg) {
       case k64x64_w2x2: st = launch_stage_t<kBF16x3, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
       case k64x32_w2x1: st = launch_stage_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRealPadded, 2>(a, nz, s); break;
       case k128x64: st = launch_stage_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
       case k64x32_w8: st = launch_stage_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(a, nz, s); break;
+      case kRC32x32_h: st = launch_stage_t<kBF16x3, 64, 128, 32, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
       default: set_last_error("tile configuration %d is not available to the parity form", cfg); return kInvalidValue;
     }
   } else {

@@ -103,6 +103,28 @@ constexpr float kLoInv = 2048.0f;
 // Real k values accumulated in FP32 by the hi.hi MFMAs between flushes into FP64.
 constexpr int kFlushK = 128;
 
+// One axis of a grid in the parity form: h magnitudes P[j] >= 0; the sample at +P[j] is grid
+// index plus[j], the one at -P[j] is minus[j] (-1: no such sample, e.g. the origin's mirror or
+// the unpaired first sample of an FFT-style grid).  Device tables.
+struct ParAxis {
+  const int* plus;
+  const int* minus;
+  const double* P;
+  int h;
+  // Regular grids: plus[j] = p0 + j where that is < n, minus[j] = m0 - j for j >= mlo where that
+  // is >= 0 (affine != 0; the tables then need not be read).
+  int affine, p0, m0, mlo, n;
+};
+
+__device__ inline int par_plus(const ParAxis& a, int j) {
+  if (a.affine) return a.p0 + j < a.n ? a.p0 + j : -1;
+  return a.plus[j];
+}
+__device__ inline int par_minus(const ParAxis& a, int j) {
+  if (a.affine) return (j >= a.mlo && a.m0 - j >= 0) ? a.m0 - j : -1;
+  return a.minus[j];
+}
+
 struct StageArgs {
   // Constant operand (expanded Fourier matrix), padded so that tile loads never leave it.
   const void* a;
@@ -150,6 +172,13 @@ struct StageArgs {
   const double* nyq;
   int64_t nyq_zstride;
   int nyq_row;
+  // Parity stage 2 with the recombination in its epilogue (tile shapes kRC*): batch slice z is one
+  // field whose four parity items are b_zstride apart; M = rows of the output axis cv, N = 2 x
+  // magnitudes of cu; out_mode is the final output kind, the field's outputs are c_nv x c_nu at
+  // out + z * c_nv * c_nu; z_scale holds four scales per field; c_sign = -1 forward, +1 backward.
+  ParAxis cu, cv;
+  int c_nu, c_nv;
+  double c_sign;
 };
 
 // Workgroup tile shapes of the stage kernel (256 threads = 4 waves in every shape):
@@ -191,30 +220,12 @@ enum TileCfg : int {
   kR64x64_w4 = 16,  // 64 x 64 tile, 2 x 2 waves of 32 x 32
   kR64x32_w4 = 17,  // 64 x 32 tile, 2 x 1 waves of 32 x 32, k-vectors split over 2
   kR32x64_w4 = 18,  // 32 x 64 tile, 1 x 2 waves of 32 x 32, k-vectors split over 2
-  kNumTileCfg = 19,
+  // Parity form, stage 2 only: 32 output rows x 32 (u', re / im) columns of all four parity items
+  // per workgroup (wave w: item w), recombined into the output quadrants in the epilogue
+  kRC32x32_w4 = 19,  // 64-deep k-tiles (split-FP16 only)
+  kRC32x32_h = 20,   // 32-deep k-tiles
+  kNumTileCfg = 21,
 };
-
-// One axis of a grid in the parity form: h magnitudes P[j] >= 0; the sample at +P[j] is grid
-// index plus[j], the one at -P[j] is minus[j] (-1: no such sample, e.g. the origin's mirror or
-// the unpaired first sample of an FFT-style grid).  Device tables.
-struct ParAxis {
-  const int* plus;
-  const int* minus;
-  const double* P;
-  int h;
-  // Regular grids: plus[j] = p0 + j where that is < n, minus[j] = m0 - j for j >= mlo where that
-  // is >= 0 (affine != 0; the tables then need not be read).
-  int affine, p0, m0, mlo, n;
-};
-
-__device__ inline int par_plus(const ParAxis& a, int j) {
-  if (a.affine) return a.p0 + j < a.n ? a.p0 + j : -1;
-  return a.plus[j];
-}
-__device__ inline int par_minus(const ParAxis& a, int j) {
-  if (a.affine) return (j >= a.mlo && a.m0 - j >= 0) ? a.m0 - j : -1;
-  return a.minus[j];
-}
 
 // Up to three split arrays (device pointers), passed by value to the split and matrix kernels.
 struct SplitOut {

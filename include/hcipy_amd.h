@@ -98,6 +98,12 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
 #define HCIPY_TILES_R64X64_W4 16 /* 64x64 tile, 2x2 waves of 32x32 */
 #define HCIPY_TILES_R64X32_W4 17 /* 64x32 tile, 2x1 waves, k-vectors split over 2 */
 #define HCIPY_TILES_R32X64_W4 18 /* 32x64 tile, 1x2 waves, k-vectors split over 2 */
+/* Parity form, stage 2 only (F16X2; BF16X3 the second): 32 output rows x 32 (u', re / im)
+ * columns of all four parity items per workgroup, one wave per item, and the recombination into
+ * the output quadrants in the epilogue (no FP64 products in the workspace, no recombination
+ * launch). */
+#define HCIPY_TILES_RC32X32_W4 19 /* 64-deep k-tiles */
+#define HCIPY_TILES_RC32X32_H 20  /* 32-deep k-tiles */
 int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction, int stage, int tile_cfg);
 int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction, int stage, int* tile_cfg);
 
@@ -214,6 +220,15 @@ int hcipy_mft_execute_noalloc(hcipy_mft_plan plan, int direction, const void* in
 int hcipy_mft_profile(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch, double alpha_re,
                       double alpha_im, int output, void* workspace, size_t workspace_bytes, int repeats,
                       float* kernel_ms, void* stream);
+/* As hcipy_mft_profile, but each step is timed inside whole executions, with the cache state the
+ * step before it leaves: `repeats` back-to-back executions of the split alone, of split and stage
+ * 1, and of all steps are each timed between two HIP events, and kernel_ms[k] is the difference of
+ * consecutive totals per execution (no events between the launches of one execution).  With the
+ * recombining stage-2 shapes of the parity form kernel_ms[2] covers that one launch, otherwise
+ * stage 2 and the recombination.  Measurement aid with no counterpart in the reference. */
+int hcipy_mft_profile_pipeline(hcipy_mft_plan plan, int direction, const void* in, void* out, int64_t batch,
+                               double alpha_re, double alpha_im, int output, void* workspace,
+                               size_t workspace_bytes, int repeats, float* kernel_ms, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Pointwise kernels of the propagation chain.  `batch` fields of `n` samples each lie

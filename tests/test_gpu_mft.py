@@ -262,9 +262,10 @@ def test_autotune_pins_supported_shapes_and_keeps_results():
     say('autotune chose', chosen)
     plan = mft.plan_for(torch.complex64, torch.device('cuda', 0))
     shapes = _lib.PARITY_TILE_CFGS['f16x2'] if plan.parity else _lib.TILE_CFGS['f16x2']
+    stage2 = _lib.PARITY_STAGE2_TILE_CFGS['f16x2'] if plan.parity else ()
     for d, key in ((_lib.FORWARD, 'forward'), (_lib.BACKWARD, 'backward')):
         for stage in (0, 1):
-            assert chosen[key][stage] in shapes
+            assert chosen[key][stage] in shapes + (stage2 if stage == 1 else ())
             assert plan.get_tiles(d, stage) == chosen[key][stage]
     a = host(mft.forward(ha.Field(dev(f), gi)))
     assert peak_rel_field(a, o.mft_forward(f.astype(np.complex128), og(gi), og(go))) < 5e-7

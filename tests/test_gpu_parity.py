@@ -100,14 +100,21 @@ def _check_against_oracle(gi, go, precision, label, all_tiles=False):
     assert peak_rel_field(out[True][1], out[False][1]) < 5e-7
     if all_tiles:
         plan.parity = True
-        for cfg in _lib.PARITY_TILE_CFGS[precision]:
+        for cfg1, cfg2 in _shape_pairs(precision):
             for d in (_lib.FORWARD, _lib.BACKWARD):
-                for stage in (0, 1):
-                    plan.set_tiles(d, stage, cfg)
+                plan.set_tiles(d, 0, cfg1)
+                plan.set_tiles(d, 1, cfg2)
             ea = peak_rel_field(host(mft.forward(ha.Field(dev(f), gi))), ref_f)
             eb = peak_rel_field(host(mft.backward(ha.Field(dev(F), go))), ref_b)
-            say('  tile shape %d: forward %.3e backward %.3e' % (cfg, ea, eb))
-            assert ea < 5e-7 and eb < 5e-7, cfg
+            say('  tile shapes %d/%d: forward %.3e backward %.3e' % (cfg1, cfg2, ea, eb))
+            assert ea < 5e-7 and eb < 5e-7, (cfg1, cfg2)
+
+
+def _shape_pairs(precision):
+    """(stage 1, stage 2) tile shapes: every shared parity shape on both stages, then the
+    recombining stage-2 shapes after the default stage 1."""
+    return ([(c, c) for c in _lib.PARITY_TILE_CFGS[precision]] +
+            [(_lib.TILES_AUTO, c) for c in _lib.PARITY_STAGE2_TILE_CFGS[precision]])
 
 
 @pytest.mark.parametrize('precision', ['f16x2', 'bf16x3'])
@@ -121,18 +128,27 @@ def test_every_parity_tile_shape(precision):
     mft = ha.MatrixFourierTransform(gi, go, precision=precision)
     plan = _plan(mft)
     assert plan.parity
-    for cfg in _lib.PARITY_TILE_CFGS[precision]:
+    for cfg1, cfg2 in _shape_pairs(precision):
         for d in (_lib.FORWARD, _lib.BACKWARD):
-            for stage in (0, 1):
-                plan.set_tiles(d, stage, cfg)
-        a = host(mft.forward(ha.Field(dev(f), gi)))
+            plan.set_tiles(d, 0, cfg1)
+            plan.set_tiles(d, 1, cfg2)
+        for output in ('field', 'intensity'):
+            if output == 'field':
+                a = host(mft.forward(ha.Field(dev(f), gi)))
+                ea = peak_rel_field(a, ref_f)
+            else:
+                a = host(mft.forward_intensity(ha.Field(dev(f), gi))).astype(np.float64)
+                ea = np.abs(a - np.abs(ref_f) ** 2).max() / (np.abs(ref_f) ** 2).max()
+            assert ea < (5e-7 if output == 'field' else 1e-6), (cfg1, cfg2, output)
         b = host(mft.backward(ha.Field(dev(F), go)))
-        say(precision, 'parity tile shape', cfg, 'errors %.3e %.3e' % (peak_rel_field(a, ref_f), peak_rel_field(b, ref_b)))
-        assert peak_rel_field(a, ref_f) < 5e-7, cfg
-        assert peak_rel_field(b, ref_b) < 5e-7, cfg
-    # shapes of the complex form only are refused while the parity form is on
+        say(precision, 'parity tile shapes', cfg1, cfg2, 'errors %.3e %.3e' % (ea, peak_rel_field(b, ref_b)))
+        assert peak_rel_field(b, ref_b) < 5e-7, (cfg1, cfg2)
+    # shapes of the complex form only are refused while the parity form is on, and the
+    # recombining shapes on stage 1
     with pytest.raises(_lib.HcipyError):
         plan.set_tiles(_lib.FORWARD, 0, _lib.TILES_C64X64_W4)
+    with pytest.raises(_lib.HcipyError):
+        plan.set_tiles(_lib.FORWARD, 0, _lib.PARITY_STAGE2_TILE_CFGS[precision][0])
 
 
 def test_parity_availability_rules():
