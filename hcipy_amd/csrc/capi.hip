@@ -46,7 +46,7 @@ void grouped_tile_shape(int cfg, int* bm, int* bn);
 int grouped_real_cfg(int prec, int stage, int64_t tiles64);
 void real_tile_shape(int cfg, int* bm, int* bn);
 int launch_stage_grouped_real(int prec, int cfg, const StageArgs* table, const int64_t* tile_start,
-                              const int* tile_item, int n, int64_t tiles, hipStream_t s);
+                              const int* tile_item, int n, int64_t tiles, hipStream_t s, int nyq_k);
 constexpr int kModePadded = 0, kModeAligned = 1, kModeUnaligned = 2;  // BMode of mft_gemm.hip
 int launch_split_rows(int prec, const float* in, int64_t ld_in, int K, int64_t rows, SplitOut o, int64_t ld_out,
                       float* row_scale, hipStream_t s);
@@ -115,11 +115,14 @@ struct Plan {
   ParAxis px{}, py{}, pu{}, pv{};  // axis tables (device: plus, minus, P)
   void* ptables = nullptr;
   // Per direction: the last magnitude of the stage-1 output axis has one sample only (the -n/2
-  // sample of an FFT-style axis) and dropping its row saves a row tile.  The parity split then
-  // forms that row of the stage-1 product in FP64 from nyq_coef (cosines, then sines, of that
-  // magnitude times the contraction magnitudes) and stage 1 runs one row fewer.
+  // sample of an FFT-style axis) and dropping its row saves a row tile.  Stage 1 then runs one
+  // row fewer; that row (the cosines, then sines, of that magnitude times the contraction
+  // magnitudes, against the data) is formed in FP64 by the parity split of a single execution
+  // (nyq_coef64) or, in grouped launches, where one split may feed items with different output
+  // axes, by stage 1's first row tile (nyq_coef32).
   bool nyq[2] = {false, false};
-  double* nyq_coef[2] = {nullptr, nullptr};
+  double* nyq_coef64[2] = {nullptr, nullptr};
+  float* nyq_coef32[2] = {nullptr, nullptr};
 };
 
 int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, int nk, double sign,
@@ -403,7 +406,7 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
   if (!st && (only < 0 || only == 0)) {
     SplitOut fo{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
     st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch, fo, w.ldf, rs,
-                             nyq ? pl.nyq_coef[fw ? 0 : 1] : nullptr, nyq_vals, s);
+                             nyq ? pl.nyq_coef64[fw ? 0 : 1] : nullptr, nyq_vals, s);
   }
   if (!st) st = mark(1);
   if (st) return st;
@@ -521,7 +524,7 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
 }
 
 // Stage arguments of one direction in the parity form for a grouped launch (stage 1 with the
-// real constant, recombining stage 2; no extra Nyquist row).  The parity split is launched here
+// real constant, recombining stage 2).  The parity split is launched here
 // unless `shared` names an earlier item that split the same input with the same axis pairing;
 // stage 1 then reads that item's split data and row scales.
 int collect_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai, char* ws,
@@ -565,7 +568,12 @@ int collect_parity(const Plan& pl, int dir, const void* in, void* out, int64_t b
   a.a_zshift = 0;
   a.ldb = w.ldf;
   a.b_zstride = 2 * hn * w.ldf;
-  a.M = (int)hc1;
+  const bool nyq = pl.nyq[fw ? 0 : 1];
+  a.M = (int)(nyq ? hc1 - 1 : hc1);
+  if (nyq) {
+    a.nyq_coef = pl.nyq_coef32[fw ? 0 : 1];
+    a.nyq_row = (int)(hc1 - 1);
+  }
   a.N = (int)(2 * hn);
   a.K = (int)hk;
   a.rs_zstride = 2 * hn;
@@ -832,6 +840,7 @@ struct GroupedLaunch {
   int stage, b_mode, cfg, first, n;
   int64_t start_off, tiles;
   bool real = false;  // parity-form launch (launch_stage_grouped_real)
+  int nyq_k = 0;      // longest contraction of the items that form an extra stage-1 row
 };
 
 // Parity-form grouped launches: stage 1 of every item, then the recombining stage 2 of every item.
@@ -856,6 +865,7 @@ void build_parity_launches(int prec, const std::vector<Collected>& col, StageArg
       hargs[na] = a;
       hargs[na].tile_cfg = cfg;
       ++na;
+      if (a.nyq_coef != nullptr) L.nyq_k = std::max(L.nyq_k, a.K);
       hstart[ns++] = acc;
       acc += ceil_div(a.M, bm) * ceil_div(a.N, bn) * (stage == 0 ? c.nz1 : c.nz2);
       ++L.n;
@@ -1166,17 +1176,28 @@ int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const d
     if (!st) st = make_parity(pl, pl.pb2, pl.py.P, pl.py.h, pl.pv.P, pl.pv.h, s);
     // Nyquist rows of stage 1: forward output u (contraction x), backward output x (contraction u)
     const HostAxis* c1[2] = {&hu, &hx};
-    const ParAxis* pc1[2] = {&pl.pu, &pl.px};
-    const ParAxis* pk[2] = {&pl.px, &pl.pu};
+    const HostAxis* kk[2] = {&hx, &hu};
     for (int dir = 0; dir < 2 && !st; ++dir) {
-      if (!nyq_row_worth(*c1[dir])) continue;
-      const int hk = pk[dir]->h;
-      st = check_hip(hipMalloc(&pl.nyq_coef[dir], 2 * (size_t)hk * sizeof(double)), "hipMalloc(nyquist row)");
-      if (!st) {
-        pl.matrix_bytes += 2 * (size_t)hk * sizeof(double);
-        st = launch_gen_parity(pc1[dir]->P + (pc1[dir]->h - 1), 1, pk[dir]->P, hk, kF64,
-                               SplitOut{{pl.nyq_coef[dir], nullptr, nullptr}}, hk, 1, s);
+      if (!nyq_row_worth(*c1[dir]) || kk[dir]->P.size() > 4096) continue;
+      const size_t hk = kk[dir]->P.size();
+      std::vector<double> cd(2 * hk);
+      std::vector<float> cf(2 * hk);
+      const double pn = c1[dir]->P.back();
+      for (size_t j = 0; j < hk; ++j) {
+        cd[j] = std::cos(pn * kk[dir]->P[j]);
+        cd[hk + j] = std::sin(pn * kk[dir]->P[j]);
       }
+      for (size_t j = 0; j < 2 * hk; ++j) cf[j] = (float)cd[j];
+      st = check_hip(hipMalloc(&pl.nyq_coef64[dir], cd.size() * sizeof(double)), "hipMalloc(nyquist row)");
+      if (!st) st = check_hip(hipMalloc(&pl.nyq_coef32[dir], cf.size() * sizeof(float)), "hipMalloc(nyquist row)");
+      if (!st) {
+        pl.matrix_bytes += cd.size() * sizeof(double) + cf.size() * sizeof(float);
+        st = check_hip(hipMemcpy(pl.nyq_coef64[dir], cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice),
+                       "copy nyquist row");
+      }
+      if (!st)
+        st = check_hip(hipMemcpy(pl.nyq_coef32[dir], cf.data(), cf.size() * sizeof(float), hipMemcpyHostToDevice),
+                       "copy nyquist row");
       if (!st) pl.nyq[dir] = true;
     }
     pl.parity = true;
@@ -1199,7 +1220,9 @@ int hcipy_mft_plan_destroy(hcipy_mft_plan plan) {
   free_parity(pl.pb1);
   free_parity(pl.pb2);
   if (pl.ptables) (void)hipFree(pl.ptables);
-  for (double* c : pl.nyq_coef)
+  for (double* c : pl.nyq_coef64)
+    if (c) (void)hipFree(c);
+  for (float* c : pl.nyq_coef32)
     if (c) (void)hipFree(c);
   if (pl.coords) (void)hipFree(pl.coords);
   delete plan;
@@ -1528,7 +1551,7 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   const int64_t* dstart = reinterpret_cast<const int64_t*>(group->dev + round_up((int64_t)(2 * (size_t)group->cap * sizeof(StageArgs)), 64));
   for (const GroupedLaunch& L : launches) {
     const int st = L.real ? launch_stage_grouped_real(prec, L.cfg, dargs + L.first, dstart + L.start_off, nullptr, L.n,
-                                                      L.tiles, s)
+                                                      L.tiles, s, L.nyq_k)
                           : launch_stage_grouped(prec, L.cfg, L.b_mode, dargs + L.first, dstart + L.start_off, nullptr,
                                                  L.n, L.tiles, s);
     if (st != kOk) return st;

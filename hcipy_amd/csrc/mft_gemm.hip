@@ -196,7 +196,8 @@ __device__ inline u32x4_t expand_odd(u32x4_t v) {
 // alternates between two sets per k-tile, and the set of tile t-1 is flushed into FP64 while
 // the MFMAs of tile t run.
 // Body of the stage kernel for workgroup tile h (in the XCD-aware order) of batch slice z.
-template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
+// NYQK: the kernel may form an extra stage-1 row itself (StageArgs::nyq_coef; grouped launches).
+template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF, bool NYQK = false>
 __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int slot_off = 0) {
   using TR = PrecTraits<P>;
   using T = typename TR::T;
@@ -504,6 +505,39 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
     }
   };
 
+  // Extra output row (first row tile of a real split-output stage): data row r of the tile is
+  // shared by NQT threads, each summing NQK consecutive k of every k-tile in FP64.
+  constexpr bool NYQ_STORE = REAL && TR::SPLIT && !COMB;  // extra row from split-formed values
+  constexpr bool NYQ_OK = NYQ_STORE && NYQK;               // extra row formed here
+  constexpr int NQT = NTHR / BN;
+  constexpr int NQK = BK / NQT;
+  static_assert(!NYQ_OK || (NQT * NQK == BK && NQK % 8 == 0 && NQT <= 64), "extra-row thread layout");
+  const bool nyq_k = NYQ_OK && p.nyq_coef != nullptr && m0 == 0 && p.out_mode == kOutSplitT;
+  float* coef_s = reinterpret_cast<float*>(smem + 2 * BUF);
+  double nacc = 0.0;
+  auto nyq_tile = [&](int t, const T* base) __attribute__((always_inline)) {
+    if constexpr (NYQ_OK) {
+      const int r = tid / NQT;
+      const int kq = (tid % NQT) * NQK;
+      const T* row = base + ABUF + r * LDK + kq;
+      const float* cf = coef_s + t * BK + kq;
+#pragma unroll
+      for (int c = 0; c < NQK; c += 8) {
+        T v[NARR][8];
+#pragma unroll
+        for (int arr = 0; arr < NARR; ++arr)
+          *reinterpret_cast<u32x4_t*>(v[arr]) = *reinterpret_cast<const u32x4_t*>(row + arr * BN * LDK + c);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          double x = (double)(float)v[0][e];
+          if constexpr (NARR == 2) x += (double)(float)v[1][e] * kLoScale;
+          if constexpr (NARR == 3) x += (double)(float)v[1][e] + (double)(float)v[2][e];
+          nacc += (double)cf[c + e] * x;
+        }
+      }
+    }
+  };
+
   // One k-tile t = kb + D; D is a compile-time position inside the PF-unrolled loop.  The main
   // loop (guarded = false) only stores full k-tiles; the tail checks bounds and masks.
   // Order: tile t + 1 goes from registers to the other LDS buffer (no wave reads it during this
@@ -522,6 +556,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
       store_set(std::integral_constant<int, NXT>{}, std::integral_constant<bool, GUARDED>{}, t + 1, (t + 1) & 1);
     load_set(std::integral_constant<int, NXT>{}, t + 1 + PF);
     compute_tile(std::integral_constant<int, Q>{}, smem + (t & 1) * BUF);
+    if constexpr (NYQ_OK) {
+      if (nyq_k) nyq_tile(t, smem + (t & 1) * BUF);
+    }
     if constexpr (NSET == 2) flush(std::integral_constant<int, QP>{});
     else if constexpr (FE == 1) flush(std::integral_constant<int, 0>{});
     else if ((t + 1) % FE == 0) flush(std::integral_constant<int, 0>{});
@@ -569,11 +606,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
       __syncthreads();
     }
   }
+  if constexpr (NYQ_OK) {
+    if (nyq_k) {
+      // the extra constant row, zero-padded to whole k-tiles, behind the operand buffers
+      const float* cf = p.nyq_coef + (int64_t)((z >> p.a_zshift) & 1) * K;
+      for (int k = tid; k < nk * BK; k += NTHR) coef_s[k] = k < K ? cf[k] : 0.0f;
+    }
+  }
   __syncthreads();
 
-  // The extra output row (real split output only): stored by the first row tile with the same
-  // scales and split as the product rows, while the k loop of this workgroup is still ahead.
-  if constexpr (REAL && TR::SPLIT) {
+  // The extra output row formed before this kernel (FP64 values in p.nyq): stored by the first
+  // row tile with the same scales and split as the product rows.
+  if constexpr (NYQ_STORE) {
     if (p.nyq != nullptr && m0 == 0 && p.out_mode == kOutSplitT) {
       const double zs = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[z] : 1.0;
       const int nend = min(n0 + BN, N);
@@ -632,6 +676,30 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
           if constexpr (THREE) acc64[i][j][r] += x + (double)accy[i][j][r];
           else acc64[i][j][r] += x * kLoScale;
         }
+  }
+  if constexpr (NYQ_OK) {
+    if (nyq_k) {
+#pragma unroll
+      for (int off = NQT / 2; off >= 1; off >>= 1) nacc += __shfl_xor(nacc, off);
+      const int n = n0 + tid / NQT;
+      if (tid % NQT == 0 && n < N) {
+        double v = nacc / (double)p.row_scale[(int64_t)z * p.rs_zstride + n] * (double)s_t[n / p.fold - item0];
+        if (p.z_scale != nullptr) v /= (double)p.z_scale[z];
+        const int64_t idx = (int64_t)z * p.out_zstride + (int64_t)(2 * p.nyq_row + (n & 1)) * p.ldo + (n >> 1);
+        if constexpr (P == kBF16x3) {
+          const __bf16 q0 = f32_to_bf16((float)v);
+          const double r1 = v - (double)(float)q0;
+          const __bf16 q1 = f32_to_bf16((float)r1);
+          static_cast<__bf16*>(p.out)[idx] = q0;
+          static_cast<__bf16*>(p.out_lo)[idx] = q1;
+          static_cast<__bf16*>(p.out_lo2)[idx] = f32_to_bf16((float)(r1 - (double)(float)q1));
+        } else {
+          const _Float16 hq = f32_to_f16((float)v);
+          static_cast<_Float16*>(p.out)[idx] = hq;
+          static_cast<_Float16*>(p.out_lo)[idx] = f32_to_f16((float)((v - (double)hq) * (double)kLoInv));
+        }
+      }
+    }
   }
 
   // Reduce the WK partial sums through LDS (the operand buffers are free now).
@@ -863,7 +931,8 @@ __global__ __launch_bounds__(64 * WM * WN * WK) void mft_stage_grouped_kernel(co
   const int64_t local = b - tile_start[lo];
   const int h = (int)(local % tiles_xy);
   // the XCD slot of this item's tile h = 0 in this batch slice
-  stage_body<P, BM, BN, BK, WM, WN, WK, BMODE, PF>(p, h, (int)(local / tiles_xy), (int)((b - h) & 7));
+  stage_body<P, BM, BN, BK, WM, WN, WK, BMODE, PF, BMODE == kRealPadded>(p, h, (int)(local / tiles_xy),
+                                                                        (int)((b - h) & 7));
 }
 
 // Split-FP16 stage kernel with the constant operand kept compact.
@@ -2006,6 +2075,21 @@ __global__ __launch_bounds__(256) void parity_combine_kernel(const double2* __re
 // ---------------------------------------------------------------------------------------------
 // Host-side launchers.
 
+// LDS behind the operand buffers for the extra constant row of a real split-output stage (one
+// float per k, whole k-tiles): at most kNyqLdsMax, i.e. contractions of up to 4096 values.
+constexpr size_t kNyqLdsMax = 16 * 1024;
+inline size_t nyq_lds_bytes(int K, int BK) { return K > 0 ? (size_t)ceil_div(K, BK) * BK * sizeof(float) : 0; }
+
+template <int P, int BM, int BN, int BK, int BMODE>
+constexpr size_t stage_lds_max() {
+  using T = typename PrecTraits<P>::T;
+  constexpr int VEC = 16 / (int)sizeof(T);
+  constexpr size_t lds = 2 * (size_t)PrecTraits<P>::NARR * (BM + BN) * (BK + VEC) * sizeof(T);
+  constexpr size_t budget = 156 * 1024;  // one CU's LDS, less the kernel's static arrays
+  constexpr size_t want = lds + ((BMODE == kRealPadded && PrecTraits<P>::SPLIT) ? kNyqLdsMax : 0);
+  return want < budget ? want : (lds < budget ? budget : lds);
+}
+
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
 static int launch_stage_t(const StageArgs& a, int nz, hipStream_t s) {
   using T = typename PrecTraits<P>::T;
@@ -2024,6 +2108,7 @@ static int launch_stage_t(const StageArgs& a, int nz, hipStream_t s) {
   constexpr int TBN = BMODE == kRealComb ? 32 : BN;
   const int gx = (int)ceil_div(a.M, TBM);
   const int gy = (int)ceil_div(a.N, TBN);
+  HCIPY_REQUIRE(a.nyq_coef == nullptr, "single stage launches take split-formed extra rows only");
   hipLaunchKernelGGL(kern, dim3((unsigned)(gx * gy), 1, (unsigned)nz), dim3(64 * WM * WN * WK), lds, s, a);
   return kOk;
 }
@@ -2119,21 +2204,25 @@ static int launch_bf16x3(const StageArgs& a, int nz, hipStream_t s) {
 }
 
 template <int P, int BM, int BN, int BK, int WM, int WN, int WK, int BMODE, int PF>
-static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, const int* tile_item, int n, int64_t tiles, hipStream_t s) {
+static int launch_grouped_t(const StageArgs* table, const int64_t* tile_start, const int* tile_item, int n, int64_t tiles, hipStream_t s,
+                            int nyq_k = 0) {
   using T = typename PrecTraits<P>::T;
   constexpr int VEC = 16 / (int)sizeof(T);
   constexpr int NARR = PrecTraits<P>::NARR;
   constexpr size_t lds = 2 * (size_t)NARR * (BM + BN) * (BK + VEC) * sizeof(T);
+  constexpr size_t lds_max = stage_lds_max<P, BM, BN, BK, BMODE>();
   static_assert(lds <= 156 * 1024, "LDS budget of one CU (with the kernel's static arrays)");
   auto kern = mft_stage_grouped_kernel<P, BM, BN, BK, WM, WN, WK, BMODE, PF>;
   static bool configured = false;
   if (!configured) {
     HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
+                                  (int)lds_max));
     configured = true;
   }
   HCIPY_REQUIRE(tiles > 0 && tiles <= INT32_MAX, "grouped launch of %lld workgroups", (long long)tiles);
-  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * WM * WN * WK), lds, s, table, tile_start, tile_item, n);
+  const size_t extra = nyq_lds_bytes(nyq_k, BK);
+  HCIPY_REQUIRE(lds + extra <= lds_max, "extra constant row of %d values exceeds the LDS reserve", nyq_k);
+  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * WM * WN * WK), lds + extra, s, table, tile_start, tile_item, n);
   return check_hip(hipGetLastError(), "mft_stage_grouped_kernel launch");
 }
 
@@ -2180,9 +2269,10 @@ bool grouped_cfg_supported(int prec, int cfg) {
 bool is_comb_cfg(int cfg);
 
 // Grouped launches of the parity form: stage 1 (real constant) and the recombining stage 2.
+// Stage-1 shapes here leave room in LDS for an extra row of up to 4096 constants.
 int grouped_real_cfg(int prec, int stage, int64_t tiles64) {
   if (stage == 1) return prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h;
-  if (tiles64 < 128) return k64x32_w8;
+  if (tiles64 < 128) return prec == kF16x2 ? k64x32_w8 : k64x32_w2x1;
   return prec == kF16x2 ? kR64x64_w4 : k64x64_w2x2;
 }
 
@@ -2193,25 +2283,26 @@ void real_tile_shape(int cfg, int* bm, int* bn) {
     *bn = 32;
   } else {
     *bm = 64;
-    *bn = cfg == k64x32_w8 ? 32 : 64;
+    *bn = (cfg == k64x32_w8 || cfg == k64x32_w2x1) ? 32 : 64;
   }
 }
 
 int launch_stage_grouped_real(int prec, int cfg, const StageArgs* table, const int64_t* tile_start,
-                              const int* tile_item, int n, int64_t tiles, hipStream_t s) {
+                              const int* tile_item, int n, int64_t tiles, hipStream_t s, int nyq_k) {
   if (n <= 0 || tiles <= 0) return kOk;
   if (prec == kF16x2) {
     switch (cfg) {
-      case kR64x64_w4: return launch_grouped_t<kF16x2, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
-      case k64x32_w8: return launch_grouped_t<kF16x2, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+      case kR64x64_w4: return launch_grouped_t<kF16x2, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s, nyq_k);
+      case k64x32_w8: return launch_grouped_t<kF16x2, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s, nyq_k);
       case kRC32x32_w4: return launch_grouped_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
       case kRC32x32_h: return launch_grouped_t<kF16x2, 64, 128, 32, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
       default: break;
     }
   } else if (prec == kBF16x3) {
     switch (cfg) {
-      case k64x64_w2x2: return launch_grouped_t<kBF16x3, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
-      case k64x32_w8: return launch_grouped_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s);
+      case k64x64_w2x2: return launch_grouped_t<kBF16x3, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s, nyq_k);
+      case k64x32_w2x1: return launch_grouped_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s, nyq_k);
+      case k64x32_w8: return launch_grouped_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s, nyq_k);
       case kRC32x32_h: return launch_grouped_t<kBF16x3, 64, 128, 32, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
       default: break;
     }

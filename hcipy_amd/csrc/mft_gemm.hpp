@@ -166,11 +166,15 @@ struct StageArgs {
   int a_zshift;
   int64_t rs_zstride;
   int t_zstride;
-  // Real split output (kOutSplitT) with one more output row than the product has: when nyq is
-  // set, the workgroups of the first row tile also store row nyq_row from the FP64 values
-  // nyq[z * nyq_zstride + n] (unscaled results, formed by the kernel that split the data).
+  // Real split output (kOutSplitT) with one more output row than the product has (row nyq_row),
+  // stored by the workgroups of the first row tile with the product rows' scales and split:
+  //  - nyq set: from the FP64 values nyq[z * nyq_zstride + n] (unscaled results, formed by the
+  //    kernel that split the data);
+  //  - nyq_coef set: formed here, sum_k c[k] B[n][k] with the constant row c = nyq_coef +
+  //    ((z >> a_zshift) & 1) * K (K floats per plane), from the data tiles the workgroup stages.
   const double* nyq;
   int64_t nyq_zstride;
+  const float* nyq_coef;
   int nyq_row;
   // Parity stage 2 with the recombination in its epilogue (tile shapes kRC*): batch slice z is one
   // field whose four parity items are b_zstride apart; M = rows of the output axis cv, N = 2 x

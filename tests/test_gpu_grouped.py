@@ -76,3 +76,57 @@ def test_grouped_broadband_intensity_and_precision_check():
     with pytest.raises(ValueError):
         ha.MftGroup(4).execute([(m32, _lib.FORWARD, dev(E), 1.0, _lib.OUTPUT_FIELD),
                                 (m64, _lib.FORWARD, dev(E.astype(np.complex128)), 1.0, _lib.OUTPUT_FIELD)])
+
+
+@pytest.mark.parametrize('precision', ['f16x2', 'bf16x3'])
+def test_grouped_parity_form(precision):
+    """Grouped launches in the parity form (every item's plan runs it): one pupil field sent to
+    several focal grids (one shared parity split), FFT-style focal and pupil axes whose unpaired
+    row the first stage-1 row tile forms itself (a single call forms it in the split), both
+    directions, batches, field and intensity outputs; against the same transforms called one by
+    one (5e-7 of the peak, 1e-6 for intensities) and against the oracle."""
+    from oracle import hcipy_oracle as o
+    from helpers import og
+    calls, refs, oracle_refs = [], [], []
+    k = 200
+    grids = [(ha.make_pupil_grid(64), ha.make_focal_grid(4, 8)),                     # focal h = 33
+             (ha.make_pupil_grid(96), ha.make_focal_grid(4, 8)),
+             (ha.make_uniform_grid([128, 128], 1.0, has_center=True), ha.make_focal_grid(3, 10))]  # pupil h = 65
+    for gi, go0 in grids:
+        E = _rand((2, gi.size), k, np.complex64)
+        k += 1
+        for wl in (0.9, 1.0, 1.1):
+            go = go0.scaled(2 * np.pi / wl)
+            w_in = float(np.asarray(gi.weights).ravel()[0])
+            w_out = float(np.asarray(go.weights).ravel()[0]) / (2 * np.pi) ** 2
+            mft = ha.MatrixFourierTransform(gi, go, precision=precision)
+            assert mft.plan_for(torch.complex64, torch.device('cuda', 0)).parity
+            alpha = complex(0.2, -0.5 * wl)
+            x = dev(E)
+            for output in (_lib.OUTPUT_FIELD, _lib.OUTPUT_INTENSITY):
+                calls.append((mft, _lib.FORWARD, x, alpha, output))
+                refs.append(host(mft._execute_plain(x.contiguous(), _lib.FORWARD, alpha, output)))
+                # the oracle's transform carries the input weight, the plain execution alpha only
+                f = o.mft_forward(E.astype(np.complex128), og(gi), og(go)) * (alpha / w_in)
+                oracle_refs.append(f if output == _lib.OUTPUT_FIELD else np.abs(f) ** 2)
+            F = _rand((3, go.size), k, np.complex64)
+            k += 1
+            calls.append((mft, _lib.BACKWARD, dev(F), alpha, _lib.OUTPUT_FIELD))
+            refs.append(host(mft._execute_plain(dev(F), _lib.BACKWARD, alpha)))
+            oracle_refs.append(o.mft_backward(F.astype(np.complex128), og(gi), og(go)) * (alpha / w_out))
+    group = ha.MftGroup(64)
+    for rep in range(2):
+        outs = group.execute(calls)
+        worst = 0.0
+        for (_, _, _, _, output), out, r, ro in zip(calls, outs, refs, oracle_refs):
+            a = host(out)
+            if output == _lib.OUTPUT_FIELD:
+                e, eo = peak_rel_field(a, r), peak_rel_field(a, ro)
+                assert e < 5e-7 and eo < 5e-7
+            else:
+                a = a.astype(np.float64)
+                e = np.abs(a - r).max() / r.max()
+                eo = np.abs(a - ro).max() / ro.max()
+                assert e < 1e-6 and eo < 1e-6
+            worst = max(worst, e, eo)
+        say('grouped parity', precision, len(calls), 'items, pass', rep, 'worst error %.2e' % worst)

@@ -67,16 +67,18 @@ def test_parity_matches_oracle_random_fields(precision, dims_in, dims_out):
 
 
 @pytest.mark.parametrize('precision', ['f16x2', 'bf16x3'])
-@pytest.mark.parametrize('dims_in,dims_out', [([128, 96], [64, 90]), ([1024, 1024], [512, 512])])
+@pytest.mark.parametrize('dims_in,dims_out', [([128, 96], [64, 90]), ([2048, 64], [64, 64]),
+                                              ([1024, 1024], [512, 512])])
 def test_parity_fft_style_grids_with_nyquist_rows(precision, dims_in, dims_out):
     """FFT-style grids (a sample at the origin and an unpaired -n/2 sample) whose unpaired
-    magnitude would take a stage-1 row tile of its own (x axes of 128 and 1024 samples, u axes
-    of 64 and 512): the parity split forms that row of the stage-1 product in FP64 instead, in
-    both directions; the small case runs every parity tile shape."""
+    magnitude would take a stage-1 row tile of its own (x axes of 128, 2048 and 1024 samples, u
+    axes of 64 and 512): the parity split forms that row of the stage-1 product in FP64 instead
+    (over 1025 contraction values with 2048 x samples), in both directions.  The smaller cases run
+    every parity tile shape."""
     gi = ha.make_uniform_grid(dims_in, [1.0, 0.8], has_center=True)
     go = ha.make_uniform_grid(dims_out, [40.0, 30.0], has_center=True)
     _check_against_oracle(gi, go, precision, 'FFT-style %s->%s' % (dims_in, dims_out),
-                          all_tiles=dims_in[0] == 128)
+                          all_tiles=dims_in[0] != 1024)
 
 
 def _check_against_oracle(gi, go, precision, label, all_tiles=False):
