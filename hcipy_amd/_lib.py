@@ -106,6 +106,9 @@ SIGNATURES = {
     'hcipy_mft_plan_get_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     'hcipy_mft_plan_set_parity': (ctypes.c_int, [_vp, ctypes.c_int]),
     'hcipy_mft_plan_get_parity': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    'hcipy_mft_plan_set_fused': (ctypes.c_int, [_vp, ctypes.c_int]),
+    'hcipy_mft_plan_get_fused': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_uint)]),
     'hcipy_mft_group_create': (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
     'hcipy_mft_group_destroy': (ctypes.c_int, [_vp]),
     'hcipy_mft_execute_grouped': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),

@@ -62,6 +62,9 @@ int launch_gen_parity(const double* c, int nc, const double* k, int nk, int prec
                       int rows_pad, hipStream_t s);
 int launch_parity_combine(const double* H, const ParAxis& au, const ParAxis& av, int nu, int nv, double sign, double ar,
                           double ai, int out_mode, void* out, int64_t batch, hipStream_t s);
+int launch_parity_fused(const FusedArgs& f, int grid, hipStream_t s);
+int fused_split_teams(int64_t ld_out);
+bool fused_supported(int prec, int64_t ld_out);
 
 namespace {
 
@@ -72,6 +75,10 @@ constexpr int64_t kAlign = 256;
 constexpr int64_t kRowPad = 128;
 int64_t padded_row(int64_t k) { return round_up(k, kRowPad); }
 constexpr int kMaxGridZ = 65535;
+// The persistent single-launch parity execution (parity_fused_kernel) did not finish on MI355X:
+// two test launches (1024^2 -> 512^2, and 256^2 -> 128^2 with every dependency wait bounded to
+// 10 ms) ran until the test's time limit.  Off until the cause is found; plans never take it.
+constexpr bool kFusedEnabled = false;
 
 size_t elem_size(int prec) { return prec == kF64 ? 8 : (prec == kF32 ? 4 : 2); }
 int n_arrays(int prec) { return split_arrays(prec); }
@@ -123,6 +130,12 @@ struct Plan {
   bool nyq[2] = {false, false};
   double* nyq_coef64[2] = {nullptr, nullptr};
   float* nyq_coef32[2] = {nullptr, nullptr};
+  // Single executions of the parity form as one persistent launch (parity_fused_kernel): on by
+  // default where available.  `sync` is the launch's counter block (FusedArgs in mft_gemm.hpp),
+  // zero between executions; executions of one plan must therefore not overlap in time.
+  bool fused = false;
+  unsigned* sync = nullptr;
+  int cus = 0;  // compute units of the plan's device (persistent grid size)
 };
 
 int make_matrix(Plan& pl, FMatrix& m, const double* a, int nc, const double* b, int nk, double sign,
@@ -402,17 +415,9 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
   double* H = reinterpret_cast<double*>(ws + w.h);
   const bool nyq = pl.nyq[fw ? 0 : 1];
   double* nyq_vals = nyq ? reinterpret_cast<double*>(ws + w.nyq) : nullptr;
-  int st = mark(0);
-  if (!st && (only < 0 || only == 0)) {
-    SplitOut fo{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
-    st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch, fo, w.ldf, rs,
-                             nyq ? pl.nyq_coef64[fw ? 0 : 1] : nullptr, nyq_vals, s);
-  }
-  if (!st) st = mark(1);
-  if (st) return st;
-  constexpr int64_t kChunk = kMaxGridZ / 4;  // fields per launch (four grid.z slices each)
-  for (int64_t b0 = 0; b0 < batch && (only < 0 || only == 1); b0 += kChunk) {
-    const int nb = (int)std::min<int64_t>(kChunk, batch - b0);
+  // stage arguments for the fields b0 ... (stage 1: four items per field; stage 2: the
+  // recombining shape cfg2, or FP64 products otherwise)
+  auto stage1_args = [&](int64_t b0) {
     StageArgs a{};
     a.tile_cfg = pl.tiles[dir][0];
     a.a = p1.p[0];
@@ -449,44 +454,97 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
     a.t_zstride = 1;
     a.t_log2k = ilog2_ceil(hk);
     a.alpha_re = 1.0;
+    return a;
+  };
+  const int cfg2 = pl.tiles[dir][1] != kTileAuto ? pl.tiles[dir][1] : (pl.prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h);
+  auto stage2_comb_args = [&](int64_t b0) {
+    const size_t ob = out_elem_bytes(out_mode) * (size_t)d.c1 * d.c2;
+    StageArgs a{};
+    a.tile_cfg = cfg2;
+    a.a = p2.p[0];
+    a.a_lo = p2.p[1];
+    a.a_lo2 = p2.p[2];
+    a.lda = p2.lda;
+    a.a_zstride = p2.plane();
+    const int64_t gz = 2 * hc1 * w.ldg;
+    a.b = ws + w.g[0] + (size_t)b0 * 4 * gz * 2;
+    a.b_lo = na > 1 ? ws + w.g[1] + (size_t)b0 * 4 * gz * 2 : nullptr;
+    a.b_lo2 = na > 2 ? ws + w.g[2] + (size_t)b0 * 4 * gz * 2 : nullptr;
+    a.ldb = w.ldg;
+    a.b_zstride = gz;
+    a.M = (int)hc2;
+    a.N = (int)(2 * hc1);
+    a.K = (int)hn;
+    a.z_scale = t + b0 * 4;
+    a.out_mode = out_mode;
+    a.out = static_cast<char*>(out) + (size_t)b0 * ob;
+    a.cu = ac1;
+    a.cv = ac2;
+    a.c_nu = (int)d.c1;
+    a.c_nv = (int)d.c2;
+    a.c_sign = fw ? -1.0 : 1.0;
+    a.alpha_re = ar;
+    a.alpha_im = ai;
+    return a;
+  };
+  // One persistent launch for a whole execution, where the plan's tile shapes are the ones the
+  // fused kernel runs (64 x 64 stage-1 tiles with 64-deep k-tiles, recombining stage 2).
+  const int64_t m1 = nyq ? hc1 - 1 : hc1;
+  const int64_t tiles64 = ceil_div(m1, 64) * ceil_div(2 * hn, 64) * 4 * batch;
+  const int cfg1 = pl.tiles[dir][0] != kTileAuto ? pl.tiles[dir][0] : (tiles64 >= 128 ? kR64x64_w4 : k64x32_w8);
+  if (only < 0 && ev == nullptr && pl.fused && pl.sync != nullptr && fused_supported(pl.prec, w.ldf) &&
+      cfg1 == kR64x64_w4 && cfg2 == kRC32x32_w4 && m1 > 0 && ceil_div(m1, 64) <= ceil_div(std::max(pl.pu.h, pl.px.h), 64)) {
+    FusedArgs f{};
+    f.in = in;
+    f.nx = (int)d.k1;
+    f.ny = (int)d.n1;
+    f.ax = ak;
+    f.ay = an;
+    f.fo = SplitOut{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
+    f.ld_out = w.ldf;
+    f.row_scale = rs;
+    f.nyq_coef = nyq ? pl.nyq_coef64[fw ? 0 : 1] : nullptr;
+    f.nyq_out = nyq_vals;
+    f.nblk = (int)ceil_div(hn, fused_split_teams(w.ldf));
+    f.s1 = stage1_args(0);
+    f.s2 = stage2_comb_args(0);
+    const int64_t t1 = ceil_div(m1, 64) * ceil_div(2 * hn, 64);
+    const int64_t t2 = ceil_div(hc2, 32) * ceil_div(2 * hc1, 32);
+    f.n0 = batch * f.nblk;
+    f.n1 = 4 * batch * t1;
+    f.n2 = batch * t2;
+    HCIPY_REQUIRE(f.n1 < INT32_MAX && 4 * batch * ceil_div(2 * hn, 64) < UINT32_MAX, "fused execution of %lld fields",
+                  (long long)batch);
+    f.sync = pl.sync;
+    // one workgroup per compute unit (the stage-2 tiles take most of a CU's LDS), fewer when
+    // the busiest queue has fewer tasks
+    const int64_t per_queue = std::max(ceil_div(f.n0, 8), std::max(ceil_div(f.n1, 8), ceil_div(f.n2, 8)));
+    const int grid = (int)std::max<int64_t>(8, std::min<int64_t>(pl.cus / 8, per_queue) * 8);
+    return launch_parity_fused(f, grid, s);
+  }
+  int st = mark(0);
+  if (!st && (only < 0 || only == 0)) {
+    SplitOut fo{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
+    st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch, fo, w.ldf, rs,
+                             nyq ? pl.nyq_coef64[fw ? 0 : 1] : nullptr, nyq_vals, s);
+  }
+  if (!st) st = mark(1);
+  if (st) return st;
+  constexpr int64_t kChunk = kMaxGridZ / 4;  // fields per launch (four grid.z slices each)
+  for (int64_t b0 = 0; b0 < batch && (only < 0 || only == 1); b0 += kChunk) {
+    const int nb = (int)std::min<int64_t>(kChunk, batch - b0);
+    const StageArgs a = stage1_args(b0);
     if ((st = launch_stage_real(pl.prec, a, 4 * nb, s))) return st;
   }
   if ((st = mark(2))) return st;
   if (only >= 0 && only != 2) return mark(3);
   // default stage-2 shape of the parity form: the recombining one
-  const int cfg2 = pl.tiles[dir][1] != kTileAuto ? pl.tiles[dir][1] : (pl.prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h);
   if (is_comb_cfg(cfg2)) {
     // stage 2 with the recombination in its epilogue: one workgroup computes all four items of a
     // tile and writes the output quadrants
-    const size_t ob = out_elem_bytes(out_mode) * (size_t)d.c1 * d.c2;
     for (int64_t b0 = 0; b0 < batch; b0 += kChunk) {
       const int nb = (int)std::min<int64_t>(kChunk, batch - b0);
-      StageArgs a{};
-      a.tile_cfg = cfg2;
-      a.a = p2.p[0];
-      a.a_lo = p2.p[1];
-      a.a_lo2 = p2.p[2];
-      a.lda = p2.lda;
-      a.a_zstride = p2.plane();
-      const int64_t gz = 2 * hc1 * w.ldg;
-      a.b = ws + w.g[0] + (size_t)b0 * 4 * gz * 2;
-      a.b_lo = na > 1 ? ws + w.g[1] + (size_t)b0 * 4 * gz * 2 : nullptr;
-      a.b_lo2 = na > 2 ? ws + w.g[2] + (size_t)b0 * 4 * gz * 2 : nullptr;
-      a.ldb = w.ldg;
-      a.b_zstride = gz;
-      a.M = (int)hc2;
-      a.N = (int)(2 * hc1);
-      a.K = (int)hn;
-      a.z_scale = t + b0 * 4;
-      a.out_mode = out_mode;
-      a.out = static_cast<char*>(out) + (size_t)b0 * ob;
-      a.cu = ac1;
-      a.cv = ac2;
-      a.c_nu = (int)d.c1;
-      a.c_nv = (int)d.c2;
-      a.c_sign = fw ? -1.0 : 1.0;
-      a.alpha_re = ar;
-      a.alpha_im = ai;
+      const StageArgs a = stage2_comb_args(b0);
       if ((st = launch_stage_real(pl.prec, a, nb, s))) return st;
     }
     return mark(3);
@@ -1201,6 +1259,18 @@ int hcipy_mft_plan_create(hcipy_mft_plan* plan, int ndim, int precision, const d
       if (!st) pl.nyq[dir] = true;
     }
     pl.parity = true;
+    // The fused execution (parity_fused_kernel) is disabled: its launches did not finish on MI355X
+    // (kFusedEnabled).  With it enabled the plan would hold its counter block: fixed lines plus one
+    // per stage-1 row tile (64 rows of the larger of the two stage-1 output axes).
+    if (!st && precision == kF16x2 && kFusedEnabled) {
+      const int row_tiles = (int)ceil_div(std::max(pl.pu.h, pl.px.h), 64);
+      const size_t bytes = (size_t)(kSyncFixedLines + row_tiles) * kSyncLine * sizeof(unsigned);
+      st = check_hip(hipMalloc(&pl.sync, bytes), "hipMalloc(fused counters)");
+      if (!st) st = check_hip(hipMemset(pl.sync, 0, bytes), "clear fused counters");
+      if (!st) st = check_hip(hipDeviceGetAttribute(&pl.cus, hipDeviceAttributeMultiprocessorCount, pl.device),
+                              "hipDeviceGetAttribute(compute units)");
+      if (!st) pl.fused = pl.cus >= 8;
+    }
   }
   if (!st) st = check_hip(hipStreamSynchronize(s), "plan creation");
   if (st) return fail(st);
@@ -1220,6 +1290,7 @@ int hcipy_mft_plan_destroy(hcipy_mft_plan plan) {
   free_parity(pl.pb1);
   free_parity(pl.pb2);
   if (pl.ptables) (void)hipFree(pl.ptables);
+  if (pl.sync) (void)hipFree(pl.sync);
   for (double* c : pl.nyq_coef64)
     if (c) (void)hipFree(c);
   for (float* c : pl.nyq_coef32)
@@ -1253,6 +1324,35 @@ int hcipy_mft_plan_set_parity(hcipy_mft_plan plan, int enable) {
   if (pl.parity != (enable != 0)) {
     pl.parity = enable != 0;
     for (auto& d : pl.tiles) d[0] = d[1] = kTileAuto;  // tile shapes of the other form do not carry over
+  }
+  return kOk;
+}
+
+int hcipy_mft_plan_set_fused(hcipy_mft_plan plan, int enable) {
+  HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
+  Plan& pl = plan->p;
+  HCIPY_REQUIRE(!enable || kFusedEnabled,
+                "the fused parity execution is disabled in this build: its persistent launches did not finish "
+                "on MI355X (cause not found); single executions run as three launches");
+  HCIPY_REQUIRE(!enable || (pl.sync != nullptr && pl.cus >= 8),
+                "the fused parity execution needs a split-FP16 plan in which the parity form is available");
+  pl.fused = enable != 0;
+  return kOk;
+}
+
+int hcipy_mft_plan_get_fused(hcipy_mft_plan plan, int* available, int* enabled, unsigned* timeouts) {
+  HCIPY_REQUIRE(plan != nullptr, "plan is NULL");
+  const Plan& pl = plan->p;
+  if (available) *available = (pl.sync != nullptr && pl.cus >= 8) ? 1 : 0;
+  if (enabled) *enabled = pl.fused ? 1 : 0;
+  if (timeouts) {
+    *timeouts = 0;
+    if (pl.sync != nullptr) {
+      unsigned* line = pl.sync + 17 * kSyncLine;
+      HCIPY_HIP(hipDeviceSynchronize());
+      HCIPY_HIP(hipMemcpy(timeouts, line, sizeof(unsigned), hipMemcpyDeviceToHost));
+      if (*timeouts != 0) HCIPY_HIP(hipMemset(line, 0, sizeof(unsigned)));
+    }
   }
   return kOk;
 }

@@ -1873,12 +1873,12 @@ __device__ inline void split_store_d(const double* v, void* const* outs, int64_t
 // A workgroup of 256 threads takes 256 / TPR values of y' (TPR threads each); a thread holds the
 // four mirror samples of XPT consecutive x' in registers (XPT * TPR >= ld_out), so the field is
 // read once.
+// Body of the split for row block `blk` of `nblk` (one workgroup of 256 threads) of field b.
 template <int FMT, int XPT, int TPR>
-__global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restrict__ in, int nx, int ny, ParAxis ax,
-                                                           ParAxis ay, SplitOut o, int64_t ld_out,
-                                                           float* __restrict__ row_scale,
-                                                           const double* __restrict__ nyq_coef,
-                                                           double* __restrict__ nyq_out) {
+__device__ __forceinline__ void parity_split_body(const float2* __restrict__ in, int nx, int ny, const ParAxis& ax,
+                                                  const ParAxis& ay, const SplitOut& o, int64_t ld_out,
+                                                  float* __restrict__ row_scale, const double* __restrict__ nyq_coef,
+                                                  double* __restrict__ nyq_out, int blk, int nblk, int64_t b) {
   constexpr int SV = XPT % 4 == 0 ? 4 : 2;  // elements per store
   static_assert(XPT % SV == 0 && (TPR == 64 || TPR == 128 || TPR == 256), "thread layout");
   constexpr int TEAMS = 256 / TPR, WPT = TPR / 64;
@@ -1889,10 +1889,8 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
   // XCD-aware row order: consecutive workgroups go round-robin over the 8 XCDs, and XCD x takes
   // the x-th eighth of the y' rows, the rows the stage-1 workgroups on that XCD read (their tile
   // order gives each XCD a contiguous range of data rows)
-  const int nblk = gridDim.x;
-  const int rb = (nblk % 8 == 0) ? (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+  const int rb = (nblk % 8 == 0) ? (blk & 7) * (nblk >> 3) + (blk >> 3) : blk;
   const int yp = rb * TEAMS + team;
-  const int64_t b = blockIdx.y;
   const int hx = ax.h, hy = ay.h;
   const bool live = yp < hy;
   const int jp = live ? par_plus(ay, yp) : -1, jm = live ? par_minus(ay, yp) : -1;
@@ -2015,6 +2013,17 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
   }
 }
 
+// One workgroup per row block (x) and field (y).
+template <int FMT, int XPT, int TPR>
+__global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restrict__ in, int nx, int ny, ParAxis ax,
+                                                           ParAxis ay, SplitOut o, int64_t ld_out,
+                                                           float* __restrict__ row_scale,
+                                                           const double* __restrict__ nyq_coef,
+                                                           double* __restrict__ nyq_out) {
+  parity_split_body<FMT, XPT, TPR>(in, nx, ny, ax, ay, o, ld_out, row_scale, nyq_coef, nyq_out, blockIdx.x, gridDim.x,
+                                   blockIdx.y);
+}
+
 // Real parity matrices of one stage, two planes of rows_pad x lda: plane 0 cos(c[i] k[j]),
 // plane 1 sin(c[i] k[j]) for i < nc, j < nk (c, k: the positive halves of the row and
 // contraction coordinates), zero elsewhere; phase and sincos in FP64, stored in the operand
@@ -2070,6 +2079,142 @@ __global__ __launch_bounds__(256) void parity_combine_kernel(const double2* __re
   const double2* Hb = H + b * 4 * per + rem;
   parity_store(Hb[0], Hb[per], Hb[2 * per], Hb[3 * per], vp, up, au, av, nu, sign, ar, ai, out_mode, out,
                b * (int64_t)nv * nu);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The parity form as one persistent launch (FusedArgs in mft_gemm.hpp describes the task order
+// and the sync block).
+
+// Longest wait for a dependency, in ticks of the 100 MHz real-time counter (s_memrealtime): 10 ms,
+// some 300 times a whole 1024^2 -> 512^2 execution, so it only ends a wait that can never be
+// satisfied.  The bound is in time, not in polls: a poll's latency grows with the number of
+// workgroups polling the same line.
+constexpr uint64_t kWaitTicks = 1000000;
+
+// Tasks with linear index = q mod 8 among the first n of a level.
+__device__ inline int64_t queue_share(int64_t n, int q) { return n > q ? (n - q + 7) / 8 : 0; }
+
+// Control flow of the persistent loop is kept uniform: a branch on threadIdx.x == 0 around the
+// queue grab, the counter increments or the polls lets the compiler route the other lanes past it,
+// and with the loop's barriers that made lanes 1..63 of wave 0 and waves 1..3 return to the
+// barrier and reread the task index before thread 0 took the next task (the previous task repeated
+// forever).  So every lane of wave 0 runs these steps (wave 0 is selected by a scalar), each atomic
+// is issued by all 64 lanes with lane 0 alone adding 1, and loop conditions are readfirstlane'd.
+__device__ inline bool wave0() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0; }
+
+// Adds 1 to *c (all lanes of wave 0 call it); returns the value before the add.
+__device__ inline unsigned wave_add1(unsigned* c, int memorder) {
+  const unsigned one = (threadIdx.x & 63) == 0 ? 1u : 0u;
+  unsigned old;
+  if (memorder == __ATOMIC_RELEASE) old = __hip_atomic_fetch_add(c, one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  else old = __hip_atomic_fetch_add(c, one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_amdgcn_readfirstlane(old);  // lane 0's: the value before its own add
+}
+
+// The workgroup's stores become visible at agent scope (every XCD), then the counter is raised.
+__device__ inline void fused_signal(unsigned* c) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (wave0()) (void)wave_add1(c, __ATOMIC_RELEASE);
+}
+
+// Wave 0 polls until every counter c[i * kSyncLine] (i < nc) reaches target(i), then acquires at
+// agent scope; the barrier orders the workgroup's later loads after it.  A wait longer than
+// kWaitTicks counts a timeout in `err` and returns.  Polls back off (about 0.25 us apart).
+template <typename TargetFn>
+__device__ inline void fused_wait(const unsigned* c, int nc, TargetFn target, unsigned* err) {
+  if (wave0()) {
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    bool late = false;
+    for (int i = 0; i < nc && !late; ++i) {
+      const unsigned want = target(i);
+      for (;;) {
+        const unsigned seen = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(c + i * kSyncLine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (seen >= want) break;
+        if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+          late = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    if (late) (void)wave_add1(err, __ATOMIC_RELAXED);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Split-FP16 parity execution: split (FMT, XPT, TPR as the split kernel), stage 1 on 64 x 64 tiles
+// with 64-deep k-tiles, stage 2 recombining 32 x 32 tiles of the four items.
+template <int XPT, int TPR>
+__global__ __launch_bounds__(256) void parity_fused_kernel(FusedArgs f) {
+  constexpr int TEAMS = 256 / TPR;
+  (void)TEAMS;
+  const int q = blockIdx.x & 7;
+  unsigned* const sync = f.sync;
+  unsigned* const err = sync + 17 * kSyncLine;
+  const int gx1 = (f.s1.M + 63) / 64, gy1 = (f.s1.N + 63) / 64, t1 = gx1 * gy1;
+  const int gx2 = (f.s2.M + 31) / 32, gy2 = (f.s2.N + 31) / 32, t2 = gx2 * gy2;
+  const int64_t c0 = queue_share(f.n0, q), c1 = queue_share(f.n1, q), c2 = queue_share(f.n2, q);
+  const int64_t nfield = f.n2 / t2;
+  __shared__ unsigned s_task;
+  for (;;) {
+    if (wave0()) s_task = wave_add1(sync + q * kSyncLine, __ATOMIC_RELAXED);  // every lane stores the same
+    __syncthreads();
+    int64_t j = (int64_t)__builtin_amdgcn_readfirstlane(s_task);  // a scalar: the branches below are uniform
+    __syncthreads();
+    if (j < c0) {
+      const int64_t L = q + 8 * j;
+      const int64_t b = L / f.nblk;
+      parity_split_body<kSplitF16x2, XPT, TPR>(static_cast<const float2*>(f.in), f.nx, f.ny, f.ax, f.ay, f.fo,
+                                               f.ld_out, f.row_scale, f.nyq_coef, f.nyq_out, (int)(L - b * f.nblk),
+                                               f.nblk, b);
+      fused_signal(sync + (8 + q) * kSyncLine);
+    } else if ((j -= c0) < c1) {
+      // every split: the per-item scale of the intermediate is the minimum over all row scales
+      fused_wait(sync + 8 * kSyncLine, 8, [&](int i) { return (unsigned)queue_share(f.n0, i); }, err);
+      const int64_t L = q + 8 * j;
+      const int z = (int)(L / t1);
+      const int h = (int)(L - (int64_t)z * t1);
+      const int so = (int)((L - h) & 7);
+      stage_body<kF16x2, 64, 64, 64, 2, 2, 1, kRealPadded, 2>(f.s1, h, z, so);
+      int bx, by;
+      tile_of(h, gx1, gy1, bx, by, so);
+      fused_signal(sync + (kSyncFixedLines + bx) * kSyncLine);
+    } else if ((j -= c1) < c2) {
+      const int64_t L = q + 8 * j;
+      const int z = (int)(L / t2);
+      const int h = (int)(L - (int64_t)z * t2);
+      const int so = (int)((L - h) & 7);
+      int bx, by;
+      tile_of(h, gx2, gy2, bx, by, so);
+      // the stage-1 row tile holding this tile's u' (16 of them, inside one 64-row tile), and row
+      // tile 0, which writes the item scales and the extra row of an FFT-style axis
+      const int u0 = 16 * by;
+      const int rt = u0 < f.s1.M ? u0 / 64 : 0;
+      const unsigned want = (unsigned)(4 * nfield * gy1);
+      unsigned* const c = sync + kSyncFixedLines * kSyncLine;
+      fused_wait(c, 1, [&](int) { return want; }, err);
+      if (rt != 0) fused_wait(c + rt * kSyncLine, 1, [&](int) { return want; }, err);
+      stage_body<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(f.s2, h, z, so);
+      __syncthreads();
+    } else {
+      break;
+    }
+  }
+  // the last workgroup to finish clears the counters for the next execution
+  if (wave0()) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned done = wave_add1(sync + 16 * kSyncLine, __ATOMIC_RELAXED);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (done == gridDim.x - 1) {
+      for (int i = 0; i < 16; ++i) __hip_atomic_store(sync + i * kSyncLine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int i = 0; i < gx1; ++i)
+        __hip_atomic_store(sync + (kSyncFixedLines + i) * kSyncLine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(sync + 16 * kSyncLine, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2513,6 +2658,33 @@ int launch_parity_combine(const double* H, const ParAxis& au, const ParAxis& av,
   hipLaunchKernelGGL(parity_combine_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(H), au, av, nu, nv, sign, ar, ai, out_mode, out, batch);
   return check_hip(hipGetLastError(), "parity_combine_kernel launch");
+}
+
+// Split tasks per field of a fused execution: the split's thread layout for half rows of ld_out
+// reals (two y' per task where a team of 128 threads covers the row).
+int fused_split_teams(int64_t ld_out) { return ld_out <= 512 ? 2 : 1; }
+
+bool fused_supported(int prec, int64_t ld_out) { return prec == kF16x2 && ld_out <= 128 * 16; }
+
+int launch_parity_fused(const FusedArgs& f, int grid, hipStream_t s) {
+  using T = _Float16;
+  constexpr int VEC = 16 / (int)sizeof(T);
+  constexpr size_t lds1 = 2 * (size_t)2 * (64 + 64) * (64 + VEC) * sizeof(T);
+  constexpr size_t lds2 = 2 * (size_t)2 * (64 + 128) * (64 + VEC) * sizeof(T);
+  constexpr size_t lds = lds1 > lds2 ? lds1 : lds2;
+  static_assert(lds <= 150 * 1024, "LDS budget of one CU (with the static arrays of the three task kinds)");
+  HCIPY_REQUIRE(grid >= 8 && grid % 8 == 0, "fused parity launch of %d workgroups (a positive multiple of 8)", grid);
+  auto launch = [&](void (*kern)(FusedArgs)) -> int {
+    HCIPY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), lds, s, f);
+    return check_hip(hipGetLastError(), "parity_fused_kernel launch");
+  };
+  HCIPY_REQUIRE(fused_supported(kF16x2, f.ld_out), "fused parity form of %lld-real half rows", (long long)f.ld_out);
+  if (f.ld_out <= 256) return launch(parity_fused_kernel<2, 128>);
+  if (f.ld_out <= 512) return launch(parity_fused_kernel<4, 128>);
+  if (f.ld_out <= 1024) return launch(parity_fused_kernel<4, 256>);
+  return launch(parity_fused_kernel<8, 256>);
 }
 
 }  // namespace hcipy

@@ -236,6 +236,41 @@ struct SplitOut {
   void* p[3];
 };
 
+// DISABLED (capi.hip kFusedEnabled): launches of this kernel did not finish on MI355X, and plans
+// never take it; the code is kept for the investigation of that failure.
+// One execution of the parity form as a single persistent launch (parity_fused_kernel): the
+// parity split, stage 1 and the recombining stage 2 run as tasks of one kernel.  Tasks are
+// numbered level by level (every split task, then every stage-1 tile, then every stage-2 tile,
+// each level in the order its own launch would dispatch them), and workgroup b takes the tasks
+// with linear index = b mod 8 within each level from queue b mod 8, in order.  A task waits only
+// for tasks of lower levels, which are taken before it, so the launch finishes whatever number
+// of its workgroups are resident; the waits are bounded anyway (a timeout sets an error flag).
+//
+// Sync block (plan-owned device memory, one 128-byte line per counter, all zero between
+// executions: the last workgroup to finish clears them):
+//   line q (0..7)       next task of queue q
+//   line 8 + q          split tasks of queue q finished
+//   line 16             workgroups finished
+//   line 17             timeouts (sticky; read and cleared by the host)
+//   line 18 + r         stage-1 tiles of row tile r finished (all items, fields and column tiles)
+constexpr int kSyncLine = 32;  // unsigned per counter line
+constexpr int kSyncFixedLines = 18;
+struct FusedArgs {
+  const void* in;   // c64 fields [batch][ny][nx]
+  int nx, ny;
+  ParAxis ax, ay;   // input axes (stage-1 and stage-2 contractions)
+  SplitOut fo;      // split components, ld_out reals per row
+  int64_t ld_out;
+  float* row_scale;
+  const double* nyq_coef;  // extra stage-1 row formed by the split (or nullptr)
+  double* nyq_out;
+  int nblk;         // split tasks per field
+  StageArgs s1;     // stage 1 over 4 * batch slices
+  StageArgs s2;     // recombining stage 2 over batch slices
+  int64_t n0, n1, n2;  // tasks of each level over all fields
+  unsigned* sync;
+};
+
 // True if `cfg` is instantiated for precision `prec`.
 bool tile_cfg_supported(int prec, int cfg);
 

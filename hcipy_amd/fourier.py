@@ -215,6 +215,36 @@ class _MftPlan:
     def parity(self, enable):
         _lib.check(_lib.load().hcipy_mft_plan_set_parity(self.handle, 1 if enable else 0), 'hcipy_mft_plan_set_parity')
 
+    @property
+    def fused_available(self):
+        """True for split-FP16 plans with the parity form available: a single transform can then run
+        as one persistent launch (parity split, stage 1 and stage 2 as tasks of one kernel)."""
+        a = ctypes.c_int()
+        _lib.check(_lib.load().hcipy_mft_plan_get_fused(self.handle, ctypes.byref(a), None, None),
+                   'hcipy_mft_plan_get_fused')
+        return bool(a.value)
+
+    @property
+    def fused(self):
+        """Whether single transforms in the parity form run as one persistent launch (on by default
+        where available, for the default tile shapes)."""
+        e = ctypes.c_int()
+        _lib.check(_lib.load().hcipy_mft_plan_get_fused(self.handle, None, ctypes.byref(e), None),
+                   'hcipy_mft_plan_get_fused')
+        return bool(e.value)
+
+    @fused.setter
+    def fused(self, enable):
+        _lib.check(_lib.load().hcipy_mft_plan_set_fused(self.handle, 1 if enable else 0), 'hcipy_mft_plan_set_fused')
+
+    def fused_timeouts(self):
+        """Waits of the persistent launch that gave up since the last call (synchronises the device;
+        0 unless a launch could not make progress)."""
+        n = ctypes.c_uint()
+        _lib.check(_lib.load().hcipy_mft_plan_get_fused(self.handle, None, None, ctypes.byref(n)),
+                   'hcipy_mft_plan_get_fused')
+        return n.value
+
     def autotune(self, direction, x, out, batch, workspace, output=_lib.OUTPUT_FIELD, repeats=20):
         """Times every tile shape of each stage on these buffers and pins the fastest.
 

@@ -123,6 +123,22 @@ int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction, int stage, int*
 int hcipy_mft_plan_set_parity(hcipy_mft_plan plan, int enable);
 int hcipy_mft_plan_get_parity(hcipy_mft_plan plan, int* available, int* enabled);
 
+/* Fused parity execution: a single execution (hcipy_mft_execute[_ex], _noalloc) of an F16X2 plan
+ * in the parity form whose stage shapes are R64X64_W4 and RC32X32_W4 (the defaults at
+ * 1024^2 -> 512^2) runs as ONE persistent launch: the parity split, stage 1 and the recombining
+ * stage 2 are tasks of one kernel that wait for each other through device counters held by the
+ * plan, instead of three launches.  The results equal the three-launch path's (the same
+ * arithmetic per tile).  On by default where available; set_fused(plan, 0) keeps three launches.
+ * Because the counters belong to the plan, executions of one plan must not overlap in time
+ * (executions on one stream never do; two streams running one plan at once need two plans).
+ * get_fused reports availability (an F16X2 parity plan), the setting, and the number of waits
+ * that gave up since the last call (0 unless a launch could not make progress; such a launch's
+ * output is not valid).  get_fused synchronises the device when `timeouts` is not NULL.
+ * No counterpart in the reference; it replaces the launch sequence of
+ * hcipy/fourier/matrix_fourier_transform.py:171-172 (two zgemm/cgemm calls). */
+int hcipy_mft_plan_set_fused(hcipy_mft_plan plan, int enable);
+int hcipy_mft_plan_get_fused(hcipy_mft_plan plan, int* available, int* enabled, unsigned* timeouts);
+
 /* Autotuner: for each stage of `direction`, times every tile shape available for the plan's
  * precision on the caller's buffers (`repeats` runs each, HIP events around the stage kernel),
  * pins the fastest on the plan and writes it to chosen[stage] (chosen may be NULL; chosen[1] is
