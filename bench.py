@@ -320,6 +320,8 @@ def main():
         dmb = ha.DeformableMirror(IFb)
         acts = torch.as_tensor(np.random.default_rng(10).standard_normal(48 * 48) * 1e-9, device=dev)
         kernels['dm_surface_1024_48x48_f64'] = (lambda: IFb.linear_combination(acts), n * 8 + 48 * 48 * 8)
+        gsurf = torch.as_tensor(np.random.default_rng(11).standard_normal(n), device=dev)
+        kernels['dm_adjoint_1024_48x48_f64'] = (lambda: IFb.linear_combination_adjoint(gsurf), n * 8 + 48 * 48 * 8)
         for key, (fn, nbytes) in kernels.items():
             sec, _ = time_graph(fn, n_chunks=10, chunk=20)
             out[key] = {'us': round(sec * 1e6, 2), 'gbs': round(nbytes / sec / 1e9, 1), 'bytes': nbytes,

@@ -113,6 +113,59 @@ __global__ __launch_bounds__(256) void dm_surface_kernel(DmGeom g, const double*
   }
 }
 
+// Without a z tilt the transformed point is (px sx, py sy) and the Gaussian factorises,
+//   exp(-(ddx^2 + ddy^2) / (2 sigma^2)) = exp(-ddx^2 / (2 sigma^2)) exp(-ddy^2 / (2 sigma^2)),
+// so a point needs one exp per actuator row and one per actuator column in reach instead of one per
+// actuator; the support test compares the same ddx^2 + ddy^2 as the general kernels.  At most
+// kSepMax actuators per axis may lie within reach of a point (a support radius of up to 3 pitches,
+// the reference's default cutoff; else the general kernels run).
+constexpr int kSepMax = 8;
+
+bool separable(const DmGeom& g) { return g.sz == 0.0 && g.cz == 1.0 && 2.0 * g.radius / g.pitch + 2.0 <= kSepMax; }
+
+__global__ __launch_bounds__(256) void dm_surface_sep_kernel(DmGeom g, const double* __restrict__ w,
+                                                             double* __restrict__ surface, int64_t b_off) {
+  const int64_t npix = g.nx * g.ny;
+  const int64_t b = blockIdx.y + b_off;
+  const double* wb = w + b * g.n_act * g.n_act;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = p % g.nx, j = p / g.nx;
+    double acc = 0.0;
+    for (int v = 0; v < g.ovs; ++v) {
+      const double fy = ((v + 0.5) / g.ovs - 0.5) * g.dy;
+      const double qy = (g.y0 + (double)j * g.dy + fy) * g.sy;
+      const int64_t ja0 = max((int64_t)0, (int64_t)ceil((qy - g.radius - g.ay0) / g.pitch));
+      const int64_t ja1 = min(g.n_act - 1, (int64_t)floor((qy + g.radius - g.ay0) / g.pitch));
+      const int nja = (int)(ja1 - ja0 + 1);
+      double ey[kSepMax], dy2[kSepMax];
+#pragma unroll
+      for (int k = 0; k < kSepMax; ++k) {
+        const double dya = qy - (g.ay0 + (double)(ja0 + k) * g.pitch);
+        dy2[k] = dya * dya;
+        ey[k] = 0.0;
+        if (k < nja) ey[k] = exp(-dy2[k] * g.inv_two_sigma2);
+      }
+      for (int u = 0; u < g.ovs; ++u) {
+        const double fx = ((u + 0.5) / g.ovs - 0.5) * g.dx;
+        const double qx = (g.x0 + (double)i * g.dx + fx) * g.sx;
+        const int64_t ia0 = max((int64_t)0, (int64_t)ceil((qx - g.radius - g.ax0) / g.pitch));
+        const int64_t ia1 = min(g.n_act - 1, (int64_t)floor((qx + g.radius - g.ax0) / g.pitch));
+        for (int64_t ia = ia0; ia <= ia1; ++ia) {
+          const double dxa = qx - (g.ax0 + (double)ia * g.pitch);
+          const double dx2 = dxa * dxa;
+          if (dx2 > g.r2max) continue;
+          const double ex = exp(-dx2 * g.inv_two_sigma2);
+#pragma unroll
+          for (int k = 0; k < kSepMax; ++k) {
+            if (k < nja && dx2 + dy2[k] <= g.r2max) acc += wb[(ja0 + k) * g.n_act + ia] * (ex * ey[k] - g.offset);
+          }
+        }
+      }
+    }
+    surface[b * npix + p] = acc * g.norm;
+  }
+}
+
 // grad_w[b][a] = sum_pixel grad_s[b][pixel] IF_a(pixel): one workgroup per (actuator, batch item),
 // a fixed thread -> pixel assignment and a fixed reduction tree (deterministic).
 __global__ __launch_bounds__(256) void dm_adjoint_kernel(DmGeom g, const double* __restrict__ gs,
@@ -152,6 +205,80 @@ __global__ __launch_bounds__(256) void dm_adjoint_kernel(DmGeom g, const double*
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) gw[b * g.n_act * g.n_act + a] = (red[0] + red[1] + red[2] + red[3]) * g.norm;
+}
+
+// Separable adjoint (no z tilt): the per-axis factors exp(-ddx^2 / (2 sigma^2)) and squared
+// distances of the window's columns and rows, per dither, are tabulated in LDS first
+// (kSepLds doubles hold 2 ovs (wi + hj) of them), then every window pixel needs multiply-adds only.
+constexpr int kSepLds = 4096;
+
+__global__ __launch_bounds__(256) void dm_adjoint_sep_kernel(DmGeom g, const double* __restrict__ gs,
+                                                             double* __restrict__ gw, int64_t b_off) {
+  __shared__ double red[4];
+  __shared__ double tab[kSepLds];
+  const int64_t a = blockIdx.x;
+  const int64_t b = blockIdx.y + b_off;
+  const int64_t npix = g.nx * g.ny;
+  const double* gsb = gs + b * npix;
+  const double mx = g.ax0 + (double)(a % g.n_act) * g.pitch;
+  const double my = g.ay0 + (double)(a / g.n_act) * g.pitch;
+  // the same pixel window as the general adjoint (its centre without the rotation)
+  const double cx = mx / g.sx;
+  const double cy = my / g.sy;
+  const int64_t i0 = max((int64_t)0, (int64_t)floor((cx - g.radius - g.x0) / g.dx) - 1);
+  const int64_t i1 = min(g.nx - 1, (int64_t)ceil((cx + g.radius - g.x0) / g.dx) + 1);
+  const int64_t j0 = max((int64_t)0, (int64_t)floor((cy - g.radius - g.y0) / g.dy) - 1);
+  const int64_t j1 = min(g.ny - 1, (int64_t)ceil((cy + g.radius - g.y0) / g.dy) + 1);
+  double acc = 0.0;
+  const int wi = (int)(i1 - i0 + 1), hj = (int)(j1 - j0 + 1);
+  const int ovs = g.ovs;
+  if (wi > 0 && hj > 0 && 2 * ovs * (wi + hj) <= kSepLds) {
+    // tab: ex[u][wi], dx2[u][wi], ey[v][hj], dy2[v][hj]
+    double* ex = tab;
+    double* dx2 = ex + ovs * wi;
+    double* ey = dx2 + ovs * wi;
+    double* dy2 = ey + ovs * hj;
+    for (int t = threadIdx.x; t < ovs * wi; t += blockDim.x) {
+      const int u = t / wi, ii = t - u * wi;
+      const double fx = ((u + 0.5) / ovs - 0.5) * g.dx;
+      const double d = (g.x0 + (double)(i0 + ii) * g.dx + fx) * g.sx - mx;
+      dx2[t] = d * d;
+      ex[t] = exp(-dx2[t] * g.inv_two_sigma2);
+    }
+    for (int t = threadIdx.x; t < ovs * hj; t += blockDim.x) {
+      const int v = t / hj, jj = t - v * hj;
+      const double fy = ((v + 0.5) / ovs - 0.5) * g.dy;
+      const double d = (g.y0 + (double)(j0 + jj) * g.dy + fy) * g.sy - my;
+      dy2[t] = d * d;
+      ey[t] = exp(-dy2[t] * g.inv_two_sigma2);
+    }
+    __syncthreads();
+    const int64_t cnt = (int64_t)wi * hj;
+    for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+      const int jj = (int)(t / wi), ii = (int)(t - (int64_t)jj * wi);
+      double val = 0.0;
+      for (int v = 0; v < ovs; ++v) {
+        const double eyv = ey[v * hj + jj], dyv = dy2[v * hj + jj];
+        for (int u = 0; u < ovs; ++u) {
+          const double r2 = dx2[u * wi + ii] + dyv;
+          if (r2 <= g.r2max) val += ex[u * wi + ii] * eyv - g.offset;
+        }
+      }
+      acc += gsb[(j0 + jj) * g.nx + (i0 + ii)] * val;
+    }
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) gw[b * g.n_act * g.n_act + a] = (red[0] + red[1] + red[2] + red[3]) * g.norm;
+}
+
+// Every actuator's window fits the separable adjoint's tables: a window spans
+// ceil(A + 2R/d) - floor(A) + 3 <= 2R/d + 5 pixels per axis.
+bool adjoint_sep_fits(const DmGeom& g) {
+  const double wi = std::min(std::ceil(2.0 * g.radius / g.dx) + 6.0, (double)g.nx);
+  const double hj = std::min(std::ceil(2.0 * g.radius / g.dy) + 6.0, (double)g.ny);
+  return 2.0 * g.ovs * (wi + hj) <= kSepLds;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -528,8 +655,12 @@ int hcipy_dm_gaussian_surface(const hcipy_dm_gaussian_desc* desc, const double* 
   const int64_t npix = g.nx * g.ny;
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     const dim3 grid((unsigned)std::min<int64_t>(ceil_div(npix, 256), 8192), (unsigned)std::min<int64_t>(65535, batch - b0));
-    hipLaunchKernelGGL(dm_surface_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, actuators, surface,
-                       b0);
+    if (separable(g))
+      hipLaunchKernelGGL(dm_surface_sep_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, actuators,
+                         surface, b0);
+    else
+      hipLaunchKernelGGL(dm_surface_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, actuators,
+                         surface, b0);
     HCIPY_HIP(hipGetLastError());
   }
   return kOk;
@@ -545,8 +676,12 @@ int hcipy_dm_gaussian_surface_adjoint(const hcipy_dm_gaussian_desc* desc, const 
   const DmGeom g = make_geom(*desc);
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     const dim3 grid((unsigned)(g.n_act * g.n_act), (unsigned)std::min<int64_t>(65535, batch - b0));
-    hipLaunchKernelGGL(dm_adjoint_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, grad_surface,
-                       grad_actuators, b0);
+    if (separable(g) && adjoint_sep_fits(g))
+      hipLaunchKernelGGL(dm_adjoint_sep_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, grad_surface,
+                         grad_actuators, b0);
+    else
+      hipLaunchKernelGGL(dm_adjoint_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, grad_surface,
+                         grad_actuators, b0);
     HCIPY_HIP(hipGetLastError());
   }
   return kOk;

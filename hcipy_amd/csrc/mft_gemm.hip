@@ -802,8 +802,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
               static_cast<__bf16*>(p.out_lo2)[idx] = f32_to_bf16((float)(r1 - (double)(float)q1));
             } else {
               const _Float16 h = f32_to_f16((float)v);
-              static_cast<_Float16*>(p.out)[idx] = h;
-              static_cast<_Float16*>(p.out_lo)[idx] = f32_to_f16((float)((v - (double)h) * (double)kLoInv));
+              __builtin_nontemporal_store(h, static_cast<_Float16*>(p.out) + idx);
+              __builtin_nontemporal_store(f32_to_f16((float)((v - (double)h) * (double)kLoInv)),
+                                          static_cast<_Float16*>(p.out_lo) + idx);
             }
           }
         }
@@ -1843,8 +1844,8 @@ __device__ inline void split_store_d(const double* v, void* const* outs, int64_t
       vh[e] = q;
       vl[e] = f32_to_f16((float)((v[e] - (double)q) * (double)kLoInv));
     }
-    *reinterpret_cast<hv_t*>(static_cast<_Float16*>(outs[0]) + row_off + k) = vh;
-    *reinterpret_cast<hv_t*>(static_cast<_Float16*>(outs[1]) + row_off + k) = vl;
+    __builtin_nontemporal_store(vh, reinterpret_cast<hv_t*>(static_cast<_Float16*>(outs[0]) + row_off + k));
+    __builtin_nontemporal_store(vl, reinterpret_cast<hv_t*>(static_cast<_Float16*>(outs[1]) + row_off + k));
   } else {
     typedef __bf16 bv_t __attribute__((ext_vector_type(NV)));
     bv_t b0, b1, b2;

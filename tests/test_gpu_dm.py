@@ -22,6 +22,9 @@ CASES = [
     (64, 1.0, 8, (0.0, 0.0, 0.0), 1),
     (96, 1.5, 12, (np.radians(15), np.radians(10), np.radians(20)), 1),
     (80, 1.2, 10, (np.radians(5), np.radians(-8), np.radians(-30)), 3),
+    # no z tilt: the separable kernels (per-axis Gaussian factors), with x / y tilts and dithers
+    (128, 1.1, 16, (np.radians(12), np.radians(-7), 0.0), 2),
+    (192, 1.0, 24, (0.0, 0.0, 0.0), 1),
 ]
 
 
