@@ -456,7 +456,8 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
     a.alpha_re = 1.0;
     return a;
   };
-  const int cfg2 = pl.tiles[dir][1] != kTileAuto ? pl.tiles[dir][1] : (pl.prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h);
+  // default stage-2 shape: the recombining one, two waves per parity item for split-FP16
+  const int cfg2 = pl.tiles[dir][1] != kTileAuto ? pl.tiles[dir][1] : (pl.prec == kF16x2 ? kRC32x32_w8 : kRC32x32_h);
   auto stage2_comb_args = [&](int64_t b0) {
     const size_t ob = out_elem_bytes(out_mode) * (size_t)d.c1 * d.c2;
     StageArgs a{};

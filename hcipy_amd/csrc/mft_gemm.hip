@@ -210,8 +210,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   constexpr int NARR = TR::NARR;
   constexpr bool COMB = (BMODE == kRealComb);
   constexpr bool REAL = (BMODE == kRealPadded || COMB);
-  static_assert(!COMB || (BM == 64 && BN == 128 && WM == 2 && WN == 2 && WK == 1 && TR::SPLIT),
-                "recombining stage: 2 planes x 32 rows, 4 items x 32 rows, one wave per item");
+  static_assert(!COMB || (BM == 64 && BN == 128 && WM == 2 && WN == 2 && (WK == 1 || WK == 2) && TR::SPLIT),
+                "recombining stage: 2 planes x 32 rows, 4 items x 32 rows, one wave (or two, splitting the "
+                "k-vectors) per item");
   constexpr int BMC = REAL ? BM : BM / 2;  // constant rows loaded per tile (compact complex, or real)
   constexpr int WTM = BM / WM;
   constexpr int WTN = COMB ? 32 : BN / WN;
@@ -242,8 +243,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   const int wk = wave / (WM * WN);
   // recombining stage: wave w takes constant plane w >> 1 (cosines for ee / eo, sines for oe / oo)
   // and parity item w
-  const int wm = COMB ? (wave >> 1) : (wave % (WM * WN)) / WN;
-  const int wn = COMB ? wave : wave % WN;
+  const int wm = COMB ? ((wave & 3) >> 1) : (wave % (WM * WN)) / WN;
+  const int wn = COMB ? (wave & 3) : wave % WN;
   constexpr int TBM = COMB ? 32 : BM;  // output rows and columns per workgroup
   constexpr int TBN = COMB ? 32 : BN;
   const int gx = (p.M + TBM - 1) / TBM;
@@ -724,9 +725,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
           }
     }
     __syncthreads();
-    if (wk > 0) return;
+    // the recombining epilogue below has barriers: its k-split waves stay (and add nothing)
+    if constexpr (!COMB) {
+      if (wk > 0) return;
+    }
 #pragma unroll
     for (int g = 1; g < WK; ++g) {
+      if (COMB && wk > 0) break;
       const RT* srcp = red + ((g - 1) * (WM * WN) * 64 + slot) * NR;
       int q = 0;
 #pragma unroll
@@ -748,9 +753,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
     constexpr int HLD = 33;  // padded row of doubles
     static_assert((size_t)4 * 32 * HLD * sizeof(double) <= 2 * (size_t)BUF * sizeof(T), "products fit in LDS");
     double* hs = reinterpret_cast<double*>(smem_raw);
-    const double zs = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[4 * z + wave] : 1.0;
+    if (wk == 0) {  // the k-split waves' partial sums are in the first four waves' accumulators
+      const double zs = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[4 * z + wave] : 1.0;
 #pragma unroll
-    for (int r = 0; r < NACC; ++r) hs[(wave * 32 + acc_row<MT>(r, lane)) * HLD + lane % MT] = acc64[0][0][r] * zs;
+      for (int r = 0; r < NACC; ++r) hs[(wave * 32 + acc_row<MT>(r, lane)) * HLD + lane % MT] = acc64[0][0][r] * zs;
+    }
     __syncthreads();
     const int64_t ob = (int64_t)z * p.c_nu * p.c_nv;
     for (int e = tid; e < 32 * 16; e += NTHR) {
@@ -2417,6 +2424,7 @@ bool is_comb_cfg(int cfg);
 // Grouped launches of the parity form: stage 1 (real constant) and the recombining stage 2.
 // Stage-1 shapes here leave room in LDS for an extra row of up to 4096 constants.
 int grouped_real_cfg(int prec, int stage, int64_t tiles64) {
+  // one wave per item: with thousands of tiles the 8-wave shape measured 3 % slower (broadband)
   if (stage == 1) return prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h;
   if (tiles64 < 128) return prec == kF16x2 ? k64x32_w8 : k64x32_w2x1;
   return prec == kF16x2 ? kR64x64_w4 : k64x64_w2x2;
@@ -2442,6 +2450,7 @@ int launch_stage_grouped_real(int prec, int cfg, const StageArgs* table, const i
       case k64x32_w8: return launch_grouped_t<kF16x2, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s, nyq_k);
       case kRC32x32_w4: return launch_grouped_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
       case kRC32x32_h: return launch_grouped_t<kF16x2, 64, 128, 32, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
+      case kRC32x32_w8: return launch_grouped_t<kF16x2, 64, 128, 64, 2, 2, 2, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
       default: break;
     }
   } else if (prec == kBF16x3) {
@@ -2560,14 +2569,14 @@ bool real_cfg_supported(int prec, int cfg, int stage) {
   if (prec == kF16x2)
     return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8 || cfg == k64x32_w8 || cfg == k128x64 ||
            cfg == k64x128 || cfg == kR64x64_w4 || cfg == kR64x32_w4 || cfg == kR32x64_w4 ||
-           (stage == 1 && (cfg == kRC32x32_w4 || cfg == kRC32x32_h));
+           (stage == 1 && (cfg == kRC32x32_w4 || cfg == kRC32x32_h || cfg == kRC32x32_w8));
   if (prec == kBF16x3)
     return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8 ||
            (stage == 1 && cfg == kRC32x32_h);
   return false;
 }
 
-bool is_comb_cfg(int cfg) { return cfg == kRC32x32_w4 || cfg == kRC32x32_h; }
+bool is_comb_cfg(int cfg) { return cfg == kRC32x32_w4 || cfg == kRC32x32_h || cfg == kRC32x32_w8; }
 
 int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
   if (a.M <= 0 || a.N <= 0 || a.K <= 0 || nz <= 0) return kOk;
@@ -2589,6 +2598,7 @@ int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
       case kR32x64_w4: st = launch_stage_t<kF16x2, 32, 64, 64, 1, 2, 2, kRealPadded, 2>(a, nz, s); break;
       case kRC32x32_w4: st = launch_stage_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
       case kRC32x32_h: st = launch_stage_t<kF16x2, 64, 128, 32, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
+      case kRC32x32_w8: st = launch_stage_t<kF16x2, 64, 128, 64, 2, 2, 2, kRealComb, 2>(a, nz, s); break;
       default: set_last_error("tile configuration %d is not available to the parity form", cfg); return kInvalidValue;
     }
   } else if (prec == kBF16x3) {

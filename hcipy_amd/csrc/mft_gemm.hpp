@@ -228,7 +228,8 @@ enum TileCfg : int {
   // per workgroup (wave w: item w), recombined into the output quadrants in the epilogue
   kRC32x32_w4 = 19,  // 64-deep k-tiles (split-FP16 only)
   kRC32x32_h = 20,   // 32-deep k-tiles
-  kNumTileCfg = 21,
+  kRC32x32_w8 = 21,  // 64-deep k-tiles, 8 waves: two per item, each on half of every k-tile (split-FP16 only)
+  kNumTileCfg = 22,
 };
 
 // Up to three split arrays (device pointers), passed by value to the split and matrix kernels.

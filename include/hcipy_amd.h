@@ -104,6 +104,7 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
  * launch). */
 #define HCIPY_TILES_RC32X32_W4 19 /* 64-deep k-tiles */
 #define HCIPY_TILES_RC32X32_H 20  /* 32-deep k-tiles */
+#define HCIPY_TILES_RC32X32_W8 21 /* 64-deep k-tiles, 8 waves: two per item, each on half of every k-tile */
 int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction, int stage, int tile_cfg);
 int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction, int stage, int* tile_cfg);
 
