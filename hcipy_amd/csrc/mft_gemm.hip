@@ -777,6 +777,57 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   }
 
   const double inv_z = (p.z_scale != nullptr) ? 1.0 / (double)p.z_scale[z] : 1.0;
+  if constexpr (REAL && P == kF16x2 && WK == 1 && TM == 1 && TN == 1) {
+    // Split output of a parity stage 1 (kOutSplitT): element (m, n) goes to row 2m + (n & 1),
+    // column n >> 1 of the hi and lo arrays.  Written from registers that is one 2-byte store per
+    // accumulator and array, four 32-byte pieces of different rows per instruction; staged in LDS
+    // (the operand buffers are free), the tile's 2 BM rows of BN / 2 values leave in 16-byte pieces.
+    if (p.out_mode == kOutSplitT && BN % 16 == 0) {
+      constexpr int RW = BN / 2;     // values per output row of the tile
+      constexpr int LDR = RW + 8;    // padded LDS row (halves)
+      constexpr int PIECES = RW / 8;  // 16-byte pieces per row
+      static_assert((size_t)2 * 2 * BM * LDR * sizeof(_Float16) <= 2 * (size_t)BUF * sizeof(T), "staging fits in LDS");
+      __syncthreads();  // every wave is done with the operand buffers
+      _Float16* sh = reinterpret_cast<_Float16*>(smem_raw);
+      _Float16* sl = sh + 2 * BM * LDR;
+      const int nl = wn * WTN + lane % MT;
+      const int n = n0 + nl;
+      if (n < N) {
+        double sc = inv_z;
+        if (p.row_scale != nullptr) sc /= (double)p.row_scale[(int64_t)z * p.rs_zstride + n];
+        sc *= (double)s_t[n / p.fold - item0];
+#pragma unroll
+        for (int r = 0; r < NACC; ++r) {
+          const int ml = wm * WTM + acc_row<MT>(r, lane);
+          const double v = acc64[0][0][r] * sc;
+          const _Float16 h = f32_to_f16((float)v);
+          const int o = (2 * ml + (nl & 1)) * LDR + (nl >> 1);
+          sh[o] = h;
+          sl[o] = f32_to_f16((float)((v - (double)h) * (double)kLoInv));
+        }
+      }
+      __syncthreads();
+      const int yc0 = n0 >> 1;         // first output column of the tile
+      const int ycols = (N + 1) >> 1;  // output columns with data (N is even here)
+      for (int t = tid; t < 2 * 2 * BM * PIECES; t += NTHR) {
+        const int arr = t / (2 * BM * PIECES);
+        const int rem = t - arr * (2 * BM * PIECES);
+        const int rl = rem / PIECES, pc = rem - rl * PIECES;
+        const int m = m0 + (rl >> 1);
+        const int col = yc0 + pc * 8;
+        if (m >= M || col >= ycols) continue;
+        const _Float16* src = (arr ? sl : sh) + rl * LDR + pc * 8;
+        _Float16* dst = static_cast<_Float16*>(arr ? p.out_lo : p.out) + (int64_t)z * p.out_zstride +
+                        (int64_t)(2 * m + (rl & 1)) * p.ldo + col;
+        if (col + 8 <= ycols) {
+          __builtin_nontemporal_store(*reinterpret_cast<const u32x4_t*>(src), reinterpret_cast<u32x4_t*>(dst));
+        } else {
+          for (int e = 0; e < ycols - col; ++e) dst[e] = src[e];
+        }
+      }
+      return;
+    }
+  }
   if constexpr (REAL) {
     // Epilogue of a real constant: one real value per accumulator register.
 #pragma unroll
