@@ -40,7 +40,8 @@ def parse():
     ap.add_argument('--cpu-seconds', type=float, default=2.0, help='t_max of the CPU timing loop')
     ap.add_argument('--paths-only', action='store_true',
                     help='only time the angular-spectrum, Lyot-chain, vortex and mixed-size grouped paths')
-    ap.add_argument('--path', default=None, choices=['poke_jacobian', 'mixed_grouped', 'broadband_grouped', 'fft_2048'],
+    ap.add_argument('--path', default=None, choices=['poke_jacobian', 'mixed_grouped', 'broadband_grouped', 'fft_2048',
+                                                     'darkhole_gradient'],
                     help='only time this one workload (for profiler runs)')
     return ap.parse_args()
 
@@ -429,15 +430,75 @@ def main():
         torch.cuda.empty_cache()
         return res
 
+    def time_darkhole_gradient(reps=10):
+        """Differentiable dark-hole optimisation step (the broadband Lyot-chain gradient): the 1024^2
+        Lyot chain of workloads.lyot_1024 (D = 1 m, f = 1 m, lambda_0 = 500 nm, focal mask radius 3
+        lambda_0/D on 256^2, Lyot stop 0.95 D, final focal grid 512^2 at q = 4) at 5 wavelengths over
+        a 10 % band; a 48 x 48 Gaussian DM with heights N(0, 1 nm); loss = sum over the wavelengths
+        of the dark-zone (3 < r < 10 lambda_0/D, x > 1 lambda_0/D) intensity sum, FP64 reduction.
+        One step = DM surface, 5 chains forward, loss, backward to the 2304 actuator heights
+        (hand-written adjoint kernels behind torch.autograd), timed eagerly with the host work."""
+        lam0 = 500e-9
+        wls = lam0 * np.linspace(0.95, 1.05, 5)
+        pupil_l, _, _, E_l, fpm, stop, ap = workloads.lyot_1024()
+        fpm_grid = ha.make_focal_grid(32, 4, spatial_resolution=lam0)
+        focal_l = ha.make_focal_grid(4, 64, spatial_resolution=lam0)
+        xf, yf = focal_l.x / lam0, focal_l.y / lam0
+        r = np.hypot(xf, yf)
+        dz_np = (r > 3) & (r < 10) & (xf > 1)
+        dz = torch.as_tensor(dz_np, device=dev)
+        count = int(dz_np.sum())
+        phase0 = np.angle(E_l) * (ap > 0)
+        IFd = ha.make_gaussian_influence_functions(pupil_l, 48, 1.0 / 48)
+        heights = np.random.default_rng(5).standard_normal(48 * 48) * 1e-9
+        res = {}
+        for cdt, precision, key in ((torch.complex64, 'f16x2', 'f16x2'), (torch.complex128, None, 'fp64')):
+            fpm_f = ha.Field(torch.as_tensor(fpm, device=dev), fpm_grid)
+            stop_f = ha.Field(torch.as_tensor(stop, device=dev), pupil_l)
+            coro = ha.LyotCoronagraph(pupil_l, fpm_f, stop_f, focal_plane_mask_grid=fpm_grid, precision=precision)
+            prop_l = ha.FraunhoferPropagator(pupil_l, focal_l, precision=precision)
+            Es = {wl: torch.as_tensor(ap * np.exp(1j * phase0 * lam0 / wl), device=dev).to(cdt) for wl in wls}
+            dm = ha.DeformableMirror(IFd)
+
+            def step():
+                act = torch.as_tensor(heights, device=dev).requires_grad_(True)
+                dm.actuators = act
+                surface = dm.surface
+                total = 0
+                for wl in wls:
+                    Ep = ha.pointwise.multiply_phase(ha.Field(Es[wl], pupil_l), surface, 4 * np.pi / wl)
+                    img = prop_l.forward(coro.forward(ha.Wavefront(ha.Field(Ep, pupil_l), wl))).intensity
+                    total = total + ha.get_mean_intensity_in_roi(img, dz) * count
+                total.backward()
+                return act.grad
+
+            for _ in range(2):
+                step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                g = step()
+            torch.cuda.synchronize()
+            sec = (time.perf_counter() - t0) / reps
+            res[key] = {'ms_per_gradient': round(sec * 1e3, 3), 'gradients_per_s': round(1.0 / sec, 1),
+                        'grad_norm': float(g.norm().item())}
+        res['workload'] = ('1024^2 Lyot chain x 5 wavelengths, 48x48 Gaussian DM, dark-zone loss, gradient to '
+                           'the actuators (eager, host work included)')
+        res['f16x2_vs_fp64_rel_grad_norm_diff'] = abs(res['f16x2']['grad_norm'] - res['fp64']['grad_norm']) / \
+            res['fp64']['grad_norm']
+        return res
+
     if args.path is not None:
         fn = {'poke_jacobian': time_poke_jacobian, 'mixed_grouped': time_mixed_grouped,
-              'broadband_grouped': time_broadband, 'fft_2048': time_fft_2048}[args.path]
+              'broadband_grouped': time_broadband, 'fft_2048': time_fft_2048,
+              'darkhole_gradient': time_darkhole_gradient}[args.path]
         print(json.dumps({args.path: fn()}))
         return
     if args.paths_only:
         out = time_other_paths()
         out['mixed_grouped'] = time_mixed_grouped()
         out['poke_jacobian'] = time_poke_jacobian()
+        out['darkhole_gradient'] = time_darkhole_gradient()
         print(json.dumps(out))
         return
 
@@ -471,6 +532,7 @@ def main():
         paths = time_other_paths()
         paths['mixed_grouped'] = time_mixed_grouped()
         paths['poke_jacobian'] = time_poke_jacobian()
+        paths['darkhole_gradient'] = time_darkhole_gradient()
 
     # Roofline of the dominant kernel (stage 1: T = E . M2, 8 flop per complex MAC of the
     # reference's product, the per-unit count the headline is credited with).
