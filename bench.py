@@ -460,8 +460,10 @@ def main():
             Es = {wl: torch.as_tensor(ap * np.exp(1j * phase0 * lam0 / wl), device=dev).to(cdt) for wl in wls}
             dm = ha.DeformableMirror(IFd)
 
+            heights_dev = torch.as_tensor(heights, device=dev)
+
             def step():
-                act = torch.as_tensor(heights, device=dev).requires_grad_(True)
+                act = heights_dev.clone().requires_grad_(True)
                 dm.actuators = act
                 surface = dm.surface
                 total = 0
@@ -482,6 +484,32 @@ def main():
             sec = (time.perf_counter() - t0) / reps
             res[key] = {'ms_per_gradient': round(sec * 1e3, 3), 'gradients_per_s': round(1.0 / sec, 1),
                         'grad_norm': float(g.norm().item())}
+            # the same step (forward and backward) captured once in a HIP graph and replayed
+            try:
+                g_eager = g.clone()
+                s = torch.cuda.Stream()
+                s.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(s):
+                    step()
+                torch.cuda.current_stream().wait_stream(s)
+                torch.cuda.synchronize()
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    g_graph = step()
+                for _ in range(2):
+                    graph.replay()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    graph.replay()
+                torch.cuda.synchronize()
+                gsec = (time.perf_counter() - t0) / reps
+                res[key]['graph_ms_per_gradient'] = round(gsec * 1e3, 3)
+                res[key]['graph_gradients_per_s'] = round(1.0 / gsec, 1)
+                res[key]['graph_equals_eager'] = bool(torch.equal(g_graph, g_eager))
+                del graph
+            except Exception as ex:  # reported, not fatal: the eager numbers above stand
+                res[key]['graph_error'] = '%s: %s' % (type(ex).__name__, str(ex)[:200])
         res['workload'] = ('1024^2 Lyot chain x 5 wavelengths, 48x48 Gaussian DM, dark-zone loss, gradient to '
                            'the actuators (eager, host work included)')
         res['f16x2_vs_fp64_rel_grad_norm_diff'] = abs(res['f16x2']['grad_norm'] - res['fp64']['grad_norm']) / \

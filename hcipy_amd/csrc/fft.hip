@@ -264,7 +264,10 @@ struct MimoCfg {
   static constexpr int P = NY / E;
   static constexpr int LINE = hr_lds_points(NY) + 1;
   static constexpr int CF = (int)(71680 / ((size_t)LINE * 2 * sizeof(T)));
-  // the Q held transforms take registers: at most 256 threads (512 for one input)
+  // the Q held transforms take registers: at most 256 threads (512 for one input), except where
+  // one line alone needs more (P = 512 or 1024 at 8192 points): then one line per workgroup.  At
+  // 8192 points these kernels spill (36 to 952 bytes per lane in FP32, 628 to 1472 in FP64, most
+  // for 3-4 inputs): slower, not wrong.
   static constexpr int CT = (Q == 1 ? 512 : 256) / P > 0 ? (Q == 1 ? 512 : 256) / P : 1;
   static constexpr int C0 = CF < CT ? CF : CT;
   static constexpr int C = C0 > 16 ? 16 : (C0 < 1 ? 1 : C0);

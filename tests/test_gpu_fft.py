@@ -284,3 +284,28 @@ def test_filter_8192_point_lines(dims, dtype, tol):
     e = peak_rel_field(out, ref)
     say('8192-point filter %s %s: matrix %.2e' % (dims, dtype.__name__, e))
     assert e < tol
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.complex128, 1e-12), (np.complex64, 2e-6)])
+@pytest.mark.parametrize('shape', [(3, 3), (4, 4), (1, 3)])
+def test_filter_8192_point_matrix_shapes(shape, dtype, tol):
+    """Matrix transfer functions of 3 and 4 inputs on 8192-point columns (8 x 4096 input, q = 2:
+    16 x 8192), forward and adjoint, against the oracle: the column kernels that hold 3 or 4
+    transforms per thread there run with 8 points per thread and spill registers (slower, still
+    exact)."""
+    p, q = shape
+    g = ha.make_uniform_grid([8, 4096], [1.0, 4096 / 8])
+    internal = o.make_fft_grid(og(g), 2)
+    rng = np.random.default_rng(16)
+    f = (rng.standard_normal((q, g.size)) + 1j * rng.standard_normal((q, g.size))).astype(dtype)
+    tfm = rng.standard_normal((p, q, internal.size)) + 1j * rng.standard_normal((p, q, internal.size))
+    fm = ha.FourierFilter(g, dev(tfm), 2)
+    out = host(fm.forward(ha.Field(dev(f), g)))
+    ref = o.fourier_filter(f.astype(np.complex128), og(g), tfm, q=2)
+    assert out.shape == (p, g.size)
+    u = (rng.standard_normal((p, g.size)) + 1j * rng.standard_normal((p, g.size))).astype(dtype)
+    back = host(fm.backward(ha.Field(dev(u), g)))
+    refb = o.fourier_filter(u.astype(np.complex128), og(g), tfm, q=2, adjoint=True)
+    e, eb = peak_rel_field(out, ref), peak_rel_field(back, refb)
+    say('8192-point matrix filter %dx%d %s: fwd %.2e adj %.2e' % (p, q, dtype.__name__, e, eb))
+    assert e < tol and eb < tol
