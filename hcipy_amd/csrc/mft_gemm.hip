@@ -819,8 +819,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
         const _Float16* src = (arr ? sl : sh) + rl * LDR + pc * 8;
         _Float16* dst = static_cast<_Float16*>(arr ? p.out_lo : p.out) + (int64_t)z * p.out_zstride +
                         (int64_t)(2 * m + (rl & 1)) * p.ldo + col;
+        // ordinary stores: the 64-byte row pieces of horizontally adjacent tiles meet in L2 and leave
+        // as whole lines (G's compulsory 4.21 MB at 1024^2 -> 512^2; nontemporal stores wrote 4.82)
         if (col + 8 <= ycols) {
-          __builtin_nontemporal_store(*reinterpret_cast<const u32x4_t*>(src), reinterpret_cast<u32x4_t*>(dst));
+          *reinterpret_cast<u32x4_t*>(dst) = *reinterpret_cast<const u32x4_t*>(src);
         } else {
           for (int e = 0; e < ycols - col; ++e) dst[e] = src[e];
         }
@@ -2337,7 +2339,7 @@ static int launch_f16x2_dma_t(const StageArgs& a, int nz, hipStream_t s) {
 }
 
 bool tile_cfg_supported(int prec, int cfg) {
-  if (cfg < 0 || cfg >= kR64x64_w4) return false;
+  if (cfg < 0 || cfg >= kR64x64_w4) return false;  // the complex form's shapes end before the parity ones
   // BF16x3: three operand arrays per matrix; the shapes whose LDS buffers and thread split fit
   if (prec == kBF16x3) return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8;
   if (cfg >= kC64x64_w4) return prec == kF16x2;  // compact-constant shapes: split-FP16 only
@@ -2475,8 +2477,13 @@ bool is_comb_cfg(int cfg);
 // Grouped launches of the parity form: stage 1 (real constant) and the recombining stage 2.
 // Stage-1 shapes here leave room in LDS for an extra row of up to 4096 constants.
 int grouped_real_cfg(int prec, int stage, int64_t tiles64) {
-  // one wave per item: with thousands of tiles the 8-wave shape measured 3 % slower (broadband)
-  if (stage == 1) return prec == kF16x2 ? kRC32x32_w4 : kRC32x32_h;
+  // Stage 2: 32-deep k-tiles (61 KB of LDS, two workgroups per CU) over the 64-deep ones (110 KB,
+  // one per CU).  Grouped launches run thousands of short-contraction tiles, where the
+  // per-workgroup prologue and the recombining epilogue dominate and a second resident workgroup
+  // hides them: 20 % more propagations/s on 32 x 32 transforms 256^2 -> 128^2 (contraction 128),
+  // 5 % on mixed 256^2 / 512^2 / 1024^2 pupils.  Stage 1 keeps 64-deep k-tiles: its 32-deep
+  // shape (kR64x64_h) measured 1 % slower on the former and 3 % on the latter.
+  if (stage == 1) return kRC32x32_h;
   if (tiles64 < 128) return prec == kF16x2 ? k64x32_w8 : k64x32_w2x1;
   return prec == kF16x2 ? kR64x64_w4 : k64x64_w2x2;
 }
@@ -2502,6 +2509,7 @@ int launch_stage_grouped_real(int prec, int cfg, const StageArgs* table, const i
       case kRC32x32_w4: return launch_grouped_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
       case kRC32x32_h: return launch_grouped_t<kF16x2, 64, 128, 32, 2, 2, 1, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
       case kRC32x32_w8: return launch_grouped_t<kF16x2, 64, 128, 64, 2, 2, 2, kRealComb, 2>(table, tile_start, tile_item, n, tiles, s);
+      case kR64x64_h: return launch_grouped_t<kF16x2, 64, 64, 32, 2, 2, 1, kRealPadded, 2>(table, tile_start, tile_item, n, tiles, s, nyq_k);
       default: break;
     }
   } else if (prec == kBF16x3) {
@@ -2619,7 +2627,7 @@ int launch_gen_matrix(const double* a, int nc, const double* b, int nk, double s
 bool real_cfg_supported(int prec, int cfg, int stage) {
   if (prec == kF16x2)
     return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8 || cfg == k64x32_w8 || cfg == k128x64 ||
-           cfg == k64x128 || cfg == kR64x64_w4 || cfg == kR64x32_w4 || cfg == kR32x64_w4 ||
+           cfg == k64x128 || cfg == kR64x64_w4 || cfg == kR64x32_w4 || cfg == kR32x64_w4 || cfg == kR64x64_h ||
            (stage == 1 && (cfg == kRC32x32_w4 || cfg == kRC32x32_h || cfg == kRC32x32_w8));
   if (prec == kBF16x3)
     return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8 ||
@@ -2650,6 +2658,7 @@ int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
       case kRC32x32_w4: st = launch_stage_t<kF16x2, 64, 128, 64, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
       case kRC32x32_h: st = launch_stage_t<kF16x2, 64, 128, 32, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
       case kRC32x32_w8: st = launch_stage_t<kF16x2, 64, 128, 64, 2, 2, 2, kRealComb, 2>(a, nz, s); break;
+      case kR64x64_h: st = launch_stage_t<kF16x2, 64, 64, 32, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
       default: set_last_error("tile configuration %d is not available to the parity form", cfg); return kInvalidValue;
     }
   } else if (prec == kBF16x3) {

@@ -229,7 +229,10 @@ enum TileCfg : int {
   kRC32x32_w4 = 19,  // 64-deep k-tiles (split-FP16 only)
   kRC32x32_h = 20,   // 32-deep k-tiles
   kRC32x32_w8 = 21,  // 64-deep k-tiles, 8 waves: two per item, each on half of every k-tile (split-FP16 only)
-  kNumTileCfg = 22,
+  // Parity form, split-FP16: 64 x 64 tile, 2 x 2 waves of 32 x 32, 32-deep k-tiles (41 KB of LDS),
+  // a candidate for short contractions
+  kR64x64_h = 22,
+  kNumTileCfg = 23,
 };
 
 // Up to three split arrays (device pointers), passed by value to the split and matrix kernels.
