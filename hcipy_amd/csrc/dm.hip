@@ -83,14 +83,24 @@ __device__ inline double poke(const DmGeom& g, double qx, double qy, double mx, 
   return (r2 <= g.r2max) ? (exp(-r2 * g.inv_two_sigma2) - g.offset) : 0.0;
 }
 
+// Rows per workgroup of the surface kernels: a workgroup is 256 pixel columns (one per thread)
+// by kSurfRows rows, launched on a 2-D grid so that no thread divides a linear pixel index.  Four
+// rows give a 1024^2 pupil four resident waves per SIMD (eight rows: two, too few to hide the LDS
+// and FP64 latencies of the separable kernel).
+constexpr int kSurfRows = 4;
+
 // surface[b][pixel] = sum_a w[b][a] IF_a(pixel)
 __global__ __launch_bounds__(256) void dm_surface_kernel(DmGeom g, const double* __restrict__ w,
                                                          double* __restrict__ surface, int64_t b_off) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.nx) return;
   const int64_t npix = g.nx * g.ny;
-  const int64_t b = blockIdx.y + b_off;
+  const int64_t b = blockIdx.z + b_off;
   const double* wb = w + b * g.n_act * g.n_act;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = p % g.nx, j = p / g.nx;
+  const int64_t jb = (int64_t)blockIdx.y * kSurfRows;
+  const int rows = (int)min((int64_t)kSurfRows, g.ny - jb);
+  for (int r = 0; r < rows; ++r) {
+    const int64_t j = jb + r;
     double acc = 0.0;
     for (int v = 0; v < g.ovs; ++v)
       for (int u = 0; u < g.ovs; ++u) {
@@ -109,69 +119,145 @@ __global__ __launch_bounds__(256) void dm_surface_kernel(DmGeom g, const double*
           }
         }
       }
-    surface[b * npix + p] = acc * g.norm;
+    surface[b * npix + j * g.nx + i] = acc * g.norm;
   }
 }
 
 // Without a z tilt the transformed point is (px sx, py sy) and the Gaussian factorises,
 //   exp(-(ddx^2 + ddy^2) / (2 sigma^2)) = exp(-ddx^2 / (2 sigma^2)) exp(-ddy^2 / (2 sigma^2)),
-// so a point needs one exp per actuator row and one per actuator column in reach instead of one per
-// actuator; the support test compares the same ddx^2 + ddy^2 as the general kernels.  At most
-// kSepMax actuators per axis may lie within reach of a point (a support radius of up to 3 pitches,
-// the reference's default cutoff; else the general kernels run).
+// so the factors of a pixel column (per x dither and actuator column in reach) and of a pixel row
+// (per y dither and actuator row in reach) are computed once and every (pixel, actuator) pair needs
+// multiply-adds only; the support test compares the same ddx^2 + ddy^2 as the general kernels.  At
+// most kSepMax actuators per axis may lie within reach of a point (a support radius of up to 3
+// pitches, the reference's default cutoff; else the general kernels run).
 constexpr int kSepMax = 8;
+constexpr int kMaxOvs = 16;  // the descriptor's oversampling limit
 
 bool separable(const DmGeom& g) { return g.sz == 0.0 && g.cz == 1.0 && 2.0 * g.radius / g.pitch + 2.0 <= kSepMax; }
 
+// Actuator weights of a surface workgroup's reach staged in LDS (when they fit; else they are read
+// from global memory): a wave's 64 columns share a few actuator columns, which LDS serves as
+// broadcasts where vector loads would issue 64 addresses each.
+constexpr int kSurfWLds = 1024;
+
+// Thread = pixel column i of the workgroup's kSurfRows rows.  The row factors of those rows are
+// tabulated in LDS by the workgroup; the column factors stay in registers over the rows.
 __global__ __launch_bounds__(256) void dm_surface_sep_kernel(DmGeom g, const double* __restrict__ w,
                                                              double* __restrict__ surface, int64_t b_off) {
-  const int64_t npix = g.nx * g.ny;
-  const int64_t b = blockIdx.y + b_off;
+  __shared__ double s_ey[kSurfRows * kMaxOvs * kSepMax];
+  __shared__ double s_dy2[kSurfRows * kMaxOvs * kSepMax];
+  __shared__ int s_ja0[kSurfRows * kMaxOvs];
+  __shared__ int s_nja[kSurfRows * kMaxOvs];
+  __shared__ double s_w[kSurfWLds];
+  const int ovs = g.ovs;
+  const int64_t b = blockIdx.z + b_off;
+  const int64_t jb = (int64_t)blockIdx.y * kSurfRows;
+  const int rows = (int)min((int64_t)kSurfRows, g.ny - jb);
+  // entry (r, v, k): pixel row jb + r, y dither v, k-th actuator row in reach
+  for (int t = threadIdx.x; t < rows * ovs * kSepMax; t += 256) {
+    const int k = t % kSepMax, rv = t / kSepMax;
+    const int r = rv / ovs, v = rv - r * ovs;
+    const double fy = ((v + 0.5) / ovs - 0.5) * g.dy;
+    const double qy = (g.y0 + (double)(jb + r) * g.dy + fy) * g.sy;
+    const int64_t ja0 = max((int64_t)0, (int64_t)ceil((qy - g.radius - g.ay0) / g.pitch));
+    const int64_t ja1 = min(g.n_act - 1, (int64_t)floor((qy + g.radius - g.ay0) / g.pitch));
+    const double d = qy - (g.ay0 + (double)(ja0 + k) * g.pitch);
+    s_dy2[t] = d * d;
+    s_ey[t] = (ja0 + k <= ja1) ? exp(-d * d * g.inv_two_sigma2) : 0.0;
+    if (k == 0) {
+      s_ja0[rv] = (int)ja0;
+      s_nja[rv] = (int)(ja1 - ja0 + 1);
+    }
+  }
+  __syncthreads();
   const double* wb = w + b * g.n_act * g.n_act;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = p % g.nx, j = p / g.nx;
-    double acc = 0.0;
-    for (int v = 0; v < g.ovs; ++v) {
-      const double fy = ((v + 0.5) / g.ovs - 0.5) * g.dy;
-      const double qy = (g.y0 + (double)j * g.dy + fy) * g.sy;
-      const int64_t ja0 = max((int64_t)0, (int64_t)ceil((qy - g.radius - g.ay0) / g.pitch));
-      const int64_t ja1 = min(g.n_act - 1, (int64_t)floor((qy + g.radius - g.ay0) / g.pitch));
-      const int nja = (int)(ja1 - ja0 + 1);
-      double ey[kSepMax], dy2[kSepMax];
+  // actuators in reach of the workgroup: q grows with the pixel index and the dither, so the
+  // first column's first dither and the last column's last dither bound the actuator columns,
+  // the first and last table rows the actuator rows
+  const int64_t ib = (int64_t)blockIdx.x * 256;
+  const int64_t ie = min(ib + 255, g.nx - 1);
+  const double qx_lo = (g.x0 + (double)ib * g.dx + (0.5 / ovs - 0.5) * g.dx) * g.sx;
+  const double qx_hi = (g.x0 + (double)ie * g.dx + ((ovs - 0.5) / ovs - 0.5) * g.dx) * g.sx;
+  const int ia_lo = (int)max((int64_t)0, (int64_t)ceil((qx_lo - g.radius - g.ax0) / g.pitch));
+  const int ia_hi = (int)min(g.n_act - 1, (int64_t)floor((qx_hi + g.radius - g.ax0) / g.pitch));
+  const int ja_lo = max(s_ja0[0], 0);
+  const int ja_hi = min(s_ja0[rows * ovs - 1] + s_nja[rows * ovs - 1] - 1, (int)g.n_act - 1);
+  const int ldw = ia_hi - ia_lo + 1;
+  const bool w_lds = ldw > 0 && ja_hi >= ja_lo && ldw * (ja_hi - ja_lo + 1) <= kSurfWLds;
+  if (w_lds) {
+    for (int t = threadIdx.x; t < ldw * (ja_hi - ja_lo + 1); t += 256) {
+      const int rr = t / ldw, cc = t - rr * ldw;
+      s_w[t] = wb[(int64_t)(ja_lo + rr) * g.n_act + ia_lo + cc];
+    }
+  }
+  __syncthreads();
+  const int64_t i = ib + threadIdx.x;
+  if (i >= g.nx) return;
+  // two partial sums per row (even and odd actuator columns): independent FP64 chains
+  double acc[kSurfRows][2];
 #pragma unroll
-      for (int k = 0; k < kSepMax; ++k) {
-        const double dya = qy - (g.ay0 + (double)(ja0 + k) * g.pitch);
-        dy2[k] = dya * dya;
-        ey[k] = 0.0;
-        if (k < nja) ey[k] = exp(-dy2[k] * g.inv_two_sigma2);
-      }
-      for (int u = 0; u < g.ovs; ++u) {
-        const double fx = ((u + 0.5) / g.ovs - 0.5) * g.dx;
-        const double qx = (g.x0 + (double)i * g.dx + fx) * g.sx;
-        const int64_t ia0 = max((int64_t)0, (int64_t)ceil((qx - g.radius - g.ax0) / g.pitch));
-        const int64_t ia1 = min(g.n_act - 1, (int64_t)floor((qx + g.radius - g.ax0) / g.pitch));
-        for (int64_t ia = ia0; ia <= ia1; ++ia) {
-          const double dxa = qx - (g.ax0 + (double)ia * g.pitch);
-          const double dx2 = dxa * dxa;
-          if (dx2 > g.r2max) continue;
-          const double ex = exp(-dx2 * g.inv_two_sigma2);
+  for (int r = 0; r < kSurfRows; ++r) acc[r][0] = acc[r][1] = 0.0;
+  for (int u = 0; u < ovs; ++u) {
+    const double fx = ((u + 0.5) / ovs - 0.5) * g.dx;
+    const double qx = (g.x0 + (double)i * g.dx + fx) * g.sx;
+    const int64_t ia0 = max((int64_t)0, (int64_t)ceil((qx - g.radius - g.ax0) / g.pitch));
+    const int64_t ia1 = min(g.n_act - 1, (int64_t)floor((qx + g.radius - g.ax0) / g.pitch));
+    const int nia = (int)(ia1 - ia0 + 1);
+    double ex[kSepMax], dx2[kSepMax];
 #pragma unroll
-          for (int k = 0; k < kSepMax; ++k) {
-            if (k < nja && dx2 + dy2[k] <= g.r2max) acc += wb[(ja0 + k) * g.n_act + ia] * (ex * ey[k] - g.offset);
+    for (int c = 0; c < kSepMax; ++c) {
+      const double d = qx - (g.ax0 + (double)(ia0 + c) * g.pitch);
+      dx2[c] = d * d;
+      ex[c] = 0.0;
+      if (c < nia) ex[c] = exp(-dx2[c] * g.inv_two_sigma2);
+    }
+#pragma unroll
+    for (int r = 0; r < kSurfRows; ++r) {
+      if (r >= rows) break;
+      for (int v = 0; v < ovs; ++v) {
+        const int rv = r * ovs + v;
+        const int ja0 = s_ja0[rv], nja = s_nja[rv];
+        for (int k = 0; k < nja; ++k) {
+          const double eyk = s_ey[rv * kSepMax + k], dyk = s_dy2[rv * kSepMax + k];
+          if (w_lds) {
+            const double* wr = s_w + (ja0 + k - ja_lo) * ldw + (int)(ia0 - ia_lo);
+#pragma unroll
+            for (int c = 0; c < kSepMax; ++c) {
+              if (c < nia && dx2[c] + dyk <= g.r2max) acc[r][c & 1] += wr[c] * (ex[c] * eyk - g.offset);
+            }
+          } else {
+            const double* wr = wb + (int64_t)(ja0 + k) * g.n_act + ia0;
+#pragma unroll
+            for (int c = 0; c < kSepMax; ++c) {
+              if (c < nia && dx2[c] + dyk <= g.r2max) acc[r][c & 1] += wr[c] * (ex[c] * eyk - g.offset);
+            }
           }
         }
       }
     }
-    surface[b * npix + p] = acc * g.norm;
   }
+  const int64_t npix = g.nx * g.ny;
+#pragma unroll
+  for (int r = 0; r < kSurfRows; ++r)
+    if (r < rows) surface[b * npix + (jb + r) * g.nx + i] = (acc[r][0] + acc[r][1]) * g.norm;
+}
+
+// Actuator of adjoint workgroup h.  Workgroups are dispatched round-robin over the 8 XCDs, each
+// with its own L2; XCD x takes the x-th contiguous eighth of the actuators (whole actuator rows
+// when n % 8 == 0), so the windows its workgroups read cover a band of the pupil that its L2 can
+// hold, instead of the whole gradient.
+__device__ inline int64_t adjoint_actuator(int64_t h, int64_t n) {
+  const int64_t q = n >> 3, r = n & 7, x = h & 7;
+  return x * q + min(x, r) + (h >> 3);
 }
 
 // grad_w[b][a] = sum_pixel grad_s[b][pixel] IF_a(pixel): one workgroup per (actuator, batch item),
-// a fixed thread -> pixel assignment and a fixed reduction tree (deterministic).
+// a fixed thread -> pixel assignment (wave w takes window rows w, w + 4, ..., lane l the columns
+// l, l + 64, ...) and a fixed reduction tree (deterministic).
 __global__ __launch_bounds__(256) void dm_adjoint_kernel(DmGeom g, const double* __restrict__ gs,
                                                          double* __restrict__ gw, int64_t b_off) {
   __shared__ double red[4];
-  const int64_t a = blockIdx.x;
+  const int64_t a = adjoint_actuator(blockIdx.x, g.n_act * g.n_act);
   const int64_t b = blockIdx.y + b_off;
   const int64_t npix = g.nx * g.ny;
   const double* gsb = gs + b * npix;
@@ -185,38 +271,48 @@ __global__ __launch_bounds__(256) void dm_adjoint_kernel(DmGeom g, const double*
   const int64_t i1 = min(g.nx - 1, (int64_t)ceil((cx + g.radius - g.x0) / g.dx) + 1);
   const int64_t j0 = max((int64_t)0, (int64_t)floor((cy - g.radius - g.y0) / g.dy) - 1);
   const int64_t j1 = min(g.ny - 1, (int64_t)ceil((cy + g.radius - g.y0) / g.dy) + 1);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double acc = 0.0;
   if (i0 <= i1 && j0 <= j1) {
-    const int64_t wi = i1 - i0 + 1;
-    const int64_t cnt = wi * (j1 - j0 + 1);
-    for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
-      const int64_t i = i0 + t % wi, j = j0 + t / wi;
-      double val = 0.0;
-      for (int v = 0; v < g.ovs; ++v)
-        for (int u = 0; u < g.ovs; ++u) {
-          double qx, qy;
-          eval_point(g, i, j, u, v, qx, qy);
-          val += poke(g, qx, qy, mx, my);
-        }
-      acc += gsb[j * g.nx + i] * val;
+    const int wi = (int)(i1 - i0 + 1), hj = (int)(j1 - j0 + 1);
+    for (int jj = wave; jj < hj; jj += 4) {
+      const int64_t j = j0 + jj;
+      const double* row = gsb + j * g.nx + i0;
+      for (int ii = lane; ii < wi; ii += 64) {
+        double val = 0.0;
+        for (int v = 0; v < g.ovs; ++v)
+          for (int u = 0; u < g.ovs; ++u) {
+            double qx, qy;
+            eval_point(g, i0 + ii, j, u, v, qx, qy);
+            val += poke(g, qx, qy, mx, my);
+          }
+        acc += row[ii] * val;
+      }
     }
   }
   acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  if (lane == 0) red[wave] = acc;
   __syncthreads();
   if (threadIdx.x == 0) gw[b * g.n_act * g.n_act + a] = (red[0] + red[1] + red[2] + red[3]) * g.norm;
 }
 
 // Separable adjoint (no z tilt): the per-axis factors exp(-ddx^2 / (2 sigma^2)) and squared
-// distances of the window's columns and rows, per dither, are tabulated in LDS first
-// (kSepLds doubles hold 2 ovs (wi + hj) of them), then every window pixel needs multiply-adds only.
+// distances of the window's columns and rows, per dither, are tabulated in LDS first (`tab_len`
+// doubles of dynamic LDS hold the 2 ovs (wi + hj) of the largest window, at most kSepLds), then
+// every window pixel needs multiply-adds only.  Wave w takes the window rows in groups of
+// kAdjRows (rows 8w .. 8w + 7, then 32 further) and each group in pieces of kAdjCh x 64 columns,
+// whose kAdjRows x kAdjCh loads a lane issues before it uses any (the loads hit L2; one at a time
+// the wave waited out their latency in turn: 101 us at a 1024^2 pupil and 48 x 48 actuators, 53 us
+// with 8 x 1; 8 x 2 and 8 x 4 measured 59 and 78 us, their registers halving the resident waves).
 constexpr int kSepLds = 4096;
+constexpr int kAdjRows = 8;
+constexpr int kAdjCh = 1;
 
 __global__ __launch_bounds__(256) void dm_adjoint_sep_kernel(DmGeom g, const double* __restrict__ gs,
-                                                             double* __restrict__ gw, int64_t b_off) {
+                                                             double* __restrict__ gw, int64_t b_off, int tab_len) {
   __shared__ double red[4];
-  __shared__ double tab[kSepLds];
-  const int64_t a = blockIdx.x;
+  extern __shared__ double tab[];
+  const int64_t a = adjoint_actuator(blockIdx.x, g.n_act * g.n_act);
   const int64_t b = blockIdx.y + b_off;
   const int64_t npix = g.nx * g.ny;
   const double* gsb = gs + b * npix;
@@ -229,10 +325,11 @@ __global__ __launch_bounds__(256) void dm_adjoint_sep_kernel(DmGeom g, const dou
   const int64_t i1 = min(g.nx - 1, (int64_t)ceil((cx + g.radius - g.x0) / g.dx) + 1);
   const int64_t j0 = max((int64_t)0, (int64_t)floor((cy - g.radius - g.y0) / g.dy) - 1);
   const int64_t j1 = min(g.ny - 1, (int64_t)ceil((cy + g.radius - g.y0) / g.dy) + 1);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double acc = 0.0;
   const int wi = (int)(i1 - i0 + 1), hj = (int)(j1 - j0 + 1);
   const int ovs = g.ovs;
-  if (wi > 0 && hj > 0 && 2 * ovs * (wi + hj) <= kSepLds) {
+  if (wi > 0 && hj > 0 && 2 * ovs * (wi + hj) <= tab_len) {
     // tab: ex[u][wi], dx2[u][wi], ey[v][hj], dy2[v][hj]
     double* ex = tab;
     double* dx2 = ex + ovs * wi;
@@ -253,32 +350,52 @@ __global__ __launch_bounds__(256) void dm_adjoint_sep_kernel(DmGeom g, const dou
       ey[t] = exp(-dy2[t] * g.inv_two_sigma2);
     }
     __syncthreads();
-    const int64_t cnt = (int64_t)wi * hj;
-    for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
-      const int jj = (int)(t / wi), ii = (int)(t - (int64_t)jj * wi);
-      double val = 0.0;
-      for (int v = 0; v < ovs; ++v) {
-        const double eyv = ey[v * hj + jj], dyv = dy2[v * hj + jj];
-        for (int u = 0; u < ovs; ++u) {
-          const double r2 = dx2[u * wi + ii] + dyv;
-          if (r2 <= g.r2max) val += ex[u * wi + ii] * eyv - g.offset;
+    for (int jg = wave * kAdjRows; jg < hj; jg += 4 * kAdjRows) {
+      const double* row0 = gsb + (j0 + jg) * g.nx + i0;
+      for (int ib = 0; ib < wi; ib += 64 * kAdjCh) {
+        double gv[kAdjRows][kAdjCh];
+#pragma unroll
+        for (int q = 0; q < kAdjRows; ++q)
+#pragma unroll
+          for (int h = 0; h < kAdjCh; ++h) {
+            const int ii = ib + h * 64 + lane;
+            gv[q][h] = (jg + q < hj && ii < wi) ? row0[(int64_t)q * g.nx + ii] : 0.0;
+          }
+#pragma unroll
+        for (int q = 0; q < kAdjRows; ++q) {
+          if (jg + q >= hj) break;
+          const int jj = jg + q;
+#pragma unroll
+          for (int h = 0; h < kAdjCh; ++h) {
+            const int ii = ib + h * 64 + lane;
+            if (ii >= wi) break;
+            double val = 0.0;
+            for (int v = 0; v < ovs; ++v) {
+              const double eyv = ey[v * hj + jj], dyv = dy2[v * hj + jj];
+              for (int u = 0; u < ovs; ++u) {
+                const double r2 = dx2[u * wi + ii] + dyv;
+                if (r2 <= g.r2max) val += ex[u * wi + ii] * eyv - g.offset;
+              }
+            }
+            acc += gv[q][h] * val;
+          }
         }
       }
-      acc += gsb[(j0 + jj) * g.nx + (i0 + ii)] * val;
     }
   }
   acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  if (lane == 0) red[wave] = acc;
   __syncthreads();
   if (threadIdx.x == 0) gw[b * g.n_act * g.n_act + a] = (red[0] + red[1] + red[2] + red[3]) * g.norm;
 }
 
-// Every actuator's window fits the separable adjoint's tables: a window spans
-// ceil(A + 2R/d) - floor(A) + 3 <= 2R/d + 5 pixels per axis.
-bool adjoint_sep_fits(const DmGeom& g) {
+// Doubles of the separable adjoint's tables for the largest window (a window spans
+// ceil(A + 2R/d) - floor(A) + 3 <= 2R/d + 5 pixels per axis); every window fits when this is at
+// most kSepLds.
+double adjoint_sep_tab(const DmGeom& g) {
   const double wi = std::min(std::ceil(2.0 * g.radius / g.dx) + 6.0, (double)g.nx);
   const double hj = std::min(std::ceil(2.0 * g.radius / g.dy) + 6.0, (double)g.ny);
-  return 2.0 * g.ovs * (wi + hj) <= kSepLds;
+  return 2.0 * g.ovs * (wi + hj);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -652,9 +769,10 @@ int hcipy_dm_gaussian_surface(const hcipy_dm_gaussian_desc* desc, const double* 
   if (batch == 0) return kOk;
   HCIPY_REQUIRE(actuators != nullptr && surface != nullptr, "NULL pointer");
   const DmGeom g = make_geom(*desc);
-  const int64_t npix = g.nx * g.ny;
+  HCIPY_REQUIRE(ceil_div(g.ny, kSurfRows) <= 65535, "pupil grid has too many rows");
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
-    const dim3 grid((unsigned)std::min<int64_t>(ceil_div(npix, 256), 8192), (unsigned)std::min<int64_t>(65535, batch - b0));
+    const dim3 grid((unsigned)ceil_div(g.nx, 256), (unsigned)ceil_div(g.ny, kSurfRows),
+                    (unsigned)std::min<int64_t>(65535, batch - b0));
     if (separable(g))
       hipLaunchKernelGGL(dm_surface_sep_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, actuators,
                          surface, b0);
@@ -676,9 +794,10 @@ int hcipy_dm_gaussian_surface_adjoint(const hcipy_dm_gaussian_desc* desc, const 
   const DmGeom g = make_geom(*desc);
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     const dim3 grid((unsigned)(g.n_act * g.n_act), (unsigned)std::min<int64_t>(65535, batch - b0));
-    if (separable(g) && adjoint_sep_fits(g))
-      hipLaunchKernelGGL(dm_adjoint_sep_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, grad_surface,
-                         grad_actuators, b0);
+    const double tab = adjoint_sep_tab(g);
+    if (separable(g) && tab <= kSepLds)
+      hipLaunchKernelGGL(dm_adjoint_sep_kernel, grid, dim3(256), (size_t)tab * sizeof(double),
+                         static_cast<hipStream_t>(stream), g, grad_surface, grad_actuators, b0, (int)tab);
     else
       hipLaunchKernelGGL(dm_adjoint_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), g, grad_surface,
                          grad_actuators, b0);

@@ -25,6 +25,8 @@ CASES = [
     # no z tilt: the separable kernels (per-axis Gaussian factors), with x / y tilts and dithers
     (128, 1.1, 16, (np.radians(12), np.radians(-7), 0.0), 2),
     (192, 1.0, 24, (0.0, 0.0, 0.0), 1),
+    # two 256-column blocks and a partial block of 8 rows in the surface kernels
+    (268, 1.0, 10, (np.radians(6), 0.0, 0.0), 2),
 ]
 
 
