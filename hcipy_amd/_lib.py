@@ -48,6 +48,9 @@ TILES_RC32X32_W4 = 19
 TILES_RC32X32_H = 20
 TILES_RC32X32_W8 = 21
 TILES_R64X64_H = 22
+TILES_128X64_H = 23
+# tile shapes of the poke sets' grouped launches (hcipy_poke_set_autotune / hcipy_poke_set_set_tiles)
+POKE_TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 4, 6, 7), 'f16x2': (0, 1, 4, 6, 7, 23), 'bf16x3': (1, 4, 7)}
 TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(16)),
              'bf16x3': (0, 1, 4, 7)}
 # tile shapes of the parity form (hcipy_mft_plan_set_parity)
@@ -152,6 +155,7 @@ SIGNATURES = {
     'hcipy_poke_set_bytes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_poke_set_execute': (ctypes.c_int, [_vp, _vp]),
     'hcipy_poke_set_autotune': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp]),
+    'hcipy_poke_set_set_tiles': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     'hcipy_poke_set_info': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double)]),
     'hcipy_poke_set_destroy': (ctypes.c_int, [_vp]),

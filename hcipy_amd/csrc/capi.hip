@@ -2014,6 +2014,18 @@ int hcipy_poke_set_autotune(hcipy_poke_set set, int repeats, int* chosen, void* 
   return st;
 }
 
+int hcipy_poke_set_set_tiles(hcipy_poke_set set, int stage1_tiles, int stage2_tiles) {
+  HCIPY_REQUIRE(set != nullptr, "NULL poke set");
+  const int cfg[2] = {stage1_tiles, stage2_tiles};
+  for (int stage = 0; stage < 2; ++stage)
+    HCIPY_REQUIRE(cfg[stage] == kTileAuto || grouped_cfg_supported(set->prec, cfg[stage]),
+                  "tile configuration %d is not available to poke sets of this precision", cfg[stage]);
+  if (int st = upload_poke_tables(set, cfg)) return st;
+  set->cfg[0] = cfg[0];
+  set->cfg[1] = cfg[1];
+  return kOk;
+}
+
 int hcipy_poke_set_destroy(hcipy_poke_set set) {
   if (set == nullptr) return kOk;
   if (set->mem) (void)hipFree(set->mem);

@@ -2445,6 +2445,9 @@ static int launch_grouped_prec(int cfg, const StageArgs* table, const int64_t* t
     if (cfg == k64x32_w8) return launch_grouped_t<P, 64, 32, BK, 2, 1, 4, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
     if (cfg == k128x64) return launch_grouped_t<P, 128, 64, BK / 2, 2, 2, 1, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
   }
+  if constexpr (P == kF16x2) {
+    if (cfg == k128x64_h) return launch_grouped_t<P, 128, 64, 32, 2, 2, 1, BMODE, PF>(table, tile_start, tile_item, n, tiles, s);
+  }
   set_last_error("tile configuration %d is not available to grouped launches", cfg);
   return kInvalidValue;
 }
@@ -2469,6 +2472,7 @@ int grouped_tile_cfg(int prec, int stage, int max_k, int64_t tiles64) {
 bool grouped_cfg_supported(int prec, int cfg) {
   if (prec == kF64) return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8;
   if (prec == kBF16x3) return cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8;
+  if (prec == kF16x2 && cfg == k128x64_h) return true;
   return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k64x64_w8 || cfg == k64x32_w8 || cfg == k128x64;
 }
 
@@ -2526,7 +2530,7 @@ int launch_stage_grouped_real(int prec, int cfg, const StageArgs* table, const i
 }
 
 void grouped_tile_shape(int cfg, int* bm, int* bn) {
-  *bm = cfg == k128x64 ? 128 : 64;
+  *bm = (cfg == k128x64 || cfg == k128x64_h) ? 128 : 64;
   *bn = (cfg == k64x32_w2x1 || cfg == k64x32_w8) ? 32 : 64;
 }
 

@@ -232,7 +232,10 @@ enum TileCfg : int {
   // Parity form, split-FP16: 64 x 64 tile, 2 x 2 waves of 32 x 32, 32-deep k-tiles (41 KB of LDS),
   // a candidate for short contractions
   kR64x64_h = 22,
-  kNumTileCfg = 23,
+  // Complex form, split-FP16, grouped launches only: as k128x64 with 32-deep k-tiles (61 KB of
+  // LDS, two workgroups per CU instead of one) for launches of very many short contractions
+  k128x64_h = 23,
+  kNumTileCfg = 24,
 };
 
 // Up to three split arrays (device pointers), passed by value to the split and matrix kernels.

@@ -390,9 +390,10 @@ class PokeSet:
     field caused by the poke (the Jacobian column of actuator a times the poke height).  Memory
     (workspaces, launch tables) is allocated once here; executions allocate nothing and can be
     captured in HIP graphs.  ``autotune=True`` times every grouped tile shape for the two poke
-    launches here and keeps the fastest (``self.tiles``)."""
+    launches here and keeps the fastest (``self.tiles``); ``tiles=(stage1, stage2)`` sets them
+    instead (``_lib.POKE_TILE_CFGS`` lists the shapes of each precision)."""
 
-    def __init__(self, items, poke_height, device=None, autotune=False, output='intensity'):
+    def __init__(self, items, poke_height, device=None, autotune=False, output='intensity', tiles=None):
         self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
         lib = _lib.load()
         items = list(items)
@@ -449,6 +450,12 @@ class PokeSet:
         self.flops_windowed = fw.value  # real flops of one execution as run
         self.flops_full = ff.value      # the same results by full transforms of every poked field
         self.tiles = (_lib.TILES_AUTO, _lib.TILES_AUTO)
+        if tiles is not None:
+            if autotune:
+                raise ValueError('give either autotune=True or tiles, not both')
+            t1, t2 = (int(t) for t in tiles)
+            _lib.check(lib.hcipy_poke_set_set_tiles(h, t1, t2), 'hcipy_poke_set_set_tiles')
+            self.tiles = (t1, t2)
         if autotune:
             chosen = (ctypes.c_int * 2)()
             with torch.cuda.device(self.device):

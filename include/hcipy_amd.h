@@ -108,6 +108,10 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
 /* F16X2 parity form, either stage: 64x64 tile, 2x2 waves of 32x32, 32-deep k-tiles (41 KB of
  * LDS; a candidate for short contractions, e.g. in hcipy_mft_plan_set_tiles after timing) */
 #define HCIPY_TILES_R64X64_H 22
+/* F16X2 complex form, poke sets only (hcipy_poke_set_autotune, hcipy_poke_set_set_tiles): 128x64
+ * tile as HCIPY_TILES_128X64 with 32-deep k-tiles, two workgroups per CU, for very many short
+ * contractions */
+#define HCIPY_TILES_128X64_H 23
 int hcipy_mft_plan_set_tiles(hcipy_mft_plan plan, int direction, int stage, int tile_cfg);
 int hcipy_mft_plan_get_tiles(hcipy_mft_plan plan, int direction, int stage, int* tile_cfg);
 
@@ -432,6 +436,11 @@ int hcipy_poke_set_execute(hcipy_poke_set set, void* stream);
  * writes them to chosen[0..1] (may be NULL).  Runs executions (the outputs are overwritten);
  * refuses a capturing stream.  No counterpart in the reference. */
 int hcipy_poke_set_autotune(hcipy_poke_set set, int repeats, int* chosen, void* stream);
+/* Tile shapes of the stage-1 and stage-2 launches of the pokes (HCIPY_TILES_*, or HCIPY_TILES_AUTO
+ * for the default), any shape hcipy_poke_set_autotune may choose; HCIPY_ERROR_INVALID_VALUE for
+ * others.  Uploads the set's launch tables with synchronous copies: not under stream capture.  No
+ * counterpart in the reference. */
+int hcipy_poke_set_set_tiles(hcipy_poke_set set, int stage1_tiles, int stage2_tiles);
 int hcipy_poke_set_destroy(hcipy_poke_set set);
 
 /* Tikhonov-regularised inverse of a real m x n matrix M (row-major, device, FP64):

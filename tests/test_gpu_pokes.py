@@ -69,6 +69,33 @@ def test_poke_set_matches_oracle_pokes(precision, cdt, tol):
     say('poke set', precision, 'worst peak-relative error %.3e' % worst)
 
 
+@pytest.mark.parametrize('precision', ['f16x2', 'bf16x3'])
+def test_poke_set_every_tile_shape(precision):
+    """Every tile shape the poke set's launches can take (each on both stages) gives the oracle's
+    pokes; shapes of other precisions are refused."""
+    from hcipy_amd import _lib
+    items, oin, h = _small_items(torch.complex64, precision)
+    items, oin = items[3:4], oin[3:4]  # the 256^2 pupil at the first wavelength
+    E, pupil, focal, wl, n_act = oin[0]
+    acts = [0, 17, 136, 255]
+    b_ref = np.abs(o.fraunhofer_forward(E, og(pupil), og(focal), wl)) ** 2
+    ref = _oracle_poke_intensity(E, pupil, focal, wl, n_act, 1.0 / n_act, h, acts)
+    for cfg in _lib.POKE_TILE_CFGS[precision]:
+        ps = ha.PokeSet(items, h, tiles=(cfg, cfg))
+        base, pokes = ps.execute()
+        torch.cuda.synchronize()
+        peak = b_ref.max()
+        eb = np.abs(host(base[0]) - b_ref).max() / peak
+        ep = np.abs(host(pokes[0])[acts] - ref).max() / peak
+        say('poke set', precision, 'tiles', cfg, 'errors %.3e %.3e' % (eb, ep))
+        assert eb < 5e-7 and ep < 5e-7, (cfg, eb, ep)
+        ps.close()
+    other = [c for c in _lib.POKE_TILE_CFGS['f16x2'] if c not in _lib.POKE_TILE_CFGS[precision]]
+    if other:
+        with pytest.raises(_lib.HcipyError):
+            ha.PokeSet(items, h, tiles=(other[0], other[0]))
+
+
 def test_poke_set_reexecutes_on_updated_fields_and_replays_in_a_graph():
     items, oin, h = _small_items(torch.complex64, 'f16x2')
     ps = ha.PokeSet(items, h, autotune=True)
