@@ -571,111 +571,118 @@ struct PokeWinItemDev {
   float* row_scale;      // [actuator][wy] (split formats)
 };
 
+// Window rows per wave of the poke window kernel.  Window heights are multiples of 64, so an
+// item's rows and an actuator's rows come in whole groups: one item search per group.
+constexpr int kPokeRows = 8;
+
 template <int NPL>  // window pixels per lane at most (wx <= 64 NPL for every item)
 __global__ __launch_bounds__(256) void poke_window_kernel(const PokeWinItemDev* __restrict__ items, int n_items,
                                                           int64_t total_rows) {
   const int lane = threadIdx.x & 63;
-  const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (R >= total_rows) return;
+  const int64_t R0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kPokeRows;
+  if (R0 >= total_rows) return;
   int lo = 0, hi = n_items - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].row0 <= R) lo = mid;
+    if (items[mid].row0 <= R0) lo = mid;
     else hi = mid - 1;
   }
   const PokeWinItemDev& it = items[lo];
   const DmGeom& g = it.g;
   const int w = it.wx;
-  const int64_t local = R - it.row0;
-  const int64_t a = local / it.wy;
-  const int r = (int)(local - a * it.wy);
+  const int64_t local0 = R0 - it.row0;
+  const int64_t a = local0 / it.wy;
+  const int r0 = (int)(local0 - a * it.wy);
   const int64_t x0 = it.org[2 * a], y0 = it.org[2 * a + 1];
-  const int64_t j = y0 + r;
   const double mx = g.ax0 + (double)(a % g.n_act) * g.pitch;
   const double my = g.ay0 + (double)(a / g.n_act) * g.pitch;
-  // the row's values stay in registers: pixel c = lane + 64 t of the window
-  double2 val[NPL];
-  double mxv = 0.0;
+  for (int rr = 0; rr < kPokeRows; ++rr) {
+    const int r = r0 + rr;
+    const int64_t j = y0 + r;
+    // the row's values stay in registers: pixel c = lane + 64 t of the window
+    double2 val[NPL];
+    double mxv = 0.0;
 #pragma unroll
-  for (int t = 0; t < NPL; ++t) {
-    const int c = lane + 64 * t;
-    const int64_t i = x0 + c;
-    double2 d = make_double2(0.0, 0.0);
-    if (c < w && i < g.nx && j < g.ny) {
-      double ifv = 0.0;
-      for (int v = 0; v < g.ovs; ++v)
-        for (int u = 0; u < g.ovs; ++u) {
-          double qx, qy;
-          eval_point(g, i, j, u, v, qx, qy);
-          ifv += poke(g, qx, qy, mx, my);
-        }
-      if (ifv != 0.0) {
-        // exp(i phi) - 1 = (-2 sin^2(phi/2), 2 sin(phi/2) cos(phi/2)): one sincos, no cancellation
-        double sh, ch;
-        sincos(0.5 * it.phase_scale * ifv * g.norm, &sh, &ch);
-        const double er = -2.0 * sh * sh, ei = 2.0 * sh * ch;
-        double fr, fi;
-        if (it.prec == 0) {
-          const double2 e = static_cast<const double2*>(it.field)[j * g.nx + i];
-          fr = e.x;
-          fi = e.y;
-        } else {
-          const float2 e = static_cast<const float2*>(it.field)[j * g.nx + i];
-          fr = e.x;
-          fi = e.y;
-        }
-        d = make_double2(fr * er - fi * ei, fr * ei + fi * er);
-        if (it.prec != 0) {
-          // complex64 values of the poke change, as the split of a complex64 field would see them
-          d = make_double2((double)(float)d.x, (double)(float)d.y);
-          mxv = fmax(mxv, fmax(fabs(d.x), fabs(d.y)));
+    for (int t = 0; t < NPL; ++t) {
+      const int c = lane + 64 * t;
+      const int64_t i = x0 + c;
+      double2 d = make_double2(0.0, 0.0);
+      if (c < w && i < g.nx && j < g.ny) {
+        double ifv = 0.0;
+        for (int v = 0; v < g.ovs; ++v)
+          for (int u = 0; u < g.ovs; ++u) {
+            double qx, qy;
+            eval_point(g, i, j, u, v, qx, qy);
+            ifv += poke(g, qx, qy, mx, my);
+          }
+        if (ifv != 0.0) {
+          // exp(i phi) - 1 = (-2 sin^2(phi/2), 2 sin(phi/2) cos(phi/2)): one sincos, no cancellation
+          double sh, ch;
+          sincos(0.5 * it.phase_scale * ifv * g.norm, &sh, &ch);
+          const double er = -2.0 * sh * sh, ei = 2.0 * sh * ch;
+          double fr, fi;
+          if (it.prec == 0) {
+            const double2 e = static_cast<const double2*>(it.field)[j * g.nx + i];
+            fr = e.x;
+            fi = e.y;
+          } else {
+            const float2 e = static_cast<const float2*>(it.field)[j * g.nx + i];
+            fr = e.x;
+            fi = e.y;
+          }
+          d = make_double2(fr * er - fi * ei, fr * ei + fi * er);
+          if (it.prec != 0) {
+            // complex64 values of the poke change, as the split of a complex64 field would see them
+            d = make_double2((double)(float)d.x, (double)(float)d.y);
+            mxv = fmax(mxv, fmax(fabs(d.x), fabs(d.y)));
+          }
         }
       }
+      val[t] = d;
     }
-    val[t] = d;
-  }
-  const int64_t row_off = (a * it.wy + r) * (int64_t)(2 * w);  // reals
-  if (it.prec == 0 || it.prec == 1) {
+    const int64_t row_off = (a * it.wy + r) * (int64_t)(2 * w);  // reals
+    if (it.prec == 0 || it.prec == 1) {
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) {
+        const int c = lane + 64 * t;
+        if (c >= w) continue;
+        if (it.prec == 0) static_cast<double2*>(it.out[0])[row_off / 2 + c] = val[t];
+        else static_cast<float2*>(it.out[0])[row_off / 2 + c] = make_float2((float)val[t].x, (float)val[t].y);
+      }
+      continue;
+    }
+    const float m = wave_max((float)mxv);
+    float s = 1.0f;
+    if (m > 0.0f && m <= 3.0e38f) {
+      int e;
+      frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)  =>  m 2^-e < 1
+      s = ldexpf(1.0f, -e);
+    }
+    if (lane == 0) it.row_scale[a * it.wy + r] = s;
 #pragma unroll
     for (int t = 0; t < NPL; ++t) {
       const int c = lane + 64 * t;
       if (c >= w) continue;
-      if (it.prec == 0) static_cast<double2*>(it.out[0])[row_off / 2 + c] = val[t];
-      else static_cast<float2*>(it.out[0])[row_off / 2 + c] = make_float2((float)val[t].x, (float)val[t].y);
-    }
-    return;
-  }
-  const float m = wave_max((float)mxv);
-  float s = 1.0f;
-  if (m > 0.0f && m <= 3.0e38f) {
-    int e;
-    frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)  =>  m 2^-e < 1
-    s = ldexpf(1.0f, -e);
-  }
-  if (lane == 0) it.row_scale[a * it.wy + r] = s;
+      const float vr = (float)val[t].x * s, vi = (float)val[t].y * s;
+      if (it.prec == 2) {
+        const _Float16 hr = (_Float16)vr, hi = (_Float16)vi;
+        static_cast<half2_t*>(it.out[0])[row_off / 2 + c] = half2_t{hr, hi};
+        static_cast<half2_t*>(it.out[1])[row_off / 2 + c] =
+            half2_t{(_Float16)((vr - (float)hr) * 2048.0f), (_Float16)((vi - (float)hi) * 2048.0f)};
+      } else {
+        const float v2[2] = {vr, vi};
+        __bf16 q0[2], q1[2], q2[2];
 #pragma unroll
-  for (int t = 0; t < NPL; ++t) {
-    const int c = lane + 64 * t;
-    if (c >= w) continue;
-    const float vr = (float)val[t].x * s, vi = (float)val[t].y * s;
-    if (it.prec == 2) {
-      const _Float16 hr = (_Float16)vr, hi = (_Float16)vi;
-      static_cast<half2_t*>(it.out[0])[row_off / 2 + c] = half2_t{hr, hi};
-      static_cast<half2_t*>(it.out[1])[row_off / 2 + c] =
-          half2_t{(_Float16)((vr - (float)hr) * 2048.0f), (_Float16)((vi - (float)hi) * 2048.0f)};
-    } else {
-      const float v2[2] = {vr, vi};
-      __bf16 q0[2], q1[2], q2[2];
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        q0[e] = (__bf16)v2[e];
-        const float rr = v2[e] - (float)q0[e];
-        q1[e] = (__bf16)rr;
-        q2[e] = (__bf16)(rr - (float)q1[e]);
+        for (int e = 0; e < 2; ++e) {
+          q0[e] = (__bf16)v2[e];
+          const float rr = v2[e] - (float)q0[e];
+          q1[e] = (__bf16)rr;
+          q2[e] = (__bf16)(rr - (float)q1[e]);
+        }
+        static_cast<bf16x2_t*>(it.out[0])[row_off / 2 + c] = bf16x2_t{q0[0], q0[1]};
+        static_cast<bf16x2_t*>(it.out[1])[row_off / 2 + c] = bf16x2_t{q1[0], q1[1]};
+        static_cast<bf16x2_t*>(it.out[2])[row_off / 2 + c] = bf16x2_t{q2[0], q2[1]};
       }
-      static_cast<bf16x2_t*>(it.out[0])[row_off / 2 + c] = bf16x2_t{q0[0], q0[1]};
-      static_cast<bf16x2_t*>(it.out[1])[row_off / 2 + c] = bf16x2_t{q1[0], q1[1]};
-      static_cast<bf16x2_t*>(it.out[2])[row_off / 2 + c] = bf16x2_t{q2[0], q2[1]};
     }
   }
 }
@@ -744,9 +751,10 @@ int upload_poke_items(const PokeWinItem* items, int n, void* host, void* dev_ite
 
 int launch_poke_windows(const void* dev_items, int n_items, int64_t total_rows, int max_wx, hipStream_t s) {
   if (total_rows <= 0) return kOk;
-  HCIPY_REQUIRE(ceil_div(total_rows, 4) <= INT32_MAX, "too many poke window rows");
+  HCIPY_REQUIRE(ceil_div(total_rows, 4 * kPokeRows) <= INT32_MAX, "too many poke window rows");
+  HCIPY_REQUIRE(total_rows % kPokeRows == 0, "poke window rows must come in groups of %d", kPokeRows);
   HCIPY_REQUIRE(max_wx >= 1 && max_wx <= kPokeMaxW, "poke window width %d", max_wx);
-  const dim3 g((unsigned)ceil_div(total_rows, 4));
+  const dim3 g((unsigned)ceil_div(total_rows, 4 * kPokeRows));
   const auto* it = static_cast<const PokeWinItemDev*>(dev_items);
   if (max_wx <= 64) hipLaunchKernelGGL(poke_window_kernel<1>, g, dim3(256), 0, s, it, n_items, total_rows);
   else if (max_wx <= 128) hipLaunchKernelGGL(poke_window_kernel<2>, g, dim3(256), 0, s, it, n_items, total_rows);
