@@ -77,7 +77,9 @@ int64_t padded_row(int64_t k) { return round_up(k, kRowPad); }
 constexpr int kMaxGridZ = 65535;
 // The persistent single-launch parity execution (parity_fused_kernel) did not finish on MI355X:
 // two test launches (1024^2 -> 512^2, and 256^2 -> 128^2 with every dependency wait bounded to
-// 10 ms) ran until the test's time limit.  Off until the cause is found; plans never take it.
+// 10 ms) ran until the test's time limit.  Its ISA showed a task loop whose exit the compiler took
+// as divergent; the loop is rewritten with a uniform skeleton that has not run on a GPU.  Off until
+// it has; plans never take it.
 constexpr bool kFusedEnabled = false;
 
 size_t elem_size(int prec) { return prec == kF64 ? 8 : (prec == kF32 ? 4 : 2); }
@@ -1334,7 +1336,8 @@ int hcipy_mft_plan_set_fused(hcipy_mft_plan plan, int enable) {
   Plan& pl = plan->p;
   HCIPY_REQUIRE(!enable || kFusedEnabled,
                 "the fused parity execution is disabled in this build: its persistent launches did not finish "
-                "on MI355X (cause not found); single executions run as three launches");
+                "on MI355X and the rewritten task loop has not run on a GPU; single executions run as three "
+                "launches");
   HCIPY_REQUIRE(!enable || (pl.sync != nullptr && pl.cus >= 8),
                 "the fused parity execution needs a split-FP16 plan in which the parity form is available");
   pl.fused = enable != 0;
