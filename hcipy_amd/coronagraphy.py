@@ -70,6 +70,29 @@ class LyotCoronagraph(OpticalElement):
                          wavefront.input_stokes_vector)
         return self.lyot_stop.forward(lyot)
 
+    def forward_multiple(self, wavefronts, group=None):
+        """``forward`` of several wavefronts (for example the wavelengths of a broadband source),
+        with the transforms to and from the focal-plane mask of all of them as grouped launches
+        (``FraunhoferPropagator.forward_multiple`` / ``backward_multiple``).  Differentiable like
+        ``forward``; the backward pass groups the adjoint transforms too.  Returns a list of
+        Wavefronts."""
+        wavefronts = list(wavefronts)
+        foc = self.prop.forward_multiple(wavefronts, group=group)
+        foc = [Wavefront(self._fpm_complement(f, False), wf.wavelength, wf.input_stokes_vector)
+               for f, wf in zip(foc, wavefronts)]
+        lyot = self.prop.backward_multiple(foc, group=group)
+        stop = self._stop()
+        out = []
+        for wf, ly in zip(wavefronts, lyot):
+            if self.lyot_stop is None or stop is not None:
+                E = pointwise.sub_mul(wf.electric_field, ly.electric_field, stop)
+                out.append(Wavefront(Field(E, wf.grid), wf.wavelength, wf.input_stokes_vector))
+            else:
+                L = Wavefront(pointwise.sub_mul(wf.electric_field, ly.electric_field), wf.wavelength,
+                              wf.input_stokes_vector)
+                out.append(self.lyot_stop.forward(L))
+        return out
+
     def backward(self, wavefront):
         wf = self.lyot_stop.backward(wavefront) if self.lyot_stop is not None else wavefront
         wf_foc = self.prop.forward(wf)

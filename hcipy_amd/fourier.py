@@ -374,6 +374,49 @@ class _MftFn(torch.autograd.Function):
         return ctx.mft._execute_plain(g.contiguous(), other, ctx.alpha.conjugate()), None, None, None
 
 
+class _GroupedMftFn(torch.autograd.Function):
+    """Several MFTs (one per call: transform, direction, scalar scale) as one grouped execution
+    and one autograd node.  Each call's vector-Jacobian product is its transform's opposite
+    direction with the conjugate scale (as for ``_MftFn``); the products of all calls run as one
+    grouped execution too."""
+
+    @staticmethod
+    def forward(ctx, group, specs, *tensors):
+        ctx.group, ctx.specs = group, specs
+        outs = group.execute([(m, d, t, a, _lib.OUTPUT_FIELD) for (m, d, a), t in zip(specs, tensors)])
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        calls = []
+        for (m, d, a), g in zip(ctx.specs, grads):
+            other = _lib.BACKWARD if d == _lib.FORWARD else _lib.FORWARD
+            calls.append((m, other, g.contiguous(), a.conjugate(), _lib.OUTPUT_FIELD))
+        return (None, None) + tuple(ctx.group.execute(calls))
+
+
+def grouped_transforms(calls, group):
+    """``calls``: sequence of ``(mft, direction, field, extra_alpha)``, each meaning
+    ``mft._apply(field, direction, extra_alpha)`` (the transform's grid weights included).  Runs
+    them as one grouped execution of ``group`` (an ``MftGroup``) and returns the output tensors;
+    with gradients recorded, the backward pass is one grouped execution as well.  Every call's
+    input grid must have scalar weights."""
+    specs, tensors = [], []
+    for i, (mft, direction, field, extra_alpha) in enumerate(calls):
+        in_grid = mft.input_grid if direction == _lib.FORWARD else mft.output_grid
+        t, _ = _as_field_tensor(field, in_grid)
+        if t.shape[-1] != in_grid.size:
+            raise ValueError('call %d: field size %d does not match the grid size %d' % (i, t.shape[-1], in_grid.size))
+        alpha = mft.scalar_scale(direction, extra_alpha)
+        if alpha is None:
+            raise ValueError('call %d: grouped transforms need grids with scalar weights' % i)
+        specs.append((mft, direction, alpha))
+        tensors.append(t)
+    if torch.is_grad_enabled() and any(t.requires_grad for t in tensors):
+        return list(_GroupedMftFn.apply(group, tuple(specs), *tensors))
+    return group.execute([(m, d, t, a, _lib.OUTPUT_FIELD) for (m, d, a), t in zip(specs, tensors)])
+
+
 class MatrixFourierTransform(FourierTransform):
     """Matrix Fourier transform between separated Cartesian grids, on MFMA.
 
@@ -456,19 +499,28 @@ class MatrixFourierTransform(FourierTransform):
         if not self.allocate_intermediate:
             self._workspace = {}
 
+    def scalar_scale(self, direction, extra_alpha=1.0):
+        """The complex scale of one direction when the weights of its input grid are a single
+        value (the transform is then alpha times the plain matrix product), else None."""
+        w = self._direction_weights(direction)
+        if np.isscalar(w) or np.size(w) == 1 or np.all(np.asarray(w) == np.asarray(w).ravel()[0]):
+            return complex(float(np.asarray(w).ravel()[0])) * complex(extra_alpha)
+        return None
+
+    def _direction_weights(self, direction):
+        if direction == _lib.FORWARD:
+            return self.input_grid.weights
+        # weights of the transform's output grid, which is this call's input grid
+        return self.output_grid.weights / (2 * np.pi) ** self.ndim
+
     def _apply(self, field, direction, extra_alpha=1.0, out=None, output=_lib.OUTPUT_FIELD):
         in_grid, out_grid = (self.input_grid, self.output_grid) if direction == _lib.FORWARD else (self.output_grid, self.input_grid)
         t, cdt = _as_field_tensor(field, in_grid)
         if t.shape[-1] != in_grid.size:
             raise ValueError('Field size %d does not match the grid size %d' % (t.shape[-1], in_grid.size))
-        if direction == _lib.FORWARD:
-            w = in_grid.weights
-        else:
-            # weights of the transform's output grid, which is this call's input grid
-            w = in_grid.weights / (2 * np.pi) ** self.ndim
-        if np.isscalar(w) or np.size(w) == 1 or np.all(np.asarray(w) == np.asarray(w).ravel()[0]):
-            alpha = complex(float(np.asarray(w).ravel()[0])) * complex(extra_alpha)
-        else:
+        alpha = self.scalar_scale(direction, extra_alpha)
+        if alpha is None:
+            w = self._direction_weights(direction)
             t = (t * torch.as_tensor(np.asarray(w), device=t.device)).to(cdt).contiguous()
             alpha = complex(extra_alpha)
         if t.requires_grad and torch.is_grad_enabled():

@@ -9,7 +9,7 @@ import torch
 
 from . import _lib
 from .field import Field, default_device
-from .fourier import FastFourierTransform, MatrixFourierTransform, MftGroup, make_fft_grid
+from .fourier import FastFourierTransform, MatrixFourierTransform, MftGroup, grouped_transforms, make_fft_grid
 from .optics import AgnosticOpticalElement, Wavefront
 
 __all__ = ['FraunhoferPropagator']
@@ -64,29 +64,63 @@ class FraunhoferPropagator(AgnosticOpticalElement):
         U = d.fourier_transform._apply(wavefront.electric_field, _lib.BACKWARD, extra_alpha=1 / d.norm_factor)
         return Wavefront(Field(U, self._input_grid), wavefront.wavelength, wavefront.input_stokes_vector)
 
+    def _group_for(self, n, group):
+        if group is not None:
+            return group
+        g = getattr(self, '_group', None)
+        if g is None or g.capacity < n:
+            g = self._group = MftGroup(max(64, n))
+        return g
+
     def forward_multiple(self, wavefronts, output='field', group=None):
         """Forward propagation of several wavefronts (for example the wavelengths of a broadband
         source, each with its own Fourier matrices) as one grouped launch per MFT stage.
         ``output`` is 'field' (complex, like ``forward``) or 'intensity' (fused |E|^2, like
-        ``forward_intensity``).  Returns a list of Wavefronts or intensity Fields."""
+        ``forward_intensity``).  Returns a list of Wavefronts or intensity Fields.  With gradients
+        recorded the result is differentiable like ``forward``, and the backward pass of the
+        transforms is one grouped launch per stage as well."""
         wavefronts = list(wavefronts)
+        if output not in ('field', 'intensity'):
+            raise ValueError("output must be 'field' or 'intensity'")
         if np.size(self._input_grid.weights) != 1:  # non-scalar weights: one call per wavefront
             if output == 'field':
                 return [self.forward(wf) for wf in wavefronts]
             return [self.forward_intensity(wf) for wf in wavefronts]
-        if group is None:
-            group = self._group = getattr(self, '_group', None) or MftGroup(max(64, len(wavefronts)))
-        kind = {'field': _lib.OUTPUT_FIELD, 'intensity': _lib.OUTPUT_INTENSITY}[output]
+        group = self._group_for(len(wavefronts), group)
+        grad = torch.is_grad_enabled() and any(isinstance(wf.electric_field, torch.Tensor) and
+                                               wf.electric_field.requires_grad for wf in wavefronts)
+        if output == 'field' or grad:
+            calls = []
+            for wf in wavefronts:
+                d = self.instance(wf.wavelength)
+                calls.append((d.fourier_transform, _lib.FORWARD, wf.electric_field, d.norm_factor))
+            outs = grouped_transforms(calls, group)
+            if output == 'field':
+                return [Wavefront(Field(o, self._output_grid), wf.wavelength, wf.input_stokes_vector)
+                        for o, wf in zip(outs, wavefronts)]
+            from .pointwise import intensity
+            return [Field(intensity(o), self._output_grid) for o in outs]
+        kind = _lib.OUTPUT_INTENSITY
         calls = []
         for wf in wavefronts:
             d = self.instance(wf.wavelength)
             w = complex(np.asarray(self._input_grid.weights).ravel()[0])
             calls.append((d.fourier_transform, _lib.FORWARD, wf.electric_field, w * d.norm_factor, kind))
-        outs = group.execute(calls)
-        if output == 'field':
-            return [Wavefront(Field(o, self._output_grid), wf.wavelength, wf.input_stokes_vector)
-                    for o, wf in zip(outs, wavefronts)]
-        return [Field(o, self._output_grid) for o in outs]
+        return [Field(o, self._output_grid) for o in group.execute(calls)]
+
+    def backward_multiple(self, wavefronts, group=None):
+        """``backward`` of several wavefronts as one grouped launch per MFT stage; differentiable
+        like ``backward``.  Returns a list of Wavefronts."""
+        wavefronts = list(wavefronts)
+        if np.size(self._output_grid.weights) != 1:  # non-scalar weights: one call per wavefront
+            return [self.backward(wf) for wf in wavefronts]
+        calls = []
+        for wf in wavefronts:
+            d = self.instance(wf.wavelength)
+            calls.append((d.fourier_transform, _lib.BACKWARD, wf.electric_field, 1 / d.norm_factor))
+        outs = grouped_transforms(calls, self._group_for(len(wavefronts), group))
+        return [Wavefront(Field(o, self._input_grid), wf.wavelength, wf.input_stokes_vector)
+                for o, wf in zip(outs, wavefronts)]
 
     def forward_intensity(self, wavefront, float64=False):
         """``forward(wavefront).intensity`` in one pass: |E|^2 is evaluated in the MFT epilogue
