@@ -60,6 +60,8 @@ int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis
                         double* nyq_out, hipStream_t s);
 int launch_gen_parity(const double* c, int nc, const double* k, int nk, int prec, SplitOut out, int64_t lda,
                       int rows_pad, hipStream_t s);
+int launch_parity_split_items(int prec, const SplitItems& items, int n, int nx, int ny, const ParAxis& ax,
+                              const ParAxis& ay, int64_t ld_out, hipStream_t s);
 int launch_parity_combine(const double* H, const ParAxis& au, const ParAxis& av, int nu, int nv, double sign, double ar,
                           double ai, int out_mode, void* out, int64_t batch, hipStream_t s);
 int launch_parity_fused(const FusedArgs& f, int grid, hipStream_t s);
@@ -584,12 +586,29 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
   return mark(3);
 }
 
+// A parity split of one single field that a grouped execution launches together with the splits
+// of its other items of the same geometry (launch_parity_split_items).
+struct SplitJob {
+  int prec, nx, ny;
+  ParAxis ax, ay;
+  int64_t ld;
+  const void* in;
+  SplitOut fo;
+  float* rs;
+};
+
+// Batches up to this size are split with the other items' fields (one job per field); larger
+// ones keep a launch of their own, which already has a workgroup row per field.
+constexpr int64_t kSplitJobBatchMax = 4;
+
 // Stage arguments of one direction in the parity form for a grouped launch (stage 1 with the
-// real constant, recombining stage 2).  The parity split is launched here
-// unless `shared` names an earlier item that split the same input with the same axis pairing;
-// stage 1 then reads that item's split data and row scales.
+// real constant, recombining stage 2).  The parity split is launched here (recorded in `jobs`
+// instead, one job per field, for batches up to kSplitJobBatchMax when `jobs` is given) unless
+// `shared` names an earlier item that split the same input with the same axis pairing; stage 1
+// then reads that item's split data and row scales.
 int collect_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch, double ar, double ai, char* ws,
-                   hipStream_t s, int output, Collected* col, const Collected* shared) {
+                   hipStream_t s, int output, Collected* col, const Collected* shared,
+                   std::vector<SplitJob>* jobs = nullptr) {
   const Dir d = direction(pl, dir);
   const bool fw = dir == HCIPY_FORWARD;
   const PMatrix& p1 = fw ? pl.pf1 : pl.pb1;
@@ -612,9 +631,20 @@ int collect_parity(const Plan& pl, int dir, const void* in, void* out, int64_t b
   } else {
     SplitOut fo{{ws + w.f[0], na > 1 ? ws + w.f[1] : nullptr, na > 2 ? ws + w.f[2] : nullptr}};
     float* rs = reinterpret_cast<float*>(ws + w.rs);
-    if (int st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch, fo,
-                                     w.ldf, rs, nullptr, nullptr, s))
+    if (jobs != nullptr && batch <= kSplitJobBatchMax) {
+      // field b: input b nx ny samples on, split rows and row scales 8 hn rows on
+      for (int64_t b = 0; b < batch; ++b) {
+        const int64_t rows = b * 8 * hn;
+        SplitOut fb = fo;
+        for (int q = 0; q < 3; ++q)
+          if (fb.p[q]) fb.p[q] = static_cast<char*>(fb.p[q]) + (size_t)rows * w.ldf * 2;
+        jobs->push_back(SplitJob{pl.prec, (int)d.k1, (int)d.n1, ak, an, w.ldf,
+                                 static_cast<const float2*>(in) + b * d.k1 * d.n1, fb, rs + rows});
+      }
+    } else if (int st = launch_parity_split(pl.prec, static_cast<const float*>(in), (int)d.k1, (int)d.n1, ak, an, batch,
+                                            fo, w.ldf, rs, nullptr, nullptr, s)) {
       return st;
+    }
     a.b = fo.p[0];
     a.b_lo = fo.p[1];
     a.b_lo2 = fo.p[2];
@@ -1572,6 +1602,7 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
   // the parity form when every item's plan runs it (2-D split plans between symmetric grids)
   bool parity = true;
   for (int i = 0; i < n_items && parity; ++i) parity = items[i].plan != nullptr && items[i].plan->p.parity;
+  std::vector<SplitJob> jobs;
   for (int i = 0; i < n_items && parity; ++i) {
     const hcipy_mft_group_item& it = items[i];
     HCIPY_REQUIRE(it.in != nullptr && it.out != nullptr && it.batch > 0, "bad item %d", i);
@@ -1594,7 +1625,30 @@ int hcipy_mft_execute_grouped(hcipy_mft_group group, const hcipy_mft_group_item*
         shared = &col[(size_t)j];
     }
     const int st = collect_parity(pl, it.direction, it.in, it.out, it.batch, it.alpha_re, it.alpha_im,
-                                  static_cast<char*>(it.workspace), s, it.output, &col[(size_t)i], shared);
+                                  static_cast<char*>(it.workspace), s, it.output, &col[(size_t)i], shared, &jobs);
+    if (st != kOk) return st;
+  }
+  // The recorded splits of single fields, in launches of up to kSplitItemsMax items of one
+  // geometry (for example one per wavelength of a broadband source): all are on the stream before
+  // the stage-1 launch that reads them.
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    if (jobs[i].in == nullptr) continue;  // taken by an earlier launch
+    const SplitJob& g = jobs[i];
+    SplitItems si{};
+    int n = 0;
+    for (size_t j = i; j < jobs.size() && n < kSplitItemsMax; ++j) {
+      SplitJob& o = jobs[j];
+      if (o.in == nullptr || o.prec != g.prec || o.nx != g.nx || o.ny != g.ny || o.ld != g.ld ||
+          !same_pairing(o.ax, g.ax) || !same_pairing(o.ay, g.ay))
+        continue;
+      si.in[n] = o.in;
+      si.o[n] = o.fo;
+      si.rs[n] = o.rs;
+      ++n;
+      if (j != i) o.in = nullptr;
+    }
+    const int st = launch_parity_split_items(g.prec, si, n, g.nx, g.ny, g.ax, g.ay, g.ld, s);
+    jobs[i].in = nullptr;
     if (st != kOk) return st;
   }
   for (int i = 0; i < n_items && !parity; ++i) {

@@ -2085,6 +2085,16 @@ __global__ __launch_bounds__(256) void parity_split_kernel(const float2* __restr
                                    blockIdx.y);
 }
 
+// The same split for up to kSplitItemsMax single fields of one geometry: workgroup row blockIdx.y
+// takes item blockIdx.y.
+template <int FMT, int XPT, int TPR>
+__global__ __launch_bounds__(256) void parity_split_items_kernel(SplitItems it, int nx, int ny, ParAxis ax, ParAxis ay,
+                                                                 int64_t ld_out) {
+  const int i = blockIdx.y;
+  parity_split_body<FMT, XPT, TPR>(static_cast<const float2*>(it.in[i]), nx, ny, ax, ay, it.o[i], ld_out, it.rs[i],
+                                   nullptr, nullptr, blockIdx.x, gridDim.x, 0);
+}
+
 // Real parity matrices of one stage, two planes of rows_pad x lda: plane 0 cos(c[i] k[j]),
 // plane 1 sin(c[i] k[j]) for i < nc, j < nk (c, k: the positive halves of the row and
 // contraction coordinates), zero elsewhere; phase and sincos in FP64, stored in the operand
@@ -2715,6 +2725,28 @@ int launch_parity_split(int prec, const float* in, int nx, int ny, const ParAxis
 #undef HCIPY_PSPLIT
   }
   return check_hip(hipGetLastError(), "parity_split_kernel launch");
+}
+
+int launch_parity_split_items(int prec, const SplitItems& items, int n, int nx, int ny, const ParAxis& ax,
+                              const ParAxis& ay, int64_t ld_out, hipStream_t s) {
+  HCIPY_REQUIRE(n >= 1 && n <= kSplitItemsMax, "%d items in one parity split launch", n);
+  HCIPY_REQUIRE(ld_out <= 128 * 16, "parity split of %d-sample half rows (at most 2048)", ax.h);
+#define HCIPY_PSPLIT(FMT, XPT, TPR)                                                                            \
+  hipLaunchKernelGGL((parity_split_items_kernel<FMT, XPT, TPR>), dim3((unsigned)ceil_div(ay.h, 256 / TPR), (unsigned)n), \
+                     dim3(256), 0, s, items, nx, ny, ax, ay, ld_out)
+#define HCIPY_PSPLIT_ALL(FMT)                             \
+  if (ld_out <= 256) HCIPY_PSPLIT(FMT, 2, 128);           \
+  else if (ld_out <= 512) HCIPY_PSPLIT(FMT, 2, 256);      \
+  else if (ld_out <= 1024) HCIPY_PSPLIT(FMT, 4, 256);     \
+  else HCIPY_PSPLIT(FMT, 8, 256);
+  if (prec == kF16x2) {
+    HCIPY_PSPLIT_ALL(kSplitF16x2)
+  } else {
+    HCIPY_PSPLIT_ALL(kSplitBF16x3)
+  }
+#undef HCIPY_PSPLIT_ALL
+#undef HCIPY_PSPLIT
+  return check_hip(hipGetLastError(), "parity_split_items_kernel launch");
 }
 
 int launch_gen_parity(const double* c, int nc, const double* k, int nk, int prec, SplitOut out, int64_t lda,

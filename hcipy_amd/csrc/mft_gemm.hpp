@@ -243,6 +243,15 @@ struct SplitOut {
   void* p[3];
 };
 
+// Parity splits of several single fields of one geometry in one launch (grouped executions whose
+// items read different inputs): input, split arrays and row scales per item, passed by value.
+constexpr int kSplitItemsMax = 16;
+struct SplitItems {
+  const void* in[kSplitItemsMax];
+  SplitOut o[kSplitItemsMax];
+  float* rs[kSplitItemsMax];
+};
+
 // DISABLED (capi.hip kFusedEnabled): launches of this kernel did not finish on MI355X, and plans
 // never take it; the code is kept for the investigation of that failure.
 // One execution of the parity form as a single persistent launch (parity_fused_kernel): the
