@@ -19,6 +19,10 @@ __all__ = ['LyotCoronagraph', 'MultiScaleCoronagraph', 'VortexCoronagraph', 'Vec
            'periodic_tukey_window']
 
 
+def _plain_tensor(t):
+    return t.as_subclass(torch.Tensor) if isinstance(t, Field) else t
+
+
 class LyotCoronagraph(OpticalElement):
     """Lyot coronagraph with a small focal-plane mask M on its own focal grid.
 
@@ -77,11 +81,25 @@ class LyotCoronagraph(OpticalElement):
         ``forward``; the backward pass groups the adjoint transforms too.  Returns a list of
         Wavefronts."""
         wavefronts = list(wavefronts)
+        stop = self._stop()
+        ok, ts = self.prop._stackable([wf.electric_field for wf in wavefronts])
+        fixed_mask = isinstance(self.focal_plane_mask, Apodizer) and not callable(self.focal_plane_mask.apodization)
+        if (ok and ts[0].is_complex() and fixed_mask and np.size(self.prop._input_grid.weights) == 1 and
+                np.size(self.prop._output_grid.weights) == 1 and (self.lyot_stop is None or stop is not None)):
+            # the wavelengths as rows of stacked tensors: each pointwise step is one launch (the
+            # mask and the stop do not depend on the wavelength)
+            wls = [wf.wavelength for wf in wavefronts]
+            E = torch.stack(ts)
+            F = self.prop.forward_stacked(E, wls, group)
+            F = self._fpm_complement(Wavefront(Field(F, self.prop._output_grid), wls[0]), False)
+            L = self.prop.backward_stacked(_plain_tensor(F), wls, group)
+            out = pointwise.sub_mul(E, L, stop)
+            return [Wavefront(Field(o, wf.grid), wf.wavelength, wf.input_stokes_vector)
+                    for o, wf in zip(_plain_tensor(out).unbind(0), wavefronts)]
         foc = self.prop.forward_multiple(wavefronts, group=group)
         foc = [Wavefront(self._fpm_complement(f, False), wf.wavelength, wf.input_stokes_vector)
                for f, wf in zip(foc, wavefronts)]
         lyot = self.prop.backward_multiple(foc, group=group)
-        stop = self._stop()
         out = []
         for wf, ly in zip(wavefronts, lyot):
             if self.lyot_stop is None or stop is not None:

@@ -294,10 +294,13 @@ class MftGroup:
         self.handle = h
         self._ws = []
 
-    def execute(self, calls):
+    def execute(self, calls, outs=None):
         """``calls``: sequence of ``(mft, direction, field, alpha, output)`` with direction
         ``_lib.FORWARD``/``_lib.BACKWARD``, a complex field tensor or Field (leading dimensions
-        = batch), a complex scale and an ``_lib.OUTPUT_*`` kind.  Returns the output tensors."""
+        = batch), a complex scale and an ``_lib.OUTPUT_*`` kind.  Returns the output tensors,
+        written into ``outs`` (contiguous tensors of the outputs' shapes and dtypes, for example
+        rows of one stacked tensor) when given."""
+        given = outs
         n = len(calls)
         if n == 0:
             return []
@@ -321,7 +324,14 @@ class MftGroup:
             odt = {_lib.OUTPUT_FIELD: cdt, _lib.OUTPUT_FIELD_C128: torch.complex128,
                    _lib.OUTPUT_INTENSITY: torch.float64 if cdt == torch.complex128 else torch.float32,
                    _lib.OUTPUT_INTENSITY_F64: torch.float64}[output]
-            out = torch.empty(lead + (out_grid.size,), dtype=odt, device=t.device)
+            if given is None:
+                out = torch.empty(lead + (out_grid.size,), dtype=odt, device=t.device)
+            else:
+                out = given[i]
+                if (out.dtype != odt or tuple(out.shape) != lead + (out_grid.size,) or not out.is_contiguous()
+                        or out.device != t.device):
+                    raise ValueError('call %d: output tensor of shape %s, %s; expected %s, %s, contiguous' %
+                                     (i, tuple(out.shape), out.dtype, lead + (out_grid.size,), odt))
             need = max(plan.workspace_bytes(batch), 256)
             if self._ws[i] is None or self._ws[i].numel() < need:
                 self._ws[i] = torch.empty(need, dtype=torch.uint8, device=t.device)
@@ -393,6 +403,62 @@ class _GroupedMftFn(torch.autograd.Function):
             other = _lib.BACKWARD if d == _lib.FORWARD else _lib.FORWARD
             calls.append((m, other, g.contiguous(), a.conjugate(), _lib.OUTPUT_FIELD))
         return (None, None) + tuple(ctx.group.execute(calls))
+
+
+class _GroupedStackFn(torch.autograd.Function):
+    """As ``_GroupedMftFn`` for transforms of one input and one output grid (for example one
+    propagator at several wavelengths) on a stacked tensor: row i of the input goes through call
+    i, and its output is row i of the stacked output, so that pointwise operations on all rows
+    run as one launch."""
+
+    @staticmethod
+    def forward(ctx, group, specs, out_size, x):
+        ctx.group, ctx.specs, ctx.in_shape = group, specs, tuple(x.shape)
+        out = torch.empty(tuple(x.shape[:-1]) + (out_size,), dtype=x.dtype, device=x.device)
+        group.execute([(m, d, x[i], a, _lib.OUTPUT_FIELD) for i, (m, d, a) in enumerate(specs)],
+                      outs=[out[i] for i in range(len(specs))])
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        g = g.contiguous()
+        gx = torch.empty(ctx.in_shape, dtype=g.dtype, device=g.device)
+        calls = []
+        for i, (m, d, a) in enumerate(ctx.specs):
+            other = _lib.BACKWARD if d == _lib.FORWARD else _lib.FORWARD
+            calls.append((m, other, g[i], a.conjugate(), _lib.OUTPUT_FIELD))
+        ctx.group.execute(calls, outs=[gx[i] for i in range(len(ctx.specs))])
+        return None, None, None, gx
+
+
+def grouped_transforms_stacked(mfts, direction, x, extra_alphas, group):
+    """Row i of the stacked complex tensor ``x`` (shape ``(n, ..., size)``) through
+    ``mfts[i]._apply(., direction, extra_alphas[i])`` as one grouped execution; returns the
+    stacked output ``(n, ..., out_size)``.  Differentiable; the transforms share one input grid
+    and one output grid (sizes), with scalar weights."""
+    n = len(mfts)
+    in_grid = mfts[0].input_grid if direction == _lib.FORWARD else mfts[0].output_grid
+    out_grid = mfts[0].output_grid if direction == _lib.FORWARD else mfts[0].input_grid
+    if x.shape[0] != n or x.shape[-1] != in_grid.size:
+        raise ValueError('stacked field of shape %s for %d transforms of %d-sample inputs' %
+                         (tuple(x.shape), n, in_grid.size))
+    specs = []
+    for i, (m, ea) in enumerate(zip(mfts, extra_alphas)):
+        g_in = m.input_grid if direction == _lib.FORWARD else m.output_grid
+        g_out = m.output_grid if direction == _lib.FORWARD else m.input_grid
+        if g_in.size != in_grid.size or g_out.size != out_grid.size:
+            raise ValueError('transform %d: grid sizes differ from the first transform' % i)
+        a = m.scalar_scale(direction, ea)
+        if a is None:
+            raise ValueError('transform %d: grouped transforms need grids with scalar weights' % i)
+        specs.append((m, direction, a))
+    x = x.to(complex_dtype_for(x.dtype)).contiguous()
+    if torch.is_grad_enabled() and x.requires_grad:
+        return _GroupedStackFn.apply(group, tuple(specs), out_grid.size, x)
+    out = torch.empty(tuple(x.shape[:-1]) + (out_grid.size,), dtype=x.dtype, device=x.device)
+    group.execute([(m, d, x[i], a, _lib.OUTPUT_FIELD) for i, (m, d, a) in enumerate(specs)],
+                  outs=[out[i] for i in range(n)])
+    return out
 
 
 def grouped_transforms(calls, group):

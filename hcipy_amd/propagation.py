@@ -9,7 +9,8 @@ import torch
 
 from . import _lib
 from .field import Field, default_device
-from .fourier import FastFourierTransform, MatrixFourierTransform, MftGroup, grouped_transforms, make_fft_grid
+from .fourier import (FastFourierTransform, MatrixFourierTransform, MftGroup, grouped_transforms,
+                      grouped_transforms_stacked, make_fft_grid)
 from .optics import AgnosticOpticalElement, Wavefront
 
 __all__ = ['FraunhoferPropagator']
@@ -72,6 +73,26 @@ class FraunhoferPropagator(AgnosticOpticalElement):
             g = self._group = MftGroup(max(64, n))
         return g
 
+    def forward_stacked(self, x, wavelengths, group=None):
+        """Row i of the stacked field tensor ``x`` (``(n, ..., input size)``) propagated forward at
+        ``wavelengths[i]``, as one grouped launch per MFT stage; returns the stacked output tensor.
+        Differentiable (the adjoints are grouped too).  The input grid must have scalar weights."""
+        ds = [self.instance(wl) for wl in wavelengths]
+        return grouped_transforms_stacked([d.fourier_transform for d in ds], _lib.FORWARD, x,
+                                          [d.norm_factor for d in ds], self._group_for(len(ds), group))
+
+    def backward_stacked(self, x, wavelengths, group=None):
+        """``backward`` of row i of the stacked ``x`` at ``wavelengths[i]``; as ``forward_stacked``."""
+        ds = [self.instance(wl) for wl in wavelengths]
+        return grouped_transforms_stacked([d.fourier_transform for d in ds], _lib.BACKWARD, x,
+                                          [1 / d.norm_factor for d in ds], self._group_for(len(ds), group))
+
+    @staticmethod
+    def _stackable(fields):
+        ts = [f.as_subclass(torch.Tensor) if isinstance(f, torch.Tensor) else None for f in fields]
+        return (all(t is not None for t in ts) and len({(tuple(t.shape), t.dtype, t.device) for t in ts}) == 1,
+                ts)
+
     def forward_multiple(self, wavefronts, output='field', group=None):
         """Forward propagation of several wavefronts (for example the wavelengths of a broadband
         source, each with its own Fourier matrices) as one grouped launch per MFT stage.
@@ -90,6 +111,15 @@ class FraunhoferPropagator(AgnosticOpticalElement):
         grad = torch.is_grad_enabled() and any(isinstance(wf.electric_field, torch.Tensor) and
                                                wf.electric_field.requires_grad for wf in wavefronts)
         if output == 'field' or grad:
+            ok, ts = self._stackable([wf.electric_field for wf in wavefronts])
+            if ok and ts[0].is_complex():
+                # one stacked tensor: the intensities below run as one launch
+                st = self.forward_stacked(torch.stack(ts), [wf.wavelength for wf in wavefronts], group)
+                if output == 'intensity':
+                    from .pointwise import intensity
+                    return [Field(i, self._output_grid) for i in intensity(st).unbind(0)]
+                return [Wavefront(Field(o, self._output_grid), wf.wavelength, wf.input_stokes_vector)
+                        for o, wf in zip(st.unbind(0), wavefronts)]
             calls = []
             for wf in wavefronts:
                 d = self.instance(wf.wavelength)
@@ -114,11 +144,15 @@ class FraunhoferPropagator(AgnosticOpticalElement):
         wavefronts = list(wavefronts)
         if np.size(self._output_grid.weights) != 1:  # non-scalar weights: one call per wavefront
             return [self.backward(wf) for wf in wavefronts]
-        calls = []
-        for wf in wavefronts:
-            d = self.instance(wf.wavelength)
-            calls.append((d.fourier_transform, _lib.BACKWARD, wf.electric_field, 1 / d.norm_factor))
-        outs = grouped_transforms(calls, self._group_for(len(wavefronts), group))
+        ok, ts = self._stackable([wf.electric_field for wf in wavefronts])
+        if ok and ts[0].is_complex():
+            outs = self.backward_stacked(torch.stack(ts), [wf.wavelength for wf in wavefronts], group).unbind(0)
+        else:
+            calls = []
+            for wf in wavefronts:
+                d = self.instance(wf.wavelength)
+                calls.append((d.fourier_transform, _lib.BACKWARD, wf.electric_field, 1 / d.norm_factor))
+            outs = grouped_transforms(calls, self._group_for(len(wavefronts), group))
         return [Wavefront(Field(o, self._input_grid), wf.wavelength, wf.input_stokes_vector)
                 for o, wf in zip(outs, wavefronts)]
 
