@@ -454,6 +454,7 @@ def main():
         res = {}
         # each precision as the per-wavelength loop (the reference's usage) and with the five
         # wavelengths as grouped launches (LyotCoronagraph / FraunhoferPropagator.forward_multiple)
+        scales = torch.as_tensor(4 * np.pi / wls, dtype=torch.float64, device=dev)
         for cdt, precision, key, grouped in ((torch.complex64, 'f16x2', 'f16x2', False),
                                              (torch.complex64, 'f16x2', 'f16x2_grouped_wavelengths', True),
                                              (torch.complex128, None, 'fp64', False),
@@ -463,6 +464,7 @@ def main():
             coro = ha.LyotCoronagraph(pupil_l, fpm_f, stop_f, focal_plane_mask_grid=fpm_grid, precision=precision)
             prop_l = ha.FraunhoferPropagator(pupil_l, focal_l, precision=precision)
             Es = {wl: torch.as_tensor(ap * np.exp(1j * phase0 * lam0 / wl), device=dev).to(cdt) for wl in wls}
+            E_stack = torch.stack([Es[wl] for wl in wls])
             dm = ha.DeformableMirror(IFd)
 
             heights_dev = torch.as_tensor(heights, device=dev)
@@ -473,11 +475,11 @@ def main():
                 surface = dm.surface
                 total = 0
                 if grouped:
-                    wfs = [ha.Wavefront(ha.Field(ha.pointwise.multiply_phase(ha.Field(Es[wl], pupil_l), surface,
-                                                                             4 * np.pi / wl), pupil_l), wl)
-                           for wl in wls]
-                    for img in prop_l.forward_multiple(coro.forward_multiple(wfs), output='intensity'):
-                        total = total + ha.get_mean_intensity_in_roi(img, dz) * count
+                    # the wavelengths as rows of stacked tensors: one phase launch with a scale per
+                    # row, grouped transforms, one intensity and one batched dark-zone mean
+                    Ep = ha.pointwise.multiply_phase(E_stack, surface, scales)
+                    F = prop_l.forward_stacked(coro.forward_stacked(Ep, list(wls)), list(wls))
+                    total = ha.get_mean_intensity_in_roi(ha.pointwise.intensity(F), dz).sum() * count
                 else:
                     for wl in wls:
                         Ep = ha.pointwise.multiply_phase(ha.Field(Es[wl], pupil_l), surface, 4 * np.pi / wl)
@@ -524,7 +526,7 @@ def main():
                 res[key]['graph_error'] = '%s: %s' % (type(ex).__name__, str(ex)[:200])
         res['workload'] = ('1024^2 Lyot chain x 5 wavelengths, 48x48 Gaussian DM, dark-zone loss, gradient to '
                            'the actuators (eager with host work, and graph replays); *_grouped_wavelengths: the '
-                           'wavelengths as grouped launches')
+                           'wavelengths as rows of stacked tensors through grouped launches')
         res['grouped_vs_loop_rel_grad_norm_diff'] = {
             p: abs(res[p + '_grouped_wavelengths']['grad_norm'] - res[p]['grad_norm']) / res[p]['grad_norm']
             for p in ('f16x2', 'fp64')}

@@ -136,6 +136,10 @@ SIGNATURES = {
                                         _c_int64, _c_int64, _vp]),
     'hcipy_pw_phase': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _vp, ctypes.c_int, _c_int64,
                                       _c_int64, _vp]),
+    'hcipy_pw_phase_rows': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _c_int64, _vp, ctypes.c_double, _vp, ctypes.c_int,
+                                           _c_int64, _c_int64, _vp]),
+    'hcipy_pw_phase_rows_adjoint': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_double, _vp, _c_int64,
+                                                   _c_int64, _c_int64, _vp]),
     'hcipy_pw_intensity': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _c_int64, _vp]),
     'hcipy_pw_power_workspace': (ctypes.c_int, [_c_int64, ctypes.POINTER(ctypes.c_size_t)]),
     'hcipy_pw_power': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_int64, ctypes.c_double, _c_int64,

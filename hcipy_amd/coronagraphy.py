@@ -83,19 +83,11 @@ class LyotCoronagraph(OpticalElement):
         wavefronts = list(wavefronts)
         stop = self._stop()
         ok, ts = self.prop._stackable([wf.electric_field for wf in wavefronts])
-        fixed_mask = isinstance(self.focal_plane_mask, Apodizer) and not callable(self.focal_plane_mask.apodization)
-        if (ok and ts[0].is_complex() and fixed_mask and np.size(self.prop._input_grid.weights) == 1 and
-                np.size(self.prop._output_grid.weights) == 1 and (self.lyot_stop is None or stop is not None)):
-            # the wavelengths as rows of stacked tensors: each pointwise step is one launch (the
-            # mask and the stop do not depend on the wavelength)
+        if ok and ts[0].is_complex() and self._stackable_chain():
             wls = [wf.wavelength for wf in wavefronts]
-            E = torch.stack(ts)
-            F = self.prop.forward_stacked(E, wls, group)
-            F = self._fpm_complement(Wavefront(Field(F, self.prop._output_grid), wls[0]), False)
-            L = self.prop.backward_stacked(_plain_tensor(F), wls, group)
-            out = pointwise.sub_mul(E, L, stop)
+            out = self.forward_stacked(torch.stack(ts), wls, group)
             return [Wavefront(Field(o, wf.grid), wf.wavelength, wf.input_stokes_vector)
-                    for o, wf in zip(_plain_tensor(out).unbind(0), wavefronts)]
+                    for o, wf in zip(out.unbind(0), wavefronts)]
         foc = self.prop.forward_multiple(wavefronts, group=group)
         foc = [Wavefront(self._fpm_complement(f, False), wf.wavelength, wf.input_stokes_vector)
                for f, wf in zip(foc, wavefronts)]
@@ -110,6 +102,27 @@ class LyotCoronagraph(OpticalElement):
                               wf.input_stokes_vector)
                 out.append(self.lyot_stop.forward(L))
         return out
+
+    def _stackable_chain(self):
+        """True when the chain can run on stacked wavelengths: a fixed focal-plane mask and Lyot
+        stop (no wavelength dependence) and grids with scalar weights."""
+        fixed_mask = isinstance(self.focal_plane_mask, Apodizer) and not callable(self.focal_plane_mask.apodization)
+        return (fixed_mask and np.size(self.prop._input_grid.weights) == 1 and
+                np.size(self.prop._output_grid.weights) == 1 and (self.lyot_stop is None or self._stop() is not None))
+
+    def forward_stacked(self, x, wavelengths, group=None):
+        """``forward`` of row i of the stacked pupil field ``x`` (``(n, ..., pupil size)``) at
+        ``wavelengths[i]``: the two transforms of all rows as grouped launches
+        (``FraunhoferPropagator.forward_stacked`` / ``backward_stacked``), the mask and the stop
+        as one launch each.  Returns the stacked Lyot-plane field; differentiable.  Needs a fixed
+        focal-plane mask (an Apodizer with an array) and Lyot stop."""
+        if not self._stackable_chain():
+            raise ValueError('forward_stacked needs a fixed focal-plane mask and Lyot stop and scalar grid weights')
+        E = _plain_tensor(x).contiguous()
+        F = self.prop.forward_stacked(E, wavelengths, group)
+        F = self._fpm_complement(Wavefront(Field(F, self.prop._output_grid), wavelengths[0]), False)
+        L = self.prop.backward_stacked(_plain_tensor(F), wavelengths, group)
+        return _plain_tensor(pointwise.sub_mul(E, L, self._stop()))
 
     def backward(self, wavefront):
         wf = self.lyot_stop.backward(wavefront) if self.lyot_stop is not None else wavefront

@@ -77,15 +77,17 @@ __global__ void sub_mul_kernel(const C* __restrict__ x, const C* __restrict__ y,
 // out = x * exp(i * scale * phi), phase evaluated in FP64.
 template <typename C, typename R>
 __global__ void phase_kernel(const C* __restrict__ x, const void* __restrict__ phi, int phi_dtype, int64_t phi_stride,
-                             double scale, C* __restrict__ out, int64_t n, int64_t b_off) {
+                             double scale, const double* __restrict__ row_scale, C* __restrict__ out, int64_t n,
+                             int64_t b_off) {
   const int64_t b = blockIdx.y + b_off;
   const C* xb = x + b * n;
   C* ob = out + b * n;
+  const double sb = row_scale != nullptr ? scale * row_scale[b] : scale;  // this field's scale
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double p, dummy;
     load_any(phi, phi_dtype, b * phi_stride + i, p, dummy);
     double s, c;
-    sincos(scale * p, &s, &c);
+    sincos(sb * p, &s, &c);
     const R re = xb[i].re, im = xb[i].im;
     ob[i] = C{(R)(re * (R)c - im * (R)s), (R)(re * (R)s + im * (R)c)};
   }
@@ -190,7 +192,8 @@ __global__ __launch_bounds__(256) void mean_partials_kernel(const double* __rest
 // summed over the batch when phi is broadcast (phi_stride 0), FP64.
 template <typename C>
 __global__ void phase_adjoint_kernel(const C* __restrict__ y, const C* __restrict__ g, double scale,
-                                     double* __restrict__ out, int64_t phi_stride, int64_t n, int64_t batch) {
+                                     const double* __restrict__ row_scale, double* __restrict__ out,
+                                     int64_t phi_stride, int64_t n, int64_t batch) {
   const int64_t nb = (phi_stride == 0) ? 1 : batch;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n * nb; k += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = k % n;
@@ -199,7 +202,8 @@ __global__ void phase_adjoint_kernel(const C* __restrict__ y, const C* __restric
     double acc = 0.0;
     for (int64_t b = b0; b < b1; ++b) {
       const C gv = g[b * n + i], yv = y[b * n + i];
-      acc += (double)gv.re * (double)yv.im - (double)gv.im * (double)yv.re;  // Im(conj(g) y)
+      const double im = (double)gv.re * (double)yv.im - (double)gv.im * (double)yv.re;  // Im(conj(g) y)
+      acc += row_scale != nullptr ? row_scale[b] * im : im;
     }
     out[k] = -scale * acc;
   }
@@ -386,6 +390,11 @@ int hcipy_pw_sub_mul(const void* x, const void* y, const void* a, int a_dtype, i
 
 int hcipy_pw_phase(const void* x, const void* phi, int phi_dtype, int64_t phi_stride, double scale, void* out,
                    int field_dtype, int64_t n, int64_t batch, void* stream) {
+  return hcipy_pw_phase_rows(x, phi, phi_dtype, phi_stride, nullptr, scale, out, field_dtype, n, batch, stream);
+}
+
+int hcipy_pw_phase_rows(const void* x, const void* phi, int phi_dtype, int64_t phi_stride, const double* row_scale,
+                        double scale, void* out, int field_dtype, int64_t n, int64_t batch, void* stream) {
   HCIPY_REQUIRE(field_dtype == HCIPY_DTYPE_C64 || field_dtype == HCIPY_DTYPE_C128, "field must be complex");
   HCIPY_REQUIRE(phi_dtype == HCIPY_DTYPE_F32 || phi_dtype == HCIPY_DTYPE_F64, "phase must be real");
   HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
@@ -396,10 +405,10 @@ int hcipy_pw_phase(const void* x, const void* phi, int phi_dtype, int64_t phi_st
     const dim3 g = grid_for(n, chunk(batch, b0));
     if (field_dtype == HCIPY_DTYPE_C64)
       hipLaunchKernelGGL((phase_kernel<C64, float>), g, dim3(kThreads), 0, s, (const C64*)x, phi, phi_dtype,
-                         phi_stride, scale, (C64*)out, n, b0);
+                         phi_stride, scale, row_scale, (C64*)out, n, b0);
     else
       hipLaunchKernelGGL((phase_kernel<C128, double>), g, dim3(kThreads), 0, s, (const C128*)x, phi, phi_dtype,
-                         phi_stride, scale, (C128*)out, n, b0);
+                         phi_stride, scale, row_scale, (C128*)out, n, b0);
     HCIPY_HIP(hipGetLastError());
   }
   return kOk;
@@ -504,6 +513,11 @@ int hcipy_pw_masked_mean_adjoint(const void* mask, int mask_dtype, int64_t mask_
 
 int hcipy_pw_phase_adjoint(const void* y, const void* g, int field_dtype, double scale, double* out,
                            int64_t phi_stride, int64_t n, int64_t batch, void* stream) {
+  return hcipy_pw_phase_rows_adjoint(y, g, field_dtype, nullptr, scale, out, phi_stride, n, batch, stream);
+}
+
+int hcipy_pw_phase_rows_adjoint(const void* y, const void* g, int field_dtype, const double* row_scale, double scale,
+                                double* out, int64_t phi_stride, int64_t n, int64_t batch, void* stream) {
   HCIPY_REQUIRE(is_complex(field_dtype), "field must be complex");
   HCIPY_REQUIRE(phi_stride == 0 || phi_stride == n, "phase stride must be 0 or n");
   HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
@@ -513,11 +527,11 @@ int hcipy_pw_phase_adjoint(const void* y, const void* g, int field_dtype, double
   const int64_t total = n * (phi_stride == 0 ? 1 : batch);
   const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, kThreads), 8192)));
   if (field_dtype == HCIPY_DTYPE_C64)
-    hipLaunchKernelGGL(phase_adjoint_kernel<C64>, grid, dim3(kThreads), 0, s, (const C64*)y, (const C64*)g, scale, out,
-                       phi_stride, n, batch);
+    hipLaunchKernelGGL(phase_adjoint_kernel<C64>, grid, dim3(kThreads), 0, s, (const C64*)y, (const C64*)g, scale,
+                       row_scale, out, phi_stride, n, batch);
   else
     hipLaunchKernelGGL(phase_adjoint_kernel<C128>, grid, dim3(kThreads), 0, s, (const C128*)y, (const C128*)g, scale,
-                       out, phi_stride, n, batch);
+                       row_scale, out, phi_stride, n, batch);
   return check_hip(hipGetLastError(), "phase_adjoint_kernel");
 }
 

@@ -100,13 +100,14 @@ def multiply(x, a, conjugate=False):
     return sub_mul(x, None, a, conjugate)
 
 
-def _phase_raw(xt, pt, stride, scale):
+def _phase_raw(xt, pt, stride, scale, rows=None):
     o = torch.empty_like(xt)
     n = xt.shape[-1]
     batch = xt.numel() // n if n else 0
-    st = _lib.load().hcipy_pw_phase(_lib.dptr(xt), _lib.dptr(pt), _lib.dtype_code(pt), stride, float(scale),
-                                    _lib.dptr(o), _lib.dtype_code(xt), n, batch, _lib.stream_ptr(xt.device))
-    _lib.check(st, 'hcipy_pw_phase')
+    st = _lib.load().hcipy_pw_phase_rows(_lib.dptr(xt), _lib.dptr(pt), _lib.dtype_code(pt), stride,
+                                         _lib.dptr(rows) if rows is not None else None, float(scale), _lib.dptr(o),
+                                         _lib.dtype_code(xt), n, batch, _lib.stream_ptr(xt.device))
+    _lib.check(st, 'hcipy_pw_phase_rows')
     return o
 
 
@@ -114,41 +115,56 @@ class _PhaseFn(torch.autograd.Function):
     """y = x exp(i s phi); adjoints g exp(-i s phi) for x and -s Im(conj(g) y) for phi."""
 
     @staticmethod
-    def forward(ctx, xt, pt, stride, scale):
-        y = _phase_raw(xt, pt, stride, scale)
+    def forward(ctx, xt, pt, stride, scale, rows):
+        y = _phase_raw(xt, pt, stride, scale, rows)
         ctx.save_for_backward(pt, y)
-        ctx.stride, ctx.scale = stride, scale
+        ctx.stride, ctx.scale, ctx.rows = stride, scale, rows
         return y
 
     @staticmethod
     def backward(ctx, g):
         pt, y = ctx.saved_tensors
         g = g.contiguous()
-        gx = _phase_raw(g, pt, ctx.stride, -ctx.scale) if ctx.needs_input_grad[0] else None
+        gx = _phase_raw(g, pt, ctx.stride, -ctx.scale, ctx.rows) if ctx.needs_input_grad[0] else None
         gp = None
         if ctx.needs_input_grad[1]:
             n = y.shape[-1]
             batch = y.numel() // n if n else 0
             gp = torch.empty(pt.shape, dtype=torch.float64, device=y.device)
-            st = _lib.load().hcipy_pw_phase_adjoint(_lib.dptr(y), _lib.dptr(g), _lib.dtype_code(y), float(ctx.scale),
-                                                    _lib.dptr(gp), ctx.stride, n, batch, _lib.stream_ptr(y.device))
-            _lib.check(st, 'hcipy_pw_phase_adjoint')
+            st = _lib.load().hcipy_pw_phase_rows_adjoint(_lib.dptr(y), _lib.dptr(g), _lib.dtype_code(y),
+                                                         _lib.dptr(ctx.rows) if ctx.rows is not None else None,
+                                                         float(ctx.scale), _lib.dptr(gp), ctx.stride, n, batch,
+                                                         _lib.stream_ptr(y.device))
+            _lib.check(st, 'hcipy_pw_phase_rows_adjoint')
             gp = gp.to(pt.dtype)
-        return gx, gp, None, None
+        return gx, gp, None, None, None
 
 
 def multiply_phase(x, phi, scale=1.0):
-    """x * exp(i * scale * phi), phase in FP64."""
+    """x * exp(i * scale * phi), phase in FP64.  ``scale`` may also be a float64 device tensor
+    with one scale per field of the batch (the first dimension of ``x``; for example 4 pi / lambda
+    of each wavelength of a stacked broadband field): one launch for all of them."""
     grid = getattr(x, 'grid', None)
     xt = _plain(x).contiguous()
     _lib.require_device(xt)
     pt, stride = _operand(phi, xt)
     if pt.is_complex():
         raise ValueError('phase must be real')
+    rows = None
+    if isinstance(scale, torch.Tensor):
+        n = xt.shape[-1]
+        batch = xt.numel() // n if n else 0
+        rows = scale
+        if rows.dtype != torch.float64 or rows.device != xt.device or rows.numel() != batch or not rows.is_contiguous():
+            raise ValueError('per-field scales must be a contiguous float64 tensor of %d values on %s' %
+                             (batch, xt.device))
+        if rows.requires_grad:
+            raise ValueError('per-field scales are constants')
+        scale = 1.0
     if needs_grad(xt, pt):
-        o = _PhaseFn.apply(xt, pt, stride, float(scale))
+        o = _PhaseFn.apply(xt, pt, stride, float(scale), rows)
     else:
-        o = _phase_raw(xt, pt, stride, scale)
+        o = _phase_raw(xt, pt, stride, scale, rows)
     return Field(o, grid) if grid is not None else o
 
 

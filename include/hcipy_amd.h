@@ -277,6 +277,11 @@ int hcipy_pw_sub_mul(const void* x, const void* y, const void* a, int a_dtype, i
  * scale = +-2k, phi = surface).  out may alias x. */
 int hcipy_pw_phase(const void* x, const void* phi, int phi_dtype, int64_t phi_stride, double scale, void* out,
                    int field_dtype, int64_t n, int64_t batch, void* stream);
+/* The same with field b scaled by scale * row_scale[b] (row_scale: `batch` doubles in device
+ * memory, or NULL): one launch for the wavelengths of a broadband source, each with its own
+ * 2k = 4 pi / lambda.  No counterpart in the reference (it evaluates one wavelength per call). */
+int hcipy_pw_phase_rows(const void* x, const void* phi, int phi_dtype, int64_t phi_stride, const double* row_scale,
+                        double scale, void* out, int field_dtype, int64_t n, int64_t batch, void* stream);
 
 /* out[i] = |x[i]|^2 (float32 for complex64, float64 for complex128).
  * Replaces Wavefront.I for scalar fields (hcipy/optics/wavefront.py:113-119). */
@@ -319,6 +324,10 @@ int hcipy_pw_masked_mean_adjoint(const void* mask, int mask_dtype, int64_t mask_
  * out = -scale Im(conj(g) y) in FP64, summed over the batch when phi_stride is 0. */
 int hcipy_pw_phase_adjoint(const void* y, const void* g, int field_dtype, double scale, double* out,
                            int64_t phi_stride, int64_t n, int64_t batch, void* stream);
+/* Adjoint of hcipy_pw_phase_rows: out = -scale sum_b row_scale[b] Im(conj(g_b) y_b) for a shared
+ * phase (phi_stride 0), -scale row_scale[b] Im(conj(g_b) y_b) per field otherwise. */
+int hcipy_pw_phase_rows_adjoint(const void* y, const void* g, int field_dtype, const double* row_scale, double scale,
+                                double* out, int64_t phi_stride, int64_t n, int64_t batch, void* stream);
 
 /* Gradient with respect to x of I = |x|^2 (hcipy_pw_intensity): out = 2 g x, g real. */
 int hcipy_pw_intensity_adjoint(const void* x, int field_dtype, const void* g, int g_dtype, int64_t g_stride, void* out,

@@ -61,6 +61,8 @@ class _Chain:
         self.IF = ha.make_gaussian_influence_functions(pupil, N_ACT, 1.0 / N_ACT)
         self.E = {wl: torch.as_tensor(ap * np.exp(1j * self.phase0 * LAM0 / wl), device='cuda').to(cdt)
                   for wl in WAVELENGTHS}
+        self.E_stack = torch.stack([self.E[wl] for wl in WAVELENGTHS])
+        self.scales = torch.as_tensor(4 * np.pi / WAVELENGTHS, dtype=torch.float64, device='cuda')
 
     def focal_field(self, wf):
         return self.prop.forward(self.coro.forward(wf))
@@ -85,6 +87,14 @@ class _Chain:
         for img in self.prop.forward_multiple(self.coro.forward_multiple(wfs), output='intensity'):
             total = total + ha.get_mean_intensity_in_roi(img, self.dz) * self.count
         return total
+
+    def loss_from_surface_stacked(self, surface):
+        """The same loss with the wavelengths as rows of stacked tensors: one phase launch with a
+        scale per row, LyotCoronagraph.forward_stacked, FraunhoferPropagator.forward_stacked and
+        one batched dark-zone mean."""
+        Ep = ha.pointwise.multiply_phase(self.E_stack, surface, self.scales)
+        F = self.prop.forward_stacked(self.coro.forward_stacked(Ep, WAVELENGTHS), WAVELENGTHS)
+        return ha.get_mean_intensity_in_roi(ha.pointwise.intensity(F), self.dz).sum() * self.count
 
     def loss(self, act):
         dm = ha.DeformableMirror(self.IF)
@@ -169,11 +179,12 @@ def test_dark_hole_surface_gradient_matches_oracle_backward():
 
 @pytest.mark.parametrize('cdt,precision,bar', [(torch.complex128, None, 1e-10), (torch.complex64, 'f16x2', 1e-6)])
 def test_dark_hole_gradient_with_grouped_wavelengths(cdt, precision, bar):
-    """The five wavelengths as grouped launches give the loss and gradient of the per-wavelength
-    loop: FP64 to 1e-10 (the same products, other tile shapes: measured 3e-16); split FP16 to 1e-6
-    of the peak gradient (grouped launches take other tile shapes, which change the FP32 partial
-    sums; measured 1.5e-8).  Graph replays of the grouped step (forward and backward) equal its
-    eager run."""
+    """The five wavelengths as grouped launches, as a list of wavefronts and as rows of stacked
+    tensors (row-scaled phase, forward_stacked, batched dark-zone means), give the loss and gradient
+    of the per-wavelength loop: FP64 to 1e-10 (the same products, other tile shapes: measured
+    3e-16); split FP16 to 1e-6 of the peak gradient (grouped launches take other tile shapes, which
+    change the FP32 partial sums; measured 1.5e-8).  Graph replays of the stacked step (forward
+    and backward) equal its eager run."""
     chain = _Chain(cdt, precision)
     a0 = torch.as_tensor(np.random.default_rng(5).standard_normal(N_ACT ** 2) * 1e-9, device='cuda')
     dm = ha.DeformableMirror(chain.IF)
@@ -186,28 +197,30 @@ def test_dark_hole_gradient_with_grouped_wavelengths(cdt, precision, bar):
         return L.detach(), a.grad
 
     L1, g1 = step(chain.loss_from_surface)
-    L2, g2 = step(chain.loss_from_surface_grouped)
-    el = abs(float(L2) - float(L1)) / abs(float(L1))
-    eg = (g2 - g1).abs().max().item() / g1.abs().max().item()
-    say('grouped wavelengths (%s): loss %.3e, gradient %.3e relative to the per-wavelength loop' % (cdt, el, eg))
-    assert el <= bar and eg <= bar
+    for name, fn in (('list', chain.loss_from_surface_grouped), ('stacked', chain.loss_from_surface_stacked)):
+        L2, g2 = step(fn)
+        el = abs(float(L2) - float(L1)) / abs(float(L1))
+        eg = (g2 - g1).abs().max().item() / g1.abs().max().item()
+        say('grouped wavelengths, %s (%s): loss %.3e, gradient %.3e relative to the per-wavelength loop' %
+            (name, cdt, el, eg))
+        assert el <= bar and eg <= bar, name
     if precision is None:
         return
-    # the grouped step captured in a graph: every replay equals the eager step
+    # the stacked step captured in a graph: every replay equals the eager step
     act = a0.clone()
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(s):
-        step(chain.loss_from_surface_grouped, act)
+        step(chain.loss_from_surface_stacked, act)
     torch.cuda.current_stream().wait_stream(s)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
-        Lg, gg = step(chain.loss_from_surface_grouped, act)
+        Lg, gg = step(chain.loss_from_surface_stacked, act)
     for r in range(3):
         graph.replay()
         torch.cuda.synchronize()
         assert torch.equal(gg, g2) and torch.equal(Lg, L2), r
         # eager work between the replays
-        step(chain.loss_from_surface_grouped)
+        step(chain.loss_from_surface_stacked)
     torch.cuda.synchronize()
