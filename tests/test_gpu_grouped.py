@@ -130,3 +130,82 @@ def test_grouped_parity_form(precision):
                 assert e < 1e-6 and eo < 1e-6
             worst = max(worst, e, eo)
         say('grouped parity', precision, len(calls), 'items, pass', rep, 'worst error %.2e' % worst)
+
+
+@pytest.mark.parametrize('precision,cdt,tol', [(None, torch.complex128, 1e-12), ('f16x2', torch.complex64, 5e-7)])
+def test_stacked_wavelengths_match_single_calls(precision, cdt, tol):
+    """Rows of a stacked field at several wavelengths: FraunhoferPropagator.forward_stacked /
+    backward_stacked and LyotCoronagraph.forward_stacked equal forward / backward at each
+    wavelength (grouped launches take other tile shapes: the split-FP16 bar is the grouped one),
+    and their vector-Jacobian products equal the single calls' (autograd through the grouped
+    adjoints); forward_multiple / backward_multiple give the same as lists."""
+    pupil = ha.make_pupil_grid(96, 1.0)
+    focal = ha.make_focal_grid(4, 8, spatial_resolution=500e-9)
+    fpm_grid = ha.make_focal_grid(8, 3, spatial_resolution=500e-9)
+    wls = list(500e-9 * np.array([0.95, 1.0, 1.05]))
+    rng = np.random.default_rng(31)
+    x = torch.as_tensor(_rand((len(wls), pupil.size), 32, np.complex128), device='cuda').to(cdt)
+    prop = ha.FraunhoferPropagator(pupil, focal, precision=precision)
+    fpm = ha.Field(torch.as_tensor((np.hypot(fpm_grid.x, fpm_grid.y) > 1.5e-6).astype(float), device='cuda'), fpm_grid)
+    stop = ha.Field(torch.as_tensor((np.hypot(pupil.x, pupil.y) < 0.45).astype(float), device='cuda'), pupil)
+    coro = ha.LyotCoronagraph(pupil, fpm, stop, focal_plane_mask_grid=fpm_grid, precision=precision)
+
+    def peak_err(a, b):
+        return (a - b).abs().max().item() / b.abs().max().item()
+
+    F = prop.forward_stacked(x, wls)
+    F1 = torch.stack([prop.forward(ha.Wavefront(ha.Field(x[i], pupil), wl)).electric_field.as_subclass(torch.Tensor)
+                      for i, wl in enumerate(wls)])
+    B = prop.backward_stacked(F1, wls)
+    B1 = torch.stack([prop.backward(ha.Wavefront(ha.Field(F1[i], focal), wl)).electric_field.as_subclass(torch.Tensor)
+                      for i, wl in enumerate(wls)])
+    L = coro.forward_stacked(x, wls)
+    L1 = torch.stack([coro.forward(ha.Wavefront(ha.Field(x[i], pupil), wl)).electric_field.as_subclass(torch.Tensor)
+                      for i, wl in enumerate(wls)])
+    Lm = torch.stack([w.electric_field.as_subclass(torch.Tensor) for w in
+                      coro.forward_multiple([ha.Wavefront(ha.Field(x[i], pupil), wl) for i, wl in enumerate(wls)])])
+    errs = [peak_err(F, F1), peak_err(B, B1), peak_err(L, L1), peak_err(Lm, L1)]
+    say('stacked wavelengths', precision, 'forward %.2e backward %.2e lyot %.2e lyot list %.2e' % tuple(errs))
+    assert max(errs) <= tol
+
+    # vector-Jacobian products of the stacked chain against the single calls'
+    g = torch.as_tensor(_rand((len(wls), focal.size), 33, np.complex128), device='cuda').to(cdt)
+    xs = x.clone().requires_grad_(True)
+    (prop.forward_stacked(coro.forward_stacked(xs, wls), wls) * g.conj()).real.sum().backward()
+    grads = []
+    for i, wl in enumerate(wls):
+        xi = x[i].clone().requires_grad_(True)
+        out = prop.forward(coro.forward(ha.Wavefront(ha.Field(xi, pupil), wl))).electric_field.as_subclass(torch.Tensor)
+        (out * g[i].conj()).real.sum().backward()
+        grads.append(xi.grad)
+    eg = peak_err(xs.grad, torch.stack(grads))
+    say('stacked wavelengths', precision, 'vjp %.2e' % eg)
+    assert eg <= tol
+
+
+def test_row_scaled_phase_matches_per_row_calls():
+    """multiply_phase with one scale per row (hcipy_pw_phase_rows) equals one call per row bit for
+    bit, and its gradient with respect to the shared phase equals the sum of the per-row
+    gradients (FP64, 1e-14 of the peak)."""
+    n, rows = 4096, 5
+    rng = np.random.default_rng(41)
+    x = torch.as_tensor(_rand((rows, n), 42, np.complex128), device='cuda')
+    phi = torch.as_tensor(rng.standard_normal(n) * 1e-9, device='cuda')
+    scales_np = 4 * np.pi / (500e-9 * np.linspace(0.95, 1.05, rows))
+    scales = torch.as_tensor(scales_np, dtype=torch.float64, device='cuda')
+    y = ha.pointwise.multiply_phase(x, phi, scales)
+    y1 = torch.stack([ha.pointwise.multiply_phase(x[i], phi, float(scales_np[i])) for i in range(rows)])
+    assert torch.equal(y, y1)
+    g = torch.as_tensor(_rand((rows, n), 43, np.complex128), device='cuda')
+    p = phi.clone().requires_grad_(True)
+    (ha.pointwise.multiply_phase(x, p, scales) * g.conj()).real.sum().backward()
+    gsum = torch.zeros_like(phi)
+    for i in range(rows):
+        pi = phi.clone().requires_grad_(True)
+        (ha.pointwise.multiply_phase(x[i], pi, float(scales_np[i])) * g[i].conj()).real.sum().backward()
+        gsum = gsum + pi.grad
+    e = (p.grad - gsum).abs().max().item() / gsum.abs().max().item()
+    say('row-scaled phase gradient vs per-row sum %.2e' % e)
+    assert e <= 1e-14
+    with pytest.raises(ValueError):
+        ha.pointwise.multiply_phase(x, phi, scales[:3])
