@@ -52,11 +52,15 @@ template <typename C, typename R>
 __global__ void sub_mul_kernel(const C* __restrict__ x, const C* __restrict__ y, const void* __restrict__ a,
                                int a_dtype, int64_t a_stride, int conj, C* __restrict__ out, int64_t n, int64_t b_off) {
   const int64_t b = blockIdx.y + b_off;
-  const C* xb = x + b * n;
+  const C* xb = x ? x + b * n : nullptr;  // absent: zero
   const C* yb = y ? y + b * n : nullptr;
   C* ob = out + b * n;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    R re = xb[i].re, im = xb[i].im;
+    R re = 0, im = 0;
+    if (xb) {
+      re = xb[i].re;
+      im = xb[i].im;
+    }
     if (yb) {
       re -= (R)yb[i].re;
       im -= (R)yb[i].im;
@@ -373,7 +377,7 @@ int hcipy_pw_sub_mul(const void* x, const void* y, const void* a, int a_dtype, i
   HCIPY_REQUIRE(a == nullptr || is_valid(a_dtype), "bad dtype of the multiplier");
   HCIPY_REQUIRE(n >= 0 && batch >= 0, "bad size");
   if (n == 0 || batch == 0) return kOk;
-  HCIPY_REQUIRE(x != nullptr && out != nullptr, "NULL pointer");
+  HCIPY_REQUIRE((x != nullptr || y != nullptr) && out != nullptr, "NULL pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
   for (int64_t b0 = 0; b0 < batch; b0 += kMaxGridY) {
     const dim3 g = grid_for(n, chunk(batch, b0));

@@ -586,8 +586,10 @@ class MatrixFourierTransform(FourierTransform):
             raise ValueError('Field size %d does not match the grid size %d' % (t.shape[-1], in_grid.size))
         alpha = self.scalar_scale(direction, extra_alpha)
         if alpha is None:
-            w = self._direction_weights(direction)
-            t = (t * torch.as_tensor(np.asarray(w), device=t.device)).to(cdt).contiguous()
+            # per-sample weights: one pass of the pointwise kernel (differentiable), then the plain product
+            from .pointwise import sub_mul
+            w = np.asarray(self._direction_weights(direction), dtype=np.float64).ravel()
+            t = sub_mul(t, None, torch.as_tensor(w, device=t.device)).as_subclass(torch.Tensor).to(cdt).contiguous()
             alpha = complex(extra_alpha)
         if t.requires_grad and torch.is_grad_enabled():
             if out is not None:

@@ -48,13 +48,16 @@ def needs_grad(*ts):
 
 
 def _sub_mul_raw(xt, yt, at, stride, conjugate, o=None):
-    o = torch.empty_like(xt) if o is None else o
-    n = xt.shape[-1]
-    batch = xt.numel() // n if n else 0
-    st = _lib.load().hcipy_pw_sub_mul(_lib.dptr(xt), _lib.dptr(yt) if yt is not None else None,
+    """(x - y) a with x or y None (x None: zero)."""
+    ref = xt if xt is not None else yt
+    o = torch.empty_like(ref) if o is None else o
+    n = ref.shape[-1]
+    batch = ref.numel() // n if n else 0
+    st = _lib.load().hcipy_pw_sub_mul(_lib.dptr(xt) if xt is not None else None,
+                                      _lib.dptr(yt) if yt is not None else None,
                                       _lib.dptr(at) if at is not None else None,
                                       _lib.dtype_code(at) if at is not None else 0, stride, int(bool(conjugate)),
-                                      _lib.dptr(o), _lib.dtype_code(xt), n, batch, _lib.stream_ptr(xt.device))
+                                      _lib.dptr(o), _lib.dtype_code(ref), n, batch, _lib.stream_ptr(ref.device))
     _lib.check(st, 'hcipy_pw_sub_mul')
     return o
 
@@ -73,7 +76,8 @@ class _SubMulFn(torch.autograd.Function):
         (at,) = ctx.saved_tensors
         g = g.contiguous()
         gx = _sub_mul_raw(g, None, at, ctx.stride, not ctx.conjugate) if at is not None else g
-        gy = gx.neg() if ctx.has_y and ctx.needs_input_grad[1] else None
+        # -g conj(a) from the kernel with x absent (exactly the negation of gx)
+        gy = _sub_mul_raw(None, g, at, ctx.stride, not ctx.conjugate) if ctx.has_y and ctx.needs_input_grad[1] else None
         return (gx if ctx.needs_input_grad[0] else None), gy, None, None, None
 
 

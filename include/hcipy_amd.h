@@ -264,7 +264,8 @@ int hcipy_mft_profile_pipeline(hcipy_mft_plan plan, int direction, const void* i
 #define HCIPY_DTYPE_F32 2
 #define HCIPY_DTYPE_F64 3
 
-/* out = (x - y) * a    (y and a optional; conj(a) if conjugate_a).
+/* out = (x - y) * a    (y and a optional; conj(a) if conjugate_a; x NULL reads as zero, for
+ * the adjoint -g conj(a) with respect to y).
  * Replaces Apodizer.forward/backward (hcipy/optics/apodization.py:25-35) with y = NULL, the
  * Babinet step F -= M F of the Lyot coronagraph (hcipy/coronagraphy/lyot.py:80) with a = 1-M,
  * and E - P^dagger F followed by the Lyot stop (lyot.py:87-90) in one pass.  out may alias x. */
