@@ -132,7 +132,8 @@ def test_grouped_parity_form(precision):
         say('grouped parity', precision, len(calls), 'items, pass', rep, 'worst error %.2e' % worst)
 
 
-@pytest.mark.parametrize('precision,cdt,tol', [(None, torch.complex128, 1e-12), ('f16x2', torch.complex64, 5e-7)])
+@pytest.mark.parametrize('precision,cdt,tol', [(None, torch.complex128, 1e-12), ('f16x2', torch.complex64, 5e-7),
+                                               ('fp32', torch.complex64, 3e-6)])
 def test_stacked_wavelengths_match_single_calls(precision, cdt, tol):
     """Rows of a stacked field at several wavelengths: FraunhoferPropagator.forward_stacked /
     backward_stacked and LyotCoronagraph.forward_stacked equal forward / backward at each
@@ -209,3 +210,23 @@ def test_row_scaled_phase_matches_per_row_calls():
     assert e <= 1e-14
     with pytest.raises(ValueError):
         ha.pointwise.multiply_phase(x, phi, scales[:3])
+
+
+
+def test_stacked_wavelengths_with_batch_dimension():
+    """Stacked fields of shape (wavelengths, batch, n): each wavelength's batch runs as one grouped
+    item and matches per-field calls (split FP16, 5e-7 of the peak)."""
+    pupil = ha.make_pupil_grid(64, 1.0)
+    focal = ha.make_focal_grid(4, 6, spatial_resolution=500e-9)
+    wls = list(500e-9 * np.array([0.97, 1.03]))
+    prop = ha.FraunhoferPropagator(pupil, focal, precision='f16x2')
+    x = torch.as_tensor(_rand((2, 3, pupil.size), 61, np.complex64), device='cuda')
+    F = prop.forward_stacked(x, wls)
+    assert tuple(F.shape) == (2, 3, focal.size)
+    worst = 0.0
+    for i, wl in enumerate(wls):
+        for k in range(3):
+            ref = prop.forward(ha.Wavefront(ha.Field(x[i, k], pupil), wl)).electric_field.as_subclass(torch.Tensor)
+            worst = max(worst, (F[i, k] - ref).abs().max().item() / ref.abs().max().item())
+    say('stacked wavelengths with a batch dimension: %.2e' % worst)
+    assert worst <= 5e-7
