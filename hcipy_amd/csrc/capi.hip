@@ -460,8 +460,9 @@ int run_parity(const Plan& pl, int dir, const void* in, void* out, int64_t batch
     a.alpha_re = 1.0;
     return a;
   };
-  // default stage-2 shape: the recombining one, two waves per parity item for split-FP16
-  const int cfg2 = pl.tiles[dir][1] != kTileAuto ? pl.tiles[dir][1] : (pl.prec == kF16x2 ? kRC32x32_w8 : kRC32x32_h);
+  // default stage-2 shape: the recombining one, two waves per parity item (1024^2 -> 512^2 split-BF16:
+  // 39.5 us per propagation against 41.2 with one wave per item)
+  const int cfg2 = pl.tiles[dir][1] != kTileAuto ? pl.tiles[dir][1] : kRC32x32_w8;
   auto stage2_comb_args = [&](int64_t b0) {
     const size_t ob = out_elem_bytes(out_mode) * (size_t)d.c1 * d.c2;
     StageArgs a{};

@@ -231,7 +231,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   static_assert(NTHR == 256 || NTHR == 512, "4 or 8 waves per workgroup");
   static_assert(TM >= 1 && TN >= 1, "wave tile smaller than MFMA tile");
   static_assert(KV_PER_TILE % WK == 0, "k-vectors must split evenly over WK");
-  static_assert(ACH % NTHR == 0 && BCH % NTHR == 0, "tiles must split evenly over the threads");
+  // Tiles with fewer 16-byte chunks than threads x AL (BL) leave the last threads without a chunk:
+  // they neither load nor store it (A_ALL / B_ALL: every thread has every chunk).
+  constexpr bool A_ALL = (ACH % NTHR == 0), B_ALL = (BCH % NTHR == 0);
 
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
@@ -277,7 +279,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   int b_kc[BL];
 #pragma unroll
   for (int i = 0; i < AL; ++i) {
-    const int ch = tid + i * NTHR;
+    const int ch = A_ALL ? tid + i * NTHR : min(tid + i * NTHR, ACH - 1);  // clamped: in bounds, unused
     const int r = ch / CPR;
     a_off[i] = r * LDK + (ch % CPR) * VEC;  // LDS offset of the compact chunk's row (x2 below)
     if constexpr (COMB) a_goff[i] = (int64_t)(r >> 5) * p.a_zstride + (int64_t)(m0 + (r & 31)) * p.lda;
@@ -286,7 +288,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
   }
 #pragma unroll
   for (int i = 0; i < BL; ++i) {
-    const int ch = tid + i * NTHR;
+    const int ch = B_ALL ? tid + i * NTHR : min(tid + i * NTHR, BCH - 1);
     const int r = ch / CPR;
     b_row[i] = n0 + (COMB ? (r & 31) : r);
     b_kc[i] = (ch % CPR) * VEC;
@@ -307,9 +309,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
 #pragma unroll
     for (int arr = 0; arr < NARR; ++arr) {
 #pragma unroll
-      for (int i = 0; i < AL; ++i) ra[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Ag[arr] + a_goff[i] + k0);
+      for (int i = 0; i < AL; ++i)
+        if (A_ALL || tid + i * NTHR < ACH) ra[S][arr][i] = *reinterpret_cast<const u32x4_t*>(Ag[arr] + a_goff[i] + k0);
 #pragma unroll
       for (int i = 0; i < BL; ++i) {
+        if (!B_ALL && tid + i * NTHR >= BCH) continue;
         if constexpr (BMODE == kRowsUnaligned) {
           const int kk = k0 + b_kc[i];
           rb[S][arr][i] = (b_row[i] < N && kk < K) ? load_masked<T>(Bg[arr] + b_off[i] + kk, K - kk)
@@ -335,6 +339,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
       T* sB = base + ABUF + arr * BN * LDK;
 #pragma unroll
       for (int i = 0; i < AL; ++i) {
+        if (!A_ALL && tid + i * NTHR >= ACH) continue;
         if constexpr (REAL) {
           *reinterpret_cast<u32x4_t*>(&sA[a_off[i]]) = ra[S][arr][i];
         } else {
@@ -347,6 +352,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& p, int h, int z, int
 #pragma unroll
       for (int i = 0; i < BL; ++i) {
         const int ch = tid + i * NTHR;
+        if (!B_ALL && ch >= BCH) continue;
         u32x4_t v = rb[S][arr][i];
         if constexpr (MASK && BMODE != kRowsUnaligned) v = mask_tail<T>(v, K - (tile * BK + b_kc[i]));
         *reinterpret_cast<u32x4_t*>(&sB[(ch / CPR) * LDK + (ch % CPR) * VEC]) = v;
@@ -2645,7 +2651,7 @@ bool real_cfg_supported(int prec, int cfg, int stage) {
            (stage == 1 && (cfg == kRC32x32_w4 || cfg == kRC32x32_h || cfg == kRC32x32_w8));
   if (prec == kBF16x3)
     return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8 ||
-           (stage == 1 && cfg == kRC32x32_h);
+           (stage == 1 && (cfg == kRC32x32_h || cfg == kRC32x32_w8));
   return false;
 }
 
@@ -2685,6 +2691,8 @@ int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
       case k128x64: st = launch_stage_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
       case k64x32_w8: st = launch_stage_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(a, nz, s); break;
       case kRC32x32_h: st = launch_stage_t<kBF16x3, 64, 128, 32, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
+      // two waves per item on 32-deep k-tiles (64-deep tiles of three arrays exceed the LDS)
+      case kRC32x32_w8: st = launch_stage_t<kBF16x3, 64, 128, 32, 2, 2, 2, kRealComb, 2>(a, nz, s); break;
       default: set_last_error("tile configuration %d is not available to the parity form", cfg); return kInvalidValue;
     }
   } else {

@@ -104,7 +104,7 @@ int hcipy_mft_plan_bytes(hcipy_mft_plan plan, size_t* bytes);
  * launch). */
 #define HCIPY_TILES_RC32X32_W4 19 /* 64-deep k-tiles */
 #define HCIPY_TILES_RC32X32_H 20  /* 32-deep k-tiles */
-#define HCIPY_TILES_RC32X32_W8 21 /* 64-deep k-tiles, 8 waves: two per item, each on half of every k-tile */
+#define HCIPY_TILES_RC32X32_W8 21 /* 64-deep k-tiles (32-deep for split-BF16), 8 waves: two per item, each on half of every k-tile */
 /* F16X2 parity form, either stage: 64x64 tile, 2x2 waves of 32x32, 32-deep k-tiles (41 KB of
  * LDS; a candidate for short contractions, e.g. in hcipy_mft_plan_set_tiles after timing) */
 #define HCIPY_TILES_R64X64_H 22
