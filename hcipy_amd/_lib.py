@@ -54,7 +54,7 @@ POKE_TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 4, 6, 7), 'f16x2': (0, 1, 4,
 TILE_CFGS = {'fp64': (0, 1, 6), 'fp32': (0, 1, 2, 3, 4, 5, 6, 7), 'f16x2': tuple(range(16)),
              'bf16x3': (0, 1, 4, 7)}
 # tile shapes of the parity form (hcipy_mft_plan_set_parity)
-PARITY_TILE_CFGS = {'f16x2': (0, 1, 4, 5, 6, 7, 16, 17, 18, 22), 'bf16x3': (0, 1, 4, 7)}
+PARITY_TILE_CFGS = {'f16x2': (0, 1, 4, 5, 6, 7, 16, 17, 18, 22), 'bf16x3': (0, 1, 4, 6, 7, 22)}
 # stage 2 of the parity form only: the recombination runs in the stage's epilogue
 PARITY_STAGE2_TILE_CFGS = {'f16x2': (19, 20, 21), 'bf16x3': (20, 21)}
 

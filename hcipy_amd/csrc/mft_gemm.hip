@@ -2650,8 +2650,8 @@ bool real_cfg_supported(int prec, int cfg, int stage) {
            cfg == k64x128 || cfg == kR64x64_w4 || cfg == kR64x32_w4 || cfg == kR32x64_w4 || cfg == kR64x64_h ||
            (stage == 1 && (cfg == kRC32x32_w4 || cfg == kRC32x32_h || cfg == kRC32x32_w8));
   if (prec == kBF16x3)
-    return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8 ||
-           (stage == 1 && (cfg == kRC32x32_h || cfg == kRC32x32_w8));
+    return cfg == k64x64_w2x2 || cfg == k64x32_w2x1 || cfg == k128x64 || cfg == k64x32_w8 || cfg == k64x64_w8 ||
+           cfg == kR64x64_h || (stage == 1 && (cfg == kRC32x32_h || cfg == kRC32x32_w8));
   return false;
 }
 
@@ -2690,6 +2690,10 @@ int launch_stage_real(int prec, const StageArgs& a, int nz, hipStream_t s) {
       case k64x32_w2x1: st = launch_stage_t<kBF16x3, 64, 32, 64, 2, 1, 2, kRealPadded, 2>(a, nz, s); break;
       case k128x64: st = launch_stage_t<kBF16x3, 128, 64, 32, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
       case k64x32_w8: st = launch_stage_t<kBF16x3, 64, 32, 128, 2, 1, 4, kRealPadded, 2>(a, nz, s); break;
+      // 32-deep k-tiles: 64 x 64 tiles on 4 waves (61 KB of LDS, two workgroups per CU) or on 8
+      // waves, two per output tile each on half of every k-tile
+      case kR64x64_h: st = launch_stage_t<kBF16x3, 64, 64, 32, 2, 2, 1, kRealPadded, 2>(a, nz, s); break;
+      case k64x64_w8: st = launch_stage_t<kBF16x3, 64, 64, 32, 2, 2, 2, kRealPadded, 2>(a, nz, s); break;
       case kRC32x32_h: st = launch_stage_t<kBF16x3, 64, 128, 32, 2, 2, 1, kRealComb, 2>(a, nz, s); break;
       // two waves per item on 32-deep k-tiles (64-deep tiles of three arrays exceed the LDS)
       case kRC32x32_w8: st = launch_stage_t<kBF16x3, 64, 128, 32, 2, 2, 2, kRealComb, 2>(a, nz, s); break;
