@@ -299,14 +299,14 @@ __global__ __launch_bounds__(256) void dm_adjoint_kernel(DmGeom g, const double*
 // Separable adjoint (no z tilt): the per-axis factors exp(-ddx^2 / (2 sigma^2)) and squared
 // distances of the window's columns and rows, per dither, are tabulated in LDS first (`tab_len`
 // doubles of dynamic LDS hold the 2 ovs (wi + hj) of the largest window, at most kSepLds), then
-// every window pixel needs multiply-adds only.  Wave w takes the window rows in groups of
-// kAdjRows (rows 8w .. 8w + 7, then 32 further) and each group in pieces of kAdjCh x 64 columns,
-// whose kAdjRows x kAdjCh loads a lane issues before it uses any (the loads hit L2; one at a time
-// the wave waited out their latency in turn: 101 us at a 1024^2 pupil and 48 x 48 actuators, 53 us
-// with 8 x 1; 8 x 2 and 8 x 4 measured 59 and 78 us, their registers halving the resident waves).
+// every window pixel needs multiply-adds only.  The window's pixels, numbered row by row, are
+// dealt to the 256 threads in turn (pixel t to thread t mod 256), so every lane has a pixel
+// whatever the window's width (64-lane row pieces left 44 % of the lanes idle on 72-pixel
+// windows); a thread issues the loads of kAdjBatch of its pixels before it uses any, since the
+// loads hit L2 and one at a time the wave waited out their latency in turn (1024^2 pupil, 48 x 48
+// actuators: 4 pixels 53.5 us at ovs 1 and 94.0 at ovs 2, 8 pixels 55.3 / 95.4, 16 61.8 / 105).
 constexpr int kSepLds = 4096;
-constexpr int kAdjRows = 8;
-constexpr int kAdjCh = 1;
+constexpr int kAdjBatch = 4;
 
 __global__ __launch_bounds__(256) void dm_adjoint_sep_kernel(DmGeom g, const double* __restrict__ gs,
                                                              double* __restrict__ gw, int64_t b_off, int tab_len) {
@@ -350,36 +350,38 @@ __global__ __launch_bounds__(256) void dm_adjoint_sep_kernel(DmGeom g, const dou
       ey[t] = exp(-dy2[t] * g.inv_two_sigma2);
     }
     __syncthreads();
-    for (int jg = wave * kAdjRows; jg < hj; jg += 4 * kAdjRows) {
-      const double* row0 = gsb + (j0 + jg) * g.nx + i0;
-      for (int ib = 0; ib < wi; ib += 64 * kAdjCh) {
-        double gv[kAdjRows][kAdjCh];
+    // pixel t = jj wi + ii of the window; t < wi hj exactly when jj < hj.  A step of 256 pixels
+    // advances (jj, ii) by (256 / wi, 256 mod wi) with one carry, as 256 mod wi < wi.
+    const int step_j = 256 / wi, step_i = 256 % wi;
+    int jj = (int)threadIdx.x / wi, ii = (int)threadIdx.x % wi;
+    const double* win = gsb + j0 * g.nx + i0;
+    for (int t0 = 0; t0 < wi * hj; t0 += 256 * kAdjBatch) {
+      double gv[kAdjBatch];
+      int pj[kAdjBatch], pi[kAdjBatch];
 #pragma unroll
-        for (int q = 0; q < kAdjRows; ++q)
+      for (int q = 0; q < kAdjBatch; ++q) {
+        pj[q] = jj;
+        pi[q] = ii;
+        gv[q] = (jj < hj) ? win[(int64_t)jj * g.nx + ii] : 0.0;
+        ii += step_i;
+        jj += step_j;
+        if (ii >= wi) {
+          ii -= wi;
+          ++jj;
+        }
+      }
 #pragma unroll
-          for (int h = 0; h < kAdjCh; ++h) {
-            const int ii = ib + h * 64 + lane;
-            gv[q][h] = (jg + q < hj && ii < wi) ? row0[(int64_t)q * g.nx + ii] : 0.0;
-          }
-#pragma unroll
-        for (int q = 0; q < kAdjRows; ++q) {
-          if (jg + q >= hj) break;
-          const int jj = jg + q;
-#pragma unroll
-          for (int h = 0; h < kAdjCh; ++h) {
-            const int ii = ib + h * 64 + lane;
-            if (ii >= wi) break;
-            double val = 0.0;
-            for (int v = 0; v < ovs; ++v) {
-              const double eyv = ey[v * hj + jj], dyv = dy2[v * hj + jj];
-              for (int u = 0; u < ovs; ++u) {
-                const double r2 = dx2[u * wi + ii] + dyv;
-                if (r2 <= g.r2max) val += ex[u * wi + ii] * eyv - g.offset;
-              }
-            }
-            acc += gv[q][h] * val;
+      for (int q = 0; q < kAdjBatch; ++q) {
+        if (pj[q] >= hj) break;
+        double val = 0.0;
+        for (int v = 0; v < ovs; ++v) {
+          const double eyv = ey[v * hj + pj[q]], dyv = dy2[v * hj + pj[q]];
+          for (int u = 0; u < ovs; ++u) {
+            const double r2 = dx2[u * wi + pi[q]] + dyv;
+            if (r2 <= g.r2max) val += ex[u * wi + pi[q]] * eyv - g.offset;
           }
         }
+        acc += gv[q] * val;
       }
     }
   }
